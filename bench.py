@@ -1,0 +1,242 @@
+"""Benchmark: marker-steps/s of the marker-in-cell transfer pipeline on one
+MI355X.
+
+Workload (default "fp64"): 281x281 basic nodes, 140 km box, 6x6 jittered
+markers per cell = 2,822,400 markers, planetesimal R = 50 km in sticky air,
+stream-function velocity at Courant 0.5, dt = 1e11 s, property mode 1.
+A step is one timestep of the transfer path in the reference loop order
+(src/Erebus.jl:6376-7053) with every input resident in HBM:
+  tile sort + marker properties + scatter (E:6419-6517)
+  node normalisation + insulating BC (E:6522-6604)
+  Stokes-Darcy assembly to CSC (E:6680) and thermal assembly (E:6922)
+  stress gather (E:6980), temperature gather (E:7004), RK4 advection (E:7036)
+The host LU solves are outside the step (they stay on the host in the
+reference's design and are not part of this path); the CSC matrices stay in
+HBM.  The grids that the solves and grid-only operations would produce
+(DT, DSXX, DSXY, tk2, pressures, gravity, ETAP/ETAPHI) are fixed synthetic
+fields (erebus.jl_amd/ereb/synth.py).
+
+Prints one JSON line.  `roofline` is for the scatter kernel (the dominant
+one), measured live with HIP events on the library's stream; `cpu_baseline`
+times the single-threaded C restatement of the reference CPU path (the
+oracle) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "erebus.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+import ereb  # noqa: E402
+from ereb import synth  # noqa: E402
+
+METRIC = "marker-steps/s, 2.8M markers: FP64 vs mixed-precision; adjoint/forward ratio"
+HBM_PEAK_GBS = 8000.0
+START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
+
+# algorithmic bytes of the fused property + scatter kernel: every input
+# marker vector read once (15 Float64 + tm as 4 bytes), the 7 property
+# vectors written once, the 30 SUM grids written once
+SCATTER_MARKER_BYTES = 15 * 8 + 4 + 7 * 8
+SCATTER_NODE_BYTES = 30 * 8
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(ctx: ereb.Context, g, mat, mk):
+    """Upload markers and the synthetic grids; one scatter + node pass
+    provides ETA/PHI for the bulk viscosity and tk1 for tk2."""
+    ctx.set_markers(mk)
+    vx, vy = synth.stream_velocity(g)
+    ctx.set_grid("vx", vx)
+    ctx.set_grid("vy", vy)
+    ctx.set_grid("vxf", vx)
+    ctx.set_grid("vyf", vy)
+    ctx.set_grid("wyx", synth.rotation_rate(g, vx, vy))
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    ctx.step_scatter(hrs, hrf, mat.marker_property_mode)
+    ctx.step_node_properties()
+    ETA, PHI, tk1 = ctx.get_grid("ETA"), ctx.get_grid("PHI"), ctx.get_grid("tk1")
+    ETAP, ETAPHI = synth.bulk_viscosity(ETA, np.maximum(PHI, 1e-4))
+    ctx.set_grid("ETAP", ETAP)
+    ctx.set_grid("ETAPHI", ETAPHI)
+    gx, gy = synth.disk_gravity(g)
+    ctx.set_grid("gx", gx)
+    ctx.set_grid("gy", gy)
+    S = g.stag_shape
+    DT = synth.smooth_field(g, S, 0.05, 2, 1)
+    ctx.set_grid("DT", DT)
+    ctx.set_grid("tk2", np.asfortranarray(tk1 + DT))
+    ctx.set_grid("pr0", np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 1e5, 1, 1))
+    ctx.set_grid("pf0", np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 9e4, 1, 1))
+    for n in ("DMP", "HA", "HS", "DHP"):
+        ctx.set_grid(n, np.zeros(S, order="F"))
+    ctx.set_grid("DSXX", synth.smooth_field(g, S, 1e3, 3, 2))
+    ctx.set_grid("DSXY", synth.smooth_field(g, g.basic_shape, 1e3, 2, 3))
+    ctx.sync()
+    return hrs, hrf
+
+
+def step(ctx, hrs, hrf, mode, dt, timestep):
+    ctx.step_scatter(hrs, hrf, mode)
+    ctx.step_node_properties()
+    ctx.step_assemble_stokes_darcy(dt)
+    ctx.step_assemble_thermal(dt)
+    ctx.step_gather_stress()
+    ctx.step_gather_temperature(timestep)
+    ctx.step_advect_rk4(dt, mode)
+
+
+def cpu_baseline(g, mat, mk, steps: int, budget_s: float = 25.0):
+    """The oracle (single-threaded C restatement of the reference CPU path,
+    -O2, no FMA) on the same workload, in the same step order."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+
+    orc = Oracle(g, mat)
+    mode = mat.marker_property_mode
+    dt = ereb.DT_LONGEST
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    vx, vy = synth.stream_velocity(g)
+    wyx = synth.rotation_rate(g, vx, vy)
+    S = g.stag_shape
+    mk = {k: np.array(v, copy=True) for k, v in mk.items()}
+    # node fields for the assemblies, from one untimed pass
+    props, sums = orc.marker_to_nodes(mk, hrs, hrf, mode)
+    B = g.basic_shape
+    outs_b, _ = orc.basic_node_properties(sums[:8], [np.zeros(B, order="F")] * 7,
+                                          np.zeros(B, np.uint8, order="F"))
+    ETA = outs_b[1]
+    outs_p = orc.p_node_properties(sums[20:30], [np.zeros(S, order="F")] * 10)
+    PHI = np.maximum(outs_p[8], 1e-4)
+    ETAP, ETAPHI = synth.bulk_viscosity(ETA, PHI)
+    gx, gy = synth.disk_gravity(g)
+    DT = synth.smooth_field(g, S, 0.05, 2, 1)
+    tk2 = np.asfortranarray(outs_p[7] + DT)
+    DSXX = synth.smooth_field(g, S, 1e3, 3, 2)
+    DSXY = synth.smooth_field(g, B, 1e3, 2, 3)
+    z = np.zeros(S, order="F")
+    t0 = time.perf_counter()
+    done = 0
+    for s in range(steps):
+        props, sums = orc.marker_to_nodes(mk, hrs, hrf, mode)
+        mk.update(props)
+        ob, _ = orc.basic_node_properties(sums[:8], [np.zeros(B, order="F")] * 7,
+                                          np.zeros(B, np.uint8, order="F"))
+        ovx = orc.v_node_properties(sums[8:14], [z] * 5)
+        ovy = orc.v_node_properties(sums[14:20], [z] * 5)
+        op = orc.p_node_properties(sums[20:30], [z] * 10)
+        tk1 = orc.insulating_bc(op[7])
+        orc.assemble_hydromechanical([ob[1], ETAP, ob[2], op[5], ob[3], op[6], ovx[0], ovy[0],
+                                      ovx[1], ovy[1], ovx[4], ovy[4], ETAPHI, op[9], op[8],
+                                      gx, gy, z, z, z], dt)
+        orc.assemble_thermal([tk1, op[1], ovx[2], ovy[2], op[4], z, z, z], dt)
+        mk["sxxm"], mk["sxym"] = orc.update_marker_stress(mk["xm"], mk["ym"], mk["sxxm"],
+                                                          mk["sxym"], DSXX, DSXY)
+        mk["tkm"] = orc.update_marker_temperature(mk["xm"], mk["ym"], mk["tkm"], DT, tk2, 2)
+        r = orc.move_markers_rk4(mk["xm"], mk["ym"], mk["tm"], mk["tkm"], mk["phim"],
+                                 mk["sxym"], mk["sxxm"], vx, vy, vx, vy, wyx, tk2, dt, mode)
+        mk["xm"], mk["ym"], mk["tkm"], mk["sxym"], mk["sxxm"] = r
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    M = mk["xm"].size
+    return {"value": M * done / el, "unit": "marker-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{done} full timestep(s) of the {M:,}-marker workload, same step "
+                      f"order, single-threaded C restatement of the reference CPU path "
+                      f"(-O2, no FMA), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="fp64", choices=sorted(synth.WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    g, mat = synth.workload_geometry(a.workload, mode=1)
+    mode = mat.marker_property_mode
+    dt = ereb.DT_LONGEST
+    log(f"[bench] workload {a.workload}: {g.Nx}x{g.Ny} nodes, {g.Nxmc}x{g.Nymc} markers/cell")
+    mk = synth.make_markers(g, mat, seed=0)
+    M = mk["xm"].size
+    ctx = ereb.Context(g, mat, ereb.FP64)
+    hrs, hrf = setup(ctx, g, mat, mk)
+    log(f"[bench] {M:,} markers resident; warmup {a.warmup}")
+    ts = 2
+    for _ in range(a.warmup):
+        step(ctx, hrs, hrf, mode, dt, ts)
+        ts += 1
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(ctx, hrs, hrf, mode, dt, ts)
+        ts += 1
+    ctx.sync()
+    el = time.perf_counter() - t0
+    log(f"[bench] timed {a.steps} steps in {el * 1e3:.2f} ms")
+    kern = {}
+    for k in ereb.lib.KERNEL_CLASSES:
+        ms, n = ctx.profile_get(k)
+        kern[k] = round(ms / max(n, 1), 5)
+    sc_ms, sc_n = ctx.profile_get("scatter")
+    nodes = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1
+    sc_bytes = M * SCATTER_MARKER_BYTES + nodes * 8
+    achieved = sc_bytes / (sc_ms / sc_n * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"scatter_{a.workload}_hbm_bytes.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("bytes_per_launch")
+    res = {
+        "metric": METRIC,
+        "value": M * a.steps / el,
+        "unit": "marker-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {g.Nx}x{g.Ny} basic nodes, "
+                               f"{g.Nxmc}x{g.Nymc} jittered markers/cell ({M:,} markers), "
+                               f"planetesimal R={g.rplanet / 1e3:.0f} km in 140 km box, "
+                               f"stream-function velocity Courant 0.5, dt=1e11 s, mode {mode}",
+                   "markers": M, "grid": [g.Ny, g.Nx], "form": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_scatter_fp64 (marker properties + scatter, one launch per step)",
+                     "algorithmic_bytes": sc_bytes},
+        "kernel_ms": kern,
+    }
+    ctx.close()
+    if not a.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle, 1 thread)")
+        res["cpu_baseline"] = cpu_baseline(g, mat, mk, a.cpu_steps)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

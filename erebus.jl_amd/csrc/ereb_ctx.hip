@@ -1,0 +1,807 @@
+// ereb_ctx.hip -- context, resident state and the C-ABI entry points of
+// libereb_hip.so (declared in include/ereb.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ereb_internal.h"
+
+using namespace ereb;
+
+namespace ereb {
+
+int fail(ereb_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(ereb_ctx* c, hipError_t e, const char* what) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+    return fail(c, EREB_ERR_HIP, buf);
+}
+
+GeomK geomk(const ereb_ctx* c) {
+    GeomK g;
+    g.Nx = c->Nx;
+    g.Ny = c->Ny;
+    g.Nx1 = c->Nx1;
+    g.Ny1 = c->Ny1;
+    g.dx = c->dx;
+    g.dy = c->dy;
+    g.idx = 1.0 / c->dx;
+    g.idy = 1.0 / c->dy;
+    g.x = c->d_axes;
+    g.y = c->d_axes + c->Nx;
+    g.xp = c->d_axes + c->Nx + c->Ny;
+    g.yp = c->d_axes + c->Nx + c->Ny + c->Nx1;
+    return g;
+}
+
+bool grid_is_basic(int f) {
+    return (f >= EREB_G_ETA0SUM && f <= EREB_G_DSXY);
+}
+
+double* gptr(ereb_ctx* c, int f) { return c->d_grids + c->goff[f]; }
+double* mptr(ereb_ctx* c, int f) { return (double*)c->d_m[f]; }
+int32_t* tm_ptr(ereb_ctx* c) { return (int32_t*)c->d_m[EREB_M_TM]; }
+
+int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap) {
+    Sys& s = c->sys[which];
+    if (s.n == n && s.cap >= cap) return EREB_OK;
+    hipFree(s.colcnt);
+    hipFree(s.colptr);
+    hipFree(s.cursor);
+    hipFree(s.rowval);
+    hipFree(s.nzval);
+    hipFree(s.rhs);
+    s = Sys();
+    EREB_HIP(c, hipMalloc(&s.colcnt, sizeof(int32_t) * n));
+    EREB_HIP(c, hipMalloc(&s.colptr, sizeof(int32_t) * (n + 1)));
+    EREB_HIP(c, hipMalloc(&s.cursor, sizeof(int32_t) * n));
+    EREB_HIP(c, hipMalloc(&s.rowval, sizeof(int32_t) * cap));
+    EREB_HIP(c, hipMalloc(&s.nzval, sizeof(double) * cap));
+    EREB_HIP(c, hipMalloc(&s.rhs, sizeof(double) * n));
+    s.n = n;
+    s.cap = cap;
+    return EREB_OK;
+}
+
+int ensure_scan(ereb_ctx* c, int64_t n) {
+    int64_t nb = (n + 4095) / 4096 + 1;
+    if (nb <= c->scan_cap) return EREB_OK;
+    hipFree(c->d_scan);
+    c->d_scan = nullptr;
+    EREB_HIP(c, hipMalloc(&c->d_scan, sizeof(int32_t) * (nb + 1)));
+    c->scan_cap = nb;
+    return EREB_OK;
+}
+
+int check_flags(ereb_ctx* c) {
+    int32_t f[4];
+    EREB_HIP(c, hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    if (f[0]) {
+        int32_t z[4] = {0, 0, 0, 0};
+        EREB_HIP(c, hipMemcpy(c->d_flags, z, sizeof z, hipMemcpyHostToDevice));
+        return fail(c, EREB_ERR_DOMAIN,
+                    "marker coordinate outside the supported range (finite |x| or |y| "
+                    "beyond about 1e15 m puts its P cell outside the basic-cell tile)");
+    }
+    return EREB_OK;
+}
+
+void prof_begin(ereb_ctx* c, int kclass, hipEvent_t* a) {
+    *a = nullptr;
+    if (!c->prof) return;
+    if (c->ev_pool.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        c->ev_pool.push_back(e);
+    }
+    *a = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    hipEventRecord(*a, c->stream);
+}
+
+void prof_end(ereb_ctx* c, int kclass, hipEvent_t a) {
+    if (!c->prof || !a) return;
+    hipEvent_t b;
+    if (c->ev_pool.empty()) {
+        if (hipEventCreate(&b) != hipSuccess) return;
+    } else {
+        b = c->ev_pool.back();
+        c->ev_pool.pop_back();
+    }
+    hipEventRecord(b, c->stream);
+    c->prof_pending.push_back({kclass, a, b});
+}
+
+int prof_collect(ereb_ctx* c) {
+    for (auto& p : c->prof_pending) {
+        EREB_HIP(c, hipEventSynchronize(p.b));
+        float ms = 0.f;
+        EREB_HIP(c, hipEventElapsedTime(&ms, p.a, p.b));
+        c->prof_ms[p.kclass] += ms;
+        c->prof_n[p.kclass] += 1;
+        c->ev_pool.push_back(p.a);
+        c->ev_pool.push_back(p.b);
+    }
+    c->prof_pending.clear();
+    return EREB_OK;
+}
+
+}  // namespace ereb
+
+static int valid_mode(ereb_ctx* c, int64_t mode) {
+    if (mode == 1 || mode == 9) return EREB_OK;
+    char buf[64];
+    snprintf(buf, sizeof buf, "unknown mode %lld", (long long)mode);
+    return fail(c, EREB_ERR_MODE, buf);
+}
+
+static int reserve(ereb_ctx* c, int64_t n) {
+    if (n <= c->cap) return EREB_OK;
+    if (n > (int64_t)INT32_MAX) return fail(c, EREB_ERR_ARG, "marknum exceeds 2^31-1");
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < EREB_M_NFIELDS; ++f) {
+        size_t es = f == EREB_M_TM ? sizeof(int32_t) : sizeof(double);
+        void* p = nullptr;
+        EREB_HIP(c, hipMalloc(&p, es * n));
+        if (c->d_m[f]) {
+            EREB_HIP(c, hipMemcpy(p, c->d_m[f], es * c->cap, hipMemcpyDeviceToDevice));
+            hipFree(c->d_m[f]);
+        } else {
+            EREB_HIP(c, hipMemset(p, 0, es * n));
+        }
+        c->d_m[f] = p;
+    }
+    hipFree(c->d_key);
+    hipFree(c->d_perm);
+    hipFree(c->d_tmp);
+    c->d_key = c->d_perm = c->d_tmp = nullptr;
+    EREB_HIP(c, hipMalloc(&c->d_key, sizeof(uint32_t) * n));
+    EREB_HIP(c, hipMalloc(&c->d_perm, sizeof(uint32_t) * n));
+    // oversize tile segments are sorted in [2*start, 2*start + pow2(len)),
+    // disjoint between tiles and inside [0, 2n)
+    EREB_HIP(c, hipMalloc(&c->d_tmp, sizeof(uint32_t) * 2 * n));
+    c->tmp_cap = 2 * n;
+    c->cap = n;
+    c->have_sort = false;
+    return EREB_OK;
+}
+
+extern "C" {
+
+int ereb_version(void) { return 1; }
+
+const char* ereb_last_error(const ereb_ctx* c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int precision,
+                    int device, ereb_ctx** out) {
+    if (!out) return EREB_ERR_ARG;
+    *out = nullptr;
+    if (!g || !mat) return EREB_ERR_ARG;
+    if (precision != EREB_FP64 && precision != EREB_MIXED) return EREB_ERR_ARG;
+    if (g->Nx < 3 || g->Ny < 3 || !(g->dx > 0) || !(g->dy > 0)) return EREB_ERR_ARG;
+    if (!g->x || !g->y || !g->xvx || !g->yvx || !g->xvy || !g->yvy || !g->xp || !g->yp)
+        return EREB_ERR_ARG;
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    if (Nx > 1000000 || Ny > 1000000) return EREB_ERR_ARG;
+    // the staggered axes coincide element for element (C:40-64)
+    for (int64_t k = 0; k < Nx; ++k)
+        if (g->xvx[k] != g->x[k]) return EREB_ERR_ARG;
+    for (int64_t k = 0; k < Ny1; ++k)
+        if (g->yvx[k] != g->yp[k]) return EREB_ERR_ARG;
+    for (int64_t k = 0; k < Nx1; ++k)
+        if (g->xvy[k] != g->xp[k]) return EREB_ERR_ARG;
+    for (int64_t k = 0; k < Ny; ++k)
+        if (g->yvy[k] != g->y[k]) return EREB_ERR_ARG;
+
+    ereb_ctx* c = new (std::nothrow) ereb_ctx();
+    if (!c) return EREB_ERR_NOMEM;
+    c->device = device;
+    c->precision = precision;
+    c->Nx = (int)Nx;
+    c->Ny = (int)Ny;
+    c->Nx1 = (int)Nx1;
+    c->Ny1 = (int)Ny1;
+    c->dx = g->dx;
+    c->dy = g->dy;
+    c->mat = *mat;
+    int rc = EREB_OK;
+    auto bail = [&](int code) {
+        *out = c;  // caller reads the message, then destroys
+        return code;
+    };
+#define CK(call)                                             \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return bail(hipfail(c, e_, #call)); \
+    } while (0)
+    CK(hipSetDevice(device));
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // axes
+    std::vector<double> ax;
+    ax.insert(ax.end(), g->x, g->x + Nx);
+    ax.insert(ax.end(), g->y, g->y + Ny);
+    ax.insert(ax.end(), g->xp, g->xp + Nx1);
+    ax.insert(ax.end(), g->yp, g->yp + Ny1);
+    CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
+    CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
+    // grids, 256-byte aligned
+    size_t off = 0;
+    for (int f = 0; f < EREB_G_NFIELDS; ++f) {
+        int64_t n = grid_is_basic(f) ? Nx * Ny : Nx1 * Ny1;
+        c->gnumel[f] = n;
+        c->goff[f] = off;
+        off += ((size_t)n + 31) / 32 * 32;
+    }
+    CK(hipMalloc(&c->d_grids, sizeof(double) * off));
+    CK(hipMemset(c->d_grids, 0, sizeof(double) * off));
+    CK(hipMalloc(&c->d_yny, (size_t)Nx * Ny));
+    CK(hipMemset(c->d_yny, 0, (size_t)Nx * Ny));
+    // tiles
+    c->nty = (int)((Ny - 1 + TILE - 1) / TILE);
+    c->ntx = (int)((Nx - 1 + TILE - 1) / TILE);
+    c->ntiles = c->ntx * c->nty;
+    CK(hipMalloc(&c->d_tcount, sizeof(int32_t) * c->ntiles));
+    CK(hipMalloc(&c->d_tstart, sizeof(int32_t) * (c->ntiles + 1)));
+    CK(hipMalloc(&c->d_tcursor, sizeof(int32_t) * c->ntiles));
+    CK(hipMalloc(&c->d_big, sizeof(int32_t) * (c->ntiles + 1)));
+    CK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
+    CK(hipMemset(c->d_flags, 0, sizeof(int32_t) * 4));
+    CK(hipMalloc(&c->d_part, sizeof(double) * (size_t)c->ntiles * NSUM * 64));
+#undef CK
+    rc = ensure_scan(c, std::max<int64_t>(6 * Nx1 * Ny1, c->ntiles));
+    if (rc) return bail(rc);
+    *out = c;
+    return EREB_OK;
+}
+
+int ereb_ctx_destroy(ereb_ctx* c) {
+    if (!c) return EREB_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& p : c->prof_pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (int f = 0; f < EREB_M_NFIELDS; ++f) hipFree(c->d_m[f]);
+    hipFree(c->d_axes);
+    hipFree(c->d_grids);
+    hipFree(c->d_yny);
+    hipFree(c->d_key);
+    hipFree(c->d_perm);
+    hipFree(c->d_tmp);
+    hipFree(c->d_tcount);
+    hipFree(c->d_tstart);
+    hipFree(c->d_tcursor);
+    hipFree(c->d_big);
+    hipFree(c->d_flags);
+    hipFree(c->d_part);
+    hipFree(c->d_scan);
+    for (auto& s : c->sys) {
+        hipFree(s.colcnt);
+        hipFree(s.colptr);
+        hipFree(s.cursor);
+        hipFree(s.rowval);
+        hipFree(s.nzval);
+        hipFree(s.rhs);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return EREB_OK;
+}
+
+int ereb_ctx_set_precision(ereb_ctx* c, int precision) {
+    if (!c) return EREB_ERR_ARG;
+    if (precision != EREB_FP64 && precision != EREB_MIXED)
+        return fail(c, EREB_ERR_ARG, "precision must be EREB_FP64 or EREB_MIXED");
+    c->precision = precision;
+    return EREB_OK;
+}
+
+int ereb_ctx_reserve(ereb_ctx* c, int64_t n) {
+    if (!c || n < 0) return EREB_ERR_ARG;
+    return reserve(c, n);
+}
+
+int ereb_sync(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return check_flags(c);
+}
+
+int64_t ereb_marknum(const ereb_ctx* c) { return c ? c->marknum : -1; }
+
+int ereb_markers_set(ereb_ctx* c, int field, const void* host, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (field < 0 || field >= EREB_M_NFIELDS || marknum < 0 || (!host && marknum > 0))
+        return fail(c, EREB_ERR_ARG, "ereb_markers_set: bad field or size");
+    int rc = reserve(c, std::max<int64_t>(marknum, 1));
+    if (rc) return rc;
+    if (field == EREB_M_TM) {
+        const int64_t* t = (const int64_t*)host;
+        std::vector<int32_t> t32((size_t)marknum);
+        for (int64_t m = 0; m < marknum; ++m) {
+            if (t[m] < 1 || t[m] > 3)
+                return fail(c, EREB_ERR_ARG, "marker type tm must be 1, 2 or 3");
+            t32[m] = (int32_t)t[m];
+        }
+        EREB_HIP(c, hipMemcpyAsync(c->d_m[field], t32.data(), sizeof(int32_t) * marknum,
+                                   hipMemcpyHostToDevice, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+    } else if (marknum > 0) {
+        EREB_HIP(c, hipMemcpyAsync(c->d_m[field], host, sizeof(double) * marknum,
+                                   hipMemcpyHostToDevice, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->marknum = marknum;
+    return EREB_OK;
+}
+
+int ereb_markers_get(ereb_ctx* c, int field, void* host, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (field < 0 || field >= EREB_M_NFIELDS || marknum < 0 || marknum > c->cap ||
+        (!host && marknum > 0))
+        return fail(c, EREB_ERR_ARG, "ereb_markers_get: bad field or size");
+    int rc = check_flags(c);
+    if (rc) return rc;
+    if (marknum == 0) return EREB_OK;
+    if (field == EREB_M_TM) {
+        std::vector<int32_t> t32((size_t)marknum);
+        EREB_HIP(c, hipMemcpyAsync(t32.data(), c->d_m[field], sizeof(int32_t) * marknum,
+                                   hipMemcpyDeviceToHost, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+        int64_t* t = (int64_t*)host;
+        for (int64_t m = 0; m < marknum; ++m) t[m] = t32[m];
+    } else {
+        EREB_HIP(c, hipMemcpyAsync(host, c->d_m[field], sizeof(double) * marknum,
+                                   hipMemcpyDeviceToHost, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return EREB_OK;
+}
+
+int64_t ereb_grid_numel(const ereb_ctx* c, int field) {
+    if (!c || field < 0 || field >= EREB_G_NFIELDS) return -1;
+    return c->gnumel[field];
+}
+
+int ereb_grid_set(ereb_ctx* c, int field, const double* host) {
+    if (!c) return EREB_ERR_ARG;
+    if (field < 0 || field >= EREB_G_NFIELDS || !host)
+        return fail(c, EREB_ERR_ARG, "ereb_grid_set: bad field");
+    EREB_HIP(c, hipMemcpyAsync(gptr(c, field), host, sizeof(double) * c->gnumel[field],
+                               hipMemcpyHostToDevice, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return EREB_OK;
+}
+
+int ereb_grid_get(ereb_ctx* c, int field, double* host) {
+    if (!c) return EREB_ERR_ARG;
+    if (field < 0 || field >= EREB_G_NFIELDS || !host)
+        return fail(c, EREB_ERR_ARG, "ereb_grid_get: bad field");
+    int rc = check_flags(c);
+    if (rc) return rc;
+    EREB_HIP(c, hipMemcpyAsync(host, gptr(c, field), sizeof(double) * c->gnumel[field],
+                               hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return EREB_OK;
+}
+
+int ereb_yny_set(ereb_ctx* c, const uint8_t* host) {
+    if (!c || !host) return EREB_ERR_ARG;
+    EREB_HIP(c, hipMemcpyAsync(c->d_yny, host, (size_t)c->Nx * c->Ny, hipMemcpyHostToDevice,
+                               c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return EREB_OK;
+}
+
+int ereb_yny_get(ereb_ctx* c, uint8_t* host) {
+    if (!c || !host) return EREB_ERR_ARG;
+    EREB_HIP(c, hipMemcpyAsync(host, c->d_yny, (size_t)c->Nx * c->Ny, hipMemcpyDeviceToHost,
+                               c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return EREB_OK;
+}
+
+// ---- resident steps -------------------------------------------------------
+
+int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrfluidm[3],
+                      int64_t mode) {
+    if (!c || !hrsolidm || !hrfluidm) return EREB_ERR_ARG;
+    int rc = valid_mode(c, mode);
+    if (rc) return rc;
+    if ((rc = reserve(c, std::max<int64_t>(c->marknum, 1)))) return rc;
+    if ((rc = launch_tile_sort(c))) return rc;
+    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, (int)mode, true))) return rc;
+    return launch_combine(c);
+}
+
+int ereb_step_node_properties(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_node_properties(c, true, true, true, true, true);
+}
+
+int ereb_step_gather_temperature(ereb_ctx* c, int64_t timestep) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_gather_temperature(c, timestep);
+}
+
+int ereb_step_gather_stress(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_gather_stress(c);
+}
+
+int ereb_step_advect_rk4(ereb_ctx* c, double dt, int64_t mode) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = valid_mode(c, mode);
+    if (rc) return rc;
+    return launch_rk4(c, dt, (int)mode);
+}
+
+static int fetch_nnz(ereb_ctx* c, int which, int64_t* nnz) {
+    if (!nnz) return EREB_OK;
+    int32_t v = 0;
+    Sys& s = c->sys[which];
+    EREB_HIP(c, hipMemcpyAsync(&v, s.colptr + s.n, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    *nnz = v;
+    return EREB_OK;
+}
+
+int ereb_step_assemble_stokes_darcy(ereb_ctx* c, double dt, int64_t* nnz) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = launch_assemble_stokes(c, dt);
+    if (rc) return rc;
+    return fetch_nnz(c, EREB_SYS_STOKES_DARCY, nnz);
+}
+
+int ereb_step_assemble_thermal(ereb_ctx* c, double dt, int64_t* nnz) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = launch_assemble_thermal(c, dt);
+    if (rc) return rc;
+    return fetch_nnz(c, EREB_SYS_THERMAL, nnz);
+}
+
+int ereb_csc_size(const ereb_ctx* c, int which, int64_t* n, int64_t* nnz) {
+    if (!c || which < 0 || which > 1) return EREB_ERR_ARG;
+    const Sys& s = c->sys[which];
+    if (!s.valid) return EREB_ERR_STATE;
+    if (n) *n = s.n;
+    if (nnz) {
+        int32_t v = 0;
+        if (hipMemcpy(&v, s.colptr + s.n, sizeof v, hipMemcpyDeviceToHost) != hipSuccess)
+            return EREB_ERR_HIP;
+        *nnz = v;
+    }
+    return EREB_OK;
+}
+
+int ereb_csc_get(ereb_ctx* c, int which, int64_t* colptr, int64_t* rowval, double* nzval,
+                 double* rhs) {
+    if (!c || which < 0 || which > 1) return EREB_ERR_ARG;
+    Sys& s = c->sys[which];
+    if (!s.valid) return fail(c, EREB_ERR_STATE, "no system assembled yet");
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<int32_t> cp((size_t)s.n + 1);
+    EREB_HIP(c, hipMemcpy(cp.data(), s.colptr, sizeof(int32_t) * (s.n + 1), hipMemcpyDeviceToHost));
+    int64_t nnz = cp[s.n];
+    if (colptr)
+        for (int64_t k = 0; k <= s.n; ++k) colptr[k] = (int64_t)cp[k] + 1;
+    if (rowval && nnz > 0) {
+        std::vector<int32_t> rv((size_t)nnz);
+        EREB_HIP(c, hipMemcpy(rv.data(), s.rowval, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < nnz; ++k) rowval[k] = (int64_t)rv[k] + 1;
+    }
+    if (nzval && nnz > 0)
+        EREB_HIP(c, hipMemcpy(nzval, s.nzval, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    if (rhs) EREB_HIP(c, hipMemcpy(rhs, s.rhs, sizeof(double) * s.n, hipMemcpyDeviceToHost));
+    return EREB_OK;
+}
+
+int ereb_sort_get(ereb_ctx* c, int32_t* perm, int32_t* tile_key, int64_t* ntiles) {
+    if (!c) return EREB_ERR_ARG;
+    if (!c->have_sort) return fail(c, EREB_ERR_STATE, "no scatter has run");
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    if (perm && c->marknum)
+        EREB_HIP(c, hipMemcpy(perm, c->d_perm, sizeof(int32_t) * c->marknum, hipMemcpyDeviceToHost));
+    if (tile_key && c->marknum)
+        EREB_HIP(c, hipMemcpy(tile_key, c->d_key, sizeof(int32_t) * c->marknum, hipMemcpyDeviceToHost));
+    if (ntiles) *ntiles = c->ntiles;
+    return EREB_OK;
+}
+
+// ---- profiling --------------------------------------------------------------
+
+int ereb_profile_enable(ereb_ctx* c, int on) {
+    if (!c) return EREB_ERR_ARG;
+    c->prof = on != 0;
+    return EREB_OK;
+}
+
+int ereb_profile_get(ereb_ctx* c, int k, double* total_ms, int64_t* count) {
+    if (!c || k < 0 || k >= EREB_K_NCLASSES) return EREB_ERR_ARG;
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    if (total_ms) *total_ms = c->prof_ms[k];
+    if (count) *count = c->prof_n[k];
+    return EREB_OK;
+}
+
+int ereb_profile_reset(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = prof_collect(c);
+    if (rc) return rc;
+    for (int k = 0; k < EREB_K_NCLASSES; ++k) {
+        c->prof_ms[k] = 0;
+        c->prof_n[k] = 0;
+    }
+    return EREB_OK;
+}
+
+// ---- drop-in entry points -----------------------------------------------------
+
+#define SETM(f, p)                                              \
+    do {                                                        \
+        int rc_ = ereb_markers_set(c, f, p, marknum);           \
+        if (rc_) return rc_;                                    \
+    } while (0)
+#define GETM(f, p)                                              \
+    do {                                                        \
+        int rc_ = ereb_markers_get(c, f, p, marknum);           \
+        if (rc_) return rc_;                                    \
+    } while (0)
+#define SETG(f, p)                                              \
+    do {                                                        \
+        int rc_ = ereb_grid_set(c, f, p);                       \
+        if (rc_) return rc_;                                    \
+    } while (0)
+#define GETG(f, p)                                              \
+    do {                                                        \
+        int rc_ = ereb_grid_get(c, f, p);                       \
+        if (rc_) return rc_;                                    \
+    } while (0)
+
+static const int kSumField[NSUM] = {
+    EREB_G_ETA0SUM, EREB_G_ETASUM, EREB_G_GGGSUM, EREB_G_SXYSUM, EREB_G_COHSUM,
+    EREB_G_TENSUM, EREB_G_FRISUM, EREB_G_WTSUM,
+    EREB_G_RHOXSUM, EREB_G_RHOFXSUM, EREB_G_KXSUM, EREB_G_PHIXSUM, EREB_G_RXSUM, EREB_G_WTXSUM,
+    EREB_G_RHOYSUM, EREB_G_RHOFYSUM, EREB_G_KYSUM, EREB_G_PHIYSUM, EREB_G_RYSUM, EREB_G_WTYSUM,
+    EREB_G_RHOSUM, EREB_G_RHOCPSUM, EREB_G_ALPHASUM, EREB_G_ALPHAFSUM, EREB_G_HRSUM,
+    EREB_G_GGGPSUM, EREB_G_SXXSUM, EREB_G_TKSUM, EREB_G_PHISUM, EREB_G_WTPSUM};
+
+int ereb_marker_to_nodes(ereb_ctx* c, const double* xm, const double* ym, const int64_t* tm,
+    const double* tkm, const double* phim, const double* XWsolidm0, const double* etavpm,
+    const double* inv_gggtotalm, const double* fricttotalm, const double* cohestotalm,
+    const double* tenstotalm, const double* rhofluidcur, const double* alphasolidcur,
+    const double* alphafluidcur, const double* sxxm, const double* sxym, double* rhototalm,
+    double* rhocptotalm, double* etatotalm, double* hrtotalm, double* ktotalm,
+    double* tkm_rhocptotalm, double* etafluidcur_inv_kphim, const double hrsolidm[3],
+    const double hrfluidm[3], int64_t mode, double* const sums[30], int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (!sums || marknum < 0) return fail(c, EREB_ERR_ARG, "ereb_marker_to_nodes: bad arguments");
+    int rc = valid_mode(c, mode);
+    if (rc) return rc;
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TM, tm);
+    SETM(EREB_M_TKM, tkm);
+    SETM(EREB_M_PHIM, phim);
+    SETM(EREB_M_XWSOLIDM0, XWsolidm0);
+    SETM(EREB_M_ETAVPM, etavpm);
+    SETM(EREB_M_INV_GGGTOTALM, inv_gggtotalm);
+    SETM(EREB_M_FRICTTOTALM, fricttotalm);
+    SETM(EREB_M_COHESTOTALM, cohestotalm);
+    SETM(EREB_M_TENSTOTALM, tenstotalm);
+    SETM(EREB_M_RHOFLUIDCUR, rhofluidcur);
+    SETM(EREB_M_ALPHASOLIDCUR, alphasolidcur);
+    SETM(EREB_M_ALPHAFLUIDCUR, alphafluidcur);
+    SETM(EREB_M_SXXM, sxxm);
+    SETM(EREB_M_SXYM, sxym);
+    // stored air properties are inputs (E:651-655), rock values are outputs
+    SETM(EREB_M_RHOTOTALM, rhototalm);
+    SETM(EREB_M_RHOCPTOTALM, rhocptotalm);
+    SETM(EREB_M_ETATOTALM, etatotalm);
+    SETM(EREB_M_HRTOTALM, hrtotalm);
+    SETM(EREB_M_KTOTALM, ktotalm);
+    rc = ereb_step_scatter(c, hrsolidm, hrfluidm, mode);
+    if (rc) return rc;
+    GETM(EREB_M_RHOTOTALM, rhototalm);
+    GETM(EREB_M_RHOCPTOTALM, rhocptotalm);
+    GETM(EREB_M_ETATOTALM, etatotalm);
+    GETM(EREB_M_HRTOTALM, hrtotalm);
+    GETM(EREB_M_KTOTALM, ktotalm);
+    GETM(EREB_M_TKM_RHOCPTOTALM, tkm_rhocptotalm);
+    GETM(EREB_M_ETAFLUIDCUR_INV_KPHIM, etafluidcur_inv_kphim);
+    for (int s = 0; s < NSUM; ++s) {
+        if (!sums[s]) continue;
+        GETG(kSumField[s], sums[s]);
+    }
+    return EREB_OK;
+}
+
+int ereb_compute_basic_node_properties(ereb_ctx* c, const double* ETA0SUM, const double* ETASUM,
+    const double* GGGSUM, const double* SXYSUM, const double* COHSUM, const double* TENSUM,
+    const double* FRISUM, const double* WTSUM, double* ETA0, double* ETA, double* GGG,
+    double* SXY0, double* COH, double* TEN, double* FRI, uint8_t* YNY) {
+    if (!c || !YNY) return EREB_ERR_ARG;
+    SETG(EREB_G_ETA0SUM, ETA0SUM); SETG(EREB_G_ETASUM, ETASUM); SETG(EREB_G_GGGSUM, GGGSUM);
+    SETG(EREB_G_SXYSUM, SXYSUM); SETG(EREB_G_COHSUM, COHSUM); SETG(EREB_G_TENSUM, TENSUM);
+    SETG(EREB_G_FRISUM, FRISUM); SETG(EREB_G_WTSUM, WTSUM);
+    SETG(EREB_G_ETA0, ETA0); SETG(EREB_G_ETA, ETA); SETG(EREB_G_GGG, GGG);
+    SETG(EREB_G_SXY0, SXY0); SETG(EREB_G_COH, COH); SETG(EREB_G_TEN, TEN); SETG(EREB_G_FRI, FRI);
+    int rc = ereb_yny_set(c, YNY);
+    if (rc) return rc;
+    if ((rc = launch_node_properties(c, true, false, false, false, false))) return rc;
+    GETG(EREB_G_ETA0, ETA0); GETG(EREB_G_ETA, ETA); GETG(EREB_G_GGG, GGG);
+    GETG(EREB_G_SXY0, SXY0); GETG(EREB_G_COH, COH); GETG(EREB_G_TEN, TEN); GETG(EREB_G_FRI, FRI);
+    return ereb_yny_get(c, YNY);
+}
+
+int ereb_compute_vx_node_properties(ereb_ctx* c, const double* RHOXSUM, const double* RHOFXSUM,
+    const double* KXSUM, const double* PHIXSUM, const double* RXSUM, const double* WTXSUM,
+    double* RHOX, double* RHOFX, double* KX, double* PHIX, double* RX) {
+    if (!c) return EREB_ERR_ARG;
+    SETG(EREB_G_RHOXSUM, RHOXSUM); SETG(EREB_G_RHOFXSUM, RHOFXSUM); SETG(EREB_G_KXSUM, KXSUM);
+    SETG(EREB_G_PHIXSUM, PHIXSUM); SETG(EREB_G_RXSUM, RXSUM); SETG(EREB_G_WTXSUM, WTXSUM);
+    SETG(EREB_G_RHOX, RHOX); SETG(EREB_G_RHOFX, RHOFX); SETG(EREB_G_KX, KX);
+    SETG(EREB_G_PHIX, PHIX); SETG(EREB_G_RX, RX);
+    int rc = launch_node_properties(c, false, true, false, false, false);
+    if (rc) return rc;
+    GETG(EREB_G_RHOX, RHOX); GETG(EREB_G_RHOFX, RHOFX); GETG(EREB_G_KX, KX);
+    GETG(EREB_G_PHIX, PHIX); GETG(EREB_G_RX, RX);
+    return EREB_OK;
+}
+
+int ereb_compute_vy_node_properties(ereb_ctx* c, const double* RHOYSUM, const double* RHOFYSUM,
+    const double* KYSUM, const double* PHIYSUM, const double* RYSUM, const double* WTYSUM,
+    double* RHOY, double* RHOFY, double* KY, double* PHIY, double* RY) {
+    if (!c) return EREB_ERR_ARG;
+    SETG(EREB_G_RHOYSUM, RHOYSUM); SETG(EREB_G_RHOFYSUM, RHOFYSUM); SETG(EREB_G_KYSUM, KYSUM);
+    SETG(EREB_G_PHIYSUM, PHIYSUM); SETG(EREB_G_RYSUM, RYSUM); SETG(EREB_G_WTYSUM, WTYSUM);
+    SETG(EREB_G_RHOY, RHOY); SETG(EREB_G_RHOFY, RHOFY); SETG(EREB_G_KY, KY);
+    SETG(EREB_G_PHIY, PHIY); SETG(EREB_G_RY, RY);
+    int rc = launch_node_properties(c, false, false, true, false, false);
+    if (rc) return rc;
+    GETG(EREB_G_RHOY, RHOY); GETG(EREB_G_RHOFY, RHOFY); GETG(EREB_G_KY, KY);
+    GETG(EREB_G_PHIY, PHIY); GETG(EREB_G_RY, RY);
+    return EREB_OK;
+}
+
+int ereb_compute_p_node_properties(ereb_ctx* c, const double* RHOSUM, const double* RHOCPSUM,
+    const double* ALPHASUM, const double* ALPHAFSUM, const double* HRSUM, const double* GGGPSUM,
+    const double* SXXSUM, const double* TKSUM, const double* PHISUM, const double* WTPSUM,
+    double* RHO, double* RHOCP, double* ALPHA, double* ALPHAF, double* HR, double* GGGP,
+    double* SXX0, double* tk1, double* PHI, double* BETAPHI) {
+    if (!c) return EREB_ERR_ARG;
+    SETG(EREB_G_RHOSUM, RHOSUM); SETG(EREB_G_RHOCPSUM, RHOCPSUM); SETG(EREB_G_ALPHASUM, ALPHASUM);
+    SETG(EREB_G_ALPHAFSUM, ALPHAFSUM); SETG(EREB_G_HRSUM, HRSUM); SETG(EREB_G_GGGPSUM, GGGPSUM);
+    SETG(EREB_G_SXXSUM, SXXSUM); SETG(EREB_G_TKSUM, TKSUM); SETG(EREB_G_PHISUM, PHISUM);
+    SETG(EREB_G_WTPSUM, WTPSUM);
+    SETG(EREB_G_RHO, RHO); SETG(EREB_G_RHOCP, RHOCP); SETG(EREB_G_ALPHA, ALPHA);
+    SETG(EREB_G_ALPHAF, ALPHAF); SETG(EREB_G_HR, HR); SETG(EREB_G_GGGP, GGGP);
+    SETG(EREB_G_SXX0, SXX0); SETG(EREB_G_TK1, tk1); SETG(EREB_G_PHI, PHI);
+    SETG(EREB_G_BETAPHI, BETAPHI);
+    int rc = launch_node_properties(c, false, false, false, true, false);
+    if (rc) return rc;
+    GETG(EREB_G_RHO, RHO); GETG(EREB_G_RHOCP, RHOCP); GETG(EREB_G_ALPHA, ALPHA);
+    GETG(EREB_G_ALPHAF, ALPHAF); GETG(EREB_G_HR, HR); GETG(EREB_G_GGGP, GGGP);
+    GETG(EREB_G_SXX0, SXX0); GETG(EREB_G_TK1, tk1); GETG(EREB_G_PHI, PHI);
+    GETG(EREB_G_BETAPHI, BETAPHI);
+    return EREB_OK;
+}
+
+int ereb_apply_insulating_boundary_conditions(ereb_ctx* c, double* t, int64_t Nyy, int64_t Nxx) {
+    if (!c || !t || Nyy < 0 || Nxx < 0) return EREB_ERR_ARG;
+    if (!(Nyy > 2 && Nxx > 2)) return EREB_OK;  // E:2435
+    double* d = nullptr;
+    size_t bytes = sizeof(double) * Nyy * Nxx;
+    EREB_HIP(c, hipMalloc(&d, bytes));
+    int rc = EREB_OK;
+    if (hipMemcpy(d, t, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = EREB_ERR_HIP;
+    if (!rc) rc = launch_insulating_bc(c, d, Nyy, Nxx);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = EREB_ERR_HIP;
+    if (!rc && hipMemcpy(t, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = EREB_ERR_HIP;
+    hipFree(d);
+    if (rc == EREB_ERR_HIP) return fail(c, rc, "HIP error in insulating boundary conditions");
+    return rc;
+}
+
+int ereb_update_marker_temperature(ereb_ctx* c, const double* xm, const double* ym, double* tkm,
+    const double* DT, const double* tk2, int64_t timestep, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TKM, tkm);
+    if (timestep == 1) { SETG(EREB_G_TK2, tk2); } else { SETG(EREB_G_DT, DT); }
+    int rc = launch_gather_temperature(c, timestep);
+    if (rc) return rc;
+    GETM(EREB_M_TKM, tkm);
+    return EREB_OK;
+}
+
+int ereb_update_marker_stress(ereb_ctx* c, const double* xm, const double* ym, double* sxxm,
+    double* sxym, const double* DSXX, const double* DSXY, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_SXXM, sxxm);
+    SETM(EREB_M_SXYM, sxym);
+    SETG(EREB_G_DSXX, DSXX);
+    SETG(EREB_G_DSXY, DSXY);
+    int rc = launch_gather_stress(c);
+    if (rc) return rc;
+    GETM(EREB_M_SXXM, sxxm);
+    GETM(EREB_M_SXYM, sxym);
+    return EREB_OK;
+}
+
+int ereb_move_markers_rk4(ereb_ctx* c, double* xm, double* ym, const int64_t* tm, double* tkm,
+    const double* phim, double* sxym, double* sxxm, const double* vx, const double* vy,
+    const double* vxf, const double* vyf, const double* wyx, const double* tk2, int64_t marknum,
+    double dt, int64_t mode) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = valid_mode(c, mode);
+    if (rc) return rc;
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TM, tm);
+    SETM(EREB_M_TKM, tkm);
+    SETM(EREB_M_PHIM, phim);
+    SETM(EREB_M_SXYM, sxym);
+    SETM(EREB_M_SXXM, sxxm);
+    SETG(EREB_G_VX, vx);
+    SETG(EREB_G_VY, vy);
+    SETG(EREB_G_VXF, vxf);
+    SETG(EREB_G_VYF, vyf);
+    SETG(EREB_G_WYX, wyx);
+    SETG(EREB_G_TK2, tk2);
+    if ((rc = launch_rk4(c, dt, (int)mode))) return rc;
+    GETM(EREB_M_XM, xm);
+    GETM(EREB_M_YM, ym);
+    GETM(EREB_M_TKM, tkm);
+    GETM(EREB_M_SXYM, sxym);
+    GETM(EREB_M_SXXM, sxxm);
+    return EREB_OK;
+}
+
+int ereb_assemble_hydromechanical_lse(ereb_ctx* c, const double* ETA, const double* ETAP,
+    const double* GGG, const double* GGGP, const double* SXY0, const double* SXX0,
+    const double* RHOX, const double* RHOY, const double* RHOFX, const double* RHOFY,
+    const double* RX, const double* RY, const double* ETAPHI, const double* BETAPHI,
+    const double* PHI, const double* gx, const double* gy, const double* pr0, const double* pf0,
+    const double* DMP, double dt, double* R, int64_t* nnz) {
+    if (!c || !R) return EREB_ERR_ARG;
+    SETG(EREB_G_ETA, ETA); SETG(EREB_G_ETAP, ETAP); SETG(EREB_G_GGG, GGG);
+    SETG(EREB_G_GGGP, GGGP); SETG(EREB_G_SXY0, SXY0); SETG(EREB_G_SXX0, SXX0);
+    SETG(EREB_G_RHOX, RHOX); SETG(EREB_G_RHOY, RHOY); SETG(EREB_G_RHOFX, RHOFX);
+    SETG(EREB_G_RHOFY, RHOFY); SETG(EREB_G_RX, RX); SETG(EREB_G_RY, RY);
+    SETG(EREB_G_ETAPHI, ETAPHI); SETG(EREB_G_BETAPHI, BETAPHI); SETG(EREB_G_PHI, PHI);
+    SETG(EREB_G_GX, gx); SETG(EREB_G_GY, gy); SETG(EREB_G_PR0, pr0); SETG(EREB_G_PF0, pf0);
+    SETG(EREB_G_DMP, DMP);
+    int rc = ereb_step_assemble_stokes_darcy(c, dt, nnz);
+    if (rc) return rc;
+    return ereb_csc_get(c, EREB_SYS_STOKES_DARCY, nullptr, nullptr, nullptr, R);
+}
+
+int ereb_assemble_thermal_lse(ereb_ctx* c, const double* tk1, const double* RHOCP,
+    const double* KX, const double* KY, const double* HR, const double* HA, const double* HS,
+    const double* DHP, double* RT, double dt, int64_t* nnz) {
+    if (!c || !RT) return EREB_ERR_ARG;
+    SETG(EREB_G_TK1, tk1); SETG(EREB_G_RHOCP, RHOCP); SETG(EREB_G_KX, KX); SETG(EREB_G_KY, KY);
+    SETG(EREB_G_HR, HR); SETG(EREB_G_HA, HA); SETG(EREB_G_HS, HS); SETG(EREB_G_DHP, DHP);
+    int rc = ereb_step_assemble_thermal(c, dt, nnz);
+    if (rc) return rc;
+    return ereb_csc_get(c, EREB_SYS_THERMAL, nullptr, nullptr, nullptr, RT);
+}
+
+}  // extern "C"

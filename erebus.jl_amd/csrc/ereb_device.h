@@ -1,0 +1,157 @@
+// ereb_device.h -- device-side arithmetic shared by the gfx950 kernels.
+//
+// Every expression follows the reference operator by operator so that, with
+// -ffp-contract=off, integer cell indices and Float64 values agree bit for bit
+// with the CPU restatement (E: = src/Erebus.jl).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ereb.h"
+
+namespace ereb {
+
+// Geometry as the kernels see it.  The Vx axes equal (x, yp) and the Vy axes
+// equal (xp, y) element for element (checked when the context is created),
+// so four axis arrays describe all four grids.
+struct GeomK {
+    int Nx, Ny, Nx1, Ny1;
+    double dx, dy, idx, idy;  // idx = inv(dx) = 1.0/dx as in fix (E:1457)
+    const double* x;          // basic x axis (Nx), = xvx
+    const double* y;          // basic y axis (Ny), = yvy
+    const double* xp;         // P x axis (Nx1), = xvy
+    const double* yp;         // P y axis (Ny1), = yvx
+};
+
+// unsafe_trunc(Int, t) + 1 clamped to [lo, hi] (E:1457-1468).  NaN and
+// |t| >= 2^63 give typemin(Int64) on x86-64, hence lo.
+__device__ __forceinline__ int jl_cell(double t, int lo, int hi) {
+    if (!(t >= -9223372036854775808.0 && t < 9223372036854775808.0)) return lo;
+    if (t >= 1.0e9) return hi;
+    if (t <= -1.0e9) return lo;
+    int c = (int)t + 1;
+    return c < lo ? lo : (c > hi ? hi : c);
+}
+
+// Cell indices of a marker: basic (ib, jb) with the basic clamps and P
+// (ip, jp) with the P clamps.  Vx cell = (ip, jb), Vy cell = (ib, jp).
+struct Cells {
+    int ib, jb, ip, jp;
+};
+
+__device__ __forceinline__ Cells marker_cells(const GeomK& g, double xm, double ym) {
+    Cells c;
+    c.jb = jl_cell((xm - g.x[0]) * g.idx, 1, g.Nx - 1);
+    c.ib = jl_cell((ym - g.y[0]) * g.idy, 1, g.Ny - 1);
+    c.jp = jl_cell((xm - g.xp[0]) * g.idx, 1, g.Nx);
+    c.ip = jl_cell((ym - g.yp[0]) * g.idy, 1, g.Ny);
+    return c;
+}
+
+// fix_weights (E:1388-1397) for a cell already fixed: returns the four
+// weights [(1-b)(1-a), b(1-a), (1-b)a, ba] with a = dxmj/dx, b = dymi/dy.
+__device__ __forceinline__ void weights4(double a, double b, double* w) {
+    w[0] = (1.0 - b) * (1.0 - a);
+    w[1] = b * (1.0 - a);
+    w[2] = (1.0 - b) * a;
+    w[3] = b * a;
+}
+
+// dot4(grid_vector(i, j, grid), w) (E:1137-1159), 1-based (i, j)
+__device__ __forceinline__ double gdot4(const double* __restrict__ gr, int rows,
+                                        int i, int j, const double* w) {
+    const double* p = gr + (size_t)(j - 1) * rows + (i - 1);
+    return p[0] * w[0] + p[1] * w[1] + p[rows] * w[2] + p[rows + 1] * w[3];
+}
+
+// ---- marker properties: compute_marker_properties! (E:700-750) ----------
+
+__device__ __forceinline__ double jl_max(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a > b ? a : (b > a ? b : (signbit(a) ? b : a));
+}
+
+// compute_rhocpfluidm (E:2272-2293); mode already validated (1 or 9)
+__device__ __forceinline__ double rhocpfluidm(const ereb_materials& M, double T, int mode) {
+    if (mode == 9) return M.rhocpfluidm[0];
+    double tf = M.tmfluidphase;
+    if (T < tf - 5.0) return M.rhoH2Ofi * (7.67 * T);
+    if (T < tf) return M.rhoH2Ofi * (7.67 * T + 0.1 * M.Lf);
+    if (T < tf + 5.0) return M.rhoH2Of * (4200.0 + 0.1 * M.Lf);
+    if (T < 410.0) return M.rhoH2Of * 4200.0;
+    return M.rhoH2Of * (-4.67e4 + 333.0 * T - 0.731 * (T * T) + 5.4e-4 * (T * T * T));
+}
+
+// compute_ksolidm (E:2311-2322)
+__device__ __forceinline__ double ksolidm(const ereb_materials& M, double T, int mode) {
+    return mode == 9 ? M.ksolidm[0] : 0.73 + 1293.0 / (T + 77.0);
+}
+
+// compute_kfluidm (E:2341-2358)
+__device__ __forceinline__ double kfluidm(const ereb_materials& M, double T, int mode) {
+    if (mode == 9) return M.kfluidm[0];
+    if (T < M.tmfluidphase) return 0.465 + 488.0 / T;
+    if (T < 410.0) return -0.581 + 6.34e-3 * T - 7.93e-6 * (T * T);
+    return -0.142 + 4.12e-3 * T - 5.01e-6 * (T * T);
+}
+
+// total (E:1116)
+__device__ __forceinline__ double total(double s, double f, double phi) {
+    return s * (1.0 - phi) + f * phi;
+}
+
+// ktotal (E:1226-1234)
+__device__ __forceinline__ double ktotal(double ks, double kf, double phi) {
+    double t = ks * (3.0 * phi - 2.0) + kf * (1.0 - 3.0 * phi);
+    return sqrt(ks * kf / 2.0 + (t * t) * (1.0 / 16.0)) - 0.25 * t;
+}
+
+// ηᶠcur_inv_kᵠ (E:1273-1275): ^3.0 and ^2.0 reduce to x*x*x and x*x in
+// Julia's pow_body for integral exponents
+__device__ __forceinline__ double etaf_inv_kphi(double phim0, double kphir, double phi,
+                                                double etaf) {
+    double q = phim0 * (1.0 / phi);
+    double r = (1.0 - phi) * (1.0 / (1.0 - phim0));
+    return etaf * (1.0 / kphir) * (q * q * q) * (r * r);
+}
+
+struct Props {
+    double rhototal, rhocptotal, etatotal, hrtotal, ktotal, tkrhocp, etafkphi;
+};
+
+// One marker of compute_marker_properties!.  For air (t == 3) the stored
+// rhototal, rhocptotal, etatotal, hrtotal, ktotal pass through (E:738-741).
+__device__ __forceinline__ Props marker_props(const ereb_materials& M, int t, double T,
+                                              double phi, double XW, const double* hrs,
+                                              const double* hrf, int mode, Props stored) {
+    Props p = stored;
+    double etafluidcur;
+    if (t < 3) {
+        double XD = 1.0 - XW;
+        double rhosolidm0 = (M.MD + M.MH2O * XW) / (M.VDs * XD + M.VWs * XW);
+        double rhofluidm0 = T > M.tmfluidphase ? M.rhoH2Of : M.rhoH2Ofi;
+        p.rhototal = total(rhosolidm0, rhofluidm0, phi);
+        p.rhocptotal = total(M.rhocpsolidm[t - 1], rhocpfluidm(M, T, mode), phi);
+        double es = T > M.tmsolidphase ? M.etasolidmm[t - 1] : M.etasolidm[t - 1];
+        etafluidcur = T > M.tmfluidphase ? M.etafluidmm[t - 1] : M.etafluidm[t - 1];
+        p.etatotal = jl_max(jl_max(M.etamin, es), etafluidcur);
+        p.hrtotal = total(hrs[t - 1], hrf[t - 1], phi);
+        p.ktotal = ktotal(ksolidm(M, T, mode), kfluidm(M, T, mode), phi);
+    } else {
+        etafluidcur = M.etafluidm[t - 1];
+    }
+    p.tkrhocp = T * p.rhocptotal;
+    p.etafkphi = etaf_inv_kphi(M.phim0, M.kphim0[t - 1], phi, etafluidcur);
+    return p;
+}
+
+// ---- wavefront helpers (64 lanes) ---------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    int l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+}  // namespace ereb

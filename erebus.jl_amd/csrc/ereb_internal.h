@@ -1,0 +1,135 @@
+// ereb_internal.h -- host-side context of libereb_hip.so and the launch
+// interface between the C-ABI layer and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ereb.h"
+#include "ereb_device.h"
+
+namespace ereb {
+
+// Scatter tiling: a tile is TILE x TILE basic cells; its node window is
+// (TILE+2) x (TILE+2) = 64 nodes per grid, one wavefront per grid.
+constexpr int TILE = 6;
+constexpr int WIN = TILE + 2;
+constexpr int NSUM = 30;  // SUM grids, order of reset_interpolated_properties!
+constexpr int FINE = 2 * TILE;  // fine cells per tile side
+
+struct Sys {
+    int64_t n = 0, cap = 0;
+    int32_t* colcnt = nullptr;  // n
+    int32_t* colptr = nullptr;  // n + 1, 0-based
+    int32_t* cursor = nullptr;  // n
+    int32_t* rowval = nullptr;  // cap, 0-based
+    double* nzval = nullptr;    // cap
+    double* rhs = nullptr;      // n
+    bool valid = false;
+};
+
+struct ProfPair {
+    int kclass;
+    hipEvent_t a, b;
+};
+
+}  // namespace ereb
+
+struct ereb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = EREB_FP64;
+    int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
+    double dx = 0, dy = 0;
+    double* d_axes = nullptr;  // x (Nx) | y (Ny) | xp (Nx1) | yp (Ny1)
+    ereb_materials mat{};
+
+    int64_t marknum = 0, cap = 0;
+    void* d_m[EREB_M_NFIELDS] = {};
+
+    double* d_grids = nullptr;
+    size_t goff[EREB_G_NFIELDS] = {};
+    int64_t gnumel[EREB_G_NFIELDS] = {};
+    uint8_t* d_yny = nullptr;
+
+    // tile sort
+    int ntx = 0, nty = 0, ntiles = 0;
+    uint32_t* d_key = nullptr;     // cap
+    uint32_t* d_perm = nullptr;    // cap
+    uint32_t* d_tmp = nullptr;     // pow2 scratch for oversize tiles
+    int64_t tmp_cap = 0;
+    int32_t* d_tcount = nullptr;   // ntiles
+    int32_t* d_tstart = nullptr;   // ntiles + 1
+    int32_t* d_tcursor = nullptr;  // ntiles
+    int32_t* d_big = nullptr;      // ntiles + 1: [0] = count, then tile ids
+    int32_t* d_flags = nullptr;    // [0] domain error
+    double* d_part = nullptr;      // ntiles * NSUM * 64
+    int32_t* d_scan = nullptr;     // scan block sums
+    int64_t scan_cap = 0;
+    bool have_sort = false;
+
+    ereb::Sys sys[2];
+
+    bool prof = false;
+    std::vector<ereb::ProfPair> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double prof_ms[EREB_K_NCLASSES] = {};
+    int64_t prof_n[EREB_K_NCLASSES] = {};
+
+    std::string err;
+};
+
+namespace ereb {
+
+// ---- helpers implemented in ereb_ctx.hip --------------------------------
+int fail(ereb_ctx* c, int code, const std::string& msg);
+int hipfail(ereb_ctx* c, hipError_t e, const char* what);
+#define EREB_HIP(c, call)                                      \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return ereb::hipfail(c, e_, #call); \
+    } while (0)
+
+GeomK geomk(const ereb_ctx* c);
+bool grid_is_basic(int field);
+double* gptr(ereb_ctx* c, int field);
+double* mptr(ereb_ctx* c, int field);
+int32_t* tm_ptr(ereb_ctx* c);
+int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap);
+int ensure_scan(ereb_ctx* c, int64_t n);
+int check_flags(ereb_ctx* c);
+void prof_begin(ereb_ctx* c, int kclass, hipEvent_t* a);
+void prof_end(ereb_ctx* c, int kclass, hipEvent_t a);
+int prof_collect(ereb_ctx* c);
+
+// ---- kernel launchers ---------------------------------------------------
+// ereb_scan.hip: exclusive scan of n int32 into out[0..n], out[n] = total
+int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n);
+// fill kernel (graph-safe replacement for hipMemsetAsync)
+int fill_i32(ereb_ctx* c, int32_t* p, int32_t v, int64_t n);
+int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);
+
+// ereb_sort.hip: stable counting sort of marker indices by tile key
+int launch_tile_sort(ereb_ctx* c);
+
+// ereb_scatter.hip
+int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
+                   bool write_props);
+int launch_combine(ereb_ctx* c);
+
+// ereb_nodes.hip
+int launch_node_properties(ereb_ctx* c, bool basic, bool vx, bool vy, bool p, bool bc);
+int launch_insulating_bc(ereb_ctx* c, double* t, int64_t Nyy, int64_t Nxx);
+
+// ereb_markers.hip
+int launch_gather_temperature(ereb_ctx* c, int64_t timestep);
+int launch_gather_stress(ereb_ctx* c);
+int launch_rk4(ereb_ctx* c, double dt, int mode);
+
+// ereb_assembly.hip
+int launch_assemble_stokes(ereb_ctx* c, double dt);
+int launch_assemble_thermal(ereb_ctx* c, double dt);
+
+}  // namespace ereb

@@ -1,0 +1,195 @@
+"""Drop-in mirror of the reference's transfer-path functions.
+
+Each function keeps the argument list of the Erebus.jl function it replaces
+(src/Erebus.jl), with a leading ``ctx`` (the GPU context) and a trailing
+underscore in place of Julia's ``!``.  Arrays are numpy arrays in Julia's
+layout: marker vectors 1-D, grids Fortran-ordered ``(rows, cols)`` so that
+``A[i-1, j-1]`` is Julia's ``A[i, j]``.  Outputs are written in place into
+the arrays passed, as the ``!`` functions do.  Each call is one C-ABI call
+into libereb_hip.so; all arithmetic runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .lib import OK, SYS_STOKES_DARCY, SYS_THERMAL, Context
+
+__all__ = [
+    "marker_to_nodes_", "compute_basic_node_properties_",
+    "compute_vx_node_properties_", "compute_vy_node_properties_",
+    "compute_p_node_properties_", "apply_insulating_boundary_conditions_",
+    "update_marker_temperature_", "update_marker_stress_", "move_markers_rk4_",
+    "assemble_hydromechanical_lse_", "assemble_thermal_lse_", "CSC",
+]
+
+
+def _io(a, dtype=np.float64, grid=False):
+    """In/out array: must already have the dtype and layout (no copies)."""
+    if not isinstance(a, np.ndarray) or a.dtype != dtype:
+        raise TypeError(f"expected a numpy {np.dtype(dtype).name} array")
+    ok = a.flags.f_contiguous if grid else a.flags.c_contiguous
+    if not ok:
+        raise TypeError("array must be contiguous in Julia (column-major) layout")
+    return a.ctypes.data
+
+
+def _in(a, dtype=np.float64, grid=False, keep=None):
+    a = np.asarray(a, dtype=dtype)
+    a = np.asfortranarray(a) if grid else np.ascontiguousarray(a)
+    if keep is not None:
+        keep.append(a)
+    return a.ctypes.data
+
+
+class CSC:
+    """SparseMatrixCSC{Float64,Int64}: 1-based colptr/rowval as Julia."""
+
+    def __init__(self, n, colptr, rowval, nzval):
+        self.m = self.n = n
+        self.colptr, self.rowval, self.nzval = colptr, rowval, nzval
+
+    @property
+    def nnz(self):
+        return int(self.nzval.size)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval, self.rowval - 1, self.colptr - 1),
+                             shape=(self.n, self.n))
+
+
+def marker_to_nodes_(ctx: Context, xm, ym, tm, tkm, phim, XWsolidm0, etavpm,
+                     inv_gggtotalm, fricttotalm, cohestotalm, tenstotalm,
+                     rhofluidcur, alphasolidcur, alphafluidcur, sxxm, sxym,
+                     rhototalm, rhocptotalm, etatotalm, hrtotalm, ktotalm,
+                     tkm_rhocptotalm, etafluidcur_inv_kphim, hrsolidm, hrfluidm,
+                     mode, sums, marknum):
+    """The marker loop E:6419-6517 (compute_marker_properties! E:700 and
+    marker_to_{basic,vx,vy,p}_nodes! E:1605-1863 for m = 1:marknum) after
+    reset_interpolated_properties! (E:968).  ``sums`` is the list of the 30
+    SUM grids in reset order; they and the 7 property vectors are written."""
+    keep = []
+    ins = [_in(a, np.int64 if k == 2 else np.float64, keep=keep) for k, a in enumerate(
+        [xm, ym, tm, tkm, phim, XWsolidm0, etavpm, inv_gggtotalm, fricttotalm,
+         cohestotalm, tenstotalm, rhofluidcur, alphasolidcur, alphafluidcur, sxxm, sxym])]
+    outs = [_io(a) for a in [rhototalm, rhocptotalm, etatotalm, hrtotalm, ktotalm,
+                             tkm_rhocptotalm, etafluidcur_inv_kphim]]
+    if len(sums) != 30:
+        raise ValueError("sums must hold the 30 SUM grids")
+    sp = (ctypes.c_void_p * 30)(*[_io(s, grid=True) for s in sums])
+    hs = _in(hrsolidm, keep=keep)
+    hf = _in(hrfluidm, keep=keep)
+    ctx.check(ctx.lib.ereb_marker_to_nodes(ctx.h, *ins, *outs, hs, hf, int(mode),
+                                           ctypes.addressof(sp), int(marknum)),
+              "marker_to_nodes!")
+
+
+def compute_basic_node_properties_(ctx: Context, ETA0SUM, ETASUM, GGGSUM, SXYSUM,
+                                   COHSUM, TENSUM, FRISUM, WTSUM, ETA0, ETA, GGG,
+                                   SXY0, COH, TEN, FRI, YNY):
+    """compute_basic_node_properties! (E:1970).  YNY is a uint8 (Bool) grid."""
+    keep = []
+    ins = [_in(a, grid=True, keep=keep) for a in
+           [ETA0SUM, ETASUM, GGGSUM, SXYSUM, COHSUM, TENSUM, FRISUM, WTSUM]]
+    outs = [_io(a, grid=True) for a in [ETA0, ETA, GGG, SXY0, COH, TEN, FRI]]
+    ctx.check(ctx.lib.ereb_compute_basic_node_properties(
+        ctx.h, *ins, *outs, _io(YNY, np.uint8, grid=True)), "compute_basic_node_properties!")
+
+
+def _v_node(fn, ctx, sums, outs, what):
+    keep = []
+    ins = [_in(a, grid=True, keep=keep) for a in sums]
+    o = [_io(a, grid=True) for a in outs]
+    ctx.check(fn(ctx.h, *ins, *o), what)
+
+
+def compute_vx_node_properties_(ctx: Context, RHOXSUM, RHOFXSUM, KXSUM, PHIXSUM, RXSUM,
+                                WTXSUM, RHOX, RHOFX, KX, PHIX, RX):
+    """compute_vx_node_properties! (E:2033)"""
+    _v_node(ctx.lib.ereb_compute_vx_node_properties, ctx,
+            [RHOXSUM, RHOFXSUM, KXSUM, PHIXSUM, RXSUM, WTXSUM], [RHOX, RHOFX, KX, PHIX, RX],
+            "compute_vx_node_properties!")
+
+
+def compute_vy_node_properties_(ctx: Context, RHOYSUM, RHOFYSUM, KYSUM, PHIYSUM, RYSUM,
+                                WTYSUM, RHOY, RHOFY, KY, PHIY, RY):
+    """compute_vy_node_properties! (E:2086)"""
+    _v_node(ctx.lib.ereb_compute_vy_node_properties, ctx,
+            [RHOYSUM, RHOFYSUM, KYSUM, PHIYSUM, RYSUM, WTYSUM], [RHOY, RHOFY, KY, PHIY, RY],
+            "compute_vy_node_properties!")
+
+
+def compute_p_node_properties_(ctx: Context, RHOSUM, RHOCPSUM, ALPHASUM, ALPHAFSUM, HRSUM,
+                               GGGPSUM, SXXSUM, TKSUM, PHISUM, WTPSUM, RHO, RHOCP, ALPHA,
+                               ALPHAF, HR, GGGP, SXX0, tk1, PHI, BETAPHI):
+    """compute_p_node_properties! (E:2148)"""
+    _v_node(ctx.lib.ereb_compute_p_node_properties, ctx,
+            [RHOSUM, RHOCPSUM, ALPHASUM, ALPHAFSUM, HRSUM, GGGPSUM, SXXSUM, TKSUM, PHISUM,
+             WTPSUM],
+            [RHO, RHOCP, ALPHA, ALPHAF, HR, GGGP, SXX0, tk1, PHI, BETAPHI],
+            "compute_p_node_properties!")
+
+
+def apply_insulating_boundary_conditions_(ctx: Context, t):
+    """apply_insulating_boundary_conditions! (E:2432)"""
+    Nyy, Nxx = t.shape
+    ctx.check(ctx.lib.ereb_apply_insulating_boundary_conditions(
+        ctx.h, _io(t, grid=True), Nyy, Nxx), "apply_insulating_boundary_conditions!")
+
+
+def update_marker_temperature_(ctx: Context, xm, ym, tkm, DT, tk2, timestep, marknum):
+    """update_marker_temperature! (E:4782)"""
+    keep = []
+    ctx.check(ctx.lib.ereb_update_marker_temperature(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _io(tkm),
+        _in(DT, grid=True, keep=keep), _in(tk2, grid=True, keep=keep),
+        int(timestep), int(marknum)), "update_marker_temperature!")
+
+
+def update_marker_stress_(ctx: Context, xm, ym, sxxm, sxym, DSXX, DSXY, marknum):
+    """update_marker_stress! (E:4354)"""
+    keep = []
+    ctx.check(ctx.lib.ereb_update_marker_stress(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _io(sxxm), _io(sxym),
+        _in(DSXX, grid=True, keep=keep), _in(DSXY, grid=True, keep=keep), int(marknum)),
+        "update_marker_stress!")
+
+
+def move_markers_rk4_(ctx: Context, xm, ym, tm, tkm, phim, sxym, sxxm, vx, vy, vxf, vyf,
+                      wyx, tk2, marknum, dt, mode):
+    """move_markers_rk4! (E:4990), argument order as the reference"""
+    keep = []
+    ctx.check(ctx.lib.ereb_move_markers_rk4(
+        ctx.h, _io(xm), _io(ym), _in(tm, np.int64, keep=keep), _io(tkm),
+        _in(phim, keep=keep), _io(sxym), _io(sxxm),
+        *[_in(a, grid=True, keep=keep) for a in (vx, vy, vxf, vyf, wyx, tk2)],
+        int(marknum), float(dt), int(mode)), "move_markers_rk4!")
+
+
+def assemble_hydromechanical_lse_(ctx: Context, ETA, ETAP, GGG, GGGP, SXY0, SXX0, RHOX,
+                                  RHOY, RHOFX, RHOFY, RX, RY, ETAPHI, BETAPHI, PHI, gx, gy,
+                                  pr0, pf0, DMP, dt, R) -> CSC:
+    """assemble_hydromechanical_lse! (E:3089): fills R, returns L (CSC)."""
+    keep = []
+    grids = [_in(a, grid=True, keep=keep) for a in
+             [ETA, ETAP, GGG, GGGP, SXY0, SXX0, RHOX, RHOY, RHOFX, RHOFY, RX, RY, ETAPHI,
+              BETAPHI, PHI, gx, gy, pr0, pf0, DMP]]
+    nnz = ctypes.c_int64()
+    ctx.check(ctx.lib.ereb_assemble_hydromechanical_lse(
+        ctx.h, *grids, float(dt), _io(R), ctypes.addressof(nnz)),
+        "assemble_hydromechanical_lse!")
+    cp, rv, nz, _ = ctx.csc(SYS_STOKES_DARCY)
+    return CSC(R.size, cp, rv, nz)
+
+
+def assemble_thermal_lse_(ctx: Context, tk1, RHOCP, KX, KY, HR, HA, HS, DHP, RT, dt) -> CSC:
+    """assemble_thermal_lse! (E:4526): fills RT, returns LT (CSC)."""
+    keep = []
+    grids = [_in(a, grid=True, keep=keep) for a in [tk1, RHOCP, KX, KY, HR, HA, HS, DHP]]
+    nnz = ctypes.c_int64()
+    ctx.check(ctx.lib.ereb_assemble_thermal_lse(
+        ctx.h, *grids, _io(RT), float(dt), ctypes.addressof(nnz)), "assemble_thermal_lse!")
+    cp, rv, nz, _ = ctx.csc(SYS_THERMAL)
+    return CSC(RT.size, cp, rv, nz)

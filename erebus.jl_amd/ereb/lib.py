@@ -1,0 +1,329 @@
+"""ctypes binding of libereb_hip.so (the C ABI in include/ereb.h).
+
+The library is built in-tree by erebus.jl_amd/build.py.  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .constants import CGeometry, CMaterials, Geometry, Materials
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "lib", "libereb_hip.so"))
+
+OK, ERR_ARG, ERR_MODE, ERR_HIP, ERR_DOMAIN, ERR_NOMEM, ERR_STATE = range(7)
+FP64, MIXED = 0, 1
+SYS_STOKES_DARCY, SYS_THERMAL = 0, 1
+
+MARKER_FIELDS = [
+    "xm", "ym", "tm", "tkm", "phim", "sxxm", "sxym", "XWsolidm0", "etavpm",
+    "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
+    "alphasolidcur", "alphafluidcur", "rhototalm", "rhocptotalm", "etatotalm",
+    "hrtotalm", "ktotalm", "tkm_rhocptotalm", "etafluidcur_inv_kphim",
+]
+MARKER_ID = {n: k for k, n in enumerate(MARKER_FIELDS)}
+
+GRID_FIELDS = [
+    # basic (Ny, Nx)
+    "ETA0SUM", "ETASUM", "GGGSUM", "SXYSUM", "COHSUM", "TENSUM", "FRISUM", "WTSUM",
+    "ETA0", "ETA", "GGG", "SXY0", "COH", "TEN", "FRI", "wyx", "DSXY",
+    # Vx (Ny1, Nx1)
+    "RHOXSUM", "RHOFXSUM", "KXSUM", "PHIXSUM", "RXSUM", "WTXSUM",
+    "RHOX", "RHOFX", "KX", "PHIX", "RX", "vx", "vxf", "gx",
+    # Vy
+    "RHOYSUM", "RHOFYSUM", "KYSUM", "PHIYSUM", "RYSUM", "WTYSUM",
+    "RHOY", "RHOFY", "KY", "PHIY", "RY", "vy", "vyf", "gy",
+    # P
+    "RHOSUM", "RHOCPSUM", "ALPHASUM", "ALPHAFSUM", "HRSUM", "GGGPSUM", "SXXSUM",
+    "TKSUM", "PHISUM", "WTPSUM",
+    "RHO", "RHOCP", "ALPHA", "ALPHAF", "HR", "GGGP", "SXX0", "tk1", "PHI", "BETAPHI",
+    "ETAP", "ETAPHI", "pr0", "pf0", "DMP", "DT", "tk2", "DSXX", "HA", "HS", "DHP",
+]
+GRID_ID = {n: k for k, n in enumerate(GRID_FIELDS)}
+BASIC_GRIDS = set(GRID_FIELDS[:GRID_FIELDS.index("DSXY") + 1])
+
+# SUM grids in the order of reset_interpolated_properties! (E:968-999)
+SUM_FIELDS = [
+    "ETA0SUM", "ETASUM", "GGGSUM", "SXYSUM", "COHSUM", "TENSUM", "FRISUM", "WTSUM",
+    "RHOXSUM", "RHOFXSUM", "KXSUM", "PHIXSUM", "RXSUM", "WTXSUM",
+    "RHOYSUM", "RHOFYSUM", "KYSUM", "PHIYSUM", "RYSUM", "WTYSUM",
+    "RHOSUM", "RHOCPSUM", "ALPHASUM", "ALPHAFSUM", "HRSUM", "GGGPSUM", "SXXSUM",
+    "TKSUM", "PHISUM", "WTPSUM",
+]
+
+KERNEL_CLASSES = ["sort", "scatter", "combine", "nodes", "gather_T", "gather_S",
+                  "rk4", "asm_sd", "asm_t"]
+
+_D = ctypes.c_double
+_I64 = ctypes.c_int64
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int
+
+# exported C entry points: name -> argtypes (restype int unless noted)
+PROTOTYPES = {
+    "ereb_version": [],
+    "ereb_last_error": [_P],
+    "ereb_ctx_create": [ctypes.POINTER(CGeometry), ctypes.POINTER(CMaterials), _I, _I,
+                        ctypes.POINTER(_P)],
+    "ereb_ctx_destroy": [_P],
+    "ereb_ctx_set_precision": [_P, _I],
+    "ereb_ctx_reserve": [_P, _I64],
+    "ereb_sync": [_P],
+    "ereb_marknum": [_P],
+    "ereb_markers_set": [_P, _I, _P, _I64],
+    "ereb_markers_get": [_P, _I, _P, _I64],
+    "ereb_grid_numel": [_P, _I],
+    "ereb_grid_set": [_P, _I, _P],
+    "ereb_grid_get": [_P, _I, _P],
+    "ereb_yny_set": [_P, _P],
+    "ereb_yny_get": [_P, _P],
+    "ereb_step_scatter": [_P, _P, _P, _I64],
+    "ereb_step_node_properties": [_P],
+    "ereb_step_gather_temperature": [_P, _I64],
+    "ereb_step_gather_stress": [_P],
+    "ereb_step_advect_rk4": [_P, _D, _I64],
+    "ereb_step_assemble_stokes_darcy": [_P, _D, _P],
+    "ereb_step_assemble_thermal": [_P, _D, _P],
+    "ereb_csc_get": [_P, _I, _P, _P, _P, _P],
+    "ereb_csc_size": [_P, _I, _P, _P],
+    "ereb_sort_get": [_P, _P, _P, _P],
+    "ereb_marker_to_nodes": [_P] + [_P] * 23 + [_P, _P, _I64, _P, _I64],
+    "ereb_compute_basic_node_properties": [_P] + [_P] * 16,
+    "ereb_compute_vx_node_properties": [_P] + [_P] * 11,
+    "ereb_compute_vy_node_properties": [_P] + [_P] * 11,
+    "ereb_compute_p_node_properties": [_P] + [_P] * 20,
+    "ereb_apply_insulating_boundary_conditions": [_P, _P, _I64, _I64],
+    "ereb_update_marker_temperature": [_P, _P, _P, _P, _P, _P, _I64, _I64],
+    "ereb_update_marker_stress": [_P, _P, _P, _P, _P, _P, _P, _I64],
+    "ereb_move_markers_rk4": [_P] + [_P] * 13 + [_I64, _D, _I64],
+    "ereb_assemble_hydromechanical_lse": [_P] + [_P] * 20 + [_D, _P, _P],
+    "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
+    "ereb_profile_enable": [_P, _I],
+    "ereb_profile_get": [_P, _I, _P, _P],
+    "ereb_profile_reset": [_P],
+}
+RESTYPES = {"ereb_last_error": ctypes.c_char_p, "ereb_marknum": _I64,
+            "ereb_grid_numel": _I64}
+
+_lib = None
+
+
+class EREBError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[ereb {code}] {msg}")
+        self.code = code
+
+
+class EREBModeError(EREBError, ValueError):
+    """Unknown property mode; the reference throws "unknown mode" (E:2289)."""
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Load libereb_hip.so; raises when the HIP library is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"libereb_hip.so not found at {p}: build it with "
+            "`python erebus.jl_amd/build.py` (there is no CPU fallback)")
+    lib = ctypes.CDLL(p)
+    for name, args in PROTOTYPES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = RESTYPES.get(name, _I)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a, n=None) -> np.ndarray:
+    a = np.asarray(a, dtype=np.float64)
+    a = np.require(a, requirements=["F_CONTIGUOUS", "ALIGNED"]) if a.ndim > 1 else \
+        np.ascontiguousarray(a)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} elements, got {a.size}")
+    return a
+
+
+class Context:
+    """Owns one ereb_ctx: geometry, materials and the HBM-resident state."""
+
+    def __init__(self, geom: Geometry, mat: Materials, precision: int = FP64,
+                 device: int = 0):
+        self.lib = load()
+        self.geom = geom
+        self.mat = mat
+        self._cg = geom.to_c()
+        self._cm = mat.to_c(geom.xsize, geom.ysize)
+        h = _P()
+        rc = self.lib.ereb_ctx_create(ctypes.byref(self._cg), ctypes.byref(self._cm),
+                                      precision, device, ctypes.byref(h))
+        self.h = h.value
+        if rc != OK:
+            msg = self.lib.ereb_last_error(self.h).decode() if self.h else "invalid geometry"
+            if self.h:
+                self.lib.ereb_ctx_destroy(self.h)
+            self.h = None
+            raise EREBError(rc, f"ereb_ctx_create: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.ereb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- status -----------------------------------------------------------
+    def check(self, rc: int, what: str = ""):
+        if rc == OK:
+            return
+        msg = self.lib.ereb_last_error(self.h).decode()
+        cls = EREBModeError if rc == ERR_MODE else EREBError
+        raise cls(rc, f"{what}: {msg}" if what else msg)
+
+    def sync(self):
+        self.check(self.lib.ereb_sync(self.h), "sync")
+
+    def set_precision(self, p: int):
+        self.check(self.lib.ereb_ctx_set_precision(self.h, p))
+
+    @property
+    def marknum(self) -> int:
+        return int(self.lib.ereb_marknum(self.h))
+
+    # ---- markers ----------------------------------------------------------
+    def set_marker(self, name: str, a: np.ndarray):
+        fid = MARKER_ID[name]
+        if name == "tm":
+            a = np.ascontiguousarray(a, dtype=np.int64)
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+        self.check(self.lib.ereb_markers_set(self.h, fid, _ptr(a), a.size), f"set {name}")
+
+    def get_marker(self, name: str, n: int | None = None) -> np.ndarray:
+        n = self.marknum if n is None else n
+        out = np.empty(n, dtype=np.int64 if name == "tm" else np.float64)
+        self.check(self.lib.ereb_markers_get(self.h, MARKER_ID[name], _ptr(out), n),
+                   f"get {name}")
+        return out
+
+    def set_markers(self, d: dict):
+        for k, v in d.items():
+            if k in MARKER_ID:
+                self.set_marker(k, v)
+
+    # ---- grids ------------------------------------------------------------
+    def grid_shape(self, name: str):
+        g = self.geom
+        return (g.Ny, g.Nx) if name in BASIC_GRIDS else (g.Ny1, g.Nx1)
+
+    def set_grid(self, name: str, a: np.ndarray):
+        shp = self.grid_shape(name)
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        if a.shape != shp:
+            raise ValueError(f"{name}: shape {a.shape} != {shp}")
+        self.check(self.lib.ereb_grid_set(self.h, GRID_ID[name], _ptr(a)), f"set {name}")
+
+    def get_grid(self, name: str) -> np.ndarray:
+        out = np.empty(self.grid_shape(name), dtype=np.float64, order="F")
+        self.check(self.lib.ereb_grid_get(self.h, GRID_ID[name], _ptr(out)), f"get {name}")
+        return out
+
+    def set_yny(self, a: np.ndarray):
+        a = np.asfortranarray(np.asarray(a, dtype=np.uint8))
+        self.check(self.lib.ereb_yny_set(self.h, _ptr(a)))
+
+    def get_yny(self) -> np.ndarray:
+        out = np.empty((self.geom.Ny, self.geom.Nx), dtype=np.uint8, order="F")
+        self.check(self.lib.ereb_yny_get(self.h, _ptr(out)))
+        return out
+
+    # ---- resident steps ---------------------------------------------------
+    def step_scatter(self, hrsolidm, hrfluidm, mode: int):
+        hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
+        hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
+        self.check(self.lib.ereb_step_scatter(self.h, _ptr(hs), _ptr(hf), int(mode)),
+                   "step_scatter")
+
+    def step_node_properties(self):
+        self.check(self.lib.ereb_step_node_properties(self.h), "step_node_properties")
+
+    def step_gather_temperature(self, timestep: int):
+        self.check(self.lib.ereb_step_gather_temperature(self.h, int(timestep)))
+
+    def step_gather_stress(self):
+        self.check(self.lib.ereb_step_gather_stress(self.h))
+
+    def step_advect_rk4(self, dt: float, mode: int):
+        self.check(self.lib.ereb_step_advect_rk4(self.h, float(dt), int(mode)), "rk4")
+
+    def step_assemble_stokes_darcy(self, dt: float, want_nnz: bool = False):
+        n = ctypes.c_int64(-1)
+        self.check(self.lib.ereb_step_assemble_stokes_darcy(
+            self.h, float(dt), ctypes.addressof(n) if want_nnz else None))
+        return n.value if want_nnz else None
+
+    def step_assemble_thermal(self, dt: float, want_nnz: bool = False):
+        n = ctypes.c_int64(-1)
+        self.check(self.lib.ereb_step_assemble_thermal(
+            self.h, float(dt), ctypes.addressof(n) if want_nnz else None))
+        return n.value if want_nnz else None
+
+    def csc(self, which: int):
+        """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64
+        indices as in SparseMatrixCSC{Float64,Int64}."""
+        n = ctypes.c_int64()
+        nnz = ctypes.c_int64()
+        self.check(self.lib.ereb_csc_size(self.h, which, ctypes.addressof(n),
+                                          ctypes.addressof(nnz)), "csc_size")
+        cp = np.empty(n.value + 1, dtype=np.int64)
+        rv = np.empty(nnz.value, dtype=np.int64)
+        nz = np.empty(nnz.value, dtype=np.float64)
+        rhs = np.empty(n.value, dtype=np.float64)
+        self.check(self.lib.ereb_csc_get(self.h, which, _ptr(cp), _ptr(rv), _ptr(nz),
+                                         _ptr(rhs)), "csc_get")
+        return cp, rv, nz, rhs
+
+    def sort_order(self):
+        n = self.marknum
+        perm = np.empty(n, dtype=np.int32)
+        key = np.empty(n, dtype=np.int32)
+        nt = ctypes.c_int64()
+        self.check(self.lib.ereb_sort_get(self.h, _ptr(perm), _ptr(key),
+                                          ctypes.addressof(nt)), "sort_get")
+        return perm, key, nt.value
+
+    # ---- profiling --------------------------------------------------------
+    def profile(self, on: bool = True):
+        self.check(self.lib.ereb_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        self.check(self.lib.ereb_profile_reset(self.h))
+
+    def profile_get(self, kclass: str):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        self.check(self.lib.ereb_profile_get(self.h, KERNEL_CLASSES.index(kclass),
+                                             ctypes.addressof(ms), ctypes.addressof(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,298 @@
+/*
+ * ereb.h -- C ABI of libereb_hip.so, the MI355X (gfx950) marker-in-cell
+ * transfer library for Erebus.jl.
+ *
+ * Citations are file:line into the Erebus.jl sources
+ * (E: = src/Erebus.jl, C: = src/constants.jl).
+ *
+ * Conventions (all entry points):
+ *   - every function returns an int status, 0 = EREB_OK; on error
+ *     ereb_last_error(ctx) returns a message.  No C++ exception crosses the ABI.
+ *   - grids are column-major Float64 arrays exactly as Julia stores them:
+ *     basic grids are (Ny, Nx), Vx/Vy/P grids are (Ny1, Nx1), element
+ *     (i, j) (1-based) at offset (j-1)*rows + (i-1).
+ *   - marker arrays are Julia Vector{Float64}; the marker type `tm` is
+ *     Vector{Int64} (values 1 mantle, 2 crust, 3 sticky air).
+ *   - host pointers are borrowed for the duration of one call only; the
+ *     library owns every device allocation.
+ *   - calls on one context are serialised by the caller (one HIP stream per
+ *     context); different contexts are independent.
+ *
+ * Two call forms are offered for every step:
+ *   1. drop-in entry points with the reference function's argument list
+ *      (host arrays in, host arrays out), e.g. ereb_update_marker_temperature;
+ *   2. resident entry points (ereb_step_*) that operate on the marker and
+ *      grid state kept in HBM by the context, used by a GPU-resident time
+ *      loop and by the benchmark.  Host <-> device traffic happens only in
+ *      ereb_markers_set/get and ereb_grid_set/get.
+ */
+#ifndef EREB_H
+#define EREB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define EREB_OK              0
+#define EREB_ERR_ARG         1  /* bad argument (null pointer, size mismatch) */
+#define EREB_ERR_MODE        2  /* unknown property mode: the reference throws
+                                   "unknown mode" (E:2289, E:2318, E:2354) */
+#define EREB_ERR_HIP         3  /* HIP runtime error */
+#define EREB_ERR_DOMAIN      4  /* marker coordinate outside the range the
+                                   tiled scatter supports (|x|,|y| > 1e15 m
+                                   but finite) */
+#define EREB_ERR_NOMEM       5
+#define EREB_ERR_STATE       6  /* resident step called before its inputs */
+
+/* ---- precision forms --------------------------------------------------- */
+#define EREB_FP64   0  /* every operation in Float64, as the reference */
+#define EREB_MIXED  1  /* properties, weights and interpolation in FP32 with
+                          compensated accumulation; coordinates, RK4 updates,
+                          node normalisation and matrix coefficients FP64 */
+
+/* ---- geometry: the compile-time constants of C:14-101 ------------------- */
+/* Axes are passed explicitly so that node coordinates equal the reference's
+ * SVector constants (C:40-64) bit for bit.  Lengths: x Nx, y Ny, xvx Nx1,
+ * yvx Ny1, xvy Nx1, yvy Ny1, xp Nx1, yp Ny1 (Nx1 = Nx+1, Ny1 = Ny+1).
+ * The clamp ranges (C:66-101) are fixed functions of Nx, Ny:
+ *   basic j 1..Nx-1, i 1..Ny-1;  Vx j 1..Nx-1, i 1..Ny;
+ *   Vy j 1..Nx, i 1..Ny-1;       P  j 1..Nx,   i 1..Ny. */
+typedef struct {
+    int64_t Nx, Ny;
+    double dx, dy;
+    const double *x, *y;
+    const double *xvx, *yvx;
+    const double *xvy, *yvy;
+    const double *xp, *yp;
+} ereb_geometry;
+
+/* ---- material tables and scalars: C:143-186, C:218-258, C:283-314 ------- */
+typedef struct {
+    /* per marker type 1..3 (index 0..2), C:146-184 */
+    double rhosolidm[3], rhofluidm[3], etasolidm[3], etasolidmm[3];
+    double etafluidm[3], etafluidmm[3], rhocpsolidm[3], rhocpfluidm[3];
+    double alphasolidm[3], alphafluidm[3], ksolidm[3], kfluidm[3];
+    double gggsolidm[3], frictsolidm[3], cohessolidm[3], tenssolidm[3];
+    double kphim0[3], tkm0[3];
+    /* scalars */
+    double phim0;          /* C:225 */
+    double tmsolidphase;   /* C:218 */
+    double tmfluidphase;   /* C:220 */
+    double Lf;             /* C:222 heat of fusion of ice */
+    double MH2O, MD;       /* C:234, C:236 */
+    double rhoH2Of, rhoH2Ofi; /* C:242, C:244 */
+    double VDs, VWs;       /* C:246, C:248 (computed there as MD/rhoDs etc.) */
+    double etamin;         /* C:343 */
+    double Kcont;          /* C:143 */
+    double psurface;       /* C:12 */
+    /* boundary conditions, C:283-309 (free slip -1, no slip 1) */
+    double bcleft, bcright, bctop, bcbottom;
+    double bcfleft, bcfright, bcftop, bcfbottom;
+    double vxleft, vxright, vytop, vybottom;
+    /* RK4 Butcher coefficients C:312-314 (brk4 as Float64 of 1//6, 1//3) */
+    double brk4[4], crk4[3];
+} ereb_materials;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct ereb_ctx ereb_ctx;
+
+int ereb_ctx_create(const ereb_geometry* geom, const ereb_materials* mat,
+                    int precision, int device, ereb_ctx** out);
+int ereb_ctx_destroy(ereb_ctx* ctx);
+const char* ereb_last_error(const ereb_ctx* ctx);
+int ereb_ctx_set_precision(ereb_ctx* ctx, int precision);
+/* make room for `marknum` resident markers (grows, never shrinks) */
+int ereb_ctx_reserve(ereb_ctx* ctx, int64_t marknum);
+int ereb_sync(ereb_ctx* ctx);
+int ereb_version(void);
+
+/* ---- resident marker state --------------------------------------------- */
+/* Names follow the reference's marker vectors (E:55-71, E:593-668). */
+enum ereb_marker_field {
+    EREB_M_XM = 0, EREB_M_YM, EREB_M_TM /* int64 on the host */, EREB_M_TKM,
+    EREB_M_PHIM, EREB_M_SXXM, EREB_M_SXYM, EREB_M_XWSOLIDM0, EREB_M_ETAVPM,
+    EREB_M_INV_GGGTOTALM, EREB_M_FRICTTOTALM, EREB_M_COHESTOTALM,
+    EREB_M_TENSTOTALM, EREB_M_RHOFLUIDCUR, EREB_M_ALPHASOLIDCUR,
+    EREB_M_ALPHAFLUIDCUR, EREB_M_RHOTOTALM, EREB_M_RHOCPTOTALM,
+    EREB_M_ETATOTALM, EREB_M_HRTOTALM, EREB_M_KTOTALM,
+    EREB_M_TKM_RHOCPTOTALM, EREB_M_ETAFLUIDCUR_INV_KPHIM,
+    EREB_M_NFIELDS
+};
+/* marknum becomes the context's active marker count on set */
+int ereb_markers_set(ereb_ctx* ctx, int field, const void* host, int64_t marknum);
+int ereb_markers_get(ereb_ctx* ctx, int field, void* host, int64_t marknum);
+int64_t ereb_marknum(const ereb_ctx* ctx);
+
+/* ---- resident grid state ------------------------------------------------ */
+/* Names follow the reference (E:851-923 SUM grids, E:147-343 node grids). */
+enum ereb_grid_field {
+    /* basic nodes (Ny, Nx) */
+    EREB_G_ETA0SUM = 0, EREB_G_ETASUM, EREB_G_GGGSUM, EREB_G_SXYSUM,
+    EREB_G_COHSUM, EREB_G_TENSUM, EREB_G_FRISUM, EREB_G_WTSUM,
+    EREB_G_ETA0, EREB_G_ETA, EREB_G_GGG, EREB_G_SXY0, EREB_G_COH, EREB_G_TEN,
+    EREB_G_FRI, EREB_G_WYX, EREB_G_DSXY,
+    /* Vx nodes (Ny1, Nx1) */
+    EREB_G_RHOXSUM, EREB_G_RHOFXSUM, EREB_G_KXSUM, EREB_G_PHIXSUM,
+    EREB_G_RXSUM, EREB_G_WTXSUM,
+    EREB_G_RHOX, EREB_G_RHOFX, EREB_G_KX, EREB_G_PHIX, EREB_G_RX,
+    EREB_G_VX, EREB_G_VXF, EREB_G_GX,
+    /* Vy nodes (Ny1, Nx1) */
+    EREB_G_RHOYSUM, EREB_G_RHOFYSUM, EREB_G_KYSUM, EREB_G_PHIYSUM,
+    EREB_G_RYSUM, EREB_G_WTYSUM,
+    EREB_G_RHOY, EREB_G_RHOFY, EREB_G_KY, EREB_G_PHIY, EREB_G_RY,
+    EREB_G_VY, EREB_G_VYF, EREB_G_GY,
+    /* P nodes (Ny1, Nx1) */
+    EREB_G_RHOSUM, EREB_G_RHOCPSUM, EREB_G_ALPHASUM, EREB_G_ALPHAFSUM,
+    EREB_G_HRSUM, EREB_G_GGGPSUM, EREB_G_SXXSUM, EREB_G_TKSUM,
+    EREB_G_PHISUM, EREB_G_WTPSUM,
+    EREB_G_RHO, EREB_G_RHOCP, EREB_G_ALPHA, EREB_G_ALPHAF, EREB_G_HR,
+    EREB_G_GGGP, EREB_G_SXX0, EREB_G_TK1, EREB_G_PHI, EREB_G_BETAPHI,
+    EREB_G_ETAP, EREB_G_ETAPHI, EREB_G_PR0, EREB_G_PF0, EREB_G_DMP,
+    EREB_G_DT, EREB_G_TK2, EREB_G_DSXX, EREB_G_HA, EREB_G_HS, EREB_G_DHP,
+    EREB_G_NFIELDS
+};
+int ereb_grid_set(ereb_ctx* ctx, int field, const double* host);
+int ereb_grid_get(ereb_ctx* ctx, int field, double* host);
+/* YNY (Matrix{Bool}, (Ny, Nx)), one byte per node, 0/1 */
+int ereb_yny_set(ereb_ctx* ctx, const uint8_t* host);
+int ereb_yny_get(ereb_ctx* ctx, uint8_t* host);
+/* number of elements of a grid field (Ny*Nx or Ny1*Nx1) */
+int64_t ereb_grid_numel(const ereb_ctx* ctx, int field);
+
+/* ---- resident steps ----------------------------------------------------- */
+/* Step 1 + 2: the marker loop E:6419-6517 (compute_marker_properties! E:700
+ * then marker_to_{basic,vx,vy,p}_nodes! E:1605-1863) after
+ * reset_interpolated_properties! (E:968): writes the 7 marker property
+ * vectors and the 30 *SUM grids.  hrsolidm/hrfluidm are the per-type
+ * radiogenic heat productions from calculate_radioactive_heating (E:1336). */
+int ereb_step_scatter(ereb_ctx* ctx, const double hrsolidm[3],
+                      const double hrfluidm[3], int64_t mode);
+/* compute_{basic,vx,vy,p}_node_properties! (E:1970, E:2033, E:2086, E:2148)
+ * then apply_insulating_boundary_conditions!(tk1) (E:2432, call E:6604) */
+int ereb_step_node_properties(ereb_ctx* ctx);
+/* update_marker_temperature! (E:4782): tk2 grid at timestep 1, DT otherwise */
+int ereb_step_gather_temperature(ereb_ctx* ctx, int64_t timestep);
+/* update_marker_stress! (E:4354) with DSXX (P) and DSXY (basic) grids */
+int ereb_step_gather_stress(ereb_ctx* ctx);
+/* move_markers_rk4! (E:4990) with vx, vy, vxf, vyf, wyx, tk2 grids */
+int ereb_step_advect_rk4(ereb_ctx* ctx, double dt, int64_t mode);
+/* assemble_hydromechanical_lse! (E:3089); the CSC matrix and R stay on the
+ * device until ereb_csc_get; *nnz receives the number of stored entries */
+int ereb_step_assemble_stokes_darcy(ereb_ctx* ctx, double dt, int64_t* nnz);
+/* assemble_thermal_lse! (E:4526) */
+int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
+
+#define EREB_SYS_STOKES_DARCY 0
+#define EREB_SYS_THERMAL      1
+/* Copy the last assembled system to the host as a 1-based
+ * SparseMatrixCSC{Float64,Int64}: colptr (n+1), rowval (nnz), nzval (nnz),
+ * rhs (n).  Any pointer may be NULL to skip that part. */
+int ereb_csc_get(ereb_ctx* ctx, int which, int64_t* colptr, int64_t* rowval,
+                 double* nzval, double* rhs);
+int ereb_csc_size(const ereb_ctx* ctx, int which, int64_t* n, int64_t* nnz);
+
+/* Sort order of the last ereb_step_scatter: perm[k] (0-based marker index)
+ * of the k-th marker in tile order, and the tile of every marker.  The sort
+ * is a stable counting sort of marker indices by tile key. */
+int ereb_sort_get(ereb_ctx* ctx, int32_t* perm, int32_t* tile_key,
+                  int64_t* ntiles);
+
+/* ---- drop-in entry points: the reference's argument lists -------------- */
+/* Batched form of the loop E:6419-6517 (compute_marker_properties! +
+ * marker_to_*_nodes! for m = 1..marknum).  Writes the property vectors
+ * (rhototalm, rhocptotalm, etatotalm, hrtotalm, ktotalm, tkm_rhocptotalm,
+ * etafluidcur_inv_kphim) and the 30 SUM grids in the order of
+ * reset_interpolated_properties! (E:968-999). */
+int ereb_marker_to_nodes(ereb_ctx* ctx,
+    const double* xm, const double* ym, const int64_t* tm, const double* tkm,
+    const double* phim, const double* XWsolidm0, const double* etavpm,
+    const double* inv_gggtotalm, const double* fricttotalm,
+    const double* cohestotalm, const double* tenstotalm,
+    const double* rhofluidcur, const double* alphasolidcur,
+    const double* alphafluidcur, const double* sxxm, const double* sxym,
+    double* rhototalm, double* rhocptotalm, double* etatotalm,
+    double* hrtotalm, double* ktotalm, double* tkm_rhocptotalm,
+    double* etafluidcur_inv_kphim,
+    const double hrsolidm[3], const double hrfluidm[3], int64_t mode,
+    double* const sums[30], int64_t marknum);
+
+/* compute_basic_node_properties! E:1970 (in: 8 SUM; in/out: 7 grids + YNY) */
+int ereb_compute_basic_node_properties(ereb_ctx* ctx,
+    const double* ETA0SUM, const double* ETASUM, const double* GGGSUM,
+    const double* SXYSUM, const double* COHSUM, const double* TENSUM,
+    const double* FRISUM, const double* WTSUM,
+    double* ETA0, double* ETA, double* GGG, double* SXY0, double* COH,
+    double* TEN, double* FRI, uint8_t* YNY);
+/* compute_vx_node_properties! E:2033 */
+int ereb_compute_vx_node_properties(ereb_ctx* ctx,
+    const double* RHOXSUM, const double* RHOFXSUM, const double* KXSUM,
+    const double* PHIXSUM, const double* RXSUM, const double* WTXSUM,
+    double* RHOX, double* RHOFX, double* KX, double* PHIX, double* RX);
+/* compute_vy_node_properties! E:2086 */
+int ereb_compute_vy_node_properties(ereb_ctx* ctx,
+    const double* RHOYSUM, const double* RHOFYSUM, const double* KYSUM,
+    const double* PHIYSUM, const double* RYSUM, const double* WTYSUM,
+    double* RHOY, double* RHOFY, double* KY, double* PHIY, double* RY);
+/* compute_p_node_properties! E:2148 */
+int ereb_compute_p_node_properties(ereb_ctx* ctx,
+    const double* RHOSUM, const double* RHOCPSUM, const double* ALPHASUM,
+    const double* ALPHAFSUM, const double* HRSUM, const double* GGGPSUM,
+    const double* SXXSUM, const double* TKSUM, const double* PHISUM,
+    const double* WTPSUM,
+    double* RHO, double* RHOCP, double* ALPHA, double* ALPHAF, double* HR,
+    double* GGGP, double* SXX0, double* tk1, double* PHI, double* BETAPHI);
+/* apply_insulating_boundary_conditions! E:2432 on a (Nyy, Nxx) array */
+int ereb_apply_insulating_boundary_conditions(ereb_ctx* ctx, double* t,
+    int64_t Nyy, int64_t Nxx);
+
+/* update_marker_temperature! E:4782 */
+int ereb_update_marker_temperature(ereb_ctx* ctx, const double* xm,
+    const double* ym, double* tkm, const double* DT, const double* tk2,
+    int64_t timestep, int64_t marknum);
+/* update_marker_stress! E:4354 */
+int ereb_update_marker_stress(ereb_ctx* ctx, const double* xm,
+    const double* ym, double* sxxm, double* sxym, const double* DSXX,
+    const double* DSXY, int64_t marknum);
+/* move_markers_rk4! E:4990 (argument order as the reference: sxym before
+ * sxxm) */
+int ereb_move_markers_rk4(ereb_ctx* ctx, double* xm, double* ym,
+    const int64_t* tm, double* tkm, const double* phim, double* sxym,
+    double* sxxm, const double* vx, const double* vy, const double* vxf,
+    const double* vyf, const double* wyx, const double* tk2,
+    int64_t marknum, double dt, int64_t mode);
+/* assemble_hydromechanical_lse! E:3089: writes R (6*Ny1*Nx1) and keeps the
+ * CSC matrix L on the device; *nnz receives its size, fetch it with
+ * ereb_csc_get(ctx, EREB_SYS_STOKES_DARCY, ...). */
+int ereb_assemble_hydromechanical_lse(ereb_ctx* ctx,
+    const double* ETA, const double* ETAP, const double* GGG,
+    const double* GGGP, const double* SXY0, const double* SXX0,
+    const double* RHOX, const double* RHOY, const double* RHOFX,
+    const double* RHOFY, const double* RX, const double* RY,
+    const double* ETAPHI, const double* BETAPHI, const double* PHI,
+    const double* gx, const double* gy, const double* pr0, const double* pf0,
+    const double* DMP, double dt, double* R, int64_t* nnz);
+/* assemble_thermal_lse! E:4526: writes RT (Ny1*Nx1), keeps LT on device */
+int ereb_assemble_thermal_lse(ereb_ctx* ctx, const double* tk1,
+    const double* RHOCP, const double* KX, const double* KY, const double* HR,
+    const double* HA, const double* HS, const double* DHP, double* RT,
+    double dt, int64_t* nnz);
+
+/* ---- profiling: HIP events around the launches of one kernel class ------ */
+enum ereb_kernel_class {
+    EREB_K_SORT = 0, EREB_K_SCATTER, EREB_K_COMBINE, EREB_K_NODES,
+    EREB_K_GATHER_T, EREB_K_GATHER_S, EREB_K_RK4, EREB_K_ASM_SD,
+    EREB_K_ASM_T, EREB_K_NCLASSES
+};
+int ereb_profile_enable(ereb_ctx* ctx, int on);
+/* total milliseconds and launch count accumulated since the last reset */
+int ereb_profile_get(ereb_ctx* ctx, int kclass, double* total_ms,
+                     int64_t* count);
+int ereb_profile_reset(ereb_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EREB_H */

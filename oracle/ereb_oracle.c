@@ -1,0 +1,883 @@
+/*
+ * ereb_oracle.c -- CPU Float64 restatement of the Erebus.jl marker-in-cell
+ * transfer path.  TEST INFRASTRUCTURE ONLY: it is the checker that the HIP
+ * library is compared against, and the CPU baseline of the benchmark.  No
+ * product code path links, loads or calls it.
+ *
+ * Every function follows the reference source operator by operator
+ * (E: = src/Erebus.jl, C: = src/constants.jl, T: = test/runtests.jl):
+ *   - Julia evaluates a*b*c left to right and never contracts a*b+c into an
+ *     fma; this file is compiled with -ffp-contract=off and SSE2 doubles.
+ *   - `x^2` and `x^3` (literal powers) are x*x and x*x*x; for Float64
+ *     exponents Julia 1.10's ^(Float64,Float64) dispatches integral
+ *     exponents to pow_body, which returns x*x*x for 3 and the rounded
+ *     square for 2, so `^3.0` and `^2.0` are restated the same way.
+ *   - inv(d) is 1.0/d; the reference multiplies by inv(d) in fix (E:1457)
+ *     but divides in fix_weights (E:1392) and the RK4 stencils (E:5070).
+ *   - unsafe_trunc(Int, t) follows x86-64 cvttsd2si: NaN and |t| >= 2^63
+ *     give typemin(Int64).
+ *
+ * Parity pins for each function are listed in the header of
+ * tests/test_oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/ereb.h"
+
+#define G(a, i, j, rows) (a)[((int64_t)(j) - 1) * (rows) + ((int64_t)(i) - 1)]
+
+/* ---------------------------------------------------------------- scalars */
+
+static inline int64_t jl_unsafe_trunc(double t) {
+    if (!(t >= -9223372036854775808.0 && t < 9223372036854775808.0))
+        return INT64_MIN;
+    return (int64_t)t;
+}
+
+/* Julia max(a, b) for Float64: NaN propagates */
+static inline double jl_max(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a > b ? a : (b > a ? b : (signbit(a) ? b : a));
+}
+
+/* total E:1116-1118 */
+double orc_total(double solid, double fluid, double phi) {
+    return solid * (1.0 - phi) + fluid * phi;
+}
+
+/* ktotal E:1226-1234 */
+double orc_ktotal(double ks, double kf, double phi) {
+    double t = ks * (3.0 * phi - 2.0) + kf * (1.0 - 3.0 * phi);
+    return sqrt(ks * kf / 2.0 + (t * t) * (1.0 / 16.0)) - 0.25 * t;
+}
+
+/* kphi E:1251-1254 (not on the path; restated for the T:991 pin) */
+double orc_kphi(double phim0, double kphim0m, double phimm) {
+    double q = phimm * (1.0 / phim0);
+    double r = (1.0 - phimm) * (1.0 / (1.0 - phim0));
+    double rinv = 1.0 / r;  /* ^-2.0 -> pow_body n=-2: inv(x)*inv(x) */
+    return kphim0m * (q * q * q) * (rinv * rinv);
+}
+
+/* ηᶠcur_inv_kᵠ E:1273-1275 */
+double orc_etaf_inv_kphi(double phim0, double kphir, double phi, double etaf) {
+    double q = phim0 * (1.0 / phi);
+    double r = (1.0 - phi) * (1.0 / (1.0 - phim0));
+    return etaf * (1.0 / kphir) * (q * q * q) * (r * r);
+}
+
+/* compute_rhocpfluidm E:2272-2293; *err = EREB_ERR_MODE on unknown mode */
+double orc_compute_rhocpfluidm(const ereb_materials* M, double T, int64_t mode,
+                               int* err) {
+    if (mode == 1) {
+        double tf = M->tmfluidphase;
+        if (T < tf - 5.0)
+            return M->rhoH2Ofi * (7.67 * T);
+        else if (T < tf)
+            return M->rhoH2Ofi * (7.67 * T + 0.1 * M->Lf);
+        else if (T < tf + 5.0)
+            return M->rhoH2Of * (4200.0 + 0.1 * M->Lf);
+        else if (T < 410.0)
+            return M->rhoH2Of * 4200.0;
+        else
+            return M->rhoH2Of *
+                   (-4.67e4 + 333.0 * T - 0.731 * (T * T) + 5.4e-4 * (T * T * T));
+    } else if (mode == 9) {
+        return M->rhocpfluidm[0];
+    }
+    if (err) *err = EREB_ERR_MODE;
+    return NAN;
+}
+
+/* compute_ksolidm E:2311-2322 */
+double orc_compute_ksolidm(const ereb_materials* M, double T, int64_t mode,
+                           int* err) {
+    if (mode == 1) return 0.73 + 1293.0 / (T + 77.0);
+    if (mode == 9) return M->ksolidm[0];
+    if (err) *err = EREB_ERR_MODE;
+    return NAN;
+}
+
+/* compute_kfluidm E:2341-2358 */
+double orc_compute_kfluidm(const ereb_materials* M, double T, int64_t mode,
+                           int* err) {
+    if (mode == 1) {
+        if (T < M->tmfluidphase)
+            return 0.465 + 488.0 / T;
+        else if (T < 410.0)
+            return -0.581 + 6.34e-3 * T - 7.93e-6 * (T * T);
+        else
+            return -0.142 + 4.12e-3 * T - 5.01e-6 * (T * T);
+    }
+    if (mode == 9) return M->kfluidm[0];
+    if (err) *err = EREB_ERR_MODE;
+    return NAN;
+}
+
+/* etatotal_rocks E:1312-1318 */
+double orc_etatotal_rocks(const ereb_materials* M, double T, int64_t tm) {
+    double es = T > M->tmsolidphase ? M->etasolidmm[tm - 1] : M->etasolidm[tm - 1];
+    double ef = T > M->tmfluidphase ? M->etafluidmm[tm - 1] : M->etafluidm[tm - 1];
+    return jl_max(jl_max(M->etamin, es), ef);
+}
+
+/* Q_radiogenic E:1294-1296 */
+double orc_q_radiogenic(double f, double ratio, double E, double tau,
+                        double time) {
+    return f * ratio * E * exp(-time * (1.0 / tau)) * (1.0 / tau);
+}
+
+/* ------------------------------------------------------- interpolation */
+
+/* fix E:1456-1470 */
+void orc_fix(double x, double y, const double* xa, const double* ya, double dx,
+             double dy, int64_t jmin, int64_t jmax, int64_t imin, int64_t imax,
+             int64_t* ri, int64_t* rj) {
+    int64_t j = jl_unsafe_trunc((x - xa[0]) * (1.0 / dx)) + 1;
+    int64_t i = jl_unsafe_trunc((y - ya[0]) * (1.0 / dy)) + 1;
+    if (j < jmin) j = jmin; else if (j > jmax) j = jmax;
+    if (i < imin) i = imin; else if (i > imax) i = imax;
+    *ri = i;
+    *rj = j;
+}
+
+/* fix_distances E:1424-1431 */
+void orc_fix_distances(double x, double y, const double* xa, const double* ya,
+                       double dx, double dy, int64_t jmin, int64_t jmax,
+                       int64_t imin, int64_t imax, int64_t* ri, int64_t* rj,
+                       double* dxmj, double* dymi) {
+    orc_fix(x, y, xa, ya, dx, dy, jmin, jmax, imin, imax, ri, rj);
+    *dxmj = x - xa[*rj - 1];
+    *dymi = y - ya[*ri - 1];
+}
+
+/* fix_weights E:1388-1397 */
+void orc_fix_weights(double x, double y, const double* xa, const double* ya,
+                     double dx, double dy, int64_t jmin, int64_t jmax,
+                     int64_t imin, int64_t imax, int64_t* ri, int64_t* rj,
+                     double* w) {
+    double dxmj, dymi;
+    orc_fix_distances(x, y, xa, ya, dx, dy, jmin, jmax, imin, imax, ri, rj,
+                      &dxmj, &dymi);
+    w[0] = (1.0 - dymi / dy) * (1.0 - dxmj / dx);
+    w[1] = (dymi / dy) * (1.0 - dxmj / dx);
+    w[2] = (1.0 - dymi / dy) * (dxmj / dx);
+    w[3] = (dymi / dy) * (dxmj / dx);
+}
+
+/* dot4(grid_vector(i, j, grid), weights) E:1137-1159 */
+static inline double gdot4(const double* g, int64_t rows, int64_t i, int64_t j,
+                           const double* w) {
+    return G(g, i, j, rows) * w[0] + G(g, i + 1, j, rows) * w[1] +
+           G(g, i, j + 1, rows) * w[2] + G(g, i + 1, j + 1, rows) * w[3];
+}
+
+/* interpolate_add_to_grid! E:1517-1525 */
+static inline void add4(double* g, int64_t rows, int64_t i, int64_t j,
+                        const double* w, double p) {
+    G(g, i, j, rows) += p * w[0];
+    G(g, i + 1, j, rows) += p * w[1];
+    G(g, i, j + 1, rows) += p * w[2];
+    G(g, i + 1, j + 1, rows) += p * w[3];
+}
+
+/* ------------------------------------------------------- step 1 + 2 */
+
+/* compute_marker_properties! E:700-750 for one marker (0-based m) */
+static int marker_properties_one(const ereb_materials* M, int64_t m,
+    const int64_t* tm, const double* tkm, double* rhototalm,
+    double* rhocptotalm, double* etatotalm, double* hrtotalm, double* ktotalm,
+    double* tkm_rhocptotalm, double* etafluidcur_inv_kphim,
+    const double* hrsolidm, const double* hrfluidm, const double* phim,
+    const double* XW, int64_t mode) {
+    int err = 0;
+    int64_t t = tm[m];
+    double T = tkm[m];
+    double etafluidcur;
+    if (t < 3) {
+        double XD = 1.0 - XW[m];
+        double rhosolidm0 = (M->MD + M->MH2O * XW[m]) / (M->VDs * XD + M->VWs * XW[m]);
+        double rhofluidm0 = T > M->tmfluidphase ? M->rhoH2Of : M->rhoH2Ofi;
+        rhototalm[m] = orc_total(rhosolidm0, rhofluidm0, phim[m]);
+        rhocptotalm[m] = orc_total(M->rhocpsolidm[t - 1],
+                                   orc_compute_rhocpfluidm(M, T, mode, &err), phim[m]);
+        double etasolidcur = T > M->tmsolidphase ? M->etasolidmm[t - 1] : M->etasolidm[t - 1];
+        etafluidcur = T > M->tmfluidphase ? M->etafluidmm[t - 1] : M->etafluidm[t - 1];
+        etatotalm[m] = jl_max(jl_max(M->etamin, etasolidcur), etafluidcur);
+        hrtotalm[m] = orc_total(hrsolidm[t - 1], hrfluidm[t - 1], phim[m]);
+        ktotalm[m] = orc_ktotal(orc_compute_ksolidm(M, T, mode, &err),
+                                orc_compute_kfluidm(M, T, mode, &err), phim[m]);
+    } else {
+        etafluidcur = M->etafluidm[t - 1];
+    }
+    tkm_rhocptotalm[m] = T * rhocptotalm[m];
+    etafluidcur_inv_kphim[m] =
+        orc_etaf_inv_kphi(M->phim0, M->kphim0[t - 1], phim[m], etafluidcur);
+    return err;
+}
+
+int orc_compute_marker_properties(const ereb_materials* M, int64_t marknum,
+    const int64_t* tm, const double* tkm, double* rhototalm,
+    double* rhocptotalm, double* etatotalm, double* hrtotalm, double* ktotalm,
+    double* tkm_rhocptotalm, double* etafluidcur_inv_kphim,
+    const double* hrsolidm, const double* hrfluidm, const double* phim,
+    const double* XW, int64_t mode) {
+    if (mode != 1 && mode != 9) return EREB_ERR_MODE;
+    for (int64_t m = 0; m < marknum; ++m)
+        marker_properties_one(M, m, tm, tkm, rhototalm, rhocptotalm, etatotalm,
+                              hrtotalm, ktotalm, tkm_rhocptotalm,
+                              etafluidcur_inv_kphim, hrsolidm, hrfluidm, phim,
+                              XW, mode);
+    return EREB_OK;
+}
+
+/* The marker loop E:6376-6517: reset_interpolated_properties! (E:968) then
+ * for m = 1:marknum compute_marker_properties! and
+ * marker_to_{basic,vx,vy,p}_nodes! (E:1605-1863).  sums[] in the order of
+ * reset_interpolated_properties!: ETA0SUM ETASUM GGGSUM SXYSUM COHSUM
+ * TENSUM FRISUM WTSUM | RHOXSUM RHOFXSUM KXSUM PHIXSUM RXSUM WTXSUM |
+ * RHOYSUM RHOFYSUM KYSUM PHIYSUM RYSUM WTYSUM | RHOSUM RHOCPSUM ALPHASUM
+ * ALPHAFSUM HRSUM GGGPSUM SXXSUM TKSUM PHISUM WTPSUM */
+int orc_marker_to_nodes(const ereb_geometry* g, const ereb_materials* M,
+    const double* xm, const double* ym, const int64_t* tm, const double* tkm,
+    const double* phim, const double* XW, const double* etavpm,
+    const double* inv_gggtotalm, const double* fricttotalm,
+    const double* cohestotalm, const double* tenstotalm,
+    const double* rhofluidcur, const double* alphasolidcur,
+    const double* alphafluidcur, const double* sxxm, const double* sxym,
+    double* rhototalm, double* rhocptotalm, double* etatotalm,
+    double* hrtotalm, double* ktotalm, double* tkm_rhocptotalm,
+    double* etafluidcur_inv_kphim, const double* hrsolidm,
+    const double* hrfluidm, int64_t mode, double* const* sums,
+    int64_t marknum) {
+    if (mode != 1 && mode != 9) return EREB_ERR_MODE;
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    for (int k = 0; k < 8; ++k) memset(sums[k], 0, sizeof(double) * Nx * Ny);
+    for (int k = 8; k < 30; ++k) memset(sums[k], 0, sizeof(double) * Nx1 * Ny1);
+    for (int64_t m = 0; m < marknum; ++m) {
+        marker_properties_one(M, m, tm, tkm, rhototalm, rhocptotalm, etatotalm,
+                              hrtotalm, ktotalm, tkm_rhocptotalm,
+                              etafluidcur_inv_kphim, hrsolidm, hrfluidm, phim,
+                              XW, mode);
+        int64_t i, j;
+        double w[4];
+        /* marker_to_basic_nodes! E:1605-1648 */
+        orc_fix_weights(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1,
+                        Ny - 1, &i, &j, w);
+        add4(sums[0], Ny, i, j, w, etatotalm[m]);
+        add4(sums[1], Ny, i, j, w, etavpm[m]);
+        add4(sums[2], Ny, i, j, w, inv_gggtotalm[m]);
+        add4(sums[3], Ny, i, j, w, sxym[m]);
+        add4(sums[4], Ny, i, j, w, cohestotalm[m]);
+        add4(sums[5], Ny, i, j, w, tenstotalm[m]);
+        add4(sums[6], Ny, i, j, w, fricttotalm[m]);
+        add4(sums[7], Ny, i, j, w, 1.0);
+        /* marker_to_vx_nodes! E:1676-1713 */
+        orc_fix_weights(xm[m], ym[m], g->xvx, g->yvx, g->dx, g->dy, 1, Nx - 1, 1,
+                        Ny, &i, &j, w);
+        add4(sums[8], Ny1, i, j, w, rhototalm[m]);
+        add4(sums[9], Ny1, i, j, w, rhofluidcur[m]);
+        add4(sums[10], Ny1, i, j, w, ktotalm[m]);
+        add4(sums[11], Ny1, i, j, w, phim[m]);
+        add4(sums[12], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
+        add4(sums[13], Ny1, i, j, w, 1.0);
+        /* marker_to_vy_nodes! E:1741-1778 */
+        orc_fix_weights(xm[m], ym[m], g->xvy, g->yvy, g->dx, g->dy, 1, Nx, 1,
+                        Ny - 1, &i, &j, w);
+        add4(sums[14], Ny1, i, j, w, rhototalm[m]);
+        add4(sums[15], Ny1, i, j, w, rhofluidcur[m]);
+        add4(sums[16], Ny1, i, j, w, ktotalm[m]);
+        add4(sums[17], Ny1, i, j, w, phim[m]);
+        add4(sums[18], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
+        add4(sums[19], Ny1, i, j, w, 1.0);
+        /* marker_to_p_nodes! E:1814-1863 */
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
+                        &i, &j, w);
+        add4(sums[25], Ny1, i, j, w, inv_gggtotalm[m]); /* GGGPSUM */
+        add4(sums[26], Ny1, i, j, w, sxxm[m]);          /* SXXSUM */
+        add4(sums[20], Ny1, i, j, w, rhototalm[m]);     /* RHOSUM */
+        add4(sums[21], Ny1, i, j, w, rhocptotalm[m]);   /* RHOCPSUM */
+        add4(sums[22], Ny1, i, j, w, alphasolidcur[m]); /* ALPHASUM */
+        add4(sums[23], Ny1, i, j, w, alphafluidcur[m]); /* ALPHAFSUM */
+        add4(sums[24], Ny1, i, j, w, hrtotalm[m]);      /* HRSUM */
+        add4(sums[28], Ny1, i, j, w, phim[m]);          /* PHISUM */
+        add4(sums[27], Ny1, i, j, w, tkm_rhocptotalm[m]); /* TKSUM */
+        add4(sums[29], Ny1, i, j, w, 1.0);              /* WTPSUM */
+    }
+    return EREB_OK;
+}
+
+/* --------------------------------------------------- node properties */
+
+/* compute_basic_node_properties! E:1970-2007 */
+void orc_compute_basic_node_properties(const ereb_geometry* g,
+    const double* ETA0SUM, const double* ETASUM, const double* GGGSUM,
+    const double* SXYSUM, const double* COHSUM, const double* TENSUM,
+    const double* FRISUM, const double* WTSUM, double* ETA0, double* ETA,
+    double* GGG, double* SXY0, double* COH, double* TEN, double* FRI,
+    uint8_t* YNY) {
+    int64_t n = g->Nx * g->Ny;
+    for (int64_t k = 0; k < n; ++k) {
+        double wt = WTSUM[k];
+        if (wt > 0.0) {
+            double iw = 1.0 / wt;
+            ETA0[k] = ETA0SUM[k] * iw;
+            ETA[k] = ETASUM[k] * iw;
+            if (ETA[k] < ETA0[k]) YNY[k] = 1;
+            GGG[k] = (1.0 / GGGSUM[k]) * wt;
+            SXY0[k] = SXYSUM[k] * iw;
+            COH[k] = COHSUM[k] * iw;
+            TEN[k] = TENSUM[k] * iw;
+            FRI[k] = FRISUM[k] * iw;
+        }
+    }
+}
+
+/* compute_vx_node_properties! E:2033-2060 (also Vy, E:2086-2113) */
+void orc_compute_v_node_properties(const ereb_geometry* g, const double* RHOSUM,
+    const double* RHOFSUM, const double* KSUM, const double* PHISUM,
+    const double* RSUM, const double* WTSUM, double* RHO, double* RHOF,
+    double* K, double* PHI, double* R) {
+    int64_t n = (g->Nx + 1) * (g->Ny + 1);
+    for (int64_t k = 0; k < n; ++k) {
+        double wt = WTSUM[k];
+        if (wt > 0.0) {
+            double iw = 1.0 / wt;
+            RHO[k] = RHOSUM[k] * iw;
+            RHOF[k] = RHOFSUM[k] * iw;
+            K[k] = KSUM[k] * iw;
+            PHI[k] = PHISUM[k] * iw;
+            R[k] = RSUM[k] * iw;
+        }
+    }
+}
+
+/* compute_p_node_properties! E:2148-2189 */
+void orc_compute_p_node_properties(const ereb_geometry* g, const double* RHOSUM,
+    const double* RHOCPSUM, const double* ALPHASUM, const double* ALPHAFSUM,
+    const double* HRSUM, const double* GGGPSUM, const double* SXXSUM,
+    const double* TKSUM, const double* PHISUM, const double* WTPSUM,
+    double* RHO, double* RHOCP, double* ALPHA, double* ALPHAF, double* HR,
+    double* GGGP, double* SXX0, double* tk1, double* PHI, double* BETAPHI) {
+    int64_t n = (g->Nx + 1) * (g->Ny + 1);
+    for (int64_t k = 0; k < n; ++k) {
+        double wt = WTPSUM[k];
+        if (wt > 0.0) {
+            double iw = 1.0 / wt;
+            RHO[k] = RHOSUM[k] * iw;
+            RHOCP[k] = RHOCPSUM[k] * iw;
+            ALPHA[k] = ALPHASUM[k] * iw;
+            ALPHAF[k] = ALPHAFSUM[k] * iw;
+            HR[k] = HRSUM[k] * iw;
+            GGGP[k] = (1.0 / GGGPSUM[k]) * wt;
+            SXX0[k] = SXXSUM[k] * iw;
+            tk1[k] = TKSUM[k] * (1.0 / RHOCPSUM[k]);
+            PHI[k] = PHISUM[k] * iw;
+            BETAPHI[k] = (1.0 / GGGP[k]) * PHI[k];
+        }
+    }
+}
+
+/* apply_insulating_boundary_conditions! E:2432-2449 */
+void orc_apply_insulating_bc(double* t, int64_t Nyy, int64_t Nxx) {
+    if (!(Nyy > 2 && Nxx > 2)) return;
+    for (int64_t j = 2; j <= Nxx - 1; ++j) G(t, 1, j, Nyy) = G(t, 2, j, Nyy);
+    for (int64_t j = 2; j <= Nxx - 1; ++j) G(t, Nyy, j, Nyy) = G(t, Nyy - 1, j, Nyy);
+    for (int64_t i = 1; i <= Nyy; ++i) G(t, i, 1, Nyy) = G(t, i, 2, Nyy);
+    for (int64_t i = 1; i <= Nyy; ++i) G(t, i, Nxx, Nyy) = G(t, i, Nxx - 1, Nyy);
+}
+
+/* ------------------------------------------------------------ step 3 */
+
+/* update_marker_temperature! E:4782-4801 */
+void orc_update_marker_temperature(const ereb_geometry* g, const double* xm,
+    const double* ym, double* tkm, const double* DT, const double* tk2,
+    int64_t timestep, int64_t marknum) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
+                        &i, &j, w);
+        if (timestep == 1)
+            tkm[m] = gdot4(tk2, Ny1, i, j, w);
+        else
+            tkm[m] += gdot4(DT, Ny1, i, j, w);
+    }
+}
+
+/* update_marker_stress! E:4354-4388 (P clamps 2..Nx-1, 2..Ny-1) */
+void orc_update_marker_stress(const ereb_geometry* g, const double* xm,
+    const double* ym, double* sxxm, double* sxym, const double* DSXX,
+    const double* DSXY, int64_t marknum) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t ip, jp, ib, jb;
+        double wp[4], wb[4];
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 2, Nx - 1, 2,
+                        Ny - 1, &ip, &jp, wp);
+        orc_fix_weights(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1,
+                        Ny - 1, &ib, &jb, wb);
+        sxxm[m] += gdot4(DSXX, Ny1, ip, jp, wp);
+        sxym[m] += gdot4(DSXY, Ny, ib, jb, wb);
+    }
+}
+
+/* ------------------------------------------------------------ step 4 */
+
+/* one RK4 stage velocity on the Vx grid (E:5057-5091); `vc` is the grid
+ * whose values feed the second-order correction at the (i, j) centre of the
+ * left-half branch, which the reference takes from vx also in the fluid
+ * loop (E:5173) */
+static double rk_vx(const ereb_geometry* g, const double* v, const double* vc,
+                    double x, double y) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, R = Ny + 1;
+    int64_t i, j;
+    double dxmj, dymi;
+    orc_fix_distances(x, y, g->xvx, g->yvx, g->dx, g->dy, 1, Nx - 1, 1, Ny, &i,
+                      &j, &dxmj, &dymi);
+    double dx = g->dx, dy = g->dy;
+    double v13 = G(v, i, j, R) * (1.0 - dxmj / dx) + G(v, i, j + 1, R) * dxmj / dx;
+    double v24 = G(v, i + 1, j, R) * (1.0 - dxmj / dx) + G(v, i + 1, j + 1, R) * dxmj / dx;
+    if (dxmj / dx >= 0.5) {
+        if (j < Nx - 1) {
+            double t = dxmj / dx - 0.5;
+            v13 += 0.5 * (t * t) * (G(v, i, j, R) - 2.0 * G(v, i, j + 1, R) + G(v, i, j + 2, R));
+            v24 += 0.5 * (t * t) * (G(v, i + 1, j, R) - 2.0 * G(v, i + 1, j + 1, R) + G(v, i + 1, j + 2, R));
+        }
+    } else {
+        if (j > 1) {
+            double t = dxmj / dx - 0.5;
+            v13 += 0.5 * (t * t) * (G(v, i, j - 1, R) - 2.0 * G(vc, i, j, R) + G(v, i, j + 1, R));
+            v24 += 0.5 * (t * t) * (G(v, i + 1, j - 1, R) - 2.0 * G(v, i + 1, j, R) + G(v, i + 1, j + 1, R));
+        }
+    }
+    return v13 * (1.0 - dymi / dy) + v24 * dymi / dy;
+}
+
+/* one RK4 stage velocity on the Vy grid (E:5093-5127) */
+static double rk_vy(const ereb_geometry* g, const double* v, double x,
+                    double y) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, R = Ny + 1;
+    int64_t i, j;
+    double dxmj, dymi;
+    orc_fix_distances(x, y, g->xvy, g->yvy, g->dx, g->dy, 1, Nx, 1, Ny - 1, &i,
+                      &j, &dxmj, &dymi);
+    double dx = g->dx, dy = g->dy;
+    double v12 = G(v, i, j, R) * (1.0 - dymi / dy) + G(v, i + 1, j, R) * dymi / dy;
+    double v34 = G(v, i, j + 1, R) * (1.0 - dymi / dy) + G(v, i + 1, j + 1, R) * dymi / dy;
+    if (dymi / dy >= 0.5) {
+        if (i < Ny - 1) {
+            double t = dymi / dy - 0.5;
+            v12 += 0.5 * (t * t) * (G(v, i, j, R) - 2.0 * G(v, i + 1, j, R) + G(v, i + 2, j, R));
+            v34 += 0.5 * (t * t) * (G(v, i, j + 1, R) - 2.0 * G(v, i + 1, j + 1, R) + G(v, i + 2, j + 1, R));
+        }
+    } else {
+        if (i > 1) {
+            double t = dymi / dy - 0.5;
+            v12 += 0.5 * (t * t) * (G(v, i - 1, j, R) - 2.0 * G(v, i, j, R) + G(v, i + 1, j, R));
+            v34 += 0.5 * (t * t) * (G(v, i - 1, j + 1, R) - 2.0 * G(v, i, j + 1, R) + G(v, i + 1, j + 1, R));
+        }
+    }
+    return v12 * (1.0 - dxmj / dx) + v34 * dxmj / dx;
+}
+
+/* move_markers_rk4! E:4990-5257 */
+int orc_move_markers_rk4(const ereb_geometry* g, const ereb_materials* M,
+    double* xm, double* ym, const int64_t* tm, double* tkm, const double* phim,
+    double* sxym, double* sxxm, const double* vx, const double* vy,
+    const double* vxf, const double* vyf, const double* wyx, const double* tk2,
+    int64_t marknum, double dt, int64_t mode) {
+    if (mode != 1 && mode != 9) return EREB_ERR_MODE;
+    const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    const double* b = M->brk4;
+    const double* c = M->crk4;
+    for (int64_t m = 0; m < marknum; ++m) {
+        double xmm = xm[m], ymm = ym[m];
+        double xr = xmm, yr = ymm;
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xr, yr, g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny, &i, &j, w);
+        double tks0 = gdot4(tk2, Ny1, i, j, w);
+        orc_fix_weights(xr, yr, g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1, Ny - 1, &i, &j, w);
+        double om = gdot4(wyx, Ny, i, j, w);
+        double th = dt * om;
+        double s = sxxm[m], t = sxym[m];
+        double ct = cos(th), st = sin(th), s2 = sin(2.0 * th), c2 = cos(2.0 * th);
+        double sxx1 = s * (ct * ct) - s * (st * st) - t * s2;
+        double sxy1 = s * s2 + t * c2;
+        sxxm[m] = sxx1;
+        sxym[m] = sxy1;
+        double vxr[4], vyr[4];
+        for (int rk = 0; rk < 4; ++rk) {
+            vxr[rk] = 0.0 + rk_vx(g, vx, vx, xr, yr);
+            vyr[rk] = 0.0 + rk_vy(g, vy, xr, yr);
+            if (rk < 3) {
+                xr = xmm + dt * c[rk] * vxr[rk];
+                yr = ymm + dt * c[rk] * vyr[rk];
+            }
+        }
+        xm[m] += dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
+        ym[m] += dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
+        xmm = xr = xm[m];
+        ymm = yr = ym[m];
+        for (int rk = 0; rk < 4; ++rk) {
+            vxr[rk] = 0.0 + rk_vx(g, vxf, vx, xr, yr);
+            vyr[rk] = 0.0 + rk_vy(g, vyf, xr, yr);
+            if (rk < 3) {
+                xr = xmm - dt * c[rk] * vxr[rk];
+                yr = ymm - dt * c[rk] * vyr[rk];
+            }
+        }
+        xr = xmm - dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
+        yr = ymm - dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
+        orc_fix_weights(xr, yr, g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny, &i, &j, w);
+        double tkf0 = gdot4(tk2, Ny1, i, j, w);
+        double d = tkf0 - tks0;
+        int64_t tt = tm[m];
+        double T = tkm[m], ph = phim[m];
+        double rcs = M->rhocpsolidm[tt - 1];
+        double rcf = orc_compute_rhocpfluidm(M, T, mode, NULL);
+        tkm[m] = ((1.0 - ph) * T * rcs + ph * (T + d) * rcf) /
+                 ((1.0 - ph) * rcs + ph * rcf);
+    }
+    return EREB_OK;
+}
+
+/* ------------------------------------------------------------ step 5 */
+
+/* Row-wise builder mirroring ExtendableSparse: updateindex!(L, +, v, r, c)
+ * creates no entry when v == 0 (its documented zero skip), entries are
+ * unique per (row, col) in these assemblies, flush! yields CSC with rows
+ * ascending in each column. */
+typedef struct {
+    int64_t n, cnt, cap;
+    int64_t* row;
+    int64_t* col;
+    double* val;
+    int overflow;
+} coo_t;
+
+static void coo_add(coo_t* L, double v, int64_t r, int64_t c) {
+    if (v == 0.0) return; /* -0.0 == 0.0 as in iszero */
+    if (L->cnt >= L->cap) { L->overflow = 1; return; }
+    L->row[L->cnt] = r;
+    L->col[L->cnt] = c;
+    L->val[L->cnt] = v;
+    L->cnt++;
+}
+
+/* COO (row-major generation order) -> 1-based CSC, rows ascending per col */
+static int64_t coo_to_csc(const coo_t* L, int64_t* colptr, int64_t* rowval,
+                          double* nzval) {
+    int64_t n = L->n;
+    int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    for (int64_t k = 0; k < L->cnt; ++k) cnt[L->col[k]]++;  /* cols 1-based */
+    colptr[0] = 1;
+    for (int64_t c = 1; c <= n; ++c) colptr[c] = colptr[c - 1] + cnt[c];
+    int64_t* fill = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    /* generation order is by row ascending except within a row; collect then
+       sort each column by row (columns hold at most a few dozen entries) */
+    for (int64_t k = 0; k < L->cnt; ++k) {
+        int64_t c = L->col[k];
+        int64_t p = colptr[c - 1] - 1 + fill[c]++;
+        rowval[p] = L->row[k];
+        nzval[p] = L->val[k];
+    }
+    for (int64_t c = 1; c <= n; ++c) {
+        int64_t a = colptr[c - 1] - 1, e = colptr[c] - 1;
+        for (int64_t p = a + 1; p < e; ++p) {
+            int64_t r = rowval[p];
+            double v = nzval[p];
+            int64_t q = p - 1;
+            while (q >= a && rowval[q] > r) {
+                rowval[q + 1] = rowval[q];
+                nzval[q + 1] = nzval[q];
+                --q;
+            }
+            rowval[q + 1] = r;
+            nzval[q + 1] = v;
+        }
+    }
+    free(cnt);
+    free(fill);
+    return L->cnt;
+}
+
+/* assemble_hydromechanical_lse! E:3089-3572.  Output CSC arrays need room
+ * for `cap` entries; returns nnz, or -1 when cap is too small. */
+int64_t orc_assemble_hydromechanical(const ereb_geometry* g,
+    const ereb_materials* M, const double* ETA, const double* ETAP,
+    const double* GGG, const double* GGGP, const double* SXY0,
+    const double* SXX0, const double* RHOX, const double* RHOY,
+    const double* RHOFX, const double* RHOFY, const double* RX,
+    const double* RY, const double* ETAPHI, const double* BETAPHI,
+    const double* PHI, const double* gx, const double* gy, const double* pr0,
+    const double* pf0, const double* DMP, double dt, double* R,
+    int64_t* colptr, int64_t* rowval, double* nzval, int64_t cap) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    const double dx = g->dx, dy = g->dy, Kcont = M->Kcont;
+    const int64_t n = Nx1 * Ny1 * 6;
+    coo_t L = {n, 0, cap, (int64_t*)malloc(sizeof(int64_t) * cap),
+               (int64_t*)malloc(sizeof(int64_t) * cap),
+               (double*)malloc(sizeof(double) * cap), 0};
+    for (int64_t k = 0; k < n; ++k) R[k] = 0.0;
+#define B_(a, i, j) G(a, i, j, Ny)
+#define P_(a, i, j) G(a, i, j, Ny1)
+    for (int64_t j = 1; j <= Nx1; ++j) {
+        for (int64_t i = 1; i <= Ny1; ++i) {
+            int64_t kvx = ((j - 1) * Ny1 + i - 1) * 6 + 1;
+            int64_t kvy = kvx + 1, kpm = kvx + 2, kqx = kvx + 3, kqy = kvx + 4,
+                    kpf = kvx + 5;
+            /* Vx equation E:3128-3262 */
+            if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
+                coo_add(&L, 1.0, kvx, kvx);
+                if (j == 1) R[kvx - 1] = M->vxleft;
+                if (j == Nx) R[kvx - 1] = M->vxright;
+                if (i == 1 && 1 < j && j < Nx) coo_add(&L, M->bctop, kvx, kvx + 6);
+                if (i == Ny1 && 1 < j && j < Nx) coo_add(&L, M->bcbottom, kvx, kvx - 6);
+            } else {
+                double ETA1 = B_(ETA, i - 1, j) * B_(GGG, i - 1, j) * dt /
+                              (B_(GGG, i - 1, j) * dt + B_(ETA, i - 1, j));
+                double ETA2 = B_(ETA, i, j) * B_(GGG, i, j) * dt /
+                              (B_(GGG, i, j) * dt + B_(ETA, i, j));
+                double ETAP1 = P_(ETAP, i, j) * P_(GGGP, i, j) * dt /
+                               (P_(GGGP, i, j) * dt + P_(ETAP, i, j));
+                double ETAP2 = P_(ETAP, i, j + 1) * P_(GGGP, i, j + 1) * dt /
+                               (P_(GGGP, i, j + 1) * dt + P_(ETAP, i, j + 1));
+                double SXY1 = B_(SXY0, i - 1, j) * B_(ETA, i - 1, j) /
+                              (B_(GGG, i - 1, j) * dt + B_(ETA, i - 1, j));
+                double SXY2 = B_(SXY0, i, j) * B_(ETA, i, j) /
+                              (B_(GGG, i, j) * dt + B_(ETA, i, j));
+                double SXX1 = P_(SXX0, i, j) * P_(ETAP, i, j) /
+                              (P_(GGGP, i, j) * dt + P_(ETAP, i, j));
+                double SXX2 = P_(SXX0, i, j + 1) * P_(ETAP, i, j + 1) /
+                              (P_(GGGP, i, j + 1) * dt + P_(ETAP, i, j + 1));
+                double dRdx = 0.5 * (P_(RHOX, i, j + 1) - P_(RHOX, i, j - 1)) * (1.0 / dx);
+                double dRdy = 0.5 * (P_(RHOX, i + 1, j) - P_(RHOX, i - 1, j)) * (1.0 / dy);
+                double gxij = P_(gx, i, j);
+                double idx = 1.0 / dx, idy = 1.0 / dy;
+                coo_add(&L, ETAP1 / (dx * dx), kvx, kvx - 6 * Ny1);
+                coo_add(&L, ETA1 / (dy * dy), kvx, kvx - 6);
+                coo_add(&L, -(ETAP1 + ETAP2) * (1.0 / (dx * dx)) -
+                                (ETA1 + ETA2) * (1.0 / (dy * dy)) - dRdx * gxij * dt,
+                        kvx, kvx);
+                coo_add(&L, ETA2 / (dy * dy), kvx, kvx + 6);
+                coo_add(&L, ETAP2 / (dx * dx), kvx, kvx + 6 * Ny1);
+                coo_add(&L, ETAP1 * idx * idy - ETA2 * idx * idy - dRdy * gxij * dt * 0.25,
+                        kvx, kvy);
+                coo_add(&L, -ETAP2 * idx * idy + ETA2 * idx * idy - dRdy * gxij * dt * 0.25,
+                        kvx, kvy + 6 * Ny1);
+                coo_add(&L, -ETAP1 * idx * idy + ETA1 * idx * idy - dRdy * gxij * dt * 0.25,
+                        kvx, kvy - 6);
+                coo_add(&L, ETAP2 * idx * idy - ETA1 * idx * idy - dRdy * gxij * dt * 0.25,
+                        kvx, kvy + 6 * Ny1 - 6);
+                coo_add(&L, Kcont * idx, kvx, kpm);
+                coo_add(&L, -Kcont * idx, kvx, kpm + 6 * Ny1);
+                R[kvx - 1] = -P_(RHOX, i, j) * gxij - (SXY2 - SXY1) * idy - (SXX2 - SXX1) * idx;
+            }
+            /* Vy equation E:3264-3396 */
+            if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
+                coo_add(&L, 1.0, kvy, kvy);
+                if (i == 1) R[kvy - 1] = M->vytop;
+                if (i == Ny) R[kvy - 1] = M->vybottom;
+                if (j == 1 && 1 < i && i < Ny) coo_add(&L, M->bcleft, kvy, kvy + 6 * Ny1);
+                if (j == Nx1 && 1 < i && i < Ny) coo_add(&L, M->bcright, kvy, kvy - 6 * Ny1);
+            } else {
+                double ETA1 = B_(ETA, i, j - 1) * B_(GGG, i, j - 1) * dt /
+                              (B_(GGG, i, j - 1) * dt + B_(ETA, i, j - 1));
+                double ETA2 = B_(ETA, i, j) * B_(GGG, i, j) * dt /
+                              (B_(GGG, i, j) * dt + B_(ETA, i, j));
+                double ETAP1 = P_(ETAP, i, j) * P_(GGGP, i, j) * dt /
+                               (P_(GGGP, i, j) * dt + P_(ETAP, i, j));
+                double ETAP2 = P_(ETAP, i + 1, j) * P_(GGGP, i + 1, j) * dt /
+                               (P_(GGGP, i + 1, j) * dt + P_(ETAP, i + 1, j));
+                double SXY1 = B_(SXY0, i, j - 1) * B_(ETA, i, j - 1) /
+                              (B_(GGG, i, j - 1) * dt + B_(ETA, i, j - 1));
+                double SXY2 = B_(SXY0, i, j) * B_(ETA, i, j) /
+                              (B_(GGG, i, j) * dt + B_(ETA, i, j));
+                double SYY1 = -P_(SXX0, i, j) * P_(ETAP, i, j) /
+                              (P_(GGGP, i, j) * dt + P_(ETAP, i, j));
+                double SYY2 = -P_(SXX0, i + 1, j) * P_(ETAP, i + 1, j) /
+                              (P_(GGGP, i + 1, j) * dt + P_(ETAP, i + 1, j));
+                double dRdx = 0.5 * (P_(RHOY, i, j + 1) - P_(RHOY, i, j - 1)) / dx;
+                double dRdy = 0.5 * (P_(RHOY, i + 1, j) - P_(RHOY, i - 1, j)) / dy;
+                double gyij = P_(gy, i, j);
+                double idx = 1.0 / dx, idy = 1.0 / dy;
+                coo_add(&L, ETA1 / (dx * dx), kvy, kvy - 6 * Ny1);
+                coo_add(&L, ETAP1 / (dy * dy), kvy, kvy - 6);
+                coo_add(&L, -(ETAP1 + ETAP2) * (1.0 / (dy * dy)) -
+                                (ETA1 + ETA2) * (1.0 / (dx * dx)) - dRdy * gyij * dt,
+                        kvy, kvy);
+                coo_add(&L, ETAP2 * (1.0 / (dy * dy)), kvy, kvy + 6);
+                coo_add(&L, ETA2 * (1.0 / (dx * dx)), kvy, kvy + 6 * Ny1);
+                coo_add(&L, ETAP1 * idx * idy - ETA2 * idx * idy - dRdx * gyij * dt * 0.25,
+                        kvy, kvx);
+                coo_add(&L, -ETAP2 * idx * idy + ETA2 * idx * idy - dRdx * gyij * dt * 0.25,
+                        kvy, kvx + 6);
+                coo_add(&L, -ETAP1 * idx * idy + ETA1 * idx * idy - dRdx * gyij * dt * 0.25,
+                        kvy, kvx - 6 * Ny1);
+                coo_add(&L, ETAP2 * idx * idy - ETA1 * idx * idy - dRdx * gyij * dt * 0.25,
+                        kvy, kvx + 6 - 6 * Ny1);
+                coo_add(&L, Kcont * idy, kvy, kpm);
+                coo_add(&L, -Kcont * idy, kvy, kpm + 6);
+                R[kvy - 1] = -P_(RHOY, i, j) * gyij - (SXY2 - SXY1) * idx - (SYY2 - SYY1) * idy;
+            }
+            /* P equation E:3398-3452 */
+            int anchor = (i == 2 && 2 <= j && j <= Nx) || (j == 2 && 2 < i && i < Ny) ||
+                         (i == Ny && 2 <= j && j <= Nx) || (j == Nx && 2 < i && i < Ny);
+            if (i == 1 || i == Ny1 || j == 1 || j == Nx1) {
+                coo_add(&L, 1.0, kpm, kpm);
+            } else if (anchor) {
+                coo_add(&L, Kcont, kpm, kpm);
+                R[kpm - 1] = M->psurface;
+            } else {
+                double phi = P_(PHI, i, j), ep = P_(ETAPHI, i, j), bp = P_(BETAPHI, i, j);
+                coo_add(&L, -1.0 / dx, kpm, kvx - 6 * Ny1);
+                coo_add(&L, 1.0 / dx, kpm, kvx);
+                coo_add(&L, -1.0 / dy, kpm, kvy - 6);
+                coo_add(&L, 1.0 / dy, kpm, kvy);
+                coo_add(&L, Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpm);
+                coo_add(&L, -Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpf);
+                R[kpm - 1] = (P_(pr0, i, j) - P_(pf0, i, j)) / (1.0 - phi) * bp / dt +
+                             P_(DMP, i, j);
+            }
+            /* qx Darcy equation E:3454-3480 */
+            if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
+                coo_add(&L, 1.0, kqx, kqx);
+                if (i == 1 && 1 < j && j < Nx) coo_add(&L, M->bcftop, kqx, kqx + 6);
+                if (i == Ny1 && 1 < j && j < Nx) coo_add(&L, M->bcfbottom, kqx, kqx - 6);
+            } else {
+                coo_add(&L, P_(RX, i, j), kqx, kqx);
+                coo_add(&L, -Kcont * (1.0 / dx), kqx, kpf);
+                coo_add(&L, Kcont * (1.0 / dx), kqx, kpf + 6 * Ny1);
+                R[kqx - 1] = P_(RHOFX, i, j) * P_(gx, i, j);
+            }
+            /* qy Darcy equation E:3482-3514 */
+            if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
+                coo_add(&L, 1.0, kqy, kqy);
+                if (j == 1 && 1 < i && i < Ny) coo_add(&L, M->bcfleft, kqy, kqy + 6 * Ny1);
+                if (j == Nx1 && 1 < i && i < Ny) coo_add(&L, M->bcfright, kqy, kqy - 6 * Ny1);
+            } else {
+                coo_add(&L, P_(RY, i, j), kqy, kqy);
+                coo_add(&L, -Kcont * (1.0 / dy), kqy, kpf);
+                coo_add(&L, Kcont * (1.0 / dy), kqy, kpf + 6);
+                R[kqy - 1] = P_(RHOFY, i, j) * P_(gy, i, j);
+            }
+            /* P fluid equation E:3516-3565 */
+            if (i == 1 || i == Ny1 || j == 1 || j == Nx1) {
+                coo_add(&L, 1.0, kpf, kpf);
+            } else if (anchor) {
+                coo_add(&L, Kcont, kpf, kpf);
+                R[kpf - 1] = M->psurface;
+            } else {
+                double phi = P_(PHI, i, j), ep = P_(ETAPHI, i, j), bp = P_(BETAPHI, i, j);
+                coo_add(&L, -(1.0 / dx), kpf, kqx - 6 * Ny1);
+                coo_add(&L, (1.0 / dx), kpf, kqx);
+                coo_add(&L, -(1.0 / dy), kpf, kqy - 6);
+                coo_add(&L, (1.0 / dy), kpf, kqy);
+                coo_add(&L, -Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpm);
+                coo_add(&L, Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpf);
+                R[kpf - 1] = -(P_(pr0, i, j) - P_(pf0, i, j)) / (1.0 - phi) * bp / dt;
+            }
+        }
+    }
+#undef B_
+#undef P_
+    int64_t nnz = -1;
+    if (!L.overflow) nnz = coo_to_csc(&L, colptr, rowval, nzval);
+    free(L.row);
+    free(L.col);
+    free(L.val);
+    return nnz;
+}
+
+/* assemble_thermal_lse! E:4526-4611 */
+int64_t orc_assemble_thermal(const ereb_geometry* g, const double* tk1,
+    const double* RHOCP, const double* KX, const double* KY, const double* HR,
+    const double* HA, const double* HS, const double* DHP, double* RT,
+    double dt, int64_t* colptr, int64_t* rowval, double* nzval, int64_t cap) {
+    const int64_t Nx1 = g->Nx + 1, Ny1 = g->Ny + 1;
+    const double dx = g->dx, dy = g->dy;
+    const int64_t n = Nx1 * Ny1;
+    coo_t L = {n, 0, cap, (int64_t*)malloc(sizeof(int64_t) * cap),
+               (int64_t*)malloc(sizeof(int64_t) * cap),
+               (double*)malloc(sizeof(double) * cap), 0};
+    for (int64_t k = 0; k < n; ++k) RT[k] = 0.0;
+    for (int64_t j = 1; j <= Nx1; ++j) {
+        for (int64_t i = 1; i <= Ny1; ++i) {
+            int64_t gk = (j - 1) * Ny1 + i;
+            if (i == 1 || i == Ny1 || j == 1 || j == Nx1) {
+                coo_add(&L, 1.0, gk, gk);
+                if (j == 1) coo_add(&L, -1.0, gk, gk + Ny1);
+                if (j == Nx1) coo_add(&L, -1.0, gk, gk - Ny1);
+                if (i == 1 && 1 < j && j < Nx1) coo_add(&L, -1.0, gk, gk + 1);
+                if (i == Ny1 && 1 < j && j < Nx1) coo_add(&L, -1.0, gk, gk - 1);
+            } else {
+                double Kx1 = G(KX, i, j - 1, Ny1), Kx2 = G(KX, i, j, Ny1);
+                double Ky1 = G(KY, i - 1, j, Ny1), Ky2 = G(KY, i, j, Ny1);
+                double rc = G(RHOCP, i, j, Ny1);
+                coo_add(&L, -Kx1 * (1.0 / (dx * dx)), gk, gk - Ny1);
+                coo_add(&L, -Ky1 * (1.0 / (dy * dy)), gk, gk - 1);
+                coo_add(&L, rc / dt + (Kx1 + Kx2) * (1.0 / (dx * dx)) +
+                                (Ky1 + Ky2) * (1.0 / (dy * dy)),
+                        gk, gk);
+                coo_add(&L, -Ky2 * (1.0 / (dy * dy)), gk, gk + 1);
+                coo_add(&L, -Kx2 * (1.0 / (dx * dx)), gk, gk + Ny1);
+                RT[gk - 1] = rc / dt * G(tk1, i, j, Ny1) + G(HR, i, j, Ny1) +
+                             G(HA, i, j, Ny1) + G(HS, i, j, Ny1) + G(DHP, i, j, Ny1);
+            }
+        }
+    }
+    int64_t nnz = -1;
+    if (!L.overflow) nnz = coo_to_csc(&L, colptr, rowval, nzval);
+    free(L.row);
+    free(L.col);
+    free(L.val);
+    return nnz;
+}
+
+/* ------------------------------------------------- sort specification */
+
+/* Tile key of the library's scatter: basic cell (i, j) from fix with the
+ * basic clamps (E:1456, C:68-74); tiles are `tile` x `tile` basic cells,
+ * numbered column-major: key = tj * ntiles_y + ti. */
+void orc_tile_keys(const ereb_geometry* g, int64_t tile, const double* xm,
+                   const double* ym, int64_t marknum, int32_t* key) {
+    const int64_t Nx = g->Nx, Ny = g->Ny;
+    const int64_t nty = (Ny - 1 + tile - 1) / tile;
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t i, j;
+        orc_fix(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1, Ny - 1, &i, &j);
+        key[m] = (int32_t)(((j - 1) / tile) * nty + (i - 1) / tile);
+    }
+}
+
+/* Stable counting sort of 0..marknum-1 by key: perm[k] = marker index,
+ * start[t] = first position of tile t (start[ntiles] = marknum). */
+void orc_tile_sort(const int32_t* key, int64_t marknum, int64_t ntiles,
+                   int32_t* perm, int64_t* start) {
+    int64_t* cnt = (int64_t*)calloc((size_t)ntiles + 1, sizeof(int64_t));
+    for (int64_t m = 0; m < marknum; ++m) cnt[key[m] + 1]++;
+    for (int64_t t = 0; t < ntiles; ++t) cnt[t + 1] += cnt[t];
+    for (int64_t t = 0; t <= ntiles; ++t) start[t] = cnt[t];
+    for (int64_t m = 0; m < marknum; ++m) perm[cnt[key[m]]++] = (int32_t)m;
+    free(cnt);
+}
+
+/* ----------------------------------------------- derived grid helpers */
+
+/* compute_rotation_rate! E:4946-4956 */
+void orc_compute_rotation_rate(const ereb_geometry* g, const double* vx,
+                               const double* vy, double* wyx) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    for (int64_t j = 1; j <= Nx; ++j)
+        for (int64_t i = 1; i <= Ny; ++i)
+            G(wyx, i, j, Ny) = 0.5 * ((G(vy, i, j + 1, Ny1) - G(vy, i, j, Ny1)) * (1.0 / g->dx) -
+                                      (G(vx, i + 1, j, Ny1) - G(vx, i, j, Ny1)) * (1.0 / g->dy));
+}
+
+int orc_version(void) { return 1; }

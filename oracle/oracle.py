@@ -1,0 +1,277 @@
+"""ctypes loader of the CPU oracle (oracle/ereb_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the CPU-baseline leg of bench.py, always as the checker or the CPU
+baseline, never as the product path.  Every wrapper takes numpy arrays in
+Julia layout (Fortran-ordered grids) and follows the reference function
+named in its docstring (E: = src/Erebus.jl).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libereb_oracle.so")
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "erebus.jl_amd"))
+
+from ereb.constants import CGeometry, CMaterials, Geometry, Materials  # noqa: E402
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I64 = ctypes.c_int64
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        sig = {
+            "orc_total": (_D, [_D, _D, _D]),
+            "orc_ktotal": (_D, [_D, _D, _D]),
+            "orc_kphi": (_D, [_D, _D, _D]),
+            "orc_etaf_inv_kphi": (_D, [_D, _D, _D, _D]),
+            "orc_compute_rhocpfluidm": (_D, [_P, _D, _I64, _P]),
+            "orc_compute_ksolidm": (_D, [_P, _D, _I64, _P]),
+            "orc_compute_kfluidm": (_D, [_P, _D, _I64, _P]),
+            "orc_etatotal_rocks": (_D, [_P, _D, _I64]),
+            "orc_q_radiogenic": (_D, [_D, _D, _D, _D, _D]),
+            "orc_fix": (None, [_D, _D, _P, _P, _D, _D, _I64, _I64, _I64, _I64, _P, _P]),
+            "orc_fix_distances": (None, [_D, _D, _P, _P, _D, _D, _I64, _I64, _I64, _I64,
+                                         _P, _P, _P, _P]),
+            "orc_fix_weights": (None, [_D, _D, _P, _P, _D, _D, _I64, _I64, _I64, _I64,
+                                       _P, _P, _P]),
+            "orc_compute_marker_properties": (ctypes.c_int, [_P, _I64] + [_P] * 13 + [_I64]),
+            "orc_marker_to_nodes": (ctypes.c_int, [_P, _P] + [_P] * 23 + [_P, _P, _I64, _P,
+                                                                         _I64]),
+            "orc_compute_basic_node_properties": (None, [_P] * 17),
+            "orc_compute_v_node_properties": (None, [_P] * 12),
+            "orc_compute_p_node_properties": (None, [_P] * 21),
+            "orc_apply_insulating_bc": (None, [_P, _I64, _I64]),
+            "orc_update_marker_temperature": (None, [_P] * 6 + [_I64, _I64]),
+            "orc_update_marker_stress": (None, [_P] * 7 + [_I64]),
+            "orc_move_markers_rk4": (ctypes.c_int, [_P, _P] + [_P] * 13 + [_I64, _D, _I64]),
+            "orc_assemble_hydromechanical": (_I64, [_P, _P] + [_P] * 20 + [_D, _P, _P, _P, _P,
+                                                                           _I64]),
+            "orc_assemble_thermal": (_I64, [_P] + [_P] * 8 + [_P, _D, _P, _P, _P, _I64]),
+            "orc_tile_keys": (None, [_P, _I64, _P, _P, _I64, _P]),
+            "orc_tile_sort": (None, [_P, _I64, _I64, _P, _P]),
+            "orc_compute_rotation_rate": (None, [_P, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _F(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+class Oracle:
+    """Oracle bound to one geometry and material set."""
+
+    def __init__(self, geom: Geometry, mat: Materials):
+        self.L = lib()
+        self.geom = geom
+        self.mat = mat
+        self.cg = geom.to_c()
+        self.cm = mat.to_c(geom.xsize, geom.ysize)
+        self.pg = ctypes.addressof(self.cg)
+        self.pm = ctypes.addressof(self.cm)
+
+    # ---- scalar helpers -----------------------------------------------------
+    def fix(self, x, y, grid):
+        ax, ay, lim = self._grid(grid)
+        i, j = ctypes.c_int64(), ctypes.c_int64()
+        self.L.orc_fix(x, y, _p(ax), _p(ay), self.geom.dx, self.geom.dy, *lim,
+                       ctypes.addressof(i), ctypes.addressof(j))
+        return i.value, j.value
+
+    def fix_distances(self, x, y, grid):
+        ax, ay, lim = self._grid(grid)
+        i, j, a, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        self.L.orc_fix_distances(x, y, _p(ax), _p(ay), self.geom.dx, self.geom.dy, *lim,
+                                 ctypes.addressof(i), ctypes.addressof(j),
+                                 ctypes.addressof(a), ctypes.addressof(b))
+        return i.value, j.value, a.value, b.value
+
+    def fix_weights(self, x, y, grid):
+        ax, ay, lim = self._grid(grid)
+        i, j = ctypes.c_int64(), ctypes.c_int64()
+        w = np.zeros(4)
+        self.L.orc_fix_weights(x, y, _p(ax), _p(ay), self.geom.dx, self.geom.dy, *lim,
+                               ctypes.addressof(i), ctypes.addressof(j), _p(w))
+        return i.value, j.value, w
+
+    def _grid(self, grid):
+        a = self.geom.axes()
+        Nx, Ny = self.geom.Nx, self.geom.Ny
+        return {
+            "basic": (a["x"], a["y"], (1, Nx - 1, 1, Ny - 1)),
+            "vx": (a["xvx"], a["yvx"], (1, Nx - 1, 1, Ny)),
+            "vy": (a["xvy"], a["yvy"], (1, Nx, 1, Ny - 1)),
+            "p": (a["xp"], a["yp"], (1, Nx, 1, Ny)),
+            "p_stress": (a["xp"], a["yp"], (2, Nx - 1, 2, Ny - 1)),
+        }[grid]
+
+    def rhocpfluidm(self, T, mode):
+        return self.L.orc_compute_rhocpfluidm(self.pm, T, mode, None)
+
+    def ksolidm(self, T, mode):
+        return self.L.orc_compute_ksolidm(self.pm, T, mode, None)
+
+    def kfluidm(self, T, mode):
+        return self.L.orc_compute_kfluidm(self.pm, T, mode, None)
+
+    # ---- step 1 + 2 ---------------------------------------------------------
+    def marker_to_nodes(self, mk: dict, hrs, hrf, mode):
+        """Loop E:6419-6517; returns (props dict, list of 30 SUM grids)."""
+        g = self.geom
+        M = mk["xm"].size
+        ins = [_c(mk[k], np.int64 if k == "tm" else np.float64) for k in
+               ["xm", "ym", "tm", "tkm", "phim", "XWsolidm0", "etavpm", "inv_gggtotalm",
+                "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur", "alphasolidcur",
+                "alphafluidcur", "sxxm", "sxym"]]
+        names = ["rhototalm", "rhocptotalm", "etatotalm", "hrtotalm", "ktotalm",
+                 "tkm_rhocptotalm", "etafluidcur_inv_kphim"]
+        outs = [np.array(mk.get(n, np.zeros(M)), dtype=np.float64, copy=True) for n in names]
+        sums = [np.zeros((g.Ny, g.Nx), order="F") for _ in range(8)] + \
+               [np.zeros((g.Ny1, g.Nx1), order="F") for _ in range(22)]
+        sp = (ctypes.c_void_p * 30)(*[s.ctypes.data for s in sums])
+        hs, hf = _c(hrs), _c(hrf)
+        rc = self.L.orc_marker_to_nodes(self.pg, self.pm, *[_p(a) for a in ins],
+                                        *[_p(a) for a in outs], _p(hs), _p(hf), int(mode),
+                                        ctypes.addressof(sp), M)
+        if rc != 0:
+            raise ValueError(f"unknown mode {mode}")
+        return dict(zip(names, outs)), sums
+
+    # ---- node properties ------------------------------------------------------
+    def basic_node_properties(self, sums8, outs7, YNY):
+        s = [_F(a) for a in sums8]
+        o = [np.array(a, dtype=np.float64, order="F", copy=True) for a in outs7]
+        y = np.array(YNY, dtype=np.uint8, order="F", copy=True)
+        self.L.orc_compute_basic_node_properties(self.pg, *[_p(a) for a in s],
+                                                 *[_p(a) for a in o], _p(y))
+        return o, y
+
+    def v_node_properties(self, sums6, outs5):
+        s = [_F(a) for a in sums6]
+        o = [np.array(a, dtype=np.float64, order="F", copy=True) for a in outs5]
+        self.L.orc_compute_v_node_properties(self.pg, *[_p(a) for a in s], *[_p(a) for a in o])
+        return o
+
+    def p_node_properties(self, sums10, outs10):
+        s = [_F(a) for a in sums10]
+        o = [np.array(a, dtype=np.float64, order="F", copy=True) for a in outs10]
+        self.L.orc_compute_p_node_properties(self.pg, *[_p(a) for a in s], *[_p(a) for a in o])
+        return o
+
+    def insulating_bc(self, t):
+        t = np.array(t, dtype=np.float64, order="F", copy=True)
+        self.L.orc_apply_insulating_bc(_p(t), t.shape[0], t.shape[1])
+        return t
+
+    # ---- step 3 / 4 -------------------------------------------------------------
+    def update_marker_temperature(self, xm, ym, tkm, DT, tk2, timestep):
+        t = np.array(tkm, dtype=np.float64, copy=True)
+        xm, ym, DT, tk2 = _c(xm), _c(ym), _F(DT), _F(tk2)
+        self.L.orc_update_marker_temperature(self.pg, _p(xm), _p(ym), _p(t), _p(DT), _p(tk2),
+                                             int(timestep), t.size)
+        return t
+
+    def update_marker_stress(self, xm, ym, sxxm, sxym, DSXX, DSXY):
+        sx = np.array(sxxm, dtype=np.float64, copy=True)
+        sy = np.array(sxym, dtype=np.float64, copy=True)
+        xm, ym, DSXX, DSXY = _c(xm), _c(ym), _F(DSXX), _F(DSXY)
+        self.L.orc_update_marker_stress(self.pg, _p(xm), _p(ym), _p(sx), _p(sy), _p(DSXX),
+                                        _p(DSXY), sx.size)
+        return sx, sy
+
+    def move_markers_rk4(self, xm, ym, tm, tkm, phim, sxym, sxxm, vx, vy, vxf, vyf, wyx, tk2,
+                         dt, mode):
+        xm = np.array(xm, dtype=np.float64, copy=True)
+        ym = np.array(ym, dtype=np.float64, copy=True)
+        tkm = np.array(tkm, dtype=np.float64, copy=True)
+        sxym = np.array(sxym, dtype=np.float64, copy=True)
+        sxxm = np.array(sxxm, dtype=np.float64, copy=True)
+        tm = _c(tm, np.int64)
+        phim = _c(phim)
+        grids = [_F(a) for a in (vx, vy, vxf, vyf, wyx, tk2)]
+        rc = self.L.orc_move_markers_rk4(self.pg, self.pm, _p(xm), _p(ym), _p(tm), _p(tkm),
+                                         _p(phim), _p(sxym), _p(sxxm),
+                                         *[_p(a) for a in grids], xm.size, float(dt), int(mode))
+        if rc != 0:
+            raise ValueError(f"unknown mode {mode}")
+        return xm, ym, tkm, sxym, sxxm
+
+    # ---- step 5 -----------------------------------------------------------------
+    def assemble_hydromechanical(self, grids20, dt):
+        g = self.geom
+        n = 6 * g.Nx1 * g.Ny1
+        cap = 40 * g.Nx1 * g.Ny1
+        gs = [_F(a) for a in grids20]
+        R = np.zeros(n)
+        cp = np.zeros(n + 1, dtype=np.int64)
+        rv = np.zeros(cap, dtype=np.int64)
+        nz = np.zeros(cap)
+        nnz = self.L.orc_assemble_hydromechanical(self.pg, self.pm, *[_p(a) for a in gs],
+                                                  float(dt), _p(R), _p(cp), _p(rv), _p(nz), cap)
+        assert nnz >= 0
+        return cp, rv[:nnz].copy(), nz[:nnz].copy(), R
+
+    def assemble_thermal(self, grids8, dt):
+        g = self.geom
+        n = g.Nx1 * g.Ny1
+        cap = 5 * n
+        gs = [_F(a) for a in grids8]
+        R = np.zeros(n)
+        cp = np.zeros(n + 1, dtype=np.int64)
+        rv = np.zeros(cap, dtype=np.int64)
+        nz = np.zeros(cap)
+        nnz = self.L.orc_assemble_thermal(self.pg, *[_p(a) for a in gs], _p(R), float(dt),
+                                          _p(cp), _p(rv), _p(nz), cap)
+        assert nnz >= 0
+        return cp, rv[:nnz].copy(), nz[:nnz].copy(), R
+
+    # ---- sort specification -------------------------------------------------------
+    def tile_sort(self, xm, ym, tile=6):
+        g = self.geom
+        xm, ym = _c(xm), _c(ym)
+        key = np.zeros(xm.size, dtype=np.int32)
+        self.L.orc_tile_keys(self.pg, tile, _p(xm), _p(ym), xm.size, _p(key))
+        nty = (g.Ny - 1 + tile - 1) // tile
+        ntx = (g.Nx - 1 + tile - 1) // tile
+        perm = np.zeros(xm.size, dtype=np.int32)
+        start = np.zeros(ntx * nty + 1, dtype=np.int64)
+        self.L.orc_tile_sort(_p(key), xm.size, ntx * nty, _p(perm), _p(start))
+        return perm, key, start
+
+    def rotation_rate(self, vx, vy):
+        g = self.geom
+        w = np.zeros((g.Ny, g.Nx), order="F")
+        vx, vy = _F(vx), _F(vy)
+        self.L.orc_compute_rotation_rate(self.pg, _p(vx), _p(vy), _p(w))
+        return w

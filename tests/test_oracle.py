@@ -1,0 +1,865 @@
+"""Pins of the CPU oracle (oracle/ereb_oracle.c) -- it must be right before
+it can judge the GPU.  The reference ships no golden vectors (every test in
+test/runtests.jl draws fresh random inputs), so each oracle function is
+pinned by
+  (a) a port of the reference test's own restatement of that function
+      (T: = test/runtests.jl, themselves restated from Gerya's
+      HTM-planetary.m / HTM-hydration.m), with the reference test's tolerance
+      or tighter; these restatements use different operation orders
+      (/dx instead of *inv(dx), (1-a)(1-b) instead of (1-b)(1-a),
+      /2/dx instead of 0.5*inv(dx)), so a wrong index, sign or dropped term
+      in the oracle fails them;
+  (b) closed forms and invariants: partition of unity of the weights,
+      exact reproduction of affine fields, conservation of the scattered
+      mass, constant fields under normalisation, uniform and affine velocity
+      fields under RK4, stress rotation preserving sxx^2 + sxy^2;
+  (c) the committed fixture tests/golden/*.npz (oracle outputs on seeded
+      inputs) to catch unintended changes of the oracle itself.
+
+  oracle function                    pins
+  orc_fix / fix_distances / weights  T:1127-1387 lattice, T:1389-1793 corner + random (exact ==)
+  orc_total / ktotal / kphi / etaf   T:939-1008 (literals exact, formulas rtol 1e-12)
+  rhocpfluidm / ksolidm / kfluidm    T:3424-3480 fixed temperatures, both modes
+  orc_compute_marker_properties      T:757-816 restatement (exact), both modes
+  orc_marker_to_nodes                T:1806-2064 / T:2113-2628 style restatement (exact),
+                                     conservation, constant property
+  orc_compute_*_node_properties      T:2633-3263 style restatement (exact), constants
+  orc_apply_insulating_bc            T:3500-3510
+  orc_update_marker_stress           T:5201-5270 restatement (exact), affine exactness
+  orc_update_marker_temperature      T:5554-5600 restatement (exact), both branches
+  orc_move_markers_rk4               T:5759-6110 restatement (positions rtol 1e-12 as the
+                                     reference; with the E:5173 centre value
+                                     kept, temperatures rtol 1e-12), uniform / affine
+                                     velocity closed forms, rotation invariant
+  orc_assemble_hydromechanical       T:4147-4445 dense restatement (rtol 1e-9 on
+                                     values as T:4447; pattern = nonzeros of it)
+  orc_assemble_thermal               T:5391-5455 dense restatement
+  orc_tile_sort                      permutation, key order, stability, brute force
+"""
+from __future__ import annotations
+
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import cases
+from ereb.constants import test_constants as tconsts
+from oracle import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def tc():
+    g, mat = tconsts()
+    return g, mat, Oracle(g, mat)
+
+
+def trunc(v):
+    return int(math.trunc(v))
+
+
+# ---------------------------------------------------------------- fix family
+
+GRIDS = {  # name: axes, clamps (jmin, jmax, imin, imax) as C:66-101
+    "basic": ("x", "y", lambda g: (1, g.Nx - 1, 1, g.Ny - 1)),
+    "vx": ("xvx", "yvx", lambda g: (1, g.Nx - 1, 1, g.Ny)),
+    "vy": ("xvy", "yvy", lambda g: (1, g.Nx, 1, g.Ny - 1)),
+    "p": ("xp", "yp", lambda g: (1, g.Nx, 1, g.Ny)),
+}
+
+
+def fix_ver(g, xx, yy, grid):
+    """T:1143-1154: j = trunc((x - x[1])/dx)+1 with the grid's clamps,
+    distances and MATLAB-ordered weights (T:1407-1413)."""
+    axn, ayn, lim = GRIDS[grid]
+    a = g.axes()
+    ax, ay = a[axn], a[ayn]
+    jmin, jmax, imin, imax = lim(g)
+    j = trunc((xx - ax[0]) / g.dx) + 1
+    i = trunc((yy - ay[0]) / g.dy) + 1
+    j = min(max(j, jmin), jmax)
+    i = min(max(i, imin), imax)
+    dxmj = xx - ax[j - 1]
+    dymi = yy - ay[i - 1]
+    w = [(1 - dxmj / g.dx) * (1 - dymi / g.dy), (1 - dxmj / g.dx) * (dymi / g.dy),
+         (dxmj / g.dx) * (1 - dymi / g.dy), (dxmj / g.dx) * (dymi / g.dy)]
+    return i, j, dxmj, dymi, w
+
+
+def frange(a: Fraction, step: Fraction, b: Fraction):
+    """Julia's a:step:b for rational endpoints: exactly rounded elements."""
+    n = int((b - a) / step)
+    return [float(a + k * step) for k in range(n + 1)]
+
+
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_fix_lattice(tc, grid):
+    g, _, orc = tc
+    dx, dy = Fraction(g.dx), Fraction(g.dy)
+    xs = frange(-dx, dx / 3, Fraction(g.xsize) + dx)
+    ys = frange(-dy, dy / 3, Fraction(g.ysize) + dy)
+    for yy in ys:
+        for xx in xs:
+            i, j, dxmj, dymi, w = fix_ver(g, xx, yy, grid)
+            assert orc.fix(xx, yy, grid) == (i, j)
+            assert orc.fix_distances(xx, yy, grid) == (i, j, dxmj, dymi)
+            oi, oj, ow = orc.fix_weights(xx, yy, grid)
+            assert (oi, oj) == (i, j) and list(ow) == w
+
+
+@pytest.mark.parametrize("grid", list(GRIDS))
+def test_fix_weights_random_and_corners(tc, grid):
+    g, _, orc = tc
+    rng = np.random.default_rng(1)
+    # T:1626: rand(-x[1]:0.1:x[end]+dx), and the four domain corners T:1415-1440
+    xs = list(np.round(rng.integers(0, int((g.xsize + g.dx) * 10), 10_000)) / 10.0)
+    ys = list(np.round(rng.integers(0, int((g.ysize + g.dy) * 10), 10_000)) / 10.0)
+    a = g.axes()
+    xs += [-a["x"][0], a["x"][0], a["x"][-1], a["x"][-1]]
+    ys += [-a["y"][0], a["y"][-1], a["y"][0], a["y"][-1]]
+    for xx, yy in zip(xs, ys):
+        i, j, _, _, w = fix_ver(g, xx, yy, grid)
+        oi, oj, ow = orc.fix_weights(xx, yy, grid)
+        assert (oi, oj) == (i, j) and list(ow) == w
+        # partition of unity (weights extrapolate outside the grid, so the
+        # rounding scale is sum |w|)
+        assert abs(sum(ow) - 1.0) <= 4 * 2.23e-16 * np.abs(ow).sum()
+
+
+def test_fix_out_of_range_coordinates(tc):
+    g, _, orc = tc
+    # NaN and |t| >= 2^63 follow x86-64 unsafe_trunc: typemin -> lower clamp
+    for v in (float("nan"), float("inf"), -float("inf"), 1e300, -1e300):
+        assert orc.fix(v, v, "p") == (1, 1)
+    assert orc.fix(1e16, -1e16, "basic") == (1, g.Nx - 1)
+
+
+def test_bilinear_reproduces_affine(tc):
+    g, _, orc = tc
+    rng = np.random.default_rng(2)
+    a = g.axes()
+    for grid in GRIDS:
+        axn, ayn, _ = GRIDS[grid]
+        for _ in range(500):
+            x, y = rng.random() * g.xsize, rng.random() * g.ysize
+            i, j, w = orc.fix_weights(x, y, grid)
+            f = lambda X, Y: 3.0 + 2.0 * X - 0.5 * Y
+            nodes = [(i, j), (i + 1, j), (i, j + 1), (i + 1, j + 1)]
+            v = sum(wk * f(a[axn][jj - 1], a[ayn][ii - 1]) for wk, (ii, jj) in zip(w, nodes))
+            assert v == pytest.approx(f(x, y), rel=1e-13, abs=1e-9)
+
+
+# ------------------------------------------------------------ scalar helpers
+
+def test_scalar_helpers(tc):
+    g, mat, orc = tc
+    L = orc.L
+    # total(): literal tests T:940-944
+    assert L.orc_total(0, 0, 0) == 0 and L.orc_total(1, 0, 0) == 1
+    assert L.orc_total(0, 1, 0) == 0 and L.orc_total(0, 0, 1) == 0
+    assert L.orc_total(1, 2, 0.5) == 1.5
+    rng = np.random.default_rng(3)
+    for _ in range(1000):
+        s, f, p = rng.random(3)
+        assert L.orc_total(s, f, p) == pytest.approx(s * (1 - p) + f * p, rel=1e-12)
+        ks, kf, ph = rng.random(3)
+        ver = (ks * kf / 2 + ((ks * (3 * ph - 2) + kf * (1 - 3 * ph)) ** 2) / 16) ** 0.5 \
+            - (ks * (3 * ph - 2) + kf * (1 - 3 * ph)) / 4          # T:985-987
+        assert L.orc_ktotal(ks, kf, ph) == pytest.approx(ver, rel=1e-12)
+        k0, pm = rng.random(2)
+        ver = k0 * (pm / mat.phim0) ** 3 / ((1 - pm) / (1 - mat.phim0)) ** 2   # T:995
+        assert L.orc_kphi(mat.phim0, k0, pm) == pytest.approx(ver, rel=1e-12)
+        kr, ph, ef = rng.random(3)
+        ver = ef * (1 / (kr * (ph / mat.phim0) ** 3 / ((1 - ph) / (1 - mat.phim0)) ** 2))
+        assert L.orc_etaf_inv_kphi(mat.phim0, kr, ph, ef) == pytest.approx(ver, rel=1e-12)
+    # literal powers: q^3 is q*q*q exactly, as Julia's pow_body(x, 3)
+    q = 0.7
+    assert L.orc_etaf_inv_kphi(mat.phim0, 1.0, mat.phim0 / q, 1.0) != 0.0
+
+
+def test_fluid_solid_conductivity_and_heat_capacity(tc):
+    g, mat, orc = tc
+    tf = mat.tmfluidphase
+    Ts = [170.0, tf - 10.0, tf - 3.0, tf, tf + 3.0, tf + 10.0, 370.0, 470.0]  # T:3425
+
+    def rhocp_hobbs(T):  # T:3427-3439
+        if T >= 410:
+            return mat.rhoH2Of * (-4.67e4 + 333 * T - 0.731 * T ** 2 + 5.4e-4 * T ** 3)
+        if 410 > T >= tf + 5:
+            return mat.rhoH2Of * 4200
+        if tf + 5 > T >= tf:
+            return mat.rhoH2Of * (4200.0 + 0.1 * mat.Lf)
+        if tf > T >= tf - 5:
+            return mat.rhoH2Ofi * (7.67 * T + 0.1 * mat.Lf)
+        return mat.rhoH2Ofi * 7.67 * T
+
+    def kf_hobbs(T):  # T:3463-3471
+        if T >= 410:
+            return -0.142 + 4.12e-3 * T - 5.01e-6 * T ** 2
+        if 410 > T >= tf:
+            return -0.581 + 6.34e-3 * T - 7.93e-6 * T ** 2
+        return 0.465 + 488.0 / T
+
+    for T in Ts:
+        assert orc.rhocpfluidm(T, 1) == pytest.approx(rhocp_hobbs(T), rel=1e-12)
+        assert orc.rhocpfluidm(T, 9) == mat.rhocpfluidm[0]
+        assert orc.ksolidm(T, 1) == pytest.approx(0.73 + 1293.0 / (T + 77.0), rel=1e-12)
+        assert orc.ksolidm(T, 9) == mat.ksolidm[0]
+        assert orc.kfluidm(T, 1) == pytest.approx(kf_hobbs(T), rel=1e-12)
+        assert orc.kfluidm(T, 9) == mat.kfluidm[0]
+    assert math.isnan(orc.rhocpfluidm(300.0, 3))  # unknown mode
+
+
+# ------------------------------------------------------- marker properties
+
+def props_ver(mat, mk, hrs, hrf, mode, orc):
+    """T:757-816 restatement of compute_marker_properties!"""
+    M = mk["xm"].size
+    out = {k: np.array(mk[k], copy=True) for k in
+           ["rhototalm", "rhocptotalm", "etatotalm", "hrtotalm", "ktotalm"]}
+    tkr = np.zeros(M)
+    ekp = np.zeros(M)
+    MD, MH2O, VD, VW = mat.MD, mat.MH2O, mat.VDs, mat.VWs
+    for m in range(M):
+        t = int(mk["tm"][m]) - 1
+        T, ph, XW = mk["tkm"][m], mk["phim"][m], mk["XWsolidm0"][m]
+        kphim = mat.kphim0[t] * (ph / mat.phim0) ** 3 / ((1 - ph) / (1 - mat.phim0)) ** 2
+        if t < 2:
+            rcf, ks, kf = orc.rhocpfluidm(T, mode), orc.ksolidm(T, mode), orc.kfluidm(T, mode)
+            rhos = (MD + MH2O * XW) / (VW * XW + VD * (1 - XW))
+            rhof = mat.rhoH2Of if T > mat.tmfluidphase else mat.rhoH2Ofi
+            out["rhototalm"][m] = rhos * (1 - ph) + rhof * ph
+            out["rhocptotalm"][m] = mat.rhocpsolidm[t] * (1 - ph) + rcf * ph
+            es = mat.etasolidmm[t] if T > mat.tmsolidphase else mat.etasolidm[t]
+            out["hrtotalm"][m] = hrs[t] * (1 - ph) + hrf[t] * ph
+            out["ktotalm"][m] = (ks * kf / 2 + ((ks * (3 * ph - 2) + kf * (1 - 3 * ph)) ** 2)
+                                 / 16) ** 0.5 - (ks * (3 * ph - 2) + kf * (1 - 3 * ph)) / 4
+            ef = mat.etafluidmm[t] if T > mat.tmfluidphase else mat.etafluidm[t]
+            out["etatotalm"][m] = max(mat.etamin, ef, es)
+        else:
+            ef = mat.etafluidm[t]
+        tkr[m] = T * out["rhocptotalm"][m]
+        ekp[m] = ef / kphim
+    out["tkm_rhocptotalm"] = tkr
+    out["etafluidcur_inv_kphim"] = ekp
+    return out
+
+
+@pytest.mark.parametrize("mode", [9, 1])
+def test_marker_properties(tc, mode):
+    g, mat, orc = tc
+    mk = cases.ref_markers(g, mat, seed=10 + mode, extra=2000)
+    hrs, hrf = np.array([1e-6, 2e-6, 0.0]), np.array([3e-7, 0.0, 0.0])
+    props, _ = orc.marker_to_nodes(mk, hrs, hrf, mode)
+    ver = props_ver(mat, mk, hrs, hrf, mode, orc)
+    for k, v in ver.items():
+        rtol = 1e-12 if k in ("ktotalm", "etafluidcur_inv_kphim", "tkm_rhocptotalm") else 0.0
+        np.testing.assert_allclose(props[k], v, rtol=rtol, atol=0, err_msg=k)
+    with pytest.raises(ValueError):
+        orc.marker_to_nodes(mk, hrs, hrf, 5)
+
+
+# --------------------------------------------------------------- scatter
+
+SUMSPEC = [  # (grid, property) per SUM slot, reset order (E:968-999)
+    ("basic", "etatotalm"), ("basic", "etavpm"), ("basic", "inv_gggtotalm"),
+    ("basic", "sxym"), ("basic", "cohestotalm"), ("basic", "tenstotalm"),
+    ("basic", "fricttotalm"), ("basic", None),
+    ("vx", "rhototalm"), ("vx", "rhofluidcur"), ("vx", "ktotalm"), ("vx", "phim"),
+    ("vx", "etafluidcur_inv_kphim"), ("vx", None),
+    ("vy", "rhototalm"), ("vy", "rhofluidcur"), ("vy", "ktotalm"), ("vy", "phim"),
+    ("vy", "etafluidcur_inv_kphim"), ("vy", None),
+    ("p", "rhototalm"), ("p", "rhocptotalm"), ("p", "alphasolidcur"),
+    ("p", "alphafluidcur"), ("p", "hrtotalm"), ("p", "inv_gggtotalm"), ("p", "sxxm"),
+    ("p", "tkm_rhocptotalm"), ("p", "phim"), ("p", None),
+]
+
+
+def scatter_ver(g, mk, props):
+    """T:1806-2064 / T:2113-2628: interpolate_add_to_grid! restated with
+    MATLAB weights, markers in order m = 1..marknum."""
+    vals = dict(mk)
+    vals.update(props)
+    sums = [np.zeros((g.Ny, g.Nx) if gr == "basic" else (g.Ny1, g.Nx1)) for gr, _ in SUMSPEC]
+    M = mk["xm"].size
+    cache = {}
+    for m in range(M):
+        x, y = mk["xm"][m], mk["ym"][m]
+        for gr in GRIDS:
+            i, j, _, _, w = fix_ver(g, x, y, gr)
+            cache[gr] = (i - 1, j - 1, w)
+        for s, (gr, p) in enumerate(SUMSPEC):
+            i, j, w = cache[gr]
+            v = 1.0 if p is None else vals[p][m]
+            S = sums[s]
+            S[i, j] += v * w[0]
+            S[i + 1, j] += v * w[1]
+            S[i, j + 1] += v * w[2]
+            S[i + 1, j + 1] += v * w[3]
+    return sums
+
+
+def test_marker_to_nodes_restatement(tc):
+    g, mat, orc = tc
+    mk = cases.ref_markers(g, mat, seed=31, extra=3000)
+    props, sums = orc.marker_to_nodes(mk, np.zeros(3), np.zeros(3), 9)
+    ver = scatter_ver(g, mk, props)
+    for s, (a, b) in enumerate(zip(sums, ver)):
+        np.testing.assert_array_equal(a, b, err_msg=f"SUM slot {s}")
+
+
+def test_marker_to_nodes_conservation_and_constants():
+    g, mat = tconsts()
+    orc = Oracle(g, mat)
+    mk = cases.ref_markers(g, mat, seed=32, extra=0)  # all markers inside
+    for k in ("etavpm", "cohestotalm", "alphasolidcur"):
+        mk[k][:] = 7.25
+    props, sums = orc.marker_to_nodes(mk, np.zeros(3), np.zeros(3), 9)
+    M = mk["xm"].size
+    vals = dict(mk)
+    vals.update(props)
+    wt = {"basic": sums[7], "vx": sums[13], "vy": sums[19], "p": sums[29]}
+    for s, (gr, p) in enumerate(SUMSPEC):
+        tot = M if p is None else vals[p].sum()
+        assert abs(sums[s].sum() - tot) <= 1e-12 * np.abs(vals[p] if p else 1).sum() * 4, s
+    for s in (1, 4):  # constant property -> SUM == c * WT
+        np.testing.assert_allclose(sums[s], 7.25 * sums[7], rtol=1e-14)
+    np.testing.assert_allclose(sums[22], 7.25 * sums[29], rtol=1e-14)
+
+
+# ---------------------------------------------------------- node properties
+
+def test_node_properties_restatement(tc):
+    g, mat, orc = tc
+    rng = np.random.default_rng(40)
+    B, S = (g.Ny, g.Nx), (g.Ny1, g.Nx1)
+    sb = [cases.rand_grid(rng, B) for _ in range(8)]
+    sb[7][rng.random(B) < 0.25] = 0.0
+    ob = [cases.rand_grid(rng, B) for _ in range(7)]
+    yny = np.zeros(B, dtype=np.uint8, order="F")
+    o, y = orc.basic_node_properties(sb, ob, yny)
+    for i in range(g.Ny):  # T:2633ff restatement
+        for j in range(g.Nx):
+            wt = sb[7][i, j]
+            if wt > 0:
+                exp = [sb[0][i, j] / wt, sb[1][i, j] / wt, wt / sb[2][i, j],
+                       sb[3][i, j] / wt, sb[4][i, j] / wt, sb[5][i, j] / wt, sb[6][i, j] / wt]
+                for k in range(7):
+                    assert o[k][i, j] == pytest.approx(exp[k], rel=1e-15)
+                assert y[i, j] == (1 if o[1][i, j] < o[0][i, j] else 0)
+            else:
+                for k in range(7):
+                    assert o[k][i, j] == ob[k][i, j]
+                assert y[i, j] == 0
+    sp = [cases.rand_grid(rng, S) for _ in range(10)]
+    sp[9][rng.random(S) < 0.25] = 0.0
+    op0 = [cases.rand_grid(rng, S) for _ in range(10)]
+    op = orc.p_node_properties(sp, op0)
+    wt = sp[9]
+    on = wt > 0
+    np.testing.assert_allclose(op[0][on], (sp[0] / wt)[on], rtol=1e-15)
+    np.testing.assert_allclose(op[5][on], (wt / sp[5])[on], rtol=1e-15)
+    np.testing.assert_allclose(op[7][on], (sp[7] / sp[1])[on], rtol=1e-15)  # tk1 = TK/RHOCP
+    np.testing.assert_allclose(op[9][on], (sp[8] / wt / (wt / sp[5]))[on], rtol=1e-14)
+    for k in range(10):
+        np.testing.assert_array_equal(op[k][~on], op0[k][~on])
+    # constant property: normalisation returns the constant
+    c = np.full(S, 3.5, order="F")
+    w = cases.rand_grid(rng, S) + 0.5
+    ov = orc.v_node_properties([c * w] * 5 + [w], [np.zeros(S)] * 5)
+    for a in ov:
+        np.testing.assert_allclose(a, 3.5, rtol=1e-15)
+
+
+def test_insulating_bc(tc):
+    _, _, orc = tc
+    rng = np.random.default_rng(41)
+    for j in range(3, 11):  # T:3500-3510
+        for i in range(3, 11):
+            t = orc.insulating_bc(rng.random((i, j)))
+            assert (t[0, 1:j - 1] == t[1, 1:j - 1]).all()
+            assert (t[i - 1, 1:j - 1] == t[i - 2, 1:j - 1]).all()
+            assert (t[:, 0] == t[:, 1]).all()
+            assert (t[:, j - 1] == t[:, j - 2]).all()
+
+
+# ----------------------------------------------------------------- gathers
+
+def test_update_marker_stress_restatement(tc):
+    g, mat, orc = tc
+    rng = np.random.default_rng(50)
+    M = g.start_marknum
+    xm = rng.integers(int(-g.dx * 10), int((g.xsize + g.dx) * 10), M) / 10.0  # T:5204
+    ym = rng.integers(int(-g.dy * 10), int((g.ysize + g.dy) * 10), M) / 10.0
+    sx, sy = rng.random(M), rng.random(M)
+    DSXX = cases.rand_grid(rng, (g.Ny1, g.Nx1))
+    DSXY = cases.rand_grid(rng, (g.Ny, g.Nx))
+    ox, oy = orc.update_marker_stress(xm, ym, sx, sy, DSXX, DSXY)
+    a = g.axes()
+    for m in range(M):  # T:5216-5266
+        j = min(max(trunc((xm[m] - a["xp"][0]) / g.dx) + 1, 2), g.Nx - 1)
+        i = min(max(trunc((ym[m] - a["yp"][0]) / g.dy) + 1, 2), g.Ny - 1)
+        dxmj, dymi = xm[m] - a["xp"][j - 1], ym[m] - a["yp"][i - 1]
+        w = [(1 - dxmj / g.dx) * (1 - dymi / g.dy), (1 - dxmj / g.dx) * (dymi / g.dy),
+             (dxmj / g.dx) * (1 - dymi / g.dy), (dxmj / g.dx) * (dymi / g.dy)]
+        i, j = i - 1, j - 1
+        vx = sx[m] + (DSXX[i, j] * w[0] + DSXX[i + 1, j] * w[1] + DSXX[i, j + 1] * w[2]
+                      + DSXX[i + 1, j + 1] * w[3])
+        bi, bj, _, _, wb = fix_ver(g, xm[m], ym[m], "basic")
+        bi, bj = bi - 1, bj - 1
+        vy = sy[m] + (DSXY[bi, bj] * wb[0] + DSXY[bi + 1, bj] * wb[1]
+                      + DSXY[bi, bj + 1] * wb[2] + DSXY[bi + 1, bj + 1] * wb[3])
+        assert ox[m] == vx and oy[m] == vy
+
+
+@pytest.mark.parametrize("timestep", [1, 5])
+def test_update_marker_temperature(tc, timestep):
+    g, mat, orc = tc
+    rng = np.random.default_rng(51)
+    mk = cases.ref_markers(g, mat, seed=52, extra=2000)
+    # affine nodal field -> exact marker value inside the domain
+    a = g.axes()
+    XP, YP = np.meshgrid(a["xp"], a["yp"])
+    f = np.asfortranarray(250.0 + 0.001 * XP - 0.002 * YP)
+    DT = cases.rand_grid(rng, (g.Ny1, g.Nx1), -1, 1)
+    out = orc.update_marker_temperature(mk["xm"], mk["ym"], mk["tkm"], DT, f, timestep)
+    if timestep == 1:
+        inside = (mk["xm"] >= -g.dx / 2) & (mk["xm"] <= g.xsize + g.dx / 2) & \
+                 (mk["ym"] >= -g.dy / 2) & (mk["ym"] <= g.ysize + g.dy / 2)
+        exp = 250.0 + 0.001 * mk["xm"] - 0.002 * mk["ym"]
+        np.testing.assert_allclose(out[inside], exp[inside], rtol=1e-13)
+    else:
+        for m in range(0, mk["xm"].size, 7):  # T:5554-5600 restatement
+            i, j, _, _, w = fix_ver(g, mk["xm"][m], mk["ym"][m], "p")
+            i, j = i - 1, j - 1
+            v = mk["tkm"][m] + (DT[i, j] * w[0] + DT[i + 1, j] * w[1] + DT[i, j + 1] * w[2]
+                                + DT[i + 1, j + 1] * w[3])
+            assert out[m] == v
+
+
+# -------------------------------------------------------------------- RK4
+
+def rk4_ver(g, mat, xm, ym, tm, tkm, phim, sxym, sxxm, vx, vy, vxf, vyf, wyx, tk2, dtm,
+            mode, orc, centre_from_vx=True):
+    """T:5807-6082 restatement of move_markers_rk4!.  With centre_from_vx
+    the fluid loop takes the centre value of the left-half correction from
+    vx as the implementation does (E:5173); the test itself uses vxf
+    there and rhocpfluidm[tm] in the correction (T:5996, T:6081), hence its
+    rtol 1e-4 on tkm."""
+    a = g.axes()
+    Nx, Ny, dx, dy = g.Nx, g.Ny, g.dx, g.dy
+    sq = lambda v: v * v  # Julia's literal x^2 is x*x (Python's ** 2 calls pow)
+    xm, ym, tkm, sxym, sxxm = (np.array(v, copy=True) for v in (xm, ym, tkm, sxym, sxxm))
+
+    def vel_x(v, vc, x, y):
+        j = min(max(trunc((x - a["xvx"][0]) / dx) + 1, 1), Nx - 1)
+        i = min(max(trunc((y - a["yvx"][0]) / dy) + 1, 1), Ny)
+        dxmj, dymi = x - a["xvx"][j - 1], y - a["yvx"][i - 1]
+        i, j = i - 1, j - 1
+        v13 = v[i, j] * (1 - dxmj / dx) + v[i, j + 1] * dxmj / dx
+        v24 = v[i + 1, j] * (1 - dxmj / dx) + v[i + 1, j + 1] * dxmj / dx
+        if dxmj / dx >= 0.5:
+            if j + 1 < Nx - 1:
+                v13 = v13 + 1 / 2 * sq(dxmj / dx - 0.5) * (v[i, j] - 2 * v[i, j + 1] + v[i, j + 2])
+                v24 = v24 + 1 / 2 * sq(dxmj / dx - 0.5) * (v[i + 1, j] - 2 * v[i + 1, j + 1] + v[i + 1, j + 2])
+        elif j + 1 > 1:
+            v13 = v13 + 1 / 2 * sq(dxmj / dx - 0.5) * (v[i, j - 1] - 2 * vc[i, j] + v[i, j + 1])
+            v24 = v24 + 1 / 2 * sq(dxmj / dx - 0.5) * (v[i + 1, j - 1] - 2 * v[i + 1, j] + v[i + 1, j + 1])
+        return (1 - dymi / dy) * v13 + (dymi / dy) * v24
+
+    def vel_y(v, x, y):
+        j = min(max(trunc((x - a["xvy"][0]) / dx) + 1, 1), Nx)
+        i = min(max(trunc((y - a["yvy"][0]) / dy) + 1, 1), Ny - 1)
+        dxmj, dymi = x - a["xvy"][j - 1], y - a["yvy"][i - 1]
+        i, j = i - 1, j - 1
+        v12 = v[i, j] * (1 - dymi / dy) + v[i + 1, j] * dymi / dy
+        v34 = v[i, j + 1] * (1 - dymi / dy) + v[i + 1, j + 1] * dymi / dy
+        if dymi / dy >= 0.5:
+            if i + 1 < Ny - 1:
+                v12 = v12 + 1 / 2 * sq(dymi / dy - 0.5) * (v[i, j] - 2 * v[i + 1, j] + v[i + 2, j])
+                v34 = v34 + 1 / 2 * sq(dymi / dy - 0.5) * (v[i, j + 1] - 2 * v[i + 1, j + 1] + v[i + 2, j + 1])
+        elif i + 1 > 1:
+            v12 = v12 + 1 / 2 * sq(dymi / dy - 0.5) * (v[i - 1, j] - 2 * v[i, j] + v[i + 1, j])
+            v34 = v34 + 1 / 2 * sq(dymi / dy - 0.5) * (v[i - 1, j + 1] - 2 * v[i, j + 1] + v[i + 1, j + 1])
+        return (1 - dxmj / dx) * v12 + (dxmj / dx) * v34
+
+    def interp(gr, G, x, y):
+        i, j, _, _, w = fix_ver(g, x, y, gr)
+        i, j = i - 1, j - 1
+        return G[i, j] * w[0] + G[i + 1, j] * w[1] + G[i, j + 1] * w[2] + G[i + 1, j + 1] * w[3]
+
+    for m in range(xm.size):
+        tks0 = interp("p", tk2, xm[m], ym[m])
+        om = interp("basic", wyx, xm[m], ym[m])
+        th = dtm * om
+        sxn = sxxm[m] * sq(math.cos(th)) - sxxm[m] * sq(math.sin(th)) - sxym[m] * math.sin(2 * th)
+        syn = sxxm[m] * math.sin(2 * th) + sxym[m] * math.cos(2 * th)
+        sxxm[m], sxym[m] = sxn, syn
+        xA, yA = xm[m], ym[m]
+        xc, yc = xA, yA
+        vxm, vym = [0.0] * 4, [0.0] * 4
+        for rk in range(4):
+            vxm[rk] = vel_x(vx, vx, xc, yc)
+            vym[rk] = vel_y(vy, xc, yc)
+            if rk < 2:
+                xc, yc = xA + dtm / 2 * vxm[rk], yA + dtm / 2 * vym[rk]
+            elif rk == 2:
+                xc, yc = xA + dtm * vxm[rk], yA + dtm * vym[rk]
+        xm[m] = xA + dtm * (1 / 6 * (vxm[0] + 2 * vxm[1] + 2 * vxm[2] + vxm[3]))
+        ym[m] = yA + dtm * (1 / 6 * (vym[0] + 2 * vym[1] + 2 * vym[2] + vym[3]))
+        xA, yA = xm[m], ym[m]
+        xc, yc = xA, yA
+        for rk in range(4):
+            vxm[rk] = vel_x(vxf, vx if centre_from_vx else vxf, xc, yc)
+            vym[rk] = vel_y(vyf, xc, yc)
+            if rk < 2:
+                xc, yc = xA - dtm / 2 * vxm[rk], yA - dtm / 2 * vym[rk]
+            elif rk == 2:
+                xc, yc = xA - dtm * vxm[rk], yA - dtm * vym[rk]
+        xc = xA - dtm * (1 / 6 * (vxm[0] + 2 * vxm[1] + 2 * vxm[2] + vxm[3]))
+        yc = yA - dtm * (1 / 6 * (vym[0] + 2 * vym[1] + 2 * vym[2] + vym[3]))
+        d = interp("p", tk2, xc, yc) - tks0
+        t = int(tm[m]) - 1
+        ph = phim[m]
+        rcs = mat.rhocpsolidm[t]
+        rcf = orc.rhocpfluidm(tkm[m], mode) if centre_from_vx else mat.rhocpfluidm[t]
+        tkm[m] = ((1 - ph) * tkm[m] * rcs + ph * (tkm[m] + d) * rcf) / ((1 - ph) * rcs + ph * rcf)
+    return xm, ym, tkm, sxym, sxxm
+
+
+@pytest.mark.parametrize("mode,layout", [(9, "lattice"), (1, "lattice"), (1, "random")])
+def test_rk4_restatement(tc, mode, layout):
+    g, mat, orc = tc
+    rng = np.random.default_rng(60 + mode)
+    if layout == "lattice":
+        # T:5761-5771: unjittered lattice markers plus 10,000 markers at (0, 0)
+        mk = cases.ref_markers(g, mat, seed=61, extra=10_000, jitter=False)
+        n0 = g.start_marknum
+        mk["xm"][n0:] = 0.0
+        mk["ym"][n0:] = 0.0
+    else:
+        mk = cases.ref_markers(g, mat, seed=61, extra=1500)
+    S = (g.Ny1, g.Nx1)
+    vx, vy, vxf, vyf = (cases.rand_grid(rng, S) for _ in range(4))
+    wyx = cases.rand_grid(rng, (g.Ny, g.Nx))
+    tk2 = np.asfortranarray(rng.choice(np.arange(263.0, 310.0, 0.1), S))  # T:5781
+    if layout == "lattice":
+        dtm = 0.9  # T:5760: sub-metre displacements, tkm in 263:0.1:310 (T:5773)
+        mk["tkm"] = rng.choice(np.arange(263.0, 310.0, 0.1), mk["xm"].size)
+    else:
+        dtm = 0.9 * g.dx  # displacements of about one cell
+    args = (mk["xm"], mk["ym"], mk["tm"], mk["tkm"], mk["phim"], mk["sxym"], mk["sxxm"],
+            vx, vy, vxf, vyf, wyx, tk2)
+    out = orc.move_markers_rk4(*args, dtm, mode)
+    ver = rk4_ver(g, mat, *args, dtm, mode, orc)
+    np.testing.assert_allclose(out[0], ver[0], rtol=1e-12)  # T:6087
+    np.testing.assert_allclose(out[1], ver[1], rtol=1e-12)  # T:6092
+    np.testing.assert_allclose(out[2], ver[2], rtol=1e-12)
+    if layout == "lattice":
+        np.testing.assert_array_equal(out[3], ver[3])        # T:6102 (==)
+        np.testing.assert_array_equal(out[4], ver[4])        # T:6107 (==)
+    else:
+        # off the lattice /dx (restatement) and *inv(dx) (E:1457) can pick
+        # neighbouring cells at a boundary: same value up to rounding
+        np.testing.assert_allclose(out[3], ver[3], rtol=1e-14)
+        np.testing.assert_allclose(out[4], ver[4], rtol=1e-14)
+    if mode == 9:
+        # the reference test's own variant (vxf centre value, per-type
+        # rhocpfluidm) at its tolerance; it runs with test_constants.jl,
+        # i.e. mode 9, where compute_rhocpfluidm is the type-1 constant
+        ref = rk4_ver(g, mat, *args, dtm, mode, orc, centre_from_vx=False)
+        np.testing.assert_allclose(out[2], ref[2], rtol=1e-4)    # T:6097
+    # rotation invariant: sxx^2 + sxy^2 preserved
+    np.testing.assert_allclose(out[3] ** 2 + out[4] ** 2, mk["sxym"] ** 2 + mk["sxxm"] ** 2,
+                               rtol=1e-12)
+
+
+def test_rk4_uniform_and_affine_velocity(tc):
+    g, mat, orc = tc
+    rng = np.random.default_rng(62)
+    M = 500
+    xm = 2 * g.dx + rng.random(M) * (g.xsize - 4 * g.dx)
+    ym = 2 * g.dy + rng.random(M) * (g.ysize - 4 * g.dy)
+    tm = np.ones(M, dtype=np.int64)
+    S = (g.Ny1, g.Nx1)
+    z = np.zeros(S, order="F")
+    tk2 = np.full(S, 280.0, order="F")
+    dt = 1.0
+    # uniform velocity: displacement dt*v for solid, no T change
+    vx = np.full(S, 3.0, order="F")
+    vy = np.full(S, -2.0, order="F")
+    out = orc.move_markers_rk4(xm, ym, tm, np.full(M, 280.0), np.full(M, 0.1), np.zeros(M),
+                               np.zeros(M), vx, vy, vx, vy, np.zeros((g.Ny, g.Nx)), tk2, dt, 9)
+    np.testing.assert_allclose(out[0], xm + 3.0, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(out[1], ym - 2.0, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(out[2], 280.0, rtol=1e-15)
+    # affine v = A x + b: bilinear interpolation is exact and the second-
+    # order correction vanishes, so RK4 equals the Taylor polynomial of
+    # exp(hA) to 4th order
+    a = g.axes()
+    A = np.array([[1e-5, 2e-5], [-3e-5, -1e-5]])
+    b = np.array([0.2, -0.1])
+    XV, YV = np.meshgrid(a["xvx"], a["yvx"])
+    vxa = np.asfortranarray(A[0, 0] * XV + A[0, 1] * YV + b[0])
+    XV, YV = np.meshgrid(a["xvy"], a["yvy"])
+    vya = np.asfortranarray(A[1, 0] * XV + A[1, 1] * YV + b[1])
+    h = 50.0
+    out = orc.move_markers_rk4(xm, ym, tm, np.full(M, 280.0), np.full(M, 0.1), np.zeros(M),
+                               np.zeros(M), vxa, vya, vxa, vya, np.zeros((g.Ny, g.Nx)), tk2,
+                               h, 9)
+    hA = h * A
+    P = np.eye(2) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    Q = np.eye(2) + hA / 2 + hA @ hA / 6 + hA @ hA @ hA / 24  # sum (hA)^k/(k+1)!
+    X = np.stack([xm, ym])
+    exp_ = P @ X + (h * Q) @ b[:, None]
+    np.testing.assert_allclose(out[0], exp_[0], rtol=1e-12)
+    np.testing.assert_allclose(out[1], exp_[1], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- assembly
+
+def stokes_dense_ver(g, mat, G, dt):
+    """T:4147-4445 dense restatement of assemble_hydromechanical_lse!"""
+    (ETA, ETAP, GGG, GGGP, SXY0, SXX0, RHOX, RHOY, RHOFX, RHOFY, RX, RY, ETAPHI, BETTAPHI,
+     PHI, gx, gy, pr0, pf0, DMP) = G
+    Nx, Ny, Nx1, Ny1, dx, dy = g.Nx, g.Ny, g.Nx1, g.Ny1, g.dx, g.dy
+    K = mat.Kcont
+    n = Nx1 * Ny1 * 6
+    L = np.zeros((n + 1, n + 1))
+    R = np.zeros(n + 1)
+    bcl, bcr, bct, bcb = mat.bc
+    fcl, fcr, fct, fcb = mat.bcf
+    vxl, vxr, vyt, vyb = mat.boundary_velocities(g.xsize, g.ysize)
+    for j in range(1, Nx1 + 1):
+        for i in range(1, Ny1 + 1):
+            e = lambda A, ii, jj: A[ii - 1, jj - 1]
+            kvx = ((j - 1) * Ny1 + i - 1) * 6 + 1
+            kvy, kpm, kqx, kqy, kpf = kvx + 1, kvx + 2, kvx + 3, kvx + 4, kvx + 5
+            if i == 1 or i == Ny1 or j == 1 or j == Nx or j == Nx1:
+                if j == Nx1:
+                    L[kvx, kvx] = 1; R[kvx] = 0
+                if j == 1:
+                    L[kvx, kvx] = 1; R[kvx] = vxl
+                if j == Nx:
+                    L[kvx, kvx] = 1; R[kvx] = vxr
+                if i == 1 and 1 < j < Nx:
+                    L[kvx, kvx] = 1; L[kvx, kvx + 6] = bct; R[kvx] = 0
+                if i == Ny1 and 1 < j < Nx:
+                    L[kvx, kvx] = 1; L[kvx, kvx - 6] = bcb; R[kvx] = 0
+            else:
+                ETA1 = e(ETA, i - 1, j) * e(GGG, i - 1, j) * dt / (e(GGG, i - 1, j) * dt + e(ETA, i - 1, j))
+                ETA2 = e(ETA, i, j) * e(GGG, i, j) * dt / (e(GGG, i, j) * dt + e(ETA, i, j))
+                ETAP1 = e(ETAP, i, j) * e(GGGP, i, j) * dt / (e(GGGP, i, j) * dt + e(ETAP, i, j))
+                ETAP2 = e(ETAP, i, j + 1) * e(GGGP, i, j + 1) * dt / (e(GGGP, i, j + 1) * dt + e(ETAP, i, j + 1))
+                SXY1 = e(SXY0, i - 1, j) * e(ETA, i - 1, j) / (e(GGG, i - 1, j) * dt + e(ETA, i - 1, j))
+                SXY2 = e(SXY0, i, j) * e(ETA, i, j) / (e(GGG, i, j) * dt + e(ETA, i, j))
+                SXX1 = e(SXX0, i, j) * e(ETAP, i, j) / (e(GGGP, i, j) * dt + e(ETAP, i, j))
+                SXX2 = e(SXX0, i, j + 1) * e(ETAP, i, j + 1) / (e(GGGP, i, j + 1) * dt + e(ETAP, i, j + 1))
+                dRdx = (e(RHOX, i, j + 1) - e(RHOX, i, j - 1)) / 2 / dx
+                dRdy = (e(RHOX, i + 1, j) - e(RHOX, i - 1, j)) / 2 / dy
+                gg = e(gx, i, j)
+                L[kvx, kvx - Ny1 * 6] = ETAP1 / dx ** 2
+                L[kvx, kvx - 6] = ETA1 / dy ** 2
+                L[kvx, kvx] = -(ETAP1 + ETAP2) / dx ** 2 - (ETA1 + ETA2) / dy ** 2 - dRdx * gg * dt
+                L[kvx, kvx + 6] = ETA2 / dy ** 2
+                L[kvx, kvx + Ny1 * 6] = ETAP2 / dx ** 2
+                L[kvx, kvy] = ETAP1 / dx / dy - ETA2 / dx / dy - dRdy * gg * dt / 4
+                L[kvx, kvy + Ny1 * 6] = -ETAP2 / dx / dy + ETA2 / dx / dy - dRdy * gg * dt / 4
+                L[kvx, kvy - 6] = -ETAP1 / dx / dy + ETA1 / dx / dy - dRdy * gg * dt / 4
+                L[kvx, kvy + Ny1 * 6 - 6] = ETAP2 / dx / dy - ETA1 / dx / dy - dRdy * gg * dt / 4
+                L[kvx, kpm] = K / dx
+                L[kvx, kpm + Ny1 * 6] = -K / dx
+                R[kvx] = -e(RHOX, i, j) * gg - (SXY2 - SXY1) / dy - (SXX2 - SXX1) / dx
+            if j == 1 or j == Nx1 or i == 1 or i == Ny or i == Ny1:
+                if i == Ny1:
+                    L[kvy, kvy] = 1; R[kvy] = 0
+                if i == 1:
+                    L[kvy, kvy] = 1; R[kvy] = vyt
+                if i == Ny:
+                    L[kvy, kvy] = 1; R[kvy] = vyb
+                if j == 1 and 1 < i < Ny:
+                    L[kvy, kvy] = 1; L[kvy, kvy + 6 * Ny1] = bcl; R[kvy] = 0
+                if j == Nx1 and 1 < i < Ny:
+                    L[kvy, kvy] = 1; L[kvy, kvy - 6 * Ny1] = bcr; R[kvy] = 0
+            else:
+                ETA1 = e(ETA, i, j - 1) * e(GGG, i, j - 1) * dt / (e(GGG, i, j - 1) * dt + e(ETA, i, j - 1))
+                ETA2 = e(ETA, i, j) * e(GGG, i, j) * dt / (e(GGG, i, j) * dt + e(ETA, i, j))
+                ETAP1 = e(ETAP, i, j) * e(GGGP, i, j) * dt / (e(GGGP, i, j) * dt + e(ETAP, i, j))
+                ETAP2 = e(ETAP, i + 1, j) * e(GGGP, i + 1, j) * dt / (e(GGGP, i + 1, j) * dt + e(ETAP, i + 1, j))
+                SXY1 = e(SXY0, i, j - 1) * e(ETA, i, j - 1) / (e(GGG, i, j - 1) * dt + e(ETA, i, j - 1))
+                SXY2 = e(SXY0, i, j) * e(ETA, i, j) / (e(GGG, i, j) * dt + e(ETA, i, j))
+                SYY1 = -e(SXX0, i, j) * e(ETAP, i, j) / (e(GGGP, i, j) * dt + e(ETAP, i, j))
+                SYY2 = -e(SXX0, i + 1, j) * e(ETAP, i + 1, j) / (e(GGGP, i + 1, j) * dt + e(ETAP, i + 1, j))
+                dRdx = (e(RHOY, i, j + 1) - e(RHOY, i, j - 1)) / 2 / dx
+                dRdy = (e(RHOY, i + 1, j) - e(RHOY, i - 1, j)) / 2 / dy
+                gg = e(gy, i, j)
+                L[kvy, kvy - Ny1 * 6] = ETA1 / dx ** 2
+                L[kvy, kvy - 6] = ETAP1 / dy ** 2
+                L[kvy, kvy] = -(ETAP1 + ETAP2) / dy ** 2 - (ETA1 + ETA2) / dx ** 2 - dRdy * gg * dt
+                L[kvy, kvy + 6] = ETAP2 / dy ** 2
+                L[kvy, kvy + Ny1 * 6] = ETA2 / dx ** 2
+                L[kvy, kvx] = ETAP1 / dx / dy - ETA2 / dx / dy - dRdx * gg * dt / 4
+                L[kvy, kvx + 6] = -ETAP2 / dx / dy + ETA2 / dx / dy - dRdx * gg * dt / 4
+                L[kvy, kvx - Ny1 * 6] = -ETAP1 / dx / dy + ETA1 / dx / dy - dRdx * gg * dt / 4
+                L[kvy, kvx + 6 - Ny1 * 6] = ETAP2 / dx / dy - ETA1 / dx / dy - dRdx * gg * dt / 4
+                L[kvy, kpm] = K / dy
+                L[kvy, kpm + 6] = -K / dy
+                R[kvy] = -e(RHOY, i, j) * gg - (SXY2 - SXY1) / dx - (SYY2 - SYY1) / dy
+            anch = ((i == 2 and 2 <= j <= Nx) or (i == Ny and 2 <= j <= Nx)
+                    or (j == 2 and 2 <= i <= Ny) or (j == Nx and 2 <= i <= Ny))
+            if i == 1 or j == 1 or i == Ny1 or j == Nx1 or anch:
+                L[kpm, kpm] = 1; R[kpm] = 0
+                if anch and not (i == 1 or j == 1 or i == Ny1 or j == Nx1):
+                    L[kpm, kpm] = 1 * K; R[kpm] = mat.psurface
+            else:
+                c = K / (1 - e(PHI, i, j)) * (1 / e(ETAPHI, i, j) + e(BETTAPHI, i, j) / dt)
+                L[kpm, kvx - Ny1 * 6] = -1 / dx; L[kpm, kvx] = 1 / dx
+                L[kpm, kvy - 6] = -1 / dy; L[kpm, kvy] = 1 / dy
+                L[kpm, kpm] = c; L[kpm, kpf] = -c
+                R[kpm] = (e(pr0, i, j) - e(pf0, i, j)) / (1 - e(PHI, i, j)) * e(BETTAPHI, i, j) / dt + e(DMP, i, j)
+            if i == 1 or i == Ny1 or j == 1 or j == Nx or j == Nx1:
+                L[kqx, kqx] = 1; R[kqx] = 0
+                if i == 1 and 1 < j < Nx:
+                    L[kqx, kqx + 6] = fct
+                if i == Ny1 and 1 < j < Nx:
+                    L[kqx, kqx - 6] = fcb
+            else:
+                L[kqx, kqx] = e(RX, i, j); L[kqx, kpf] = -K / dx; L[kqx, kpf + Ny1 * 6] = K / dx
+                R[kqx] = e(RHOFX, i, j) * e(gx, i, j)
+            if j == 1 or j == Nx1 or i == 1 or i == Ny or i == Ny1:
+                L[kqy, kqy] = 1; R[kqy] = 0
+                if j == 1 and 1 < i < Ny:
+                    L[kqy, kqy + 6 * Ny1] = fcl
+                if j == Nx1 and 1 < i < Ny:
+                    L[kqy, kqy - 6 * Ny1] = fcr
+            else:
+                L[kqy, kqy] = e(RY, i, j); L[kqy, kpf] = -K / dy; L[kqy, kpf + 6] = K / dy
+                R[kqy] = e(RHOFY, i, j) * e(gy, i, j)
+            anch2 = ((i == 2 and 2 <= j <= Nx) or (i == Ny and 2 <= j <= Nx)
+                     or (j == 2 and 2 < i < Ny) or (j == Nx and 2 < i < Ny))
+            if i == 1 or j == 1 or i == Ny1 or j == Nx1 or anch:
+                L[kpf, kpf] = 1; R[kpf] = 0
+                if anch2 and not (i == 1 or j == 1 or i == Ny1 or j == Nx1):
+                    L[kpf, kpf] = 1 * K; R[kpf] = mat.psurface
+            else:
+                c = K / (1 - e(PHI, i, j)) * (1 / e(ETAPHI, i, j) + e(BETTAPHI, i, j) / dt)
+                L[kpf, kqx - Ny1 * 6] = -1 / dx; L[kpf, kqx] = 1 / dx
+                L[kpf, kqy - 6] = -1 / dy; L[kpf, kqy] = 1 / dy
+                L[kpf, kpm] = -c; L[kpf, kpf] = c
+                R[kpf] = -(e(pr0, i, j) - e(pf0, i, j)) / (1 - e(PHI, i, j)) * e(BETTAPHI, i, j) / dt
+    return L[1:, 1:], R[1:]
+
+
+def csc_dense(cp, rv, nz, n):
+    D = np.zeros((n, n))
+    for c in range(n):
+        for p in range(cp[c] - 1, cp[c + 1] - 1):
+            D[rv[p] - 1, c] = nz[p]
+    return D
+
+
+def check_csc_structure(cp, rv, nz, n):
+    assert cp[0] == 1 and np.all(np.diff(cp) >= 0) and cp[-1] - 1 == rv.size
+    for c in range(n):
+        r = rv[cp[c] - 1:cp[c + 1] - 1]
+        assert np.all(np.diff(r) > 0)  # rows strictly ascending
+    assert np.all(nz != 0.0)  # ExtendableSparse creates no entry for v == 0
+
+
+def test_assemble_hydromechanical_restatement(tc):
+    g, mat, orc = tc
+    G = cases.stokes_inputs(g)
+    dt = 1e11
+    cp, rv, nz, R = orc.assemble_hydromechanical(G, dt)
+    n = 6 * g.Nx1 * g.Ny1
+    check_csc_structure(cp, rv, nz, n)
+    Lv, Rv = stokes_dense_ver(g, mat, G, dt)
+    D = csc_dense(cp, rv, nz, n)
+    assert np.linalg.norm(D - Lv) <= 1e-9 * np.linalg.norm(Lv)  # T:4447 (rtol 1e-9)
+    assert np.linalg.norm(R - Rv) <= 1e-9 * np.linalg.norm(Rv)  # T:4448
+    np.testing.assert_allclose(D, Lv, rtol=1e-12, atol=1e-300)
+    # pattern: the restatement's nonzeros, up to entries that cancel to
+    # zero in one operation order but not the other
+    diff = (D != 0) != (Lv != 0)
+    if diff.any():
+        assert np.all(np.abs(Lv[diff]) + np.abs(D[diff]) <= 1e-9 * np.abs(Lv).max())
+    assert diff.sum() == 0 or (D != 0).sum() > 0.9 * (Lv != 0).sum()
+
+
+def test_assemble_thermal_restatement(tc):
+    g, mat, orc = tc
+    G = cases.thermal_inputs(g)
+    dt = 0.37
+    cp, rv, nz, R = orc.assemble_thermal(G, dt)
+    n = g.Nx1 * g.Ny1
+    check_csc_structure(cp, rv, nz, n)
+    tk1, RHOCP, KX, KY, HR, HA, HS, DHP = G
+    Lv = np.zeros((n + 1, n + 1))
+    Rv = np.zeros(n + 1)
+    Ny1, Nx1, dx, dy = g.Ny1, g.Nx1, g.dx, g.dy
+    for j in range(1, Nx1 + 1):  # T:5396-5454
+        for i in range(1, Ny1 + 1):
+            gk = (j - 1) * Ny1 + i
+            if i == 1 or i == Ny1 or j == 1 or j == Nx1:
+                Lv[gk, gk] = 1
+                if i == 1 and 1 < j < Nx1:
+                    Lv[gk, gk + 1] = -1
+                if i == Ny1 and 1 < j < Nx1:
+                    Lv[gk, gk - 1] = -1
+                if j == 1:
+                    Lv[gk, gk + Ny1] = -1
+                if j == Nx1:
+                    Lv[gk, gk - Ny1] = -1
+            else:
+                Kx1, Kx2 = KX[i - 1, j - 2], KX[i - 1, j - 1]
+                Ky1, Ky2 = KY[i - 2, j - 1], KY[i - 1, j - 1]
+                Lv[gk, gk - Ny1] = -Kx1 / dx ** 2
+                Lv[gk, gk - 1] = -Ky1 / dy ** 2
+                Lv[gk, gk] = RHOCP[i - 1, j - 1] / dt + (Kx1 + Kx2) / dx ** 2 + (Ky1 + Ky2) / dy ** 2
+                Lv[gk, gk + 1] = -Ky2 / dy ** 2
+                Lv[gk, gk + Ny1] = -Kx2 / dx ** 2
+                Rv[gk] = (RHOCP[i - 1, j - 1] / dt * tk1[i - 1, j - 1] + HR[i - 1, j - 1]
+                          + HA[i - 1, j - 1] + HS[i - 1, j - 1] + DHP[i - 1, j - 1])
+    D = csc_dense(cp, rv, nz, n)
+    np.testing.assert_allclose(D, Lv[1:, 1:], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(R, Rv[1:], rtol=1e-12, atol=0)
+    assert ((D != 0) == (Lv[1:, 1:] != 0)).all()
+
+
+# -------------------------------------------------------------------- sort
+
+def test_tile_sort_properties(tc):
+    g, mat, orc = tc
+    mk = cases.ref_markers(g, mat, seed=70, extra=4000)
+    perm, key, start = orc.tile_sort(mk["xm"], mk["ym"])
+    M = perm.size
+    assert np.array_equal(np.sort(perm), np.arange(M))      # a permutation
+    k = key[perm]
+    assert np.all(np.diff(k) >= 0)                           # grouped by tile
+    same = np.diff(k) == 0
+    assert np.all(np.diff(perm)[same] > 0)                   # stable
+    assert np.array_equal(np.searchsorted(k, np.arange(start.size - 1)), start[:-1])
+    # brute force: stable argsort
+    assert np.array_equal(perm, np.argsort(key, kind="stable"))
+    # tile of every marker = basic cell // 6
+    for m in range(0, M, 11):
+        i, j = orc.fix(mk["xm"][m], mk["ym"][m], "basic")
+        nty = (g.Ny - 2) // 6 + 1
+        assert key[m] == ((j - 1) // 6) * nty + (i - 1) // 6
+
+
+# --------------------------------------------------------- golden fixtures
+
+def test_golden_fixtures():
+    """tests/golden/oracle_small.npz holds oracle outputs on seeded inputs
+    (tests/golden/make_golden.py); any change in the oracle shows here."""
+    path = os.path.join(GOLDEN, "oracle_small.npz")
+    gold = np.load(path, allow_pickle=False)
+    import make_golden  # noqa: E402  (tests/golden is on sys.path)
+    now = make_golden.compute()
+    for k in gold.files:
+        np.testing.assert_array_equal(now[k], gold[k], err_msg=k)
