@@ -4,13 +4,20 @@
 // ExtendableSparse's updateindex!(L, +, v, row, col) creates no entry when
 // v == 0, so the pattern depends on the values.  Every value is computed by
 // the same expression as the reference (no fma contraction), which makes
-// the zero tests, the pattern and the stored values bit-exact:
-//   1. count: one thread per P cell (i, j) evaluates its rows; nonzero
-//      entries bump their column's counter (integer atomics), R is written
-//   2. scan: column pointers
-//   3. fill: the same evaluation reserves a slot per entry in its column
-//   4. sort: each column (at most a few dozen entries) is ordered by row,
-//      which removes the arrival order of step 3 (CSC rows ascending).
+// the zero tests, the pattern and the stored values bit-exact.
+//
+// Every row of P cell (i', j') references columns of cells (i'+di, j'+dj)
+// with |di|, |dj| <= 1 only.  Seen from a column of cell (i, j), its
+// candidate rows are therefore the 9 x NU rows of the neighbouring cells,
+// and ordering them by (dj, di, unknown) orders them by row index.  So:
+//   1. mask : one thread per P cell evaluates its rows; each nonzero entry
+//             sets bit ((dj+1)*3 + di+1)*NU + unknown in its column's 64-bit
+//             mask (atomicOr: the result does not depend on order); R is
+//             written here
+//   2. scan : column counts = popcount(mask), column pointers
+//   3. fill : the same evaluation places each entry at colptr + popcount of
+//             the lower bits of its column's mask: rows come out ascending
+//             in every column, with no sort and independent of thread order.
 #include "ereb_internal.h"
 
 namespace ereb {
@@ -26,25 +33,43 @@ struct AsmArgs {
     // thermal
     const double *tk1, *RHOCP, *KX, *KY, *HR, *HA, *HS, *DHP;
     double* R;
-    int32_t *colcnt, *colptr, *cursor, *rowval;
+    int32_t *colcnt, *colptr, *rowval;
+    uint64_t* mask;
     double* nzval;
 };
 
-struct CountSink {
-    int32_t* colcnt;
+// bit of row `row` among the candidate rows of column `col` (1-based),
+// NU unknowns per cell, Ny1 cells per grid column
+template <int NU>
+__device__ __forceinline__ int cand_bit(int64_t row, int64_t col, int Ny1) {
+    const int64_t cr = (row - 1) / NU, cc = (col - 1) / NU;
+    const int di = (int)(cr % Ny1) - (int)(cc % Ny1);
+    const int dj = (int)(cr / Ny1) - (int)(cc / Ny1);
+    return ((dj + 1) * 3 + (di + 1)) * NU + (int)((row - 1) % NU);
+}
+
+template <int NU>
+struct MaskSink {
+    uint64_t* mask;
+    int Ny1;
     __device__ void operator()(double v, int64_t row, int64_t col) const {
-        if (v != 0.0) atomicAdd(&colcnt[col - 1], 1);
+        if (v != 0.0)
+            atomicOr((unsigned long long*)&mask[col - 1],
+                     1ull << cand_bit<NU>(row, col, Ny1));
     }
 };
 
+template <int NU>
 struct FillSink {
     const int32_t* colptr;
-    int32_t* cursor;
+    const uint64_t* mask;
     int32_t* rowval;
     double* nzval;
+    int Ny1;
     __device__ void operator()(double v, int64_t row, int64_t col) const {
         if (v != 0.0) {
-            const int32_t p = colptr[col - 1] + atomicAdd(&cursor[col - 1], 1);
+            const int b = cand_bit<NU>(row, col, Ny1);
+            const int32_t p = colptr[col - 1] + (int32_t)__popcll(mask[col - 1] & ((1ull << b) - 1));
             rowval[p] = (int32_t)(row - 1);
             nzval[p] = v;
         }
@@ -247,15 +272,25 @@ __device__ __forceinline__ void thermal_cell(const AsmArgs& A, int i, int j, con
 }
 
 template <int SYS>
-__global__ __launch_bounds__(256) void k_asm_count(AsmArgs A) {
+__global__ __launch_bounds__(256) void k_asm_mask(AsmArgs A) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.Nx1 * A.Ny1) return;
     const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
-    CountSink s{A.colcnt};
     if (SYS == 0)
-        stokes_cell(A, i, j, s, true);
+        stokes_cell(A, i, j, MaskSink<6>{A.mask, A.Ny1}, true);
     else
-        thermal_cell(A, i, j, s, true);
+        thermal_cell(A, i, j, MaskSink<1>{A.mask, A.Ny1}, true);
+}
+
+__global__ __launch_bounds__(256) void k_mask_count(const uint64_t* __restrict__ mask, int64_t n,
+                                                    int32_t* __restrict__ cnt) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) cnt[c] = (int32_t)__popcll(mask[c]);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) p[c] = 0;
 }
 
 template <int SYS>
@@ -263,31 +298,10 @@ __global__ __launch_bounds__(256) void k_asm_fill(AsmArgs A) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.Nx1 * A.Ny1) return;
     const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
-    FillSink s{A.colptr, A.cursor, A.rowval, A.nzval};
     if (SYS == 0)
-        stokes_cell(A, i, j, s, false);
+        stokes_cell(A, i, j, FillSink<6>{A.colptr, A.mask, A.rowval, A.nzval, A.Ny1}, false);
     else
-        thermal_cell(A, i, j, s, false);
-}
-
-__global__ __launch_bounds__(256) void k_sort_columns(const int32_t* __restrict__ colptr, int64_t n,
-                                                      int32_t* __restrict__ rowval,
-                                                      double* __restrict__ nzval) {
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n) return;
-    const int32_t a = colptr[col], e = colptr[col + 1];
-    for (int32_t p = a + 1; p < e; ++p) {
-        const int32_t r = rowval[p];
-        const double v = nzval[p];
-        int32_t q = p - 1;
-        while (q >= a && rowval[q] > r) {
-            rowval[q + 1] = rowval[q];
-            nzval[q + 1] = nzval[q];
-            --q;
-        }
-        rowval[q + 1] = r;
-        nzval[q + 1] = v;
-    }
+        thermal_cell(A, i, j, FillSink<1>{A.colptr, A.mask, A.rowval, A.nzval, A.Ny1}, false);
 }
 
 static AsmArgs asm_args(ereb_ctx* c, double dt, Sys& s) {
@@ -319,7 +333,7 @@ static AsmArgs asm_args(ereb_ctx* c, double dt, Sys& s) {
     A.R = s.rhs;
     A.colcnt = s.colcnt;
     A.colptr = s.colptr;
-    A.cursor = s.cursor;
+    A.mask = s.mask;
     A.rowval = s.rowval;
     A.nzval = s.nzval;
     return A;
@@ -337,14 +351,13 @@ static int assemble(ereb_ctx* c, double dt) {
     AsmArgs A = asm_args(c, dt, s);
     hipEvent_t ev;
     prof_begin(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, &ev);
-    if ((rc = fill_i32(c, s.colcnt, 0, n))) return rc;
-    if ((rc = fill_i32(c, s.cursor, 0, n))) return rc;
+    const unsigned ncol = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(ncol), dim3(256), 0, c->stream, s.mask, n);
     const unsigned nb = (unsigned)((ncell + 255) / 256);
-    hipLaunchKernelGGL(k_asm_count<SYS>, dim3(nb), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_asm_mask<SYS>, dim3(nb), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_mask_count, dim3(ncol), dim3(256), 0, c->stream, s.mask, n, s.colcnt);
     if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
     hipLaunchKernelGGL(k_asm_fill<SYS>, dim3(nb), dim3(256), 0, c->stream, A);
-    hipLaunchKernelGGL(k_sort_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       s.colptr, n, s.rowval, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;

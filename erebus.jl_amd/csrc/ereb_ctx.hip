@@ -55,14 +55,14 @@ int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap) {
     if (s.n == n && s.cap >= cap) return EREB_OK;
     hipFree(s.colcnt);
     hipFree(s.colptr);
-    hipFree(s.cursor);
+    hipFree(s.mask);
     hipFree(s.rowval);
     hipFree(s.nzval);
     hipFree(s.rhs);
     s = Sys();
     EREB_HIP(c, hipMalloc(&s.colcnt, sizeof(int32_t) * n));
     EREB_HIP(c, hipMalloc(&s.colptr, sizeof(int32_t) * (n + 1)));
-    EREB_HIP(c, hipMalloc(&s.cursor, sizeof(int32_t) * n));
+    EREB_HIP(c, hipMalloc(&s.mask, sizeof(uint64_t) * n));
     EREB_HIP(c, hipMalloc(&s.rowval, sizeof(int32_t) * cap));
     EREB_HIP(c, hipMalloc(&s.nzval, sizeof(double) * cap));
     EREB_HIP(c, hipMalloc(&s.rhs, sizeof(double) * n));
@@ -251,15 +251,16 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     c->nty = (int)((Ny - 1 + TILE - 1) / TILE);
     c->ntx = (int)((Nx - 1 + TILE - 1) / TILE);
     c->ntiles = c->ntx * c->nty;
-    CK(hipMalloc(&c->d_tcount, sizeof(int32_t) * c->ntiles));
-    CK(hipMalloc(&c->d_tstart, sizeof(int32_t) * (c->ntiles + 1)));
-    CK(hipMalloc(&c->d_tcursor, sizeof(int32_t) * c->ntiles));
-    CK(hipMalloc(&c->d_big, sizeof(int32_t) * (c->ntiles + 1)));
+    c->nkeys = (int64_t)c->ntiles * NFINE;
+    CK(hipMalloc(&c->d_fcount, sizeof(int32_t) * c->nkeys));
+    CK(hipMalloc(&c->d_fstart, sizeof(int32_t) * (c->nkeys + 1)));
+    CK(hipMalloc(&c->d_fcursor, sizeof(int32_t) * c->nkeys));
+    CK(hipMalloc(&c->d_big, sizeof(int32_t) * (c->nkeys + 1)));
     CK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
     CK(hipMemset(c->d_flags, 0, sizeof(int32_t) * 4));
     CK(hipMalloc(&c->d_part, sizeof(double) * (size_t)c->ntiles * NSUM * 64));
 #undef CK
-    rc = ensure_scan(c, std::max<int64_t>(6 * Nx1 * Ny1, c->ntiles));
+    rc = ensure_scan(c, std::max<int64_t>(6 * Nx1 * Ny1, c->nkeys));
     if (rc) return bail(rc);
     *out = c;
     return EREB_OK;
@@ -280,9 +281,9 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_key);
     hipFree(c->d_perm);
     hipFree(c->d_tmp);
-    hipFree(c->d_tcount);
-    hipFree(c->d_tstart);
-    hipFree(c->d_tcursor);
+    hipFree(c->d_fcount);
+    hipFree(c->d_fstart);
+    hipFree(c->d_fcursor);
     hipFree(c->d_big);
     hipFree(c->d_flags);
     hipFree(c->d_part);
@@ -290,7 +291,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (auto& s : c->sys) {
         hipFree(s.colcnt);
         hipFree(s.colptr);
-        hipFree(s.cursor);
+        hipFree(s.mask);
         hipFree(s.rowval);
         hipFree(s.nzval);
         hipFree(s.rhs);
@@ -421,7 +422,7 @@ int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrflui
     int rc = valid_mode(c, mode);
     if (rc) return rc;
     if ((rc = reserve(c, std::max<int64_t>(c->marknum, 1)))) return rc;
-    if ((rc = launch_tile_sort(c))) return rc;
+    if ((rc = launch_fine_sort(c))) return rc;
     if ((rc = launch_scatter(c, hrsolidm, hrfluidm, (int)mode, true))) return rc;
     return launch_combine(c);
 }
@@ -508,15 +509,15 @@ int ereb_csc_get(ereb_ctx* c, int which, int64_t* colptr, int64_t* rowval, doubl
     return EREB_OK;
 }
 
-int ereb_sort_get(ereb_ctx* c, int32_t* perm, int32_t* tile_key, int64_t* ntiles) {
+int ereb_sort_get(ereb_ctx* c, int32_t* perm, int32_t* fine_key, int64_t* nkeys) {
     if (!c) return EREB_ERR_ARG;
     if (!c->have_sort) return fail(c, EREB_ERR_STATE, "no scatter has run");
     EREB_HIP(c, hipStreamSynchronize(c->stream));
     if (perm && c->marknum)
         EREB_HIP(c, hipMemcpy(perm, c->d_perm, sizeof(int32_t) * c->marknum, hipMemcpyDeviceToHost));
-    if (tile_key && c->marknum)
-        EREB_HIP(c, hipMemcpy(tile_key, c->d_key, sizeof(int32_t) * c->marknum, hipMemcpyDeviceToHost));
-    if (ntiles) *ntiles = c->ntiles;
+    if (fine_key && c->marknum)
+        EREB_HIP(c, hipMemcpy(fine_key, c->d_key, sizeof(int32_t) * c->marknum, hipMemcpyDeviceToHost));
+    if (nkeys) *nkeys = c->nkeys;
     return EREB_OK;
 }
 

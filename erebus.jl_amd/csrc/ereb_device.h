@@ -66,6 +66,12 @@ __device__ __forceinline__ double gdot4(const double* __restrict__ gr, int rows,
 
 // ---- marker properties: compute_marker_properties! (E:700-750) ----------
 
+// per-type table entry for t in 1..3 without a dynamic index (a dynamic
+// index into a by-value kernel argument forces a copy to scratch memory)
+__device__ __forceinline__ double sel3(const double (&a)[3], int t) {
+    return t == 1 ? a[0] : (t == 2 ? a[1] : a[2]);
+}
+
 __device__ __forceinline__ double jl_max(double a, double b) {
     if (a != a || b != b) return a + b;
     return a > b ? a : (b > a ? b : (signbit(a) ? b : a));
@@ -122,8 +128,8 @@ struct Props {
 // One marker of compute_marker_properties!.  For air (t == 3) the stored
 // rhototal, rhocptotal, etatotal, hrtotal, ktotal pass through (E:738-741).
 __device__ __forceinline__ Props marker_props(const ereb_materials& M, int t, double T,
-                                              double phi, double XW, const double* hrs,
-                                              const double* hrf, int mode, Props stored) {
+                                              double phi, double XW, const double (&hrs)[3],
+                                              const double (&hrf)[3], int mode, Props stored) {
     Props p = stored;
     double etafluidcur;
     if (t < 3) {
@@ -131,17 +137,17 @@ __device__ __forceinline__ Props marker_props(const ereb_materials& M, int t, do
         double rhosolidm0 = (M.MD + M.MH2O * XW) / (M.VDs * XD + M.VWs * XW);
         double rhofluidm0 = T > M.tmfluidphase ? M.rhoH2Of : M.rhoH2Ofi;
         p.rhototal = total(rhosolidm0, rhofluidm0, phi);
-        p.rhocptotal = total(M.rhocpsolidm[t - 1], rhocpfluidm(M, T, mode), phi);
-        double es = T > M.tmsolidphase ? M.etasolidmm[t - 1] : M.etasolidm[t - 1];
-        etafluidcur = T > M.tmfluidphase ? M.etafluidmm[t - 1] : M.etafluidm[t - 1];
+        p.rhocptotal = total(sel3(M.rhocpsolidm, t), rhocpfluidm(M, T, mode), phi);
+        double es = T > M.tmsolidphase ? sel3(M.etasolidmm, t) : sel3(M.etasolidm, t);
+        etafluidcur = T > M.tmfluidphase ? sel3(M.etafluidmm, t) : sel3(M.etafluidm, t);
         p.etatotal = jl_max(jl_max(M.etamin, es), etafluidcur);
-        p.hrtotal = total(hrs[t - 1], hrf[t - 1], phi);
+        p.hrtotal = total(sel3(hrs, t), sel3(hrf, t), phi);
         p.ktotal = ktotal(ksolidm(M, T, mode), kfluidm(M, T, mode), phi);
     } else {
-        etafluidcur = M.etafluidm[t - 1];
+        etafluidcur = M.etafluidm[2];
     }
     p.tkrhocp = T * p.rhocptotal;
-    p.etafkphi = etaf_inv_kphi(M.phim0, M.kphim0[t - 1], phi, etafluidcur);
+    p.etafkphi = etaf_inv_kphi(M.phim0, sel3(M.kphim0, t), phi, etafluidcur);
     return p;
 }
 

@@ -18,12 +18,13 @@ constexpr int TILE = 6;
 constexpr int WIN = TILE + 2;
 constexpr int NSUM = 30;  // SUM grids, order of reset_interpolated_properties!
 constexpr int FINE = 2 * TILE;  // fine cells per tile side
+constexpr int NFINE = FINE * FINE;  // fine cells (sort keys) per tile
 
 struct Sys {
     int64_t n = 0, cap = 0;
     int32_t* colcnt = nullptr;  // n
     int32_t* colptr = nullptr;  // n + 1, 0-based
-    int32_t* cursor = nullptr;  // n
+    uint64_t* mask = nullptr;   // n: bit per candidate row of each column
     int32_t* rowval = nullptr;  // cap, 0-based
     double* nzval = nullptr;    // cap
     double* rhs = nullptr;      // n
@@ -54,16 +55,17 @@ struct ereb_ctx {
     int64_t gnumel[EREB_G_NFIELDS] = {};
     uint8_t* d_yny = nullptr;
 
-    // tile sort
+    // fine-key sort
     int ntx = 0, nty = 0, ntiles = 0;
+    int64_t nkeys = 0;             // ntiles * NFINE
     uint32_t* d_key = nullptr;     // cap
     uint32_t* d_perm = nullptr;    // cap
-    uint32_t* d_tmp = nullptr;     // pow2 scratch for oversize tiles
+    uint32_t* d_tmp = nullptr;     // 2*cap scratch for oversize key segments
     int64_t tmp_cap = 0;
-    int32_t* d_tcount = nullptr;   // ntiles
-    int32_t* d_tstart = nullptr;   // ntiles + 1
-    int32_t* d_tcursor = nullptr;  // ntiles
-    int32_t* d_big = nullptr;      // ntiles + 1: [0] = count, then tile ids
+    int32_t* d_fcount = nullptr;   // nkeys
+    int32_t* d_fstart = nullptr;   // nkeys + 1
+    int32_t* d_fcursor = nullptr;  // nkeys
+    int32_t* d_big = nullptr;      // nkeys + 1: [0] = count, then key ids
     int32_t* d_flags = nullptr;    // [0] domain error
     double* d_part = nullptr;      // ntiles * NSUM * 64
     int32_t* d_scan = nullptr;     // scan block sums
@@ -111,8 +113,8 @@ int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n);
 int fill_i32(ereb_ctx* c, int32_t* p, int32_t v, int64_t n);
 int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);
 
-// ereb_sort.hip: stable counting sort of marker indices by tile key
-int launch_tile_sort(ereb_ctx* c);
+// ereb_sort.hip: stable counting sort of marker indices by fine key
+int launch_fine_sort(ereb_ctx* c);
 
 // ereb_scatter.hip
 int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode,

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const double d = tkf0 - tks0;
     const int tt = min(max(A.tm[m], 1), 3);  // set validates 1..3
     const double T = A.tkm[m], ph = A.phim[m];
-    const double rcs = A.M.rhocpsolidm[tt - 1];
+    const double rcs = sel3(A.M.rhocpsolidm, tt);
     const double rcf = rhocpfluidm(A.M, T, A.mode);
     A.tkm[m] = ((1.0 - ph) * T * rcs + ph * (T + d) * rcf) / ((1.0 - ph) * rcs + ph * rcf);
 }

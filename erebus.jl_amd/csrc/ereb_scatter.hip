@@ -3,30 +3,42 @@
 // marker->node scatter onto the basic, Vx, Vy and P grids
 // (marker_to_{basic,vx,vy,p}_nodes!, E:1605-1863, loop E:6419-6517).
 //
-// Layout and algorithm (one workgroup of 4 wavefronts per tile of
-// TILE x TILE basic cells, markers taken in tile-sorted order):
+// Geometry of the reduction:
 //   - A marker's four stencils follow from two cells: basic (ib, jb) and
-//     P (ip, jp); Vx uses (ip, jb), Vy uses (ib, jp) because the staggered
-//     axes coincide (C:40-64) and so do the clamps (C:66-101).  The pair
-//     (ip-ib, jp-jb) is in {0,1}^2, so every marker of a tile lands in a
-//     (TILE+2)^2 = 64-node window of each grid, and its "fine cell"
-//     (2*(ib-i0) + ip-ib, 2*(jb-j0) + jp-jb) fixes all 16 target nodes.
-//   - Chunks of CH markers are staged in LDS (18 property values and the
-//     four fractions dxmj/dx, dymi/dy of the basic and P stencils), then
-//     grouped by fine cell with a stable in-LDS counting sort.
-//   - Wavefront g owns grid g; lane (r, c) owns window node (r, c) and sums,
-//     in a fixed order (chunk, fine column, fine row, marker), the products
-//     property*weight of the 16 fine cells around it.  No atomics: each
-//     window sum has one owner, so results are identical run to run.
-//   - Tile windows overlap by two node rows/columns; k_combine adds the at
-//     most 2 x 2 window partials of every node in a fixed tile order.
+//     P (ip, jp); Vx uses (ip, jb), Vy uses (ib, jp), because the staggered
+//     axes (C:40-64) and clamps (C:66-101) coincide that way.  The pair
+//     (ip-ib, jp-jb) is in {0,1}^2, so a marker's "fine cell" (a quarter of
+//     its basic cell) fixes all 16 nodes it touches, and every marker of a
+//     tile of TILE x TILE basic cells lands in a (TILE+2)^2 = 64-node window
+//     of each grid.
+//   - Markers arrive sorted by fine key (ereb_sort.hip): a tile's markers are
+//     contiguous and grouped by fine cell in ascending marker index.
+//
+// One workgroup (4 wavefronts) per tile, streaming the tile in chunks of at
+// most CH markers and NFMAX fine cells:
+//   1. load   : one thread per marker stages the 18 property values and the
+//               fractions dxmj/dx, dymi/dy of the basic and P stencils in
+//               LDS (the 7 marker properties are computed here and written
+//               back to HBM);
+//   2. tasks  : wavefront g owns grid g; a task is (fine cell, corner): the
+//               sum over the fine cell's markers, in order, of
+//               property * weight for that corner (and of the weight);
+//   3. combine: lane (r, c) of wavefront g owns window node (r, c) of grid g
+//               and adds the partials of the at most 16 fine cells around it
+//               in a fixed (chunk, fine column, fine row) order.
+// No atomics: every sum has exactly one owner and a fixed order, so results
+// are identical run to run.  Tile windows overlap by two node rows/columns;
+// k_combine adds the at most 2 x 2 window partials of each node in a fixed
+// tile order.
 #include "ereb_internal.h"
 
 namespace ereb {
 
 constexpr int CH = 256;     // markers per LDS chunk
-constexpr int NLF = 18;     // property values staged per marker
-constexpr int NFINE = FINE * FINE;
+constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
+constexpr int NFMAX = 32;    // fine cells per chunk
+constexpr int NLF = 18;      // property values staged per marker
+constexpr int NPART = 120;   // partials per fine cell: 4 corners x (8 + 6 + 6 + 10)
 
 // staged property indices
 enum {
@@ -41,7 +53,7 @@ struct ScatterArgs {
     double hrs[3], hrf[3];
     int mode, write_props, nty;
     const uint32_t* perm;
-    const int32_t* tstart;
+    const int32_t* fstart;
     const double *xm, *ym;
     const int32_t* tm;
     const double *tkm, *phim, *XW, *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas,
@@ -50,11 +62,11 @@ struct ScatterArgs {
     double* part;
 };
 
-// per-grid SUM slot offset and staged-property list (SUM order of
-// reset_interpolated_properties!, E:968-999)
+// per-grid SUM slot offset, partial offset and staged-property list (SUM
+// order of reset_interpolated_properties!, E:968-999)
 template <int G> struct GridSpec;
 template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
-    static constexpr int slot0 = 0, nf = 7;
+    static constexpr int slot0 = 0, poff = 0, nf = 7;
     static constexpr bool prow = false, pcol = false;
     __device__ static constexpr int f(int q) {
         return q == 0 ? L_ETATOTAL : q == 1 ? L_ETAVP : q == 2 ? L_INVGGG : q == 3 ? L_SXY
@@ -62,17 +74,17 @@ template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
     }
 };
 template <> struct GridSpec<1> {  // Vx: RHOX RHOFX KX PHIX RX (+WTX)
-    static constexpr int slot0 = 8, nf = 5;
+    static constexpr int slot0 = 8, poff = 32, nf = 5;
     static constexpr bool prow = true, pcol = false;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<2> {  // Vy: RHOY RHOFY KY PHIY RY (+WTY)
-    static constexpr int slot0 = 14, nf = 5;
+    static constexpr int slot0 = 14, poff = 56, nf = 5;
     static constexpr bool prow = false, pcol = true;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<3> {  // P: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PHI (+WTP)
-    static constexpr int slot0 = 20, nf = 9;
+    static constexpr int slot0 = 20, poff = 80, nf = 9;
     static constexpr bool prow = true, pcol = true;
     __device__ static constexpr int f(int q) {
         return q == 0 ? L_RHOTOTAL : q == 1 ? L_RHOCP : q == 2 ? L_ALPHAS : q == 3 ? L_ALPHAF
@@ -82,37 +94,56 @@ template <> struct GridSpec<3> {  // P: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PH
 };
 
 struct ScatterLDS {
-    double sf[NLF][CH];
-    double sfr[4][CH];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
-    int32_t fstart[NFINE + 1];
-    int32_t wbase[4][NFINE];
-    int16_t fine[CH];
-    int16_t order[CH];
+    double sf[NLF][CHP];
+    double sfr[4][CHP];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
+    double part[NFMAX][NPART];
+    int32_t fs[NFINE + 1];
 };
 
+// tasks of grid G: (fine cell fl of the chunk, corner) for this wavefront
 template <int G>
-__device__ __forceinline__ void accumulate(const ScatterLDS& L, double* acc, int r, int c) {
+__device__ __forceinline__ void tasks(ScatterLDS& L, int lane, int f_lo, int nf, int c0,
+                                      int cend) {
+    using S = GridSpec<G>;
+    const double* fa = S::pcol ? L.sfr[2] : L.sfr[0];
+    const double* fb = S::prow ? L.sfr[3] : L.sfr[1];
+    for (int task = lane; task < nf * 4; task += 64) {
+        const int fl = task >> 2, corner = task & 3;
+        const int ci = corner & 1, cj = corner >> 1;  // weights order (i,j),(i+1,j),(i,j+1),(i+1,j+1)
+        const int f = f_lo + fl;
+        const int k0 = max(L.fs[f], c0) - c0, k1 = min(L.fs[f + 1], cend) - c0;
+        double acc[S::nf + 1];
+#pragma unroll
+        for (int q = 0; q <= S::nf; ++q) acc[q] = 0.0;
+        for (int s = k0; s < k1; ++s) {
+            const double a = fa[s], b = fb[s];
+            const double w = (ci ? b : 1.0 - b) * (cj ? a : 1.0 - a);
+#pragma unroll
+            for (int q = 0; q < S::nf; ++q) acc[q] += L.sf[S::f(q)][s] * w;
+            acc[S::nf] += w;
+        }
+        double* p = &L.part[fl][S::poff + corner * (S::nf + 1)];
+#pragma unroll
+        for (int q = 0; q <= S::nf; ++q) p[q] = acc[q];
+    }
+}
+
+// node (r, c) of grid G adds the partials of its fine cells in this chunk
+template <int G>
+__device__ __forceinline__ void combine(const ScatterLDS& L, double* acc, int r, int c, int f_lo,
+                                        int f_hi) {
     using S = GridSpec<G>;
     const int fi_lo = S::prow ? 2 * r - 3 : 2 * r - 2;
     const int fj_lo = S::pcol ? 2 * c - 3 : 2 * c - 2;
-    const double* fa = S::pcol ? L.sfr[2] : L.sfr[0];
-    const double* fb = S::prow ? L.sfr[3] : L.sfr[1];
     for (int fj = max(0, fj_lo); fj <= min(FINE - 1, fj_lo + 3); ++fj) {
         const int cj = c - (S::pcol ? (fj + 1) >> 1 : fj >> 1);
         for (int fi = max(0, fi_lo); fi <= min(FINE - 1, fi_lo + 3); ++fi) {
+            const int f = fj * FINE + fi;
+            if (f < f_lo || f > f_hi) continue;
             const int ci = r - (S::prow ? (fi + 1) >> 1 : fi >> 1);
-            const int fc = fj * FINE + fi;
-            const int k1 = L.fstart[fc + 1];
-            for (int k = L.fstart[fc]; k < k1; ++k) {
-                const int s = L.order[k];
-                const double a = fa[s], b = fb[s];
-                const double wy = ci ? b : 1.0 - b;
-                const double wx = cj ? a : 1.0 - a;
-                const double w = wy * wx;
+            const double* p = &L.part[f - f_lo][S::poff + (ci + 2 * cj) * (S::nf + 1)];
 #pragma unroll
-                for (int q = 0; q < S::nf; ++q) acc[q] += L.sf[S::f(q)][s] * w;
-                acc[S::nf] += w;
-            }
+            for (int q = 0; q <= S::nf; ++q) acc[q] += p[q];
         }
     }
 }
@@ -125,28 +156,33 @@ __device__ __forceinline__ void write_part(double* part, int tile, const double*
     for (int q = 0; q <= S::nf; ++q) p[(size_t)q * 64] = acc[q];
 }
 
-__global__ __launch_bounds__(256) void k_scatter_fp64(ScatterArgs A) {
+__global__ __launch_bounds__(256, 2) void k_scatter_fp64(ScatterArgs A) {
     __shared__ ScatterLDS L;
     const int tile = blockIdx.x;
-    const int ty = tile % A.nty, tx = tile / A.nty;
-    const int i0 = ty * TILE, j0 = tx * TILE;  // first basic cell (0-based)
-    const int S0 = A.tstart[tile], E0 = A.tstart[tile + 1];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int r = lane & 7, c = lane >> 3;
+    for (int i = t; i <= NFINE; i += 256) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
+    __syncthreads();
+    const int S0 = L.fs[0], E0 = L.fs[NFINE];
     double acc[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) acc[q] = 0.0;
 
-    for (int c0 = S0; c0 < E0; c0 += CH) {
-        const int n = min(CH, E0 - c0);
-        int f = -1;
-        if (t < n) {
+    int f_lo = 0;
+    for (int c0 = S0; c0 < E0;) {
+        // chunk [c0, cend): at most CH markers and NFMAX fine cells
+        while (L.fs[f_lo + 1] <= c0) ++f_lo;
+        int cend = min(c0 + CH, E0);
+        if (f_lo + NFMAX < NFINE) cend = min(cend, L.fs[f_lo + NFMAX]);
+        if (cend <= c0) cend = min(c0 + CH, E0);  // a single fine cell longer than CH
+        int f_hi = f_lo;
+        while (f_hi + 1 < NFINE && L.fs[f_hi + 1] < cend) ++f_hi;
+        const int nf = f_hi - f_lo + 1;
+        // ---- 1. load ------------------------------------------------------
+        if (t < cend - c0) {
             const uint32_t m = A.perm[c0 + t];
             const double x = A.xm[m], y = A.ym[m];
             const Cells cl = marker_cells(A.g, x, y);
-            const int fi = 2 * (cl.ib - 1 - i0) + (cl.ip - cl.ib);
-            const int fj = 2 * (cl.jb - 1 - j0) + (cl.jp - cl.jb);
-            if (fi >= 0 && fi < FINE && fj >= 0 && fj < FINE) f = fj * FINE + fi;
             L.sfr[0][t] = (x - A.g.x[cl.jb - 1]) / A.g.dx;
             L.sfr[1][t] = (y - A.g.y[cl.ib - 1]) / A.g.dy;
             L.sfr[2][t] = (x - A.g.xp[cl.jp - 1]) / A.g.dx;
@@ -192,69 +228,24 @@ __global__ __launch_bounds__(256) void k_scatter_fp64(ScatterArgs A) {
             L.sf[L_HR][t] = p.hrtotal;
             L.sf[L_TKRHOCP][t] = p.tkrhocp;
         }
-        // ---- stable grouping of the chunk by fine cell -----------------
-        for (int i = t; i < 4 * NFINE; i += 256) (&L.wbase[0][0])[i] = 0;
         __syncthreads();
-        int rank = 0;
-        {
-            uint64_t todo = __ballot(f >= 0);
-            while (todo) {
-                const int leader = __ffsll((unsigned long long)todo) - 1;
-                const int lf = __shfl(f, leader);
-                const uint64_t mm = __ballot(f == lf);
-                if (f == lf) rank = (int)__popcll(mm & lanemask_lt());
-                if (lane == leader) L.wbase[wave][lf] = (int)__popcll(mm);
-                todo &= ~mm;
-            }
-        }
-        __syncthreads();
-        // fine-cell totals -> starts (wave 0: 3 fine cells per lane), and
-        // per-wave bases inside each fine cell
-        if (wave == 0) {
-            int tot[3], s = 0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int fc = lane * 3 + k;
-                tot[k] = fc < NFINE ? L.wbase[0][fc] + L.wbase[1][fc] + L.wbase[2][fc] +
-                                          L.wbase[3][fc]
-                                    : 0;
-                s += tot[k];
-            }
-            int incl = s;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
-            int run = incl - s;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int fc = lane * 3 + k;
-                if (fc < NFINE) {
-                    L.fstart[fc] = run;
-                    int b = run;
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) {
-                        const int cw = L.wbase[w][fc];
-                        L.wbase[w][fc] = b;
-                        b += cw;
-                    }
-                }
-                run += tot[k];
-            }
-            if (lane == 63) L.fstart[NFINE] = incl;
-        }
-        __syncthreads();
-        if (f >= 0) L.order[L.wbase[wave][f] + rank] = (int16_t)t;
-        __syncthreads();
-        // ---- node-owned accumulation --------------------------------------
+        // ---- 2. per (fine cell, corner) partial sums, wavefront g = grid g --
         switch (wave) {
-            case 0: accumulate<0>(L, acc, r, c); break;
-            case 1: accumulate<1>(L, acc, r, c); break;
-            case 2: accumulate<2>(L, acc, r, c); break;
-            default: accumulate<3>(L, acc, r, c); break;
+            case 0: tasks<0>(L, lane, f_lo, nf, c0, cend); break;
+            case 1: tasks<1>(L, lane, f_lo, nf, c0, cend); break;
+            case 2: tasks<2>(L, lane, f_lo, nf, c0, cend); break;
+            default: tasks<3>(L, lane, f_lo, nf, c0, cend); break;
         }
         __syncthreads();
+        // ---- 3. node-owned combine -----------------------------------------
+        switch (wave) {
+            case 0: combine<0>(L, acc, r, c, f_lo, f_hi); break;
+            case 1: combine<1>(L, acc, r, c, f_lo, f_hi); break;
+            case 2: combine<2>(L, acc, r, c, f_lo, f_hi); break;
+            default: combine<3>(L, acc, r, c, f_lo, f_hi); break;
+        }
+        __syncthreads();
+        c0 = cend;
     }
     switch (wave) {
         case 0: write_part<0>(A.part, tile, acc, lane); break;
@@ -311,7 +302,7 @@ int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode, 
     A.write_props = write_props ? 1 : 0;
     A.nty = c->nty;
     A.perm = c->d_perm;
-    A.tstart = c->d_tstart;
+    A.fstart = c->d_fstart;
     A.xm = mptr(c, EREB_M_XM);
     A.ym = mptr(c, EREB_M_YM);
     A.tm = tm_ptr(c);

@@ -1,19 +1,22 @@
-// ereb_sort.hip -- stable counting sort of marker indices by scatter tile.
+// ereb_sort.hip -- stable counting sort of marker indices by fine key.
 //
-// key(m) = tile of the basic cell (fix with the basic clamps, E:1456) of
-// marker m; tiles are TILE x TILE basic cells numbered column-major.  The
-// result perm[] lists marker indices grouped by tile in ascending tile
-// order and, inside a tile, in ascending marker index (a stable sort).
+// The fine key of a marker is its "fine cell": the basic cell (ib, jb)
+// (fix with the basic clamps, E:1456) refined by the P cell offsets
+// (ip-ib, jp-jb) in {0,1}^2, numbered tile-major: tiles of TILE x TILE
+// basic cells in column-major order, then FINE x FINE fine cells inside a
+// tile, column-major.  All markers of one fine cell share the same four
+// stencils on every grid.  The result perm[] lists marker indices in
+// ascending key and, inside a key, ascending marker index -- exactly a
+// stable counting sort, which makes every later summation order canonical.
 //
-//   1. k_tile_count : key per marker, per-tile counts (wavefront-aggregated
-//                     integer atomics: counts do not depend on order)
-//   2. scan         : tile_start = exclusive scan of the counts
-//   3. k_tile_place : wavefront-aggregated slot reservation per tile
-//   4. k_segsort_*  : each tile segment is sorted ascending in LDS (bitonic),
-//                     which removes the arrival order of step 3 and makes the
-//                     result deterministic.  Segments longer than 4096
-//                     markers go to a 1024-thread variant (LDS up to 16384,
-//                     global-memory bitonic beyond).
+//   1. k_fine_count : key per marker, per-key counts (wavefront-aggregated
+//                     integer atomics: the counts do not depend on order)
+//   2. scan         : key starts
+//   3. k_fine_place : wavefront-aggregated slot reservation per key
+//   4. k_fine_order : each key segment (about 9 markers) is sorted by marker
+//                     index in place, removing the arrival order of step 3;
+//                     segments longer than SEG_THREAD go to k_fine_order_big
+//                     (bitonic in LDS up to 16384, global memory beyond).
 #include "ereb_internal.h"
 
 namespace ereb {
@@ -25,10 +28,10 @@ __device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key,
     int32_t result = 0;
     const int lane = __lane_id();
     while (todo) {
-        int leader = __ffsll((unsigned long long)todo) - 1;
-        uint32_t lk = __shfl(key, leader);
-        bool mine = active && key == lk;
-        uint64_t m = __ballot(mine);
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const bool mine = active && key == lk;
+        const uint64_t m = __ballot(mine);
         int32_t base = 0;
         if (lane == leader) base = atomicAdd(&counters[lk], (int32_t)__popcll(m));
         base = __shfl(base, leader);
@@ -38,49 +41,87 @@ __device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key,
     return result;
 }
 
-__global__ __launch_bounds__(256) void k_tile_count(GeomK g, int nty, const double* __restrict__ xm,
+// fine key of a marker (see the file header); sets *bad when the P cell is
+// not within one cell of the basic cell (coordinates beyond ~1e15 m)
+__device__ __forceinline__ uint32_t fine_key(const Cells& c, int nty, bool* bad) {
+    int di = c.ip - c.ib, dj = c.jp - c.jb;
+    *bad = di < 0 || di > 1 || dj < 0 || dj > 1;
+    di = min(max(di, 0), 1);
+    dj = min(max(dj, 0), 1);
+    const int ty = (c.ib - 1) / TILE, tx = (c.jb - 1) / TILE;
+    const int fi = 2 * (c.ib - 1 - ty * TILE) + di;
+    const int fj = 2 * (c.jb - 1 - tx * TILE) + dj;
+    return (uint32_t)((tx * nty + ty) * NFINE + fj * FINE + fi);
+}
+
+__global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const double* __restrict__ xm,
                                                     const double* __restrict__ ym, int64_t M,
                                                     uint32_t* __restrict__ key,
-                                                    int32_t* __restrict__ tcount,
+                                                    int32_t* __restrict__ count,
                                                     int32_t* __restrict__ flags) {
-    int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = m < M;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = m < M;
     uint32_t k = 0;
     if (active) {
-        Cells c = marker_cells(g, xm[m], ym[m]);
-        int di = c.ip - c.ib, dj = c.jp - c.jb;
-        if (di < 0 || di > 1 || dj < 0 || dj > 1) flags[0] = 1;
-        k = (uint32_t)(((c.jb - 1) / TILE) * nty + (c.ib - 1) / TILE);
+        bool bad;
+        k = fine_key(marker_cells(g, xm[m], ym[m]), nty, &bad);
+        if (bad) flags[0] = 1;
         key[m] = k;
     }
-    wave_reserve(tcount, k, active);
+    wave_reserve(count, k, active);
 }
 
 __global__ __launch_bounds__(256) void k_copy_i32(const int32_t* __restrict__ a,
                                                   int32_t* __restrict__ b, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
 }
 
-__global__ __launch_bounds__(256) void k_tile_place(const uint32_t* __restrict__ key, int64_t M,
+__global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key, int64_t M,
                                                     int32_t* __restrict__ cursor,
                                                     uint32_t* __restrict__ perm) {
-    int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = m < M;
-    uint32_t k = active ? key[m] : 0u;
-    int32_t pos = wave_reserve(cursor, k, active);
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = m < M;
+    const uint32_t k = active ? key[m] : 0u;
+    const int32_t pos = wave_reserve(cursor, k, active);
     if (active) perm[pos] = (uint32_t)m;
 }
 
-// bitonic sort (ascending) of s[0..n), n a power of two, by nthreads threads
-__device__ __forceinline__ void bitonic_lds(uint32_t* s, int n) {
+constexpr int SEG_THREAD = 48;  // longest segment ordered by one thread
+constexpr int SEG_BIG = 16384;  // longest segment ordered in LDS (64 KB)
+
+__global__ __launch_bounds__(256) void k_fine_order(const int32_t* __restrict__ start, int64_t nkeys,
+                                                    uint32_t* __restrict__ perm,
+                                                    int32_t* __restrict__ big) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const int32_t a = start[k], e = start[k + 1];
+    if (e - a <= 1) return;
+    if (e - a > SEG_THREAD) {
+        const int slot = atomicAdd(&big[0], 1);
+        big[1 + slot] = (int32_t)k;
+        return;
+    }
+    for (int32_t p = a + 1; p < e; ++p) {  // insertion sort, ascending
+        const uint32_t v = perm[p];
+        int32_t q = p - 1;
+        while (q >= a && perm[q] > v) {
+            perm[q + 1] = perm[q];
+            --q;
+        }
+        perm[q + 1] = v;
+    }
+}
+
+// bitonic sort (ascending) of s[0..n), n a power of two, by the block
+__device__ __forceinline__ void bitonic(uint32_t* s, int n) {
     for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                int ixj = i ^ j;
+                const int ixj = i ^ j;
                 if (ixj > i) {
-                    uint32_t a = s[i], b = s[ixj];
-                    bool up = (i & k) == 0;
+                    const uint32_t a = s[i], b = s[ixj];
+                    const bool up = (i & k) == 0;
                     if ((a > b) == up) {
                         s[i] = b;
                         s[ixj] = a;
@@ -92,80 +133,55 @@ __device__ __forceinline__ void bitonic_lds(uint32_t* s, int n) {
     }
 }
 
-constexpr int SEG_SMALL = 4096;
-constexpr int SEG_BIG = 16384;  // 64 KB of LDS
-
-__global__ __launch_bounds__(256) void k_segsort_small(const int32_t* __restrict__ tstart,
-                                                       uint32_t* __restrict__ perm,
-                                                       int32_t* __restrict__ big) {
-    __shared__ uint32_t s[SEG_SMALL];
-    const int t = blockIdx.x;
-    const int32_t a = tstart[t], L = tstart[t + 1] - a;
-    if (L <= 1) return;
-    if (L > SEG_SMALL) {
-        if (threadIdx.x == 0) {
-            int slot = atomicAdd(&big[0], 1);
-            big[1 + slot] = t;
-        }
-        return;
-    }
-    int n = 1;
-    while (n < L) n <<= 1;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = i < L ? perm[a + i] : 0xFFFFFFFFu;
-    __syncthreads();
-    bitonic_lds(s, n);
-    for (int i = threadIdx.x; i < L; i += blockDim.x) perm[a + i] = s[i];
-}
-
-__global__ __launch_bounds__(1024) void k_segsort_big(const int32_t* __restrict__ tstart,
-                                                      uint32_t* __restrict__ perm,
-                                                      const int32_t* __restrict__ big,
-                                                      uint32_t* __restrict__ tmp) {
+__global__ __launch_bounds__(1024) void k_fine_order_big(const int32_t* __restrict__ start,
+                                                         uint32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ big,
+                                                         uint32_t* __restrict__ tmp) {
     extern __shared__ uint32_t sdyn[];
     const int nbig = big[0];
     for (int e = blockIdx.x; e < nbig; e += gridDim.x) {
-        const int t = big[1 + e];
-        const int32_t a = tstart[t], L = tstart[t + 1] - a;
+        const int32_t k = big[1 + e];
+        const int32_t a = start[k], L = start[k + 1] - a;
         int n = 1;
         while (n < L) n <<= 1;
-        // LDS when it fits, else a scratch region [2a, 2a + n) of tmp, which
-        // is disjoint between tiles because n < 2L
+        // LDS when it fits, else the scratch region [2a, 2a + n) of tmp,
+        // disjoint between segments because n < 2L
         uint32_t* s = n <= SEG_BIG ? sdyn : tmp + 2 * (int64_t)a;
         for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = i < L ? perm[a + i] : 0xFFFFFFFFu;
         __syncthreads();
-        bitonic_lds(s, n);
+        bitonic(s, n);
         for (int i = threadIdx.x; i < L; i += blockDim.x) perm[a + i] = s[i];
         __syncthreads();
     }
 }
 
-int launch_tile_sort(ereb_ctx* c) {
+int launch_fine_sort(ereb_ctx* c) {
     const int64_t M = c->marknum;
+    const int64_t nk = c->nkeys;
     int rc;
     hipEvent_t ev;
     prof_begin(c, EREB_K_SORT, &ev);
-    if ((rc = fill_i32(c, c->d_tcount, 0, c->ntiles))) return rc;
+    if ((rc = fill_i32(c, c->d_fcount, 0, nk))) return rc;
     if ((rc = fill_i32(c, c->d_flags, 0, 4))) return rc;
     if ((rc = fill_i32(c, c->d_big, 0, 1))) return rc;
-    GeomK g = geomk(c);
     if (M > 0) {
-        unsigned nb = (unsigned)((M + 255) / 256);
-        hipLaunchKernelGGL(k_tile_count, dim3(nb), dim3(256), 0, c->stream, g, c->nty,
-                           mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), M, c->d_key, c->d_tcount,
+        const unsigned nb = (unsigned)((M + 255) / 256);
+        hipLaunchKernelGGL(k_fine_count, dim3(nb), dim3(256), 0, c->stream, geomk(c), c->nty,
+                           mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), M, c->d_key, c->d_fcount,
                            c->d_flags);
     }
-    if ((rc = scan_exclusive(c, c->d_tcount, c->d_tstart, c->ntiles))) return rc;
+    if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk))) return rc;
     if (M > 0) {
-        unsigned nt = (unsigned)((c->ntiles + 255) / 256);
-        hipLaunchKernelGGL(k_copy_i32, dim3(nt), dim3(256), 0, c->stream, c->d_tstart,
-                           c->d_tcursor, (int64_t)c->ntiles);
-        unsigned nb = (unsigned)((M + 255) / 256);
-        hipLaunchKernelGGL(k_tile_place, dim3(nb), dim3(256), 0, c->stream, c->d_key, M,
-                           c->d_tcursor, c->d_perm);
-        hipLaunchKernelGGL(k_segsort_small, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream,
-                           c->d_tstart, c->d_perm, c->d_big);
-        hipLaunchKernelGGL(k_segsort_big, dim3(64), dim3(1024), SEG_BIG * sizeof(uint32_t),
-                           c->stream, c->d_tstart, c->d_perm, c->d_big, c->d_tmp);
+        const unsigned nkb = (unsigned)((nk + 255) / 256);
+        hipLaunchKernelGGL(k_copy_i32, dim3(nkb), dim3(256), 0, c->stream, c->d_fstart,
+                           c->d_fcursor, nk);
+        const unsigned nb = (unsigned)((M + 255) / 256);
+        hipLaunchKernelGGL(k_fine_place, dim3(nb), dim3(256), 0, c->stream, c->d_key, M,
+                           c->d_fcursor, c->d_perm);
+        hipLaunchKernelGGL(k_fine_order, dim3(nkb), dim3(256), 0, c->stream, c->d_fstart, nk,
+                           c->d_perm, c->d_big);
+        hipLaunchKernelGGL(k_fine_order_big, dim3(64), dim3(1024), SEG_BIG * sizeof(uint32_t),
+                           c->stream, c->d_fstart, c->d_perm, c->d_big, c->d_tmp);
     }
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_SORT, ev);

@@ -195,10 +195,13 @@ int ereb_csc_get(ereb_ctx* ctx, int which, int64_t* colptr, int64_t* rowval,
 int ereb_csc_size(const ereb_ctx* ctx, int which, int64_t* n, int64_t* nnz);
 
 /* Sort order of the last ereb_step_scatter: perm[k] (0-based marker index)
- * of the k-th marker in tile order, and the tile of every marker.  The sort
- * is a stable counting sort of marker indices by tile key. */
-int ereb_sort_get(ereb_ctx* ctx, int32_t* perm, int32_t* tile_key,
-                  int64_t* ntiles);
+ * of the k-th marker in sorted order, the fine key of every marker and the
+ * number of keys.  The sort is a stable counting sort of marker indices by
+ * fine key: the basic cell (fix, E:1456) refined by the P-cell offsets,
+ * numbered tile-major (tiles of 6x6 basic cells, column-major; 12x12 fine
+ * cells per tile, column-major). */
+int ereb_sort_get(ereb_ctx* ctx, int32_t* perm, int32_t* fine_key,
+                  int64_t* nkeys);
 
 /* ---- drop-in entry points: the reference's argument lists -------------- */
 /* Batched form of the loop E:6419-6517 (compute_marker_properties! +

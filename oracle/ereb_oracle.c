@@ -842,28 +842,39 @@ int64_t orc_assemble_thermal(const ereb_geometry* g, const double* tk1,
 
 /* ------------------------------------------------- sort specification */
 
-/* Tile key of the library's scatter: basic cell (i, j) from fix with the
- * basic clamps (E:1456, C:68-74); tiles are `tile` x `tile` basic cells,
- * numbered column-major: key = tj * ntiles_y + ti. */
-void orc_tile_keys(const ereb_geometry* g, int64_t tile, const double* xm,
+/* Fine key of the library's sort: the basic cell (ib, jb) from fix with the
+ * basic clamps (E:1456, C:68-74) refined by the P-cell offsets
+ * (ip-ib, jp-jb) in {0,1}^2 (P clamps, C:94-101), numbered tile-major:
+ * tiles of `tile` x `tile` basic cells in column-major order, then the
+ * 2*tile x 2*tile fine cells of a tile in column-major order.  Offsets
+ * outside {0,1} (coordinates beyond ~1e15 m) are clamped into it. */
+void orc_fine_keys(const ereb_geometry* g, int64_t tile, const double* xm,
                    const double* ym, int64_t marknum, int32_t* key) {
     const int64_t Nx = g->Nx, Ny = g->Ny;
     const int64_t nty = (Ny - 1 + tile - 1) / tile;
+    const int64_t fine = 2 * tile;
     for (int64_t m = 0; m < marknum; ++m) {
-        int64_t i, j;
-        orc_fix(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1, Ny - 1, &i, &j);
-        key[m] = (int32_t)(((j - 1) / tile) * nty + (i - 1) / tile);
+        int64_t ib, jb, ip, jp;
+        orc_fix(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1, Ny - 1, &ib, &jb);
+        orc_fix(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny, &ip, &jp);
+        int64_t di = ip - ib, dj = jp - jb;
+        di = di < 0 ? 0 : (di > 1 ? 1 : di);
+        dj = dj < 0 ? 0 : (dj > 1 ? 1 : dj);
+        const int64_t ty = (ib - 1) / tile, tx = (jb - 1) / tile;
+        const int64_t fi = 2 * (ib - 1 - ty * tile) + di;
+        const int64_t fj = 2 * (jb - 1 - tx * tile) + dj;
+        key[m] = (int32_t)((tx * nty + ty) * fine * fine + fj * fine + fi);
     }
 }
 
 /* Stable counting sort of 0..marknum-1 by key: perm[k] = marker index,
- * start[t] = first position of tile t (start[ntiles] = marknum). */
-void orc_tile_sort(const int32_t* key, int64_t marknum, int64_t ntiles,
-                   int32_t* perm, int64_t* start) {
-    int64_t* cnt = (int64_t*)calloc((size_t)ntiles + 1, sizeof(int64_t));
+ * start[q] = first position of key q (start[nkeys] = marknum). */
+void orc_key_sort(const int32_t* key, int64_t marknum, int64_t nkeys,
+                  int32_t* perm, int64_t* start) {
+    int64_t* cnt = (int64_t*)calloc((size_t)nkeys + 1, sizeof(int64_t));
     for (int64_t m = 0; m < marknum; ++m) cnt[key[m] + 1]++;
-    for (int64_t t = 0; t < ntiles; ++t) cnt[t + 1] += cnt[t];
-    for (int64_t t = 0; t <= ntiles; ++t) start[t] = cnt[t];
+    for (int64_t t = 0; t < nkeys; ++t) cnt[t + 1] += cnt[t];
+    for (int64_t t = 0; t <= nkeys; ++t) start[t] = cnt[t];
     for (int64_t m = 0; m < marknum; ++m) perm[cnt[key[m]]++] = (int32_t)m;
     free(cnt);
 }

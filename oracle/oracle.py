@@ -65,8 +65,8 @@ def lib():
             "orc_assemble_hydromechanical": (_I64, [_P, _P] + [_P] * 20 + [_D, _P, _P, _P, _P,
                                                                            _I64]),
             "orc_assemble_thermal": (_I64, [_P] + [_P] * 8 + [_P, _D, _P, _P, _P, _I64]),
-            "orc_tile_keys": (None, [_P, _I64, _P, _P, _I64, _P]),
-            "orc_tile_sort": (None, [_P, _I64, _I64, _P, _P]),
+            "orc_fine_keys": (None, [_P, _I64, _P, _P, _I64, _P]),
+            "orc_key_sort": (None, [_P, _I64, _I64, _P, _P]),
             "orc_compute_rotation_rate": (None, [_P, _P, _P, _P]),
         }
         for name, (res, args) in sig.items():
@@ -257,16 +257,19 @@ class Oracle:
         return cp, rv[:nnz].copy(), nz[:nnz].copy(), R
 
     # ---- sort specification -------------------------------------------------------
-    def tile_sort(self, xm, ym, tile=6):
+    def fine_sort(self, xm, ym, tile=6):
+        """The library's sort specification: stable counting sort by fine key
+        (see orc_fine_keys); returns perm, key, start."""
         g = self.geom
         xm, ym = _c(xm), _c(ym)
         key = np.zeros(xm.size, dtype=np.int32)
-        self.L.orc_tile_keys(self.pg, tile, _p(xm), _p(ym), xm.size, _p(key))
+        self.L.orc_fine_keys(self.pg, tile, _p(xm), _p(ym), xm.size, _p(key))
         nty = (g.Ny - 1 + tile - 1) // tile
         ntx = (g.Nx - 1 + tile - 1) // tile
+        nkeys = ntx * nty * (2 * tile) ** 2
         perm = np.zeros(xm.size, dtype=np.int32)
-        start = np.zeros(ntx * nty + 1, dtype=np.int64)
-        self.L.orc_tile_sort(_p(key), xm.size, ntx * nty, _p(perm), _p(start))
+        start = np.zeros(nkeys + 1, dtype=np.int64)
+        self.L.orc_key_sort(_p(key), xm.size, nkeys, _p(perm), _p(start))
         return perm, key, start
 
     def rotation_rate(self, vx, vy):

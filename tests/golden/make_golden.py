@@ -58,7 +58,7 @@ def compute() -> dict:
     out.update(sd_colptr=cp, sd_rowval=rv, sd_nzval=nz, sd_R=R)
     cp, rv, nz, R = orc.assemble_thermal(cases.thermal_inputs(g), 3e10)
     out.update(t_colptr=cp, t_rowval=rv, t_nzval=nz, t_R=R)
-    perm, key, start = orc.tile_sort(mk["xm"], mk["ym"])
+    perm, key, start = orc.fine_sort(mk["xm"], mk["ym"])
     out.update(sort_perm=perm, sort_key=key, sort_start=start)
     return out
 

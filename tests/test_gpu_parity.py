@@ -73,8 +73,8 @@ def test_scatter_dropin_vs_oracle(mode):
     absb = cases.abs_sums(g, mk, props_o)
     for s, (a, b, c) in enumerate(zip(sums, sums_o, absb)):
         _sum_close(a, b, c, ereb.SUM_FIELDS[s])
-    operm, okey, _ = Oracle(g, mat).tile_sort(mk["xm"], mk["ym"])
-    _exact(key, okey, "tile keys")
+    operm, okey, _ = Oracle(g, mat).fine_sort(mk["xm"], mk["ym"])
+    _exact(key, okey, "fine keys")
     _exact(perm, operm, "sort order")
 
 

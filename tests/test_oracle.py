@@ -34,7 +34,8 @@ pinned by
   orc_assemble_hydromechanical       T:4147-4445 dense restatement (rtol 1e-9 on
                                      values as T:4447; pattern = nonzeros of it)
   orc_assemble_thermal               T:5391-5455 dense restatement
-  orc_tile_sort                      permutation, key order, stability, brute force
+  orc_fine_keys / orc_key_sort       permutation, key order, stability, brute force,
+                                     Vx/Vy cells = (ip, jb)/(ib, jp)
 """
 from __future__ import annotations
 
@@ -832,24 +833,31 @@ def test_assemble_thermal_restatement(tc):
 
 # -------------------------------------------------------------------- sort
 
-def test_tile_sort_properties(tc):
+def test_fine_sort_properties(tc):
     g, mat, orc = tc
     mk = cases.ref_markers(g, mat, seed=70, extra=4000)
-    perm, key, start = orc.tile_sort(mk["xm"], mk["ym"])
+    perm, key, start = orc.fine_sort(mk["xm"], mk["ym"])
     M = perm.size
     assert np.array_equal(np.sort(perm), np.arange(M))      # a permutation
     k = key[perm]
-    assert np.all(np.diff(k) >= 0)                           # grouped by tile
+    assert np.all(np.diff(k) >= 0)                           # grouped by key
     same = np.diff(k) == 0
     assert np.all(np.diff(perm)[same] > 0)                   # stable
     assert np.array_equal(np.searchsorted(k, np.arange(start.size - 1)), start[:-1])
-    # brute force: stable argsort
-    assert np.array_equal(perm, np.argsort(key, kind="stable"))
-    # tile of every marker = basic cell // 6
-    for m in range(0, M, 11):
-        i, j = orc.fix(mk["xm"][m], mk["ym"][m], "basic")
-        nty = (g.Ny - 2) // 6 + 1
-        assert key[m] == ((j - 1) // 6) * nty + (i - 1) // 6
+    assert np.array_equal(perm, np.argsort(key, kind="stable"))  # brute force
+    # key = (tile, fine cell) from the basic and P cells of fix()
+    nty = (g.Ny - 2) // 6 + 1
+    for m in range(0, M, 7):
+        ib, jb = orc.fix(mk["xm"][m], mk["ym"][m], "basic")
+        ip, jp = orc.fix(mk["xm"][m], mk["ym"][m], "p")
+        assert ip - ib in (0, 1) and jp - jb in (0, 1)
+        ty, tx = (ib - 1) // 6, (jb - 1) // 6
+        fi = 2 * (ib - 1 - 6 * ty) + ip - ib
+        fj = 2 * (jb - 1 - 6 * tx) + jp - jb
+        assert key[m] == (tx * nty + ty) * 144 + fj * 12 + fi
+        # Vx cell = (ip, jb), Vy cell = (ib, jp): one fine cell fixes all stencils
+        assert orc.fix(mk["xm"][m], mk["ym"][m], "vx") == (ip, jb)
+        assert orc.fix(mk["xm"][m], mk["ym"][m], "vy") == (ib, jp)
 
 
 # --------------------------------------------------------- golden fixtures
