@@ -35,7 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "erebus.jl_amd"))
 import numpy as np  # noqa: E402
 
 import ereb  # noqa: E402
-from ereb import synth  # noqa: E402
+from ereb import precision, synth  # noqa: E402
 
 METRIC = "marker-steps/s, 2.8M markers: FP64 vs mixed-precision; adjoint/forward ratio"
 HBM_PEAK_GBS = 8000.0
@@ -43,47 +43,53 @@ START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 
 # algorithmic bytes of the fused property + scatter kernel: every input
 # marker vector read once (15 Float64 + tm as 4 bytes), the 7 property
-# vectors written once, the 30 SUM grids written once
+# vectors written once, the 30 SUM grids (Float64) written once
 SCATTER_MARKER_BYTES = 15 * 8 + 4 + 7 * 8
-SCATTER_NODE_BYTES = 30 * 8
+# mixed form: the 6 state vectors (xm, ym, tkm, phim, sxxm, sxym) stay
+# Float64, the 9 property inputs and 7 property outputs are float
+SCATTER_MARKER_BYTES_MIXED = 6 * 8 + 4 + 9 * 4 + 7 * 4
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(ctx: ereb.Context, g, mat, mk):
-    """Upload markers and the synthetic grids; one scatter + node pass
-    provides ETA/PHI for the bulk viscosity and tk1 for tk2."""
-    ctx.set_markers(mk)
+INPUT_GRIDS = ["vx", "vy", "vxf", "vyf", "wyx", "ETAP", "ETAPHI", "gx", "gy", "DT", "tk2",
+               "pr0", "pf0", "DMP", "HA", "HS", "DHP", "DSXX", "DSXY"]
+
+
+def make_inputs(g, mat, mk):
+    """The synthetic grids that the solves and grid-only operations would
+    produce, derived once from a Float64 scatter + node pass (ETA and PHI for
+    the bulk viscosity, tk1 for tk2), so that every form runs on identical
+    inputs."""
     vx, vy = synth.stream_velocity(g)
-    ctx.set_grid("vx", vx)
-    ctx.set_grid("vy", vy)
-    ctx.set_grid("vxf", vx)
-    ctx.set_grid("vyf", vy)
-    ctx.set_grid("wyx", synth.rotation_rate(g, vx, vy))
     hrs, hrf = mat.radioactive_heating(START_TIME)
-    ctx.step_scatter(hrs, hrf, mat.marker_property_mode)
-    ctx.step_node_properties()
-    ETA, PHI, tk1 = ctx.get_grid("ETA"), ctx.get_grid("PHI"), ctx.get_grid("tk1")
-    ETAP, ETAPHI = synth.bulk_viscosity(ETA, np.maximum(PHI, 1e-4))
-    ctx.set_grid("ETAP", ETAP)
-    ctx.set_grid("ETAPHI", ETAPHI)
-    gx, gy = synth.disk_gravity(g)
-    ctx.set_grid("gx", gx)
-    ctx.set_grid("gy", gy)
+    with ereb.Context(g, mat, ereb.FP64) as ctx:
+        ctx.set_markers(mk)
+        ctx.step_scatter(hrs, hrf, mat.marker_property_mode)
+        ctx.step_node_properties()
+        ETA, PHI, tk1 = ctx.get_grid("ETA"), ctx.get_grid("PHI"), ctx.get_grid("tk1")
+    grids = {"vx": vx, "vy": vy, "vxf": vx, "vyf": vy, "wyx": synth.rotation_rate(g, vx, vy)}
+    grids["ETAP"], grids["ETAPHI"] = synth.bulk_viscosity(ETA, np.maximum(PHI, 1e-4))
+    grids["gx"], grids["gy"] = synth.disk_gravity(g)
     S = g.stag_shape
-    DT = synth.smooth_field(g, S, 0.05, 2, 1)
-    ctx.set_grid("DT", DT)
-    ctx.set_grid("tk2", np.asfortranarray(tk1 + DT))
-    ctx.set_grid("pr0", np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 1e5, 1, 1))
-    ctx.set_grid("pf0", np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 9e4, 1, 1))
+    grids["DT"] = synth.smooth_field(g, S, 0.05, 2, 1)
+    grids["tk2"] = np.asfortranarray(tk1 + grids["DT"])
+    grids["pr0"] = np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 1e5, 1, 1)
+    grids["pf0"] = np.full(S, mat.psurface, order="F") + synth.smooth_field(g, S, 9e4, 1, 1)
     for n in ("DMP", "HA", "HS", "DHP"):
-        ctx.set_grid(n, np.zeros(S, order="F"))
-    ctx.set_grid("DSXX", synth.smooth_field(g, S, 1e3, 3, 2))
-    ctx.set_grid("DSXY", synth.smooth_field(g, g.basic_shape, 1e3, 2, 3))
+        grids[n] = np.zeros(S, order="F")
+    grids["DSXX"] = synth.smooth_field(g, S, 1e3, 3, 2)
+    grids["DSXY"] = synth.smooth_field(g, g.basic_shape, 1e3, 2, 3)
+    return grids, hrs, hrf
+
+
+def setup(ctx: ereb.Context, mk, grids):
+    ctx.set_markers(mk)
+    for n in INPUT_GRIDS:
+        ctx.set_grid(n, grids[n])
     ctx.sync()
-    return hrs, hrf
 
 
 def step(ctx, hrs, hrf, mode, dt, timestep):
@@ -157,14 +163,60 @@ def cpu_baseline(g, mat, mk, steps: int, budget_s: float = 25.0):
                       f"(-O2, no FMA), {el:.1f} s"}
 
 
+def run_form(g, mat, mk, grids, hrs, hrf, precision, steps, warmup):
+    """Warmup + timed steps of one precision form; returns the live context
+    (state after warmup + steps), marker-steps/s, ms per step and the mean
+    duration of each kernel class (HIP events on the library stream)."""
+    mode = mat.marker_property_mode
+    dt = ereb.DT_LONGEST
+    ctx = ereb.Context(g, mat, precision)
+    setup(ctx, mk, grids)
+    ts = 2
+    for _ in range(warmup):
+        step(ctx, hrs, hrf, mode, dt, ts)
+        ts += 1
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(ctx, hrs, hrf, mode, dt, ts)
+        ts += 1
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kern = {}
+    for k in ereb.lib.KERNEL_CLASSES:
+        ms, n = ctx.profile_get(k)
+        kern[k] = round(ms / max(n, 1), 5)
+    ctx.profile(False)
+    M = mk["xm"].size
+    return ctx, M * steps / el, el * 1e3 / steps, kern
+
+
+def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
+    sums = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1  # elements of the 30 SUM grids
+    sc_bytes = M * marker_bytes + sums * 8
+    achieved = sc_bytes / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"scatter_{form}_{workload}_hbm_bytes.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("bytes_per_launch")
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": f"{kernel_name} (marker properties + scatter, one launch per step)",
+            "algorithmic_bytes": sc_bytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fp64", choices=sorted(synth.WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -173,48 +225,21 @@ def main():
 
     g, mat = synth.workload_geometry(a.workload, mode=1)
     mode = mat.marker_property_mode
-    dt = ereb.DT_LONGEST
     log(f"[bench] workload {a.workload}: {g.Nx}x{g.Ny} nodes, {g.Nxmc}x{g.Nymc} markers/cell")
     mk = synth.make_markers(g, mat, seed=0)
     M = mk["xm"].size
-    ctx = ereb.Context(g, mat, ereb.FP64)
-    hrs, hrf = setup(ctx, g, mat, mk)
-    log(f"[bench] {M:,} markers resident; warmup {a.warmup}")
-    ts = 2
-    for _ in range(a.warmup):
-        step(ctx, hrs, hrf, mode, dt, ts)
-        ts += 1
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step(ctx, hrs, hrf, mode, dt, ts)
-        ts += 1
-    ctx.sync()
-    el = time.perf_counter() - t0
-    log(f"[bench] timed {a.steps} steps in {el * 1e3:.2f} ms")
-    kern = {}
-    for k in ereb.lib.KERNEL_CLASSES:
-        ms, n = ctx.profile_get(k)
-        kern[k] = round(ms / max(n, 1), 5)
-    sc_ms, sc_n = ctx.profile_get("scatter")
-    nodes = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1
-    sc_bytes = M * SCATTER_MARKER_BYTES + nodes * 8
-    achieved = sc_bytes / (sc_ms / sc_n * 1e-3) / 1e9
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"scatter_{a.workload}_hbm_bytes.json")
-    if os.path.exists(pmc):
-        traffic = json.load(open(pmc)).get("bytes_per_launch")
+    grids, hrs, hrf = make_inputs(g, mat, mk)
+    log(f"[bench] {M:,} markers; FP64 form: warmup {a.warmup}, {a.steps} timed steps")
+    ctx64, value, ms, kern = run_form(g, mat, mk, grids, hrs, hrf, ereb.FP64, a.steps, a.warmup)
+    log(f"[bench] FP64 {ms:.3f} ms/step")
     res = {
         "metric": METRIC,
-        "value": M * a.steps / el,
+        "value": value,
         "unit": "marker-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el * 1e3 / a.steps,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -225,13 +250,36 @@ def main():
                                f"planetesimal R={g.rplanet / 1e3:.0f} km in 140 km box, "
                                f"stream-function velocity Courant 0.5, dt=1e11 s, mode {mode}",
                    "markers": M, "grid": [g.Ny, g.Nx], "form": "fp64"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_scatter_fp64 (marker properties + scatter, one launch per step)",
-                     "algorithmic_bytes": sc_bytes},
+        "roofline": scatter_roofline(g, M, kern["scatter"], SCATTER_MARKER_BYTES,
+                                     "k_scatter<double>", "fp64", a.workload),
         "kernel_ms": kern,
     }
-    ctx.close()
+    if not a.no_mixed:
+        log("[bench] mixed-precision form")
+        ctxm, vm, msm, kernm = run_form(g, mat, mk, grids, hrs, hrf, ereb.MIXED, a.steps,
+                                        a.warmup)
+        errs = precision.field_errors(ctxm, ctx64)
+        wk, wv = precision.worst(errs)
+        nodal = set(ereb.SUM_FIELDS + precision.NODE_FIELDS)
+        wn, wnv = precision.worst(errs, nodal)
+        res["mixed"] = {
+            "value": vm, "ms_per_step": msm, "speedup_vs_fp64": vm / value,
+            "dtype": "f32 properties/weights/interpolation, f64 state/normalisation/coefficients",
+            "roofline": scatter_roofline(g, M, kernm["scatter"], SCATTER_MARKER_BYTES_MIXED,
+                                         "k_scatter<float>", "mixed", a.workload),
+            "kernel_ms": kernm,
+            "errors_vs_fp64": {
+                "after_steps": a.warmup + a.steps,
+                "metric": "per field [max|a-ref|/max|ref|, rms(a-ref)/rms(ref)]",
+                "worst_field": wk, "worst_max_rel": wv,
+                "worst_nodal_field": wn, "worst_nodal_max_rel": wnv,
+                "fields": {k: [float(f"{e[0]:.3e}"), float(f"{e[1]:.3e}")]
+                           for k, e in errs.items()},
+            },
+        }
+        log(f"[bench] mixed {msm:.3f} ms/step, worst nodal field {wn} {wnv:.2e}")
+        ctxm.close()
+    ctx64.close()
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, 1 thread)")
         res["cpu_baseline"] = cpu_baseline(g, mat, mk, a.cpu_steps)

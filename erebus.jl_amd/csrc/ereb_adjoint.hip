@@ -1,0 +1,423 @@
+// ereb_adjoint.hip -- reverse-mode (adjoint) kernels of the transfer path
+// with respect to the marker temperature tkm and porosity phim.  The
+// reference has no adjoint; these are the transposes of the forward kernels:
+//   scatter^T  (marker_to_*_nodes!, E:1605-1863, with the property
+//              derivatives of compute_marker_properties!, E:700-750): a
+//              gather -- each marker reads the SUM cotangents at its 16
+//              nodes, no atomics;
+//   nodes^T    (compute_*_node_properties!, E:1970-2189, and the insulating
+//              boundary copy on tk1, E:2432-2449).
+// The gather^T and RK4^T kernels sit in ereb_markers.hip next to their
+// forward kernels.  Marker positions and the grid weights do not depend on
+// tkm or phim (the velocity is prescribed), so they carry no cotangent.
+#include <algorithm>
+#include <vector>
+
+#include "ereb_internal.h"
+
+namespace ereb {
+
+struct ScatterAdjArgs {
+    GeomK g;
+    const ereb_materials* __restrict__ M;
+    double hrs[3], hrf[3];
+    int mode;
+    int64_t n;
+    const double *xm, *ym, *tkm, *phim, *XW;
+    const int32_t* tm;
+    const double *st_rho, *st_rhocp, *st_eta, *st_hr, *st_k;  // stored properties (air)
+    const double* gs;  // cotangents of the 30 SUM grids, concatenated (basic Ny*Nx, then Ny1*Nx1)
+    int64_t goff[NSUM];
+    double *gT, *gphi;  // accumulated (+=)
+};
+
+__device__ __forceinline__ double gdot4_at(const double* gr, int rows, int i, int j,
+                                           const double* w) {
+    return gdot4(gr, rows, i, j, w);
+}
+
+// scatter^T: one thread per marker (storage order)
+__global__ __launch_bounds__(256) void k_scatter_adj(ScatterAdjArgs A) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.n) return;
+    const GeomK& g = A.g;
+    const ereb_materials& M = *A.M;
+    const double x = A.xm[m], y = A.ym[m], T = A.tkm[m], phi = A.phim[m];
+    const int tt = min(max(A.tm[m], 1), 3);
+    const Cells cl = marker_cells(g, x, y);
+    const double ab = (x - g.x[cl.jb - 1]) / g.dx, bb = (y - g.y[cl.ib - 1]) / g.dy;
+    const double ap = (x - g.xp[cl.jp - 1]) / g.dx, bp = (y - g.yp[cl.ip - 1]) / g.dy;
+    double wvx[4], wvy[4], wp[4];
+    weights4(ab, bp, wvx);  // Vx cell (ip, jb)
+    weights4(ap, bb, wvy);  // Vy cell (ib, jp)
+    weights4(ap, bp, wp);   // P cell (ip, jp)
+    Props st{A.st_rho[m], A.st_rhocp[m], A.st_eta[m], A.st_hr[m], A.st_k[m], 0.0, 0.0};
+    const Props p = marker_props(M, tt, T, phi, A.XW[m], A.hrs, A.hrf, A.mode, st);
+    const PropsD d = marker_props_d(M, tt, T, phi, A.XW[m], A.hrs, A.hrf, A.mode, p);
+    auto G = [&](int slot) { return A.gs + A.goff[slot]; };
+    const int R = g.Ny1;
+    double gT = 0.0, gp = 0.0;
+    // Vx (slots 8..13: RHO RHOF K PHI R WT) and Vy (14..19)
+    for (int v = 0; v < 2; ++v) {
+        const int s0 = v == 0 ? 8 : 14;
+        const double* w = v == 0 ? wvx : wvy;
+        const int i = v == 0 ? cl.ip : cl.ib, j = v == 0 ? cl.jb : cl.jp;
+        const double grho = gdot4_at(G(s0 + 0), R, i, j, w);
+        const double gk = gdot4_at(G(s0 + 2), R, i, j, w);
+        const double gph = gdot4_at(G(s0 + 3), R, i, j, w);
+        const double gr = gdot4_at(G(s0 + 4), R, i, j, w);
+        gT += gk * d.k_t;
+        gp += grho * d.rho_p + gk * d.k_p + gph + gr * d.ef_p;
+    }
+    // P (slots 20..29: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PHI WT)
+    {
+        const double grho = gdot4_at(G(20), R, cl.ip, cl.jp, wp);
+        const double grc = gdot4_at(G(21), R, cl.ip, cl.jp, wp);
+        const double ghr = gdot4_at(G(24), R, cl.ip, cl.jp, wp);
+        const double gtk = gdot4_at(G(27), R, cl.ip, cl.jp, wp);
+        const double gph = gdot4_at(G(28), R, cl.ip, cl.jp, wp);
+        gT += grc * d.rhocp_t + gtk * d.tk_t;
+        gp += grho * d.rho_p + grc * d.rhocp_p + ghr * d.hr_p + gtk * d.tk_p + gph;
+    }
+    A.gT[m] += gT;
+    A.gphi[m] += gp;
+}
+
+// nodes^T on the P grid: cotangents of RHO RHOCP ALPHA ALPHAF HR GGGP SXX0 tk1
+// PHI BETAPHI -> cotangents of the 10 P SUM grids (WTPSUM: the weights carry
+// no cotangent).  Zero-weight nodes keep their old values in the forward
+// pass, so their SUM cotangents are zero.
+struct NodesAdjArgs {
+    int n;  // Ny1 * Nx1
+    const double *RHOCPSUM, *TKSUM, *GGGPSUM, *PHISUM, *WTPSUM;
+    const double* gnode[10];  // RHO RHOCP ALPHA ALPHAF HR GGGP SXX0 tk1 PHI BETAPHI
+    double* gsum[10];         // RHOSUM ... PHISUM, WTPSUM (set to 0)
+};
+
+__global__ __launch_bounds__(256) void k_nodes_adj_p(NodesAdjArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.n) return;
+    const double wt = A.WTPSUM[k];
+    double out[10] = {};
+    if (wt > 0.0) {
+        const double iw = 1.0 / wt;
+        const double* const* gn = A.gnode;
+        out[0] = gn[0][k] * iw;  // RHO
+        out[2] = gn[2][k] * iw;  // ALPHA
+        out[3] = gn[3][k] * iw;  // ALPHAF
+        out[4] = gn[4][k] * iw;  // HR
+        out[6] = gn[6][k] * iw;  // SXX0
+        // tk1 = TKSUM * inv(RHOCPSUM); RHOCP = RHOCPSUM * iw
+        const double irc = 1.0 / A.RHOCPSUM[k];
+        out[7] = gn[7][k] * irc;
+        out[1] = gn[1][k] * iw - gn[7][k] * A.TKSUM[k] * irc * irc;
+        // GGGP = inv(GGGPSUM) * wt; PHI = PHISUM * iw; BETAPHI = inv(GGGP) * PHI
+        const double gggp = (1.0 / A.GGGPSUM[k]) * wt, ph = A.PHISUM[k] * iw;
+        const double g_gggp = gn[5][k] - gn[9][k] * ph / (gggp * gggp);
+        const double g_phi = gn[8][k] + gn[9][k] / gggp;
+        out[5] = -g_gggp * wt / (A.GGGPSUM[k] * A.GGGPSUM[k]);
+        out[8] = g_phi * iw;
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) A.gsum[q][k] = out[q];
+}
+
+// nodes^T on a Vx or Vy grid: RHO RHOF K PHI R = SUM * iw
+__global__ __launch_bounds__(256) void k_nodes_adj_v(int n, const double* __restrict__ wts,
+                                                     const double* g0, const double* g1,
+                                                     const double* g2, const double* g3,
+                                                     const double* g4, double* s0, double* s1,
+                                                     double* s2, double* s3, double* s4,
+                                                     double* s5) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double wt = wts[k];
+    const double iw = wt > 0.0 ? 1.0 / wt : 0.0;
+    s0[k] = g0[k] * iw;
+    s1[k] = g1[k] * iw;
+    s2[k] = g2[k] * iw;
+    s3[k] = g3[k] * iw;
+    s4[k] = g4[k] * iw;
+    s5[k] = 0.0;
+}
+
+// insulating boundary copy on a (Nyy, Nxx) array, transposed: the forward
+// pass copies rows 2 and Nyy-1 into rows 1 and Nyy (columns 2..Nxx-1), then
+// columns 2 and Nxx-1 into columns 1 and Nxx (all rows); the adjoint undoes
+// the column step first, then the row step.
+__global__ void k_bc_adj_cols(double* t, int Nyy, int Nxx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nyy) return;
+    double* c1 = t + i;
+    c1[(size_t)1 * Nyy] += c1[0];
+    c1[0] = 0.0;
+    c1[(size_t)(Nxx - 2) * Nyy] += c1[(size_t)(Nxx - 1) * Nyy];
+    c1[(size_t)(Nxx - 1) * Nyy] = 0.0;
+}
+
+__global__ void k_bc_adj_rows(double* t, int Nyy, int Nxx) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;  // columns 2..Nxx-1 (0-based 1..)
+    if (j >= Nxx - 1) return;
+    double* col = t + (size_t)j * Nyy;
+    col[1] += col[0];
+    col[0] = 0.0;
+    col[Nyy - 2] += col[Nyy - 1];
+    col[Nyy - 1] = 0.0;
+}
+
+int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
+                       const double* gsum_base, const int64_t* gsum_off, double* gT,
+                       double* gphi) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return EREB_OK;
+    if (c->precision != EREB_FP64)
+        return fail(c, EREB_ERR_ARG, "the adjoint kernels run in the Float64 form");
+    ScatterAdjArgs A;
+    A.g = geomk(c);
+    A.M = c->d_mat;
+    for (int k = 0; k < 3; ++k) {
+        A.hrs[k] = hrs[k];
+        A.hrf[k] = hrf[k];
+    }
+    A.mode = mode;
+    A.n = M;
+    A.xm = mptr(c, EREB_M_XM);
+    A.ym = mptr(c, EREB_M_YM);
+    A.tkm = mptr(c, EREB_M_TKM);
+    A.phim = mptr(c, EREB_M_PHIM);
+    A.XW = mptr(c, EREB_M_XWSOLIDM0);
+    A.tm = tm_ptr(c);
+    A.st_rho = mptr(c, EREB_M_RHOTOTALM);
+    A.st_rhocp = mptr(c, EREB_M_RHOCPTOTALM);
+    A.st_eta = mptr(c, EREB_M_ETATOTALM);
+    A.st_hr = mptr(c, EREB_M_HRTOTALM);
+    A.st_k = mptr(c, EREB_M_KTOTALM);
+    A.gs = gsum_base;
+    for (int s = 0; s < NSUM; ++s) A.goff[s] = gsum_off[s];
+    A.gT = gT;
+    A.gphi = gphi;
+    hipLaunchKernelGGL(k_scatter_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                       c->stream, A);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
+
+int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off) {
+    // off: offsets (in doubles from gbase) of the grid fields, by ereb_grid_field
+    auto G = [&](int f) { return gbase + off[f]; };
+    const int n = c->Nx1 * c->Ny1;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    // tk1 boundary copy first (it was applied last in the forward pass)
+    if (c->Ny1 > 2 && c->Nx1 > 2) {
+        hipLaunchKernelGGL(k_bc_adj_cols, dim3((unsigned)((c->Ny1 + 255) / 256)), dim3(256), 0,
+                           c->stream, G(EREB_G_TK1), c->Ny1, c->Nx1);
+        hipLaunchKernelGGL(k_bc_adj_rows, dim3((unsigned)((c->Nx1 + 255) / 256)), dim3(256), 0,
+                           c->stream, G(EREB_G_TK1), c->Ny1, c->Nx1);
+    }
+    NodesAdjArgs A;
+    A.n = n;
+    A.RHOCPSUM = gptr(c, EREB_G_RHOCPSUM);
+    A.TKSUM = gptr(c, EREB_G_TKSUM);
+    A.GGGPSUM = gptr(c, EREB_G_GGGPSUM);
+    A.PHISUM = gptr(c, EREB_G_PHISUM);
+    A.WTPSUM = gptr(c, EREB_G_WTPSUM);
+    const int pin[10] = {EREB_G_RHO, EREB_G_RHOCP, EREB_G_ALPHA, EREB_G_ALPHAF, EREB_G_HR,
+                         EREB_G_GGGP, EREB_G_SXX0, EREB_G_TK1, EREB_G_PHI, EREB_G_BETAPHI};
+    const int pout[10] = {EREB_G_RHOSUM, EREB_G_RHOCPSUM, EREB_G_ALPHASUM, EREB_G_ALPHAFSUM,
+                          EREB_G_HRSUM, EREB_G_GGGPSUM, EREB_G_SXXSUM, EREB_G_TKSUM,
+                          EREB_G_PHISUM, EREB_G_WTPSUM};
+    for (int q = 0; q < 10; ++q) {
+        A.gnode[q] = G(pin[q]);
+        A.gsum[q] = G(pout[q]);
+    }
+    hipLaunchKernelGGL(k_nodes_adj_p, dim3(nb), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_nodes_adj_v, dim3(nb), dim3(256), 0, c->stream, n,
+                       (const double*)gptr(c, EREB_G_WTXSUM), G(EREB_G_RHOX), G(EREB_G_RHOFX),
+                       G(EREB_G_KX), G(EREB_G_PHIX), G(EREB_G_RX), G(EREB_G_RHOXSUM),
+                       G(EREB_G_RHOFXSUM), G(EREB_G_KXSUM), G(EREB_G_PHIXSUM),
+                       G(EREB_G_RXSUM), G(EREB_G_WTXSUM));
+    hipLaunchKernelGGL(k_nodes_adj_v, dim3(nb), dim3(256), 0, c->stream, n,
+                       (const double*)gptr(c, EREB_G_WTYSUM), G(EREB_G_RHOY), G(EREB_G_RHOFY),
+                       G(EREB_G_KY), G(EREB_G_PHIY), G(EREB_G_RY), G(EREB_G_RHOYSUM),
+                       G(EREB_G_RHOFYSUM), G(EREB_G_KYSUM), G(EREB_G_PHIYSUM),
+                       G(EREB_G_RYSUM), G(EREB_G_WTYSUM));
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
+
+}  // namespace ereb
+
+// ---- adjoint driver: gradient of sum(tk1^2) after n steps ------------------
+
+namespace ereb {
+
+__global__ void k_twice(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = 2.0 * a[i];
+}
+
+static const int kStateFields[5] = {EREB_M_XM, EREB_M_YM, EREB_M_TKM, EREB_M_SXXM,
+                                    EREB_M_SXYM};
+
+// copy marker fields of the resident state to (save) or from (restore) dst,
+// nf fields of M doubles each
+static int state_copy(ereb_ctx* c, double* buf, int nf, bool save) {
+    const size_t bytes = sizeof(double) * (size_t)c->marknum;
+    for (int q = 0; q < nf; ++q) {
+        double* dev = mptr(c, kStateFields[q]);
+        double* b = buf + (size_t)q * c->marknum;
+        EREB_HIP(c, hipMemcpyAsync(save ? b : dev, save ? dev : b, bytes,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    }
+    return EREB_OK;
+}
+
+// one timestep of the differentiated map: scatter, node properties with the
+// insulating tk1 boundary, tk2 := tk1 (no heat solve), the temperature
+// gather (only at timestep 1: later it adds DT = tk2 - tk1 = 0), RK4
+static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const double* hrs,
+                            const double* hrf, bool rk4) {
+    int rc;
+    if ((rc = launch_fine_sort(c))) return rc;
+    if ((rc = launch_scatter(c, hrs, hrf, mode, true))) return rc;
+    if ((rc = launch_combine(c))) return rc;
+    if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
+    EREB_HIP(c, hipMemcpyAsync(gptr(c, EREB_G_TK2), gptr(c, EREB_G_TK1),
+                               sizeof(double) * c->gnumel[EREB_G_TK1], hipMemcpyDeviceToDevice,
+                               c->stream));
+    if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
+    if (rk4 && (rc = launch_rk4(c, dt, mode))) return rc;
+    return EREB_OK;
+}
+
+struct AdjBuffers {
+    double *save = nullptr, *cps = nullptr, *seg = nullptr, *tp = nullptr;
+    double *gT = nullptr, *gT2 = nullptr, *gP = nullptr, *gg = nullptr;
+    ~AdjBuffers() {
+        hipFree(save);
+        hipFree(cps);
+        hipFree(seg);
+        hipFree(tp);
+        hipFree(gT);
+        hipFree(gT2);
+        hipFree(gP);
+        hipFree(gg);
+    }
+};
+
+}  // namespace ereb
+
+using namespace ereb;
+
+extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every, double dt,
+                                     int64_t mode, const double hrsolidm[3],
+                                     const double hrfluidm[3], double* loss, double* tk1_out,
+                                     double* grad_tkm, double* grad_phim) {
+    if (!c || !hrsolidm || !hrfluidm || !loss || nsteps < 0 || every < 1) return EREB_ERR_ARG;
+    if (mode != 1 && mode != 9) return fail(c, EREB_ERR_MODE, "unknown mode");
+    const bool grad = grad_tkm || grad_phim;
+    if (grad && c->precision != EREB_FP64)
+        return fail(c, EREB_ERR_ARG, "the adjoint kernels run in the Float64 form");
+    const int64_t M = c->marknum;
+    if (M <= 0) return fail(c, EREB_ERR_STATE, "no markers");
+    const int md = (int)mode;
+    const int64_t nseg = (nsteps + every - 1) / every;
+    const size_t MB = sizeof(double) * (size_t)M;
+    AdjBuffers B;
+    int rc;
+    // the markers are restored at the end
+    EREB_HIP(c, hipMalloc(&B.save, 5 * MB));
+    if ((rc = state_copy(c, B.save, 5, true))) return rc;
+    if (grad) {
+        EREB_HIP(c, hipMalloc(&B.cps, 3 * MB * (size_t)std::max<int64_t>(nseg, 1)));
+        EREB_HIP(c, hipMalloc(&B.seg, 3 * MB * (size_t)every));
+        EREB_HIP(c, hipMalloc(&B.tp, MB));
+        EREB_HIP(c, hipMalloc(&B.gT, MB));
+        EREB_HIP(c, hipMalloc(&B.gT2, MB));
+        EREB_HIP(c, hipMalloc(&B.gP, MB));
+        size_t gsz = 0;
+        for (int f = 0; f < EREB_G_NFIELDS; ++f)
+            gsz = std::max(gsz, c->goff[f] + (size_t)c->gnumel[f]);
+        EREB_HIP(c, hipMalloc(&B.gg, sizeof(double) * gsz));
+    }
+    // forward, with a checkpoint of (xm, ym, tkm) every `every` steps
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if (grad && k % every == 0 && (rc = state_copy(c, B.cps + 3 * (size_t)M * (k / every), 3,
+                                                       true)))
+            return rc;
+        if ((rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, true))) return rc;
+    }
+    // loss on the final state
+    int rcs;
+    if ((rcs = launch_fine_sort(c)) || (rcs = launch_scatter(c, hrsolidm, hrfluidm, md, true)) ||
+        (rcs = launch_combine(c)) ||
+        (rcs = launch_node_properties(c, true, true, true, true, true)))
+        return rcs;
+    const int64_t n = c->gnumel[EREB_G_TK1];
+    std::vector<double> tk1((size_t)n);
+    EREB_HIP(c, hipMemcpyAsync(tk1.data(), gptr(c, EREB_G_TK1), sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_flags(c))) return rc;
+    double L = 0.0;
+    for (int64_t i = 0; i < n; ++i) L += tk1[i] * tk1[i];
+    *loss = L;
+    if (tk1_out) std::copy(tk1.begin(), tk1.end(), tk1_out);
+    if (grad) {
+        auto G = [&](int f) { return B.gg + c->goff[f]; };
+        int64_t goff[EREB_G_NFIELDS];
+        for (int f = 0; f < EREB_G_NFIELDS; ++f) goff[f] = (int64_t)c->goff[f];
+        int64_t soff[NSUM];
+        for (int s = 0; s < NSUM; ++s) soff[s] = goff[sum_grid_field(s)];
+        size_t gsz = 0;
+        for (int f = 0; f < EREB_G_NFIELDS; ++f)
+            gsz = std::max(gsz, c->goff[f] + (size_t)c->gnumel[f]);
+        const unsigned nbg = (unsigned)((n + 255) / 256);
+        if ((rc = fill_f64(c, B.gT, 0.0, M)) || (rc = fill_f64(c, B.gP, 0.0, M)) ||
+            (rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz)))
+            return rc;
+        // d L / d tk1 = 2 tk1 on the final state
+        hipLaunchKernelGGL(k_twice, dim3(nbg), dim3(256), 0, c->stream,
+                           (const double*)gptr(c, EREB_G_TK1), G(EREB_G_TK1), n);
+        if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
+        if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT, B.gP)))
+            return rc;
+        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+            const int64_t k0 = sg * every, k1 = std::min(nsteps, k0 + every);
+            // recompute the segment from its checkpoint, keeping each step's start state
+            if ((rc = state_copy(c, B.cps + 3 * (size_t)M * sg, 3, false))) return rc;
+            for (int64_t k = k0; k < k1; ++k) {
+                if ((rc = state_copy(c, B.seg + 3 * (size_t)M * (k - k0), 3, true))) return rc;
+                if (k + 1 < k1 && (rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, true)))
+                    return rc;
+            }
+            for (int64_t k = k1 - 1; k >= k0; --k) {
+                // state at the start of step k; sums, tk1 = tk2 and (k = 0) the gather
+                if ((rc = state_copy(c, B.seg + 3 * (size_t)M * (k - k0), 3, false))) return rc;
+                if ((rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, false))) return rc;
+                if ((rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz))) return rc;
+                // RK4^T (markers at the step start, tkm after the gather)
+                if ((rc = launch_rk4_adj(c, dt, md, B.gT, B.gT2, B.gP, G(EREB_G_TK2)))) return rc;
+                if (k == 0 && (rc = launch_gather_T_adj(c, 1, B.gT2, G(EREB_G_TK2)))) return rc;
+                // tk2 := tk1, then node properties^T
+                EREB_HIP(c, hipMemcpyAsync(G(EREB_G_TK1), G(EREB_G_TK2), sizeof(double) * n,
+                                           hipMemcpyDeviceToDevice, c->stream));
+                if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
+                // scatter^T at the step start with the temperature the scatter saw
+                EREB_HIP(c, hipMemcpyAsync(mptr(c, EREB_M_TKM),
+                                           B.seg + 3 * (size_t)M * (k - k0) + 2 * (size_t)M, MB,
+                                           hipMemcpyDeviceToDevice, c->stream));
+                if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT2, B.gP)))
+                    return rc;
+                std::swap(B.gT, B.gT2);
+            }
+        }
+        if (grad_tkm)
+            EREB_HIP(c, hipMemcpyAsync(grad_tkm, B.gT, MB, hipMemcpyDeviceToHost, c->stream));
+        if (grad_phim)
+            EREB_HIP(c, hipMemcpyAsync(grad_phim, B.gP, MB, hipMemcpyDeviceToHost, c->stream));
+    }
+    if ((rc = state_copy(c, B.save, 5, false))) return rc;
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return check_flags(c);
+}

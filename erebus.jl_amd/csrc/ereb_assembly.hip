@@ -2,22 +2,28 @@
 // and assemble_thermal_lse! (E:4526-4611) straight into CSC.
 //
 // ExtendableSparse's updateindex!(L, +, v, row, col) creates no entry when
-// v == 0, so the pattern depends on the values.  Every value is computed by
-// the same expression as the reference (no fma contraction), which makes
-// the zero tests, the pattern and the stored values bit-exact.
+// v == 0 and adds v to an existing entry, so the pattern depends on the
+// values.  Every value is computed by the same expression as the reference
+// (no fma contraction), which makes the zero tests, the pattern and the
+// stored values bit-exact.
 //
-// Every row of P cell (i', j') references columns of cells (i'+di, j'+dj)
-// with |di|, |dj| <= 1 only.  Seen from a column of cell (i, j), its
-// candidate rows are therefore the 9 x NU rows of the neighbouring cells,
-// and ordering them by (dj, di, unknown) orders them by row index.  So:
-//   1. mask : one thread per P cell evaluates its rows; each nonzero entry
-//             sets bit ((dj+1)*3 + di+1)*NU + unknown in its column's 64-bit
-//             mask (atomicOr: the result does not depend on order); R is
-//             written here
-//   2. scan : column counts = popcount(mask), column pointers
-//   3. fill : the same evaluation places each entry at colptr + popcount of
-//             the lower bits of its column's mask: rows come out ascending
-//             in every column, with no sort and independent of thread order.
+// Every equation references a fixed stencil of columns relative to its own
+// row: at most 11 (row, column) offsets per Stokes-Darcy equation, 5 for the
+// thermal one.  Each offset is a "slot" of that equation; the slot tables
+// are discovered once per process by running the same equation code on the
+// host with a recording sink, so they cannot drift from the kernels.
+//   1. rows  : one thread per (P cell, equation) evaluates its row, writes
+//              each nonzero entry into its slot (slot values of a cell are
+//              contiguous) and a presence bit per slot; R is written here;
+//   2. count : one thread per column walks the rows that can reference it
+//              (the reverse slot table) in ascending row order and counts
+//              the present entries; an exclusive scan gives the column
+//              pointers;
+//   3. fill  : the same walk writes rowval/nzval of each column contiguously.
+// No atomics and no sort: rows come out ascending in every column.
+#include <algorithm>
+#include <vector>
+
 #include "ereb_internal.h"
 
 namespace ereb {
@@ -33,53 +39,72 @@ struct AsmArgs {
     // thermal
     const double *tk1, *RHOCP, *KX, *KY, *HR, *HA, *HS, *DHP;
     double* R;
-    int32_t *colcnt, *colptr, *rowval;
-    uint64_t* mask;
-    double* nzval;
 };
 
-// bit of row `row` among the candidate rows of column `col` (1-based),
-// NU unknowns per cell, Ny1 cells per grid column
-template <int NU>
-__device__ __forceinline__ int cand_bit(int64_t row, int64_t col, int Ny1) {
-    const int64_t cr = (row - 1) / NU, cc = (col - 1) / NU;
-    const int di = (int)(cr % Ny1) - (int)(cc % Ny1);
-    const int dj = (int)(cr / Ny1) - (int)(cc / Ny1);
-    return ((dj + 1) * 3 + (di + 1)) * NU + (int)((row - 1) % NU);
+// Stencil slot tables of one system.  A slot key encodes the column of an
+// entry relative to the row's cell: ((dcj + 1) * 3 + (dci + 1)) * NU + u, with
+// (dci, dcj) in {-1, 0, 1}^2 the column cell minus the row cell and u the
+// column's unknown.
+constexpr int SLOT_E = 6;    // equations (unknowns) per cell, at most
+constexpr int SLOT_S = 16;   // slots per equation, at most (presence bits in uint16)
+constexpr int SLOT_K = 54;   // keys 0 .. 9*NU-1
+constexpr int SLOT_R = 32;   // reverse entries per column unknown, at most
+struct SlotTables {
+    int ne, nstot;                       // equations per cell, slots per cell
+    int slot0[SLOT_E];                   // first slot of equation e within its cell
+    int ns[SLOT_E];                      // slots of equation e
+    int8_t lut[SLOT_E][SLOT_K];          // key -> slot of equation e, -1 if none
+    int nrev[SLOT_E];                    // reverse entries of column unknown u
+    int8_t rev_e[SLOT_E][SLOT_R], rev_s[SLOT_E][SLOT_R];
+    int8_t rev_di[SLOT_E][SLOT_R], rev_dj[SLOT_E][SLOT_R];  // column cell - row cell
+};
+
+// key of the column offset `off` = col - row of a row of equation e
+__host__ __device__ __forceinline__ int slot_key(int64_t off, int e, int nu, int Ny1) {
+    const int64_t t = off + e;  // column index relative to the first unknown of the row cell
+    const int64_t dcell = t >= 0 ? t / nu : -((-t + nu - 1) / nu);
+    const int u = (int)(t - dcell * nu);
+    const int64_t dcj = dcell >= 0 ? (dcell + 1) / Ny1 : -((-dcell + 1) / Ny1);
+    const int64_t dci = dcell - dcj * Ny1;
+    if (dci < -1 || dci > 1 || dcj < -1 || dcj > 1) return -1;
+    return (int)(((dcj + 1) * 3 + (dci + 1)) * nu + u);
 }
 
-template <int NU>
-struct MaskSink {
-    uint64_t* mask;
-    int Ny1;
+// Row sink: the entry (row, col) goes to its slot; the offset col - row is the
+// same for all lanes at every call site (the stencils are fixed), so it is
+// read from the first active lane and the table lookups are scalar; a lane
+// whose offset differs, or an offset outside the table, raises flag 1.
+struct SlotSink {
+    const SlotTables* __restrict__ T;
+    double* slotv;     // slots of the row's cell, equation e's first slot
+    uint32_t* pres;    // presence bits of the row (register)
+    int32_t* flags;
+    int e, nu, Ny1;
     __device__ void operator()(double v, int64_t row, int64_t col) const {
-        if (v != 0.0)
-            atomicOr((unsigned long long*)&mask[col - 1],
-                     1ull << cand_bit<NU>(row, col, Ny1));
-    }
-};
-
-template <int NU>
-struct FillSink {
-    const int32_t* colptr;
-    const uint64_t* mask;
-    int32_t* rowval;
-    double* nzval;
-    int Ny1;
-    __device__ void operator()(double v, int64_t row, int64_t col) const {
+        const int64_t off = col - row;
+        const int offu = __builtin_amdgcn_readfirstlane((int)off);
+        const int key = slot_key(offu, e, nu, Ny1);
+        const int sl = key < 0 ? -1 : (int)T->lut[e][key];
+        if (off != offu || sl < 0) {
+            flags[1] = 1;
+            return;
+        }
         if (v != 0.0) {
-            const int b = cand_bit<NU>(row, col, Ny1);
-            const int32_t p = colptr[col - 1] + (int32_t)__popcll(mask[col - 1] & ((1ull << b) - 1));
-            rowval[p] = (int32_t)(row - 1);
-            nzval[p] = v;
+            const uint32_t bit = 1u << sl;
+            if (*pres & bit)
+                slotv[sl] += v;
+            else
+                slotv[sl] = v;
+            *pres |= bit;
         }
     }
 };
 
-// One P cell (i, j) of assemble_hydromechanical_lse!; rows kvx..kpf.
-template <class Sink>
-__device__ __forceinline__ void stokes_cell(const AsmArgs& A, int i, int j, const Sink& add,
-                                            bool writeR) {
+// One equation EQ (0 Vx, 1 Vy, 2 P, 3 qx, 4 qy, 5 P fluid) of P cell (i, j)
+// of assemble_hydromechanical_lse!: row kvx + EQ.
+template <int EQ, class Sink>
+__host__ __device__ __forceinline__ void stokes_cell(const AsmArgs& A, int i, int j,
+                                                     const Sink& add, bool writeR) {
     const int Nx = A.Nx, Ny = A.Ny, Nx1 = A.Nx1, Ny1 = A.Ny1;
     const double dx = A.dx, dy = A.dy, dt = A.dt, Kcont = A.Kcont;
     auto B = [&](const double* a, int ii, int jj) { return a[(size_t)(jj - 1) * Ny + (ii - 1)]; };
@@ -87,164 +112,172 @@ __device__ __forceinline__ void stokes_cell(const AsmArgs& A, int i, int j, cons
     const int64_t kvx = ((int64_t)(j - 1) * Ny1 + i - 1) * 6 + 1;
     const int64_t kvy = kvx + 1, kpm = kvx + 2, kqx = kvx + 3, kqy = kvx + 4, kpf = kvx + 5;
     double rvx = 0.0, rvy = 0.0, rpm = 0.0, rqx = 0.0, rqy = 0.0, rpf = 0.0;
-    // Vx equation E:3128-3262
-    if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
-        add(1.0, kvx, kvx);
-        if (j == 1) rvx = A.vxleft;
-        if (j == Nx) rvx = A.vxright;
-        if (i == 1 && 1 < j && j < Nx) add(A.bctop, kvx, kvx + 6);
-        if (i == Ny1 && 1 < j && j < Nx) add(A.bcbottom, kvx, kvx - 6);
-    } else {
-        const double ETA1 = B(A.ETA, i - 1, j) * B(A.GGG, i - 1, j) * dt /
-                            (B(A.GGG, i - 1, j) * dt + B(A.ETA, i - 1, j));
-        const double ETA2 = B(A.ETA, i, j) * B(A.GGG, i, j) * dt /
-                            (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
-        const double ETAP1 = P(A.ETAP, i, j) * P(A.GGGP, i, j) * dt /
-                             (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
-        const double ETAP2 = P(A.ETAP, i, j + 1) * P(A.GGGP, i, j + 1) * dt /
-                             (P(A.GGGP, i, j + 1) * dt + P(A.ETAP, i, j + 1));
-        const double SXY1 = B(A.SXY0, i - 1, j) * B(A.ETA, i - 1, j) /
-                            (B(A.GGG, i - 1, j) * dt + B(A.ETA, i - 1, j));
-        const double SXY2 = B(A.SXY0, i, j) * B(A.ETA, i, j) / (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
-        const double SXX1 = P(A.SXX0, i, j) * P(A.ETAP, i, j) /
-                            (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
-        const double SXX2 = P(A.SXX0, i, j + 1) * P(A.ETAP, i, j + 1) /
-                            (P(A.GGGP, i, j + 1) * dt + P(A.ETAP, i, j + 1));
-        const double dRdx = 0.5 * (P(A.RHOX, i, j + 1) - P(A.RHOX, i, j - 1)) * (1.0 / dx);
-        const double dRdy = 0.5 * (P(A.RHOX, i + 1, j) - P(A.RHOX, i - 1, j)) * (1.0 / dy);
-        const double gxij = P(A.gx, i, j);
-        const double idx = 1.0 / dx, idy = 1.0 / dy;
-        add(ETAP1 / (dx * dx), kvx, kvx - 6 * Ny1);
-        add(ETA1 / (dy * dy), kvx, kvx - 6);
-        add(-(ETAP1 + ETAP2) * (1.0 / (dx * dx)) - (ETA1 + ETA2) * (1.0 / (dy * dy)) -
-                dRdx * gxij * dt,
-            kvx, kvx);
-        add(ETA2 / (dy * dy), kvx, kvx + 6);
-        add(ETAP2 / (dx * dx), kvx, kvx + 6 * Ny1);
-        add(ETAP1 * idx * idy - ETA2 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy);
-        add(-ETAP2 * idx * idy + ETA2 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy + 6 * Ny1);
-        add(-ETAP1 * idx * idy + ETA1 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy - 6);
-        add(ETAP2 * idx * idy - ETA1 * idx * idy - dRdy * gxij * dt * 0.25, kvx,
-            kvy + 6 * Ny1 - 6);
-        add(Kcont * idx, kvx, kpm);
-        add(-Kcont * idx, kvx, kpm + 6 * Ny1);
-        rvx = -P(A.RHOX, i, j) * gxij - (SXY2 - SXY1) * idy - (SXX2 - SXX1) * idx;
+    if (EQ == 0) {
+        // Vx equation E:3128-3262
+        if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
+            add(1.0, kvx, kvx);
+            if (j == 1) rvx = A.vxleft;
+            if (j == Nx) rvx = A.vxright;
+            if (i == 1 && 1 < j && j < Nx) add(A.bctop, kvx, kvx + 6);
+            if (i == Ny1 && 1 < j && j < Nx) add(A.bcbottom, kvx, kvx - 6);
+        } else {
+            const double ETA1 = B(A.ETA, i - 1, j) * B(A.GGG, i - 1, j) * dt /
+                                (B(A.GGG, i - 1, j) * dt + B(A.ETA, i - 1, j));
+            const double ETA2 = B(A.ETA, i, j) * B(A.GGG, i, j) * dt /
+                                (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
+            const double ETAP1 = P(A.ETAP, i, j) * P(A.GGGP, i, j) * dt /
+                                 (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
+            const double ETAP2 = P(A.ETAP, i, j + 1) * P(A.GGGP, i, j + 1) * dt /
+                                 (P(A.GGGP, i, j + 1) * dt + P(A.ETAP, i, j + 1));
+            const double SXY1 = B(A.SXY0, i - 1, j) * B(A.ETA, i - 1, j) /
+                                (B(A.GGG, i - 1, j) * dt + B(A.ETA, i - 1, j));
+            const double SXY2 = B(A.SXY0, i, j) * B(A.ETA, i, j) / (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
+            const double SXX1 = P(A.SXX0, i, j) * P(A.ETAP, i, j) /
+                                (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
+            const double SXX2 = P(A.SXX0, i, j + 1) * P(A.ETAP, i, j + 1) /
+                                (P(A.GGGP, i, j + 1) * dt + P(A.ETAP, i, j + 1));
+            const double dRdx = 0.5 * (P(A.RHOX, i, j + 1) - P(A.RHOX, i, j - 1)) * (1.0 / dx);
+            const double dRdy = 0.5 * (P(A.RHOX, i + 1, j) - P(A.RHOX, i - 1, j)) * (1.0 / dy);
+            const double gxij = P(A.gx, i, j);
+            const double idx = 1.0 / dx, idy = 1.0 / dy;
+            add(ETAP1 / (dx * dx), kvx, kvx - 6 * Ny1);
+            add(ETA1 / (dy * dy), kvx, kvx - 6);
+            add(-(ETAP1 + ETAP2) * (1.0 / (dx * dx)) - (ETA1 + ETA2) * (1.0 / (dy * dy)) -
+                    dRdx * gxij * dt,
+                kvx, kvx);
+            add(ETA2 / (dy * dy), kvx, kvx + 6);
+            add(ETAP2 / (dx * dx), kvx, kvx + 6 * Ny1);
+            add(ETAP1 * idx * idy - ETA2 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy);
+            add(-ETAP2 * idx * idy + ETA2 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy + 6 * Ny1);
+            add(-ETAP1 * idx * idy + ETA1 * idx * idy - dRdy * gxij * dt * 0.25, kvx, kvy - 6);
+            add(ETAP2 * idx * idy - ETA1 * idx * idy - dRdy * gxij * dt * 0.25, kvx,
+                kvy + 6 * Ny1 - 6);
+            add(Kcont * idx, kvx, kpm);
+            add(-Kcont * idx, kvx, kpm + 6 * Ny1);
+            rvx = -P(A.RHOX, i, j) * gxij - (SXY2 - SXY1) * idy - (SXX2 - SXX1) * idx;
+        }
     }
-    // Vy equation E:3264-3396
-    if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
-        add(1.0, kvy, kvy);
-        if (i == 1) rvy = A.vytop;
-        if (i == Ny) rvy = A.vybottom;
-        if (j == 1 && 1 < i && i < Ny) add(A.bcleft, kvy, kvy + 6 * Ny1);
-        if (j == Nx1 && 1 < i && i < Ny) add(A.bcright, kvy, kvy - 6 * Ny1);
-    } else {
-        const double ETA1 = B(A.ETA, i, j - 1) * B(A.GGG, i, j - 1) * dt /
-                            (B(A.GGG, i, j - 1) * dt + B(A.ETA, i, j - 1));
-        const double ETA2 = B(A.ETA, i, j) * B(A.GGG, i, j) * dt /
-                            (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
-        const double ETAP1 = P(A.ETAP, i, j) * P(A.GGGP, i, j) * dt /
-                             (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
-        const double ETAP2 = P(A.ETAP, i + 1, j) * P(A.GGGP, i + 1, j) * dt /
-                             (P(A.GGGP, i + 1, j) * dt + P(A.ETAP, i + 1, j));
-        const double SXY1 = B(A.SXY0, i, j - 1) * B(A.ETA, i, j - 1) /
-                            (B(A.GGG, i, j - 1) * dt + B(A.ETA, i, j - 1));
-        const double SXY2 = B(A.SXY0, i, j) * B(A.ETA, i, j) / (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
-        const double SYY1 = -P(A.SXX0, i, j) * P(A.ETAP, i, j) /
-                            (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
-        const double SYY2 = -P(A.SXX0, i + 1, j) * P(A.ETAP, i + 1, j) /
-                            (P(A.GGGP, i + 1, j) * dt + P(A.ETAP, i + 1, j));
-        const double dRdx = 0.5 * (P(A.RHOY, i, j + 1) - P(A.RHOY, i, j - 1)) / dx;
-        const double dRdy = 0.5 * (P(A.RHOY, i + 1, j) - P(A.RHOY, i - 1, j)) / dy;
-        const double gyij = P(A.gy, i, j);
-        const double idx = 1.0 / dx, idy = 1.0 / dy;
-        add(ETA1 / (dx * dx), kvy, kvy - 6 * Ny1);
-        add(ETAP1 / (dy * dy), kvy, kvy - 6);
-        add(-(ETAP1 + ETAP2) * (1.0 / (dy * dy)) - (ETA1 + ETA2) * (1.0 / (dx * dx)) -
-                dRdy * gyij * dt,
-            kvy, kvy);
-        add(ETAP2 * (1.0 / (dy * dy)), kvy, kvy + 6);
-        add(ETA2 * (1.0 / (dx * dx)), kvy, kvy + 6 * Ny1);
-        add(ETAP1 * idx * idy - ETA2 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx);
-        add(-ETAP2 * idx * idy + ETA2 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx + 6);
-        add(-ETAP1 * idx * idy + ETA1 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx - 6 * Ny1);
-        add(ETAP2 * idx * idy - ETA1 * idx * idy - dRdx * gyij * dt * 0.25, kvy,
-            kvx + 6 - 6 * Ny1);
-        add(Kcont * idy, kvy, kpm);
-        add(-Kcont * idy, kvy, kpm + 6);
-        rvy = -P(A.RHOY, i, j) * gyij - (SXY2 - SXY1) * idx - (SYY2 - SYY1) * idy;
+    if (EQ == 1) {
+        // Vy equation E:3264-3396
+        if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
+            add(1.0, kvy, kvy);
+            if (i == 1) rvy = A.vytop;
+            if (i == Ny) rvy = A.vybottom;
+            if (j == 1 && 1 < i && i < Ny) add(A.bcleft, kvy, kvy + 6 * Ny1);
+            if (j == Nx1 && 1 < i && i < Ny) add(A.bcright, kvy, kvy - 6 * Ny1);
+        } else {
+            const double ETA1 = B(A.ETA, i, j - 1) * B(A.GGG, i, j - 1) * dt /
+                                (B(A.GGG, i, j - 1) * dt + B(A.ETA, i, j - 1));
+            const double ETA2 = B(A.ETA, i, j) * B(A.GGG, i, j) * dt /
+                                (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
+            const double ETAP1 = P(A.ETAP, i, j) * P(A.GGGP, i, j) * dt /
+                                 (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
+            const double ETAP2 = P(A.ETAP, i + 1, j) * P(A.GGGP, i + 1, j) * dt /
+                                 (P(A.GGGP, i + 1, j) * dt + P(A.ETAP, i + 1, j));
+            const double SXY1 = B(A.SXY0, i, j - 1) * B(A.ETA, i, j - 1) /
+                                (B(A.GGG, i, j - 1) * dt + B(A.ETA, i, j - 1));
+            const double SXY2 = B(A.SXY0, i, j) * B(A.ETA, i, j) / (B(A.GGG, i, j) * dt + B(A.ETA, i, j));
+            const double SYY1 = -P(A.SXX0, i, j) * P(A.ETAP, i, j) /
+                                (P(A.GGGP, i, j) * dt + P(A.ETAP, i, j));
+            const double SYY2 = -P(A.SXX0, i + 1, j) * P(A.ETAP, i + 1, j) /
+                                (P(A.GGGP, i + 1, j) * dt + P(A.ETAP, i + 1, j));
+            const double dRdx = 0.5 * (P(A.RHOY, i, j + 1) - P(A.RHOY, i, j - 1)) / dx;
+            const double dRdy = 0.5 * (P(A.RHOY, i + 1, j) - P(A.RHOY, i - 1, j)) / dy;
+            const double gyij = P(A.gy, i, j);
+            const double idx = 1.0 / dx, idy = 1.0 / dy;
+            add(ETA1 / (dx * dx), kvy, kvy - 6 * Ny1);
+            add(ETAP1 / (dy * dy), kvy, kvy - 6);
+            add(-(ETAP1 + ETAP2) * (1.0 / (dy * dy)) - (ETA1 + ETA2) * (1.0 / (dx * dx)) -
+                    dRdy * gyij * dt,
+                kvy, kvy);
+            add(ETAP2 * (1.0 / (dy * dy)), kvy, kvy + 6);
+            add(ETA2 * (1.0 / (dx * dx)), kvy, kvy + 6 * Ny1);
+            add(ETAP1 * idx * idy - ETA2 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx);
+            add(-ETAP2 * idx * idy + ETA2 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx + 6);
+            add(-ETAP1 * idx * idy + ETA1 * idx * idy - dRdx * gyij * dt * 0.25, kvy, kvx - 6 * Ny1);
+            add(ETAP2 * idx * idy - ETA1 * idx * idy - dRdx * gyij * dt * 0.25, kvy,
+                kvx + 6 - 6 * Ny1);
+            add(Kcont * idy, kvy, kpm);
+            add(-Kcont * idy, kvy, kpm + 6);
+            rvy = -P(A.RHOY, i, j) * gyij - (SXY2 - SXY1) * idx - (SYY2 - SYY1) * idy;
+        }
     }
-    // P equation E:3398-3452
+    // ghost and pressure-anchor cells of both continuity equations
     const bool anchor = (i == 2 && 2 <= j && j <= Nx) || (j == 2 && 2 < i && i < Ny) ||
                         (i == Ny && 2 <= j && j <= Nx) || (j == Nx && 2 < i && i < Ny);
     const bool pedge = i == 1 || i == Ny1 || j == 1 || j == Nx1;
-    if (pedge) {
-        add(1.0, kpm, kpm);
-    } else if (anchor) {
-        add(Kcont, kpm, kpm);
-        rpm = A.psurface;
-    } else {
-        const double phi = P(A.PHI, i, j), ep = P(A.ETAPHI, i, j), bp = P(A.BETAPHI, i, j);
-        add(-1.0 / dx, kpm, kvx - 6 * Ny1);
-        add(1.0 / dx, kpm, kvx);
-        add(-1.0 / dy, kpm, kvy - 6);
-        add(1.0 / dy, kpm, kvy);
-        add(Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpm);
-        add(-Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpf);
-        rpm = (P(A.pr0, i, j) - P(A.pf0, i, j)) / (1.0 - phi) * bp / dt + P(A.DMP, i, j);
+    if (EQ == 2) {
+        // P equation E:3398-3452
+        if (pedge) {
+            add(1.0, kpm, kpm);
+        } else if (anchor) {
+            add(Kcont, kpm, kpm);
+            rpm = A.psurface;
+        } else {
+            const double phi = P(A.PHI, i, j), ep = P(A.ETAPHI, i, j), bp = P(A.BETAPHI, i, j);
+            add(-1.0 / dx, kpm, kvx - 6 * Ny1);
+            add(1.0 / dx, kpm, kvx);
+            add(-1.0 / dy, kpm, kvy - 6);
+            add(1.0 / dy, kpm, kvy);
+            add(Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpm);
+            add(-Kcont / (1.0 - phi) * ((1.0 / ep) + bp / dt), kpm, kpf);
+            rpm = (P(A.pr0, i, j) - P(A.pf0, i, j)) / (1.0 - phi) * bp / dt + P(A.DMP, i, j);
+        }
     }
-    // qx Darcy equation E:3454-3480
-    if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
-        add(1.0, kqx, kqx);
-        if (i == 1 && 1 < j && j < Nx) add(A.bcftop, kqx, kqx + 6);
-        if (i == Ny1 && 1 < j && j < Nx) add(A.bcfbottom, kqx, kqx - 6);
-    } else {
-        add(P(A.RX, i, j), kqx, kqx);
-        add(-Kcont * (1.0 / dx), kqx, kpf);
-        add(Kcont * (1.0 / dx), kqx, kpf + 6 * Ny1);
-        rqx = P(A.RHOFX, i, j) * P(A.gx, i, j);
+    if (EQ == 3) {
+        // qx Darcy equation E:3454-3480
+        if (i == 1 || i == Ny1 || j == 1 || j == Nx || j == Nx1) {
+            add(1.0, kqx, kqx);
+            if (i == 1 && 1 < j && j < Nx) add(A.bcftop, kqx, kqx + 6);
+            if (i == Ny1 && 1 < j && j < Nx) add(A.bcfbottom, kqx, kqx - 6);
+        } else {
+            add(P(A.RX, i, j), kqx, kqx);
+            add(-Kcont * (1.0 / dx), kqx, kpf);
+            add(Kcont * (1.0 / dx), kqx, kpf + 6 * Ny1);
+            rqx = P(A.RHOFX, i, j) * P(A.gx, i, j);
+        }
     }
-    // qy Darcy equation E:3482-3514
-    if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
-        add(1.0, kqy, kqy);
-        if (j == 1 && 1 < i && i < Ny) add(A.bcfleft, kqy, kqy + 6 * Ny1);
-        if (j == Nx1 && 1 < i && i < Ny) add(A.bcfright, kqy, kqy - 6 * Ny1);
-    } else {
-        add(P(A.RY, i, j), kqy, kqy);
-        add(-Kcont * (1.0 / dy), kqy, kpf);
-        add(Kcont * (1.0 / dy), kqy, kpf + 6);
-        rqy = P(A.RHOFY, i, j) * P(A.gy, i, j);
+    if (EQ == 4) {
+        // qy Darcy equation E:3482-3514
+        if (i == 1 || i == Ny || i == Ny1 || j == 1 || j == Nx1) {
+            add(1.0, kqy, kqy);
+            if (j == 1 && 1 < i && i < Ny) add(A.bcfleft, kqy, kqy + 6 * Ny1);
+            if (j == Nx1 && 1 < i && i < Ny) add(A.bcfright, kqy, kqy - 6 * Ny1);
+        } else {
+            add(P(A.RY, i, j), kqy, kqy);
+            add(-Kcont * (1.0 / dy), kqy, kpf);
+            add(Kcont * (1.0 / dy), kqy, kpf + 6);
+            rqy = P(A.RHOFY, i, j) * P(A.gy, i, j);
+        }
     }
-    // P fluid equation E:3516-3565
-    if (pedge) {
-        add(1.0, kpf, kpf);
-    } else if (anchor) {
-        add(Kcont, kpf, kpf);
-        rpf = A.psurface;
-    } else {
-        const double phi = P(A.PHI, i, j), ep = P(A.ETAPHI, i, j), bp = P(A.BETAPHI, i, j);
-        add(-(1.0 / dx), kpf, kqx - 6 * Ny1);
-        add((1.0 / dx), kpf, kqx);
-        add(-(1.0 / dy), kpf, kqy - 6);
-        add((1.0 / dy), kpf, kqy);
-        add(-Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpm);
-        add(Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpf);
-        rpf = -(P(A.pr0, i, j) - P(A.pf0, i, j)) / (1.0 - phi) * bp / dt;
+    if (EQ == 5) {
+        // P fluid equation E:3516-3565
+        if (pedge) {
+            add(1.0, kpf, kpf);
+        } else if (anchor) {
+            add(Kcont, kpf, kpf);
+            rpf = A.psurface;
+        } else {
+            const double phi = P(A.PHI, i, j), ep = P(A.ETAPHI, i, j), bp = P(A.BETAPHI, i, j);
+            add(-(1.0 / dx), kpf, kqx - 6 * Ny1);
+            add((1.0 / dx), kpf, kqx);
+            add(-(1.0 / dy), kpf, kqy - 6);
+            add((1.0 / dy), kpf, kqy);
+            add(-Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpm);
+            add(Kcont / (1.0 - phi) * (1.0 / ep + bp / dt), kpf, kpf);
+            rpf = -(P(A.pr0, i, j) - P(A.pf0, i, j)) / (1.0 - phi) * bp / dt;
+        }
     }
     if (writeR) {
-        double* R = A.R + (kvx - 1);
-        R[0] = rvx;
-        R[1] = rvy;
-        R[2] = rpm;
-        R[3] = rqx;
-        R[4] = rqy;
-        R[5] = rpf;
+        const double r[6] = {rvx, rvy, rpm, rqx, rqy, rpf};
+        A.R[kvx - 1 + EQ] = r[EQ];
     }
 }
 
 // One P cell of assemble_thermal_lse! (E:4534-4605)
 template <class Sink>
-__device__ __forceinline__ void thermal_cell(const AsmArgs& A, int i, int j, const Sink& add,
-                                             bool writeR) {
+__host__ __device__ __forceinline__ void thermal_cell(const AsmArgs& A, int i, int j,
+                                                      const Sink& add, bool writeR) {
     const int Nx1 = A.Nx1, Ny1 = A.Ny1;
     const double dx = A.dx, dy = A.dy, dt = A.dt;
     auto P = [&](const double* a, int ii, int jj) { return a[(size_t)(jj - 1) * Ny1 + (ii - 1)]; };
@@ -271,37 +304,159 @@ __device__ __forceinline__ void thermal_cell(const AsmArgs& A, int i, int j, con
     if (writeR) A.R[gk - 1] = rt;
 }
 
+template <class Sink>
+__host__ __device__ __forceinline__ void stokes_eq(const AsmArgs& A, int e, int i, int j,
+                                                   const Sink& add, bool writeR) {
+    switch (e) {
+        case 0: stokes_cell<0>(A, i, j, add, writeR); break;
+        case 1: stokes_cell<1>(A, i, j, add, writeR); break;
+        case 2: stokes_cell<2>(A, i, j, add, writeR); break;
+        case 3: stokes_cell<3>(A, i, j, add, writeR); break;
+        case 4: stokes_cell<4>(A, i, j, add, writeR); break;
+        default: stokes_cell<5>(A, i, j, add, writeR); break;
+    }
+}
+
+// 1. rows: thread per P cell, blockIdx.y = equation (wave-uniform)
 template <int SYS>
-__global__ __launch_bounds__(256) void k_asm_mask(AsmArgs A) {
+__global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A, const SlotTables* __restrict__ T,
+                                                  uint16_t* __restrict__ pres_out,
+                                                  double* __restrict__ slotv,
+                                                  int32_t* __restrict__ flags) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A.Nx1 * A.Ny1) return;
+    const int e = blockIdx.y;
     const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
+    uint32_t pres = 0;
+    SlotSink sink{T, slotv + (size_t)k * T->nstot + T->slot0[e], &pres, flags, e,
+                  SYS == 0 ? 6 : 1, A.Ny1};
     if (SYS == 0)
-        stokes_cell(A, i, j, MaskSink<6>{A.mask, A.Ny1}, true);
+        stokes_eq(A, e, i, j, sink, true);
     else
-        thermal_cell(A, i, j, MaskSink<1>{A.mask, A.Ny1}, true);
+        thermal_cell(A, i, j, sink, true);
+    pres_out[(size_t)k * T->ne + e] = (uint16_t)pres;
 }
 
-__global__ __launch_bounds__(256) void k_mask_count(const uint64_t* __restrict__ mask, int64_t n,
-                                                    int32_t* __restrict__ cnt) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n) cnt[c] = (int32_t)__popcll(mask[c]);
+// 2./3. columns: thread per P cell, blockIdx.y = the column's unknown u
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_asm_cols(int Nx1, int Ny1,
+                                                  const SlotTables* __restrict__ T,
+                                                  const uint16_t* __restrict__ pres,
+                                                  const double* __restrict__ slotv,
+                                                  int32_t* __restrict__ colcnt,
+                                                  const int32_t* __restrict__ colptr,
+                                                  int32_t* __restrict__ rowval,
+                                                  double* __restrict__ nzval) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Nx1 * Ny1) return;
+    const int u = blockIdx.y, ne = T->ne, nstot = T->nstot;
+    const int ic = k % Ny1, jc = k / Ny1;
+    const int64_t col = (int64_t)k * ne + u;
+    int32_t p = FILL ? colptr[col] : 0;
+    const int nr = T->nrev[u];
+    for (int r = 0; r < nr; ++r) {
+        const int ir = ic - T->rev_di[u][r], jr = jc - T->rev_dj[u][r];
+        if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
+        const int e = T->rev_e[u][r], sl = T->rev_s[u][r];
+        const int64_t cell = (int64_t)jr * Ny1 + ir;
+        if (!((pres[cell * ne + e] >> sl) & 1)) continue;
+        if (FILL) {
+            rowval[p] = (int32_t)(cell * ne + e);
+            nzval[p] = slotv[cell * nstot + T->slot0[e] + sl];
+        }
+        ++p;
+    }
+    if (!FILL) colcnt[col] = p;
 }
 
-__global__ __launch_bounds__(256) void k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n) p[c] = 0;
-}
+// ---- slot tables, discovered on the host from the equation code -----------
+
+struct RecordSink {
+    std::vector<int>* keys;  // keys of equation e in first-use order
+    int e, nu, Ny1;
+    bool* bad;
+    void operator()(double, int64_t row, int64_t col) const {
+        const int key = slot_key(col - row, e, nu, Ny1);
+        if (key < 0) {
+            *bad = true;
+            return;
+        }
+        for (int k : *keys)
+            if (k == key) return;
+        keys->push_back(key);
+    }
+};
 
 template <int SYS>
-__global__ __launch_bounds__(256) void k_asm_fill(AsmArgs A) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.Nx1 * A.Ny1) return;
-    const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
-    if (SYS == 0)
-        stokes_cell(A, i, j, FillSink<6>{A.colptr, A.mask, A.rowval, A.nzval, A.Ny1}, false);
-    else
-        thermal_cell(A, i, j, FillSink<1>{A.colptr, A.mask, A.rowval, A.nzval, A.Ny1}, false);
+static bool discover_slots(SlotTables& T) {
+    // a small grid with every boundary, anchor and interior case; the
+    // recording sink keeps every add, zero or not
+    const int Nx = 9, Ny = 9, Nx1 = 10, Ny1 = 10;
+    std::vector<double> one((size_t)Nx1 * Ny1, 1.0), rhs((size_t)Nx1 * Ny1 * 6, 0.0);
+    AsmArgs A{};
+    A.Nx = Nx; A.Ny = Ny; A.Nx1 = Nx1; A.Ny1 = Ny1;
+    A.dx = A.dy = A.dt = A.Kcont = 1.0;
+    const double* o = one.data();
+    A.ETA = A.ETAP = A.GGG = A.GGGP = A.SXY0 = A.SXX0 = A.RHOX = A.RHOY = o;
+    A.RHOFX = A.RHOFY = A.RX = A.RY = A.ETAPHI = A.BETAPHI = A.PHI = o;
+    A.gx = A.gy = A.pr0 = A.pf0 = A.DMP = o;
+    A.tk1 = A.RHOCP = A.KX = A.KY = A.HR = A.HA = A.HS = A.DHP = o;
+    std::vector<double> phi((size_t)Nx1 * Ny1, 0.5);
+    A.PHI = phi.data();
+    A.R = rhs.data();
+    const int ne = SYS == 0 ? 6 : 1, nu = ne;
+    std::vector<int> keys[SLOT_E];
+    bool bad = false;
+    for (int e = 0; e < ne; ++e) {
+        RecordSink rec{&keys[e], e, nu, Ny1, &bad};
+        for (int j = 1; j <= Nx1; ++j)
+            for (int i = 1; i <= Ny1; ++i) {
+                if (SYS == 0)
+                    stokes_eq(A, e, i, j, rec, false);
+                else
+                    thermal_cell(A, i, j, rec, false);
+            }
+    }
+    if (bad) return false;
+    T = SlotTables{};
+    T.ne = ne;
+    int tot = 0;
+    for (int e = 0; e < ne; ++e) {
+        if ((int)keys[e].size() > SLOT_S) return false;
+        T.slot0[e] = tot;
+        T.ns[e] = (int)keys[e].size();
+        tot += T.ns[e];
+        for (int k = 0; k < SLOT_K; ++k) T.lut[e][k] = -1;
+        for (int sl = 0; sl < T.ns[e]; ++sl) T.lut[e][keys[e][sl]] = (int8_t)sl;
+    }
+    T.nstot = tot;
+    // reverse table: rows referencing a column of unknown u, ascending row
+    // order = (row cell column, row cell row, equation) ascending
+    struct Rev { int dj, di, e, sl; };
+    for (int u = 0; u < nu; ++u) {
+        std::vector<Rev> r;
+        for (int e = 0; e < ne; ++e)
+            for (int sl = 0; sl < T.ns[e]; ++sl) {
+                const int key = keys[e][sl];
+                if (key % nu != u) continue;
+                const int dci = (key / nu) % 3 - 1, dcj = key / (3 * nu) - 1;
+                r.push_back(Rev{dcj, dci, e, sl});
+            }
+        std::sort(r.begin(), r.end(), [](const Rev& a, const Rev& b) {
+            if (a.dj != b.dj) return a.dj > b.dj;  // row cell = column cell - offset
+            if (a.di != b.di) return a.di > b.di;
+            return a.e < b.e;
+        });
+        if ((int)r.size() > SLOT_R) return false;
+        T.nrev[u] = (int)r.size();
+        for (size_t k = 0; k < r.size(); ++k) {
+            T.rev_e[u][k] = (int8_t)r[k].e;
+            T.rev_s[u][k] = (int8_t)r[k].sl;
+            T.rev_di[u][k] = (int8_t)r[k].di;
+            T.rev_dj[u][k] = (int8_t)r[k].dj;
+        }
+    }
+    return true;
 }
 
 static AsmArgs asm_args(ereb_ctx* c, double dt, Sys& s) {
@@ -331,33 +486,39 @@ static AsmArgs asm_args(ereb_ctx* c, double dt, Sys& s) {
     A.KY = gptr(c, EREB_G_KY); A.HR = gptr(c, EREB_G_HR); A.HA = gptr(c, EREB_G_HA);
     A.HS = gptr(c, EREB_G_HS); A.DHP = gptr(c, EREB_G_DHP);
     A.R = s.rhs;
-    A.colcnt = s.colcnt;
-    A.colptr = s.colptr;
-    A.mask = s.mask;
-    A.rowval = s.rowval;
-    A.nzval = s.nzval;
     return A;
 }
 
 template <int SYS>
 static int assemble(ereb_ctx* c, double dt) {
-    const int64_t ncell = (int64_t)c->Nx1 * c->Ny1;
-    // at most 40 entries per P cell (Stokes-Darcy: 11+11+6+3+3+6), 5 (thermal)
-    const int64_t n = SYS == 0 ? 6 * ncell : ncell;
-    const int64_t cap = SYS == 0 ? 40 * ncell : 5 * ncell;
-    int rc = ensure_sys(c, SYS, n, cap);
-    if (rc) return rc;
     Sys& s = c->sys[SYS];
+    if (!s.tables) {
+        SlotTables T;
+        if (!discover_slots<SYS>(T))
+            return fail(c, EREB_ERR_STATE, "assembly: stencil slot discovery failed");
+        EREB_HIP(c, hipMalloc(&s.tables, sizeof(SlotTables)));
+        EREB_HIP(c, hipMemcpy(s.tables, &T, sizeof T, hipMemcpyHostToDevice));
+        s.ne = T.ne;
+        s.nstot = T.nstot;
+    }
+    const int64_t ncell = (int64_t)c->Nx1 * c->Ny1;
+    const int64_t n = ncell * s.ne;
+    const int64_t cap = ncell * s.nstot;  // every slot present
+    int rc = ensure_sys(c, SYS, n, cap, ncell);
+    if (rc) return rc;
     AsmArgs A = asm_args(c, dt, s);
+    const SlotTables* T = (const SlotTables*)s.tables;
     hipEvent_t ev;
     prof_begin(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, &ev);
-    const unsigned ncol = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u64, dim3(ncol), dim3(256), 0, c->stream, s.mask, n);
-    const unsigned nb = (unsigned)((ncell + 255) / 256);
-    hipLaunchKernelGGL(k_asm_mask<SYS>, dim3(nb), dim3(256), 0, c->stream, A);
-    hipLaunchKernelGGL(k_mask_count, dim3(ncol), dim3(256), 0, c->stream, s.mask, n, s.colcnt);
+    const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
+    hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, T, s.pres, s.slotv,
+                       c->d_flags);
+    hipLaunchKernelGGL(k_asm_cols<false>, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
+                       s.pres, s.slotv, s.colcnt, (const int32_t*)nullptr, (int32_t*)nullptr,
+                       (double*)nullptr);
     if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
-    hipLaunchKernelGGL(k_asm_fill<SYS>, dim3(nb), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_asm_cols<true>, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
+                       s.pres, s.slotv, s.colcnt, s.colptr, s.rowval, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;

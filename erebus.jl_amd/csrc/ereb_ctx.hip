@@ -47,27 +47,54 @@ bool grid_is_basic(int f) {
 }
 
 double* gptr(ereb_ctx* c, int f) { return c->d_grids + c->goff[f]; }
+// In the mixed-precision form the 16 material and derived property vectors
+// are stored as float; the state (xm, ym, tkm, phim, sxxm, sxym) stays
+// double in both forms and tm is int32.
+bool marker_is_f32(int precision, int f) {
+    if (precision != EREB_MIXED) return false;
+    return f == EREB_M_XWSOLIDM0 || (f >= EREB_M_ETAVPM && f <= EREB_M_ETAFLUIDCUR_INV_KPHIM);
+}
+size_t marker_es(int precision, int f) {
+    return f == EREB_M_TM ? sizeof(int32_t)
+                          : (marker_is_f32(precision, f) ? sizeof(float) : sizeof(double));
+}
 double* mptr(ereb_ctx* c, int f) { return (double*)c->d_m[f]; }
+float* fptr(ereb_ctx* c, int f) { return (float*)c->d_m[f]; }
 int32_t* tm_ptr(ereb_ctx* c) { return (int32_t*)c->d_m[EREB_M_TM]; }
 
-int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap) {
+__global__ void k_cvt_f64_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_cvt_f32_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell) {
     Sys& s = c->sys[which];
-    if (s.n == n && s.cap >= cap) return EREB_OK;
+    if (s.n == n && s.cap >= cap && s.ncell == ncell) return EREB_OK;
     hipFree(s.colcnt);
     hipFree(s.colptr);
-    hipFree(s.mask);
+    hipFree(s.pres);
+    hipFree(s.slotv);
     hipFree(s.rowval);
     hipFree(s.nzval);
     hipFree(s.rhs);
-    s = Sys();
+    s.colcnt = s.colptr = s.rowval = nullptr;
+    s.pres = nullptr;
+    s.slotv = s.nzval = s.rhs = nullptr;
+    s.valid = false;
     EREB_HIP(c, hipMalloc(&s.colcnt, sizeof(int32_t) * n));
     EREB_HIP(c, hipMalloc(&s.colptr, sizeof(int32_t) * (n + 1)));
-    EREB_HIP(c, hipMalloc(&s.mask, sizeof(uint64_t) * n));
+    EREB_HIP(c, hipMalloc(&s.pres, sizeof(uint16_t) * n));
+    EREB_HIP(c, hipMalloc(&s.slotv, sizeof(double) * ncell * s.nstot));
     EREB_HIP(c, hipMalloc(&s.rowval, sizeof(int32_t) * cap));
     EREB_HIP(c, hipMalloc(&s.nzval, sizeof(double) * cap));
     EREB_HIP(c, hipMalloc(&s.rhs, sizeof(double) * n));
     s.n = n;
     s.cap = cap;
+    s.ncell = ncell;
     return EREB_OK;
 }
 
@@ -85,9 +112,13 @@ int check_flags(ereb_ctx* c) {
     int32_t f[4];
     EREB_HIP(c, hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost, c->stream));
     EREB_HIP(c, hipStreamSynchronize(c->stream));
-    if (f[0]) {
+    if (f[0] || f[1]) {
         int32_t z[4] = {0, 0, 0, 0};
         EREB_HIP(c, hipMemcpy(c->d_flags, z, sizeof z, hipMemcpyHostToDevice));
+        if (f[1])
+            return fail(c, EREB_ERR_STATE,
+                        "assembly: an equation referenced a column outside its stencil "
+                        "slot table");
         return fail(c, EREB_ERR_DOMAIN,
                     "marker coordinate outside the supported range (finite |x| or |y| "
                     "beyond about 1e15 m puts its P cell outside the basic-cell tile)");
@@ -149,7 +180,7 @@ static int reserve(ereb_ctx* c, int64_t n) {
     if (n > (int64_t)INT32_MAX) return fail(c, EREB_ERR_ARG, "marknum exceeds 2^31-1");
     EREB_HIP(c, hipStreamSynchronize(c->stream));
     for (int f = 0; f < EREB_M_NFIELDS; ++f) {
-        size_t es = f == EREB_M_TM ? sizeof(int32_t) : sizeof(double);
+        const size_t es = marker_es(c->precision, f);
         void* p = nullptr;
         EREB_HIP(c, hipMalloc(&p, es * n));
         if (c->d_m[f]) {
@@ -233,6 +264,8 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     ax.insert(ax.end(), g->y, g->y + Ny);
     ax.insert(ax.end(), g->xp, g->xp + Nx1);
     ax.insert(ax.end(), g->yp, g->yp + Ny1);
+    CK(hipMalloc(&c->d_mat, sizeof(ereb_materials)));
+    CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
     CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
     // grids, 256-byte aligned
@@ -276,6 +309,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (auto e : c->ev_pool) hipEventDestroy(e);
     for (int f = 0; f < EREB_M_NFIELDS; ++f) hipFree(c->d_m[f]);
     hipFree(c->d_axes);
+    hipFree(c->d_mat);
     hipFree(c->d_grids);
     hipFree(c->d_yny);
     hipFree(c->d_key);
@@ -291,10 +325,12 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (auto& s : c->sys) {
         hipFree(s.colcnt);
         hipFree(s.colptr);
-        hipFree(s.mask);
+        hipFree(s.pres);
+        hipFree(s.slotv);
         hipFree(s.rowval);
         hipFree(s.nzval);
         hipFree(s.rhs);
+        hipFree(s.tables);
     }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -305,6 +341,24 @@ int ereb_ctx_set_precision(ereb_ctx* c, int precision) {
     if (!c) return EREB_ERR_ARG;
     if (precision != EREB_FP64 && precision != EREB_MIXED)
         return fail(c, EREB_ERR_ARG, "precision must be EREB_FP64 or EREB_MIXED");
+    if (precision == c->precision) return EREB_OK;
+    // convert the resident property vectors between float and double
+    for (int f = 0; f < EREB_M_NFIELDS && c->cap > 0; ++f) {
+        if (marker_is_f32(precision, f) == marker_is_f32(c->precision, f)) continue;
+        void* p = nullptr;
+        EREB_HIP(c, hipMalloc(&p, marker_es(precision, f) * c->cap));
+        const unsigned nb = (unsigned)((c->cap + 255) / 256);
+        if (marker_is_f32(precision, f))
+            hipLaunchKernelGGL(k_cvt_f64_f32, dim3(nb), dim3(256), 0, c->stream,
+                               (const double*)c->d_m[f], (float*)p, c->cap);
+        else
+            hipLaunchKernelGGL(k_cvt_f32_f64, dim3(nb), dim3(256), 0, c->stream,
+                               (const float*)c->d_m[f], (double*)p, c->cap);
+        EREB_HIP(c, hipGetLastError());
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+        hipFree(c->d_m[f]);
+        c->d_m[f] = p;
+    }
     c->precision = precision;
     return EREB_OK;
 }
@@ -339,6 +393,13 @@ int ereb_markers_set(ereb_ctx* c, int field, const void* host, int64_t marknum) 
         EREB_HIP(c, hipMemcpyAsync(c->d_m[field], t32.data(), sizeof(int32_t) * marknum,
                                    hipMemcpyHostToDevice, c->stream));
         EREB_HIP(c, hipStreamSynchronize(c->stream));
+    } else if (marker_is_f32(c->precision, field) && marknum > 0) {
+        const double* h = (const double*)host;
+        std::vector<float> f32((size_t)marknum);
+        for (int64_t m = 0; m < marknum; ++m) f32[m] = (float)h[m];
+        EREB_HIP(c, hipMemcpyAsync(c->d_m[field], f32.data(), sizeof(float) * marknum,
+                                   hipMemcpyHostToDevice, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
     } else if (marknum > 0) {
         EREB_HIP(c, hipMemcpyAsync(c->d_m[field], host, sizeof(double) * marknum,
                                    hipMemcpyHostToDevice, c->stream));
@@ -363,6 +424,13 @@ int ereb_markers_get(ereb_ctx* c, int field, void* host, int64_t marknum) {
         EREB_HIP(c, hipStreamSynchronize(c->stream));
         int64_t* t = (int64_t*)host;
         for (int64_t m = 0; m < marknum; ++m) t[m] = t32[m];
+    } else if (marker_is_f32(c->precision, field)) {
+        std::vector<float> f32((size_t)marknum);
+        EREB_HIP(c, hipMemcpyAsync(f32.data(), c->d_m[field], sizeof(float) * marknum,
+                                   hipMemcpyDeviceToHost, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+        double* h = (double*)host;
+        for (int64_t m = 0; m < marknum; ++m) h[m] = (double)f32[m];
     } else {
         EREB_HIP(c, hipMemcpyAsync(host, c->d_m[field], sizeof(double) * marknum,
                                    hipMemcpyDeviceToHost, c->stream));

@@ -151,6 +151,162 @@ __device__ __forceinline__ Props marker_props(const ereb_materials& M, int t, do
     return p;
 }
 
+// ---- derivatives of the marker properties (adjoint kernels) -------------
+// d/dT of the piecewise fluid heat capacity, conductivities (mode 1; mode 9
+// uses constants).  The property switches at the phase temperatures are
+// steps: their derivative is zero away from the switch.
+
+__device__ __forceinline__ double d_rhocpfluidm(const ereb_materials& M, double T, int mode) {
+    if (mode == 9) return 0.0;
+    const double tf = M.tmfluidphase;
+    if (T < tf) return M.rhoH2Ofi * 7.67;
+    if (T < 410.0) return 0.0;
+    return M.rhoH2Of * (333.0 - 2.0 * 0.731 * T + 3.0 * 5.4e-4 * (T * T));
+}
+
+__device__ __forceinline__ double d_ksolidm(double T, int mode) {
+    if (mode == 9) return 0.0;
+    const double u = T + 77.0;
+    return -1293.0 / (u * u);
+}
+
+__device__ __forceinline__ double d_kfluidm(const ereb_materials& M, double T, int mode) {
+    if (mode == 9) return 0.0;
+    if (T < M.tmfluidphase) return -488.0 / (T * T);
+    if (T < 410.0) return 6.34e-3 - 2.0 * 7.93e-6 * T;
+    return 4.12e-3 - 2.0 * 5.01e-6 * T;
+}
+
+// partial derivatives of the properties that depend on T or phi
+struct PropsD {
+    double rho_p;                  // d rhototal / d phi (d/dT is zero: step in T)
+    double rhocp_t, rhocp_p;       // rhocptotal
+    double hr_p;                   // hrtotal
+    double k_t, k_p;               // ktotal
+    double tk_t, tk_p;             // tkm_rhocptotalm = T * rhocptotal
+    double ef_p;                   // etafluidcur_inv_kphim
+};
+
+__device__ __forceinline__ PropsD marker_props_d(const ereb_materials& M, int t, double T,
+                                                 double phi, double XW, const double (&hrs)[3],
+                                                 const double (&hrf)[3], int mode,
+                                                 const Props& p) {
+    PropsD d{};
+    if (t < 3) {
+        const double XD = 1.0 - XW;
+        const double rhosolidm0 = (M.MD + M.MH2O * XW) / (M.VDs * XD + M.VWs * XW);
+        const double rhofluidm0 = T > M.tmfluidphase ? M.rhoH2Of : M.rhoH2Ofi;
+        d.rho_p = rhofluidm0 - rhosolidm0;
+        const double rcs = sel3(M.rhocpsolidm, t), rcf = rhocpfluidm(M, T, mode);
+        d.rhocp_t = phi * d_rhocpfluidm(M, T, mode);
+        d.rhocp_p = rcf - rcs;
+        d.hr_p = sel3(hrf, t) - sel3(hrs, t);
+        // ktotal = sqrt(A) - t/4, t = ks(3phi-2) + kf(1-3phi), A = ks kf/2 + t^2/16
+        const double ks = ksolidm(M, T, mode), kf = kfluidm(M, T, mode);
+        const double tt = ks * (3.0 * phi - 2.0) + kf * (1.0 - 3.0 * phi);
+        const double sa = sqrt(ks * kf / 2.0 + (tt * tt) * (1.0 / 16.0));
+        const double dks = (kf * 0.5 + tt * (3.0 * phi - 2.0) * 0.125) / (2.0 * sa) -
+                           0.25 * (3.0 * phi - 2.0);
+        const double dkf = (ks * 0.5 + tt * (1.0 - 3.0 * phi) * 0.125) / (2.0 * sa) -
+                           0.25 * (1.0 - 3.0 * phi);
+        const double dtp = 3.0 * (ks - kf);
+        d.k_p = tt * dtp * 0.125 / (2.0 * sa) - 0.25 * dtp;
+        d.k_t = dks * d_ksolidm(T, mode) + dkf * d_kfluidm(M, T, mode);
+    }
+    // tkm_rhocptotalm = T * rhocptotal (air keeps its stored rhocptotal)
+    d.tk_t = p.rhocptotal + T * d.rhocp_t;
+    d.tk_p = T * d.rhocp_p;
+    // etafluidcur_inv_kphim = etaf / kphir * (phim0/phi)^3 * ((1-phi)/(1-phim0))^2
+    d.ef_p = p.etafkphi * (-3.0 / phi - 2.0 / (1.0 - phi));
+    return d;
+}
+
+// ---- FP32 property evaluation of the mixed-precision form ----------------
+// The same formulas as the Float64 functions above, evaluated in single
+// precision (inputs rounded to float once, Float64 table entries converted on
+// use).  Only the mixed form calls these.
+
+__device__ __forceinline__ float self3(const double (&a)[3], int t) {
+    return (float)(t == 1 ? a[0] : (t == 2 ? a[1] : a[2]));
+}
+
+__device__ __forceinline__ float rhocpfluidm_f(const ereb_materials& M, float T, int mode) {
+    if (mode == 9) return (float)M.rhocpfluidm[0];
+    const float tf = (float)M.tmfluidphase, Lf = (float)M.Lf;
+    const float rw = (float)M.rhoH2Of, ri = (float)M.rhoH2Ofi;
+    if (T < tf - 5.0f) return ri * (7.67f * T);
+    if (T < tf) return ri * (7.67f * T + 0.1f * Lf);
+    if (T < tf + 5.0f) return rw * (4200.0f + 0.1f * Lf);
+    if (T < 410.0f) return rw * 4200.0f;
+    return rw * (-4.67e4f + 333.0f * T - 0.731f * (T * T) + 5.4e-4f * (T * T * T));
+}
+
+__device__ __forceinline__ float ksolidm_f(const ereb_materials& M, float T, int mode) {
+    return mode == 9 ? (float)M.ksolidm[0] : 0.73f + 1293.0f / (T + 77.0f);
+}
+
+__device__ __forceinline__ float kfluidm_f(const ereb_materials& M, float T, int mode) {
+    if (mode == 9) return (float)M.kfluidm[0];
+    if (T < (float)M.tmfluidphase) return 0.465f + 488.0f / T;
+    if (T < 410.0f) return -0.581f + 6.34e-3f * T - 7.93e-6f * (T * T);
+    return -0.142f + 4.12e-3f * T - 5.01e-6f * (T * T);
+}
+
+__device__ __forceinline__ float total_f(float s, float f, float phi) {
+    return s * (1.0f - phi) + f * phi;
+}
+
+__device__ __forceinline__ float ktotal_f(float ks, float kf, float phi) {
+    const float t = ks * (3.0f * phi - 2.0f) + kf * (1.0f - 3.0f * phi);
+    return sqrtf(ks * kf * 0.5f + (t * t) * (1.0f / 16.0f)) - 0.25f * t;
+}
+
+__device__ __forceinline__ float etaf_inv_kphi_f(float phim0, float kphir, float phi, float etaf) {
+    const float q = phim0 / phi;
+    const float r = (1.0f - phi) / (1.0f - phim0);
+    return etaf / kphir * (q * q * q) * (r * r);
+}
+
+struct PropsF {
+    float rhototal, rhocptotal, etatotal, hrtotal, ktotal, tkrhocp, etafkphi;
+};
+
+__device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t, float T,
+                                                 float phi, float XW, const double (&hrs)[3],
+                                                 const double (&hrf)[3], int mode,
+                                                 PropsF stored) {
+    PropsF p = stored;
+    float etafluidcur;
+    if (t < 3) {
+        const float XD = 1.0f - XW;
+        const float rhosolidm0 = ((float)M.MD + (float)M.MH2O * XW) /
+                                 ((float)M.VDs * XD + (float)M.VWs * XW);
+        const float rhofluidm0 = T > (float)M.tmfluidphase ? (float)M.rhoH2Of : (float)M.rhoH2Ofi;
+        p.rhototal = total_f(rhosolidm0, rhofluidm0, phi);
+        p.rhocptotal = total_f(self3(M.rhocpsolidm, t), rhocpfluidm_f(M, T, mode), phi);
+        const float es = T > (float)M.tmsolidphase ? self3(M.etasolidmm, t) : self3(M.etasolidm, t);
+        etafluidcur = T > (float)M.tmfluidphase ? self3(M.etafluidmm, t) : self3(M.etafluidm, t);
+        p.etatotal = fmaxf(fmaxf((float)M.etamin, es), etafluidcur);
+        p.hrtotal = total_f(self3(hrs, t), self3(hrf, t), phi);
+        p.ktotal = ktotal_f(ksolidm_f(M, T, mode), kfluidm_f(M, T, mode), phi);
+    } else {
+        etafluidcur = (float)M.etafluidm[2];
+    }
+    p.tkrhocp = T * p.rhocptotal;
+    p.etafkphi = etaf_inv_kphi_f((float)M.phim0, self3(M.kphim0, t), phi, etafluidcur);
+    return p;
+}
+
+// Compensated FP32 accumulation (Knuth's TwoSum): s + c carries the sum of
+// the added values with an error of O(u^2) relative to their magnitudes.
+__device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += e;
+}
+
 // ---- wavefront helpers (64 lanes) ---------------------------------------
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

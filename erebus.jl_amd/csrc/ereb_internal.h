@@ -21,13 +21,16 @@ constexpr int FINE = 2 * TILE;  // fine cells per tile side
 constexpr int NFINE = FINE * FINE;  // fine cells (sort keys) per tile
 
 struct Sys {
-    int64_t n = 0, cap = 0;
-    int32_t* colcnt = nullptr;  // n
-    int32_t* colptr = nullptr;  // n + 1, 0-based
-    uint64_t* mask = nullptr;   // n: bit per candidate row of each column
-    int32_t* rowval = nullptr;  // cap, 0-based
-    double* nzval = nullptr;    // cap
-    double* rhs = nullptr;      // n
+    int64_t n = 0, cap = 0, ncell = 0;
+    int ne = 0, nstot = 0;       // equations and stencil slots per P cell
+    void* tables = nullptr;      // device copy of the stencil slot tables
+    int32_t* colcnt = nullptr;   // n
+    int32_t* colptr = nullptr;   // n + 1, 0-based
+    uint16_t* pres = nullptr;    // n: presence bits of each row's slots
+    double* slotv = nullptr;     // ncell * nstot: entry values by slot
+    int32_t* rowval = nullptr;   // cap, 0-based
+    double* nzval = nullptr;     // cap
+    double* rhs = nullptr;       // n
     bool valid = false;
 };
 
@@ -46,6 +49,7 @@ struct ereb_ctx {
     double dx = 0, dy = 0;
     double* d_axes = nullptr;  // x (Nx) | y (Ny) | xp (Nx1) | yp (Ny1)
     ereb_materials mat{};
+    ereb_materials* d_mat = nullptr;  // device copy, read by kernels through a pointer
 
     int64_t marknum = 0, cap = 0;
     void* d_m[EREB_M_NFIELDS] = {};
@@ -98,8 +102,11 @@ GeomK geomk(const ereb_ctx* c);
 bool grid_is_basic(int field);
 double* gptr(ereb_ctx* c, int field);
 double* mptr(ereb_ctx* c, int field);
+float* fptr(ereb_ctx* c, int field);
+bool marker_is_f32(int precision, int field);
+size_t marker_es(int precision, int field);
 int32_t* tm_ptr(ereb_ctx* c);
-int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap);
+int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell);
 int ensure_scan(ereb_ctx* c, int64_t n);
 int check_flags(ereb_ctx* c);
 void prof_begin(ereb_ctx* c, int kclass, hipEvent_t* a);
@@ -133,5 +140,18 @@ int launch_rk4(ereb_ctx* c, double dt, int mode);
 // ereb_assembly.hip
 int launch_assemble_stokes(ereb_ctx* c, double dt);
 int launch_assemble_thermal(ereb_ctx* c, double dt);
+
+// grid field of SUM slot s (order of reset_interpolated_properties!)
+int sum_grid_field(int s);
+
+// adjoints (ereb_adjoint.hip, ereb_markers.hip); cotangent grids live in a
+// buffer with the layout of the grid fields (offsets goff)
+int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
+                       const double* gsum_base, const int64_t* gsum_off, double* gT,
+                       double* gphi);
+int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off);
+int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid);
+int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* gout, double* gin,
+                   double* gphi, double* gtk2);
 
 }  // namespace ereb

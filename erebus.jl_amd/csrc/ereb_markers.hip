@@ -3,10 +3,37 @@
 // move_markers_rk4! (E:4990-5257).  One thread per marker in storage order
 // (coalesced SoA reads); the grids (at most a few MB each) are read through
 // the L2/MALL.
+//
+// RK4 is bound by FP64 instruction issue: the reference divides by dx or dy
+// 87 times per marker.  Here those quotients are products with the rounded
+// reciprocals 1/dx, 1/dy (within an ulp of the quotient) and products and
+// sums may fuse into fma, so positions and temperatures agree with the
+// reference to ~1e-15 relative rather than bit for bit; cell indices
+// computed from given positions stay exact.
 #include "ereb_internal.h"
 
 namespace ereb {
 
+// R = double: the reference's Float64 interpolation.  R = float (mixed form):
+// weights and the bilinear interpolation in FP32, the increment added to the
+// Float64 marker value.
+template <class R>
+__device__ __forceinline__ R dot4_r(const double* __restrict__ gr, int rows, int i, int j,
+                                    const R* w) {
+    const double* p = gr + (size_t)(j - 1) * rows + (i - 1);
+    return (R)p[0] * w[0] + (R)p[1] * w[1] + (R)p[rows] * w[2] + (R)p[rows + 1] * w[3];
+}
+
+template <class R>
+__device__ __forceinline__ void weights4_r(double a, double b, R* w) {
+    const R ar = (R)a, br = (R)b;
+    w[0] = (R(1) - br) * (R(1) - ar);
+    w[1] = br * (R(1) - ar);
+    w[2] = (R(1) - br) * ar;
+    w[3] = br * ar;
+}
+
+template <class R>
 __global__ __launch_bounds__(256) void k_gather_T(GeomK g, const double* __restrict__ xm,
                                                   const double* __restrict__ ym,
                                                   double* __restrict__ tkm,
@@ -17,15 +44,16 @@ __global__ __launch_bounds__(256) void k_gather_T(GeomK g, const double* __restr
     const double x = xm[m], y = ym[m];
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
-    double w[4];
-    weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
-    const double v = gdot4(grid, g.Ny1, ip, jp, w);
+    R w[4];
+    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
+    const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
     if (first)
         tkm[m] = v;
     else
         tkm[m] += v;
 }
 
+template <class R>
 __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restrict__ xm,
                                                   const double* __restrict__ ym,
                                                   double* __restrict__ sxxm,
@@ -40,12 +68,16 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 2, g.Ny - 1);
     const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
     const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
-    double wp[4], wb[4];
-    weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, wp);
-    weights4((x - g.x[jb - 1]) / g.dx, (y - g.y[ib - 1]) / g.dy, wb);
-    sxxm[m] += gdot4(DSXX, g.Ny1, ip, jp, wp);
-    sxym[m] += gdot4(DSXY, g.Ny, ib, jb, wb);
+    R wp[4], wb[4];
+    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, wp);
+    weights4_r<R>((x - g.x[jb - 1]) / g.dx, (y - g.y[ib - 1]) / g.dy, wb);
+    sxxm[m] += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
+    sxym[m] += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
 }
+
+// RK4 region: a*b + c may be fused (the quotients above already make the
+// results agree to rounding rather than bit for bit)
+#pragma clang fp contract(fast)
 
 // RK4 stage velocity on the Vx grid (E:5057-5091).  `vc` supplies the
 // centre value of the left-half correction: the reference reads vx there
@@ -56,25 +88,26 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
     const int j = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
     const int i = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     const double dxmj = x - g.x[j - 1], dymi = y - g.yp[i - 1];
-    const double dx = g.dx, dy = g.dy;
+    const double idx = g.idx, idy = g.idy;
     const double* p = v + (size_t)(j - 1) * R + (i - 1);  // v[i, j]
-    double v13 = p[0] * (1.0 - dxmj / dx) + p[R] * dxmj / dx;
-    double v24 = p[1] * (1.0 - dxmj / dx) + p[R + 1] * dxmj / dx;
-    if (dxmj / dx >= 0.5) {
+    const double fx = dxmj * idx, fy = dymi * idy;
+    double v13 = p[0] * (1.0 - fx) + (p[R] * dxmj) * idx;
+    double v24 = p[1] * (1.0 - fx) + (p[R + 1] * dxmj) * idx;
+    if (fx >= 0.5) {
         if (j < g.Nx - 1) {
-            const double t = dxmj / dx - 0.5;
+            const double t = fx - 0.5;
             v13 += 0.5 * (t * t) * (p[0] - 2.0 * p[R] + p[2 * R]);
             v24 += 0.5 * (t * t) * (p[1] - 2.0 * p[R + 1] + p[2 * R + 1]);
         }
     } else {
         if (j > 1) {
-            const double t = dxmj / dx - 0.5;
+            const double t = fx - 0.5;
             const double cc = vc[(size_t)(j - 1) * R + (i - 1)];
             v13 += 0.5 * (t * t) * (p[-R] - 2.0 * cc + p[R]);
             v24 += 0.5 * (t * t) * (p[1 - R] - 2.0 * p[1] + p[R + 1]);
         }
     }
-    return v13 * (1.0 - dymi / dy) + v24 * dymi / dy;
+    return v13 * (1.0 - fy) + (v24 * dymi) * idy;
 }
 
 // RK4 stage velocity on the Vy grid (E:5093-5127)
@@ -84,24 +117,25 @@ __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict
     const int j = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int i = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
     const double dxmj = x - g.xp[j - 1], dymi = y - g.y[i - 1];
-    const double dx = g.dx, dy = g.dy;
+    const double idx = g.idx, idy = g.idy;
     const double* p = v + (size_t)(j - 1) * R + (i - 1);
-    double v12 = p[0] * (1.0 - dymi / dy) + p[1] * dymi / dy;
-    double v34 = p[R] * (1.0 - dymi / dy) + p[R + 1] * dymi / dy;
-    if (dymi / dy >= 0.5) {
+    const double fx = dxmj * idx, fy = dymi * idy;
+    double v12 = p[0] * (1.0 - fy) + (p[1] * dymi) * idy;
+    double v34 = p[R] * (1.0 - fy) + (p[R + 1] * dymi) * idy;
+    if (fy >= 0.5) {
         if (i < g.Ny - 1) {
-            const double t = dymi / dy - 0.5;
+            const double t = fy - 0.5;
             v12 += 0.5 * (t * t) * (p[0] - 2.0 * p[1] + p[2]);
             v34 += 0.5 * (t * t) * (p[R] - 2.0 * p[R + 1] + p[R + 2]);
         }
     } else {
         if (i > 1) {
-            const double t = dymi / dy - 0.5;
+            const double t = fy - 0.5;
             v12 += 0.5 * (t * t) * (p[-1] - 2.0 * p[0] + p[1]);
             v34 += 0.5 * (t * t) * (p[R - 1] - 2.0 * p[R] + p[R + 1]);
         }
     }
-    return v12 * (1.0 - dxmj / dx) + v34 * dxmj / dx;
+    return v12 * (1.0 - fx) + (v34 * dxmj) * idx;
 }
 
 struct RK4Args {
@@ -116,33 +150,16 @@ struct RK4Args {
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
 };
 
-__global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= A.n) return;
+// Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
+// to (xn, yn) and the fluid back-trace (E:5138-5226) from (xn, yn) to
+// (xb, yb).
+__device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0, double& xn,
+                                          double& yn, double& xb, double& yb) {
     const GeomK& g = A.g;
     const double dt = A.dt;
     const double* b = A.M.brk4;
     const double* cr = A.M.crk4;
-    double xmm = A.xm[m], ymm = A.ym[m];
-    double xr = xmm, yr = ymm;
-    double w[4];
-    // solid temperature at the start position (P grid) and rotation rate
-    // (basic grid), E:5013-5040
-    int jp = jl_cell((xr - g.xp[0]) * g.idx, 1, g.Nx);
-    int ip = jl_cell((yr - g.yp[0]) * g.idy, 1, g.Ny);
-    weights4((xr - g.xp[jp - 1]) / g.dx, (yr - g.yp[ip - 1]) / g.dy, w);
-    const double tks0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
-    const int jb = jl_cell((xr - g.x[0]) * g.idx, 1, g.Nx - 1);
-    const int ib = jl_cell((yr - g.y[0]) * g.idy, 1, g.Ny - 1);
-    weights4((xr - g.x[jb - 1]) / g.dx, (yr - g.y[ib - 1]) / g.dy, w);
-    const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
-    // analytic stress rotation E:5042-5048
-    const double th = dt * om;
-    const double s = A.sxxm[m], t = A.sxym[m];
-    const double ct = cos(th), st = sin(th), s2 = sin(2.0 * th), c2 = cos(2.0 * th);
-    A.sxxm[m] = s * (ct * ct) - s * (st * st) - t * s2;
-    A.sxym[m] = s * s2 + t * c2;
-    // RK4 on the solid velocity E:5055-5136
+    double xmm = x0, ymm = y0, xr = x0, yr = y0;
     double vxr[4], vyr[4];
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
@@ -153,11 +170,8 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
             yr = ymm + dt * cr[rk] * vyr[rk];
         }
     }
-    const double xn = xmm + dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
-    const double yn = ymm + dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
-    A.xm[m] = xn;
-    A.ym[m] = yn;
-    // RK4 back-trace on the fluid velocity E:5138-5226
+    xn = xmm + dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
+    yn = ymm + dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
     xmm = xr = xn;
     ymm = yr = yn;
 #pragma unroll
@@ -169,11 +183,50 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
             yr = ymm - dt * cr[rk] * vyr[rk];
         }
     }
-    xr = xmm - dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
-    yr = ymm - dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
-    jp = jl_cell((xr - g.xp[0]) * g.idx, 1, g.Nx);
-    ip = jl_cell((yr - g.yp[0]) * g.idy, 1, g.Ny);
-    weights4((xr - g.xp[jp - 1]) / g.dx, (yr - g.yp[ip - 1]) / g.dy, w);
+    xb = xmm - dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
+    yb = ymm - dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
+}
+
+// bilinear P-grid stencil of a position: cell (ip, jp), weights w
+__device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, int& ip, int& jp,
+                                          double* w) {
+    jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
+    ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
+    weights4((x - g.xp[jp - 1]) * g.idx, (y - g.yp[ip - 1]) * g.idy, w);
+}
+
+__global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.n) return;
+    const GeomK& g = A.g;
+    const double dt = A.dt;
+    const double xmm = A.xm[m], ymm = A.ym[m];
+    double xr = xmm, yr = ymm;
+    double w[4];
+    // solid temperature at the start position (P grid) and rotation rate
+    // (basic grid), E:5013-5040
+    int jp = jl_cell((xr - g.xp[0]) * g.idx, 1, g.Nx);
+    int ip = jl_cell((yr - g.yp[0]) * g.idy, 1, g.Ny);
+    weights4((xr - g.xp[jp - 1]) * g.idx, (yr - g.yp[ip - 1]) * g.idy, w);
+    const double tks0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
+    const int jb = jl_cell((xr - g.x[0]) * g.idx, 1, g.Nx - 1);
+    const int ib = jl_cell((yr - g.y[0]) * g.idy, 1, g.Ny - 1);
+    weights4((xr - g.x[jb - 1]) * g.idx, (yr - g.y[ib - 1]) * g.idy, w);
+    const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
+    // analytic stress rotation E:5042-5048
+    const double th = dt * om;
+    const double s = A.sxxm[m], t = A.sxym[m];
+    double ct, st, s2, c2;  // one shared argument reduction per angle
+    sincos(th, &st, &ct);
+    sincos(2.0 * th, &s2, &c2);
+    A.sxxm[m] = s * (ct * ct) - s * (st * st) - t * s2;
+    A.sxym[m] = s * s2 + t * c2;
+    // solid RK4 step and fluid back-trace
+    double xn, yn;
+    rk4_paths(A, xmm, ymm, xn, yn, xr, yr);
+    A.xm[m] = xn;
+    A.ym[m] = yn;
+    p_stencil(g, xr, yr, ip, jp, w);
     const double tkf0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
     // fluid-solid temperature correction E:5243-5252
     const double d = tkf0 - tks0;
@@ -184,15 +237,118 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     A.tkm[m] = ((1.0 - ph) * T * rcs + ph * (T + d) * rcf) / ((1.0 - ph) * rcs + ph * rcf);
 }
 
+// ---- adjoints (with respect to tkm, phim and the tk2 grid) ---------------
+
+__device__ __forceinline__ void atomic_add4(double* gr, int rows, int i, int j, const double* w,
+                                            double v) {
+    double* p = gr + (size_t)(j - 1) * rows + (i - 1);
+    atomicAdd(p, w[0] * v);
+    atomicAdd(p + 1, w[1] * v);
+    atomicAdd(p + rows, w[2] * v);
+    atomicAdd(p + rows + 1, w[3] * v);
+}
+
+// RK4^T of the temperature correction E:5243-5252: given the cotangent gout of
+// tkm after the step, the cotangent of tkm before it (gin), the increment of
+// the phim cotangent and the tk2 cotangent (both stencils, by atomics).
+// The marker paths are recomputed from the step-start positions; positions
+// and stresses do not depend on tkm, phim or tk2.
+__global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __restrict__ gout,
+                                                 double* __restrict__ gin,
+                                                 double* __restrict__ gphi,
+                                                 double* __restrict__ gtk2) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= A.n) return;
+    const GeomK& g = A.g;
+    const double x0 = A.xm[m], y0 = A.ym[m];
+    int ips, jps, ipb, jpb;
+    double ws[4], wb[4];
+    p_stencil(g, x0, y0, ips, jps, ws);
+    const double tks0 = gdot4(A.tk2, g.Ny1, ips, jps, ws);
+    double xn, yn, xb, yb;
+    rk4_paths(A, x0, y0, xn, yn, xb, yb);
+    p_stencil(g, xb, yb, ipb, jpb, wb);
+    const double tkf0 = gdot4(A.tk2, g.Ny1, ipb, jpb, wb);
+    const double d = tkf0 - tks0;
+    const int tt = min(max(A.tm[m], 1), 3);
+    const double T = A.tkm[m], ph = A.phim[m];
+    const double rcs = sel3(A.M.rhocpsolidm, tt);
+    const double rcf = rhocpfluidm(A.M, T, A.mode), drcf = d_rhocpfluidm(A.M, T, A.mode);
+    const double num = (1.0 - ph) * T * rcs + ph * (T + d) * rcf;
+    const double den = (1.0 - ph) * rcs + ph * rcf;
+    const double F_d = ph * rcf / den;
+    const double F_T = ((1.0 - ph) * rcs + ph * rcf + ph * (T + d) * drcf) / den -
+                       num * (ph * drcf) / (den * den);
+    const double F_p = (-T * rcs + (T + d) * rcf) / den - num * (rcf - rcs) / (den * den);
+    const double go = gout[m];
+    gin[m] = go * F_T;
+    gphi[m] += go * F_p;
+    const double gd = go * F_d;
+    if (gd != 0.0) {
+        atomic_add4(gtk2, g.Ny1, ipb, jpb, wb, gd);
+        atomic_add4(gtk2, g.Ny1, ips, jps, ws, -gd);
+    }
+}
+
+#pragma clang fp contract(off)
+
+// gather^T of update_marker_temperature! (E:4782-4801): the marker cotangent
+// goes to the interpolated grid (tk2 at timestep 1, DT later); at timestep 1
+// the marker value was overwritten, so its cotangent becomes zero.
+__global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __restrict__ xm,
+                                                      const double* __restrict__ ym,
+                                                      double* __restrict__ gT,
+                                                      double* __restrict__ ggrid, int64_t M,
+                                                      int first) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const double x = xm[m], y = ym[m];
+    const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
+    const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
+    double w[4];
+    weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
+    const double v = gT[m];
+    if (v != 0.0) atomic_add4(ggrid, g.Ny1, ip, jp, w, v);
+    if (first) gT[m] = 0.0;
+}
+
+int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return EREB_OK;
+    hipLaunchKernelGGL(k_gather_T_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                       c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), gT, ggrid, M,
+                       timestep == 1 ? 1 : 0);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
+
+static RK4Args rk4_args(ereb_ctx* c, double dt, int mode);
+
+int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* gout, double* gin,
+                   double* gphi, double* gtk2) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return EREB_OK;
+    RK4Args A = rk4_args(c, dt, mode);
+    hipLaunchKernelGGL(k_rk4_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+                       A, gout, gin, gphi, gtk2);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
+
 int launch_gather_temperature(ereb_ctx* c, int64_t timestep) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     hipEvent_t ev;
     prof_begin(c, EREB_K_GATHER_T, &ev);
     const double* grid = gptr(c, timestep == 1 ? EREB_G_TK2 : EREB_G_DT);
-    hipLaunchKernelGGL(k_gather_T, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
-                       geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), mptr(c, EREB_M_TKM),
-                       grid, M, timestep == 1 ? 1 : 0);
+    if (c->precision == EREB_MIXED)
+        hipLaunchKernelGGL(k_gather_T<float>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                           c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM),
+                           mptr(c, EREB_M_TKM), grid, M, timestep == 1 ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_gather_T<double>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                           c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM),
+                           mptr(c, EREB_M_TKM), grid, M, timestep == 1 ? 1 : 0);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_GATHER_T, ev);
     return EREB_OK;
@@ -205,7 +361,8 @@ int launch_gather_stress(ereb_ctx* c) {
         return fail(c, EREB_ERR_ARG, "update_marker_stress! needs Nx, Ny >= 4 (P clamps 2..N-1)");
     hipEvent_t ev;
     prof_begin(c, EREB_K_GATHER_S, &ev);
-    hipLaunchKernelGGL(k_gather_S, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+    auto kern = c->precision == EREB_MIXED ? k_gather_S<float> : k_gather_S<double>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
                        geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), mptr(c, EREB_M_SXXM),
                        mptr(c, EREB_M_SXYM), gptr(c, EREB_G_DSXX), gptr(c, EREB_G_DSXY), M);
     EREB_HIP(c, hipGetLastError());
@@ -213,9 +370,8 @@ int launch_gather_stress(ereb_ctx* c) {
     return EREB_OK;
 }
 
-int launch_rk4(ereb_ctx* c, double dt, int mode) {
+static RK4Args rk4_args(ereb_ctx* c, double dt, int mode) {
     const int64_t M = c->marknum;
-    if (M <= 0) return EREB_OK;
     RK4Args A;
     A.g = geomk(c);
     A.M = c->mat;
@@ -235,6 +391,13 @@ int launch_rk4(ereb_ctx* c, double dt, int mode) {
     A.vyf = gptr(c, EREB_G_VYF);
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
+    return A;
+}
+
+int launch_rk4(ereb_ctx* c, double dt, int mode) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return EREB_OK;
+    RK4Args A = rk4_args(c, dt, mode);
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     hipLaunchKernelGGL(k_rk4, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A);

@@ -14,31 +14,40 @@
 //   - Markers arrive sorted by fine key (ereb_sort.hip): a tile's markers are
 //     contiguous and grouped by fine cell in ascending marker index.
 //
-// One workgroup (4 wavefronts) per tile, streaming the tile in chunks of at
-// most CH markers and NFMAX fine cells:
-//   1. load   : one thread per marker stages the 18 property values and the
-//               fractions dxmj/dx, dymi/dy of the basic and P stencils in
-//               LDS (the 7 marker properties are computed here and written
-//               back to HBM);
-//   2. tasks  : wavefront g owns grid g; a task is (fine cell, corner): the
-//               sum over the fine cell's markers, in order, of
-//               property * weight for that corner (and of the weight);
-//   3. combine: lane (r, c) of wavefront g owns window node (r, c) of grid g
-//               and adds the partials of the at most 16 fine cells around it
-//               in a fixed (chunk, fine column, fine row) order.
+// One workgroup of six wavefronts per tile.  The tile is walked one basic
+// column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
+// contiguous run of sorted markers, split into chunks of at most CH markers
+// when it is longer:
+//   1. stage  : one thread per marker computes the 7 marker properties
+//               (written back to HBM) and stages the 18 property values and
+//               the fractions dxmj/dx, dymi/dy of the basic and P stencils in
+//               LDS.  Each thread's inputs for the next chunk are loaded into
+//               registers before the current chunk's tasks start, so the
+//               gather latency overlaps phases 2 and 3;
+//   2. tasks  : each wavefront owns a set of sums (basic values 0-3, basic
+//               4-7 with the weight, Vx, Vy, P values 0-4, P 5-9 with the
+//               weight).  A task is (fine cell, corner column cj): the sums
+//               over the fine cell's markers, in order, of property * weight
+//               for the corners (0, cj) and (1, cj);
+//   3. combine: lane (r, c) of a wavefront owns window node (r, c) of its
+//               grid and adds the partials of the at most 8 fine cells of the
+//               column that touch it, in a fixed (column, fine column, fine
+//               row) order.
 // No atomics: every sum has exactly one owner and a fixed order, so results
 // are identical run to run.  Tile windows overlap by two node rows/columns;
 // k_combine adds the at most 2 x 2 window partials of each node in a fixed
-// tile order.
+// tile order.  Sums accumulate with fused multiply-add, which changes the
+// rounding of the sequential sums only.
 #include "ereb_internal.h"
 
 namespace ereb {
 
-constexpr int CH = 256;     // markers per LDS chunk
+constexpr int NTHR = 384;    // six wavefronts: basic (2 halves), Vx, Vy, P (2 halves)
+constexpr int CH = 256;      // markers per LDS chunk (one per thread in the load phase)
 constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
-constexpr int NFMAX = 32;    // fine cells per chunk
-constexpr int NLF = 18;      // property values staged per marker
-constexpr int NPART = 120;   // partials per fine cell: 4 corners x (8 + 6 + 6 + 10)
+constexpr int NCF = 2 * FINE;  // fine cells in one basic column of a tile
+constexpr int NLF = 18;        // property values staged per marker
+constexpr int NPART = NCF * 4 * NSUM;  // (fine cell, cj, ci) x SUM partials per column
 
 // staged property indices
 enum {
@@ -47,23 +56,52 @@ enum {
     L_SXX, L_RHOCP, L_ALPHAS, L_ALPHAF, L_HR, L_TKRHOCP                  // P
 };
 
+// R is the type of the property vectors, the LDS staging and the per-fine-cell
+// sums: double in the Float64 form, float in the mixed form.  The marker
+// state (coordinates, temperature, porosity, stresses) is double in both.
+template <class R>
 struct ScatterArgs {
     GeomK g;
-    ereb_materials M;
+    const ereb_materials* __restrict__ M;
     double hrs[3], hrf[3];
     int mode, write_props, nty;
-    const uint32_t* perm;
-    const int32_t* fstart;
-    const double *xm, *ym;
+    const uint32_t* __restrict__ perm;
+    const int32_t* __restrict__ fstart;
+    const double *xm, *ym, *tkm, *phim, *sxx, *sxy;
     const int32_t* tm;
-    const double *tkm, *phim, *XW, *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas,
-        *alphaf, *sxx, *sxy;
-    double *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
+    const R *XW, *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas, *alphaf;
+    R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
     double* part;
 };
 
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
+
+// per-node accumulators of the combine: plain double sums in the Float64 form,
+// compensated float sums (value + error term) in the mixed form
+template <class R> struct NodeAcc;
+template <> struct NodeAcc<double> {
+    double s[10];
+    __device__ void zero() {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) s[q] = 0.0;
+    }
+    __device__ void add(int q, double v) { s[q] += v; }
+    __device__ double get(int q) const { return s[q]; }
+};
+template <> struct NodeAcc<float> {
+    float s[10], c[10];
+    __device__ void zero() {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) s[q] = c[q] = 0.0f;
+    }
+    __device__ void add(int q, float v) { two_sum_add(s[q], c[q], v); }
+    __device__ double get(int q) const { return (double)s[q] + (double)c[q]; }
+};
+
 // per-grid SUM slot offset, partial offset and staged-property list (SUM
-// order of reset_interpolated_properties!, E:968-999)
+// order of reset_interpolated_properties!, E:968-999).  Partials of grid G
+// sit at part[poff + ((fl*2 + cj)*2 + ci)*(nf+1) + q].
 template <int G> struct GridSpec;
 template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
     static constexpr int slot0 = 0, poff = 0, nf = 7;
@@ -74,17 +112,17 @@ template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
     }
 };
 template <> struct GridSpec<1> {  // Vx: RHOX RHOFX KX PHIX RX (+WTX)
-    static constexpr int slot0 = 8, poff = 32, nf = 5;
+    static constexpr int slot0 = 8, poff = NCF * 4 * 8, nf = 5;
     static constexpr bool prow = true, pcol = false;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<2> {  // Vy: RHOY RHOFY KY PHIY RY (+WTY)
-    static constexpr int slot0 = 14, poff = 56, nf = 5;
+    static constexpr int slot0 = 14, poff = NCF * 4 * 14, nf = 5;
     static constexpr bool prow = false, pcol = true;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<3> {  // P: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PHI (+WTP)
-    static constexpr int slot0 = 20, poff = 80, nf = 9;
+    static constexpr int slot0 = 20, poff = NCF * 4 * 20, nf = 9;
     static constexpr bool prow = true, pcol = true;
     __device__ static constexpr int f(int q) {
         return q == 0 ? L_RHOTOTAL : q == 1 ? L_RHOCP : q == 2 ? L_ALPHAS : q == 3 ? L_ALPHAF
@@ -93,165 +131,305 @@ template <> struct GridSpec<3> {  // P: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PH
     }
 };
 
+template <class R>
 struct ScatterLDS {
-    double sf[NLF][CHP];
-    double sfr[4][CHP];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
-    double part[NFMAX][NPART];
+    R sf[NLF][CHP];
+    R sfr[4][CHP];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
+    R part[NPART];
+    ereb_materials mat;  // material table, read per marker with a per-lane type index
+    double ax[4][WIN];   // tile window of the x, y, xp, yp axes
     int32_t fs[NFINE + 1];
 };
 
-// tasks of grid G: (fine cell fl of the chunk, corner) for this wavefront
-template <int G>
-__device__ __forceinline__ void tasks(ScatterLDS& L, int lane, int f_lo, int nf, int c0,
-                                      int cend) {
+// Values q = Q0 .. Q0+QN-1 of grid G's NQ = nf + 1 sums (q == nf is the weight
+// sum) for the task (fine cell fl of the column, corner column cj): lanes
+// 0..47 of one wavefront.  The marker values of iteration s + 1 are read from
+// LDS while iteration s accumulates.
+template <class R, int G, int Q0, int QN>
+__device__ __forceinline__ void tasks(ScatterLDS<R>& L, int lane, int f0, int c0, int cend) {
     using S = GridSpec<G>;
-    const double* fa = S::pcol ? L.sfr[2] : L.sfr[0];
-    const double* fb = S::prow ? L.sfr[3] : L.sfr[1];
-    for (int task = lane; task < nf * 4; task += 64) {
-        const int fl = task >> 2, corner = task & 3;
-        const int ci = corner & 1, cj = corner >> 1;  // weights order (i,j),(i+1,j),(i,j+1),(i+1,j+1)
-        const int f = f_lo + fl;
-        const int k0 = max(L.fs[f], c0) - c0, k1 = min(L.fs[f + 1], cend) - c0;
-        double acc[S::nf + 1];
+    constexpr int NQ = S::nf + 1;
+    static_assert(Q0 + QN <= NQ, "value range");
+    constexpr bool HASW = Q0 + QN == NQ;   // this part holds the weight sum
+    constexpr int NV = HASW ? QN - 1 : QN;  // staged property values read per marker
+    if (lane >= 2 * NCF) return;
+    const int fl = lane >> 1, cj = lane & 1;
+    const int k0 = max(L.fs[f0 + fl], c0) - c0, k1 = min(L.fs[f0 + fl + 1], cend) - c0;
+    const R* fa = S::pcol ? L.sfr[2] : L.sfr[0];
+    const R* fb = S::prow ? L.sfr[3] : L.sfr[1];
+    R s0[QN], s1[QN];  // corners (0, cj) and (1, cj)
 #pragma unroll
-        for (int q = 0; q <= S::nf; ++q) acc[q] = 0.0;
+    for (int q = 0; q < QN; ++q) s0[q] = s1[q] = R(0);
+    if (k0 < k1) {
+        R v[NV], a = fa[k0], b = fb[k0];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = L.sf[S::f(Q0 + q)][k0];
         for (int s = k0; s < k1; ++s) {
-            const double a = fa[s], b = fb[s];
-            const double w = (ci ? b : 1.0 - b) * (cj ? a : 1.0 - a);
+            const int sn = min(s + 1, k1 - 1);
+            R vn[NV];
 #pragma unroll
-            for (int q = 0; q < S::nf; ++q) acc[q] += L.sf[S::f(q)][s] * w;
-            acc[S::nf] += w;
+            for (int q = 0; q < NV; ++q) vn[q] = L.sf[S::f(Q0 + q)][sn];
+            const R an = fa[sn], bn = fb[sn];
+            // weights order (i,j),(i+1,j),(i,j+1),(i+1,j+1) = (1-b)(1-a), b(1-a), (1-b)a, ba
+            const R wa = cj ? a : R(1) - a;
+            const R w0 = (R(1) - b) * wa, w1 = b * wa;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                s0[q] = fma_r(v[q], w0, s0[q]);
+                s1[q] = fma_r(v[q], w1, s1[q]);
+            }
+            if (HASW) {
+                s0[QN - 1] += w0;
+                s1[QN - 1] += w1;
+            }
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[q] = vn[q];
+            a = an;
+            b = bn;
         }
-        double* p = &L.part[fl][S::poff + corner * (S::nf + 1)];
+    }
+    R* p = &L.part[S::poff + (fl * 2 + cj) * 2 * NQ + Q0];
 #pragma unroll
-        for (int q = 0; q <= S::nf; ++q) p[q] = acc[q];
+    for (int q = 0; q < QN; ++q) {
+        p[q] = s0[q];
+        p[NQ + q] = s1[q];
     }
 }
 
-// node (r, c) of grid G adds the partials of its fine cells in this chunk
-template <int G>
-__device__ __forceinline__ void combine(const ScatterLDS& L, double* acc, int r, int c, int f_lo,
-                                        int f_hi) {
+// node (r, c) of grid G adds values Q0 .. Q0+QN-1 of the partials of the fine
+// cells of basic column k that touch it: fine columns 2k + h, fine rows
+// fi_lo .. fi_lo + 3
+template <class R, int G, int Q0, int QN>
+__device__ __forceinline__ void combine(const ScatterLDS<R>& L, NodeAcc<R>& acc, int r, int c,
+                                        int k) {
     using S = GridSpec<G>;
+    constexpr int NQ = S::nf + 1;
     const int fi_lo = S::prow ? 2 * r - 3 : 2 * r - 2;
-    const int fj_lo = S::pcol ? 2 * c - 3 : 2 * c - 2;
-    for (int fj = max(0, fj_lo); fj <= min(FINE - 1, fj_lo + 3); ++fj) {
-        const int cj = c - (S::pcol ? (fj + 1) >> 1 : fj >> 1);
-        for (int fi = max(0, fi_lo); fi <= min(FINE - 1, fi_lo + 3); ++fi) {
-            const int f = fj * FINE + fi;
-            if (f < f_lo || f > f_hi) continue;
-            const int ci = r - (S::prow ? (fi + 1) >> 1 : fi >> 1);
-            const double* p = &L.part[f - f_lo][S::poff + (ci + 2 * cj) * (S::nf + 1)];
 #pragma unroll
-            for (int q = 0; q <= S::nf; ++q) acc[q] += p[q];
+    for (int h = 0; h < 2; ++h) {
+        const int fj = 2 * k + h;
+        const int cj = c - (S::pcol ? (fj + 1) >> 1 : fj >> 1);
+        if (cj < 0 || cj > 1) continue;
+        R v[4][QN];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int fi = min(max(fi_lo + d, 0), FINE - 1);  // clamped; skipped below
+            const int ci = 1 - (d >> 1);  // = r - (grid row of fine row fi)
+            const R* p = &L.part[S::poff + (((h * FINE + fi) * 2 + cj) * 2 + ci) * NQ + Q0];
+#pragma unroll
+            for (int q = 0; q < QN; ++q) v[d][q] = p[q];
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int fi = fi_lo + d;
+            if (fi < 0 || fi >= FINE) continue;
+#pragma unroll
+            for (int q = 0; q < QN; ++q) acc.add(q, v[d][q]);
         }
     }
 }
 
-template <int G>
-__device__ __forceinline__ void write_part(double* part, int tile, const double* acc, int lane) {
+template <class R, int G, int Q0, int QN>
+__device__ __forceinline__ void write_part(double* part, int tile, const NodeAcc<R>& acc,
+                                           int lane) {
     using S = GridSpec<G>;
-    double* p = part + ((size_t)tile * NSUM + S::slot0) * 64 + lane;
+    double* p = part + ((size_t)tile * NSUM + S::slot0 + Q0) * 64 + lane;
 #pragma unroll
-    for (int q = 0; q <= S::nf; ++q) p[(size_t)q * 64] = acc[q];
+    for (int q = 0; q < QN; ++q) p[(size_t)q * 64] = acc.get(q);
 }
 
-__global__ __launch_bounds__(256, 2) void k_scatter_fp64(ScatterArgs A) {
-    __shared__ ScatterLDS L;
+// Raw inputs of one marker, loaded one chunk ahead of their use so that the
+// global-memory latency of the gather through perm overlaps the previous
+// chunk's task and combine phases.
+template <class R>
+struct Raw {
+    double x, y, T, phi, sxy, sxx;
+    R XW, etavp, invggg, cohes, tens, frict, rhof, alphas, alphaf;
+    R st[5];  // stored rhototal, rhocptotal, etatotal, hrtotal, ktotal (kept by air)
+    int32_t tm;
+    uint32_t m;
+};
+
+template <class R>
+__device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, uint32_t m, Raw<R>& r) {
+    r.m = m;
+    r.x = A.xm[m];
+    r.y = A.ym[m];
+    r.tm = A.tm[m];
+    r.T = A.tkm[m];
+    r.phi = A.phim[m];
+    r.XW = A.XW[m];
+    r.etavp = A.etavp[m];
+    r.invggg = A.invggg[m];
+    r.sxy = A.sxy[m];
+    r.cohes = A.cohes[m];
+    r.tens = A.tens[m];
+    r.frict = A.frict[m];
+    r.rhof = A.rhof[m];
+    r.alphas = A.alphas[m];
+    r.alphaf = A.alphaf[m];
+    r.sxx = A.sxx[m];
+    r.st[0] = A.rhototal[m];
+    r.st[1] = A.rhocptotal[m];
+    r.st[2] = A.etatotal[m];
+    r.st[3] = A.hrtotal[m];
+    r.st[4] = A.ktotal[m];
+}
+
+// properties of the marker (written back to HBM), stencil fractions and the
+// 18 staged values into LDS slot t
+// (j0, i0): 0-based first basic column and row of the tile.  Every marker of
+// the tile has jb - 1, ib - 1 in [0, TILE) and jp - 1, ip - 1 in [0, TILE]
+// relative to them (its fine key was derived from these cells).
+// Float64 properties (bit-exact with the reference) or FP32 properties
+__device__ __forceinline__ Props props_r(const ereb_materials& M, int t, const Raw<double>& r,
+                                         const double (&hrs)[3], const double (&hrf)[3],
+                                         int mode) {
+    Props st{r.st[0], r.st[1], r.st[2], r.st[3], r.st[4], 0.0, 0.0};
+    return marker_props(M, t, r.T, r.phi, r.XW, hrs, hrf, mode, st);
+}
+__device__ __forceinline__ PropsF props_r(const ereb_materials& M, int t, const Raw<float>& r,
+                                          const double (&hrs)[3], const double (&hrf)[3],
+                                          int mode) {
+    PropsF st{r.st[0], r.st[1], r.st[2], r.st[3], r.st[4], 0.0f, 0.0f};
+    return marker_props_f(M, t, (float)r.T, (float)r.phi, r.XW, hrs, hrf, mode, st);
+}
+
+template <class R>
+__device__ __forceinline__ void stage_raw(const ScatterArgs<R>& A, ScatterLDS<R>& L, int t,
+                                          const Raw<R>& r, int j0, int i0) {
+    const Cells cl = marker_cells(A.g, r.x, r.y);
+    // fractions in double from the double coordinates, then rounded to R
+    L.sfr[0][t] = (R)((r.x - L.ax[0][cl.jb - 1 - j0]) / A.g.dx);
+    L.sfr[1][t] = (R)((r.y - L.ax[1][cl.ib - 1 - i0]) / A.g.dy);
+    L.sfr[2][t] = (R)((r.x - L.ax[2][cl.jp - 1 - j0]) / A.g.dx);
+    L.sfr[3][t] = (R)((r.y - L.ax[3][cl.ip - 1 - i0]) / A.g.dy);
+    const int tt = min(max(r.tm, 1), 3);  // set validates 1..3
+    const auto p = props_r(L.mat, tt, r, A.hrs, A.hrf, A.mode);
+    if (A.write_props) {
+        const uint32_t m = r.m;
+        if (tt < 3) {
+            A.rhototal[m] = p.rhototal;
+            A.rhocptotal[m] = p.rhocptotal;
+            A.etatotal[m] = p.etatotal;
+            A.hrtotal[m] = p.hrtotal;
+            A.ktotal[m] = p.ktotal;
+        }
+        A.tkrhocp[m] = p.tkrhocp;
+        A.etafkphi[m] = p.etafkphi;
+    }
+    L.sf[L_ETATOTAL][t] = p.etatotal;
+    L.sf[L_ETAVP][t] = r.etavp;
+    L.sf[L_INVGGG][t] = r.invggg;
+    L.sf[L_SXY][t] = (R)r.sxy;
+    L.sf[L_COHES][t] = r.cohes;
+    L.sf[L_TENS][t] = r.tens;
+    L.sf[L_FRICT][t] = r.frict;
+    L.sf[L_RHOTOTAL][t] = p.rhototal;
+    L.sf[L_RHOF][t] = r.rhof;
+    L.sf[L_KTOTAL][t] = p.ktotal;
+    L.sf[L_PHIM][t] = (R)r.phi;
+    L.sf[L_ETAFKPHI][t] = p.etafkphi;
+    L.sf[L_SXX][t] = (R)r.sxx;
+    L.sf[L_RHOCP][t] = p.rhocptotal;
+    L.sf[L_ALPHAS][t] = r.alphas;
+    L.sf[L_ALPHAF][t] = r.alphaf;
+    L.sf[L_HR][t] = p.hrtotal;
+    L.sf[L_TKRHOCP][t] = p.tkrhocp;
+}
+
+// Chunks of a tile: basic column k, markers [c0, cend) with cend - c0 <= CH.
+// k == TILE marks the end.  All fields are wave-uniform.
+struct Chunk {
+    int k, c0, cend;
+};
+
+template <class R>
+__device__ __forceinline__ Chunk chunk_from(const ScatterLDS<R>& L, int k, int c0) {
+    for (; k < TILE; ++k) {
+        const int ce = __builtin_amdgcn_readfirstlane(L.fs[(k + 1) * NCF]);
+        if (c0 < ce) return Chunk{k, c0, min(c0 + CH, ce)};
+        if (k + 1 < TILE) c0 = max(c0, __builtin_amdgcn_readfirstlane(L.fs[(k + 1) * NCF]));
+    }
+    return Chunk{TILE, 0, 0};
+}
+
+template <class R>
+__device__ __forceinline__ Chunk chunk_next(const ScatterLDS<R>& L, const Chunk& c) {
+    return c.k < TILE ? chunk_from(L, c.k, c.cend) : c;
+}
+
+template <class R>
+__device__ __forceinline__ uint32_t chunk_perm(const ScatterArgs<R>& A, const Chunk& c, int t) {
+    return (c.k < TILE && t < c.cend - c.c0) ? A.perm[c.c0 + t] : 0u;
+}
+
+template <class R>
+__global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
+    __shared__ ScatterLDS<R> L;
     const int tile = blockIdx.x;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int r = lane & 7, c = lane >> 3;
-    for (int i = t; i <= NFINE; i += 256) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
+    const int j0 = (tile / A.nty) * TILE, i0 = (tile % A.nty) * TILE;
+    for (int i = t; i <= NFINE; i += NTHR) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
+    constexpr int NMAT = (int)(sizeof(ereb_materials) / sizeof(double));
+    static_assert(sizeof(ereb_materials) % sizeof(double) == 0, "materials are doubles");
+    if (t < NMAT) ((double*)&L.mat)[t] = ((const double*)A.M)[t];
+    if (t < 4 * WIN) {
+        const int a = t / WIN, k = t % WIN;
+        const double* axis = a == 0 ? A.g.x : a == 1 ? A.g.y : a == 2 ? A.g.xp : A.g.yp;
+        const int n = a == 0 ? A.g.Nx : a == 1 ? A.g.Ny : a == 2 ? A.g.Nx1 : A.g.Ny1;
+        const int k0 = (a & 1) ? i0 : j0;
+        L.ax[a][k] = k0 + k < n ? axis[k0 + k] : 0.0;
+    }
     __syncthreads();
-    const int S0 = L.fs[0], E0 = L.fs[NFINE];
-    double acc[10];
-#pragma unroll
-    for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+    NodeAcc<R> acc;
+    acc.zero();
 
-    int f_lo = 0;
-    for (int c0 = S0; c0 < E0;) {
-        // chunk [c0, cend): at most CH markers and NFMAX fine cells
-        while (L.fs[f_lo + 1] <= c0) ++f_lo;
-        int cend = min(c0 + CH, E0);
-        if (f_lo + NFMAX < NFINE) cend = min(cend, L.fs[f_lo + NFMAX]);
-        if (cend <= c0) cend = min(c0 + CH, E0);  // a single fine cell longer than CH
-        int f_hi = f_lo;
-        while (f_hi + 1 < NFINE && L.fs[f_hi + 1] < cend) ++f_hi;
-        const int nf = f_hi - f_lo + 1;
-        // ---- 1. load ------------------------------------------------------
-        if (t < cend - c0) {
-            const uint32_t m = A.perm[c0 + t];
-            const double x = A.xm[m], y = A.ym[m];
-            const Cells cl = marker_cells(A.g, x, y);
-            L.sfr[0][t] = (x - A.g.x[cl.jb - 1]) / A.g.dx;
-            L.sfr[1][t] = (y - A.g.y[cl.ib - 1]) / A.g.dy;
-            L.sfr[2][t] = (x - A.g.xp[cl.jp - 1]) / A.g.dx;
-            L.sfr[3][t] = (y - A.g.yp[cl.ip - 1]) / A.g.dy;
-            const int tt = min(max(A.tm[m], 1), 3);  // set validates 1..3
-            const double T = A.tkm[m], phi = A.phim[m];
-            Props st;
-            st.rhototal = A.rhototal[m];
-            st.rhocptotal = A.rhocptotal[m];
-            st.etatotal = A.etatotal[m];
-            st.hrtotal = A.hrtotal[m];
-            st.ktotal = A.ktotal[m];
-            st.tkrhocp = 0.0;
-            st.etafkphi = 0.0;
-            const Props p = marker_props(A.M, tt, T, phi, A.XW[m], A.hrs, A.hrf, A.mode, st);
-            if (A.write_props) {
-                if (tt < 3) {
-                    A.rhototal[m] = p.rhototal;
-                    A.rhocptotal[m] = p.rhocptotal;
-                    A.etatotal[m] = p.etatotal;
-                    A.hrtotal[m] = p.hrtotal;
-                    A.ktotal[m] = p.ktotal;
-                }
-                A.tkrhocp[m] = p.tkrhocp;
-                A.etafkphi[m] = p.etafkphi;
-            }
-            L.sf[L_ETATOTAL][t] = p.etatotal;
-            L.sf[L_ETAVP][t] = A.etavp[m];
-            L.sf[L_INVGGG][t] = A.invggg[m];
-            L.sf[L_SXY][t] = A.sxy[m];
-            L.sf[L_COHES][t] = A.cohes[m];
-            L.sf[L_TENS][t] = A.tens[m];
-            L.sf[L_FRICT][t] = A.frict[m];
-            L.sf[L_RHOTOTAL][t] = p.rhototal;
-            L.sf[L_RHOF][t] = A.rhof[m];
-            L.sf[L_KTOTAL][t] = p.ktotal;
-            L.sf[L_PHIM][t] = phi;
-            L.sf[L_ETAFKPHI][t] = p.etafkphi;
-            L.sf[L_SXX][t] = A.sxx[m];
-            L.sf[L_RHOCP][t] = p.rhocptotal;
-            L.sf[L_ALPHAS][t] = A.alphas[m];
-            L.sf[L_ALPHAF][t] = A.alphaf[m];
-            L.sf[L_HR][t] = p.hrtotal;
-            L.sf[L_TKRHOCP][t] = p.tkrhocp;
-        }
+    Chunk cur = chunk_from(L, 0, __builtin_amdgcn_readfirstlane(L.fs[0]));
+    Chunk nx = chunk_next(L, cur);
+    Raw<R> raw;
+    if (cur.k < TILE && t < cur.cend - cur.c0) load_raw(A, chunk_perm(A, cur, t), raw);
+    uint32_t pnx = chunk_perm(A, nx, t);
+    while (cur.k < TILE) {
+        // ---- 1. stage the chunk's markers (loaded one chunk ahead) ---------
+        if (t < cur.cend - cur.c0) stage_raw(A, L, t, raw, j0, i0);
         __syncthreads();
-        // ---- 2. per (fine cell, corner) partial sums, wavefront g = grid g --
+        // prefetch: fields of the next chunk, perm of the one after
+        const Chunk nx2 = chunk_next(L, nx);
+        if (nx.k < TILE && t < nx.cend - nx.c0) load_raw(A, pnx, raw);
+        pnx = chunk_perm(A, nx2, t);
+        // ---- 2. (fine cell, corner column) partials, wavefront g = grid g --
+        const int f0 = cur.k * NCF;
         switch (wave) {
-            case 0: tasks<0>(L, lane, f_lo, nf, c0, cend); break;
-            case 1: tasks<1>(L, lane, f_lo, nf, c0, cend); break;
-            case 2: tasks<2>(L, lane, f_lo, nf, c0, cend); break;
-            default: tasks<3>(L, lane, f_lo, nf, c0, cend); break;
+            case 0: tasks<R, 0, 0, 4>(L, lane, f0, cur.c0, cur.cend); break;
+            case 1: tasks<R, 0, 4, 4>(L, lane, f0, cur.c0, cur.cend); break;
+            case 2: tasks<R, 1, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+            case 3: tasks<R, 2, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+            case 4: tasks<R, 3, 0, 5>(L, lane, f0, cur.c0, cur.cend); break;
+            default: tasks<R, 3, 5, 5>(L, lane, f0, cur.c0, cur.cend); break;
         }
         __syncthreads();
         // ---- 3. node-owned combine -----------------------------------------
         switch (wave) {
-            case 0: combine<0>(L, acc, r, c, f_lo, f_hi); break;
-            case 1: combine<1>(L, acc, r, c, f_lo, f_hi); break;
-            case 2: combine<2>(L, acc, r, c, f_lo, f_hi); break;
-            default: combine<3>(L, acc, r, c, f_lo, f_hi); break;
+            case 0: combine<R, 0, 0, 4>(L, acc, r, c, cur.k); break;
+            case 1: combine<R, 0, 4, 4>(L, acc, r, c, cur.k); break;
+            case 2: combine<R, 1, 0, 6>(L, acc, r, c, cur.k); break;
+            case 3: combine<R, 2, 0, 6>(L, acc, r, c, cur.k); break;
+            case 4: combine<R, 3, 0, 5>(L, acc, r, c, cur.k); break;
+            default: combine<R, 3, 5, 5>(L, acc, r, c, cur.k); break;
         }
         __syncthreads();
-        c0 = cend;
+        cur = nx;
+        nx = nx2;
     }
     switch (wave) {
-        case 0: write_part<0>(A.part, tile, acc, lane); break;
-        case 1: write_part<1>(A.part, tile, acc, lane); break;
-        case 2: write_part<2>(A.part, tile, acc, lane); break;
-        default: write_part<3>(A.part, tile, acc, lane); break;
+        case 0: write_part<R, 0, 0, 4>(A.part, tile, acc, lane); break;
+        case 1: write_part<R, 0, 4, 4>(A.part, tile, acc, lane); break;
+        case 2: write_part<R, 1, 0, 6>(A.part, tile, acc, lane); break;
+        case 3: write_part<R, 2, 0, 6>(A.part, tile, acc, lane); break;
+        case 4: write_part<R, 3, 0, 5>(A.part, tile, acc, lane); break;
+        default: write_part<R, 3, 5, 5>(A.part, tile, acc, lane); break;
     }
 }
 
@@ -288,12 +466,15 @@ static const int kSumGrid[NSUM] = {
     EREB_G_RHOSUM, EREB_G_RHOCPSUM, EREB_G_ALPHASUM, EREB_G_ALPHAFSUM, EREB_G_HRSUM,
     EREB_G_GGGPSUM, EREB_G_SXXSUM, EREB_G_TKSUM, EREB_G_PHISUM, EREB_G_WTPSUM};
 
-int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode, bool write_props) {
-    if (c->precision != EREB_FP64)
-        return fail(c, EREB_ERR_ARG, "mixed-precision scatter is not available in this build");
-    ScatterArgs A;
+int sum_grid_field(int s) { return kSumGrid[s]; }
+
+template <class R>
+static int launch_scatter_t(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
+                            bool write_props) {
+    auto P = [&](int f) { return (R*)c->d_m[f]; };
+    ScatterArgs<R> A;
     A.g = geomk(c);
-    A.M = c->mat;
+    A.M = c->d_mat;
     for (int k = 0; k < 3; ++k) {
         A.hrs[k] = hrs[k];
         A.hrf[k] = hrf[k];
@@ -305,34 +486,41 @@ int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode, 
     A.fstart = c->d_fstart;
     A.xm = mptr(c, EREB_M_XM);
     A.ym = mptr(c, EREB_M_YM);
-    A.tm = tm_ptr(c);
     A.tkm = mptr(c, EREB_M_TKM);
     A.phim = mptr(c, EREB_M_PHIM);
-    A.XW = mptr(c, EREB_M_XWSOLIDM0);
-    A.etavp = mptr(c, EREB_M_ETAVPM);
-    A.invggg = mptr(c, EREB_M_INV_GGGTOTALM);
-    A.frict = mptr(c, EREB_M_FRICTTOTALM);
-    A.cohes = mptr(c, EREB_M_COHESTOTALM);
-    A.tens = mptr(c, EREB_M_TENSTOTALM);
-    A.rhof = mptr(c, EREB_M_RHOFLUIDCUR);
-    A.alphas = mptr(c, EREB_M_ALPHASOLIDCUR);
-    A.alphaf = mptr(c, EREB_M_ALPHAFLUIDCUR);
     A.sxx = mptr(c, EREB_M_SXXM);
     A.sxy = mptr(c, EREB_M_SXYM);
-    A.rhototal = mptr(c, EREB_M_RHOTOTALM);
-    A.rhocptotal = mptr(c, EREB_M_RHOCPTOTALM);
-    A.etatotal = mptr(c, EREB_M_ETATOTALM);
-    A.hrtotal = mptr(c, EREB_M_HRTOTALM);
-    A.ktotal = mptr(c, EREB_M_KTOTALM);
-    A.tkrhocp = mptr(c, EREB_M_TKM_RHOCPTOTALM);
-    A.etafkphi = mptr(c, EREB_M_ETAFLUIDCUR_INV_KPHIM);
+    A.tm = tm_ptr(c);
+    A.XW = P(EREB_M_XWSOLIDM0);
+    A.etavp = P(EREB_M_ETAVPM);
+    A.invggg = P(EREB_M_INV_GGGTOTALM);
+    A.frict = P(EREB_M_FRICTTOTALM);
+    A.cohes = P(EREB_M_COHESTOTALM);
+    A.tens = P(EREB_M_TENSTOTALM);
+    A.rhof = P(EREB_M_RHOFLUIDCUR);
+    A.alphas = P(EREB_M_ALPHASOLIDCUR);
+    A.alphaf = P(EREB_M_ALPHAFLUIDCUR);
+    A.rhototal = P(EREB_M_RHOTOTALM);
+    A.rhocptotal = P(EREB_M_RHOCPTOTALM);
+    A.etatotal = P(EREB_M_ETATOTALM);
+    A.hrtotal = P(EREB_M_HRTOTALM);
+    A.ktotal = P(EREB_M_KTOTALM);
+    A.tkrhocp = P(EREB_M_TKM_RHOCPTOTALM);
+    A.etafkphi = P(EREB_M_ETAFLUIDCUR_INV_KPHIM);
     A.part = c->d_part;
     hipEvent_t ev;
     prof_begin(c, EREB_K_SCATTER, &ev);
-    hipLaunchKernelGGL(k_scatter_fp64, dim3((unsigned)c->ntiles), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)c->ntiles), dim3(NTHR), 0, c->stream, A);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_SCATTER, ev);
     return EREB_OK;
+}
+
+int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode, bool write_props) {
+    // the property vectors are float exactly when the context is in the mixed form
+    if (marker_is_f32(c->precision, EREB_M_ETATOTALM))
+        return launch_scatter_t<float>(c, hrs, hrf, mode, write_props);
+    return launch_scatter_t<double>(c, hrs, hrf, mode, write_props);
 }
 
 int launch_combine(ereb_ctx* c) {

@@ -162,7 +162,7 @@ int launch_fine_sort(ereb_ctx* c) {
     hipEvent_t ev;
     prof_begin(c, EREB_K_SORT, &ev);
     if ((rc = fill_i32(c, c->d_fcount, 0, nk))) return rc;
-    if ((rc = fill_i32(c, c->d_flags, 0, 4))) return rc;
+    if ((rc = fill_i32(c, c->d_flags, 0, 1))) return rc;
     if ((rc = fill_i32(c, c->d_big, 0, 1))) return rc;
     if (M > 0) {
         const unsigned nb = (unsigned)((M + 255) / 256);

@@ -103,6 +103,7 @@ PROTOTYPES = {
     "ereb_move_markers_rk4": [_P] + [_P] * 13 + [_I64, _D, _I64],
     "ereb_assemble_hydromechanical_lse": [_P] + [_P] * 20 + [_D, _P, _P],
     "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
+    "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
     "ereb_profile_enable": [_P, _I],
     "ereb_profile_get": [_P, _I, _P, _P],
     "ereb_profile_reset": [_P],
@@ -313,6 +314,25 @@ class Context:
         self.check(self.lib.ereb_sort_get(self.h, _ptr(perm), _ptr(key),
                                           ctypes.addressof(nt)), "sort_get")
         return perm, key, nt.value
+
+    # ---- adjoint ----------------------------------------------------------
+    def adjoint_tk1_loss(self, nsteps: int, checkpoint_every: int, dt: float, mode: int,
+                         hrsolidm, hrfluidm, grad: bool = True, want_tk1: bool = False):
+        """L = sum(tk1^2) after nsteps steps of the transfer map and, with
+        grad, dL/dtkm and dL/dphim of the initial markers (adjoint kernels,
+        checkpoints every checkpoint_every steps).  Returns (L, gT, gphi,
+        tk1) with None for what was not asked; the markers are unchanged."""
+        hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
+        hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
+        n = self.marknum
+        L = ctypes.c_double()
+        gT = np.empty(n) if grad else None
+        gP = np.empty(n) if grad else None
+        tk1 = np.empty(self.grid_shape("tk1"), order="F") if want_tk1 else None
+        self.check(self.lib.ereb_adjoint_tk1_loss(
+            self.h, int(nsteps), int(checkpoint_every), float(dt), int(mode), _ptr(hs),
+            _ptr(hf), ctypes.addressof(L), _ptr(tk1), _ptr(gT), _ptr(gP)), "adjoint")
+        return L.value, gT, gP, tk1
 
     # ---- profiling --------------------------------------------------------
     def profile(self, on: bool = True):

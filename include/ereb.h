@@ -283,6 +283,28 @@ int ereb_assemble_thermal_lse(ereb_ctx* ctx, const double* tk1,
     const double* HA, const double* HS, const double* DHP, double* RT,
     double dt, int64_t* nnz);
 
+/* ---- adjoint (no counterpart in the reference) -------------------------- */
+/* Gradient of L = sum over the P nodes of tk1^2 after `nsteps` timesteps of
+ * the transfer map on the resident markers, with respect to the initial
+ * marker temperature tkm and porosity phim.  One timestep is: scatter
+ * (compute_marker_properties! + marker_to_*_nodes!, E:6419-6517), node
+ * properties with the insulating tk1 boundary (E:6522-6604), tk2 := tk1 (no
+ * heat solve), the temperature gather at timestep 1 (E:7004; later steps add
+ * DT = tk2 - tk1 = 0) and RK4 advection through the resident vx/vy/vxf/vyf
+ * grids (E:7036).  tk1 is taken after one more scatter + node pass on the
+ * final state.  The reverse pass runs the hand-written adjoint kernels
+ * (scatter^T as a gather, node properties^T, gather^T and RK4^T), recomputing
+ * the forward steps from checkpoints of (xm, ym, tkm) taken every
+ * `checkpoint_every` steps.  Positions do not depend on tkm or phim.
+ * loss: required; tk1_out (Ny1*Nx1, column-major), grad_tkm and grad_phim
+ * (marknum) may be NULL; with both gradients NULL only the forward map runs.
+ * The resident markers are left as they were.  Float64 form only for the
+ * gradient. */
+int ereb_adjoint_tk1_loss(ereb_ctx* ctx, int64_t nsteps, int64_t checkpoint_every,
+                          double dt, int64_t mode, const double hrsolidm[3],
+                          const double hrfluidm[3], double* loss, double* tk1_out,
+                          double* grad_tkm, double* grad_phim);
+
 /* ---- profiling: HIP events around the launches of one kernel class ------ */
 enum ereb_kernel_class {
     EREB_K_SORT = 0, EREB_K_SCATTER, EREB_K_COMBINE, EREB_K_NODES,
