@@ -16,10 +16,16 @@ HBM.  The grids that the solves and grid-only operations would produce
 (DT, DSXX, DSXY, tk2, pressures, gravity, ETAP/ETAPHI) are fixed synthetic
 fields (erebus.jl_amd/ereb/synth.py).
 
-Prints one JSON line.  `roofline` is for the scatter kernel (the dominant
-one), measured live with HIP events on the library's stream; `cpu_baseline`
-times the single-threaded C restatement of the reference CPU path (the
-oracle) on a bounded sample of the same workload.
+Prints one JSON line.  `value` is the Float64 form.  `roofline` is for the
+scatter kernel (the dominant one), measured live with HIP events on the
+library's stream.  `mixed` runs the same steps in the mixed-precision form
+on identical inputs and reports its throughput and the per-field error
+against the Float64 state after the same steps.  `adjoint` times the
+gradient of sum(tk1^2) after 20 steps (1,000,000 markers, 201x201 nodes,
+checkpoints every 5 steps) against the forward map and checks it against
+central finite differences on 64 markers.  `cpu_baseline` times the
+single-threaded C restatement of the reference CPU path (the oracle) on a
+bounded sample of the Float64 workload.
 """
 from __future__ import annotations
 
@@ -198,14 +204,84 @@ def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
     sums = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1  # elements of the 30 SUM grids
     sc_bytes = M * marker_bytes + sums * 8
     achieved = sc_bytes / (kern_ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # summary (scaled by a plain copy of known size), when one exists
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"scatter_{form}_{workload}_hbm_bytes.json")
+    pmc = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if os.path.exists(pmc):
-        traffic = json.load(open(pmc)).get("bytes_per_launch")
+        for k, v in json.load(open(pmc)).get("kernels", {}).items():
+            if k.startswith(f"void ereb::k_scatter<{kernel_name.split('<')[1]}"):
+                traffic = v["bytes_per_launch"]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": f"{kernel_name} (marker properties + scatter, one launch per step)",
             "algorithmic_bytes": sc_bytes}
+
+
+BREAKS = (263.0, 268.0, 273.0, 278.0, 410.0)  # kinks and steps of T-dependent properties
+
+
+def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str = "adj"):
+    """Gradient of L = sum(tk1^2) after nsteps steps with respect to the
+    initial marker temperature and porosity: time of the forward map, time
+    of the checkpointed adjoint (forward with checkpoints + reverse sweep),
+    and central finite differences (h = 1e-6 relative) on nfd random planet
+    markers away from the property kinks."""
+    g, mat = synth.workload_geometry(workload, mode=1)
+    mode = mat.marker_property_mode
+    mk = synth.make_markers(g, mat, seed=0)
+    M = mk["xm"].size
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    vx, vy = synth.stream_velocity(g)
+    dt = ereb.DT_LONGEST
+    with ereb.Context(g, mat, ereb.FP64) as ctx:
+        ctx.set_markers(mk)
+        for n, a in (("vx", vx), ("vy", vy), ("vxf", vx), ("vyf", vy),
+                     ("wyx", np.zeros(g.basic_shape, order="F"))):
+            ctx.set_grid(n, a)
+        ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf, grad=True)  # warm
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L, _, _, _ = ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf, grad=False)
+        t_fwd = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            L2, gT, gP, _ = ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf)
+        t_adj = (time.perf_counter() - t0) / reps
+        rng = np.random.default_rng(12345)
+        ok = np.flatnonzero((mk["tm"] < 3) & np.all(
+            np.abs(mk["tkm"][:, None] - np.array(BREAKS)[None, :]) > 0.5, axis=1))
+        idx = rng.choice(ok, nfd, replace=False)
+        fd_err = {}
+        for name, grad in (("tkm", gT), ("phim", gP)):
+            fd = np.empty(nfd)
+            for q, m in enumerate(idx):
+                h = 1e-6 * abs(mk[name][m])
+                tk = []
+                for sgn in (1.0, -1.0):
+                    a = mk[name].copy()
+                    a[m] += sgn * h
+                    ctx.set_marker(name, a)
+                    tk.append(ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf,
+                                                   grad=False, want_tk1=True)[3])
+                fd[q] = float(np.sum((tk[0] - tk[1]) * (tk[0] + tk[1]))) / (2 * h)
+            ctx.set_marker(name, mk[name])
+            ad = grad[idx]
+            fd_err[name] = {
+                "max_rel_err": float(np.max(np.abs(fd - ad) / np.abs(ad))),
+                "normwise_rel_err": float(np.max(np.abs(fd - ad)) / np.max(np.abs(ad))),
+            }
+    return {
+        "workload": f"{workload}: {g.Nx}x{g.Ny} basic nodes, {g.Nxmc}x{g.Nymc} jittered "
+                    f"markers/cell ({M:,} markers), prescribed stream-function velocity",
+        "loss": "sum over P nodes of tk1^2 after the last step (tk2 := tk1)",
+        "steps": nsteps, "checkpoint_every": every, "markers": M,
+        "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+        "adjoint_forward_ratio": t_adj / t_fwd,
+        "forward_marker_steps_per_s": M * nsteps / t_fwd,
+        "fd_check": {"markers": nfd, "h_rel": 1e-6, **fd_err},
+    }
 
 
 def main():
@@ -217,6 +293,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -268,6 +345,9 @@ def main():
             "roofline": scatter_roofline(g, M, kernm["scatter"], SCATTER_MARKER_BYTES_MIXED,
                                          "k_scatter<float>", "mixed", a.workload),
             "kernel_ms": kernm,
+            "pattern_difference": {
+                "stokes_darcy": precision.pattern_difference(ctxm.csc(0), ctx64.csc(0)),
+                "thermal": precision.pattern_difference(ctxm.csc(1), ctx64.csc(1))},
             "errors_vs_fp64": {
                 "after_steps": a.warmup + a.steps,
                 "metric": "per field [max|a-ref|/max|ref|, rms(a-ref)/rms(ref)]",
@@ -280,6 +360,10 @@ def main():
         log(f"[bench] mixed {msm:.3f} ms/step, worst nodal field {wn} {wnv:.2e}")
         ctxm.close()
     ctx64.close()
+    if not a.no_adjoint:
+        log("[bench] adjoint leg")
+        res["adjoint"] = adjoint_leg()
+        log(f"[bench] adjoint/forward {res['adjoint']['adjoint_forward_ratio']:.2f}")
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, 1 thread)")
         res["cpu_baseline"] = cpu_baseline(g, mat, mk, a.cpu_steps)

@@ -259,8 +259,8 @@ __global__ void k_twice(const double* __restrict__ a, double* __restrict__ b, in
 static const int kStateFields[5] = {EREB_M_XM, EREB_M_YM, EREB_M_TKM, EREB_M_SXXM,
                                     EREB_M_SXYM};
 
-// copy marker fields of the resident state to (save) or from (restore) dst,
-// nf fields of M doubles each
+// copy the first nf of (xm, ym, tkm, sxxm, sxym) of the resident markers to
+// (save) or from (restore) buf, M doubles each
 static int state_copy(ereb_ctx* c, double* buf, int nf, bool save) {
     const size_t bytes = sizeof(double) * (size_t)c->marknum;
     for (int q = 0; q < nf; ++q) {
@@ -272,11 +272,18 @@ static int state_copy(ereb_ctx* c, double* buf, int nf, bool save) {
     return EREB_OK;
 }
 
+static size_t grid_doubles(const ereb_ctx* c) {
+    size_t n = 0;
+    for (int f = 0; f < EREB_G_NFIELDS; ++f) n = std::max(n, c->goff[f] + (size_t)c->gnumel[f]);
+    return n;
+}
+
 // one timestep of the differentiated map: scatter, node properties with the
 // insulating tk1 boundary, tk2 := tk1 (no heat solve), the temperature
-// gather (only at timestep 1: later it adds DT = tk2 - tk1 = 0), RK4
+// gather (only at timestep 1: later it adds DT = tk2 - tk1 = 0), RK4;
+// xb/yb record the fluid back-trace end points when given
 static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const double* hrs,
-                            const double* hrf, bool rk4) {
+                            const double* hrf, double* xb, double* yb) {
     int rc;
     if ((rc = launch_fine_sort(c))) return rc;
     if ((rc = launch_scatter(c, hrs, hrf, mode, true))) return rc;
@@ -286,29 +293,52 @@ static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const d
                                sizeof(double) * c->gnumel[EREB_G_TK1], hipMemcpyDeviceToDevice,
                                c->stream));
     if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
-    if (rk4 && (rc = launch_rk4(c, dt, mode))) return rc;
-    return EREB_OK;
+    return launch_rk4(c, dt, mode, xb, yb);
 }
 
 struct AdjBuffers {
-    double *save = nullptr, *cps = nullptr, *seg = nullptr, *tp = nullptr;
+    double *save = nullptr, *cps = nullptr, *st = nullptr, *snap = nullptr, *xb = nullptr;
     double *gT = nullptr, *gT2 = nullptr, *gP = nullptr, *gg = nullptr;
-    ~AdjBuffers() {
-        hipFree(save);
-        hipFree(cps);
-        hipFree(seg);
-        hipFree(tp);
-        hipFree(gT);
-        hipFree(gT2);
-        hipFree(gP);
-        hipFree(gg);
-    }
 };
+
+// the buffers of one call, carved from the context's grow-only pool
+static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int64_t every,
+                       size_t gsz) {
+    const size_t M = (size_t)c->marknum;
+    const size_t sizes[9] = {5 * M,
+                             grad ? 3 * M * (size_t)nseg : 0,
+                             grad ? 3 * M * (size_t)every : 0,
+                             grad ? gsz * (size_t)every : 0,
+                             grad ? 2 * M * (size_t)every : 0,
+                             grad ? M : 0, grad ? M : 0, grad ? M : 0, grad ? gsz : 0};
+    size_t tot = 0;
+    for (size_t z : sizes) tot += (z + 31) / 32 * 32;  // 256-byte aligned pieces
+    if (tot > c->adj_pool_n) {
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+        hipFree(c->adj_pool);
+        c->adj_pool = nullptr;
+        c->adj_pool_n = 0;
+        EREB_HIP(c, hipMalloc(&c->adj_pool, sizeof(double) * tot));
+        c->adj_pool_n = tot;
+    }
+    double* p = c->adj_pool;
+    double** dst[9] = {&B.save, &B.cps, &B.st, &B.snap, &B.xb, &B.gT, &B.gT2, &B.gP, &B.gg};
+    for (int k = 0; k < 9; ++k) {
+        *dst[k] = sizes[k] ? p : nullptr;
+        p += (sizes[k] + 31) / 32 * 32;
+    }
+    return EREB_OK;
+}
 
 }  // namespace ereb
 
 using namespace ereb;
 
+// Forward with checkpoints of (xm, ym, tkm) every `every` steps; the reverse
+// sweep recomputes each segment from its checkpoint, recording per step the
+// start state, the grid fields and the fluid back-trace end points (the last
+// segment is recorded during the forward pass itself), then runs the adjoint
+// kernels backwards through the segment.
 extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every, double dt,
                                      int64_t mode, const double hrsolidm[3],
                                      const double hrfluidm[3], double* loss, double* tk1_out,
@@ -323,36 +353,43 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
     const int md = (int)mode;
     const int64_t nseg = (nsteps + every - 1) / every;
     const size_t MB = sizeof(double) * (size_t)M;
+    const size_t gsz = grid_doubles(c);
+    const size_t GB = sizeof(double) * gsz;
     AdjBuffers B;
     int rc;
+    if ((rc = adj_buffers(c, B, grad, nseg, std::min(every, std::max<int64_t>(nsteps, 1)), gsz)))
+        return rc;
     // the markers are restored at the end
-    EREB_HIP(c, hipMalloc(&B.save, 5 * MB));
     if ((rc = state_copy(c, B.save, 5, true))) return rc;
-    if (grad) {
-        EREB_HIP(c, hipMalloc(&B.cps, 3 * MB * (size_t)std::max<int64_t>(nseg, 1)));
-        EREB_HIP(c, hipMalloc(&B.seg, 3 * MB * (size_t)every));
-        EREB_HIP(c, hipMalloc(&B.tp, MB));
-        EREB_HIP(c, hipMalloc(&B.gT, MB));
-        EREB_HIP(c, hipMalloc(&B.gT2, MB));
-        EREB_HIP(c, hipMalloc(&B.gP, MB));
-        size_t gsz = 0;
-        for (int f = 0; f < EREB_G_NFIELDS; ++f)
-            gsz = std::max(gsz, c->goff[f] + (size_t)c->gnumel[f]);
-        EREB_HIP(c, hipMalloc(&B.gg, sizeof(double) * gsz));
-    }
-    // forward, with a checkpoint of (xm, ym, tkm) every `every` steps
+    // record step k of a segment starting at k0: start state before it,
+    // grids and back-trace end points from it
+    auto run_step = [&](int64_t k, int64_t k0, bool record) -> int {
+        int r;
+        double* xb = nullptr;
+        double* yb = nullptr;
+        if (record) {
+            if ((r = state_copy(c, B.st + 3 * (size_t)M * (k - k0), 3, true))) return r;
+            xb = B.xb + 2 * (size_t)M * (k - k0);
+            yb = xb + M;
+        }
+        if ((r = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, xb, yb))) return r;
+        if (record)
+            EREB_HIP(c, hipMemcpyAsync(B.snap + gsz * (size_t)(k - k0), c->d_grids, GB,
+                                       hipMemcpyDeviceToDevice, c->stream));
+        return EREB_OK;
+    };
     for (int64_t k = 0; k < nsteps; ++k) {
-        if (grad && k % every == 0 && (rc = state_copy(c, B.cps + 3 * (size_t)M * (k / every), 3,
-                                                       true)))
+        const int64_t sg = k / every;
+        if (grad && k % every == 0 &&
+            (rc = state_copy(c, B.cps + 3 * (size_t)M * sg, 3, true)))
             return rc;
-        if ((rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, true))) return rc;
+        if ((rc = run_step(k, sg * every, grad && sg == nseg - 1))) return rc;
     }
     // loss on the final state
-    int rcs;
-    if ((rcs = launch_fine_sort(c)) || (rcs = launch_scatter(c, hrsolidm, hrfluidm, md, true)) ||
-        (rcs = launch_combine(c)) ||
-        (rcs = launch_node_properties(c, true, true, true, true, true)))
-        return rcs;
+    if ((rc = launch_fine_sort(c)) || (rc = launch_scatter(c, hrsolidm, hrfluidm, md, true)) ||
+        (rc = launch_combine(c)) ||
+        (rc = launch_node_properties(c, true, true, true, true, true)))
+        return rc;
     const int64_t n = c->gnumel[EREB_G_TK1];
     std::vector<double> tk1((size_t)n);
     EREB_HIP(c, hipMemcpyAsync(tk1.data(), gptr(c, EREB_G_TK1), sizeof(double) * n,
@@ -369,14 +406,11 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
         for (int f = 0; f < EREB_G_NFIELDS; ++f) goff[f] = (int64_t)c->goff[f];
         int64_t soff[NSUM];
         for (int s = 0; s < NSUM; ++s) soff[s] = goff[sum_grid_field(s)];
-        size_t gsz = 0;
-        for (int f = 0; f < EREB_G_NFIELDS; ++f)
-            gsz = std::max(gsz, c->goff[f] + (size_t)c->gnumel[f]);
         const unsigned nbg = (unsigned)((n + 255) / 256);
         if ((rc = fill_f64(c, B.gT, 0.0, M)) || (rc = fill_f64(c, B.gP, 0.0, M)) ||
             (rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz)))
             return rc;
-        // d L / d tk1 = 2 tk1 on the final state
+        // d L / d tk1 = 2 tk1 on the final state, then nodes^T and scatter^T there
         hipLaunchKernelGGL(k_twice, dim3(nbg), dim3(256), 0, c->stream,
                            (const double*)gptr(c, EREB_G_TK1), G(EREB_G_TK1), n);
         if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
@@ -384,29 +418,33 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
             return rc;
         for (int64_t sg = nseg - 1; sg >= 0; --sg) {
             const int64_t k0 = sg * every, k1 = std::min(nsteps, k0 + every);
-            // recompute the segment from its checkpoint, keeping each step's start state
-            if ((rc = state_copy(c, B.cps + 3 * (size_t)M * sg, 3, false))) return rc;
-            for (int64_t k = k0; k < k1; ++k) {
-                if ((rc = state_copy(c, B.seg + 3 * (size_t)M * (k - k0), 3, true))) return rc;
-                if (k + 1 < k1 && (rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, true)))
-                    return rc;
+            if (sg != nseg - 1) {  // the last segment was recorded by the forward pass
+                if ((rc = state_copy(c, B.cps + 3 * (size_t)M * sg, 3, false))) return rc;
+                for (int64_t k = k0; k < k1; ++k)
+                    if ((rc = run_step(k, k0, true))) return rc;
             }
             for (int64_t k = k1 - 1; k >= k0; --k) {
-                // state at the start of step k; sums, tk1 = tk2 and (k = 0) the gather
-                if ((rc = state_copy(c, B.seg + 3 * (size_t)M * (k - k0), 3, false))) return rc;
-                if ((rc = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, false))) return rc;
+                double* stk = B.st + 3 * (size_t)M * (k - k0);
+                const double* xb = B.xb + 2 * (size_t)M * (k - k0);
+                // the grids and the start state of step k
+                EREB_HIP(c, hipMemcpyAsync(c->d_grids, B.snap + gsz * (size_t)(k - k0), GB,
+                                           hipMemcpyDeviceToDevice, c->stream));
+                if ((rc = state_copy(c, stk, 3, false))) return rc;
+                if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
                 if ((rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz))) return rc;
-                // RK4^T (markers at the step start, tkm after the gather)
-                if ((rc = launch_rk4_adj(c, dt, md, B.gT, B.gT2, B.gP, G(EREB_G_TK2)))) return rc;
+                // RK4^T (tkm as RK4 saw it), then gather^T at timestep 1
+                if ((rc = launch_rk4_adj(c, dt, md, xb, xb + M, B.gT, B.gT2, B.gP,
+                                         G(EREB_G_TK2))))
+                    return rc;
                 if (k == 0 && (rc = launch_gather_T_adj(c, 1, B.gT2, G(EREB_G_TK2)))) return rc;
                 // tk2 := tk1, then node properties^T
                 EREB_HIP(c, hipMemcpyAsync(G(EREB_G_TK1), G(EREB_G_TK2), sizeof(double) * n,
                                            hipMemcpyDeviceToDevice, c->stream));
                 if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
-                // scatter^T at the step start with the temperature the scatter saw
-                EREB_HIP(c, hipMemcpyAsync(mptr(c, EREB_M_TKM),
-                                           B.seg + 3 * (size_t)M * (k - k0) + 2 * (size_t)M, MB,
-                                           hipMemcpyDeviceToDevice, c->stream));
+                // scatter^T with the temperature the scatter saw
+                if (k == 0)
+                    EREB_HIP(c, hipMemcpyAsync(mptr(c, EREB_M_TKM), stk + 2 * (size_t)M, MB,
+                                               hipMemcpyDeviceToDevice, c->stream));
                 if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT2, B.gP)))
                     return rc;
                 std::swap(B.gT, B.gT2);

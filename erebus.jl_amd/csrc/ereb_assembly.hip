@@ -13,8 +13,9 @@
 // are discovered once per process by running the same equation code on the
 // host with a recording sink, so they cannot drift from the kernels.
 //   1. rows  : one thread per (P cell, equation) evaluates its row, writes
-//              each nonzero entry into its slot (slot values of a cell are
-//              contiguous) and a presence bit per slot; R is written here;
+//              each nonzero entry into its slot (slot-major: consecutive
+//              cells are consecutive in memory) and a presence bit per slot;
+//              R is written here;
 //   2. count : one thread per column walks the rows that can reference it
 //              (the reverse slot table) in ascending row order and counts
 //              the present entries; an exclusive scan gives the column
@@ -76,7 +77,8 @@ __host__ __device__ __forceinline__ int slot_key(int64_t off, int e, int nu, int
 // whose offset differs, or an offset outside the table, raises flag 1.
 struct SlotSink {
     const SlotTables* __restrict__ T;
-    double* slotv;     // slots of the row's cell, equation e's first slot
+    double* slotv;     // value of slot 0 of the row; slot s at slotv[s * stride]
+    int64_t stride;    // P cells: slots are stored slot-major
     uint32_t* pres;    // presence bits of the row (register)
     int32_t* flags;
     int e, nu, Ny1;
@@ -92,9 +94,9 @@ struct SlotSink {
         if (v != 0.0) {
             const uint32_t bit = 1u << sl;
             if (*pres & bit)
-                slotv[sl] += v;
+                slotv[sl * stride] += v;
             else
-                slotv[sl] = v;
+                slotv[sl * stride] = v;
             *pres |= bit;
         }
     }
@@ -327,42 +329,52 @@ __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A, const SlotTables* _
     if (k >= A.Nx1 * A.Ny1) return;
     const int e = blockIdx.y;
     const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
+    const int64_t ncell = (int64_t)A.Nx1 * A.Ny1;
     uint32_t pres = 0;
-    SlotSink sink{T, slotv + (size_t)k * T->nstot + T->slot0[e], &pres, flags, e,
+    SlotSink sink{T, slotv + (size_t)T->slot0[e] * ncell + k, ncell, &pres, flags, e,
                   SYS == 0 ? 6 : 1, A.Ny1};
     if (SYS == 0)
         stokes_eq(A, e, i, j, sink, true);
     else
         thermal_cell(A, i, j, sink, true);
-    pres_out[(size_t)k * T->ne + e] = (uint16_t)pres;
+    pres_out[(size_t)e * ncell + k] = (uint16_t)pres;
 }
 
-// 2./3. columns: thread per P cell, blockIdx.y = the column's unknown u
+// 2./3. columns: one thread per column (consecutive threads, consecutive
+// columns, so each block writes one contiguous stretch of rowval/nzval); the
+// slot tables are staged in LDS
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_asm_cols(int Nx1, int Ny1,
-                                                  const SlotTables* __restrict__ T,
+                                                  const SlotTables* __restrict__ Tg,
                                                   const uint16_t* __restrict__ pres,
                                                   const double* __restrict__ slotv,
                                                   int32_t* __restrict__ colcnt,
                                                   const int32_t* __restrict__ colptr,
                                                   int32_t* __restrict__ rowval,
                                                   double* __restrict__ nzval) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Nx1 * Ny1) return;
-    const int u = blockIdx.y, ne = T->ne, nstot = T->nstot;
-    const int ic = k % Ny1, jc = k / Ny1;
-    const int64_t col = (int64_t)k * ne + u;
+    __shared__ SlotTables T;
+    static_assert(sizeof(SlotTables) % 4 == 0, "table size");
+    for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
+        ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
+    __syncthreads();
+    const int ne = T.ne;
+    const int64_t ncell = (int64_t)Nx1 * Ny1;
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncell * ne) return;
+    const int u = (int)(col % ne);
+    const int64_t k = col / ne;
+    const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
     int32_t p = FILL ? colptr[col] : 0;
-    const int nr = T->nrev[u];
+    const int nr = T.nrev[u];
     for (int r = 0; r < nr; ++r) {
-        const int ir = ic - T->rev_di[u][r], jr = jc - T->rev_dj[u][r];
+        const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
         if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
-        const int e = T->rev_e[u][r], sl = T->rev_s[u][r];
+        const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
         const int64_t cell = (int64_t)jr * Ny1 + ir;
-        if (!((pres[cell * ne + e] >> sl) & 1)) continue;
+        if (!((pres[(size_t)e * ncell + cell] >> sl) & 1)) continue;
         if (FILL) {
             rowval[p] = (int32_t)(cell * ne + e);
-            nzval[p] = slotv[cell * nstot + T->slot0[e] + sl];
+            nzval[p] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
         }
         ++p;
     }
@@ -513,11 +525,12 @@ static int assemble(ereb_ctx* c, double dt) {
     const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, T, s.pres, s.slotv,
                        c->d_flags);
-    hipLaunchKernelGGL(k_asm_cols<false>, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
+    const dim3 cgrid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_asm_cols<false>, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
                        s.pres, s.slotv, s.colcnt, (const int32_t*)nullptr, (int32_t*)nullptr,
                        (double*)nullptr);
     if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
-    hipLaunchKernelGGL(k_asm_cols<true>, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
+    hipLaunchKernelGGL(k_asm_cols<true>, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
                        s.pres, s.slotv, s.colcnt, s.colptr, s.rowval, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);

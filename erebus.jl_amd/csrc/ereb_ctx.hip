@@ -310,6 +310,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (int f = 0; f < EREB_M_NFIELDS; ++f) hipFree(c->d_m[f]);
     hipFree(c->d_axes);
     hipFree(c->d_mat);
+    hipFree(c->adj_pool);
     hipFree(c->d_grids);
     hipFree(c->d_yny);
     hipFree(c->d_key);

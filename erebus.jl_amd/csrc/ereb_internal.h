@@ -78,6 +78,9 @@ struct ereb_ctx {
 
     ereb::Sys sys[2];
 
+    double* adj_pool = nullptr;  // grow-only device pool of the adjoint driver
+    size_t adj_pool_n = 0;       // doubles
+
     bool prof = false;
     std::vector<ereb::ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;
@@ -135,7 +138,9 @@ int launch_insulating_bc(ereb_ctx* c, double* t, int64_t Nyy, int64_t Nxx);
 // ereb_markers.hip
 int launch_gather_temperature(ereb_ctx* c, int64_t timestep);
 int launch_gather_stress(ereb_ctx* c);
-int launch_rk4(ereb_ctx* c, double dt, int mode);
+// xb_out/yb_out (optional): fluid back-trace end points, for the adjoint
+int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out = nullptr,
+               double* yb_out = nullptr);
 
 // ereb_assembly.hip
 int launch_assemble_stokes(ereb_ctx* c, double dt);
@@ -151,7 +156,7 @@ int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mo
                        double* gphi);
 int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off);
 int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid);
-int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* gout, double* gin,
-                   double* gphi, double* gtk2);
+int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,
+                   const double* gout, double* gin, double* gphi, double* gtk2);
 
 }  // namespace ereb

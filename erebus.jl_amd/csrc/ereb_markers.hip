@@ -148,6 +148,7 @@ struct RK4Args {
     const int32_t* tm;
     const double* phim;
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
+    double *xb_out, *yb_out;  // optional: fluid back-trace end points (adjoint recording)
 };
 
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
@@ -226,6 +227,10 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     rk4_paths(A, xmm, ymm, xn, yn, xr, yr);
     A.xm[m] = xn;
     A.ym[m] = yn;
+    if (A.xb_out) {
+        A.xb_out[m] = xr;
+        A.yb_out[m] = yr;
+    }
     p_stencil(g, xr, yr, ip, jp, w);
     const double tkf0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
     // fluid-solid temperature correction E:5243-5252
@@ -251,9 +256,11 @@ __device__ __forceinline__ void atomic_add4(double* gr, int rows, int i, int j, 
 // RK4^T of the temperature correction E:5243-5252: given the cotangent gout of
 // tkm after the step, the cotangent of tkm before it (gin), the increment of
 // the phim cotangent and the tk2 cotangent (both stencils, by atomics).
-// The marker paths are recomputed from the step-start positions; positions
-// and stresses do not depend on tkm, phim or tk2.
-__global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __restrict__ gout,
+// Positions and stresses do not depend on tkm, phim or tk2; the fluid
+// back-trace end points (xb, yb) were recorded by the forward step.
+__global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __restrict__ xbs,
+                                                 const double* __restrict__ ybs,
+                                                 const double* __restrict__ gout,
                                                  double* __restrict__ gin,
                                                  double* __restrict__ gphi,
                                                  double* __restrict__ gtk2) {
@@ -265,9 +272,7 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
     double ws[4], wb[4];
     p_stencil(g, x0, y0, ips, jps, ws);
     const double tks0 = gdot4(A.tk2, g.Ny1, ips, jps, ws);
-    double xn, yn, xb, yb;
-    rk4_paths(A, x0, y0, xn, yn, xb, yb);
-    p_stencil(g, xb, yb, ipb, jpb, wb);
+    p_stencil(g, xbs[m], ybs[m], ipb, jpb, wb);
     const double tkf0 = gdot4(A.tk2, g.Ny1, ipb, jpb, wb);
     const double d = tkf0 - tks0;
     const int tt = min(max(A.tm[m], 1), 3);
@@ -324,13 +329,13 @@ int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid
 
 static RK4Args rk4_args(ereb_ctx* c, double dt, int mode);
 
-int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* gout, double* gin,
-                   double* gphi, double* gtk2) {
+int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,
+                   const double* gout, double* gin, double* gphi, double* gtk2) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     RK4Args A = rk4_args(c, dt, mode);
     hipLaunchKernelGGL(k_rk4_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
-                       A, gout, gin, gphi, gtk2);
+                       A, xb, yb, gout, gin, gphi, gtk2);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -391,13 +396,16 @@ static RK4Args rk4_args(ereb_ctx* c, double dt, int mode) {
     A.vyf = gptr(c, EREB_G_VYF);
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
+    A.xb_out = A.yb_out = nullptr;
     return A;
 }
 
-int launch_rk4(ereb_ctx* c, double dt, int mode) {
+int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     RK4Args A = rk4_args(c, dt, mode);
+    A.xb_out = xb_out;
+    A.yb_out = yb_out;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     hipLaunchKernelGGL(k_rk4, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A);

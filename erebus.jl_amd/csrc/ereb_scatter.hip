@@ -14,7 +14,7 @@
 //   - Markers arrive sorted by fine key (ereb_sort.hip): a tile's markers are
 //     contiguous and grouped by fine cell in ascending marker index.
 //
-// One workgroup of six wavefronts per tile.  The tile is walked one basic
+// One workgroup of four wavefronts per tile.  The tile is walked one basic
 // column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
 // contiguous run of sorted markers, split into chunks of at most CH markers
 // when it is longer:
@@ -24,11 +24,10 @@
 //               LDS.  Each thread's inputs for the next chunk are loaded into
 //               registers before the current chunk's tasks start, so the
 //               gather latency overlaps phases 2 and 3;
-//   2. tasks  : each wavefront owns a set of sums (basic values 0-3, basic
-//               4-7 with the weight, Vx, Vy, P values 0-4, P 5-9 with the
-//               weight).  A task is (fine cell, corner column cj): the sums
-//               over the fine cell's markers, in order, of property * weight
-//               for the corners (0, cj) and (1, cj);
+//   2. tasks  : wavefront g owns the sums of grid g (basic, Vx, Vy, P).  A
+//               task is (fine cell, corner column cj): the sums over the fine
+//               cell's markers, in order, of property * weight for the
+//               corners (0, cj) and (1, cj), and of the weights;
 //   3. combine: lane (r, c) of a wavefront owns window node (r, c) of its
 //               grid and adds the partials of the at most 8 fine cells of the
 //               column that touch it, in a fixed (column, fine column, fine
@@ -42,7 +41,7 @@
 
 namespace ereb {
 
-constexpr int NTHR = 384;    // six wavefronts: basic (2 halves), Vx, Vy, P (2 halves)
+constexpr int NTHR = 256;    // four wavefronts, one per grid
 constexpr int CH = 256;      // markers per LDS chunk (one per thread in the load phase)
 constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
 constexpr int NCF = 2 * FINE;  // fine cells in one basic column of a tile
@@ -402,34 +401,28 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
         // ---- 2. (fine cell, corner column) partials, wavefront g = grid g --
         const int f0 = cur.k * NCF;
         switch (wave) {
-            case 0: tasks<R, 0, 0, 4>(L, lane, f0, cur.c0, cur.cend); break;
-            case 1: tasks<R, 0, 4, 4>(L, lane, f0, cur.c0, cur.cend); break;
-            case 2: tasks<R, 1, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
-            case 3: tasks<R, 2, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
-            case 4: tasks<R, 3, 0, 5>(L, lane, f0, cur.c0, cur.cend); break;
-            default: tasks<R, 3, 5, 5>(L, lane, f0, cur.c0, cur.cend); break;
+            case 0: tasks<R, 0, 0, 8>(L, lane, f0, cur.c0, cur.cend); break;
+            case 1: tasks<R, 1, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+            case 2: tasks<R, 2, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+            default: tasks<R, 3, 0, 10>(L, lane, f0, cur.c0, cur.cend); break;
         }
         __syncthreads();
         // ---- 3. node-owned combine -----------------------------------------
         switch (wave) {
-            case 0: combine<R, 0, 0, 4>(L, acc, r, c, cur.k); break;
-            case 1: combine<R, 0, 4, 4>(L, acc, r, c, cur.k); break;
-            case 2: combine<R, 1, 0, 6>(L, acc, r, c, cur.k); break;
-            case 3: combine<R, 2, 0, 6>(L, acc, r, c, cur.k); break;
-            case 4: combine<R, 3, 0, 5>(L, acc, r, c, cur.k); break;
-            default: combine<R, 3, 5, 5>(L, acc, r, c, cur.k); break;
+            case 0: combine<R, 0, 0, 8>(L, acc, r, c, cur.k); break;
+            case 1: combine<R, 1, 0, 6>(L, acc, r, c, cur.k); break;
+            case 2: combine<R, 2, 0, 6>(L, acc, r, c, cur.k); break;
+            default: combine<R, 3, 0, 10>(L, acc, r, c, cur.k); break;
         }
         __syncthreads();
         cur = nx;
         nx = nx2;
     }
     switch (wave) {
-        case 0: write_part<R, 0, 0, 4>(A.part, tile, acc, lane); break;
-        case 1: write_part<R, 0, 4, 4>(A.part, tile, acc, lane); break;
-        case 2: write_part<R, 1, 0, 6>(A.part, tile, acc, lane); break;
-        case 3: write_part<R, 2, 0, 6>(A.part, tile, acc, lane); break;
-        case 4: write_part<R, 3, 0, 5>(A.part, tile, acc, lane); break;
-        default: write_part<R, 3, 5, 5>(A.part, tile, acc, lane); break;
+        case 0: write_part<R, 0, 0, 8>(A.part, tile, acc, lane); break;
+        case 1: write_part<R, 1, 0, 6>(A.part, tile, acc, lane); break;
+        case 2: write_part<R, 2, 0, 6>(A.part, tile, acc, lane); break;
+        default: write_part<R, 3, 0, 10>(A.part, tile, acc, lane); break;
     }
 }
 

@@ -14,31 +14,54 @@
 //   2. scan         : key starts
 //   3. k_fine_place : wavefront-aggregated slot reservation per key
 //   4. k_fine_order : each key segment (about 9 markers) is sorted by marker
-//                     index in place, removing the arrival order of step 3;
+//                     index in place (in registers up to 16 markers),
+//                     removing the arrival order of step 3;
 //                     segments longer than SEG_THREAD go to k_fine_order_big
 //                     (bitonic in LDS up to 16384, global memory beyond).
 #include "ereb_internal.h"
 
 namespace ereb {
 
-// For every active lane: reserve one slot in counters[key]; returns the
-// slot index.  One atomic per distinct key in the wavefront.
-__device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key, bool active) {
+// Lanes of a wavefront grouped by key: the lowest lane of each group is its
+// leader; every lane learns its leader, its rank in the group, and a leader
+// the group size.  Ballots and shuffles only, no memory traffic.
+struct KeyGroup {
+    int leader, rank, count;  // count is non-zero on leaders only
+};
+__device__ __forceinline__ KeyGroup wave_group(uint32_t key, bool active) {
     uint64_t todo = __ballot(active);
-    int32_t result = 0;
+    KeyGroup g{(int)__lane_id(), 0, 0};
     const int lane = __lane_id();
     while (todo) {
         const int leader = __ffsll((unsigned long long)todo) - 1;
         const uint32_t lk = __shfl(key, leader);
         const bool mine = active && key == lk;
         const uint64_t m = __ballot(mine);
-        int32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counters[lk], (int32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (mine) result = base + (int32_t)__popcll(m & lanemask_lt());
+        if (mine) {
+            g.leader = leader;
+            g.rank = (int)__popcll(m & lanemask_lt());
+        }
+        if (lane == leader) g.count = (int)__popcll(m);
         todo &= ~m;
     }
-    return result;
+    return g;
+}
+
+// For every active lane: reserve one slot in counters[key]; returns the slot
+// index.  All leaders issue their atomic together (one round trip per
+// wavefront); within a key the slots follow lane order.
+__device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key, bool active) {
+    const KeyGroup g = wave_group(key, active);
+    int32_t base = 0;
+    if (g.count > 0) base = atomicAdd(&counters[key], g.count);
+    base = __shfl(base, g.leader);
+    return base + g.rank;
+}
+
+// counters[key] += number of active lanes with that key (no return value)
+__device__ __forceinline__ void wave_count(int32_t* counters, uint32_t key, bool active) {
+    const KeyGroup g = wave_group(key, active);
+    if (g.count > 0) atomicAdd(&counters[key], g.count);
 }
 
 // fine key of a marker (see the file header); sets *bad when the P cell is
@@ -68,7 +91,7 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
         if (bad) flags[0] = 1;
         key[m] = k;
     }
-    wave_reserve(count, k, active);
+    wave_count(count, k, active);
 }
 
 __global__ __launch_bounds__(256) void k_copy_i32(const int32_t* __restrict__ a,
@@ -87,6 +110,7 @@ __global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__
     if (active) perm[pos] = (uint32_t)m;
 }
 
+constexpr int SEG_REG = 16;     // segments ordered in registers by rank
 constexpr int SEG_THREAD = 48;  // longest segment ordered by one thread
 constexpr int SEG_BIG = 16384;  // longest segment ordered in LDS (64 KB)
 
@@ -100,6 +124,22 @@ __global__ __launch_bounds__(256) void k_fine_order(const int32_t* __restrict__ 
     if (e - a > SEG_THREAD) {
         const int slot = atomicAdd(&big[0], 1);
         big[1 + slot] = (int32_t)k;
+        return;
+    }
+    if (e - a <= SEG_REG) {
+        // registers: each value's rank among the (distinct) segment values is
+        // its position
+        uint32_t v[SEG_REG];
+        const int n = e - a;
+#pragma unroll
+        for (int i = 0; i < SEG_REG; ++i) v[i] = i < n ? perm[a + i] : 0xffffffffu;
+#pragma unroll
+        for (int i = 0; i < SEG_REG; ++i) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < SEG_REG; ++j) r += (v[j] < v[i]) ? 1 : 0;
+            if (i < n) perm[a + r] = v[i];
+        }
         return;
     }
     for (int32_t p = a + 1; p < e; ++p) {  // insertion sort, ascending
