@@ -36,7 +36,10 @@ __device__ __forceinline__ double gdot4_at(const double* gr, int rows, int i, in
     return gdot4(gr, rows, i, j, w);
 }
 
-// scatter^T: one thread per marker (storage order)
+// scatter^T: one thread per marker (storage order).  FULL = false reads only
+// the RHOCPSUM and TKSUM cotangents (all others zero), as for a loss that
+// depends on the sums through tk1 alone.
+template <bool FULL>
 __global__ __launch_bounds__(256) void k_scatter_adj(ScatterAdjArgs A) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
@@ -57,6 +60,13 @@ __global__ __launch_bounds__(256) void k_scatter_adj(ScatterAdjArgs A) {
     auto G = [&](int slot) { return A.gs + A.goff[slot]; };
     const int R = g.Ny1;
     double gT = 0.0, gp = 0.0;
+    if (!FULL) {
+        const double grc = gdot4_at(G(21), R, cl.ip, cl.jp, wp);
+        const double gtk = gdot4_at(G(27), R, cl.ip, cl.jp, wp);
+        A.gT[m] += grc * d.rhocp_t + gtk * d.tk_t;
+        A.gphi[m] += grc * d.rhocp_p + gtk * d.tk_p;
+        return;
+    }
     // Vx (slots 8..13: RHO RHOF K PHI R WT) and Vy (14..19)
     for (int v = 0; v < 2; ++v) {
         const int s0 = v == 0 ? 8 : 14;
@@ -167,7 +177,7 @@ __global__ void k_bc_adj_rows(double* t, int Nyy, int Nxx) {
 
 int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
                        const double* gsum_base, const int64_t* gsum_off, double* gT,
-                       double* gphi) {
+                       double* gphi, bool only_tk) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     if (c->precision != EREB_FP64)
@@ -196,8 +206,8 @@ int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mo
     for (int s = 0; s < NSUM; ++s) A.goff[s] = gsum_off[s];
     A.gT = gT;
     A.gphi = gphi;
-    hipLaunchKernelGGL(k_scatter_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                       c->stream, A);
+    hipLaunchKernelGGL(only_tk ? k_scatter_adj<false> : k_scatter_adj<true>,
+                       dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -414,7 +424,7 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
         hipLaunchKernelGGL(k_twice, dim3(nbg), dim3(256), 0, c->stream,
                            (const double*)gptr(c, EREB_G_TK1), G(EREB_G_TK1), n);
         if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
-        if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT, B.gP)))
+        if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT, B.gP, true)))
             return rc;
         for (int64_t sg = nseg - 1; sg >= 0; --sg) {
             const int64_t k0 = sg * every, k1 = std::min(nsteps, k0 + every);
@@ -445,7 +455,8 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
                 if (k == 0)
                     EREB_HIP(c, hipMemcpyAsync(mptr(c, EREB_M_TKM), stk + 2 * (size_t)M, MB,
                                                hipMemcpyDeviceToDevice, c->stream));
-                if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT2, B.gP)))
+                if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT2, B.gP,
+                                             true)))
                     return rc;
                 std::swap(B.gT, B.gT2);
             }

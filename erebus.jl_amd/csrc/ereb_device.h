@@ -25,12 +25,14 @@ struct GeomK {
 
 // unsafe_trunc(Int, t) + 1 clamped to [lo, hi] (E:1457-1468).  NaN and
 // |t| >= 2^63 give typemin(Int64) on x86-64, hence lo.
+// Branch-free: NaN and t >= 2^63 go to -1e9 (hence lo), then the value is
+// clamped to [-1e9, 1e9] so the conversion stays in range; for every input
+// the clamped result equals the case analysis above.
 __device__ __forceinline__ int jl_cell(double t, int lo, int hi) {
-    if (!(t >= -9223372036854775808.0 && t < 9223372036854775808.0)) return lo;
-    if (t >= 1.0e9) return hi;
-    if (t <= -1.0e9) return lo;
-    int c = (int)t + 1;
-    return c < lo ? lo : (c > hi ? hi : c);
+    const double t2 = t < 0x1p63 ? t : -1.0e9;
+    const double tc = fmin(fmax(t2, -1.0e9), 1.0e9);
+    const int c = (int)tc + 1;
+    return min(max(c, lo), hi);
 }
 
 // Cell indices of a marker: basic (ib, jb) with the basic clamps and P

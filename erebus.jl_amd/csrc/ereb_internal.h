@@ -153,7 +153,7 @@ int sum_grid_field(int s);
 // buffer with the layout of the grid fields (offsets goff)
 int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
                        const double* gsum_base, const int64_t* gsum_off, double* gT,
-                       double* gphi);
+                       double* gphi, bool only_tk);
 int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off);
 int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid);
 int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,

@@ -93,19 +93,15 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
     const double fx = dxmj * idx, fy = dymi * idy;
     double v13 = p[0] * (1.0 - fx) + (p[R] * dxmj) * idx;
     double v24 = p[1] * (1.0 - fx) + (p[R + 1] * dxmj) * idx;
-    if (fx >= 0.5) {
-        if (j < g.Nx - 1) {
-            const double t = fx - 0.5;
-            v13 += 0.5 * (t * t) * (p[0] - 2.0 * p[R] + p[2 * R]);
-            v24 += 0.5 * (t * t) * (p[1] - 2.0 * p[R + 1] + p[2 * R + 1]);
-        }
-    } else {
-        if (j > 1) {
-            const double t = fx - 0.5;
-            const double cc = vc[(size_t)(j - 1) * R + (i - 1)];
-            v13 += 0.5 * (t * t) * (p[-R] - 2.0 * cc + p[R]);
-            v24 += 0.5 * (t * t) * (p[1 - R] - 2.0 * p[1] + p[R + 1]);
-        }
+    // second-order correction on columns j..j+2 (fx >= 0.5) or j-1..j+1; one
+    // code path for both, so lanes on either side do not diverge
+    const bool right = fx >= 0.5;
+    if (right ? j < g.Nx - 1 : j > 1) {
+        const double t = fx - 0.5;
+        const double* q = right ? p : p - R;  // first column of the stencil
+        const double c13 = right ? p[R] : vc[(size_t)(j - 1) * R + (i - 1)];
+        v13 += 0.5 * (t * t) * (q[0] - 2.0 * c13 + q[2 * R]);
+        v24 += 0.5 * (t * t) * (q[1] - 2.0 * q[R + 1] + q[2 * R + 1]);
     }
     return v13 * (1.0 - fy) + (v24 * dymi) * idy;
 }
@@ -122,18 +118,13 @@ __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict
     const double fx = dxmj * idx, fy = dymi * idy;
     double v12 = p[0] * (1.0 - fy) + (p[1] * dymi) * idy;
     double v34 = p[R] * (1.0 - fy) + (p[R + 1] * dymi) * idy;
-    if (fy >= 0.5) {
-        if (i < g.Ny - 1) {
-            const double t = fy - 0.5;
-            v12 += 0.5 * (t * t) * (p[0] - 2.0 * p[1] + p[2]);
-            v34 += 0.5 * (t * t) * (p[R] - 2.0 * p[R + 1] + p[R + 2]);
-        }
-    } else {
-        if (i > 1) {
-            const double t = fy - 0.5;
-            v12 += 0.5 * (t * t) * (p[-1] - 2.0 * p[0] + p[1]);
-            v34 += 0.5 * (t * t) * (p[R - 1] - 2.0 * p[R] + p[R + 1]);
-        }
+    // second-order correction on rows i..i+2 (fy >= 0.5) or i-1..i+1
+    const bool down = fy >= 0.5;
+    if (down ? i < g.Ny - 1 : i > 1) {
+        const double t = fy - 0.5;
+        const double* q = down ? p : p - 1;  // first row of the stencil
+        v12 += 0.5 * (t * t) * (q[0] - 2.0 * q[1] + q[2]);
+        v34 += 0.5 * (t * t) * (q[R] - 2.0 * q[R + 1] + q[R + 2]);
     }
     return v12 * (1.0 - fx) + (v34 * dxmj) * idx;
 }
@@ -253,6 +244,69 @@ __device__ __forceinline__ void atomic_add4(double* gr, int rows, int i, int j, 
     atomicAdd(p + rows + 1, w[3] * v);
 }
 
+constexpr int ADJ_WIN = 2048;  // LDS window of the workgroup pre-aggregation (doubles)
+
+struct AddWin {
+    double v[ADJ_WIN];
+    int box[4];  // imin, imax, jmin, jmax (1-based cells)
+};
+
+// Workgroup-wide gr[stencil of (i, j)] += w * v (bilinear, `rows` rows): the
+// contributions are first summed in an LDS window over the workgroup's cells
+// (LDS atomics), then added to gr with one atomic per touched node; when the
+// cells span more than the window, every lane adds straight to gr.  Every
+// thread of the workgroup must call it (it synchronises); `on` marks the
+// lanes that contribute.
+__device__ void block_add4(double* gr, int rows, bool on, int i, int j, const double* w,
+                           double v, AddWin& W) {
+    on = on && v != 0.0;
+    int imin = on ? i : INT32_MAX, imax = on ? i : INT32_MIN;
+    int jmin = on ? j : INT32_MAX, jmax = on ? j : INT32_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        imin = min(imin, __shfl_xor(imin, o));
+        imax = max(imax, __shfl_xor(imax, o));
+        jmin = min(jmin, __shfl_xor(jmin, o));
+        jmax = max(jmax, __shfl_xor(jmax, o));
+    }
+    __syncthreads();  // every thread is done with the previous call's box and window
+    if (threadIdx.x == 0) {
+        W.box[0] = INT32_MAX;
+        W.box[1] = INT32_MIN;
+        W.box[2] = INT32_MAX;
+        W.box[3] = INT32_MIN;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&W.box[0], imin);
+        atomicMax(&W.box[1], imax);
+        atomicMin(&W.box[2], jmin);
+        atomicMax(&W.box[3], jmax);
+    }
+    __syncthreads();
+    const int i0 = W.box[0], i1 = W.box[1], j0 = W.box[2], j1 = W.box[3];
+    if (i0 > i1) return;  // uniform: no contribution in the workgroup
+    const int h = i1 - i0 + 2, wdt = j1 - j0 + 2;  // nodes i0..i1+1, j0..j1+1
+    if ((int64_t)h * wdt > ADJ_WIN) {
+        if (on) atomic_add4(gr, rows, i, j, w, v);
+        return;
+    }
+    for (int q = threadIdx.x; q < h * wdt; q += blockDim.x) W.v[q] = 0.0;
+    __syncthreads();
+    if (on) {
+        double* p = W.v + (j - j0) * h + (i - i0);
+        atomicAdd(p, w[0] * v);
+        atomicAdd(p + 1, w[1] * v);
+        atomicAdd(p + h, w[2] * v);
+        atomicAdd(p + h + 1, w[3] * v);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < h * wdt; q += blockDim.x) {
+        const double a = W.v[q];
+        if (a != 0.0) atomicAdd(&gr[(size_t)(j0 - 1 + q / h) * rows + (i0 - 1 + q % h)], a);
+    }
+    __syncthreads();  // the window is reused by the next call
+}
+
 // RK4^T of the temperature correction E:5243-5252: given the cotangent gout of
 // tkm after the step, the cotangent of tkm before it (gin), the increment of
 // the phim cotangent and the tk2 cotangent (both stencils, by atomics).
@@ -264,8 +318,10 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
                                                  double* __restrict__ gin,
                                                  double* __restrict__ gphi,
                                                  double* __restrict__ gtk2) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= A.n) return;
+    __shared__ AddWin W;
+    const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = m0 < A.n;
+    const int64_t m = on ? m0 : A.n - 1;  // idle lanes read a valid marker, add nothing
     const GeomK& g = A.g;
     const double x0 = A.xm[m], y0 = A.ym[m];
     int ips, jps, ipb, jpb;
@@ -286,13 +342,13 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
                        num * (ph * drcf) / (den * den);
     const double F_p = (-T * rcs + (T + d) * rcf) / den - num * (rcf - rcs) / (den * den);
     const double go = gout[m];
-    gin[m] = go * F_T;
-    gphi[m] += go * F_p;
-    const double gd = go * F_d;
-    if (gd != 0.0) {
-        atomic_add4(gtk2, g.Ny1, ipb, jpb, wb, gd);
-        atomic_add4(gtk2, g.Ny1, ips, jps, ws, -gd);
+    if (on) {
+        gin[m] = go * F_T;
+        gphi[m] += go * F_p;
     }
+    const double gd = go * F_d;
+    block_add4(gtk2, g.Ny1, on, ipb, jpb, wb, gd, W);
+    block_add4(gtk2, g.Ny1, on, ips, jps, ws, -gd, W);
 }
 
 #pragma clang fp contract(off)
@@ -305,16 +361,18 @@ __global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __r
                                                       double* __restrict__ gT,
                                                       double* __restrict__ ggrid, int64_t M,
                                                       int first) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
+    __shared__ AddWin W;
+    const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = m0 < M;
+    const int64_t m = on ? m0 : M - 1;
     const double x = xm[m], y = ym[m];
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     double w[4];
     weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
     const double v = gT[m];
-    if (v != 0.0) atomic_add4(ggrid, g.Ny1, ip, jp, w, v);
-    if (first) gT[m] = 0.0;
+    block_add4(ggrid, g.Ny1, on, ip, jp, w, v, W);
+    if (on && first) gT[m] = 0.0;
 }
 
 int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid) {
