@@ -467,6 +467,7 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
             EREB_HIP(c, hipMemcpyAsync(grad_phim, B.gP, MB, hipMemcpyDeviceToHost, c->stream));
     }
     if ((rc = state_copy(c, B.save, 5, false))) return rc;
+    ++c->pos_gen;  // the last sort was of moved positions
     EREB_HIP(c, hipStreamSynchronize(c->stream));
     return check_flags(c);
 }

@@ -43,7 +43,7 @@ GeomK geomk(const ereb_ctx* c) {
 }
 
 bool grid_is_basic(int f) {
-    return (f >= EREB_G_ETA0SUM && f <= EREB_G_DSXY);
+    return (f >= EREB_G_ETA0SUM && f <= EREB_G_DSXY) || f == EREB_G_YNY_INV_ETA;
 }
 
 double* gptr(ereb_ctx* c, int f) { return c->d_grids + c->goff[f]; }
@@ -203,6 +203,7 @@ static int reserve(ereb_ctx* c, int64_t n) {
     c->tmp_cap = 2 * n;
     c->cap = n;
     c->have_sort = false;
+    ++c->pos_gen;
     return EREB_OK;
 }
 
@@ -406,6 +407,7 @@ int ereb_markers_set(ereb_ctx* c, int field, const void* host, int64_t marknum) 
                                    hipMemcpyHostToDevice, c->stream));
         EREB_HIP(c, hipStreamSynchronize(c->stream));
     }
+    if (field == EREB_M_XM || field == EREB_M_YM || marknum != c->marknum) ++c->pos_gen;
     c->marknum = marknum;
     return EREB_OK;
 }
@@ -861,6 +863,126 @@ int ereb_assemble_hydromechanical_lse(ereb_ctx* c, const double* ETA, const doub
     int rc = ereb_step_assemble_stokes_darcy(c, dt, nnz);
     if (rc) return rc;
     return ereb_csc_get(c, EREB_SYS_STOKES_DARCY, nullptr, nullptr, nullptr, R);
+}
+
+// ---- viscosity, compaction and (de)hydration ----------------------------------
+
+static int valid_hydration(ereb_ctx* c, const ereb_hydration* h) {
+    if (!h) return fail(c, EREB_ERR_ARG, "ereb_hydration constants missing");
+    const int64_t m = h->rate_mode;
+    if (m == 1 || m == 2 || m == 3 || m == 9) return EREB_OK;
+    char buf[64];
+    snprintf(buf, sizeof buf, "unknown mode %lld", (long long)m);
+    return fail(c, EREB_ERR_MODE, buf);
+}
+
+int ereb_step_marker_viscosity(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_marker_viscosity(c);
+}
+
+int ereb_step_marker_porosity(ereb_ctx* c, const ereb_hydration* hyd, double dt) {
+    if (!c) return EREB_ERR_ARG;
+    if (!hyd) return fail(c, EREB_ERR_ARG, "ereb_hydration constants missing");
+    return launch_marker_porosity(c, *hyd, dt);
+}
+
+int ereb_step_melt_composition(ereb_ctx* c) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_melt_composition(c);
+}
+
+int ereb_step_thermochemical_reaction(ereb_ctx* c, const ereb_hydration* hyd, double dt,
+                                      int64_t timestep, int64_t titer) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = valid_hydration(c, hyd);
+    if (rc) return rc;
+    return launch_thermochemical(c, *hyd, dt, timestep, titer);
+}
+
+int ereb_update_marker_viscosity(ereb_ctx* c, const double* xm, const double* ym,
+    const int64_t* tm, const double* tkm, double* etatotalm, double* etavpm, const uint8_t* YNY,
+    const double* YNY_inv_ETA, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (!YNY || !etavpm) return fail(c, EREB_ERR_ARG, "ereb_update_marker_viscosity: null array");
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TM, tm);
+    SETM(EREB_M_TKM, tkm);
+    SETM(EREB_M_ETATOTALM, etatotalm);  // air markers keep theirs (E:789-792)
+    int rc = ereb_yny_set(c, YNY);
+    if (rc) return rc;
+    SETG(EREB_G_YNY_INV_ETA, YNY_inv_ETA);
+    if ((rc = launch_marker_viscosity(c))) return rc;
+    GETM(EREB_M_ETATOTALM, etatotalm);
+    GETM(EREB_M_ETAVPM, etavpm);
+    return EREB_OK;
+}
+
+int ereb_update_marker_porosity(ereb_ctx* c, const ereb_hydration* hyd, const double* xm,
+    const double* ym, const int64_t* tm, double* phim, const double* APHI, double dt,
+    int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (!hyd) return fail(c, EREB_ERR_ARG, "ereb_hydration constants missing");
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TM, tm);
+    SETM(EREB_M_PHIM, phim);
+    SETG(EREB_G_APHI, APHI);
+    int rc = launch_marker_porosity(c, *hyd, dt);
+    if (rc) return rc;
+    GETM(EREB_M_PHIM, phim);
+    return EREB_OK;
+}
+
+int ereb_update_p_nodes_melt_composition(ereb_ctx* c, const double* xm, const double* ym,
+    const double* XWsolidm0, double* XWS, double* XWSSUM, double* WTPSUM, int64_t marknum) {
+    if (!c) return EREB_ERR_ARG;
+    if (!XWS || !XWSSUM || !WTPSUM)
+        return fail(c, EREB_ERR_ARG, "ereb_update_p_nodes_melt_composition: null array");
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_XWSOLIDM0, XWsolidm0);
+    int rc = launch_melt_composition(c);
+    if (rc) return rc;
+    GETG(EREB_G_XWS, XWS);
+    GETG(EREB_G_XWSSUM, XWSSUM);
+    GETG(EREB_G_WTPSUM, WTPSUM);
+    return EREB_OK;
+}
+
+int ereb_perform_thermochemical_reaction(ereb_ctx* c, const ereb_hydration* hyd, double* DMP,
+    double* DHP, double* DMPSUM, double* DHPSUM, double* WTPSUM, const double* pf,
+    const double* tk2, const int64_t* tm, const double* xm, const double* ym, double* XWsm0,
+    double* XWsm, double* phim, double* phinewm, double* pfm0, int64_t marknum, double dt,
+    int64_t timestep, int64_t titer) {
+    if (!c) return EREB_ERR_ARG;
+    int rc = valid_hydration(c, hyd);
+    if (rc) return rc;
+    if (!DMP || !DHP || !DMPSUM || !DHPSUM || !WTPSUM)
+        return fail(c, EREB_ERR_ARG, "ereb_perform_thermochemical_reaction: null array");
+    SETM(EREB_M_XM, xm);
+    SETM(EREB_M_YM, ym);
+    SETM(EREB_M_TM, tm);
+    SETM(EREB_M_XWSOLIDM0, XWsm0);
+    SETM(EREB_M_XWSOLIDM, XWsm);
+    SETM(EREB_M_PHIM, phim);
+    SETM(EREB_M_PHINEWM, phinewm);
+    SETM(EREB_M_PFM0, pfm0);
+    SETG(EREB_G_PF, pf);
+    SETG(EREB_G_TK2, tk2);
+    if ((rc = launch_thermochemical(c, *hyd, dt, timestep, titer))) return rc;
+    GETG(EREB_G_DMP, DMP);
+    GETG(EREB_G_DHP, DHP);
+    GETG(EREB_G_DMPSUM, DMPSUM);
+    GETG(EREB_G_DHPSUM, DHPSUM);
+    GETG(EREB_G_WTPSUM, WTPSUM);
+    GETM(EREB_M_XWSOLIDM0, XWsm0);
+    GETM(EREB_M_XWSOLIDM, XWsm);
+    GETM(EREB_M_PHIM, phim);
+    GETM(EREB_M_PHINEWM, phinewm);
+    GETM(EREB_M_PFM0, pfm0);
+    return EREB_OK;
 }
 
 int ereb_assemble_thermal_lse(ereb_ctx* c, const double* tk1, const double* RHOCP,

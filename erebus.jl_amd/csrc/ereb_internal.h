@@ -75,6 +75,9 @@ struct ereb_ctx {
     int32_t* d_scan = nullptr;     // scan block sums
     int64_t scan_cap = 0;
     bool have_sort = false;
+    // the sort is current while sort_gen == pos_gen; pos_gen advances
+    // whenever positions or the marker count may have changed
+    uint64_t pos_gen = 1, sort_gen = 0;
 
     ereb::Sys sys[2];
 
@@ -125,6 +128,8 @@ int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);
 
 // ereb_sort.hip: stable counting sort of marker indices by fine key
 int launch_fine_sort(ereb_ctx* c);
+// runs the sort unless the last one still matches the resident positions
+int ensure_sort(ereb_ctx* c);
 
 // ereb_scatter.hip
 int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
@@ -148,6 +153,13 @@ int launch_assemble_thermal(ereb_ctx* c, double dt);
 
 // grid field of SUM slot s (order of reset_interpolated_properties!)
 int sum_grid_field(int s);
+
+// ereb_hydration.hip
+int launch_marker_viscosity(ereb_ctx* c);
+int launch_marker_porosity(ereb_ctx* c, const ereb_hydration& H, double dt);
+int launch_melt_composition(ereb_ctx* c);
+int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64_t timestep,
+                          int64_t titer);
 
 // adjoints (ereb_adjoint.hip, ereb_markers.hip); cotangent grids live in a
 // buffer with the layout of the grid fields (offsets goff)

@@ -464,6 +464,7 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     RK4Args A = rk4_args(c, dt, mode);
     A.xb_out = xb_out;
     A.yb_out = yb_out;
+    ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     hipLaunchKernelGGL(k_rk4, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A);

@@ -226,7 +226,13 @@ int launch_fine_sort(ereb_ctx* c) {
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_SORT, ev);
     c->have_sort = true;
+    c->sort_gen = c->pos_gen;
     return EREB_OK;
+}
+
+int ensure_sort(ereb_ctx* c) {
+    if (c->have_sort && c->sort_gen == c->pos_gen) return EREB_OK;
+    return launch_fine_sort(c);
 }
 
 }  // namespace ereb

@@ -22,6 +22,8 @@ __all__ = [
     "compute_p_node_properties_", "apply_insulating_boundary_conditions_",
     "update_marker_temperature_", "update_marker_stress_", "move_markers_rk4_",
     "assemble_hydromechanical_lse_", "assemble_thermal_lse_", "CSC",
+    "update_marker_viscosity_", "update_marker_porosity_",
+    "update_p_nodes_melt_composition_", "perform_thermochemical_reaction_",
 ]
 
 
@@ -193,3 +195,49 @@ def assemble_thermal_lse_(ctx: Context, tk1, RHOCP, KX, KY, HR, HA, HS, DHP, RT,
         ctx.h, *grids, _io(RT), float(dt), ctypes.addressof(nnz)), "assemble_thermal_lse!")
     cp, rv, nz, _ = ctx.csc(SYS_THERMAL)
     return CSC(RT.size, cp, rv, nz)
+
+
+def update_marker_viscosity_(ctx: Context, xm, ym, tm, tkm, etatotalm, etavpm, YNY,
+                             YNY_inv_ETA, marknum):
+    """update_marker_viscosity! (E:774) for m = 1..marknum (the loop
+    E:6950-6953); YNY is the Bool (Ny, Nx) yielding mask."""
+    keep = []
+    ctx.check(ctx.lib.ereb_update_marker_viscosity(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _in(tm, np.int64, keep=keep),
+        _in(tkm, keep=keep), _io(etatotalm), _io(etavpm),
+        _in(YNY, np.uint8, grid=True, keep=keep), _in(YNY_inv_ETA, grid=True, keep=keep),
+        int(marknum)), "update_marker_viscosity!")
+
+
+def update_marker_porosity_(ctx: Context, xm, ym, tm, phim, APHI, dt, marknum, hyd=None):
+    """update_marker_porosity! (E:4822); hyd: the constants of C:227-279
+    (ereb.Hydration, defaults as src/constants.jl)."""
+    keep = []
+    ctx.check(ctx.lib.ereb_update_marker_porosity(
+        ctx.h, ctx._hyd(hyd), _in(xm, keep=keep), _in(ym, keep=keep),
+        _in(tm, np.int64, keep=keep), _io(phim), _in(APHI, grid=True, keep=keep), float(dt),
+        int(marknum)), "update_marker_porosity!")
+
+
+def update_p_nodes_melt_composition_(ctx: Context, xm, ym, XWsolidm0, XWS, XWSSUM, WTPSUM,
+                                     marknum):
+    """update_p_nodes_melt_composition! (E:1929)"""
+    keep = []
+    ctx.check(ctx.lib.ereb_update_p_nodes_melt_composition(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _in(XWsolidm0, keep=keep),
+        _io(XWS, grid=True), _io(XWSSUM, grid=True), _io(WTPSUM, grid=True), int(marknum)),
+        "update_p_nodes_melt_composition!")
+
+
+def perform_thermochemical_reaction_(ctx: Context, DMP, DHP, DMPSUM, DHPSUM, WTPSUM, pf, tk2,
+                                     tm, xm, ym, XWsm0, XWsm, phim, phinewm, pfm0, marknum,
+                                     dt, timestep, titer, hyd=None):
+    """perform_thermochemical_reaction! (E:2558), without its @info lines"""
+    keep = []
+    ctx.check(ctx.lib.ereb_perform_thermochemical_reaction(
+        ctx.h, ctx._hyd(hyd), _io(DMP, grid=True), _io(DHP, grid=True),
+        _io(DMPSUM, grid=True), _io(DHPSUM, grid=True), _io(WTPSUM, grid=True),
+        _in(pf, grid=True, keep=keep), _in(tk2, grid=True, keep=keep),
+        _in(tm, np.int64, keep=keep), _in(xm, keep=keep), _in(ym, keep=keep), _io(XWsm0),
+        _io(XWsm), _io(phim), _io(phinewm), _io(pfm0), int(marknum), float(dt),
+        int(timestep), int(titer)), "perform_thermochemical_reaction!")

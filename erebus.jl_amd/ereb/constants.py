@@ -160,6 +160,55 @@ class Materials:
         return np.array(hrs, dtype=np.float64), np.array(hrf, dtype=np.float64)
 
 
+class CHydration(ctypes.Structure):
+    """Mirror of ``ereb_hydration`` (include/ereb.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in [
+        "phimin", "phimax", "RG", "VH2Of", "VH2Ofi", "dHWD", "dSWD", "dVWD", "pfcoeff",
+        "dtreaction", "log_completion_rate", "A_I", "b_I", "c_I", "Sxo_B", "Tscl_B", "To_B",
+        "Sxo_T", "To_T", "Ea_T"]] + [("rate_mode", ctypes.c_int64)]
+
+
+@dataclass
+class Hydration:
+    """Compaction and silicate (de)hydration constants (C:227-279)."""
+    phimin: float = 1.0e-4                            # C:227
+    RG: float = 8.314                                 # C:232
+    dHWD: float = 40000.0                             # C:254
+    dSWD: float = 60.0                                # C:256
+    pfcoeff: float = 0.5                              # C:260
+    dtreaction: float = 1.0e10                        # C:266
+    completion_rate: float = 0.01                     # C:268 (log taken)
+    A_I: float = 1.0e-11                              # C:273
+    b_I: float = 2.5e-4
+    c_I: float = 543.0
+    Sxo_B: float = 2.0e-11                            # C:276
+    Tscl_B: float = 10.0
+    To_B: float = 293.0
+    Sxo_T: float = 2.0e-11                            # C:279
+    To_T: float = 293.0
+    Ea_T: float = 63.8e3
+    rate_mode: int = 1                                # C:271
+
+    @property
+    def phimax(self) -> float:     # C:229
+        return 1.0 - self.phimin
+
+    def to_c(self, mat: "Materials") -> CHydration:
+        """Derived constants from the material set: VH2O (C:250-252), dVWD
+        = VDs + VH2Of - VWs (C:258)."""
+        c = CHydration()
+        for n in ["phimin", "RG", "dHWD", "dSWD", "pfcoeff", "dtreaction", "A_I", "b_I",
+                  "c_I", "Sxo_B", "Tscl_B", "To_B", "Sxo_T", "To_T", "Ea_T"]:
+            setattr(c, n, float(getattr(self, n)))
+        c.phimax = self.phimax
+        c.VH2Of = mat.MH2O / mat.rhoH2Of
+        c.VH2Ofi = mat.MH2O / mat.rhoH2Ofi
+        c.dVWD = mat.VDs + c.VH2Of - mat.VWs
+        c.log_completion_rate = math.log(self.completion_rate)
+        c.rate_mode = int(self.rate_mode)
+        return c
+
+
 @dataclass
 class Geometry:
     """Staggered-grid geometry (C:14-65) and marker lattice (C:102-124)."""

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .constants import CGeometry, CMaterials, Geometry, Materials
+from .constants import CGeometry, CMaterials, Geometry, Hydration, Materials
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "lib", "libereb_hip.so"))
@@ -24,6 +24,8 @@ MARKER_FIELDS = [
     "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
     "alphasolidcur", "alphafluidcur", "rhototalm", "rhocptotalm", "etatotalm",
     "hrtotalm", "ktotalm", "tkm_rhocptotalm", "etafluidcur_inv_kphim",
+    # hydration state (E:2558-2694)
+    "XWsolidm", "phinewm", "pfm0",
 ]
 MARKER_ID = {n: k for k, n in enumerate(MARKER_FIELDS)}
 
@@ -42,9 +44,13 @@ GRID_FIELDS = [
     "TKSUM", "PHISUM", "WTPSUM",
     "RHO", "RHOCP", "ALPHA", "ALPHAF", "HR", "GGGP", "SXX0", "tk1", "PHI", "BETAPHI",
     "ETAP", "ETAPHI", "pr0", "pf0", "DMP", "DT", "tk2", "DSXX", "HA", "HS", "DHP",
+    # basic (Ny, Nx)
+    "YNY_inv_ETA",
+    # P
+    "APHI", "XWS", "XWSSUM", "pf", "DMPSUM", "DHPSUM",
 ]
 GRID_ID = {n: k for k, n in enumerate(GRID_FIELDS)}
-BASIC_GRIDS = set(GRID_FIELDS[:GRID_FIELDS.index("DSXY") + 1])
+BASIC_GRIDS = set(GRID_FIELDS[:GRID_FIELDS.index("DSXY") + 1]) | {"YNY_inv_ETA"}
 
 # SUM grids in the order of reset_interpolated_properties! (E:968-999)
 SUM_FIELDS = [
@@ -56,7 +62,7 @@ SUM_FIELDS = [
 ]
 
 KERNEL_CLASSES = ["sort", "scatter", "combine", "nodes", "gather_T", "gather_S",
-                  "rk4", "asm_sd", "asm_t"]
+                  "rk4", "asm_sd", "asm_t", "viscosity", "porosity", "melt", "reaction"]
 
 _D = ctypes.c_double
 _I64 = ctypes.c_int64
@@ -104,6 +110,14 @@ PROTOTYPES = {
     "ereb_assemble_hydromechanical_lse": [_P] + [_P] * 20 + [_D, _P, _P],
     "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
     "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
+    "ereb_step_marker_viscosity": [_P],
+    "ereb_step_marker_porosity": [_P, _P, _D],
+    "ereb_step_melt_composition": [_P],
+    "ereb_step_thermochemical_reaction": [_P, _P, _D, _I64, _I64],
+    "ereb_update_marker_viscosity": [_P] + [_P] * 8 + [_I64],
+    "ereb_update_marker_porosity": [_P, _P] + [_P] * 5 + [_D, _I64],
+    "ereb_update_p_nodes_melt_composition": [_P] + [_P] * 6 + [_I64],
+    "ereb_perform_thermochemical_reaction": [_P, _P] + [_P] * 15 + [_I64, _D, _I64, _I64],
     "ereb_profile_enable": [_P, _I],
     "ereb_profile_get": [_P, _I, _P, _P],
     "ereb_profile_reset": [_P],
@@ -290,6 +304,29 @@ class Context:
         self.check(self.lib.ereb_step_assemble_thermal(
             self.h, float(dt), ctypes.addressof(n) if want_nnz else None))
         return n.value if want_nnz else None
+
+    # ---- viscosity, compaction and (de)hydration ---------------------------
+    def _hyd(self, hyd):
+        hyd = Hydration() if hyd is None else hyd
+        c = hyd.to_c(self.mat)
+        self._hyd_keep = c
+        return ctypes.addressof(c)
+
+    def step_marker_viscosity(self):
+        self.check(self.lib.ereb_step_marker_viscosity(self.h), "step_marker_viscosity")
+
+    def step_marker_porosity(self, dt: float, hyd: Hydration | None = None):
+        self.check(self.lib.ereb_step_marker_porosity(self.h, self._hyd(hyd), float(dt)),
+                   "step_marker_porosity")
+
+    def step_melt_composition(self):
+        self.check(self.lib.ereb_step_melt_composition(self.h), "step_melt_composition")
+
+    def step_thermochemical_reaction(self, dt: float, timestep: int, titer: int,
+                                     hyd: Hydration | None = None):
+        self.check(self.lib.ereb_step_thermochemical_reaction(
+            self.h, self._hyd(hyd), float(dt), int(timestep), int(titer)),
+            "step_thermochemical_reaction")
 
     def csc(self, which: int):
         """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64

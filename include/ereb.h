@@ -119,6 +119,10 @@ enum ereb_marker_field {
     EREB_M_ALPHAFLUIDCUR, EREB_M_RHOTOTALM, EREB_M_RHOCPTOTALM,
     EREB_M_ETATOTALM, EREB_M_HRTOTALM, EREB_M_KTOTALM,
     EREB_M_TKM_RHOCPTOTALM, EREB_M_ETAFLUIDCUR_INV_KPHIM,
+    /* hydration state (E:2558-2694, E:7011-7014): current wet-solid molar
+     * fraction XWsolidm, next porosity generation phinewm and the previous
+     * iteration's marker fluid pressure pfm0 */
+    EREB_M_XWSOLIDM, EREB_M_PHINEWM, EREB_M_PFM0,
     EREB_M_NFIELDS
 };
 /* marknum becomes the context's active marker count on set */
@@ -152,6 +156,13 @@ enum ereb_grid_field {
     EREB_G_GGGP, EREB_G_SXX0, EREB_G_TK1, EREB_G_PHI, EREB_G_BETAPHI,
     EREB_G_ETAP, EREB_G_ETAPHI, EREB_G_PR0, EREB_G_PF0, EREB_G_DMP,
     EREB_G_DT, EREB_G_TK2, EREB_G_DSXX, EREB_G_HA, EREB_G_HS, EREB_G_DHP,
+    /* inverse viscoplastic viscosity at yielding basic nodes (Ny, Nx), read
+     * by update_marker_viscosity! (E:774) */
+    EREB_G_YNY_INV_ETA,
+    /* P nodes: Dln[(1-phi)/phi]/Dt (E:4822), melt composition (E:1929),
+     * fluid pressure and the thermochemical transfer sums (E:2558) */
+    EREB_G_APHI, EREB_G_XWS, EREB_G_XWSSUM, EREB_G_PF, EREB_G_DMPSUM,
+    EREB_G_DHPSUM,
     EREB_G_NFIELDS
 };
 int ereb_grid_set(ereb_ctx* ctx, int field, const double* host);
@@ -305,11 +316,73 @@ int ereb_adjoint_tk1_loss(ereb_ctx* ctx, int64_t nsteps, int64_t checkpoint_ever
                           const double hrfluidm[3], double* loss, double* tk1_out,
                           double* grad_tkm, double* grad_phim);
 
+/* ---- hydration, compaction and viscoplastic viscosity ------------------- */
+/* Constants of the compaction and silicate (de)hydration model, C:227-279
+ * (plain data so that test and production constant sets can be passed). */
+typedef struct {
+    double phimin, phimax;        /* C:227, C:229 */
+    double RG;                    /* C:232 molar gas constant */
+    double VH2Of, VH2Ofi;         /* C:250, C:252: MH2O/rhoH2Of, MH2O/rhoH2Ofi */
+    double dHWD, dSWD, dVWD;      /* C:254-258 (dVWD = VDs + VH2Of - VWs) */
+    double pfcoeff;               /* C:260 */
+    double dtreaction;            /* C:266 */
+    double log_completion_rate;   /* C:268, log(0.01) */
+    double A_I, b_I, c_I;         /* C:273 */
+    double Sxo_B, Tscl_B, To_B;   /* C:276 */
+    double Sxo_T, To_T, Ea_T;     /* C:279 */
+    int64_t rate_mode;            /* reaction_rate_coeff_mode C:271: 1, 2, 3, 9 */
+} ereb_hydration;
+
+/* Resident forms (marker and grid state in HBM; see the drop-in forms
+ * below for the argument meaning). */
+/* update_marker_viscosity! (E:774-804) for every marker: the @threads loop
+ * E:6950-6953; reads the resident YNY and the YNY_INV_ETA grid */
+int ereb_step_marker_viscosity(ereb_ctx* ctx);
+/* update_marker_porosity! (E:4822-4856) with the APHI grid */
+int ereb_step_marker_porosity(ereb_ctx* ctx, const ereb_hydration* hyd, double dt);
+/* update_p_nodes_melt_composition! (E:1929-1939): XWSSUM, WTPSUM, XWS */
+int ereb_step_melt_composition(ereb_ctx* ctx);
+/* perform_thermochemical_reaction! (E:2558-2694) with the pf and tk2 grids:
+ * DMPSUM, DHPSUM, WTPSUM, DMP, DHP and the marker hydration state.  An
+ * unknown hyd->rate_mode is rejected before any marker is touched (the
+ * reference throws from compute_Δtreaction, E:2382-2397, at the first rock
+ * marker). */
+int ereb_step_thermochemical_reaction(ereb_ctx* ctx, const ereb_hydration* hyd,
+                                      double dt, int64_t timestep, int64_t titer);
+
+/* Drop-in forms with the reference's argument lists. */
+/* Batched update_marker_viscosity!(m, xm, ym, tm, tkm, etatotalm, etavpm,
+ * YNY, YNY_inv_ETA) (E:774) over m = 1..marknum (E:6950-6953).  YNY is
+ * Matrix{Bool} (Ny, Nx) as bytes. */
+int ereb_update_marker_viscosity(ereb_ctx* ctx, const double* xm,
+    const double* ym, const int64_t* tm, const double* tkm, double* etatotalm,
+    double* etavpm, const uint8_t* YNY, const double* YNY_inv_ETA,
+    int64_t marknum);
+/* update_marker_porosity!(xm, ym, tm, phim, APHI, dt, marknum) E:4822 */
+int ereb_update_marker_porosity(ereb_ctx* ctx, const ereb_hydration* hyd,
+    const double* xm, const double* ym, const int64_t* tm, double* phim,
+    const double* APHI, double dt, int64_t marknum);
+/* update_p_nodes_melt_composition!(xm, ym, XWsolidm0, XWS, XWSSUM, WTPSUM,
+ * marknum) E:1929 */
+int ereb_update_p_nodes_melt_composition(ereb_ctx* ctx, const double* xm,
+    const double* ym, const double* XWsolidm0, double* XWS, double* XWSSUM,
+    double* WTPSUM, int64_t marknum);
+/* perform_thermochemical_reaction!(DMP, DHP, DMPSUM, DHPSUM, WTPSUM, pf, tk2,
+ * tm, xm, ym, XWsm0, XWsm, phim, phinewm, pfm0, marknum, dt, timestep,
+ * titer) E:2558 (the reference's diagnostic @info lines are not emitted) */
+int ereb_perform_thermochemical_reaction(ereb_ctx* ctx,
+    const ereb_hydration* hyd, double* DMP, double* DHP, double* DMPSUM,
+    double* DHPSUM, double* WTPSUM, const double* pf, const double* tk2,
+    const int64_t* tm, const double* xm, const double* ym, double* XWsm0,
+    double* XWsm, double* phim, double* phinewm, double* pfm0,
+    int64_t marknum, double dt, int64_t timestep, int64_t titer);
+
 /* ---- profiling: HIP events around the launches of one kernel class ------ */
 enum ereb_kernel_class {
     EREB_K_SORT = 0, EREB_K_SCATTER, EREB_K_COMBINE, EREB_K_NODES,
     EREB_K_GATHER_T, EREB_K_GATHER_S, EREB_K_RK4, EREB_K_ASM_SD,
-    EREB_K_ASM_T, EREB_K_NCLASSES
+    EREB_K_ASM_T, EREB_K_VISCOSITY, EREB_K_POROSITY, EREB_K_MELT,
+    EREB_K_REACTION, EREB_K_NCLASSES
 };
 int ereb_profile_enable(ereb_ctx* ctx, int on);
 /* total milliseconds and launch count accumulated since the last reset */

@@ -842,6 +842,192 @@ int64_t orc_assemble_thermal(const ereb_geometry* g, const double* tk1,
 
 /* ------------------------------------------------- sort specification */
 
+/* ------------------------ viscosity, compaction and (de)hydration (next) */
+
+/* Julia min(a, b) for Float64: NaN propagates */
+static inline double jl_min(double a, double b) {
+    if (a != a || b != b) return a + b;
+    return a < b ? a : (b < a ? b : (signbit(a) ? a : b));
+}
+
+/* update_marker_viscosity! E:774-804 for m = 1..marknum (loop E:6950-6953) */
+void orc_update_marker_viscosity(const ereb_geometry* g, const ereb_materials* M,
+    const double* xm, const double* ym, const int64_t* tm, const double* tkm,
+    double* etatotalm, double* etavpm, const uint8_t* YNY,
+    const double* YNY_inv_ETA, int64_t marknum) {
+    const int64_t Nx = g->Nx, Ny = g->Ny;
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1,
+                        Ny - 1, &i, &j, w);
+        if (tm[m] < 3) etatotalm[m] = orc_etatotal_rocks(M, tkm[m], tm[m]);
+        const int any = G(YNY, i, j, Ny) || G(YNY, i + 1, j, Ny) ||
+                        G(YNY, i, j + 1, Ny) || G(YNY, i + 1, j + 1, Ny);
+        if (any) {
+            double v = gdot4(YNY_inv_ETA, Ny, i, j, w);
+            v = 1.0 / v;
+            etavpm[m] = v > etatotalm[m] ? etatotalm[m] : v;
+        } else {
+            etavpm[m] = etatotalm[m];
+        }
+    }
+}
+
+/* update_marker_porosity! E:4822-4856 */
+void orc_update_marker_porosity(const ereb_geometry* g, const ereb_hydration* H,
+    const double* xm, const double* ym, const int64_t* tm, double* phim,
+    const double* APHI, double dt, int64_t marknum) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    for (int64_t m = 0; m < marknum; ++m) {
+        if (!(tm[m] < 3)) continue;
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
+                        &i, &j, w);
+        const double aphim = gdot4(APHI, Ny1, i, j, w);
+        const double v = phim[m] / ((1.0 - phim[m]) * exp(aphim * dt) + phim[m]);
+        phim[m] = jl_max(H->phimin, jl_min(H->phimax, v));
+    }
+}
+
+/* update_p_nodes_melt_composition! E:1929-1939: molarfraction_marker_to_p_nodes!
+ * (E:1883-1908) for every marker, then compute_molarfraction! E:2239-2252 */
+void orc_update_p_nodes_melt_composition(const ereb_geometry* g, const double* xm,
+    const double* ym, const double* XWsolidm0, double* XWS, double* XWSSUM,
+    double* WTPSUM, int64_t marknum) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    memset(XWSSUM, 0, sizeof(double) * Nx1 * Ny1);
+    memset(WTPSUM, 0, sizeof(double) * Nx1 * Ny1);
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
+                        &i, &j, w);
+        add4(XWSSUM, Ny1, i, j, w, XWsolidm0[m]);
+        add4(WTPSUM, Ny1, i, j, w, 1.0);
+    }
+    for (int64_t k = 0; k < Nx1 * Ny1; ++k)
+        XWS[k] = WTPSUM[k] > 0.0 ? XWSSUM[k] * (1.0 / WTPSUM[k]) : 0.0;
+}
+
+/* compute_Δtreaction E:2382-2397; *err = EREB_ERR_MODE on unknown mode.
+ * Mode 3 divides by Sxo_B, as the reference does. */
+double orc_dtreaction(const ereb_hydration* H, double T, double phi, int64_t mode,
+                      int* err) {
+    if (mode == 1) {
+        const double d = T - H->c_I;
+        return -H->log_completion_rate / (H->A_I * phi) * exp(H->b_I * (d * d));
+    } else if (mode == 2) {
+        return -H->log_completion_rate / (H->Sxo_B * phi) *
+               pow(2.0, (H->To_B - T) / H->Tscl_B);
+    } else if (mode == 3) {
+        return -H->log_completion_rate / (H->Sxo_B * phi) *
+               exp(H->Ea_T / H->RG * (1.0 / T - 1.0 / H->To_T));
+    } else if (mode == 9) {
+        return H->dtreaction;
+    }
+    if (err) *err = EREB_ERR_MODE;
+    return NAN;
+}
+
+/* compute_gibbs_free_energy E:2470-2482 */
+double orc_gibbs_free_energy(const ereb_hydration* H, double T, double pf, double XDs,
+                             double XWs, double dt, double dtr) {
+    if (dt < dtr)
+        return (H->dHWD - T * H->dSWD + pf * H->dVWD + H->RG * T * log(XDs / XWs)) *
+               (1.0 - dt / dtr);
+    return 0.0;
+}
+
+/* compute_reaction_constant E:2521-2524 */
+double orc_reaction_constant(const ereb_hydration* H, double T, double pf, double dG) {
+    return exp(-(H->dHWD - T * H->dSWD + H->dVWD * pf - dG) / (H->RG * T));
+}
+
+/* compute_relative_enthalpy E:2502-2504 */
+double orc_relative_enthalpy(const ereb_materials* M, const ereb_hydration* H,
+                             double Xs, double XWs) {
+    return -Xs * XWs * H->dHWD / (M->MD + M->MH2O);
+}
+
+/* perform_thermochemical_reaction! E:2558-2694 (without the @info lines):
+ * the reaction of every rock marker, the ΔM, ΔH and weight sums onto P
+ * nodes for the fluid-bearing ones, then compute_thermodynamic_xfer!
+ * E:2208-2222.  Returns EREB_ERR_MODE for an unknown rate mode. */
+int orc_perform_thermochemical_reaction(const ereb_geometry* g,
+    const ereb_materials* M, const ereb_hydration* H, double* DMP, double* DHP,
+    double* DMPSUM, double* DHPSUM, double* WTPSUM, const double* pf,
+    const double* tk2, const int64_t* tm, const double* xm, const double* ym,
+    double* XWsm0, double* XWsm, double* phim, double* phinewm, double* pfm0,
+    int64_t marknum, double dt, int64_t timestep, int64_t titer) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    int err = 0;
+    memset(DMPSUM, 0, sizeof(double) * Nx1 * Ny1);
+    memset(DHPSUM, 0, sizeof(double) * Nx1 * Ny1);
+    memset(WTPSUM, 0, sizeof(double) * Nx1 * Ny1);
+    for (int64_t m = 0; m < marknum; ++m) {
+        if (!(tm[m] < 3)) continue;
+        int64_t i, j;
+        double w[4];
+        orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
+                        &i, &j, w);
+        const double tknm = gdot4(tk2, Ny1, i, j, w);
+        double pfnm = jl_max(0.0, gdot4(pf, Ny1, i, j, w));
+        if (titer > 2) pfnm = pfnm * (1.0 - H->pfcoeff) + pfm0[m] * H->pfcoeff;
+        pfm0[m] = pfnm;
+        const double XDs0 = 1.0 - XWsm0[m];
+        const double VH2O = tknm > M->tmfluidphase ? H->VH2Of : H->VH2Ofi;
+        const double Xf0 = phim[m] * (XWsm0[m] * M->VWs + XDs0 * M->VDs) /
+                           ((1.0 - phim[m]) * VH2O +
+                            phim[m] * (XWsm0[m] * M->VWs + XDs0 * M->VDs));
+        const double Xs0 = 1.0 - Xf0;
+        const double XH2Ot = (XWsm0[m] * Xs0 + Xf0) / (1.0 + XWsm0[m] * Xs0);
+        const double XDt = 1.0 - XH2Ot;
+        const double rhos0 = (M->MD + M->MH2O * XWsm0[m]) / (M->VDs * XDs0 + M->VWs * XWsm0[m]);
+        const double rhof0 = tknm > M->tmfluidphase ? M->rhoH2Of : M->rhoH2Ofi;
+        const double dtr = orc_dtreaction(H, tknm, phim[m], H->rate_mode, &err);
+        if (err) return EREB_ERR_MODE;
+        const double Ht0 = orc_relative_enthalpy(M, H, Xs0, XWsm0[m]);
+        const double dG0 = orc_gibbs_free_energy(H, tknm, pfnm, XDs0, XWsm0[m], dt, dtr);
+        const double KWD = orc_reaction_constant(H, tknm, pfnm, dG0);
+        const double XWs1 = 1.0 / (KWD + 1.0);
+        const double XDs1 = 1.0 - XWs1;
+        const double Xs1 = XDt / (1.0 - XDt * XWs1);
+        const double Xf1 = 1.0 - Xs1;
+        if (0.0 < Xf1 && Xf1 < 1.0) {
+            const double phi1 = Xf1 * VH2O / (Xf1 * VH2O + Xs1 * (XWs1 * M->VWs + XDs1 * M->VDs));
+            const double rhos1 = (M->MD + M->MH2O * XWs1) / (M->VDs * XDs1 + M->VWs * XWs1);
+            const double rhof1 = tknm > M->tmfluidphase ? M->rhoH2Of : M->rhoH2Ofi;
+            const double Ht1 = orc_relative_enthalpy(M, H, Xs1, XWs1);
+            const double dHt = Ht1 - Ht0;
+            const double RV = (rhos1 * (1.0 - phi1) + rhof1 * phi1) /
+                              (rhos0 * (1.0 - phim[m]) + rhof0 * phim[m]);
+            const double Gmass = (rhos0 * RV * (1.0 - phim[m]) - rhos1 * (1.0 - phi1)) / dt;
+            const double dMm = (1.0 - RV) / dt;
+            const double dHm = Gmass * dHt;
+            XWsm[m] = XWs1;
+            phinewm[m] = phi1;
+            if (timestep == 1) {
+                XWsm0[m] = XWsm[m];
+                phim[m] = phinewm[m];
+            }
+            add4(DMPSUM, Ny1, i, j, w, dMm);
+            add4(DHPSUM, Ny1, i, j, w, dHm);
+            add4(WTPSUM, Ny1, i, j, w, 1.0);
+        }
+    }
+    for (int64_t k = 0; k < Nx1 * Ny1; ++k) {
+        if (WTPSUM[k] > 0.0) {
+            DMP[k] = DMPSUM[k] * (1.0 / WTPSUM[k]);
+            DHP[k] = DHPSUM[k] * (1.0 / WTPSUM[k]);
+        } else {
+            DMP[k] = DHP[k] = 0.0;
+        }
+    }
+    return EREB_OK;
+}
+
 /* Fine key of the library's sort: the basic cell (ib, jb) from fix with the
  * basic clamps (E:1456, C:68-74) refined by the P-cell offsets
  * (ip-ib, jp-jb) in {0,1}^2 (P clamps, C:94-101), numbered tile-major:

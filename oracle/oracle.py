@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libereb_oracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "erebus.jl_amd"))
 
-from ereb.constants import CGeometry, CMaterials, Geometry, Materials  # noqa: E402
+from ereb.constants import CGeometry, CMaterials, Geometry, Hydration, Materials  # noqa: E402
 
 _P = ctypes.c_void_p
 _D = ctypes.c_double
@@ -68,6 +68,15 @@ def lib():
             "orc_fine_keys": (None, [_P, _I64, _P, _P, _I64, _P]),
             "orc_key_sort": (None, [_P, _I64, _I64, _P, _P]),
             "orc_compute_rotation_rate": (None, [_P, _P, _P, _P]),
+            "orc_update_marker_viscosity": (None, [_P, _P] + [_P] * 8 + [_I64]),
+            "orc_update_marker_porosity": (None, [_P, _P] + [_P] * 5 + [_D, _I64]),
+            "orc_update_p_nodes_melt_composition": (None, [_P] + [_P] * 6 + [_I64]),
+            "orc_dtreaction": (_D, [_P, _D, _D, _I64, _P]),
+            "orc_gibbs_free_energy": (_D, [_P, _D, _D, _D, _D, _D, _D]),
+            "orc_reaction_constant": (_D, [_P, _D, _D, _D]),
+            "orc_relative_enthalpy": (_D, [_P, _P, _D, _D]),
+            "orc_perform_thermochemical_reaction": (ctypes.c_int, [_P, _P, _P] + [_P] * 15 +
+                                                    [_I64, _D, _I64, _I64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -271,6 +280,76 @@ class Oracle:
         start = np.zeros(nkeys + 1, dtype=np.int64)
         self.L.orc_key_sort(_p(key), xm.size, nkeys, _p(perm), _p(start))
         return perm, key, start
+
+    # ---- viscosity, compaction and (de)hydration -----------------------------------
+    def _h(self, hyd):
+        self._hc = (Hydration() if hyd is None else hyd).to_c(self.mat)
+        return ctypes.addressof(self._hc)
+
+    def update_marker_viscosity(self, xm, ym, tm, tkm, etatotalm, YNY, YNY_inv_ETA):
+        """update_marker_viscosity! (E:774) for every marker: (etatotalm, etavpm)"""
+        et = np.array(etatotalm, dtype=np.float64, copy=True)
+        ev = np.zeros(et.size)
+        xm, ym, tkm = _c(xm), _c(ym), _c(tkm)
+        tm = _c(tm, np.int64)
+        Y = np.asfortranarray(np.asarray(YNY, dtype=np.uint8))
+        YI = _F(YNY_inv_ETA)
+        self.L.orc_update_marker_viscosity(self.pg, self.pm, _p(xm), _p(ym), _p(tm), _p(tkm),
+                                           _p(et), _p(ev), _p(Y), _p(YI), et.size)
+        return et, ev
+
+    def update_marker_porosity(self, xm, ym, tm, phim, APHI, dt, hyd=None):
+        """update_marker_porosity! (E:4822): new phim"""
+        ph = np.array(phim, dtype=np.float64, copy=True)
+        xm, ym, A = _c(xm), _c(ym), _F(APHI)
+        tm = _c(tm, np.int64)
+        self.L.orc_update_marker_porosity(self.pg, self._h(hyd), _p(xm), _p(ym), _p(tm),
+                                          _p(ph), _p(A), float(dt), ph.size)
+        return ph
+
+    def melt_composition(self, xm, ym, XWsolidm0):
+        """update_p_nodes_melt_composition! (E:1929): (XWS, XWSSUM, WTPSUM)"""
+        g = self.geom
+        xm, ym, xw = _c(xm), _c(ym), _c(XWsolidm0)
+        out = [np.zeros((g.Ny1, g.Nx1), order="F") for _ in range(3)]
+        self.L.orc_update_p_nodes_melt_composition(self.pg, _p(xm), _p(ym), _p(xw),
+                                                   *[_p(a) for a in out], xm.size)
+        return tuple(out)
+
+    def dtreaction(self, T, phi, mode, hyd=None):
+        err = ctypes.c_int(0)
+        v = self.L.orc_dtreaction(self._h(hyd), T, phi, int(mode), ctypes.addressof(err))
+        if err.value:
+            raise ValueError(f"unknown mode {mode}")
+        return v
+
+    def gibbs_free_energy(self, T, pf, XDs, XWs, dt, dtr, hyd=None):
+        return self.L.orc_gibbs_free_energy(self._h(hyd), T, pf, XDs, XWs, dt, dtr)
+
+    def reaction_constant(self, T, pf, dG, hyd=None):
+        return self.L.orc_reaction_constant(self._h(hyd), T, pf, dG)
+
+    def relative_enthalpy(self, Xs, XWs, hyd=None):
+        return self.L.orc_relative_enthalpy(self.pm, self._h(hyd), Xs, XWs)
+
+    def thermochemical_reaction(self, mk: dict, pf, tk2, dt, timestep, titer, hyd=None):
+        """perform_thermochemical_reaction! (E:2558).  mk holds tm, xm, ym,
+        XWsolidm0, XWsolidm, phim, phinewm, pfm0; returns (dict of the five
+        updated marker vectors, DMP, DHP, DMPSUM, DHPSUM, WTPSUM)."""
+        g = self.geom
+        names = ["XWsolidm0", "XWsolidm", "phim", "phinewm", "pfm0"]
+        io = {n: np.array(mk[n], dtype=np.float64, copy=True) for n in names}
+        xm, ym = _c(mk["xm"]), _c(mk["ym"])
+        tm = _c(mk["tm"], np.int64)
+        grids = [np.zeros((g.Ny1, g.Nx1), order="F") for _ in range(5)]
+        pf, tk2 = _F(pf), _F(tk2)
+        rc = self.L.orc_perform_thermochemical_reaction(
+            self.pg, self.pm, self._h(hyd), *[_p(a) for a in grids], _p(pf), _p(tk2), _p(tm),
+            _p(xm), _p(ym), *[_p(io[n]) for n in names], xm.size, float(dt), int(timestep),
+            int(titer))
+        if rc != 0:
+            raise ValueError("unknown mode")
+        return (io, *grids)
 
     def rotation_rate(self, vx, vy):
         g = self.geom
