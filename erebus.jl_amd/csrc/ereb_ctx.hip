@@ -985,6 +985,27 @@ int ereb_perform_thermochemical_reaction(ereb_ctx* c, const ereb_hydration* hyd,
     return EREB_OK;
 }
 
+int ereb_step_backtrace_pressures(ereb_ctx* c, double dt) {
+    if (!c) return EREB_ERR_ARG;
+    return launch_backtrace_pressures(c, dt);
+}
+
+int ereb_backtrace_pressures_rk4(ereb_ctx* c, const double* pr, double* pr0, const double* ps,
+    double* ps0, const double* pf, double* pf0, const double* vx, const double* vy,
+    const double* vxf, const double* vyf, double dt) {
+    if (!c) return EREB_ERR_ARG;
+    if (!pr0 || !ps0 || !pf0)
+        return fail(c, EREB_ERR_ARG, "ereb_backtrace_pressures_rk4: null array");
+    SETG(EREB_G_PR, pr); SETG(EREB_G_PS, ps); SETG(EREB_G_PF, pf);
+    SETG(EREB_G_VX, vx); SETG(EREB_G_VY, vy); SETG(EREB_G_VXF, vxf); SETG(EREB_G_VYF, vyf);
+    int rc = launch_backtrace_pressures(c, dt);
+    if (rc) return rc;
+    GETG(EREB_G_PR0, pr0);
+    GETG(EREB_G_PS0, ps0);
+    GETG(EREB_G_PF0, pf0);
+    return EREB_OK;
+}
+
 int ereb_assemble_thermal_lse(ereb_ctx* c, const double* tk1, const double* RHOCP,
     const double* KX, const double* KY, const double* HR, const double* HA, const double* HS,
     const double* DHP, double* RT, double dt, int64_t* nnz) {

@@ -161,6 +161,9 @@ int launch_melt_composition(ereb_ctx* c);
 int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64_t timestep,
                           int64_t titer);
 
+// ereb_backtrace.hip
+int launch_backtrace_pressures(ereb_ctx* c, double dt);
+
 // adjoints (ereb_adjoint.hip, ereb_markers.hip); cotangent grids live in a
 // buffer with the layout of the grid fields (offsets goff)
 int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mode,

@@ -24,6 +24,7 @@ __all__ = [
     "assemble_hydromechanical_lse_", "assemble_thermal_lse_", "CSC",
     "update_marker_viscosity_", "update_marker_porosity_",
     "update_p_nodes_melt_composition_", "perform_thermochemical_reaction_",
+    "backtrace_pressures_rk4_",
 ]
 
 
@@ -241,3 +242,13 @@ def perform_thermochemical_reaction_(ctx: Context, DMP, DHP, DMPSUM, DHPSUM, WTP
         _in(tm, np.int64, keep=keep), _in(xm, keep=keep), _in(ym, keep=keep), _io(XWsm0),
         _io(XWsm), _io(phim), _io(phinewm), _io(pfm0), int(marknum), float(dt),
         int(timestep), int(titer)), "perform_thermochemical_reaction!")
+
+
+def backtrace_pressures_rk4_(ctx: Context, pr, pr0, ps, ps0, pf, pf0, vx, vy, vxf, vyf, dt):
+    """backtrace_pressures_rk4! (E:5284): fills pr0, ps0, pf0"""
+    keep = []
+    ctx.check(ctx.lib.ereb_backtrace_pressures_rk4(
+        ctx.h, _in(pr, grid=True, keep=keep), _io(pr0, grid=True),
+        _in(ps, grid=True, keep=keep), _io(ps0, grid=True), _in(pf, grid=True, keep=keep),
+        _io(pf0, grid=True), *[_in(a, grid=True, keep=keep) for a in (vx, vy, vxf, vyf)],
+        float(dt)), "backtrace_pressures_rk4!")

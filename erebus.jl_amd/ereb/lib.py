@@ -48,6 +48,7 @@ GRID_FIELDS = [
     "YNY_inv_ETA",
     # P
     "APHI", "XWS", "XWSSUM", "pf", "DMPSUM", "DHPSUM",
+    "pr", "ps", "ps0",
 ]
 GRID_ID = {n: k for k, n in enumerate(GRID_FIELDS)}
 BASIC_GRIDS = set(GRID_FIELDS[:GRID_FIELDS.index("DSXY") + 1]) | {"YNY_inv_ETA"}
@@ -62,7 +63,8 @@ SUM_FIELDS = [
 ]
 
 KERNEL_CLASSES = ["sort", "scatter", "combine", "nodes", "gather_T", "gather_S",
-                  "rk4", "asm_sd", "asm_t", "viscosity", "porosity", "melt", "reaction"]
+                  "rk4", "asm_sd", "asm_t", "viscosity", "porosity", "melt", "reaction",
+                  "backtrace"]
 
 _D = ctypes.c_double
 _I64 = ctypes.c_int64
@@ -118,6 +120,8 @@ PROTOTYPES = {
     "ereb_update_marker_porosity": [_P, _P] + [_P] * 5 + [_D, _I64],
     "ereb_update_p_nodes_melt_composition": [_P] + [_P] * 6 + [_I64],
     "ereb_perform_thermochemical_reaction": [_P, _P] + [_P] * 15 + [_I64, _D, _I64, _I64],
+    "ereb_step_backtrace_pressures": [_P, _D],
+    "ereb_backtrace_pressures_rk4": [_P] + [_P] * 10 + [_D],
     "ereb_profile_enable": [_P, _I],
     "ereb_profile_get": [_P, _I, _P, _P],
     "ereb_profile_reset": [_P],
@@ -327,6 +331,10 @@ class Context:
         self.check(self.lib.ereb_step_thermochemical_reaction(
             self.h, self._hyd(hyd), float(dt), int(timestep), int(titer)),
             "step_thermochemical_reaction")
+
+    def step_backtrace_pressures(self, dt: float):
+        self.check(self.lib.ereb_step_backtrace_pressures(self.h, float(dt)),
+                   "step_backtrace_pressures")
 
     def csc(self, which: int):
         """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64

@@ -163,6 +163,9 @@ enum ereb_grid_field {
      * fluid pressure and the thermochemical transfer sums (E:2558) */
     EREB_G_APHI, EREB_G_XWS, EREB_G_XWSSUM, EREB_G_PF, EREB_G_DMPSUM,
     EREB_G_DHPSUM,
+    /* P nodes: total and solid pressure and the solid pressure of the
+     * previous generation (E:5284) */
+    EREB_G_PR, EREB_G_PS, EREB_G_PS0,
     EREB_G_NFIELDS
 };
 int ereb_grid_set(ereb_ctx* ctx, int field, const double* host);
@@ -377,12 +380,24 @@ int ereb_perform_thermochemical_reaction(ereb_ctx* ctx,
     double* XWsm, double* phim, double* phinewm, double* pfm0,
     int64_t marknum, double dt, int64_t timestep, int64_t titer);
 
+/* ---- pressure back-trace ------------------------------------------------ */
+/* backtrace_pressures_rk4! (E:5284-5496) on the resident grids: PR, PS, PF,
+ * vx, vy, vxf, vyf in; PR0, PS0, PF0 out (pr0 .= pr etc., then every
+ * interior P node takes the pressures at its RK4 departure point) */
+int ereb_step_backtrace_pressures(ereb_ctx* ctx, double dt);
+/* drop-in form: backtrace_pressures_rk4!(pr, pr0, ps, ps0, pf, pf0, vx, vy,
+ * vxf, vyf, dt); all (Ny1, Nx1) */
+int ereb_backtrace_pressures_rk4(ereb_ctx* ctx, const double* pr, double* pr0,
+    const double* ps, double* ps0, const double* pf, double* pf0,
+    const double* vx, const double* vy, const double* vxf, const double* vyf,
+    double dt);
+
 /* ---- profiling: HIP events around the launches of one kernel class ------ */
 enum ereb_kernel_class {
     EREB_K_SORT = 0, EREB_K_SCATTER, EREB_K_COMBINE, EREB_K_NODES,
     EREB_K_GATHER_T, EREB_K_GATHER_S, EREB_K_RK4, EREB_K_ASM_SD,
     EREB_K_ASM_T, EREB_K_VISCOSITY, EREB_K_POROSITY, EREB_K_MELT,
-    EREB_K_REACTION, EREB_K_NCLASSES
+    EREB_K_REACTION, EREB_K_BACKTRACE, EREB_K_NCLASSES
 };
 int ereb_profile_enable(ereb_ctx* ctx, int on);
 /* total milliseconds and launch count accumulated since the last reset */

@@ -1028,6 +1028,112 @@ int orc_perform_thermochemical_reaction(const ereb_geometry* g,
     return EREB_OK;
 }
 
+/* ------------------------------------------------ pressure back-trace (next) */
+
+/* One stage velocity of backtrace_pressures_rk4! on the Vx grid (E:5299-5332):
+ * the stencil of E:5057-5091 without the E:5173 centre quirk, combined as
+ * (1-dymi/dy)*v13 + (dymi/dy)*v24 */
+static double bt_vx(const ereb_geometry* g, const double* v, double x, double y) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, R = Ny + 1;
+    int64_t i, j;
+    double dxmj, dymi;
+    orc_fix_distances(x, y, g->xvx, g->yvx, g->dx, g->dy, 1, Nx - 1, 1, Ny, &i,
+                      &j, &dxmj, &dymi);
+    const double dx = g->dx, dy = g->dy;
+    double v13 = G(v, i, j, R) * (1.0 - dxmj / dx) + G(v, i, j + 1, R) * dxmj / dx;
+    double v24 = G(v, i + 1, j, R) * (1.0 - dxmj / dx) + G(v, i + 1, j + 1, R) * dxmj / dx;
+    if (dxmj / dx >= 0.5) {
+        if (j < Nx - 1) {
+            const double t = dxmj / dx - 0.5;
+            v13 += 0.5 * (t * t) * (G(v, i, j, R) - 2.0 * G(v, i, j + 1, R) + G(v, i, j + 2, R));
+            v24 += 0.5 * (t * t) * (G(v, i + 1, j, R) - 2.0 * G(v, i + 1, j + 1, R) + G(v, i + 1, j + 2, R));
+        }
+    } else if (j > 1) {
+        const double t = dxmj / dx - 0.5;
+        v13 += 0.5 * (t * t) * (G(v, i, j - 1, R) - 2.0 * G(v, i, j, R) + G(v, i, j + 1, R));
+        v24 += 0.5 * (t * t) * (G(v, i + 1, j - 1, R) - 2.0 * G(v, i + 1, j, R) + G(v, i + 1, j + 1, R));
+    }
+    return (1.0 - dymi / dy) * v13 + (dymi / dy) * v24;
+}
+
+/* ... and on the Vy grid (E:5333-5364); `1/2*(...)` of the left branch
+ * (E:5359-5360) is 0.5*(...) */
+static double bt_vy(const ereb_geometry* g, const double* v, double x, double y) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, R = Ny + 1;
+    int64_t i, j;
+    double dxmj, dymi;
+    orc_fix_distances(x, y, g->xvy, g->yvy, g->dx, g->dy, 1, Nx, 1, Ny - 1, &i,
+                      &j, &dxmj, &dymi);
+    const double dx = g->dx, dy = g->dy;
+    double v12 = G(v, i, j, R) * (1.0 - dymi / dy) + G(v, i + 1, j, R) * dymi / dy;
+    double v34 = G(v, i, j + 1, R) * (1.0 - dymi / dy) + G(v, i + 1, j + 1, R) * dymi / dy;
+    if (dymi / dy >= 0.5) {
+        if (i < Ny - 1) {
+            const double t = dymi / dy - 0.5;
+            v12 += 0.5 * (t * t) * (G(v, i, j, R) - 2.0 * G(v, i + 1, j, R) + G(v, i + 2, j, R));
+            v34 += 0.5 * (t * t) * (G(v, i, j + 1, R) - 2.0 * G(v, i + 1, j + 1, R) + G(v, i + 2, j + 1, R));
+        }
+    } else if (i > 1) {
+        const double t = dymi / dy - 0.5;
+        v12 += 0.5 * (t * t) * (G(v, i - 1, j, R) - 2.0 * G(v, i, j, R) + G(v, i + 1, j, R));
+        v34 += 0.5 * (t * t) * (G(v, i - 1, j + 1, R) - 2.0 * G(v, i, j + 1, R) + G(v, i + 1, j + 1, R));
+    }
+    return (1.0 - dxmj / dx) * v12 + (dxmj / dx) * v34;
+}
+
+/* RK4 back-trace of P node (ii, jj) through (vx, vy) (E:5296-5376): the
+ * departure point (*xo, *yo).  The final step is xA - dt*1//6*(...), which
+ * Julia parses as xA - (dt*(1//6))*(...) (// binds tighter than *). */
+static void bt_node(const ereb_geometry* g, const double* vx, const double* vy,
+                    int64_t ii, int64_t jj, double dt, double* xo, double* yo) {
+    const double xA = g->xp[jj - 1], yA = g->yp[ii - 1];
+    double xc = xA, yc = yA, vxm[4], vym[4];
+    for (int rk = 0; rk < 4; ++rk) {
+        vxm[rk] = bt_vx(g, vx, xc, yc);
+        vym[rk] = bt_vy(g, vy, xc, yc);
+        if (rk == 0 || rk == 1) {
+            xc = xA - 0.5 * dt * vxm[rk];
+            yc = yA - 0.5 * dt * vym[rk];
+        } else if (rk == 2) {
+            xc = xA - dt * vxm[rk];
+            yc = yA - dt * vym[rk];
+        }
+    }
+    const double sixth = 1.0 / 6.0; /* Float64(1//6) */
+    *xo = xA - dt * sixth * (vxm[0] + 2.0 * vxm[1] + 2.0 * vxm[2] + vxm[3]);
+    *yo = yA - dt * sixth * (vym[0] + 2.0 * vym[1] + 2.0 * vym[2] + vym[3]);
+}
+
+/* backtrace_pressures_rk4! E:5284-5496: pr0 .= pr, ps0 .= ps, pf0 .= pf, then
+ * the interior P nodes (ii = 2..Ny, jj = 2..Nx) take the pressures at their
+ * departure points: solid velocities for pr and ps, fluid ones for pf */
+void orc_backtrace_pressures_rk4(const ereb_geometry* g, const double* pr,
+    double* pr0, const double* ps, double* ps0, const double* pf, double* pf0,
+    const double* vx, const double* vy, const double* vxf, const double* vyf,
+    double dt) {
+    const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
+    memcpy(pr0, pr, sizeof(double) * Nx1 * Ny1);
+    memcpy(ps0, ps, sizeof(double) * Nx1 * Ny1);
+    for (int64_t jj = 2; jj <= Nx; ++jj)
+        for (int64_t ii = 2; ii <= Ny; ++ii) {
+            double x, y, w[4];
+            int64_t i, j;
+            bt_node(g, vx, vy, ii, jj, dt, &x, &y);
+            orc_fix_weights(x, y, g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny, &i, &j, w);
+            G(pr0, ii, jj, Ny1) = gdot4(pr, Ny1, i, j, w);
+            G(ps0, ii, jj, Ny1) = gdot4(ps, Ny1, i, j, w);
+        }
+    memcpy(pf0, pf, sizeof(double) * Nx1 * Ny1);
+    for (int64_t jj = 2; jj <= Nx; ++jj)
+        for (int64_t ii = 2; ii <= Ny; ++ii) {
+            double x, y, w[4];
+            int64_t i, j;
+            bt_node(g, vxf, vyf, ii, jj, dt, &x, &y);
+            orc_fix_weights(x, y, g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny, &i, &j, w);
+            G(pf0, ii, jj, Ny1) = gdot4(pf, Ny1, i, j, w);
+        }
+}
+
 /* Fine key of the library's sort: the basic cell (ib, jb) from fix with the
  * basic clamps (E:1456, C:68-74) refined by the P-cell offsets
  * (ip-ib, jp-jb) in {0,1}^2 (P clamps, C:94-101), numbered tile-major:

@@ -77,6 +77,7 @@ def lib():
             "orc_relative_enthalpy": (_D, [_P, _P, _D, _D]),
             "orc_perform_thermochemical_reaction": (ctypes.c_int, [_P, _P, _P] + [_P] * 15 +
                                                     [_I64, _D, _I64, _I64]),
+            "orc_backtrace_pressures_rk4": (None, [_P] + [_P] * 10 + [_D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -350,6 +351,16 @@ class Oracle:
         if rc != 0:
             raise ValueError("unknown mode")
         return (io, *grids)
+
+    def backtrace_pressures(self, pr, ps, pf, vx, vy, vxf, vyf, dt):
+        """backtrace_pressures_rk4! (E:5284): (pr0, ps0, pf0)"""
+        g = self.geom
+        ins = [_F(a) for a in (pr, ps, pf, vx, vy, vxf, vyf)]
+        out = [np.zeros((g.Ny1, g.Nx1), order="F") for _ in range(3)]
+        self.L.orc_backtrace_pressures_rk4(self.pg, _p(ins[0]), _p(out[0]), _p(ins[1]),
+                                           _p(out[1]), _p(ins[2]), _p(out[2]),
+                                           *[_p(a) for a in ins[3:]], float(dt))
+        return tuple(out)
 
     def rotation_rate(self, vx, vy):
         g = self.geom
