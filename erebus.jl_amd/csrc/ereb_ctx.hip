@@ -207,6 +207,10 @@ static int reserve(ereb_ctx* c, int64_t n) {
     return EREB_OK;
 }
 
+namespace ereb {
+int reserve_markers(ereb_ctx* c, int64_t n) { return reserve(c, n); }
+}  // namespace ereb
+
 extern "C" {
 
 int ereb_version(void) { return 1; }
@@ -312,6 +316,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_axes);
     hipFree(c->d_mat);
     hipFree(c->adj_pool);
+    hipFree(c->rep_pool);
     hipFree(c->d_grids);
     hipFree(c->d_yny);
     hipFree(c->d_key);
@@ -1003,6 +1008,73 @@ int ereb_backtrace_pressures_rk4(ereb_ctx* c, const double* pr, double* pr0, con
     GETG(EREB_G_PR0, pr0);
     GETG(EREB_G_PS0, ps0);
     GETG(EREB_G_PF0, pf0);
+    return EREB_OK;
+}
+
+int ereb_step_replenish_markers(ereb_ctx* c, const ereb_marker_grid* mg, int randomized,
+                                uint64_t seed, double* mdis, int64_t* mnum,
+                                int64_t* marknum_out) {
+    if (!c) return EREB_ERR_ARG;
+    if (!mg) return fail(c, EREB_ERR_ARG, "ereb_marker_grid missing");
+    int64_t added = 0;
+    int rc = launch_replenish(c, *mg, randomized, seed, &added);
+    if (rc) return rc;
+    const int64_t A = c->rep_n;
+    if (mdis)
+        EREB_HIP(c, hipMemcpyAsync(mdis, c->rep_mdis, sizeof(double) * A,
+                                   hipMemcpyDeviceToHost, c->stream));
+    std::vector<int32_t> mn;
+    if (mnum) {
+        mn.resize((size_t)A);
+        EREB_HIP(c, hipMemcpyAsync(mn.data(), c->rep_mnum, sizeof(int32_t) * A,
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    for (int64_t a = 0; mnum && a < A; ++a) mnum[a] = mn[(size_t)a];
+    if (marknum_out) *marknum_out = c->marknum;
+    return EREB_OK;
+}
+
+int ereb_replenish_markers(ereb_ctx* c, double* xm, double* ym, int64_t* tm, double* tkm,
+    double* phim, double* sxxm, double* sxym, double* etavpm, double* phinewm, double* pfm0,
+    double* XWsolidm, double* XWsolidm0, double* rhototalm, double* rhocptotalm,
+    double* etatotalm, double* hrtotalm, double* ktotalm, double* inv_gggtotalm,
+    double* fricttotalm, double* cohestotalm, double* tenstotalm, double* rhofluidcur,
+    double* alphasolidcur, double* alphafluidcur, double* tkm_rhocptotalm,
+    double* etafluidcur_inv_kphim, double* mdis, int64_t* mnum, int64_t marknum,
+    int64_t capacity, const ereb_marker_grid* mg, int randomized, uint64_t seed,
+    int64_t* marknum_out) {
+    if (!c) return EREB_ERR_ARG;
+    const struct {
+        int f;
+        void* p;
+    } v[] = {{EREB_M_XM, xm}, {EREB_M_YM, ym}, {EREB_M_TM, tm}, {EREB_M_TKM, tkm},
+             {EREB_M_PHIM, phim}, {EREB_M_SXXM, sxxm}, {EREB_M_SXYM, sxym},
+             {EREB_M_ETAVPM, etavpm}, {EREB_M_PHINEWM, phinewm}, {EREB_M_PFM0, pfm0},
+             {EREB_M_XWSOLIDM, XWsolidm}, {EREB_M_XWSOLIDM0, XWsolidm0},
+             {EREB_M_RHOTOTALM, rhototalm}, {EREB_M_RHOCPTOTALM, rhocptotalm},
+             {EREB_M_ETATOTALM, etatotalm}, {EREB_M_HRTOTALM, hrtotalm},
+             {EREB_M_KTOTALM, ktotalm}, {EREB_M_INV_GGGTOTALM, inv_gggtotalm},
+             {EREB_M_FRICTTOTALM, fricttotalm}, {EREB_M_COHESTOTALM, cohestotalm},
+             {EREB_M_TENSTOTALM, tenstotalm}, {EREB_M_RHOFLUIDCUR, rhofluidcur},
+             {EREB_M_ALPHASOLIDCUR, alphasolidcur}, {EREB_M_ALPHAFLUIDCUR, alphafluidcur},
+             {EREB_M_TKM_RHOCPTOTALM, tkm_rhocptotalm},
+             {EREB_M_ETAFLUIDCUR_INV_KPHIM, etafluidcur_inv_kphim}};
+    static_assert(sizeof v / sizeof v[0] == EREB_M_NFIELDS, "every marker vector");
+    for (const auto& e : v)
+        if (!e.p) return fail(c, EREB_ERR_ARG, "ereb_replenish_markers: null marker vector");
+    if (marknum < 0 || capacity < marknum)
+        return fail(c, EREB_ERR_ARG, "ereb_replenish_markers: bad marknum or capacity");
+    for (const auto& e : v) SETM(e.f, e.p);
+    int64_t n = 0;
+    int rc = ereb_step_replenish_markers(c, mg, randomized, seed, mdis, mnum, &n);
+    if (rc) return rc;
+    if (marknum_out) *marknum_out = n;
+    if (n > capacity)
+        return fail(c, EREB_ERR_ARG, "ereb_replenish_markers: the new marker count exceeds "
+                                     "capacity; the result is resident (ereb_markers_get)");
+    for (const auto& e : v)
+        if ((rc = ereb_markers_get(c, e.f, e.p, n))) return rc;
     return EREB_OK;
 }
 

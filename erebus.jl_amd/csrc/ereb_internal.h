@@ -84,6 +84,14 @@ struct ereb_ctx {
     double* adj_pool = nullptr;  // grow-only device pool of the adjoint driver
     size_t adj_pool_n = 0;       // doubles
 
+    // marker replenishment: grow-only scratch, and the last call's lattice
+    // results (views into it)
+    void* rep_pool = nullptr;
+    size_t rep_bytes = 0;
+    double* rep_mdis = nullptr;
+    int32_t* rep_mnum = nullptr;
+    int64_t rep_n = 0;
+
     bool prof = false;
     std::vector<ereb::ProfPair> prof_pending;
     std::vector<hipEvent_t> ev_pool;
@@ -115,6 +123,8 @@ int32_t* tm_ptr(ereb_ctx* c);
 int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell);
 int ensure_scan(ereb_ctx* c, int64_t n);
 int check_flags(ereb_ctx* c);
+// grow the resident marker arrays to hold n markers (contents kept)
+int reserve_markers(ereb_ctx* c, int64_t n);
 void prof_begin(ereb_ctx* c, int kclass, hipEvent_t* a);
 void prof_end(ereb_ctx* c, int kclass, hipEvent_t a);
 int prof_collect(ereb_ctx* c);
@@ -163,6 +173,10 @@ int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64
 
 // ereb_backtrace.hip
 int launch_backtrace_pressures(ereb_ctx* c, double dt);
+
+// ereb_replenish.hip: *added = number of markers appended
+int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, uint64_t seed,
+                     int64_t* added);
 
 // adjoints (ereb_adjoint.hip, ereb_markers.hip); cotangent grids live in a
 // buffer with the layout of the grid fields (offsets goff)

@@ -1,0 +1,264 @@
+// ereb_replenish.hip -- replenish_markers! (E:5586-5698): add markers where
+// the marker launch lattice has gone empty.
+//
+// The reference runs three sequential loops; here each is a data-parallel
+// pass with the same result:
+//   1. nearest marker of every anchor among the markers whose marker cell
+//      touches it (update_marker_population_geometry!, E:5518-5544): a
+//      64-bit atomicMin of the distance bits (non-negative doubles order as
+//      their bit patterns), then an atomicMin of the marker index among the
+//      markers at that distance -- the lowest index, as the strict `<` of the
+//      sequential loop keeps the first;
+//   2. every anchor left without a marker scans its (dii, djj) window in the
+//      reference's order for the markers found in pass 1 (E:5641-5654); it
+//      reads only pass-1 results, as the reference does (it tests mnum > 0);
+//   3. an exclusive scan of the "new marker" flags in column-major anchor
+//      order gives each new marker its index, so markers are appended in
+//      the reference's order, copying every resident marker vector.
+#include <algorithm>
+#include <cstring>
+
+#include "ereb_internal.h"
+
+namespace ereb {
+
+struct MGridK {
+    int Nxm, Nym, dii, djj;
+    double dxm, dym, idxm, idym, mdis_init;
+    const double* xxm;
+    const double* yym;
+};
+
+// distance E:1097-1099
+__device__ __forceinline__ double mdist(double x1, double y1, double x2, double y2) {
+    const double a = x1 - x2, b = y1 - y2;
+    return sqrt(a * a + b * b);
+}
+
+// fix on the marker lattice (clamps 1..Nxm-1, 1..Nym-1), 1-based (i, j)
+__device__ __forceinline__ void mcell(const MGridK& G, double x, double y, int& i, int& j) {
+    j = jl_cell((x - G.xxm[0]) * G.idxm, 1, G.Nxm - 1);
+    i = jl_cell((y - G.yym[0]) * G.idym, 1, G.Nym - 1);
+}
+
+__global__ __launch_bounds__(256) void k_anchor_init(uint64_t* __restrict__ dbits,
+                                                     int32_t* __restrict__ pick, uint64_t init,
+                                                     int64_t n) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    dbits[a] = init;
+    pick[a] = INT32_MAX;
+}
+
+// pass 1a: smallest distance per anchor
+__global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __restrict__ xm,
+                                                    const double* __restrict__ ym, int64_t M,
+                                                    uint64_t* __restrict__ dbits) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const double x = xm[m], y = ym[m];
+    int i, j;
+    mcell(G, x, y, i, j);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ii = i + (q & 1), jj = j + (q >> 1);
+        const double d = mdist(x, y, G.xxm[jj - 1], G.yym[ii - 1]);
+        if (d < G.mdis_init)
+            atomicMin((unsigned long long*)&dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)],
+                      (unsigned long long)__double_as_longlong(d));
+    }
+}
+
+// pass 1b: lowest marker index at that distance
+__global__ __launch_bounds__(256) void k_anchor_pick(MGridK G, const double* __restrict__ xm,
+                                                     const double* __restrict__ ym, int64_t M,
+                                                     const uint64_t* __restrict__ dbits,
+                                                     int32_t* __restrict__ pick) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const double x = xm[m], y = ym[m];
+    int i, j;
+    mcell(G, x, y, i, j);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ii = i + (q & 1), jj = j + (q >> 1);
+        const double d = mdist(x, y, G.xxm[jj - 1], G.yym[ii - 1]);
+        const size_t a = (size_t)(jj - 1) * G.Nym + (ii - 1);
+        if (d < G.mdis_init && (uint64_t)__double_as_longlong(d) == dbits[a])
+            atomicMin(&pick[a], (int32_t)(m + 1));
+    }
+}
+
+// pass 2: final mdis / mnum of every anchor and the "new marker" flag
+__global__ __launch_bounds__(256) void k_anchor_fill(MGridK G, const double* __restrict__ xm,
+                                                     const double* __restrict__ ym,
+                                                     const uint64_t* __restrict__ dbits,
+                                                     const int32_t* __restrict__ pick,
+                                                     double* __restrict__ mdis,
+                                                     int32_t* __restrict__ mnum,
+                                                     int32_t* __restrict__ flag) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= (int64_t)G.Nxm * G.Nym) return;
+    const int p = pick[a];
+    if (p != INT32_MAX) {
+        mdis[a] = __longlong_as_double((long long)dbits[a]);
+        mnum[a] = p;
+        flag[a] = 0;
+        return;
+    }
+    const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
+    const double xa = G.xxm[j - 1], ya = G.yym[i - 1];
+    double best = G.mdis_init;
+    int num = 0;
+    const int jlo = max(j - G.djj, 1), jhi = min(j + G.djj, G.Nxm);
+    const int ilo = max(i - G.dii, 1), ihi = min(i + G.dii, G.Nym);
+    for (int jj = jlo; jj <= jhi; ++jj)
+        for (int ii = ilo; ii <= ihi; ++ii) {
+            const int q = pick[(size_t)(jj - 1) * G.Nym + (ii - 1)];
+            if (q == INT32_MAX) continue;  // mnum[ii, jj] > 0 only for pass-1 anchors
+            const double d = mdist(xm[q - 1], ym[q - 1], xa, ya);
+            if (d < best) {
+                best = d;
+                num = -q;
+            }
+        }
+    mdis[a] = best;
+    mnum[a] = num;
+    flag[a] = num < 0 ? 1 : 0;
+}
+
+struct MarkerFields {
+    void* f[EREB_M_NFIELDS];
+    int es[EREB_M_NFIELDS];  // element size in bytes: 4 or 8
+};
+
+// splitmix64 finaliser: a counter-based uniform in [0, 1)
+__device__ __forceinline__ double u01(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1p-53;
+}
+
+// pass 3: append the new markers (positions from the scan of the flags)
+__global__ __launch_bounds__(256) void k_anchor_append(MGridK G, MarkerFields F,
+                                                       const int32_t* __restrict__ mnum,
+                                                       const int32_t* __restrict__ flag,
+                                                       const int32_t* __restrict__ pos,
+                                                       int64_t M, int randomized, uint64_t seed) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= (int64_t)G.Nxm * G.Nym || !flag[a]) return;
+    const int64_t k = M + pos[a];
+    const int64_t src = (int64_t)(-mnum[a]) - 1;
+#pragma unroll
+    for (int f = 0; f < EREB_M_NFIELDS; ++f) {
+        if (F.es[f] == 8)
+            ((double*)F.f[f])[k] = ((const double*)F.f[f])[src];
+        else
+            ((int32_t*)F.f[f])[k] = ((const int32_t*)F.f[f])[src];
+    }
+    const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
+    double x = G.xxm[j - 1], y = G.yym[i - 1];
+    if (randomized) {  // E:5662-5663 with a counter-based generator
+        x = x + (u01(seed ^ (2 * (uint64_t)a)) - 0.5) * G.dxm;
+        y = y + (u01(seed ^ (2 * (uint64_t)a + 1)) - 0.5) * G.dym;
+    }
+    ((double*)F.f[EREB_M_XM])[k] = x;
+    ((double*)F.f[EREB_M_YM])[k] = y;
+}
+
+int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, uint64_t seed,
+                     int64_t* added) {
+    const int64_t M = c->marknum;
+    const int64_t A = mg.Nxm * mg.Nym;
+    if (mg.Nxm < 2 || mg.Nym < 2 || !mg.xxm || !mg.yym || A > INT32_MAX)
+        return fail(c, EREB_ERR_ARG, "marker lattice needs Nxm, Nym >= 2 and both axes");
+    int rc;
+    // scratch: distance bits, picks, mdis, mnum, flags, positions, axes
+    const size_t need = A * (sizeof(uint64_t) + sizeof(double) + 4 * sizeof(int32_t)) +
+                        sizeof(int32_t) + sizeof(double) * (mg.Nxm + mg.Nym) + 256 * 8;
+    if (need > c->rep_bytes) {
+        hipFree(c->rep_pool);
+        c->rep_pool = nullptr;
+        c->rep_bytes = 0;
+        EREB_HIP(c, hipMalloc(&c->rep_pool, need));
+        c->rep_bytes = need;
+    }
+    char* p = (char*)c->rep_pool;
+    auto carve = [&](size_t bytes) {
+        char* q = p;
+        p += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    uint64_t* dbits = (uint64_t*)carve(sizeof(uint64_t) * A);
+    double* mdis = (double*)carve(sizeof(double) * A);
+    int32_t* pick = (int32_t*)carve(sizeof(int32_t) * A);
+    int32_t* mnum = (int32_t*)carve(sizeof(int32_t) * A);
+    int32_t* flag = (int32_t*)carve(sizeof(int32_t) * A);
+    int32_t* pos = (int32_t*)carve(sizeof(int32_t) * (A + 1));
+    double* axes = (double*)carve(sizeof(double) * (mg.Nxm + mg.Nym));
+    if ((size_t)(p - (char*)c->rep_pool) > c->rep_bytes)
+        return fail(c, EREB_ERR_NOMEM, "replenish scratch carve overflow");
+    EREB_HIP(c, hipMemcpyAsync(axes, mg.xxm, sizeof(double) * mg.Nxm, hipMemcpyHostToDevice,
+                               c->stream));
+    EREB_HIP(c, hipMemcpyAsync(axes + mg.Nxm, mg.yym, sizeof(double) * mg.Nym,
+                               hipMemcpyHostToDevice, c->stream));
+    MGridK G;
+    G.Nxm = (int)mg.Nxm;
+    G.Nym = (int)mg.Nym;
+    G.dii = (int)std::min<int64_t>(std::max<int64_t>(mg.dii, 0), mg.Nym);
+    G.djj = (int)std::min<int64_t>(std::max<int64_t>(mg.djj, 0), mg.Nxm);
+    G.dxm = mg.dxm;
+    G.dym = mg.dym;
+    G.idxm = 1.0 / mg.dxm;
+    G.idym = 1.0 / mg.dym;
+    G.mdis_init = mg.mdis_init;
+    G.xxm = axes;
+    G.yym = axes + mg.Nxm;
+    hipEvent_t ev;
+    prof_begin(c, EREB_K_REPLENISH, &ev);
+    const dim3 ga((unsigned)((A + 255) / 256)), gm((unsigned)((M + 255) / 256));
+    uint64_t init_bits;
+    std::memcpy(&init_bits, &mg.mdis_init, sizeof init_bits);
+    hipLaunchKernelGGL(k_anchor_init, ga, dim3(256), 0, c->stream, dbits, pick, init_bits, A);
+    if (M > 0) {
+        hipLaunchKernelGGL(k_anchor_min, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
+                           mptr(c, EREB_M_YM), M, dbits);
+        hipLaunchKernelGGL(k_anchor_pick, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
+                           mptr(c, EREB_M_YM), M, (const uint64_t*)dbits, pick);
+    }
+    hipLaunchKernelGGL(k_anchor_fill, ga, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
+                       mptr(c, EREB_M_YM), (const uint64_t*)dbits, (const int32_t*)pick, mdis,
+                       mnum, flag);
+    EREB_HIP(c, hipGetLastError());
+    if ((rc = scan_exclusive(c, flag, pos, A))) return rc;
+    int32_t total = 0;
+    EREB_HIP(c, hipMemcpyAsync(&total, pos + A, sizeof total, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    if (M + total > (int64_t)INT32_MAX) return fail(c, EREB_ERR_ARG, "marknum exceeds 2^31-1");
+    if (M + total > c->cap && (rc = reserve_markers(c, std::max<int64_t>(M + total,
+                                                                         c->cap + c->cap / 4))))
+        return rc;
+    if (total > 0) {
+        MarkerFields F;
+        for (int f = 0; f < EREB_M_NFIELDS; ++f) {
+            F.f[f] = c->d_m[f];
+            F.es[f] = (int)marker_es(c->precision, f);
+        }
+        hipLaunchKernelGGL(k_anchor_append, ga, dim3(256), 0, c->stream, G, F,
+                           (const int32_t*)mnum, (const int32_t*)flag, (const int32_t*)pos, M,
+                           randomized, seed);
+        EREB_HIP(c, hipGetLastError());
+    }
+    prof_end(c, EREB_K_REPLENISH, ev);
+    c->marknum = M + total;
+    ++c->pos_gen;
+    c->rep_mdis = mdis;
+    c->rep_mnum = mnum;
+    c->rep_n = A;
+    *added = total;
+    return EREB_OK;
+}
+
+}  // namespace ereb

@@ -24,7 +24,7 @@ __all__ = [
     "assemble_hydromechanical_lse_", "assemble_thermal_lse_", "CSC",
     "update_marker_viscosity_", "update_marker_porosity_",
     "update_p_nodes_melt_composition_", "perform_thermochemical_reaction_",
-    "backtrace_pressures_rk4_",
+    "backtrace_pressures_rk4_", "replenish_markers_",
 ]
 
 
@@ -252,3 +252,43 @@ def backtrace_pressures_rk4_(ctx: Context, pr, pr0, ps, ps0, pf, pf0, vx, vy, vx
         _in(ps, grid=True, keep=keep), _io(ps0, grid=True), _in(pf, grid=True, keep=keep),
         _io(pf0, grid=True), *[_in(a, grid=True, keep=keep) for a in (vx, vy, vxf, vyf)],
         float(dt)), "backtrace_pressures_rk4!")
+
+
+# argument order of replenish_markers! (E:5586-5614)
+REPLENISH_FIELDS = [
+    "xm", "ym", "tm", "tkm", "phim", "sxxm", "sxym", "etavpm", "phinewm", "pfm0", "XWsolidm",
+    "XWsolidm0", "rhototalm", "rhocptotalm", "etatotalm", "hrtotalm", "ktotalm",
+    "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
+    "alphasolidcur", "alphafluidcur", "tkm_rhocptotalm", "etafluidcur_inv_kphim",
+]
+
+
+def replenish_markers_(ctx: Context, xm, ym, tm, tkm, phim, sxxm, sxym, etavpm, phinewm, pfm0,
+                       XWsolidm, XWsolidm0, rhototalm, rhocptotalm, etatotalm, hrtotalm,
+                       ktotalm, inv_gggtotalm, fricttotalm, cohestotalm, tenstotalm,
+                       rhofluidcur, alphasolidcur, alphafluidcur, tkm_rhocptotalm,
+                       etafluidcur_inv_kphim, mdis, mnum, randomized=False, seed=0):
+    """replenish_markers! (E:5586).  Julia grows the vectors with push!;
+    here each argument is a numpy vector and the function returns the list
+    of the 26 grown vectors (in argument order) and the new marker count,
+    filling mdis and mnum ((Nym, Nxm), Fortran order) in place."""
+    vecs = [xm, ym, tm, tkm, phim, sxxm, sxym, etavpm, phinewm, pfm0, XWsolidm, XWsolidm0,
+            rhototalm, rhocptotalm, etatotalm, hrtotalm, ktotalm, inv_gggtotalm, fricttotalm,
+            cohestotalm, tenstotalm, rhofluidcur, alphasolidcur, alphafluidcur,
+            tkm_rhocptotalm, etafluidcur_inv_kphim]
+    n0 = len(xm)
+    g = ctx.geom
+    cap = n0 + g.Nxm * g.Nym  # at most one new marker per anchor
+    buf = []
+    for name, v in zip(REPLENISH_FIELDS, vecs):
+        dt = np.int64 if name == "tm" else np.float64
+        b = np.zeros(cap, dtype=dt)
+        b[:n0] = v
+        buf.append(b)
+    mg = g.marker_grid_c()
+    n = ctypes.c_int64()
+    ctx.check(ctx.lib.ereb_replenish_markers(
+        ctx.h, *[b.ctypes.data for b in buf], _io(mdis, grid=True),
+        _io(mnum, np.int64, grid=True), n0, cap, ctypes.addressof(mg), int(bool(randomized)),
+        int(seed), ctypes.addressof(n)), "replenish_markers!")
+    return [b[:n.value].copy() for b in buf], n.value

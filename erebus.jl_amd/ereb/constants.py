@@ -160,6 +160,14 @@ class Materials:
         return np.array(hrs, dtype=np.float64), np.array(hrf, dtype=np.float64)
 
 
+class CMarkerGrid(ctypes.Structure):
+    """Mirror of ``ereb_marker_grid`` (include/ereb.h)."""
+    _fields_ = [("Nxm", ctypes.c_int64), ("Nym", ctypes.c_int64),
+                ("dxm", ctypes.c_double), ("dym", ctypes.c_double),
+                ("xxm", _DP), ("yym", _DP), ("dii", ctypes.c_int64), ("djj", ctypes.c_int64),
+                ("mdis_init", ctypes.c_double)]
+
+
 class CHydration(ctypes.Structure):
     """Mirror of ``ereb_hydration`` (include/ereb.h)."""
     _fields_ = [(n, ctypes.c_double) for n in [
@@ -287,6 +295,28 @@ class Geometry:
             for a in self._axes.values():
                 a.setflags(write=False)
         return self._axes
+
+    def marker_axes(self) -> tuple:
+        """Marker launch lattice xxm = dxm/2:dxm:xsize-dxm/2 and yym (C:116,
+        C:119): Julia lifts these float ranges to the rationals
+        (k - 1/2) * xsize / Nxm, so each element is that rational rounded
+        once."""
+        from fractions import Fraction
+        fx, fy = Fraction(self.xsize) / self.Nxm, Fraction(self.ysize) / self.Nym
+        xxm = np.array([float((k + Fraction(1, 2)) * fx) for k in range(self.Nxm)])
+        yym = np.array([float((k + Fraction(1, 2)) * fy) for k in range(self.Nym)])
+        return xxm, yym
+
+    def marker_grid_c(self) -> CMarkerGrid:
+        """ereb_marker_grid of this geometry: dii = 5*Nymc, djj = 5*Nxmc
+        (E:5638-5639), mdis_init = 1e30 (C:122)."""
+        xxm, yym = self.marker_axes()
+        c = CMarkerGrid()
+        c.Nxm, c.Nym, c.dxm, c.dym = self.Nxm, self.Nym, self.dxm, self.dym
+        c.xxm, c.yym = xxm.ctypes.data_as(_DP), yym.ctypes.data_as(_DP)
+        c.dii, c.djj, c.mdis_init = 5 * self.Nymc, 5 * self.Nxmc, 1.0e30
+        c._keep = (xxm, yym)
+        return c
 
     def to_c(self) -> CGeometry:
         a = self.axes()

@@ -64,7 +64,7 @@ SUM_FIELDS = [
 
 KERNEL_CLASSES = ["sort", "scatter", "combine", "nodes", "gather_T", "gather_S",
                   "rk4", "asm_sd", "asm_t", "viscosity", "porosity", "melt", "reaction",
-                  "backtrace"]
+                  "backtrace", "replenish"]
 
 _D = ctypes.c_double
 _I64 = ctypes.c_int64
@@ -122,6 +122,8 @@ PROTOTYPES = {
     "ereb_perform_thermochemical_reaction": [_P, _P] + [_P] * 15 + [_I64, _D, _I64, _I64],
     "ereb_step_backtrace_pressures": [_P, _D],
     "ereb_backtrace_pressures_rk4": [_P] + [_P] * 10 + [_D],
+    "ereb_step_replenish_markers": [_P, _P, _I, ctypes.c_uint64, _P, _P, _P],
+    "ereb_replenish_markers": [_P] + [_P] * 28 + [_I64, _I64, _P, _I, ctypes.c_uint64, _P],
     "ereb_profile_enable": [_P, _I],
     "ereb_profile_get": [_P, _I, _P, _P],
     "ereb_profile_reset": [_P],
@@ -335,6 +337,19 @@ class Context:
     def step_backtrace_pressures(self, dt: float):
         self.check(self.lib.ereb_step_backtrace_pressures(self.h, float(dt)),
                    "step_backtrace_pressures")
+
+    def step_replenish_markers(self, randomized: bool = False, seed: int = 0):
+        """replenish_markers! on the resident markers: (new marknum, mdis,
+        mnum), mdis and mnum (Nym, Nxm) Fortran-ordered as Julia's."""
+        g = self.geom
+        mg = g.marker_grid_c()
+        mdis = np.empty((g.Nym, g.Nxm), order="F")
+        mnum = np.empty((g.Nym, g.Nxm), dtype=np.int64, order="F")
+        n = ctypes.c_int64()
+        self.check(self.lib.ereb_step_replenish_markers(
+            self.h, ctypes.addressof(mg), int(bool(randomized)), int(seed), _ptr(mdis),
+            _ptr(mnum), ctypes.addressof(n)), "step_replenish_markers")
+        return n.value, mdis, mnum
 
     def csc(self, which: int):
         """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64

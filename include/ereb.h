@@ -392,12 +392,58 @@ int ereb_backtrace_pressures_rk4(ereb_ctx* ctx, const double* pr, double* pr0,
     const double* vx, const double* vy, const double* vxf, const double* vyf,
     double dt);
 
+/* ---- marker replenishment ------------------------------------------------ */
+/* The marker launch lattice (C:104-135): Nxm x Nym anchor points.  The
+ * anchors are the Julia ranges xxm = dxm/2:dxm:xsize-dxm/2 and yym (C:116,
+ * C:119), passed element by element; the search half-widths are dii =
+ * 5*Nymc and djj = 5*Nxmc (E:5639-5640); mdis_init = 1e30 (C:122). */
+typedef struct {
+    int64_t Nxm, Nym;
+    double dxm, dym;
+    const double* xxm;   /* Nxm */
+    const double* yym;   /* Nym */
+    int64_t dii, djj;
+    double mdis_init;
+} ereb_marker_grid;
+
+/* replenish_markers! (E:5586-5698) on the resident markers: every anchor
+ * takes its nearest marker among those whose marker cell touches it (ties:
+ * the lowest marker index, as the sequential strict-< loop); an anchor left
+ * without one takes the nearest of the markers found by the anchors within
+ * (dii, djj) and, if there is one, gets a new marker appended (anchors in
+ * column-major order) with that marker's properties, at the anchor
+ * (randomized = 0, the reference's test mode) or jittered by U(-1/2, 1/2)
+ * cells (randomized = 1).  The jitter is drawn from a counter-based
+ * generator keyed by (seed, anchor), not from Julia's MersenneTwister
+ * stream.  mdis (Nym*Nxm) and mnum (Int64, Nym*Nxm; negative for the
+ * anchors that received a marker) may be NULL.  *marknum_out receives the
+ * new marker count; the resident arrays grow as needed. */
+int ereb_step_replenish_markers(ereb_ctx* ctx, const ereb_marker_grid* mg,
+    int randomized, uint64_t seed, double* mdis, int64_t* mnum,
+    int64_t* marknum_out);
+/* Drop-in form with the reference's argument list: the 26 marker vectors
+ * (capacity `capacity` each; the first marknum are read), mdis and mnum.
+ * Writes the new marker count to *marknum_out; when it exceeds capacity
+ * the call returns EREB_ERR_ARG and leaves the host arrays unchanged (the
+ * result stays resident: resize and read it with ereb_markers_get). */
+int ereb_replenish_markers(ereb_ctx* ctx, double* xm, double* ym, int64_t* tm,
+    double* tkm, double* phim, double* sxxm, double* sxym, double* etavpm,
+    double* phinewm, double* pfm0, double* XWsolidm, double* XWsolidm0,
+    double* rhototalm, double* rhocptotalm, double* etatotalm,
+    double* hrtotalm, double* ktotalm, double* inv_gggtotalm,
+    double* fricttotalm, double* cohestotalm, double* tenstotalm,
+    double* rhofluidcur, double* alphasolidcur, double* alphafluidcur,
+    double* tkm_rhocptotalm, double* etafluidcur_inv_kphim, double* mdis,
+    int64_t* mnum, int64_t marknum, int64_t capacity,
+    const ereb_marker_grid* mg, int randomized, uint64_t seed,
+    int64_t* marknum_out);
+
 /* ---- profiling: HIP events around the launches of one kernel class ------ */
 enum ereb_kernel_class {
     EREB_K_SORT = 0, EREB_K_SCATTER, EREB_K_COMBINE, EREB_K_NODES,
     EREB_K_GATHER_T, EREB_K_GATHER_S, EREB_K_RK4, EREB_K_ASM_SD,
     EREB_K_ASM_T, EREB_K_VISCOSITY, EREB_K_POROSITY, EREB_K_MELT,
-    EREB_K_REACTION, EREB_K_BACKTRACE, EREB_K_NCLASSES
+    EREB_K_REACTION, EREB_K_BACKTRACE, EREB_K_REPLENISH, EREB_K_NCLASSES
 };
 int ereb_profile_enable(ereb_ctx* ctx, int on);
 /* total milliseconds and launch count accumulated since the last reset */

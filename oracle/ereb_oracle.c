@@ -1134,6 +1134,80 @@ void orc_backtrace_pressures_rk4(const ereb_geometry* g, const double* pr,
         }
 }
 
+/* -------------------------------------------- marker replenishment (next) */
+
+/* distance E:1097-1099 */
+static inline double orc_distance(double x1, double y1, double x2, double y2) {
+    const double a = x1 - x2, b = y1 - y2;
+    return sqrt(a * a + b * b);
+}
+
+/* replenish_markers! E:5586-5698 with randomized = false (the reference's
+ * test mode): fields[f] is the marker vector of ereb_marker_field f
+ * (int64_t* for EREB_M_TM, double* otherwise) with room for `cap` markers;
+ * mdis (Nym*Nxm) and mnum (Nym*Nxm) are overwritten.  Returns the new
+ * marker count, or -1 when it would exceed cap (vectors then partly
+ * appended). */
+int64_t orc_replenish_markers(const ereb_marker_grid* mg, void* const* fields,
+                              int64_t marknum, int64_t cap, double* mdis,
+                              int64_t* mnum) {
+    const int64_t Nxm = mg->Nxm, Nym = mg->Nym;
+    double* xm = (double*)fields[EREB_M_XM];
+    double* ym = (double*)fields[EREB_M_YM];
+    for (int64_t k = 0; k < Nxm * Nym; ++k) {
+        mdis[k] = mg->mdis_init;
+        mnum[k] = 0;
+    }
+    /* update_marker_population_geometry! E:5518-5544 for every marker */
+    for (int64_t m = 0; m < marknum; ++m) {
+        int64_t i, j;
+        orc_fix(xm[m], ym[m], mg->xxm, mg->yym, mg->dxm, mg->dym, 1, Nxm - 1, 1,
+                Nym - 1, &i, &j);
+        const int64_t ci[4] = {i, i + 1, i, i + 1}, cj[4] = {j, j, j + 1, j + 1};
+        for (int q = 0; q < 4; ++q) {
+            const double d = orc_distance(xm[m], ym[m], mg->xxm[cj[q] - 1], mg->yym[ci[q] - 1]);
+            if (d < G(mdis, ci[q], cj[q], Nym)) {
+                G(mdis, ci[q], cj[q], Nym) = d;
+                G(mnum, ci[q], cj[q], Nym) = m + 1;
+            }
+        }
+    }
+    int64_t n = marknum;
+    for (int64_t j = 1; j <= Nxm; ++j)
+        for (int64_t i = 1; i <= Nym; ++i) {
+            if (G(mnum, i, j, Nym) != 0) continue;
+            const int64_t jlo = j - mg->djj > 1 ? j - mg->djj : 1;
+            const int64_t jhi = j + mg->djj < Nxm ? j + mg->djj : Nxm;
+            const int64_t ilo = i - mg->dii > 1 ? i - mg->dii : 1;
+            const int64_t ihi = i + mg->dii < Nym ? i + mg->dii : Nym;
+            for (int64_t jj = jlo; jj <= jhi; ++jj)
+                for (int64_t ii = ilo; ii <= ihi; ++ii)
+                    if (G(mnum, ii, jj, Nym) > 0) {
+                        const int64_t m = G(mnum, ii, jj, Nym);
+                        const double d = orc_distance(xm[m - 1], ym[m - 1], mg->xxm[j - 1],
+                                                      mg->yym[i - 1]);
+                        if (d < G(mdis, i, j, Nym)) {
+                            G(mdis, i, j, Nym) = d;
+                            G(mnum, i, j, Nym) = -m;
+                        }
+                    }
+            if (G(mnum, i, j, Nym) < 0) {
+                if (n >= cap) return -1;
+                const int64_t src = -G(mnum, i, j, Nym) - 1;
+                for (int f = 0; f < EREB_M_NFIELDS; ++f) {
+                    if (f == EREB_M_TM)
+                        ((int64_t*)fields[f])[n] = ((int64_t*)fields[f])[src];
+                    else
+                        ((double*)fields[f])[n] = ((double*)fields[f])[src];
+                }
+                xm[n] = mg->xxm[j - 1];
+                ym[n] = mg->yym[i - 1];
+                ++n;
+            }
+        }
+    return n;
+}
+
 /* Fine key of the library's sort: the basic cell (ib, jb) from fix with the
  * basic clamps (E:1456, C:68-74) refined by the P-cell offsets
  * (ip-ib, jp-jb) in {0,1}^2 (P clamps, C:94-101), numbered tile-major:

@@ -20,6 +20,7 @@ LIB = os.path.join(HERE, "_build", "libereb_oracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "erebus.jl_amd"))
 
 from ereb.constants import CGeometry, CMaterials, Geometry, Hydration, Materials  # noqa: E402
+from ereb.lib import MARKER_FIELDS  # noqa: E402
 
 _P = ctypes.c_void_p
 _D = ctypes.c_double
@@ -78,6 +79,7 @@ def lib():
             "orc_perform_thermochemical_reaction": (ctypes.c_int, [_P, _P, _P] + [_P] * 15 +
                                                     [_I64, _D, _I64, _I64]),
             "orc_backtrace_pressures_rk4": (None, [_P] + [_P] * 10 + [_D]),
+            "orc_replenish_markers": (_I64, [_P, _P, _I64, _I64, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -361,6 +363,27 @@ class Oracle:
                                            _p(out[1]), _p(ins[2]), _p(out[2]),
                                            *[_p(a) for a in ins[3:]], float(dt))
         return tuple(out)
+
+    def replenish_markers(self, mk: dict):
+        """replenish_markers! (E:5586) with randomized = false: (dict of the
+        grown marker vectors, new marknum, mdis, mnum)"""
+        g = self.geom
+        n0 = mk["xm"].size
+        cap = n0 + g.Nxm * g.Nym
+        buf = {}
+        for f in MARKER_FIELDS:
+            dt = np.int64 if f == "tm" else np.float64
+            b = np.zeros(cap, dtype=dt)
+            b[:n0] = mk[f]
+            buf[f] = b
+        ptrs = (ctypes.c_void_p * len(MARKER_FIELDS))(*[buf[f].ctypes.data for f in MARKER_FIELDS])
+        mg = g.marker_grid_c()
+        mdis = np.zeros((g.Nym, g.Nxm), order="F")
+        mnum = np.zeros((g.Nym, g.Nxm), dtype=np.int64, order="F")
+        n = self.L.orc_replenish_markers(ctypes.addressof(mg), ctypes.addressof(ptrs), n0, cap,
+                                         _p(mdis), _p(mnum))
+        assert n >= 0
+        return {f: buf[f][:n].copy() for f in MARKER_FIELDS}, n, mdis, mnum
 
     def rotation_rate(self, vx, vy):
         g = self.geom
