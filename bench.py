@@ -23,7 +23,11 @@ on identical inputs and reports its throughput and the per-field error
 against the Float64 state after the same steps.  `adjoint` times the
 gradient of sum(tk1^2) after 20 steps (1,000,000 markers, 201x201 nodes,
 checkpoints every 5 steps) against the forward map and checks it against
-central finite differences on 64 markers.  `cpu_baseline` times the
+central finite differences on 64 markers.  `next_rows` gives the mean
+device time and algorithmic bandwidth of the transfers outside the step
+(viscosity, porosity, melt composition, thermochemical reaction, pressure
+back-trace, marker replenishment) on the same resident state.
+`cpu_baseline` times the
 single-threaded C restatement of the reference CPU path (the oracle) on a
 bounded sample of the Float64 workload.
 """
@@ -218,6 +222,65 @@ def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
             "algorithmic_bytes": sc_bytes}
 
 
+# algorithmic bytes per marker of the transfers outside the timed step
+# (every Julia-visible marker vector read or written once; tm as 4 bytes)
+NEXT_ROW_BYTES = {
+    # xm ym tm tkm etatotalm (r) + etatotalm etavpm (w)
+    "viscosity": 2 * 8 + 4 + 8 + 8 + 2 * 8,
+    # xm ym tm phim (r) + phim (w)
+    "porosity": 2 * 8 + 4 + 8 + 8,
+    # xm ym XWsolidm0 (r) and the sort order (4)
+    "melt": 3 * 8 + 4,
+    # xm ym tm XWsolidm0 phim pfm0 (r) + pfm0 XWsolidm phinewm XWsolidm0 phim (w), perm
+    "reaction": 2 * 8 + 4 + 3 * 8 + 5 * 8 + 4,
+    # xm ym read by the two lattice passes
+    "replenish": 2 * 2 * 8,
+}
+
+
+def next_rows_leg(ctx, g, mat, hrs, hrf, reps: int = 5):
+    """Mean device time of each remaining marker<->grid transfer on the
+    resident Float64 state (HIP events on the library stream), called in
+    the reference's order after a scatter (so the P-grid scatters reuse its
+    sort, as at E:6629 and E:7019), with synthetic YNY_inv_ETA, APHI, pf,
+    pr and ps grids; the pressure back-trace is per node."""
+    rng = np.random.default_rng(5)
+    B, S = g.basic_shape, g.stag_shape
+    ctx.set_grid("YNY_inv_ETA", 1e-19 * (1 + rng.random(B)))
+    ctx.set_yny((rng.random(B) < 0.3).astype(np.uint8))
+    ctx.set_grid("APHI", 1e-12 * (rng.random(S) - 0.5))
+    for n in ("pf", "pr", "ps"):
+        ctx.set_grid(n, mat.psurface + 1e5 * rng.random(S))
+    hyd = ereb.Hydration()
+    dt = ereb.DT_LONGEST
+    M = ctx.marknum
+    out = {}
+    ctx.step_scatter(hrs, hrf, mat.marker_property_mode)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(reps):
+        ctx.step_marker_viscosity()
+        ctx.step_thermochemical_reaction(dt, 2, 3, hyd=hyd)
+        ctx.step_marker_porosity(dt, hyd=hyd)
+        ctx.step_melt_composition()
+        ctx.step_backtrace_pressures(dt)
+        ctx.step_replenish_markers()
+    ctx.sync()
+    for k in ("viscosity", "reaction", "porosity", "melt", "backtrace", "replenish"):
+        ms, n = ctx.profile_get(k)
+        ms /= max(n, 1)
+        if k == "backtrace":
+            nbytes = 10 * 8 * g.Ny1 * g.Nx1
+        else:
+            nbytes = M * NEXT_ROW_BYTES[k] + (3 * 8 * g.Ny1 * g.Nx1 if k in ("melt", "reaction")
+                                             else 0)
+        out[k] = {"ms": round(ms, 4), "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    ctx.profile(False)
+    out["added_markers"] = ctx.marknum - M
+    return out
+
+
 BREAKS = (263.0, 268.0, 273.0, 278.0, 410.0)  # kinks and steps of T-dependent properties
 
 
@@ -294,6 +357,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-next-rows", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -359,6 +423,9 @@ def main():
         }
         log(f"[bench] mixed {msm:.3f} ms/step, worst nodal field {wn} {wnv:.2e}")
         ctxm.close()
+    if not a.no_next_rows:
+        log("[bench] remaining transfer rows")
+        res["next_rows"] = next_rows_leg(ctx64, g, mat, hrs, hrf)
     ctx64.close()
     if not a.no_adjoint:
         log("[bench] adjoint leg")
