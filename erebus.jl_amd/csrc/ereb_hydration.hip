@@ -103,10 +103,15 @@ struct PScatterBase {
 template <class R>
 struct MeltOp {
     static constexpr int S = 2;  // XWSSUM, WTPSUM
+    static constexpr int GROUP = 4;  // markers of a fine cell loaded together
     const R* __restrict__ XW0;
-    __device__ __forceinline__ bool operator()(uint32_t m, const GeomK&, const double*, int,
-                                               int, double* v) const {
-        v[0] = (double)XW0[m];
+    struct In {
+        double xw;
+    };
+    __device__ __forceinline__ In load(uint32_t m) const { return In{(double)XW0[m]}; }
+    __device__ __forceinline__ bool operator()(uint32_t, const In& in, const GeomK&,
+                                               const double*, int, int, double* v) const {
+        v[0] = in.xw;
         v[1] = 1.0;
         return true;
     }
@@ -133,6 +138,7 @@ __device__ __forceinline__ double dtreaction(const ereb_hydration& H, double T, 
 template <class R>
 struct ReactionOp {
     static constexpr int S = 3;  // DMPSUM, DHPSUM, WTPSUM
+    static constexpr int GROUP = 1;  // arithmetic-bound: one marker at a time
     const ereb_materials* __restrict__ M;
     ereb_hydration H;
     const int32_t* __restrict__ tm;
@@ -146,15 +152,25 @@ struct ReactionOp {
     double dt;
     int timestep, titer;
 
-    __device__ __forceinline__ bool operator()(uint32_t m, const GeomK& g, const double* w,
-                                               int ip, int jp, double* v) const {
-        if (!(tm[m] < 3)) return false;
+    struct In {
+        int t;
+        double pf0, XW0, ph;
+    };
+    // every marker's inputs (rock or not), so that the loads of a group of
+    // markers can be issued together
+    __device__ __forceinline__ In load(uint32_t m) const {
+        return In{tm[m], pfm0[m], (double)XWsm0[m], phim[m]};
+    }
+    __device__ __forceinline__ bool operator()(uint32_t m, const In& in, const GeomK& g,
+                                               const double* w, int ip, int jp,
+                                               double* v) const {
+        if (!(in.t < 3)) return false;
         const ereb_materials& Mt = *M;
         const double tknm = gdot4(tk2, g.Ny1, ip, jp, w);
         double pfnm = jl_max(0.0, gdot4(pf, g.Ny1, ip, jp, w));
-        if (titer > 2) pfnm = pfnm * (1.0 - H.pfcoeff) + pfm0[m] * H.pfcoeff;
+        if (titer > 2) pfnm = pfnm * (1.0 - H.pfcoeff) + in.pf0 * H.pfcoeff;
         pfm0[m] = pfnm;
-        const double XW0 = (double)XWsm0[m], ph = phim[m];
+        const double XW0 = in.XW0, ph = in.ph;
         const double XDs0 = 1.0 - XW0;
         const bool liquid = tknm > Mt.tmfluidphase;
         const double VH2O = liquid ? H.VH2Of : H.VH2Ofi;
@@ -219,16 +235,33 @@ __global__ __launch_bounds__(PS_THREADS) void k_pscatter(PScatterBase B, Op op) 
         if (k0 < k1) {
             // markers exist only in fine cells whose P cell is on the grid
             const double xa = B.g.xp[jp - 1], ya = B.g.yp[ip - 1];
-            for (int k = k0; k < k1; ++k) {
-                const uint32_t m = B.perm[k];
-                double w[4];
-                weights4((B.xm[m] - xa) / B.g.dx, (B.ym[m] - ya) / B.g.dy, w);
-                double v[S];
-                if (op(m, B.g, w, ip, jp, v)) {
+            // groups of Op::GROUP markers: their loads are issued together,
+            // then the markers are summed one after the other in sorted order
+            constexpr int PS_GROUP = Op::GROUP;
+            for (int k = k0; k < k1; k += PS_GROUP) {
+                uint32_t mm[PS_GROUP];
+                double xs[PS_GROUP], ys[PS_GROUP];
+                typename Op::In in[PS_GROUP];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
+                for (int u = 0; u < PS_GROUP; ++u) mm[u] = B.perm[min(k + u, k1 - 1)];
 #pragma unroll
-                        for (int s = 0; s < S; ++s) acc[c][s] += v[s] * w[c];
+                for (int u = 0; u < PS_GROUP; ++u) {
+                    xs[u] = B.xm[mm[u]];
+                    ys[u] = B.ym[mm[u]];
+                    in[u] = op.load(mm[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < PS_GROUP; ++u) {
+                    if (k + u >= k1) break;
+                    double w[4];
+                    weights4((xs[u] - xa) / B.g.dx, (ys[u] - ya) / B.g.dy, w);
+                    double v[S];
+                    if (op(mm[u], in[u], B.g, w, ip, jp, v)) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+#pragma unroll
+                            for (int s = 0; s < S; ++s) acc[c][s] += v[s] * w[c];
+                    }
                 }
             }
         }

@@ -10,8 +10,9 @@
 //      markers at that distance -- the lowest index, as the strict `<` of the
 //      sequential loop keeps the first;
 //   2. every anchor left without a marker scans its (dii, djj) window in the
-//      reference's order for the markers found in pass 1 (E:5641-5654); it
-//      reads only pass-1 results, as the reference does (it tests mnum > 0);
+//      reference's order for the markers found in pass 1 (E:5641-5654), one
+//      wavefront per such anchor; it reads only pass-1 results, as the
+//      reference does (it tests mnum > 0);
 //   3. an exclusive scan of the "new marker" flags in column-major anchor
 //      order gives each new marker its index, so markers are appended in
 //      the reference's order, copying every resident marker vector.
@@ -63,9 +64,11 @@ __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __re
     for (int q = 0; q < 4; ++q) {
         const int ii = i + (q & 1), jj = j + (q >> 1);
         const double d = mdist(x, y, G.xxm[jj - 1], G.yym[ii - 1]);
-        if (d < G.mdis_init)
-            atomicMin((unsigned long long*)&dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)],
-                      (unsigned long long)__double_as_longlong(d));
+        const uint64_t b = (uint64_t)__double_as_longlong(d);
+        uint64_t* t = &dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)];
+        // the stored value only decreases: skip atomics that cannot win
+        if (d < G.mdis_init && b < __atomic_load_n(t, __ATOMIC_RELAXED))
+            atomicMin((unsigned long long*)t, (unsigned long long)b);
     }
 }
 
@@ -89,42 +92,83 @@ __global__ __launch_bounds__(256) void k_anchor_pick(MGridK G, const double* __r
     }
 }
 
-// pass 2: final mdis / mnum of every anchor and the "new marker" flag
-__global__ __launch_bounds__(256) void k_anchor_fill(MGridK G, const double* __restrict__ xm,
-                                                     const double* __restrict__ ym,
+// pass 2a: mdis / mnum of the anchors that have a pass-1 marker; the others
+// get (mdis_init, 0, no flag) and go to the list of empty anchors
+__global__ __launch_bounds__(256) void k_anchor_fill(MGridK G,
                                                      const uint64_t* __restrict__ dbits,
                                                      const int32_t* __restrict__ pick,
                                                      double* __restrict__ mdis,
                                                      int32_t* __restrict__ mnum,
-                                                     int32_t* __restrict__ flag) {
+                                                     int32_t* __restrict__ flag,
+                                                     int32_t* __restrict__ empty,
+                                                     int32_t* __restrict__ nempty) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= (int64_t)G.Nxm * G.Nym) return;
     const int p = pick[a];
+    flag[a] = 0;
     if (p != INT32_MAX) {
         mdis[a] = __longlong_as_double((long long)dbits[a]);
         mnum[a] = p;
-        flag[a] = 0;
         return;
     }
-    const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
-    const double xa = G.xxm[j - 1], ya = G.yym[i - 1];
-    double best = G.mdis_init;
-    int num = 0;
-    const int jlo = max(j - G.djj, 1), jhi = min(j + G.djj, G.Nxm);
-    const int ilo = max(i - G.dii, 1), ihi = min(i + G.dii, G.Nym);
-    for (int jj = jlo; jj <= jhi; ++jj)
-        for (int ii = ilo; ii <= ihi; ++ii) {
+    mdis[a] = G.mdis_init;
+    mnum[a] = 0;
+    empty[atomicAdd(nempty, 1)] = (int32_t)a;  // list order does not affect any result
+}
+
+// pass 2b: one wavefront per empty anchor scans its window in the
+// reference's order (jj outer, ii inner), reading only pass-1 picks (the
+// reference tests mnum > 0, so anchors filled here never feed another).
+// Lanes take window positions w, w + 64, ...; the wavefront keeps the
+// smallest distance and, among equal ones, the first position -- the pick
+// of the sequential strict-< scan.
+__global__ __launch_bounds__(256) void k_anchor_search(MGridK G, const double* __restrict__ xm,
+                                                       const double* __restrict__ ym,
+                                                       const int32_t* __restrict__ pick,
+                                                       const int32_t* __restrict__ empty,
+                                                       const int32_t* __restrict__ nempty,
+                                                       double* __restrict__ mdis,
+                                                       int32_t* __restrict__ mnum,
+                                                       int32_t* __restrict__ flag) {
+    const int lane = threadIdx.x & 63;
+    const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwv = (int)((gridDim.x * blockDim.x) >> 6);
+    const int ne = *nempty;
+    for (int e = wv; e < ne; e += nwv) {  // wave-uniform bound: every wave exits
+        const int64_t a = empty[e];
+        const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
+        const double xa = G.xxm[j - 1], ya = G.yym[i - 1];
+        const int jlo = max(j - G.djj, 1), jhi = min(j + G.djj, G.Nxm);
+        const int ilo = max(i - G.dii, 1), ihi = min(i + G.dii, G.Nym);
+        const int h = ihi - ilo + 1, nw = h * (jhi - jlo + 1);
+        double best = G.mdis_init;
+        int bw = INT32_MAX, bq = 0;
+        for (int w = lane; w < nw; w += 64) {
+            const int jj = jlo + w / h, ii = ilo + w % h;
             const int q = pick[(size_t)(jj - 1) * G.Nym + (ii - 1)];
-            if (q == INT32_MAX) continue;  // mnum[ii, jj] > 0 only for pass-1 anchors
+            if (q == INT32_MAX) continue;
             const double d = mdist(xm[q - 1], ym[q - 1], xa, ya);
-            if (d < best) {
+            if (d < best) {  // positions rise within a lane: strict < keeps the first
                 best = d;
-                num = -q;
+                bw = w;
+                bq = q;
             }
         }
-    mdis[a] = best;
-    mnum[a] = num;
-    flag[a] = num < 0 ? 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(best, off);
+            const int ow = __shfl_xor(bw, off), oq = __shfl_xor(bq, off);
+            if (od < best || (od == best && ow < bw)) {
+                best = od;
+                bw = ow;
+                bq = oq;
+            }
+        }
+        if (lane == 0 && bw != INT32_MAX) {
+            mdis[a] = best;
+            mnum[a] = -bq;
+            flag[a] = 1;
+        }
+    }
 }
 
 struct MarkerFields {
@@ -175,9 +219,10 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     if (mg.Nxm < 2 || mg.Nym < 2 || !mg.xxm || !mg.yym || A > INT32_MAX)
         return fail(c, EREB_ERR_ARG, "marker lattice needs Nxm, Nym >= 2 and both axes");
     int rc;
-    // scratch: distance bits, picks, mdis, mnum, flags, positions, axes
-    const size_t need = A * (sizeof(uint64_t) + sizeof(double) + 4 * sizeof(int32_t)) +
-                        sizeof(int32_t) + sizeof(double) * (mg.Nxm + mg.Nym) + 256 * 8;
+    // scratch: distance bits, picks, mdis, mnum, flags, positions, the
+    // empty-anchor list and its count, axes
+    const size_t need = A * (sizeof(uint64_t) + sizeof(double) + 5 * sizeof(int32_t)) +
+                        2 * sizeof(int32_t) + sizeof(double) * (mg.Nxm + mg.Nym) + 256 * 10;
     if (need > c->rep_bytes) {
         hipFree(c->rep_pool);
         c->rep_pool = nullptr;
@@ -197,6 +242,8 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     int32_t* mnum = (int32_t*)carve(sizeof(int32_t) * A);
     int32_t* flag = (int32_t*)carve(sizeof(int32_t) * A);
     int32_t* pos = (int32_t*)carve(sizeof(int32_t) * (A + 1));
+    int32_t* empty = (int32_t*)carve(sizeof(int32_t) * A);
+    int32_t* nempty = (int32_t*)carve(sizeof(int32_t));
     double* axes = (double*)carve(sizeof(double) * (mg.Nxm + mg.Nym));
     if ((size_t)(p - (char*)c->rep_pool) > c->rep_bytes)
         return fail(c, EREB_ERR_NOMEM, "replenish scratch carve overflow");
@@ -228,9 +275,12 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
         hipLaunchKernelGGL(k_anchor_pick, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
                            mptr(c, EREB_M_YM), M, (const uint64_t*)dbits, pick);
     }
-    hipLaunchKernelGGL(k_anchor_fill, ga, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
-                       mptr(c, EREB_M_YM), (const uint64_t*)dbits, (const int32_t*)pick, mdis,
-                       mnum, flag);
+    if ((rc = fill_i32(c, nempty, 0, 1))) return rc;
+    hipLaunchKernelGGL(k_anchor_fill, ga, dim3(256), 0, c->stream, G, (const uint64_t*)dbits,
+                       (const int32_t*)pick, mdis, mnum, flag, empty, nempty);
+    hipLaunchKernelGGL(k_anchor_search, dim3(1024), dim3(256), 0, c->stream, G,
+                       mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), (const int32_t*)pick,
+                       (const int32_t*)empty, (const int32_t*)nempty, mdis, mnum, flag);
     EREB_HIP(c, hipGetLastError());
     if ((rc = scan_exclusive(c, flag, pos, A))) return rc;
     int32_t total = 0;
@@ -252,8 +302,10 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
         EREB_HIP(c, hipGetLastError());
     }
     prof_end(c, EREB_K_REPLENISH, ev);
-    c->marknum = M + total;
-    ++c->pos_gen;
+    if (total > 0) {
+        c->marknum = M + total;
+        ++c->pos_gen;
+    }
     c->rep_mdis = mdis;
     c->rep_mnum = mnum;
     c->rep_n = A;

@@ -46,7 +46,14 @@ constexpr int CH = 256;      // markers per LDS chunk (one per thread in the loa
 constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
 constexpr int NCF = 2 * FINE;  // fine cells in one basic column of a tile
 constexpr int NLF = 18;        // property values staged per marker
-constexpr int NPART = NCF * 4 * NSUM;  // (fine cell, cj, ci) x SUM partials per column
+// Partials of one column: value q of grid G for (half column h, corner row
+// ci, corner column cj, fine row fi) at
+//   part[poff + q*PQ + h*PH + ci*PCI + cj*PCJ + fi].
+// In the combine, the 16 lanes that read one value hold 8 fine rows 2 apart
+// and the two corner columns: fine rows 2 doubles apart and an odd corner
+// column stride (13 doubles) put the 16 reads in 32 distinct LDS banks.
+constexpr int PCJ = FINE + 1, PCI = 2 * PCJ, PH = 2 * PCI, PQ = 2 * PH;
+constexpr int NPART = NSUM * PQ;
 
 // staged property indices
 enum {
@@ -99,8 +106,7 @@ template <> struct NodeAcc<float> {
 };
 
 // per-grid SUM slot offset, partial offset and staged-property list (SUM
-// order of reset_interpolated_properties!, E:968-999).  Partials of grid G
-// sit at part[poff + ((fl*2 + cj)*2 + ci)*(nf+1) + q].
+// order of reset_interpolated_properties!, E:968-999).
 template <int G> struct GridSpec;
 template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
     static constexpr int slot0 = 0, poff = 0, nf = 7;
@@ -111,17 +117,17 @@ template <> struct GridSpec<0> {  // basic: ETA0 ETA GGG SXY COH TEN FRI (+WT)
     }
 };
 template <> struct GridSpec<1> {  // Vx: RHOX RHOFX KX PHIX RX (+WTX)
-    static constexpr int slot0 = 8, poff = NCF * 4 * 8, nf = 5;
+    static constexpr int slot0 = 8, poff = PQ * 8, nf = 5;
     static constexpr bool prow = true, pcol = false;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<2> {  // Vy: RHOY RHOFY KY PHIY RY (+WTY)
-    static constexpr int slot0 = 14, poff = NCF * 4 * 14, nf = 5;
+    static constexpr int slot0 = 14, poff = PQ * 14, nf = 5;
     static constexpr bool prow = false, pcol = true;
     __device__ static constexpr int f(int q) { return L_RHOTOTAL + q; }
 };
 template <> struct GridSpec<3> {  // P: RHO RHOCP ALPHA ALPHAF HR GGGP SXX TK PHI (+WTP)
-    static constexpr int slot0 = 20, poff = NCF * 4 * 20, nf = 9;
+    static constexpr int slot0 = 20, poff = PQ * 20, nf = 9;
     static constexpr bool prow = true, pcol = true;
     __device__ static constexpr int f(int q) {
         return q == 0 ? L_RHOTOTAL : q == 1 ? L_RHOCP : q == 2 ? L_ALPHAS : q == 3 ? L_ALPHAF
@@ -187,11 +193,12 @@ __device__ __forceinline__ void tasks(ScatterLDS<R>& L, int lane, int f0, int c0
             b = bn;
         }
     }
-    R* p = &L.part[S::poff + (fl * 2 + cj) * 2 * NQ + Q0];
+    const int h = fl >= FINE ? 1 : 0, fi = fl - h * FINE;
+    R* p = &L.part[S::poff + Q0 * PQ + h * PH + cj * PCJ + fi];
 #pragma unroll
     for (int q = 0; q < QN; ++q) {
-        p[q] = s0[q];
-        p[NQ + q] = s1[q];
+        p[q * PQ] = s0[q];
+        p[q * PQ + PCI] = s1[q];
     }
 }
 
@@ -202,7 +209,6 @@ template <class R, int G, int Q0, int QN>
 __device__ __forceinline__ void combine(const ScatterLDS<R>& L, NodeAcc<R>& acc, int r, int c,
                                         int k) {
     using S = GridSpec<G>;
-    constexpr int NQ = S::nf + 1;
     const int fi_lo = S::prow ? 2 * r - 3 : 2 * r - 2;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -214,9 +220,9 @@ __device__ __forceinline__ void combine(const ScatterLDS<R>& L, NodeAcc<R>& acc,
         for (int d = 0; d < 4; ++d) {
             const int fi = min(max(fi_lo + d, 0), FINE - 1);  // clamped; skipped below
             const int ci = 1 - (d >> 1);  // = r - (grid row of fine row fi)
-            const R* p = &L.part[S::poff + (((h * FINE + fi) * 2 + cj) * 2 + ci) * NQ + Q0];
+            const R* p = &L.part[S::poff + Q0 * PQ + h * PH + ci * PCI + cj * PCJ + fi];
 #pragma unroll
-            for (int q = 0; q < QN; ++q) v[d][q] = p[q];
+            for (int q = 0; q < QN; ++q) v[d][q] = p[q * PQ];
         }
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
