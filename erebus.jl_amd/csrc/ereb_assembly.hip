@@ -340,18 +340,13 @@ __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A, const SlotTables* _
     pres_out[(size_t)e * ncell + k] = (uint16_t)pres;
 }
 
-// 2./3. columns: one thread per column (consecutive threads, consecutive
-// columns, so each block writes one contiguous stretch of rowval/nzval); the
-// slot tables are staged in LDS
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_asm_cols(int Nx1, int Ny1,
-                                                  const SlotTables* __restrict__ Tg,
-                                                  const uint16_t* __restrict__ pres,
-                                                  const double* __restrict__ slotv,
-                                                  int32_t* __restrict__ colcnt,
-                                                  const int32_t* __restrict__ colptr,
-                                                  int32_t* __restrict__ rowval,
-                                                  double* __restrict__ nzval) {
+// 2. column counts: one thread per column walks the rows that can hold an
+// entry in it (the reverse slot table of its unknown), in ascending row
+// order, and counts the present ones; the slot tables are staged in LDS
+__global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
+                                                   const SlotTables* __restrict__ Tg,
+                                                   const uint16_t* __restrict__ pres,
+                                                   int32_t* __restrict__ colcnt) {
     __shared__ SlotTables T;
     static_assert(sizeof(SlotTables) % 4 == 0, "table size");
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
@@ -364,21 +359,67 @@ __global__ __launch_bounds__(256) void k_asm_cols(int Nx1, int Ny1,
     const int u = (int)(col % ne);
     const int64_t k = col / ne;
     const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
-    int32_t p = FILL ? colptr[col] : 0;
+    int32_t p = 0;
     const int nr = T.nrev[u];
     for (int r = 0; r < nr; ++r) {
         const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
         if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
         const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
         const int64_t cell = (int64_t)jr * Ny1 + ir;
-        if (!((pres[(size_t)e * ncell + cell] >> sl) & 1)) continue;
-        if (FILL) {
-            rowval[p] = (int32_t)(cell * ne + e);
-            nzval[p] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
-        }
-        ++p;
+        p += (pres[(size_t)e * ncell + cell] >> sl) & 1;
     }
-    if (!FILL) colcnt[col] = p;
+    colcnt[col] = p;
+}
+
+// 3. rowval/nzval: the same column walk, with the entries of the block's
+// columns staged in LDS at their CSC positions and then written out by
+// consecutive threads (a thread's own entries are ~7 apart from its
+// neighbours', so direct stores would fill a fraction of each cache line).
+// Rows come out ascending in every column with no sort.
+// blockDim.x * (most entries of a column) <= ASM_STAGE, checked on the host.
+constexpr int ASM_STAGE = 4096;
+
+__global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
+                                                  const SlotTables* __restrict__ Tg,
+                                                  const uint16_t* __restrict__ pres,
+                                                  const double* __restrict__ slotv,
+                                                  const int32_t* __restrict__ colptr,
+                                                  int32_t* __restrict__ rowval,
+                                                  double* __restrict__ nzval) {
+    __shared__ SlotTables T;
+    __shared__ int32_t srow[ASM_STAGE];
+    __shared__ double sval[ASM_STAGE];
+    for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
+        ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
+    __syncthreads();
+    const int ne = T.ne;
+    const int64_t ncell = (int64_t)Nx1 * Ny1, n = ncell * ne;
+    const int64_t col0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t colN = min(col0 + (int64_t)blockDim.x, n);
+    const int32_t base = colptr[col0], total = colptr[colN] - base;
+    const int64_t col = col0 + threadIdx.x;
+    if (col < n) {
+        const int u = (int)(col % ne);
+        const int64_t k = col / ne;
+        const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
+        int32_t p = colptr[col] - base;
+        const int nr = T.nrev[u];
+        for (int r = 0; r < nr; ++r) {
+            const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
+            if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
+            const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
+            const int64_t cell = (int64_t)jr * Ny1 + ir;
+            if (!((pres[(size_t)e * ncell + cell] >> sl) & 1)) continue;
+            srow[p] = (int32_t)(cell * ne + e);
+            sval[p] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
+            ++p;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        rowval[base + q] = srow[q];
+        nzval[base + q] = sval[q];
+    }
 }
 
 // ---- slot tables, discovered on the host from the equation code -----------
@@ -512,6 +553,8 @@ static int assemble(ereb_ctx* c, double dt) {
         EREB_HIP(c, hipMemcpy(s.tables, &T, sizeof T, hipMemcpyHostToDevice));
         s.ne = T.ne;
         s.nstot = T.nstot;
+        s.maxrev = 0;
+        for (int u = 0; u < T.ne; ++u) s.maxrev = std::max(s.maxrev, T.nrev[u]);
     }
     const int64_t ncell = (int64_t)c->Nx1 * c->Ny1;
     const int64_t n = ncell * s.ne;
@@ -526,12 +569,16 @@ static int assemble(ereb_ctx* c, double dt) {
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, T, s.pres, s.slotv,
                        c->d_flags);
     const dim3 cgrid((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL(k_asm_cols<false>, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
-                       s.pres, s.slotv, s.colcnt, (const int32_t*)nullptr, (int32_t*)nullptr,
-                       (double*)nullptr);
+    hipLaunchKernelGGL(k_asm_count, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
+                       s.colcnt);
     if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
-    hipLaunchKernelGGL(k_asm_cols<true>, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T,
-                       s.pres, s.slotv, s.colcnt, s.colptr, s.rowval, s.nzval);
+    // a block's entries must fit the LDS stage: 256 columns of up to 16
+    // entries, else 128 of up to 32 (SLOT_R)
+    const int fb = s.maxrev <= ASM_STAGE / 256 ? 256 : 128;
+    static_assert(ASM_STAGE / 128 >= SLOT_R, "stage holds 128 columns of SLOT_R entries");
+    hipLaunchKernelGGL(k_asm_fill, dim3((unsigned)((n + fb - 1) / fb)), dim3(fb), 0, c->stream,
+                       c->Nx1, c->Ny1, T, s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval,
+                       s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;

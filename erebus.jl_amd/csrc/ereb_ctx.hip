@@ -230,6 +230,9 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         return EREB_ERR_ARG;
     const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
     if (Nx > 1000000 || Ny > 1000000) return EREB_ERR_ARG;
+    // grid fields stay below 2^28 elements: kernels index them with 32-bit
+    // byte offsets
+    if (Nx1 * Ny1 > ((int64_t)1 << 28)) return EREB_ERR_ARG;
     // the staggered axes coincide element for element (C:40-64)
     for (int64_t k = 0; k < Nx; ++k)
         if (g->xvx[k] != g->x[k]) return EREB_ERR_ARG;

@@ -59,10 +59,18 @@ __device__ __forceinline__ void weights4(double a, double b, double* w) {
     w[3] = b * a;
 }
 
+// Address of grid element (i, j), 1-based, column-major.  Grid fields hold
+// fewer than 2^28 elements (checked when the context is created), so the
+// element index is computed in 32-bit arithmetic.
+__device__ __forceinline__ const double* gptr_at(const double* __restrict__ base, int i, int j,
+                                                 int rows) {
+    return base + (uint32_t)((j - 1) * rows + (i - 1));
+}
+
 // dot4(grid_vector(i, j, grid), w) (E:1137-1159), 1-based (i, j)
 __device__ __forceinline__ double gdot4(const double* __restrict__ gr, int rows,
                                         int i, int j, const double* w) {
-    const double* p = gr + (size_t)(j - 1) * rows + (i - 1);
+    const double* p = gptr_at(gr, i, j, rows);
     return p[0] * w[0] + p[1] * w[1] + p[rows] * w[2] + p[rows + 1] * w[3];
 }
 

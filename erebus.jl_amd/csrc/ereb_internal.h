@@ -23,6 +23,7 @@ constexpr int NFINE = FINE * FINE;  // fine cells (sort keys) per tile
 struct Sys {
     int64_t n = 0, cap = 0, ncell = 0;
     int ne = 0, nstot = 0;       // equations and stencil slots per P cell
+    int maxrev = 0;              // most entries of one column (slot tables)
     void* tables = nullptr;      // device copy of the stencil slot tables
     int32_t* colcnt = nullptr;   // n
     int32_t* colptr = nullptr;   // n + 1, 0-based

@@ -89,7 +89,7 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
     const int i = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     const double dxmj = x - g.x[j - 1], dymi = y - g.yp[i - 1];
     const double idx = g.idx, idy = g.idy;
-    const double* p = v + (size_t)(j - 1) * R + (i - 1);  // v[i, j]
+    const double* p = gptr_at(v, i, j, R);  // v[i, j]
     const double fx = dxmj * idx, fy = dymi * idy;
     double v13 = p[0] * (1.0 - fx) + (p[R] * dxmj) * idx;
     double v24 = p[1] * (1.0 - fx) + (p[R + 1] * dxmj) * idx;
@@ -99,7 +99,7 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
     if (right ? j < g.Nx - 1 : j > 1) {
         const double t = fx - 0.5;
         const double* q = right ? p : p - R;  // first column of the stencil
-        const double c13 = right ? p[R] : vc[(size_t)(j - 1) * R + (i - 1)];
+        const double c13 = right ? p[R] : *gptr_at(vc, i, j, R);
         v13 += 0.5 * (t * t) * (q[0] - 2.0 * c13 + q[2 * R]);
         v24 += 0.5 * (t * t) * (q[1] - 2.0 * q[R + 1] + q[2 * R + 1]);
     }
@@ -114,7 +114,7 @@ __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict
     const int i = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
     const double dxmj = x - g.xp[j - 1], dymi = y - g.y[i - 1];
     const double idx = g.idx, idy = g.idy;
-    const double* p = v + (size_t)(j - 1) * R + (i - 1);
+    const double* p = gptr_at(v, i, j, R);
     const double fx = dxmj * idx, fy = dymi * idy;
     double v12 = p[0] * (1.0 - fy) + (p[1] * dymi) * idy;
     double v34 = p[R] * (1.0 - fy) + (p[R + 1] * dymi) * idy;
