@@ -318,6 +318,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (int f = 0; f < EREB_M_NFIELDS; ++f) hipFree(c->d_m[f]);
     hipFree(c->d_axes);
     hipFree(c->d_mat);
+    if (c->step_exec) hipGraphExecDestroy(c->step_exec);
     hipFree(c->adj_pool);
     hipFree(c->rep_pool);
     hipFree(c->d_grids);
@@ -504,6 +505,99 @@ int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrflui
     if ((rc = launch_fine_sort(c))) return rc;
     if ((rc = launch_scatter(c, hrsolidm, hrfluidm, (int)mode, true))) return rc;
     return launch_combine(c);
+}
+
+// The transfer step in the reference's order: marker properties + scatter
+// (E:6419-6517), node properties + insulating tk1 boundary (E:6522-6604),
+// Stokes-Darcy and thermal assembly (E:6680, E:6922), stress and
+// temperature gathers (E:6980, E:7004), RK4 (E:7036).
+static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
+                             double dt, int64_t timestep) {
+    int rc;
+    if ((rc = ereb_step_scatter(c, hrs, hrf, mode))) return rc;
+    if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
+    if ((rc = launch_assemble_stokes(c, dt))) return rc;
+    if ((rc = launch_assemble_thermal(c, dt))) return rc;
+    if ((rc = launch_gather_stress(c))) return rc;
+    if ((rc = launch_gather_temperature(c, timestep))) return rc;
+    return launch_rk4(c, dt, (int)mode);
+}
+
+static StepKey step_key(const ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
+                        double dt, int64_t timestep) {
+    StepKey k;
+    std::memset(&k, 0, sizeof k);  // padding included: keys are compared bytewise
+    k.marknum = c->marknum;
+    k.cap = c->cap;
+    k.mode = mode;
+    k.precision = c->precision;
+    k.first = timestep == 1;
+    k.dt = dt;
+    for (int q = 0; q < 3; ++q) {
+        k.hrs[q] = hrs[q];
+        k.hrf[q] = hrf[q];
+    }
+    int n = 0;
+    for (int f = 0; f < EREB_M_NFIELDS; ++f) k.bufs[n++] = c->d_m[f];
+    const void* more[] = {c->d_key, c->d_perm, c->d_tmp, c->d_scan, c->d_grids, c->d_part};
+    for (const void* p : more) k.bufs[n++] = p;
+    for (const Sys& s : c->sys) {
+        const void* sb[] = {s.colcnt, s.colptr, s.pres, s.slotv, s.rowval, s.nzval};
+        for (const void* p : sb) k.bufs[n++] = p;
+    }
+    return k;
+}
+
+int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrfluidm[3],
+                       int64_t mode, double dt, int64_t timestep) {
+    if (!c || !hrsolidm || !hrfluidm) return EREB_ERR_ARG;
+    int rc = valid_mode(c, mode);
+    if (rc) return rc;
+    // eager: when asked, while profiling (events around every launch), and
+    // for the first step (it makes the allocations a capture must not)
+    if (!c->use_graphs || c->prof || !c->step_warm) {
+        rc = transfer_launches(c, hrsolidm, hrfluidm, mode, dt, timestep);
+        if (!rc) c->step_warm = true;
+        return rc;
+    }
+    const StepKey k = step_key(c, hrsolidm, hrfluidm, mode, dt, timestep);
+    if (!c->step_exec || std::memcmp(&k, &c->step_key, sizeof k) != 0) {
+        if (c->step_exec) {
+            hipGraphExecDestroy(c->step_exec);
+            c->step_exec = nullptr;
+        }
+        const uint64_t pos0 = c->pos_gen;
+        hipGraph_t g = nullptr;
+        EREB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = transfer_launches(c, hrsolidm, hrfluidm, mode, dt, timestep);
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : hipfail(c, e, "hipStreamEndCapture");
+        }
+        c->pos_gen = pos0;  // the captured launches have not run yet
+        const hipError_t ei = hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->step_exec = nullptr;
+            return hipfail(c, ei, "hipGraphInstantiate");
+        }
+        c->step_key = k;
+    }
+    EREB_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    // host state of the replayed step, as the eager launches leave it: the
+    // sort is of the positions before RK4, which then moved them
+    c->have_sort = true;
+    c->sort_gen = c->pos_gen;
+    ++c->pos_gen;
+    c->sys[0].valid = c->sys[1].valid = true;
+    return EREB_OK;
+}
+
+int ereb_ctx_set_graphs(ereb_ctx* c, int on) {
+    if (!c) return EREB_ERR_ARG;
+    c->use_graphs = on != 0;
+    return EREB_OK;
 }
 
 int ereb_step_node_properties(ereb_ctx* c) {

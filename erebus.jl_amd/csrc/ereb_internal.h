@@ -40,6 +40,16 @@ struct ProfPair {
     hipEvent_t a, b;
 };
 
+// What a captured transfer step depends on: its scalar arguments and every
+// device buffer its kernels were launched with.  A replay is valid while
+// all of them are unchanged.
+struct StepKey {
+    int64_t marknum, cap, mode;
+    int precision, first;
+    double dt, hrs[3], hrf[3];
+    const void* bufs[EREB_M_NFIELDS + 18];
+};
+
 }  // namespace ereb
 
 struct ereb_ctx {
@@ -92,6 +102,12 @@ struct ereb_ctx {
     double* rep_mdis = nullptr;
     int32_t* rep_mnum = nullptr;
     int64_t rep_n = 0;
+
+    // HIP graph of the resident transfer step (ereb_step_transfer)
+    bool use_graphs = true;
+    bool step_warm = false;  // one eager step has run (allocations made)
+    hipGraphExec_t step_exec = nullptr;
+    ereb::StepKey step_key{};
 
     bool prof = false;
     std::vector<ereb::ProfPair> prof_pending;

@@ -112,6 +112,8 @@ PROTOTYPES = {
     "ereb_assemble_hydromechanical_lse": [_P] + [_P] * 20 + [_D, _P, _P],
     "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
     "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
+    "ereb_step_transfer": [_P, _P, _P, _I64, _D, _I64],
+    "ereb_ctx_set_graphs": [_P, _I],
     "ereb_step_marker_viscosity": [_P],
     "ereb_step_marker_porosity": [_P, _P, _D],
     "ereb_step_melt_composition": [_P],
@@ -286,6 +288,16 @@ class Context:
         hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
         self.check(self.lib.ereb_step_scatter(self.h, _ptr(hs), _ptr(hf), int(mode)),
                    "step_scatter")
+
+    def step_transfer(self, hrsolidm, hrfluidm, mode: int, dt: float, timestep: int):
+        """The whole resident transfer step (graph-replayed after the first)."""
+        hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
+        hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
+        self.check(self.lib.ereb_step_transfer(self.h, _ptr(hs), _ptr(hf), int(mode), float(dt),
+                                               int(timestep)), "step_transfer")
+
+    def set_graphs(self, on: bool):
+        self.check(self.lib.ereb_ctx_set_graphs(self.h, int(bool(on))), "set_graphs")
 
     def step_node_properties(self):
         self.check(self.lib.ereb_step_node_properties(self.h), "step_node_properties")

@@ -199,6 +199,21 @@ int ereb_step_assemble_stokes_darcy(ereb_ctx* ctx, double dt, int64_t* nnz);
 /* assemble_thermal_lse! (E:4526) */
 int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
 
+/* The whole transfer step on the resident state, in the reference's order:
+ * ereb_step_scatter, ereb_step_node_properties,
+ * ereb_step_assemble_stokes_darcy, ereb_step_assemble_thermal,
+ * ereb_step_gather_stress, ereb_step_gather_temperature(timestep),
+ * ereb_step_advect_rk4 (E:6419-7053 without the host solves).  After the
+ * first call the step's launches are captured once into a HIP graph and
+ * replayed; the graph is captured again when an argument, the marker count,
+ * the precision form or a device buffer changes.  Results are those of the
+ * separate calls, bit for bit.  Eager while profiling is enabled. */
+int ereb_step_transfer(ereb_ctx* ctx, const double hrsolidm[3],
+                       const double hrfluidm[3], int64_t mode, double dt,
+                       int64_t timestep);
+/* graph replay of ereb_step_transfer on (default) or off */
+int ereb_ctx_set_graphs(ereb_ctx* ctx, int on);
+
 #define EREB_SYS_STOKES_DARCY 0
 #define EREB_SYS_THERMAL      1
 /* Copy the last assembled system to the host as a 1-based

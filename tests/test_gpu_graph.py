@@ -1,0 +1,88 @@
+"""The graph-replayed transfer step (ereb_step_transfer) equals the same
+launches run eagerly, bit for bit, step after step: marker state and
+properties, SUM and node grids, both assembled systems.  Between replays the
+host does its own work (reads, a melt-composition scatter that relies on the
+sort bookkeeping of the replayed step) and an argument change forces a new
+capture."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ereb
+from ereb import synth
+
+pytestmark = pytest.mark.gpu
+
+START_TIME = 2.25e6 * 365.25 * 24 * 3600
+
+
+def _ctx(g, mat, mk, graphs):
+    ctx = ereb.Context(g, mat, ereb.FP64)
+    ctx.set_graphs(graphs)
+    ctx.set_markers(mk)
+    vx, vy = synth.stream_velocity(g)
+    S = g.stag_shape
+    rng = np.random.default_rng(3)
+    grids = {"vx": vx, "vy": vy, "vxf": vx, "vyf": vy,
+             "wyx": synth.rotation_rate(g, vx, vy), "ETAP": np.full(S, 1e18),
+             "ETAPHI": np.full(S, 1e19), "gx": np.full(S, -1.0), "gy": np.full(S, -2.0),
+             "DT": 0.01 * rng.random(S), "tk2": 250 + 50 * rng.random(S),
+             "pr0": np.full(S, 1e5), "pf0": np.full(S, 9e4), "DSXX": 1e3 * rng.random(S),
+             "DSXY": 1e3 * rng.random(g.basic_shape)}
+    for n, a in grids.items():
+        ctx.set_grid(n, a)
+    return ctx
+
+
+def _state(ctx):
+    out = {n: ctx.get_marker(n) for n in ereb.MARKER_FIELDS}
+    for n in ereb.SUM_FIELDS + ["ETA", "RHO", "tk1", "RX", "KY", "XWS"]:
+        out[n] = ctx.get_grid(n)
+    for which in (0, 1):
+        for k, a in enumerate(ctx.csc(which)):
+            out[f"csc{which}.{k}"] = a
+    return out
+
+
+def test_graph_replay_equals_eager():
+    g, mat = synth.workload_geometry("adj41", mode=1)
+    mk = synth.make_markers(g, mat, seed=5)
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    dt = ereb.DT_LONGEST
+    A = _ctx(g, mat, mk, graphs=False)
+    B = _ctx(g, mat, mk, graphs=True)
+    try:
+        for ts in range(1, 8):
+            if ts == 5:
+                dt = 0.5 * dt  # new scalar: B captures again
+            for c in (A, B):
+                c.step_transfer(hrs, hrf, 1, dt, ts)
+            if ts == 3:  # reuses or redoes the sort: must match
+                for c in (A, B):
+                    c.step_melt_composition()
+            sa, sb = _state(A), _state(B)
+            for k in sa:
+                assert np.array_equal(sa[k], sb[k], equal_nan=True), (ts, k)
+        # the replayed steps also equal the separate resident calls
+        C = _ctx(g, mat, mk, graphs=False)
+        dt = ereb.DT_LONGEST
+        for ts in range(1, 8):
+            if ts == 5:
+                dt = 0.5 * dt
+            C.step_scatter(hrs, hrf, 1)
+            C.step_node_properties()
+            C.step_assemble_stokes_darcy(dt)
+            C.step_assemble_thermal(dt)
+            C.step_gather_stress()
+            C.step_gather_temperature(ts)
+            C.step_advect_rk4(dt, 1)
+            if ts == 3:
+                C.step_melt_composition()
+        sc = _state(C)
+        for k in sc:
+            assert np.array_equal(sc[k], sb[k], equal_nan=True), k
+        C.close()
+    finally:
+        A.close()
+        B.close()
