@@ -518,9 +518,8 @@ static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, 
     if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
     if ((rc = launch_assemble_stokes(c, dt))) return rc;
     if ((rc = launch_assemble_thermal(c, dt))) return rc;
-    if ((rc = launch_gather_stress(c))) return rc;
-    if ((rc = launch_gather_temperature(c, timestep))) return rc;
-    return launch_rk4(c, dt, (int)mode);
+    // the stress and temperature gathers and RK4 in one pass over the markers
+    return launch_gathers_rk4(c, dt, (int)mode, timestep);
 }
 
 static StepKey step_key(const ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,

@@ -240,25 +240,31 @@ __device__ __forceinline__ float self3(const double (&a)[3], int t) {
     return (float)(t == 1 ? a[0] : (t == 2 ? a[1] : a[2]));
 }
 
-__device__ __forceinline__ float rhocpfluidm_f(const ereb_materials& M, float T, int mode) {
+// FP32 forms of the temperature-dependent laws.  The branch conditions are
+// evaluated on the Float64 temperature against the Float64 breakpoints, as in
+// the Float64 form: rounding T to float first would move markers within half
+// a float spacing (1.5e-5 K at 273 K) of a breakpoint to the other branch.
+__device__ __forceinline__ float rhocpfluidm_f(const ereb_materials& M, double Td, int mode) {
     if (mode == 9) return (float)M.rhocpfluidm[0];
-    const float tf = (float)M.tmfluidphase, Lf = (float)M.Lf;
+    const double tf = M.tmfluidphase;
+    const float T = (float)Td, Lf = (float)M.Lf;
     const float rw = (float)M.rhoH2Of, ri = (float)M.rhoH2Ofi;
-    if (T < tf - 5.0f) return ri * (7.67f * T);
-    if (T < tf) return ri * (7.67f * T + 0.1f * Lf);
-    if (T < tf + 5.0f) return rw * (4200.0f + 0.1f * Lf);
-    if (T < 410.0f) return rw * 4200.0f;
+    if (Td < tf - 5.0) return ri * (7.67f * T);
+    if (Td < tf) return ri * (7.67f * T + 0.1f * Lf);
+    if (Td < tf + 5.0) return rw * (4200.0f + 0.1f * Lf);
+    if (Td < 410.0) return rw * 4200.0f;
     return rw * (-4.67e4f + 333.0f * T - 0.731f * (T * T) + 5.4e-4f * (T * T * T));
 }
 
-__device__ __forceinline__ float ksolidm_f(const ereb_materials& M, float T, int mode) {
-    return mode == 9 ? (float)M.ksolidm[0] : 0.73f + 1293.0f / (T + 77.0f);
+__device__ __forceinline__ float ksolidm_f(const ereb_materials& M, double Td, int mode) {
+    return mode == 9 ? (float)M.ksolidm[0] : 0.73f + 1293.0f / ((float)Td + 77.0f);
 }
 
-__device__ __forceinline__ float kfluidm_f(const ereb_materials& M, float T, int mode) {
+__device__ __forceinline__ float kfluidm_f(const ereb_materials& M, double Td, int mode) {
     if (mode == 9) return (float)M.kfluidm[0];
-    if (T < (float)M.tmfluidphase) return 0.465f + 488.0f / T;
-    if (T < 410.0f) return -0.581f + 6.34e-3f * T - 7.93e-6f * (T * T);
+    const float T = (float)Td;
+    if (Td < M.tmfluidphase) return 0.465f + 488.0f / T;
+    if (Td < 410.0) return -0.581f + 6.34e-3f * T - 7.93e-6f * (T * T);
     return -0.142f + 4.12e-3f * T - 5.01e-6f * (T * T);
 }
 
@@ -281,24 +287,25 @@ struct PropsF {
     float rhototal, rhocptotal, etatotal, hrtotal, ktotal, tkrhocp, etafkphi;
 };
 
-__device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t, float T,
+__device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t, double Td,
                                                  float phi, float XW, const double (&hrs)[3],
                                                  const double (&hrf)[3], int mode,
                                                  PropsF stored) {
+    const float T = (float)Td;
     PropsF p = stored;
     float etafluidcur;
     if (t < 3) {
         const float XD = 1.0f - XW;
         const float rhosolidm0 = ((float)M.MD + (float)M.MH2O * XW) /
                                  ((float)M.VDs * XD + (float)M.VWs * XW);
-        const float rhofluidm0 = T > (float)M.tmfluidphase ? (float)M.rhoH2Of : (float)M.rhoH2Ofi;
+        const float rhofluidm0 = Td > M.tmfluidphase ? (float)M.rhoH2Of : (float)M.rhoH2Ofi;
         p.rhototal = total_f(rhosolidm0, rhofluidm0, phi);
-        p.rhocptotal = total_f(self3(M.rhocpsolidm, t), rhocpfluidm_f(M, T, mode), phi);
-        const float es = T > (float)M.tmsolidphase ? self3(M.etasolidmm, t) : self3(M.etasolidm, t);
-        etafluidcur = T > (float)M.tmfluidphase ? self3(M.etafluidmm, t) : self3(M.etafluidm, t);
+        p.rhocptotal = total_f(self3(M.rhocpsolidm, t), rhocpfluidm_f(M, Td, mode), phi);
+        const float es = Td > M.tmsolidphase ? self3(M.etasolidmm, t) : self3(M.etasolidm, t);
+        etafluidcur = Td > M.tmfluidphase ? self3(M.etafluidmm, t) : self3(M.etafluidm, t);
         p.etatotal = fmaxf(fmaxf((float)M.etamin, es), etafluidcur);
         p.hrtotal = total_f(self3(hrs, t), self3(hrf, t), phi);
-        p.ktotal = ktotal_f(ksolidm_f(M, T, mode), kfluidm_f(M, T, mode), phi);
+        p.ktotal = ktotal_f(ksolidm_f(M, Td, mode), kfluidm_f(M, Td, mode), phi);
     } else {
         etafluidcur = (float)M.etafluidm[2];
     }

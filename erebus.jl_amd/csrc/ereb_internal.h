@@ -173,6 +173,8 @@ int launch_gather_stress(ereb_ctx* c);
 // xb_out/yb_out (optional): fluid back-trace end points, for the adjoint
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out = nullptr,
                double* yb_out = nullptr);
+// stress gather, temperature gather (timestep) and RK4 in one kernel
+int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep);
 
 // ereb_assembly.hip
 int launch_assemble_stokes(ereb_ctx* c, double dt);

@@ -33,6 +33,35 @@ __device__ __forceinline__ void weights4_r(double a, double b, R* w) {
     w[3] = br * ar;
 }
 
+// update_marker_temperature! (E:4782-4801) for one marker: the new tkm
+template <class R>
+__device__ __forceinline__ double gather_T_one(const GeomK& g, double x, double y, double T,
+                                               const double* __restrict__ grid, bool first) {
+    const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
+    const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
+    R w[4];
+    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
+    const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
+    return first ? v : T + v;
+}
+
+// update_marker_stress! (E:4354-4388) for one marker: sxxm, sxym updated in
+// place; P grid with the clamps 2..Nx-1, 2..Ny-1 of E:4364-4367
+template <class R>
+__device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y, double& sxx,
+                                             double& sxy, const double* __restrict__ DSXX,
+                                             const double* __restrict__ DSXY) {
+    const int jp = jl_cell((x - g.xp[0]) * g.idx, 2, g.Nx - 1);
+    const int ip = jl_cell((y - g.yp[0]) * g.idy, 2, g.Ny - 1);
+    const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
+    const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
+    R wp[4], wb[4];
+    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, wp);
+    weights4_r<R>((x - g.x[jb - 1]) / g.dx, (y - g.y[ib - 1]) / g.dy, wb);
+    sxx += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
+    sxy += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
+}
+
 template <class R>
 __global__ __launch_bounds__(256) void k_gather_T(GeomK g, const double* __restrict__ xm,
                                                   const double* __restrict__ ym,
@@ -41,16 +70,7 @@ __global__ __launch_bounds__(256) void k_gather_T(GeomK g, const double* __restr
                                                   int first) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    const double x = xm[m], y = ym[m];
-    const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
-    const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
-    R w[4];
-    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
-    const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
-    if (first)
-        tkm[m] = v;
-    else
-        tkm[m] += v;
+    tkm[m] = gather_T_one<R>(g, xm[m], ym[m], first ? 0.0 : tkm[m], grid, first != 0);
 }
 
 template <class R>
@@ -62,17 +82,10 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
                                                   const double* __restrict__ DSXY, int64_t M) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    const double x = xm[m], y = ym[m];
-    // P grid with the clamps 2..Nx-1, 2..Ny-1 of E:4364-4367
-    const int jp = jl_cell((x - g.xp[0]) * g.idx, 2, g.Nx - 1);
-    const int ip = jl_cell((y - g.yp[0]) * g.idy, 2, g.Ny - 1);
-    const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
-    const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
-    R wp[4], wb[4];
-    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, wp);
-    weights4_r<R>((x - g.x[jb - 1]) / g.dx, (y - g.y[ib - 1]) / g.dy, wb);
-    sxxm[m] += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
-    sxym[m] += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
+    double sxx = sxxm[m], sxy = sxym[m];
+    gather_S_one<R>(g, xm[m], ym[m], sxx, sxy, DSXX, DSXY);
+    sxxm[m] = sxx;
+    sxym[m] = sxy;
 }
 
 // RK4 region: a*b + c may be fused (the quotients above already make the
@@ -140,6 +153,10 @@ struct RK4Args {
     const double* phim;
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
     double *xb_out, *yb_out;  // optional: fluid back-trace end points (adjoint recording)
+    // fused gathers (k_rk4<1 or 2>): DSXX, DSXY and the temperature grid
+    // (tk2 at timestep 1, else DT)
+    const double *DSXX, *DSXY, *tgrid;
+    int first;
 };
 
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
@@ -187,12 +204,27 @@ __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, in
     weights4((x - g.xp[jp - 1]) * g.idx, (y - g.yp[ip - 1]) * g.idy, w);
 }
 
+// GATHER = 0: RK4 alone.  GATHER = 1 (Float64) or 2 (FP32 interpolation, the
+// mixed form): the stress gather, then the temperature gather, then RK4 on
+// the gathered values, as the three separate kernels would compute them
+// (the gathers' arithmetic is defined outside the fused-FP region), with
+// each marker's state read and written once.
+template <int GATHER>
 __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
     const GeomK& g = A.g;
     const double dt = A.dt;
     const double xmm = A.xm[m], ymm = A.ym[m];
+    double s = A.sxxm[m], t = A.sxym[m];
+    double T = GATHER && A.first ? 0.0 : A.tkm[m];
+    if (GATHER == 1) {
+        gather_S_one<double>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
+        T = gather_T_one<double>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+    } else if (GATHER == 2) {
+        gather_S_one<float>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
+        T = gather_T_one<float>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+    }
     double xr = xmm, yr = ymm;
     double w[4];
     // solid temperature at the start position (P grid) and rotation rate
@@ -207,7 +239,6 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
-    const double s = A.sxxm[m], t = A.sxym[m];
     double ct, st, s2, c2;  // one shared argument reduction per angle
     sincos(th, &st, &ct);
     sincos(2.0 * th, &s2, &c2);
@@ -227,7 +258,7 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     // fluid-solid temperature correction E:5243-5252
     const double d = tkf0 - tks0;
     const int tt = min(max(A.tm[m], 1), 3);  // set validates 1..3
-    const double T = A.tkm[m], ph = A.phim[m];
+    const double ph = A.phim[m];
     const double rcs = sel3(A.M.rhocpsolidm, tt);
     const double rcf = rhocpfluidm(A.M, T, A.mode);
     A.tkm[m] = ((1.0 - ph) * T * rcs + ph * (T + d) * rcf) / ((1.0 - ph) * rcs + ph * rcf);
@@ -455,6 +486,8 @@ static RK4Args rk4_args(ereb_ctx* c, double dt, int mode) {
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
     A.xb_out = A.yb_out = nullptr;
+    A.DSXX = A.DSXY = A.tgrid = nullptr;
+    A.first = 0;
     return A;
 }
 
@@ -467,7 +500,31 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
-    hipLaunchKernelGGL(k_rk4, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_rk4<0>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+                       A);
+    EREB_HIP(c, hipGetLastError());
+    prof_end(c, EREB_K_RK4, ev);
+    return EREB_OK;
+}
+
+int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return EREB_OK;
+    if (c->Nx < 4 || c->Ny < 4)
+        return fail(c, EREB_ERR_ARG, "update_marker_stress! needs Nx, Ny >= 4 (P clamps 2..N-1)");
+    RK4Args A = rk4_args(c, dt, mode);
+    A.DSXX = gptr(c, EREB_G_DSXX);
+    A.DSXY = gptr(c, EREB_G_DSXY);
+    A.first = timestep == 1 ? 1 : 0;
+    A.tgrid = gptr(c, A.first ? EREB_G_TK2 : EREB_G_DT);
+    ++c->pos_gen;
+    hipEvent_t ev;
+    prof_begin(c, EREB_K_RK4, &ev);
+    const dim3 grid((unsigned)((M + 255) / 256));
+    if (c->precision == EREB_MIXED)
+        hipLaunchKernelGGL(k_rk4<2>, grid, dim3(256), 0, c->stream, A);
+    else
+        hipLaunchKernelGGL(k_rk4<1>, grid, dim3(256), 0, c->stream, A);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;

@@ -297,7 +297,7 @@ __device__ __forceinline__ PropsF props_r(const ereb_materials& M, int t, const 
                                           const double (&hrs)[3], const double (&hrf)[3],
                                           int mode) {
     PropsF st{r.st[0], r.st[1], r.st[2], r.st[3], r.st[4], 0.0f, 0.0f};
-    return marker_props_f(M, t, (float)r.T, (float)r.phi, r.XW, hrs, hrf, mode, st);
+    return marker_props_f(M, t, r.T, (float)r.phi, r.XW, hrs, hrf, mode, st);
 }
 
 template <class R>

@@ -203,7 +203,8 @@ int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
  * ereb_step_scatter, ereb_step_node_properties,
  * ereb_step_assemble_stokes_darcy, ereb_step_assemble_thermal,
  * ereb_step_gather_stress, ereb_step_gather_temperature(timestep),
- * ereb_step_advect_rk4 (E:6419-7053 without the host solves).  After the
+ * ereb_step_advect_rk4 (E:6419-7053 without the host solves); the two
+ * gathers and RK4 run as one kernel (kernel class EREB_K_RK4).  After the
  * first call the step's launches are captured once into a HIP graph and
  * replayed; the graph is captured again when an argument, the marker count,
  * the precision form or a device buffer changes.  Results are those of the

@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 START_TIME = 2.25e6 * 365.25 * 24 * 3600
 
 
-def _ctx(g, mat, mk, graphs):
-    ctx = ereb.Context(g, mat, ereb.FP64)
+def _ctx(g, mat, mk, graphs, precision=ereb.FP64):
+    ctx = ereb.Context(g, mat, precision)
     ctx.set_graphs(graphs)
     ctx.set_markers(mk)
     vx, vy = synth.stream_velocity(g)
@@ -45,13 +45,16 @@ def _state(ctx):
     return out
 
 
-def test_graph_replay_equals_eager():
+@pytest.mark.parametrize("precision", [ereb.FP64, ereb.MIXED])
+def test_graph_replay_equals_eager(precision):
+    """Also: the transfer step's fused gathers + RK4 kernel equals the
+    separate gather and RK4 kernels (the third context), in both forms."""
     g, mat = synth.workload_geometry("adj41", mode=1)
     mk = synth.make_markers(g, mat, seed=5)
     hrs, hrf = mat.radioactive_heating(START_TIME)
     dt = ereb.DT_LONGEST
-    A = _ctx(g, mat, mk, graphs=False)
-    B = _ctx(g, mat, mk, graphs=True)
+    A = _ctx(g, mat, mk, graphs=False, precision=precision)
+    B = _ctx(g, mat, mk, graphs=True, precision=precision)
     try:
         for ts in range(1, 8):
             if ts == 5:
@@ -65,7 +68,7 @@ def test_graph_replay_equals_eager():
             for k in sa:
                 assert np.array_equal(sa[k], sb[k], equal_nan=True), (ts, k)
         # the replayed steps also equal the separate resident calls
-        C = _ctx(g, mat, mk, graphs=False)
+        C = _ctx(g, mat, mk, graphs=False, precision=precision)
         dt = ereb.DT_LONGEST
         for ts in range(1, 8):
             if ts == 5:

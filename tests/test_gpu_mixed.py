@@ -31,6 +31,37 @@ def _check(errs, names, tol=TOL):
     assert not bad, f"fields above {tol} normwise: {bad}"
 
 
+def test_mixed_properties_at_temperature_breakpoints():
+    """Regression: the FP32 property laws must branch on the Float64
+    temperature.  Rounded to float first, a marker within half a float
+    spacing (1.5e-5 K at 273 K) of a breakpoint of compute_rhocpfluidm
+    (E:2272-2293: 268, 273, 278, 410 K), compute_kfluidm (273, 410 K) or the
+    fluid/solid phase switches (273, 1416 K) took the other branch: rhocp
+    off by up to 40 %.  Every marker property per marker within 1e-6."""
+    g, mat = synth.workload_geometry("cpu41", mode=1)
+    mk = synth.make_markers(g, mat, seed=4)
+    rng = np.random.default_rng(8)
+    rock = np.flatnonzero(mk["tm"] < 3)
+    br = np.array([268.0, 273.0, 278.0, 410.0, 1416.0])
+    off = np.array([-1e-5, -3e-6, -1e-9, 0.0, 1e-9, 3e-6, 1e-5])
+    T = mk["tkm"].copy()
+    T[rock] = (br[rng.integers(0, br.size, rock.size)] +
+               off[rng.integers(0, off.size, rock.size)])
+    mk["tkm"] = T
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    props_o, _ = Oracle(g, mat).marker_to_nodes(mk, hrs, hrf, 1)
+    with ereb.Context(g, mat, ereb.MIXED) as ctx:
+        ctx.set_markers(mk)
+        ctx.step_scatter(hrs, hrf, 1)
+        for n in precision.MARKER_PROPS:
+            got, ref = ctx.get_marker(n)[rock], props_o[n][rock]
+            # k_fluid is negative far above 410 K, so ktotal is NaN there in
+            # both (Julia's sqrt would throw): NaN must match NaN
+            both = np.isnan(got) & np.isnan(ref)
+            err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+            assert np.all(both | (err <= TOL)), (n, np.nanmax(err))
+
+
 @pytest.mark.parametrize("mode", [1, 9])
 def test_mixed_scatter_and_nodes_vs_oracle(mode):
     g, mat = synth.workload_geometry("cpu41", mode=mode)
