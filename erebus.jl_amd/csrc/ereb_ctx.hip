@@ -104,6 +104,8 @@ int ensure_scan(ereb_ctx* c, int64_t n) {
     hipFree(c->d_scan);
     c->d_scan = nullptr;
     EREB_HIP(c, hipMalloc(&c->d_scan, sizeof(int32_t) * (nb + 1)));
+    // [0, nb): block sums; [nb]: the last-block counter, which must start at 0
+    EREB_HIP(c, hipMemset(c->d_scan, 0, sizeof(int32_t) * (nb + 1)));
     c->scan_cap = nb;
     return EREB_OK;
 }
@@ -294,6 +296,8 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     c->ntiles = c->ntx * c->nty;
     c->nkeys = (int64_t)c->ntiles * NFINE;
     CK(hipMalloc(&c->d_fcount, sizeof(int32_t) * c->nkeys));
+    // the sort leaves the key counts at zero for the next sort
+    CK(hipMemset(c->d_fcount, 0, sizeof(int32_t) * c->nkeys));
     CK(hipMalloc(&c->d_fstart, sizeof(int32_t) * (c->nkeys + 1)));
     CK(hipMalloc(&c->d_fcursor, sizeof(int32_t) * c->nkeys));
     CK(hipMalloc(&c->d_big, sizeof(int32_t) * (c->nkeys + 1)));

@@ -86,6 +86,7 @@ struct ereb_ctx {
     int32_t* d_scan = nullptr;     // scan block sums
     int64_t scan_cap = 0;
     bool have_sort = false;
+    bool fcount_dirty = false;  // a sort stopped between counting and scanning
     // the sort is current while sort_gen == pos_gen; pos_gen advances
     // whenever positions or the marker count may have changed
     uint64_t pos_gen = 1, sort_gen = 0;
@@ -147,8 +148,10 @@ void prof_end(ereb_ctx* c, int kclass, hipEvent_t a);
 int prof_collect(ereb_ctx* c);
 
 // ---- kernel launchers ---------------------------------------------------
-// ereb_scan.hip: exclusive scan of n int32 into out[0..n], out[n] = total
-int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n);
+// ereb_scan.hip: exclusive scan of n int32 into out[0..n], out[n] = total;
+// optionally a copy of out[0..n) in out2 and in[] zeroed once read
+int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n,
+                   int32_t* out2 = nullptr, bool zero_in = false);
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 int fill_i32(ereb_ctx* c, int32_t* p, int32_t v, int64_t n);
 int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);

@@ -81,9 +81,11 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
                                                     const double* __restrict__ ym, int64_t M,
                                                     uint32_t* __restrict__ key,
                                                     int32_t* __restrict__ count,
-                                                    int32_t* __restrict__ flags) {
+                                                    int32_t* __restrict__ flags,
+                                                    int32_t* __restrict__ big) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = m < M;
+    if (m == 0) big[0] = 0;  // k_fine_order, which fills the list, runs after
     uint32_t k = 0;
     if (active) {
         bool bad;
@@ -92,12 +94,6 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
         key[m] = k;
     }
     wave_count(count, k, active);
-}
-
-__global__ __launch_bounds__(256) void k_copy_i32(const int32_t* __restrict__ a,
-                                                  int32_t* __restrict__ b, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = a[i];
 }
 
 __global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key, int64_t M,
@@ -201,20 +197,23 @@ int launch_fine_sort(ereb_ctx* c) {
     int rc;
     hipEvent_t ev;
     prof_begin(c, EREB_K_SORT, &ev);
-    if ((rc = fill_i32(c, c->d_fcount, 0, nk))) return rc;
-    if ((rc = fill_i32(c, c->d_flags, 0, 1))) return rc;
-    if ((rc = fill_i32(c, c->d_big, 0, 1))) return rc;
+    // the key counts are zero here (left so by the previous sort's scan)
+    // unless a sort stopped half-way; a domain error flag stays set until
+    // the host reads it (check_flags)
+    if (c->fcount_dirty && (rc = fill_i32(c, c->d_fcount, 0, nk))) return rc;
     if (M > 0) {
+        c->fcount_dirty = true;
         const unsigned nb = (unsigned)((M + 255) / 256);
         hipLaunchKernelGGL(k_fine_count, dim3(nb), dim3(256), 0, c->stream, geomk(c), c->nty,
                            mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), M, c->d_key, c->d_fcount,
-                           c->d_flags);
+                           c->d_flags, c->d_big);
     }
-    if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk))) return rc;
+    // key starts; the cursors of the placement start as a copy of them; the
+    // counts are cleared for the next sort
+    if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk, c->d_fcursor, true))) return rc;
+    c->fcount_dirty = false;
     if (M > 0) {
         const unsigned nkb = (unsigned)((nk + 255) / 256);
-        hipLaunchKernelGGL(k_copy_i32, dim3(nkb), dim3(256), 0, c->stream, c->d_fstart,
-                           c->d_fcursor, nk);
         const unsigned nb = (unsigned)((M + 255) / 256);
         hipLaunchKernelGGL(k_fine_place, dim3(nb), dim3(256), 0, c->stream, c->d_key, M,
                            c->d_fcursor, c->d_perm);
