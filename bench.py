@@ -289,6 +289,37 @@ def next_rows_leg(ctx, g, mat, hrs, hrf, reps: int = 5):
     return out
 
 
+def handoff_leg(ctx, reps: int = 5):
+    """Solver hand-off of the last assembled Stokes-Darcy system (host wall
+    time, device-to-host copies included): the pattern-change query, a full
+    fetch (1-based Int64 colptr/rowval, nzval, R) into pageable arrays, and
+    the values-only fetch (nzval, R) into pageable and page-locked arrays."""
+    which = ereb.SYS_STOKES_DARCY
+    n, nnz = ctx.csc_size(which)
+    full_b = 8 * (n + 1) + 16 * nnz + 8 * n
+    vals_b = 8 * nnz + 8 * n
+    page = (np.empty(n + 1, np.int64), np.empty(nnz, np.int64), np.empty(nnz),
+            np.empty(n))
+    pinned = (None, None, ctx.host_array(nnz, np.float64), ctx.host_array(n, np.float64))
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    q = timed(lambda: ctx.csc_pattern_changed(which))
+    full = timed(lambda: ctx.csc(which, out=page))
+    vp = timed(lambda: ctx.csc(which, pattern=False, out=(None, None) + page[2:]))
+    vq = timed(lambda: ctx.csc(which, pattern=False, out=pinned))
+    return {"system": f"Stokes-Darcy n={n:,} nnz={nnz:,}",
+            "pattern_query_ms": round(q, 4),
+            "full_fetch_pageable": {"ms": round(full, 3), "GB_per_s": round(full_b / full / 1e6, 2)},
+            "values_pageable": {"ms": round(vp, 3), "GB_per_s": round(vals_b / vp / 1e6, 2)},
+            "values_pinned": {"ms": round(vq, 3), "GB_per_s": round(vals_b / vq / 1e6, 2)}}
+
+
 BREAKS = (263.0, 268.0, 273.0, 278.0, 410.0)  # kinks and steps of T-dependent properties
 
 
@@ -431,6 +462,8 @@ def main():
         }
         log(f"[bench] mixed {msm:.3f} ms/step, worst nodal field {wn} {wnv:.2e}")
         ctxm.close()
+    log("[bench] solver hand-off")
+    res["handoff"] = handoff_leg(ctx64)
     if not a.no_next_rows:
         log("[bench] remaining transfer rows")
         res["next_rows"] = next_rows_leg(ctx64, g, mat, hrs, hrf)

@@ -81,10 +81,15 @@ int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell) {
     hipFree(s.rowval);
     hipFree(s.nzval);
     hipFree(s.rhs);
-    s.colcnt = s.colptr = s.rowval = nullptr;
-    s.pres = nullptr;
+    hipFree(s.pres_fetched);
+    hipFree(s.diff);
+    hipFree(s.stage);
+    s.colcnt = s.colptr = s.rowval = s.diff = nullptr;
+    s.pres = s.pres_fetched = nullptr;
     s.slotv = s.nzval = s.rhs = nullptr;
+    s.stage = nullptr;
     s.valid = false;
+    s.have_fetched = false;
     EREB_HIP(c, hipMalloc(&s.colcnt, sizeof(int32_t) * n));
     EREB_HIP(c, hipMalloc(&s.colptr, sizeof(int32_t) * (n + 1)));
     EREB_HIP(c, hipMalloc(&s.pres, sizeof(uint16_t) * n));
@@ -92,6 +97,9 @@ int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell) {
     EREB_HIP(c, hipMalloc(&s.rowval, sizeof(int32_t) * cap));
     EREB_HIP(c, hipMalloc(&s.nzval, sizeof(double) * cap));
     EREB_HIP(c, hipMalloc(&s.rhs, sizeof(double) * n));
+    EREB_HIP(c, hipMalloc(&s.pres_fetched, sizeof(uint16_t) * n));
+    EREB_HIP(c, hipMalloc(&s.diff, sizeof(int32_t)));
+    EREB_HIP(c, hipMalloc(&s.stage, sizeof(int64_t) * (std::max(n, cap) + 1)));
     s.n = n;
     s.cap = cap;
     s.ncell = ncell;
@@ -346,6 +354,9 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.nzval);
         hipFree(s.rhs);
         hipFree(s.tables);
+        hipFree(s.pres_fetched);
+        hipFree(s.diff);
+        hipFree(s.stage);
     }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -649,17 +660,38 @@ int ereb_step_assemble_thermal(ereb_ctx* c, double dt, int64_t* nnz) {
     return fetch_nnz(c, EREB_SYS_THERMAL, nnz);
 }
 
-int ereb_csc_size(const ereb_ctx* c, int which, int64_t* n, int64_t* nnz) {
-    if (!c || which < 0 || which > 1) return EREB_ERR_ARG;
+// 0-based int32 CSC indices to the 1-based Int64 of SparseMatrixCSC{Float64,Int64}
+__global__ void k_index_1based(const int32_t* __restrict__ a, int64_t* __restrict__ b,
+                               int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (int64_t)a[i] + 1;
+}
+
+// flag[0] = 1 when the two presence tables differ anywhere (every writer
+// stores the same value)
+__global__ void k_pres_diff(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
+                            int64_t n, int32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != b[i]) flag[0] = 1;
+}
+
+static unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+static int sys_nnz(ereb_ctx* c, const Sys& s, int64_t* nnz) {
+    int32_t v = 0;
+    EREB_HIP(c, hipMemcpyAsync(&v, s.colptr + s.n, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    *nnz = v;
+    return EREB_OK;
+}
+
+int ereb_csc_size(const ereb_ctx* cc, int which, int64_t* n, int64_t* nnz) {
+    if (!cc || which < 0 || which > 1) return EREB_ERR_ARG;
+    ereb_ctx* c = const_cast<ereb_ctx*>(cc);
     const Sys& s = c->sys[which];
     if (!s.valid) return EREB_ERR_STATE;
     if (n) *n = s.n;
-    if (nnz) {
-        int32_t v = 0;
-        if (hipMemcpy(&v, s.colptr + s.n, sizeof v, hipMemcpyDeviceToHost) != hipSuccess)
-            return EREB_ERR_HIP;
-        *nnz = v;
-    }
+    if (nnz) return sys_nnz(c, s, nnz);
     return EREB_OK;
 }
 
@@ -668,20 +700,70 @@ int ereb_csc_get(ereb_ctx* c, int which, int64_t* colptr, int64_t* rowval, doubl
     if (!c || which < 0 || which > 1) return EREB_ERR_ARG;
     Sys& s = c->sys[which];
     if (!s.valid) return fail(c, EREB_ERR_STATE, "no system assembled yet");
-    EREB_HIP(c, hipStreamSynchronize(c->stream));
-    std::vector<int32_t> cp((size_t)s.n + 1);
-    EREB_HIP(c, hipMemcpy(cp.data(), s.colptr, sizeof(int32_t) * (s.n + 1), hipMemcpyDeviceToHost));
-    int64_t nnz = cp[s.n];
-    if (colptr)
-        for (int64_t k = 0; k <= s.n; ++k) colptr[k] = (int64_t)cp[k] + 1;
+    int64_t nnz = 0;
+    int rc = sys_nnz(c, s, &nnz);
+    if (rc) return rc;
+    // stream order keeps the staging buffer's two uses apart
+    if (colptr) {
+        hipLaunchKernelGGL(k_index_1based, dim3(blocks256(s.n + 1)), dim3(256), 0, c->stream,
+                           (const int32_t*)s.colptr, s.stage, s.n + 1);
+        EREB_HIP(c, hipMemcpyAsync(colptr, s.stage, sizeof(int64_t) * (s.n + 1),
+                                   hipMemcpyDeviceToHost, c->stream));
+    }
     if (rowval && nnz > 0) {
-        std::vector<int32_t> rv((size_t)nnz);
-        EREB_HIP(c, hipMemcpy(rv.data(), s.rowval, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
-        for (int64_t k = 0; k < nnz; ++k) rowval[k] = (int64_t)rv[k] + 1;
+        hipLaunchKernelGGL(k_index_1based, dim3(blocks256(nnz)), dim3(256), 0, c->stream,
+                           (const int32_t*)s.rowval, s.stage, nnz);
+        EREB_HIP(c, hipMemcpyAsync(rowval, s.stage, sizeof(int64_t) * nnz,
+                                   hipMemcpyDeviceToHost, c->stream));
     }
     if (nzval && nnz > 0)
-        EREB_HIP(c, hipMemcpy(nzval, s.nzval, sizeof(double) * nnz, hipMemcpyDeviceToHost));
-    if (rhs) EREB_HIP(c, hipMemcpy(rhs, s.rhs, sizeof(double) * s.n, hipMemcpyDeviceToHost));
+        EREB_HIP(c, hipMemcpyAsync(nzval, s.nzval, sizeof(double) * nnz, hipMemcpyDeviceToHost,
+                                   c->stream));
+    if (rhs)
+        EREB_HIP(c, hipMemcpyAsync(rhs, s.rhs, sizeof(double) * s.n, hipMemcpyDeviceToHost,
+                                   c->stream));
+    if (colptr || rowval) {
+        EREB_HIP(c, hipMemcpyAsync(s.pres_fetched, s.pres, sizeof(uint16_t) * s.n,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        s.have_fetched = true;
+    }
+    EREB_HIP(c, hipGetLastError());
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return EREB_OK;
+}
+
+int ereb_csc_pattern_changed(ereb_ctx* c, int which, int* changed) {
+    if (!c || which < 0 || which > 1 || !changed) return EREB_ERR_ARG;
+    Sys& s = c->sys[which];
+    if (!s.valid) return fail(c, EREB_ERR_STATE, "no system assembled yet");
+    if (!s.have_fetched) {
+        *changed = 1;
+        return EREB_OK;
+    }
+    int rc = fill_i32(c, s.diff, 0, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pres_diff, dim3(blocks256(s.n)), dim3(256), 0, c->stream,
+                       (const uint16_t*)s.pres, (const uint16_t*)s.pres_fetched, s.n, s.diff);
+    int32_t f = 0;
+    EREB_HIP(c, hipMemcpyAsync(&f, s.diff, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipGetLastError());
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    *changed = f != 0;
+    return EREB_OK;
+}
+
+int ereb_host_alloc(ereb_ctx* c, int64_t bytes, void** out) {
+    if (!c || !out || bytes < 0) return EREB_ERR_ARG;
+    *out = nullptr;
+    if (bytes == 0) return EREB_OK;
+    EREB_HIP(c, hipSetDevice(c->device));
+    EREB_HIP(c, hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+    return EREB_OK;
+}
+
+int ereb_host_free(ereb_ctx* c, void* p) {
+    if (!c) return EREB_ERR_ARG;
+    if (p) EREB_HIP(c, hipHostFree(p));
     return EREB_OK;
 }
 

@@ -33,6 +33,12 @@ struct Sys {
     double* nzval = nullptr;     // cap
     double* rhs = nullptr;       // n
     bool valid = false;
+    // solver hand-off: the pattern (pres) as of the last host fetch of
+    // colptr/rowval, a comparison flag, and 1-based Int64 index staging
+    uint16_t* pres_fetched = nullptr;  // n
+    int32_t* diff = nullptr;           // 1
+    int64_t* stage = nullptr;          // cap + 1
+    bool have_fetched = false;
 };
 
 struct ProfPair {

@@ -99,6 +99,9 @@ PROTOTYPES = {
     "ereb_step_assemble_thermal": [_P, _D, _P],
     "ereb_csc_get": [_P, _I, _P, _P, _P, _P],
     "ereb_csc_size": [_P, _I, _P, _P],
+    "ereb_csc_pattern_changed": [_P, _I, _P],
+    "ereb_host_alloc": [_P, _I64, _P],
+    "ereb_host_free": [_P, _P],
     "ereb_sort_get": [_P, _P, _P, _P],
     "ereb_marker_to_nodes": [_P] + [_P] * 23 + [_P, _P, _I64, _P, _I64],
     "ereb_compute_basic_node_properties": [_P] + [_P] * 16,
@@ -185,6 +188,7 @@ class Context:
     def __init__(self, geom: Geometry, mat: Materials, precision: int = FP64,
                  device: int = 0):
         self.lib = load()
+        self._pinned = []
         self.geom = geom
         self.mat = mat
         self._cg = geom.to_c()
@@ -202,6 +206,9 @@ class Context:
 
     def close(self):
         if self.h:
+            for p in self._pinned:
+                self.lib.ereb_host_free(self.h, p)
+            self._pinned = []
             self.lib.ereb_ctx_destroy(self.h)
             self.h = None
 
@@ -363,20 +370,52 @@ class Context:
             _ptr(mnum), ctypes.addressof(n)), "step_replenish_markers")
         return n.value, mdis, mnum
 
-    def csc(self, which: int):
-        """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64
-        indices as in SparseMatrixCSC{Float64,Int64}."""
+    def csc_size(self, which: int):
+        """(n, nnz) of the last assembled system."""
         n = ctypes.c_int64()
         nnz = ctypes.c_int64()
         self.check(self.lib.ereb_csc_size(self.h, which, ctypes.addressof(n),
                                           ctypes.addressof(nnz)), "csc_size")
-        cp = np.empty(n.value + 1, dtype=np.int64)
-        rv = np.empty(nnz.value, dtype=np.int64)
-        nz = np.empty(nnz.value, dtype=np.float64)
-        rhs = np.empty(n.value, dtype=np.float64)
-        self.check(self.lib.ereb_csc_get(self.h, which, _ptr(cp), _ptr(rv), _ptr(nz),
-                                         _ptr(rhs)), "csc_get")
-        return cp, rv, nz, rhs
+        return n.value, nnz.value
+
+    def csc(self, which: int, pattern: bool = True, out=None):
+        """(colptr, rowval, nzval, rhs) of the last assembly, 1-based Int64
+        indices as in SparseMatrixCSC{Float64,Int64}.  pattern=False fetches
+        only nzval and rhs (colptr, rowval returned as None).  out: arrays
+        (colptr, rowval, nzval, rhs) to fill instead of new ones, e.g. from
+        host_array; entries may be None to skip that part."""
+        n, nnz = self.csc_size(which)
+        if out is None:
+            out = (np.empty(n + 1, dtype=np.int64) if pattern else None,
+                   np.empty(nnz, dtype=np.int64) if pattern else None,
+                   np.empty(nnz, dtype=np.float64), np.empty(n, dtype=np.float64))
+        want = [(np.int64, n + 1), (np.int64, nnz), (np.float64, nnz), (np.float64, n)]
+        for a, (dt, size) in zip(out, want):
+            if a is not None and (a.dtype != dt or a.size < size or not a.flags.c_contiguous):
+                raise ValueError(f"csc: output array needs {size} contiguous {np.dtype(dt)}")
+        self.check(self.lib.ereb_csc_get(self.h, which, *[_ptr(a) for a in out]), "csc_get")
+        return tuple(None if a is None else a[:size] for a, (_, size) in zip(out, want))
+
+    def csc_pattern_changed(self, which: int) -> bool:
+        """True when the last assembled pattern differs from the one last
+        fetched with colptr/rowval (or none was fetched)."""
+        ch = ctypes.c_int()
+        self.check(self.lib.ereb_csc_pattern_changed(self.h, which, ctypes.addressof(ch)),
+                   "csc_pattern_changed")
+        return bool(ch.value)
+
+    def host_array(self, n: int, dtype) -> np.ndarray:
+        """Page-locked host array of n elements (ereb_host_alloc); valid until
+        the context closes, which frees it."""
+        dt = np.dtype(dtype)
+        nbytes = int(n) * dt.itemsize
+        p = _P()
+        self.check(self.lib.ereb_host_alloc(self.h, nbytes, ctypes.byref(p)), "host_alloc")
+        if not p.value:
+            return np.empty(0, dtype=dt)
+        self._pinned.append(p.value)
+        return np.frombuffer((ctypes.c_char * nbytes).from_address(p.value), dtype=dt,
+                             count=int(n))
 
     def sort_order(self):
         n = self.marknum

@@ -224,6 +224,21 @@ int ereb_csc_get(ereb_ctx* ctx, int which, int64_t* colptr, int64_t* rowval,
                  double* nzval, double* rhs);
 int ereb_csc_size(const ereb_ctx* ctx, int which, int64_t* n, int64_t* nnz);
 
+/* Solver hand-off.  The reference gives L to LinearSolve's UMFPACK cache at
+ * every plastic iteration (`hydromech_cache.A = L`, E:6719-6749) and solves
+ * LT with `\` (E:6925).  UMFPACK keeps its symbolic analysis while the
+ * sparsity pattern is unchanged, and the caller then needs only nzval and
+ * the right-hand side: ereb_csc_get(ctx, which, NULL, NULL, nzval, rhs).
+ * *changed = 0 when the pattern of the last assembled system equals the one
+ * last fetched by ereb_csc_get with colptr or rowval non-NULL, 1 when it
+ * differs or none was fetched since the system's buffers were (re)sized. */
+int ereb_csc_pattern_changed(ereb_ctx* ctx, int which, int* changed);
+/* Page-locked host memory on the context's device for hand-off arrays
+ * (device-to-host copies into it run at the link's full rate); release it
+ * with ereb_host_free.  bytes == 0 gives NULL. */
+int ereb_host_alloc(ereb_ctx* ctx, int64_t bytes, void** out);
+int ereb_host_free(ereb_ctx* ctx, void* p);
+
 /* Sort order of the last ereb_step_scatter: perm[k] (0-based marker index)
  * of the k-th marker in sorted order, the fine key of every marker and the
  * number of keys.  The sort is a stable counting sort of marker indices by
