@@ -6,7 +6,7 @@ markers per cell = 2,822,400 markers, planetesimal R = 50 km in sticky air,
 stream-function velocity at Courant 0.5, dt = 1e11 s, property mode 1.
 A step is one timestep of the transfer path in the reference loop order
 (src/Erebus.jl:6376-7053) with every input resident in HBM:
-  tile sort + marker properties + scatter (E:6419-6517)
+  tile sort with the marker properties in its key pass, scatter (E:6419-6517)
   node normalisation + insulating BC (E:6522-6604)
   Stokes-Darcy assembly to CSC (E:6680) and thermal assembly (E:6922)
   stress gather (E:6980), temperature gather (E:7004), RK4 advection (E:7036)
@@ -51,13 +51,14 @@ METRIC = "marker-steps/s, 2.8M markers: FP64 vs mixed-precision; adjoint/forward
 HBM_PEAK_GBS = 8000.0
 START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 
-# algorithmic bytes of the fused property + scatter kernel: every input
-# marker vector read once (15 Float64 + tm as 4 bytes), the 7 property
-# vectors written once, the 30 SUM grids (Float64) written once
-SCATTER_MARKER_BYTES = 15 * 8 + 4 + 7 * 8
-# mixed form: the 6 state vectors (xm, ym, tkm, phim, sxxm, sxym) stay
-# Float64, the 9 property inputs and 7 property outputs are float
-SCATTER_MARKER_BYTES_MIXED = 6 * 8 + 4 + 9 * 4 + 7 * 4
+# algorithmic bytes of the scatter kernel (marker_to_*_nodes!): every marker
+# vector it reads once -- xm, ym and the 18 staged values (13 marker vectors
+# and the 7 properties that the sort's key pass evaluated), i.e. 20 Float64
+# vectors -- and the 30 SUM grids (Float64) written once
+SCATTER_MARKER_BYTES = 20 * 8
+# mixed form: the 5 state vectors (xm, ym, phim, sxxm, sxym) stay Float64,
+# the 8 other property inputs and the 7 properties are float
+SCATTER_MARKER_BYTES_MIXED = 5 * 8 + 15 * 4
 
 
 def log(*a):
@@ -220,7 +221,7 @@ def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
                 traffic = v["bytes_per_launch"]
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": f"{kernel_name} (marker properties + scatter, one launch per step)",
+            "kernel": f"{kernel_name} (marker->node scatter, one launch per step)",
             "algorithmic_bytes": sc_bytes}
 
 

@@ -295,8 +295,7 @@ static size_t grid_doubles(const ereb_ctx* c) {
 static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const double* hrs,
                             const double* hrf, double* xb, double* yb) {
     int rc;
-    if ((rc = launch_fine_sort(c))) return rc;
-    if ((rc = launch_scatter(c, hrs, hrf, mode, true))) return rc;
+    if ((rc = launch_scatter(c, hrs, hrf, mode))) return rc;
     if ((rc = launch_combine(c))) return rc;
     if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
     EREB_HIP(c, hipMemcpyAsync(gptr(c, EREB_G_TK2), gptr(c, EREB_G_TK1),
@@ -396,7 +395,7 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
         if ((rc = run_step(k, sg * every, grad && sg == nseg - 1))) return rc;
     }
     // loss on the final state
-    if ((rc = launch_fine_sort(c)) || (rc = launch_scatter(c, hrsolidm, hrfluidm, md, true)) ||
+    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, md)) ||
         (rc = launch_combine(c)) ||
         (rc = launch_node_properties(c, true, true, true, true, true)))
         return rc;

@@ -517,8 +517,7 @@ int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrflui
     int rc = valid_mode(c, mode);
     if (rc) return rc;
     if ((rc = reserve(c, std::max<int64_t>(c->marknum, 1)))) return rc;
-    if ((rc = launch_fine_sort(c))) return rc;
-    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, (int)mode, true))) return rc;
+    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, (int)mode))) return rc;
     return launch_combine(c);
 }
 

@@ -314,6 +314,67 @@ __device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t,
     return p;
 }
 
+// compute_marker_properties! (E:700-750) for marker m in storage order:
+// reads tm, tkm, phim, XWsolidm0 (rocks) or the stored rhototal, rhocptotal,
+// etatotal, hrtotal, ktotal (air, which keeps them) and writes the property
+// vectors.  R is the type of the property vectors: double in the Float64
+// form, float in the mixed form.
+template <class R>
+struct PropArgs {
+    const ereb_materials* __restrict__ M;
+    double hrs[3], hrf[3];
+    int mode;
+    const int32_t* __restrict__ tm;
+    const double *__restrict__ tkm, *__restrict__ phim;
+    const R* __restrict__ XW;
+    R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
+    __device__ void apply(int64_t m) const;
+};
+template <>
+__device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
+    const int t = min(max(tm[m], 1), 3);  // set validates 1..3
+    Props st{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double xw = 0.0;
+    if (t == 3)
+        st = Props{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0, 0.0};
+    else
+        xw = XW[m];
+    const Props p = marker_props(*M, t, tkm[m], phim[m], xw, hrs, hrf, mode, st);
+    if (t < 3) {
+        rhototal[m] = p.rhototal;
+        rhocptotal[m] = p.rhocptotal;
+        etatotal[m] = p.etatotal;
+        hrtotal[m] = p.hrtotal;
+        ktotal[m] = p.ktotal;
+    }
+    tkrhocp[m] = p.tkrhocp;
+    etafkphi[m] = p.etafkphi;
+}
+template <>
+__device__ __forceinline__ void PropArgs<float>::apply(int64_t m) const {
+    const int t = min(max(tm[m], 1), 3);
+    PropsF st{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float xw = 0.0f;
+    if (t == 3)
+        st = PropsF{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0f, 0.0f};
+    else
+        xw = XW[m];
+    const PropsF p = marker_props_f(*M, t, tkm[m], (float)phim[m], xw, hrs, hrf, mode, st);
+    if (t < 3) {
+        rhototal[m] = p.rhototal;
+        rhocptotal[m] = p.rhocptotal;
+        etatotal[m] = p.etatotal;
+        hrtotal[m] = p.hrtotal;
+        ktotal[m] = p.ktotal;
+    }
+    tkrhocp[m] = p.tkrhocp;
+    etafkphi[m] = p.etafkphi;
+}
+// no property evaluation
+struct NoProps {
+    __device__ void apply(int64_t) const {}
+};
+
 // Compensated FP32 accumulation (Knuth's TwoSum): s + c carries the sum of
 // the added values with an error of O(u^2) relative to their magnitudes.
 __device__ __forceinline__ void two_sum_add(float& s, float& c, float x) {

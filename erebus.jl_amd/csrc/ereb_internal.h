@@ -162,14 +162,21 @@ int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n,
 int fill_i32(ereb_ctx* c, int32_t* p, int32_t v, int64_t n);
 int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);
 
-// ereb_sort.hip: stable counting sort of marker indices by fine key
-int launch_fine_sort(ereb_ctx* c);
+// ereb_sort.hip: stable counting sort of marker indices by fine key; with
+// props, the pass that computes the keys also evaluates the marker
+// properties (compute_marker_properties!) with these heating rates and mode
+struct PropsReq {
+    const double* hrs;
+    const double* hrf;
+    int mode;
+};
+int launch_fine_sort(ereb_ctx* c, const PropsReq* props = nullptr);
 // runs the sort unless the last one still matches the resident positions
 int ensure_sort(ereb_ctx* c);
 
-// ereb_scatter.hip
-int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
-                   bool write_props);
+// ereb_scatter.hip: the sort with the marker properties (launch_fine_sort),
+// then the scatter of the properties
+int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode);
 int launch_combine(ereb_ctx* c);
 
 // ereb_nodes.hip

@@ -1,7 +1,10 @@
-// ereb_scatter.hip -- step 1 + 2 of the transfer pipeline: marker properties
-// (compute_marker_properties!, E:700-750) fused into the bilinear
+// ereb_scatter.hip -- step 2 of the transfer pipeline: the bilinear
 // marker->node scatter onto the basic, Vx, Vy and P grids
-// (marker_to_{basic,vx,vy,p}_nodes!, E:1605-1863, loop E:6419-6517).
+// (marker_to_{basic,vx,vy,p}_nodes!, E:1605-1863, loop E:6419-6517).  Step 1,
+// the marker properties (compute_marker_properties!, E:700-750), is evaluated
+// in storage order by the sort's key pass (k_fine_count in ereb_sort.hip),
+// where the property vectors are written with coalesced stores; the scatter
+// reads them through the sort order.
 //
 // Geometry of the reduction:
 //   - A marker's four stencils follow from two cells: basic (ib, jb) and
@@ -18,8 +21,7 @@
 // column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
 // contiguous run of sorted markers, split into chunks of at most CH markers
 // when it is longer:
-//   1. stage  : one thread per marker computes the 7 marker properties
-//               (written back to HBM) and stages the 18 property values and
+//   1. stage  : one thread per marker stages the 18 property values and
 //               the fractions dxmj/dx, dymi/dy of the basic and P stencils in
 //               LDS.  Each thread's inputs for the next chunk are loaded into
 //               registers before the current chunk's tasks start, so the
@@ -68,15 +70,12 @@ enum {
 template <class R>
 struct ScatterArgs {
     GeomK g;
-    const ereb_materials* __restrict__ M;
-    double hrs[3], hrf[3];
-    int mode, write_props, nty;
+    int nty;
     const uint32_t* __restrict__ perm;
     const int32_t* __restrict__ fstart;
-    const double *xm, *ym, *tkm, *phim, *sxx, *sxy;
-    const int32_t* tm;
-    const R *XW, *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas, *alphaf;
-    R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
+    const double *xm, *ym, *phim, *sxx, *sxy;
+    const R *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas, *alphaf;
+    const R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
     double* part;
 };
 
@@ -141,7 +140,6 @@ struct ScatterLDS {
     R sf[NLF][CHP];
     R sfr[4][CHP];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
     R part[NPART];
-    ereb_materials mat;  // material table, read per marker with a per-lane type index
     double ax[4][WIN];   // tile window of the x, y, xp, yp axes
     int32_t fs[NFINE + 1];
 };
@@ -243,27 +241,22 @@ __device__ __forceinline__ void write_part(double* part, int tile, const NodeAcc
     for (int q = 0; q < QN; ++q) p[(size_t)q * 64] = acc.get(q);
 }
 
-// Raw inputs of one marker, loaded one chunk ahead of their use so that the
+// Inputs of one marker, loaded one chunk ahead of their use so that the
 // global-memory latency of the gather through perm overlaps the previous
-// chunk's task and combine phases.
+// chunk's task and combine phases.  The 7 marker properties were evaluated
+// in storage order by the sort's key pass (k_fine_count).
 template <class R>
 struct Raw {
-    double x, y, T, phi, sxy, sxx;
-    R XW, etavp, invggg, cohes, tens, frict, rhof, alphas, alphaf;
-    R st[5];  // stored rhototal, rhocptotal, etatotal, hrtotal, ktotal (kept by air)
-    int32_t tm;
-    uint32_t m;
+    double x, y, phi, sxy, sxx;
+    R etavp, invggg, cohes, tens, frict, rhof, alphas, alphaf;
+    R pr[7];  // rhototal, rhocptotal, etatotal, hrtotal, ktotal, tkrhocp, etafkphi
 };
 
 template <class R>
 __device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, uint32_t m, Raw<R>& r) {
-    r.m = m;
     r.x = A.xm[m];
     r.y = A.ym[m];
-    r.tm = A.tm[m];
-    r.T = A.tkm[m];
     r.phi = A.phim[m];
-    r.XW = A.XW[m];
     r.etavp = A.etavp[m];
     r.invggg = A.invggg[m];
     r.sxy = A.sxy[m];
@@ -274,32 +267,19 @@ __device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, uint32_t m, Ra
     r.alphas = A.alphas[m];
     r.alphaf = A.alphaf[m];
     r.sxx = A.sxx[m];
-    r.st[0] = A.rhototal[m];
-    r.st[1] = A.rhocptotal[m];
-    r.st[2] = A.etatotal[m];
-    r.st[3] = A.hrtotal[m];
-    r.st[4] = A.ktotal[m];
+    r.pr[0] = A.rhototal[m];
+    r.pr[1] = A.rhocptotal[m];
+    r.pr[2] = A.etatotal[m];
+    r.pr[3] = A.hrtotal[m];
+    r.pr[4] = A.ktotal[m];
+    r.pr[5] = A.tkrhocp[m];
+    r.pr[6] = A.etafkphi[m];
 }
 
-// properties of the marker (written back to HBM), stencil fractions and the
-// 18 staged values into LDS slot t
+// stencil fractions and the 18 staged values of a marker into LDS slot t.
 // (j0, i0): 0-based first basic column and row of the tile.  Every marker of
 // the tile has jb - 1, ib - 1 in [0, TILE) and jp - 1, ip - 1 in [0, TILE]
 // relative to them (its fine key was derived from these cells).
-// Float64 properties (bit-exact with the reference) or FP32 properties
-__device__ __forceinline__ Props props_r(const ereb_materials& M, int t, const Raw<double>& r,
-                                         const double (&hrs)[3], const double (&hrf)[3],
-                                         int mode) {
-    Props st{r.st[0], r.st[1], r.st[2], r.st[3], r.st[4], 0.0, 0.0};
-    return marker_props(M, t, r.T, r.phi, r.XW, hrs, hrf, mode, st);
-}
-__device__ __forceinline__ PropsF props_r(const ereb_materials& M, int t, const Raw<float>& r,
-                                          const double (&hrs)[3], const double (&hrf)[3],
-                                          int mode) {
-    PropsF st{r.st[0], r.st[1], r.st[2], r.st[3], r.st[4], 0.0f, 0.0f};
-    return marker_props_f(M, t, r.T, (float)r.phi, r.XW, hrs, hrf, mode, st);
-}
-
 template <class R>
 __device__ __forceinline__ void stage_raw(const ScatterArgs<R>& A, ScatterLDS<R>& L, int t,
                                           const Raw<R>& r, int j0, int i0) {
@@ -309,38 +289,24 @@ __device__ __forceinline__ void stage_raw(const ScatterArgs<R>& A, ScatterLDS<R>
     L.sfr[1][t] = (R)((r.y - L.ax[1][cl.ib - 1 - i0]) / A.g.dy);
     L.sfr[2][t] = (R)((r.x - L.ax[2][cl.jp - 1 - j0]) / A.g.dx);
     L.sfr[3][t] = (R)((r.y - L.ax[3][cl.ip - 1 - i0]) / A.g.dy);
-    const int tt = min(max(r.tm, 1), 3);  // set validates 1..3
-    const auto p = props_r(L.mat, tt, r, A.hrs, A.hrf, A.mode);
-    if (A.write_props) {
-        const uint32_t m = r.m;
-        if (tt < 3) {
-            A.rhototal[m] = p.rhototal;
-            A.rhocptotal[m] = p.rhocptotal;
-            A.etatotal[m] = p.etatotal;
-            A.hrtotal[m] = p.hrtotal;
-            A.ktotal[m] = p.ktotal;
-        }
-        A.tkrhocp[m] = p.tkrhocp;
-        A.etafkphi[m] = p.etafkphi;
-    }
-    L.sf[L_ETATOTAL][t] = p.etatotal;
+    L.sf[L_ETATOTAL][t] = r.pr[2];
     L.sf[L_ETAVP][t] = r.etavp;
     L.sf[L_INVGGG][t] = r.invggg;
     L.sf[L_SXY][t] = (R)r.sxy;
     L.sf[L_COHES][t] = r.cohes;
     L.sf[L_TENS][t] = r.tens;
     L.sf[L_FRICT][t] = r.frict;
-    L.sf[L_RHOTOTAL][t] = p.rhototal;
+    L.sf[L_RHOTOTAL][t] = r.pr[0];
     L.sf[L_RHOF][t] = r.rhof;
-    L.sf[L_KTOTAL][t] = p.ktotal;
+    L.sf[L_KTOTAL][t] = r.pr[4];
     L.sf[L_PHIM][t] = (R)r.phi;
-    L.sf[L_ETAFKPHI][t] = p.etafkphi;
+    L.sf[L_ETAFKPHI][t] = r.pr[6];
     L.sf[L_SXX][t] = (R)r.sxx;
-    L.sf[L_RHOCP][t] = p.rhocptotal;
+    L.sf[L_RHOCP][t] = r.pr[1];
     L.sf[L_ALPHAS][t] = r.alphas;
     L.sf[L_ALPHAF][t] = r.alphaf;
-    L.sf[L_HR][t] = p.hrtotal;
-    L.sf[L_TKRHOCP][t] = p.tkrhocp;
+    L.sf[L_HR][t] = r.pr[3];
+    L.sf[L_TKRHOCP][t] = r.pr[5];
 }
 
 // Chunks of a tile: basic column k, markers [c0, cend) with cend - c0 <= CH.
@@ -377,9 +343,6 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
     const int r = lane & 7, c = lane >> 3;
     const int j0 = (tile / A.nty) * TILE, i0 = (tile % A.nty) * TILE;
     for (int i = t; i <= NFINE; i += NTHR) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
-    constexpr int NMAT = (int)(sizeof(ereb_materials) / sizeof(double));
-    static_assert(sizeof(ereb_materials) % sizeof(double) == 0, "materials are doubles");
-    if (t < NMAT) ((double*)&L.mat)[t] = ((const double*)A.M)[t];
     if (t < 4 * WIN) {
         const int a = t / WIN, k = t % WIN;
         const double* axis = a == 0 ? A.g.x : a == 1 ? A.g.y : a == 2 ? A.g.xp : A.g.yp;
@@ -468,29 +431,18 @@ static const int kSumGrid[NSUM] = {
 int sum_grid_field(int s) { return kSumGrid[s]; }
 
 template <class R>
-static int launch_scatter_t(ereb_ctx* c, const double* hrs, const double* hrf, int mode,
-                            bool write_props) {
-    auto P = [&](int f) { return (R*)c->d_m[f]; };
+static int launch_scatter_t(ereb_ctx* c) {
+    auto P = [&](int f) { return (const R*)c->d_m[f]; };
     ScatterArgs<R> A;
     A.g = geomk(c);
-    A.M = c->d_mat;
-    for (int k = 0; k < 3; ++k) {
-        A.hrs[k] = hrs[k];
-        A.hrf[k] = hrf[k];
-    }
-    A.mode = mode;
-    A.write_props = write_props ? 1 : 0;
     A.nty = c->nty;
     A.perm = c->d_perm;
     A.fstart = c->d_fstart;
     A.xm = mptr(c, EREB_M_XM);
     A.ym = mptr(c, EREB_M_YM);
-    A.tkm = mptr(c, EREB_M_TKM);
     A.phim = mptr(c, EREB_M_PHIM);
     A.sxx = mptr(c, EREB_M_SXXM);
     A.sxy = mptr(c, EREB_M_SXYM);
-    A.tm = tm_ptr(c);
-    A.XW = P(EREB_M_XWSOLIDM0);
     A.etavp = P(EREB_M_ETAVPM);
     A.invggg = P(EREB_M_INV_GGGTOTALM);
     A.frict = P(EREB_M_FRICTTOTALM);
@@ -515,11 +467,14 @@ static int launch_scatter_t(ereb_ctx* c, const double* hrs, const double* hrf, i
     return EREB_OK;
 }
 
-int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode, bool write_props) {
+int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode) {
+    // compute_marker_properties! (E:700-750) in the sort's pass over the markers
+    const PropsReq req{hrs, hrf, mode};
+    int rc = launch_fine_sort(c, &req);
+    if (rc) return rc;
     // the property vectors are float exactly when the context is in the mixed form
-    if (marker_is_f32(c->precision, EREB_M_ETATOTALM))
-        return launch_scatter_t<float>(c, hrs, hrf, mode, write_props);
-    return launch_scatter_t<double>(c, hrs, hrf, mode, write_props);
+    if (marker_is_f32(c->precision, EREB_M_ETATOTALM)) return launch_scatter_t<float>(c);
+    return launch_scatter_t<double>(c);
 }
 
 int launch_combine(ereb_ctx* c) {

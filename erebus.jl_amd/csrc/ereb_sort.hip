@@ -77,12 +77,15 @@ __device__ __forceinline__ uint32_t fine_key(const Cells& c, int nty, bool* bad)
     return (uint32_t)((tx * nty + ty) * NFINE + fj * FINE + fi);
 }
 
+// fine key and count of every marker; with PropArgs also the marker
+// properties, in the same pass over the markers in storage order
+template <class P>
 __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const double* __restrict__ xm,
                                                     const double* __restrict__ ym, int64_t M,
                                                     uint32_t* __restrict__ key,
                                                     int32_t* __restrict__ count,
                                                     int32_t* __restrict__ flags,
-                                                    int32_t* __restrict__ big) {
+                                                    int32_t* __restrict__ big, P props) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = m < M;
     if (m == 0) big[0] = 0;  // k_fine_order, which fills the list, runs after
@@ -94,6 +97,31 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
         key[m] = k;
     }
     wave_count(count, k, active);
+    if (active) props.apply(m);
+}
+
+template <class R>
+static PropArgs<R> prop_args(ereb_ctx* c, const PropsReq& q) {
+    PropArgs<R> P;
+    P.M = c->d_mat;
+    for (int k = 0; k < 3; ++k) {
+        P.hrs[k] = q.hrs[k];
+        P.hrf[k] = q.hrf[k];
+    }
+    P.mode = q.mode;
+    P.tm = tm_ptr(c);
+    P.tkm = mptr(c, EREB_M_TKM);
+    P.phim = mptr(c, EREB_M_PHIM);
+    auto f = [&](int field) { return (R*)c->d_m[field]; };
+    P.XW = f(EREB_M_XWSOLIDM0);
+    P.rhototal = f(EREB_M_RHOTOTALM);
+    P.rhocptotal = f(EREB_M_RHOCPTOTALM);
+    P.etatotal = f(EREB_M_ETATOTALM);
+    P.hrtotal = f(EREB_M_HRTOTALM);
+    P.ktotal = f(EREB_M_KTOTALM);
+    P.tkrhocp = f(EREB_M_TKM_RHOCPTOTALM);
+    P.etafkphi = f(EREB_M_ETAFLUIDCUR_INV_KPHIM);
+    return P;
 }
 
 __global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key, int64_t M,
@@ -191,7 +219,7 @@ __global__ __launch_bounds__(1024) void k_fine_order_big(const int32_t* __restri
     }
 }
 
-int launch_fine_sort(ereb_ctx* c) {
+int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
     const int64_t M = c->marknum;
     const int64_t nk = c->nkeys;
     int rc;
@@ -204,9 +232,21 @@ int launch_fine_sort(ereb_ctx* c) {
     if (M > 0) {
         c->fcount_dirty = true;
         const unsigned nb = (unsigned)((M + 255) / 256);
-        hipLaunchKernelGGL(k_fine_count, dim3(nb), dim3(256), 0, c->stream, geomk(c), c->nty,
-                           mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), M, c->d_key, c->d_fcount,
-                           c->d_flags, c->d_big);
+        const GeomK g = geomk(c);
+        const double *xm = mptr(c, EREB_M_XM), *ym = mptr(c, EREB_M_YM);
+        // the property vectors are float exactly when the context is in the mixed form
+        if (!props)
+            hipLaunchKernelGGL(k_fine_count<NoProps>, dim3(nb), dim3(256), 0, c->stream, g,
+                               c->nty, xm, ym, M, c->d_key, c->d_fcount, c->d_flags, c->d_big,
+                               NoProps{});
+        else if (marker_is_f32(c->precision, EREB_M_ETATOTALM))
+            hipLaunchKernelGGL(k_fine_count<PropArgs<float>>, dim3(nb), dim3(256), 0, c->stream,
+                               g, c->nty, xm, ym, M, c->d_key, c->d_fcount, c->d_flags,
+                               c->d_big, prop_args<float>(c, *props));
+        else
+            hipLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256), 0,
+                               c->stream, g, c->nty, xm, ym, M, c->d_key, c->d_fcount,
+                               c->d_flags, c->d_big, prop_args<double>(c, *props));
     }
     // key starts; the cursors of the placement start as a copy of them; the
     // counts are cleared for the next sort
