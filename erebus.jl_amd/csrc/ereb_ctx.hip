@@ -112,8 +112,6 @@ int ensure_scan(ereb_ctx* c, int64_t n) {
     hipFree(c->d_scan);
     c->d_scan = nullptr;
     EREB_HIP(c, hipMalloc(&c->d_scan, sizeof(int32_t) * (nb + 1)));
-    // [0, nb): block sums; [nb]: the last-block counter, which must start at 0
-    EREB_HIP(c, hipMemset(c->d_scan, 0, sizeof(int32_t) * (nb + 1)));
     c->scan_cap = nb;
     return EREB_OK;
 }

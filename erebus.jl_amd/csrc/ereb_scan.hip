@@ -39,15 +39,10 @@ __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, 
     return r;
 }
 
-// Block sums; the last block to finish (counted on done[0], which it resets
-// to zero for the next scan) then scans the nb block sums in place and
-// writes the total to out[n].  No block waits for another.
+// sum of each block's SCAN_B inputs
 __global__ __launch_bounds__(SCAN_T) void k_scan_blocksum(const int32_t* __restrict__ in,
-                                                          int64_t n, int32_t* __restrict__ bs,
-                                                          int64_t nb, int32_t* __restrict__ out,
-                                                          int32_t* __restrict__ done) {
+                                                          int64_t n, int32_t* __restrict__ bs) {
     __shared__ int32_t sh[SCAN_T / 64 + 1];
-    __shared__ int32_t last;
     int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_PER;
     int32_t s = 0;
 #pragma unroll
@@ -55,36 +50,24 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocksum(const int32_t* __restr
         if (base + k < n) s += in[base + k];
     int32_t tot;
     block_exclusive_scan(s, sh, &tot);
-    if (threadIdx.x == 0) {
-        bs[blockIdx.x] = tot;
-        __threadfence();  // the sum is visible before this block is counted
-        last = atomicAdd(done, 1) == (int32_t)nb - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // every block's sum is visible to the last block
-    int32_t carry = 0;
-    for (int64_t b0 = 0; b0 < nb; b0 += SCAN_T) {
-        const int64_t i = b0 + threadIdx.x;
-        const int32_t v = i < nb ? __atomic_load_n(&bs[i], __ATOMIC_RELAXED) : 0;
-        int32_t t2;
-        const int32_t e = block_exclusive_scan(v, sh, &t2);
-        if (i < nb) bs[i] = carry + e;
-        carry += t2;
-    }
-    if (threadIdx.x == 0) {
-        out[n] = carry;
-        *done = 0;
-    }
+    if (threadIdx.x == 0) bs[blockIdx.x] = tot;
 }
 
-// out[i] = exclusive prefix of in; optionally also out2[i] = out[i], and
-// in[i] = 0 once read (for inputs that must start from zero next time)
+// out[i] = exclusive prefix of in, the block's offset being the sum of the
+// block sums before it (each block adds them up itself); the last block also
+// writes the total to out[n].  Optionally also out2[i] = out[i], and in[i] = 0
+// once read (for inputs that must start from zero next time).
 __global__ __launch_bounds__(SCAN_T) void k_scan_apply(int32_t* __restrict__ in, int64_t n,
                                                        const int32_t* __restrict__ bs,
                                                        int32_t* __restrict__ out,
                                                        int32_t* __restrict__ out2, int zero_in) {
     __shared__ int32_t sh[SCAN_T / 64 + 1];
+    int32_t off = 0, tot;
+    for (int64_t b0 = 0; b0 < blockIdx.x; b0 += SCAN_T) {
+        const int64_t b = b0 + threadIdx.x;
+        block_exclusive_scan(b < blockIdx.x ? bs[b] : 0, sh, &tot);
+        off += tot;
+    }
     int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_PER;
     int32_t v[SCAN_PER];
     int32_t s = 0;
@@ -93,8 +76,8 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(int32_t* __restrict__ in,
         v[k] = base + k < n ? in[base + k] : 0;
         s += v[k];
     }
-    int32_t tot;
-    int32_t e = block_exclusive_scan(s, sh, &tot) + bs[blockIdx.x];
+    int32_t e = block_exclusive_scan(s, sh, &tot) + off;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) {
         if (base + k < n) {
@@ -112,9 +95,8 @@ int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n, int3
     if (rc) return rc;
     int64_t nb = (n + SCAN_B - 1) / SCAN_B;
     if (nb == 0) nb = 1;
-    int32_t* done = c->d_scan + c->scan_cap;  // last-block counter, zero between scans
     hipLaunchKernelGGL(k_scan_blocksum, dim3((unsigned)nb), dim3(SCAN_T), 0, c->stream, in, n,
-                       c->d_scan, nb, out, done);
+                       c->d_scan);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_T), 0, c->stream,
                        const_cast<int32_t*>(in), n, (const int32_t*)c->d_scan, out, out2,
                        zero_in ? 1 : 0);
