@@ -17,17 +17,20 @@
 //   - Markers arrive sorted by fine key (ereb_sort.hip): a tile's markers are
 //     contiguous and grouped by fine cell in ascending marker index.
 //
-// One workgroup of four wavefronts per tile.  The tile is walked one basic
+// One workgroup of eight wavefronts per tile.  The tile is walked one basic
 // column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
 // contiguous run of sorted markers, split into chunks of at most CH markers
 // when it is longer:
-//   1. stage  : one thread per marker stages the 18 property values and
-//               the fractions dxmj/dx, dymi/dy of the basic and P stencils in
-//               LDS.  Each thread's inputs for the next chunk are loaded into
-//               registers before the current chunk's tasks start, so the
-//               gather latency overlaps phases 2 and 3;
-//   2. tasks  : wavefront g owns the sums of grid g (basic, Vx, Vy, P).  A
-//               task is (fine cell, corner column cj): the sums over the fine
+//   1. stage  : two threads per marker (one per half of its inputs) stage
+//               the 18 property values and the fractions dxmj/dx, dymi/dy of
+//               the basic and P stencils in LDS.  Each thread's inputs for
+//               the next chunk are loaded into registers before the current
+//               chunk's tasks start, so the gather latency overlaps phases 2
+//               and 3;
+//   2. tasks  : wavefronts g and g + 4 own the first and second half of the
+//               sums of grid g (basic, Vx, Vy, P; the second half includes
+//               the weight sum).  A task is (fine cell, corner column cj):
+//               the sums over the fine
 //               cell's markers, in order, of property * weight for the
 //               corners (0, cj) and (1, cj), and of the weights;
 //   3. combine: lane (r, c) of a wavefront owns window node (r, c) of its
@@ -43,8 +46,8 @@
 
 namespace ereb {
 
-constexpr int NTHR = 256;    // four wavefronts, one per grid
-constexpr int CH = 256;      // markers per LDS chunk (one per thread in the load phase)
+constexpr int NTHR = 512;    // eight wavefronts, two per grid
+constexpr int CH = 256;      // markers per LDS chunk (two threads each in the load phase)
 constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
 constexpr int NCF = 2 * FINE;  // fine cells in one basic column of a tile
 constexpr int NLF = 18;        // property values staged per marker
@@ -243,70 +246,78 @@ __device__ __forceinline__ void write_part(double* part, int tile, const NodeAcc
 
 // Inputs of one marker, loaded one chunk ahead of their use so that the
 // global-memory latency of the gather through perm overlaps the previous
-// chunk's task and combine phases.  The 7 marker properties were evaluated
-// in storage order by the sort's key pass (k_fine_count).
-template <class R>
-struct Raw {
-    double x, y, phi, sxy, sxx;
-    R etavp, invggg, cohes, tens, frict, rhof, alphas, alphaf;
-    R pr[7];  // rhototal, rhocptotal, etatotal, hrtotal, ktotal, tkrhocp, etafkphi
-};
+// chunk's task and combine phases.  The two halves of the workgroup load
+// half of each marker's inputs each (part 0: xm, ym, phim, sxxm, sxym,
+// etavp, 1/G, cohesion, tension, friction; part 1: rhofluid, alphasolid,
+// alphafluid and the 7 marker properties, which the sort's key pass
+// evaluated in storage order).  Values are held as double (float inputs of
+// the mixed form convert exactly).
+constexpr int NRAW = 10;
 
 template <class R>
-__device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, uint32_t m, Raw<R>& r) {
-    r.x = A.xm[m];
-    r.y = A.ym[m];
-    r.phi = A.phim[m];
-    r.etavp = A.etavp[m];
-    r.invggg = A.invggg[m];
-    r.sxy = A.sxy[m];
-    r.cohes = A.cohes[m];
-    r.tens = A.tens[m];
-    r.frict = A.frict[m];
-    r.rhof = A.rhof[m];
-    r.alphas = A.alphas[m];
-    r.alphaf = A.alphaf[m];
-    r.sxx = A.sxx[m];
-    r.pr[0] = A.rhototal[m];
-    r.pr[1] = A.rhocptotal[m];
-    r.pr[2] = A.etatotal[m];
-    r.pr[3] = A.hrtotal[m];
-    r.pr[4] = A.ktotal[m];
-    r.pr[5] = A.tkrhocp[m];
-    r.pr[6] = A.etafkphi[m];
+__device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, int part, uint32_t m,
+                                         double (&v)[NRAW]) {
+    if (part == 0) {
+        v[0] = A.xm[m];
+        v[1] = A.ym[m];
+        v[2] = A.phim[m];
+        v[3] = A.sxx[m];
+        v[4] = A.sxy[m];
+        v[5] = A.etavp[m];
+        v[6] = A.invggg[m];
+        v[7] = A.cohes[m];
+        v[8] = A.tens[m];
+        v[9] = A.frict[m];
+    } else {
+        v[0] = A.rhof[m];
+        v[1] = A.alphas[m];
+        v[2] = A.alphaf[m];
+        v[3] = A.rhototal[m];
+        v[4] = A.rhocptotal[m];
+        v[5] = A.etatotal[m];
+        v[6] = A.hrtotal[m];
+        v[7] = A.ktotal[m];
+        v[8] = A.tkrhocp[m];
+        v[9] = A.etafkphi[m];
+    }
 }
 
-// stencil fractions and the 18 staged values of a marker into LDS slot t.
-// (j0, i0): 0-based first basic column and row of the tile.  Every marker of
-// the tile has jb - 1, ib - 1 in [0, TILE) and jp - 1, ip - 1 in [0, TILE]
-// relative to them (its fine key was derived from these cells).
+// one half of a marker's staged values into LDS slot t: part 0 also the
+// stencil fractions.  (j0, i0): 0-based first basic column and row of the
+// tile.  Every marker of the tile has jb - 1, ib - 1 in [0, TILE) and
+// jp - 1, ip - 1 in [0, TILE] relative to them (its fine key was derived
+// from these cells).
 template <class R>
-__device__ __forceinline__ void stage_raw(const ScatterArgs<R>& A, ScatterLDS<R>& L, int t,
-                                          const Raw<R>& r, int j0, int i0) {
-    const Cells cl = marker_cells(A.g, r.x, r.y);
-    // fractions in double from the double coordinates, then rounded to R
-    L.sfr[0][t] = (R)((r.x - L.ax[0][cl.jb - 1 - j0]) / A.g.dx);
-    L.sfr[1][t] = (R)((r.y - L.ax[1][cl.ib - 1 - i0]) / A.g.dy);
-    L.sfr[2][t] = (R)((r.x - L.ax[2][cl.jp - 1 - j0]) / A.g.dx);
-    L.sfr[3][t] = (R)((r.y - L.ax[3][cl.ip - 1 - i0]) / A.g.dy);
-    L.sf[L_ETATOTAL][t] = r.pr[2];
-    L.sf[L_ETAVP][t] = r.etavp;
-    L.sf[L_INVGGG][t] = r.invggg;
-    L.sf[L_SXY][t] = (R)r.sxy;
-    L.sf[L_COHES][t] = r.cohes;
-    L.sf[L_TENS][t] = r.tens;
-    L.sf[L_FRICT][t] = r.frict;
-    L.sf[L_RHOTOTAL][t] = r.pr[0];
-    L.sf[L_RHOF][t] = r.rhof;
-    L.sf[L_KTOTAL][t] = r.pr[4];
-    L.sf[L_PHIM][t] = (R)r.phi;
-    L.sf[L_ETAFKPHI][t] = r.pr[6];
-    L.sf[L_SXX][t] = (R)r.sxx;
-    L.sf[L_RHOCP][t] = r.pr[1];
-    L.sf[L_ALPHAS][t] = r.alphas;
-    L.sf[L_ALPHAF][t] = r.alphaf;
-    L.sf[L_HR][t] = r.pr[3];
-    L.sf[L_TKRHOCP][t] = r.pr[5];
+__device__ __forceinline__ void stage_raw(const ScatterArgs<R>& A, ScatterLDS<R>& L, int part,
+                                          int t, const double (&v)[NRAW], int j0, int i0) {
+    if (part == 0) {
+        const double x = v[0], y = v[1];
+        const Cells cl = marker_cells(A.g, x, y);
+        // fractions in double from the double coordinates, then rounded to R
+        L.sfr[0][t] = (R)((x - L.ax[0][cl.jb - 1 - j0]) / A.g.dx);
+        L.sfr[1][t] = (R)((y - L.ax[1][cl.ib - 1 - i0]) / A.g.dy);
+        L.sfr[2][t] = (R)((x - L.ax[2][cl.jp - 1 - j0]) / A.g.dx);
+        L.sfr[3][t] = (R)((y - L.ax[3][cl.ip - 1 - i0]) / A.g.dy);
+        L.sf[L_PHIM][t] = (R)v[2];
+        L.sf[L_SXX][t] = (R)v[3];
+        L.sf[L_SXY][t] = (R)v[4];
+        L.sf[L_ETAVP][t] = (R)v[5];
+        L.sf[L_INVGGG][t] = (R)v[6];
+        L.sf[L_COHES][t] = (R)v[7];
+        L.sf[L_TENS][t] = (R)v[8];
+        L.sf[L_FRICT][t] = (R)v[9];
+    } else {
+        L.sf[L_RHOF][t] = (R)v[0];
+        L.sf[L_ALPHAS][t] = (R)v[1];
+        L.sf[L_ALPHAF][t] = (R)v[2];
+        L.sf[L_RHOTOTAL][t] = (R)v[3];
+        L.sf[L_RHOCP][t] = (R)v[4];
+        L.sf[L_ETATOTAL][t] = (R)v[5];
+        L.sf[L_HR][t] = (R)v[6];
+        L.sf[L_KTOTAL][t] = (R)v[7];
+        L.sf[L_TKRHOCP][t] = (R)v[8];
+        L.sf[L_ETAFKPHI][t] = (R)v[9];
+    }
 }
 
 // Chunks of a tile: basic column k, markers [c0, cend) with cend - c0 <= CH.
@@ -340,6 +351,7 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
     __shared__ ScatterLDS<R> L;
     const int tile = blockIdx.x;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int part = wave >> 2, ts = t % CH;  // load half, marker slot
     const int r = lane & 7, c = lane >> 3;
     const int j0 = (tile / A.nty) * TILE, i0 = (tile % A.nty) * TILE;
     for (int i = t; i <= NFINE; i += NTHR) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
@@ -356,42 +368,55 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
 
     Chunk cur = chunk_from(L, 0, __builtin_amdgcn_readfirstlane(L.fs[0]));
     Chunk nx = chunk_next(L, cur);
-    Raw<R> raw;
-    if (cur.k < TILE && t < cur.cend - cur.c0) load_raw(A, chunk_perm(A, cur, t), raw);
-    uint32_t pnx = chunk_perm(A, nx, t);
+    double raw[NRAW];
+    if (cur.k < TILE && ts < cur.cend - cur.c0) load_raw(A, part, chunk_perm(A, cur, ts), raw);
+    uint32_t pnx = chunk_perm(A, nx, ts);
     while (cur.k < TILE) {
         // ---- 1. stage the chunk's markers (loaded one chunk ahead) ---------
-        if (t < cur.cend - cur.c0) stage_raw(A, L, t, raw, j0, i0);
+        if (ts < cur.cend - cur.c0) stage_raw(A, L, part, ts, raw, j0, i0);
         __syncthreads();
         // prefetch: fields of the next chunk, perm of the one after
         const Chunk nx2 = chunk_next(L, nx);
-        if (nx.k < TILE && t < nx.cend - nx.c0) load_raw(A, pnx, raw);
-        pnx = chunk_perm(A, nx2, t);
-        // ---- 2. (fine cell, corner column) partials, wavefront g = grid g --
+        if (nx.k < TILE && ts < nx.cend - nx.c0) load_raw(A, part, pnx, raw);
+        pnx = chunk_perm(A, nx2, ts);
+        // ---- 2. (fine cell, corner column) partials: wavefronts g and g + 4
+        // own the first and second half of grid g's sums ---------------------
         const int f0 = cur.k * NCF;
         switch (wave) {
-            case 0: tasks<R, 0, 0, 8>(L, lane, f0, cur.c0, cur.cend); break;
-            case 1: tasks<R, 1, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
-            case 2: tasks<R, 2, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
-            default: tasks<R, 3, 0, 10>(L, lane, f0, cur.c0, cur.cend); break;
+            case 0: tasks<R, 0, 0, 4>(L, lane, f0, cur.c0, cur.cend); break;
+            case 1: tasks<R, 1, 0, 3>(L, lane, f0, cur.c0, cur.cend); break;
+            case 2: tasks<R, 2, 0, 3>(L, lane, f0, cur.c0, cur.cend); break;
+            case 3: tasks<R, 3, 0, 5>(L, lane, f0, cur.c0, cur.cend); break;
+            case 4: tasks<R, 0, 4, 4>(L, lane, f0, cur.c0, cur.cend); break;
+            case 5: tasks<R, 1, 3, 3>(L, lane, f0, cur.c0, cur.cend); break;
+            case 6: tasks<R, 2, 3, 3>(L, lane, f0, cur.c0, cur.cend); break;
+            default: tasks<R, 3, 5, 5>(L, lane, f0, cur.c0, cur.cend); break;
         }
         __syncthreads();
         // ---- 3. node-owned combine -----------------------------------------
         switch (wave) {
-            case 0: combine<R, 0, 0, 8>(L, acc, r, c, cur.k); break;
-            case 1: combine<R, 1, 0, 6>(L, acc, r, c, cur.k); break;
-            case 2: combine<R, 2, 0, 6>(L, acc, r, c, cur.k); break;
-            default: combine<R, 3, 0, 10>(L, acc, r, c, cur.k); break;
+            case 0: combine<R, 0, 0, 4>(L, acc, r, c, cur.k); break;
+            case 1: combine<R, 1, 0, 3>(L, acc, r, c, cur.k); break;
+            case 2: combine<R, 2, 0, 3>(L, acc, r, c, cur.k); break;
+            case 3: combine<R, 3, 0, 5>(L, acc, r, c, cur.k); break;
+            case 4: combine<R, 0, 4, 4>(L, acc, r, c, cur.k); break;
+            case 5: combine<R, 1, 3, 3>(L, acc, r, c, cur.k); break;
+            case 6: combine<R, 2, 3, 3>(L, acc, r, c, cur.k); break;
+            default: combine<R, 3, 5, 5>(L, acc, r, c, cur.k); break;
         }
         __syncthreads();
         cur = nx;
         nx = nx2;
     }
     switch (wave) {
-        case 0: write_part<R, 0, 0, 8>(A.part, tile, acc, lane); break;
-        case 1: write_part<R, 1, 0, 6>(A.part, tile, acc, lane); break;
-        case 2: write_part<R, 2, 0, 6>(A.part, tile, acc, lane); break;
-        default: write_part<R, 3, 0, 10>(A.part, tile, acc, lane); break;
+        case 0: write_part<R, 0, 0, 4>(A.part, tile, acc, lane); break;
+        case 1: write_part<R, 1, 0, 3>(A.part, tile, acc, lane); break;
+        case 2: write_part<R, 2, 0, 3>(A.part, tile, acc, lane); break;
+        case 3: write_part<R, 3, 0, 5>(A.part, tile, acc, lane); break;
+        case 4: write_part<R, 0, 4, 4>(A.part, tile, acc, lane); break;
+        case 5: write_part<R, 1, 3, 3>(A.part, tile, acc, lane); break;
+        case 6: write_part<R, 2, 3, 3>(A.part, tile, acc, lane); break;
+        default: write_part<R, 3, 5, 5>(A.part, tile, acc, lane); break;
     }
 }
 
