@@ -9,6 +9,7 @@ extrapolated weights are exercised.
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse
 
 from ereb.constants import Geometry, Materials
 
@@ -149,17 +150,22 @@ def abs_sums(g: Geometry, mk: dict, props: dict) -> list:
     allv = dict(mk)
     allv.update(props)
     out = []
+    M = x.size
     for gname in ("basic", "vx", "vy", "p"):
         ax, ay, lim, shp = spec[gname]
         i, j = _cells(x, y, ax[0], ay[0], g.dx, g.dy, *lim)
         A = np.abs((x - ax[j - 1]) / g.dx)
         B = np.abs((y - ay[i - 1]) / g.dy)
-        w = [np.abs(1 - B) * np.abs(1 - A), B * np.abs(1 - A), np.abs(1 - B) * A, B * A]
-        offs = [(0, 0), (1, 0), (0, 1), (1, 1)]
-        for fname in fields[gname]:
-            p = np.ones_like(x) if fname is None else np.abs(allv[fname])
-            s = np.zeros(shp, order="F")
-            for (di, dj), wk in zip(offs, w):
-                np.add.at(s, (i - 1 + di, j - 1 + dj), p * wk)
-            out.append(s)
+        # |weights| as a sparse (node x marker) matrix, 4 entries per marker
+        # (column-major node index), applied to all |property| columns at once
+        w = np.stack([np.abs(1 - B) * np.abs(1 - A), B * np.abs(1 - A), np.abs(1 - B) * A,
+                      B * A], axis=1)
+        node = np.stack([(j - 1 + dj) * shp[0] + (i - 1 + di)
+                         for di, dj in ((0, 0), (1, 0), (0, 1), (1, 1))], axis=1)
+        W = scipy.sparse.csc_matrix((w.ravel(), node.ravel(), np.arange(0, 4 * M + 1, 4)),
+                                    shape=(shp[0] * shp[1], M))
+        P = np.stack([np.ones_like(x) if f is None else np.abs(allv[f])
+                      for f in fields[gname]], axis=1)
+        S = W @ P
+        out.extend(S[:, k].reshape(shp, order="F") for k in range(P.shape[1]))
     return out
