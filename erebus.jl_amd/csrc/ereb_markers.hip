@@ -239,9 +239,12 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
-    double ct, st, s2, c2;  // one shared argument reduction per angle
+    // sin(2θ) and cos(2θ) from the double-angle identities (within a few ulp
+    // of the library values; one argument reduction and polynomial instead
+    // of two)
+    double ct, st;
     sincos(th, &st, &ct);
-    sincos(2.0 * th, &s2, &c2);
+    const double s2 = 2.0 * (st * ct), c2 = (ct - st) * (ct + st);
     A.sxxm[m] = s * (ct * ct) - s * (st * st) - t * s2;
     A.sxym[m] = s * s2 + t * c2;
     // solid RK4 step and fluid back-trace

@@ -17,19 +17,21 @@
 //   - Markers arrive sorted by fine key (ereb_sort.hip): a tile's markers are
 //     contiguous and grouped by fine cell in ascending marker index.
 //
-// One workgroup of eight wavefronts per tile.  The tile is walked one basic
+// One workgroup per tile (eight wavefronts in the Float64 form, four in the
+// mixed form; ScatterShape below).  The tile is walked one basic
 // column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
 // contiguous run of sorted markers, split into chunks of at most CH markers
 // when it is longer:
-//   1. stage  : two threads per marker (one per half of its inputs) stage
-//               the 18 property values and the fractions dxmj/dx, dymi/dy of
+//   1. stage  : two threads per marker (one per half of its inputs; one
+//               thread in the four-wavefront form) stage the 18 property values and the fractions dxmj/dx, dymi/dy of
 //               the basic and P stencils in LDS.  Each thread's inputs for
 //               the next chunk are loaded into registers before the current
 //               chunk's tasks start, so the gather latency overlaps phases 2
 //               and 3;
 //   2. tasks  : wavefronts g and g + 4 own the first and second half of the
 //               sums of grid g (basic, Vx, Vy, P; the second half includes
-//               the weight sum).  A task is (fine cell, corner column cj):
+//               the weight sum); with four wavefronts, wavefront g owns them
+//               all.  A task is (fine cell, corner column cj):
 //               the sums over the fine
 //               cell's markers, in order, of property * weight for the
 //               corners (0, cj) and (1, cj), and of the weights;
@@ -46,8 +48,7 @@
 
 namespace ereb {
 
-constexpr int NTHR = 512;    // eight wavefronts, two per grid
-constexpr int CH = 256;      // markers per LDS chunk (two threads each in the load phase)
+constexpr int CH = 256;      // markers per LDS chunk
 constexpr int CHP = CH + 1;  // padded stride: fields of one marker in different banks
 constexpr int NCF = 2 * FINE;  // fine cells in one basic column of a tile
 constexpr int NLF = 18;        // property values staged per marker
@@ -346,12 +347,23 @@ __device__ __forceinline__ uint32_t chunk_perm(const ScatterArgs<R>& A, const Ch
     return (c.k < TILE && t < c.cend - c.c0) ? A.perm[c.c0 + t] : 0u;
 }
 
+// Wavefronts per workgroup: 8 in the Float64 form (two per grid, each thread
+// loading half of a marker's inputs; 116 VGPRs, 4 waves per SIMD), 4 in the
+// mixed form (one per grid, each thread loading a whole marker; its smaller
+// LDS footprint already fits 4 workgroups per CU, and halving the work per
+// thread there measured 36% slower).
+template <class R> struct ScatterShape;
+template <> struct ScatterShape<double> { static constexpr int NW = 8; };
+template <> struct ScatterShape<float> { static constexpr int NW = 4; };
+
 template <class R>
-__global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
+__global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(ScatterArgs<R> A) {
+    constexpr int NW = ScatterShape<R>::NW, NTHR = NW * 64;
+    constexpr int NSET = NW == 8 ? 1 : 2;  // input halves per thread
     __shared__ ScatterLDS<R> L;
     const int tile = blockIdx.x;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int part = wave >> 2, ts = t % CH;  // load half, marker slot
+    const int part = NW == 8 ? wave >> 2 : 0, ts = t % CH;  // first input half, marker slot
     const int r = lane & 7, c = lane >> 3;
     const int j0 = (tile / A.nty) * TILE, i0 = (tile % A.nty) * TILE;
     for (int i = t; i <= NFINE; i += NTHR) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
@@ -368,21 +380,38 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
 
     Chunk cur = chunk_from(L, 0, __builtin_amdgcn_readfirstlane(L.fs[0]));
     Chunk nx = chunk_next(L, cur);
-    double raw[NRAW];
-    if (cur.k < TILE && ts < cur.cend - cur.c0) load_raw(A, part, chunk_perm(A, cur, ts), raw);
+    double raw[NSET][NRAW];
+    if (cur.k < TILE && ts < cur.cend - cur.c0) {
+        const uint32_t m = chunk_perm(A, cur, ts);
+#pragma unroll
+        for (int q = 0; q < NSET; ++q) load_raw(A, part + q, m, raw[q]);
+    }
     uint32_t pnx = chunk_perm(A, nx, ts);
     while (cur.k < TILE) {
         // ---- 1. stage the chunk's markers (loaded one chunk ahead) ---------
-        if (ts < cur.cend - cur.c0) stage_raw(A, L, part, ts, raw, j0, i0);
+        if (ts < cur.cend - cur.c0) {
+#pragma unroll
+            for (int q = 0; q < NSET; ++q) stage_raw(A, L, part + q, ts, raw[q], j0, i0);
+        }
         __syncthreads();
         // prefetch: fields of the next chunk, perm of the one after
         const Chunk nx2 = chunk_next(L, nx);
-        if (nx.k < TILE && ts < nx.cend - nx.c0) load_raw(A, part, pnx, raw);
+        if (nx.k < TILE && ts < nx.cend - nx.c0) {
+#pragma unroll
+            for (int q = 0; q < NSET; ++q) load_raw(A, part + q, pnx, raw[q]);
+        }
         pnx = chunk_perm(A, nx2, ts);
         // ---- 2. (fine cell, corner column) partials: wavefronts g and g + 4
         // own the first and second half of grid g's sums ---------------------
         const int f0 = cur.k * NCF;
-        switch (wave) {
+        if constexpr (NW == 4) {
+            switch (wave) {
+                case 0: tasks<R, 0, 0, 8>(L, lane, f0, cur.c0, cur.cend); break;
+                case 1: tasks<R, 1, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+                case 2: tasks<R, 2, 0, 6>(L, lane, f0, cur.c0, cur.cend); break;
+                default: tasks<R, 3, 0, 10>(L, lane, f0, cur.c0, cur.cend); break;
+            }
+        } else switch (wave) {
             case 0: tasks<R, 0, 0, 4>(L, lane, f0, cur.c0, cur.cend); break;
             case 1: tasks<R, 1, 0, 3>(L, lane, f0, cur.c0, cur.cend); break;
             case 2: tasks<R, 2, 0, 3>(L, lane, f0, cur.c0, cur.cend); break;
@@ -394,7 +423,14 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
         }
         __syncthreads();
         // ---- 3. node-owned combine -----------------------------------------
-        switch (wave) {
+        if constexpr (NW == 4) {
+            switch (wave) {
+                case 0: combine<R, 0, 0, 8>(L, acc, r, c, cur.k); break;
+                case 1: combine<R, 1, 0, 6>(L, acc, r, c, cur.k); break;
+                case 2: combine<R, 2, 0, 6>(L, acc, r, c, cur.k); break;
+                default: combine<R, 3, 0, 10>(L, acc, r, c, cur.k); break;
+            }
+        } else switch (wave) {
             case 0: combine<R, 0, 0, 4>(L, acc, r, c, cur.k); break;
             case 1: combine<R, 1, 0, 3>(L, acc, r, c, cur.k); break;
             case 2: combine<R, 2, 0, 3>(L, acc, r, c, cur.k); break;
@@ -407,6 +443,15 @@ __global__ __launch_bounds__(NTHR, 2) void k_scatter(ScatterArgs<R> A) {
         __syncthreads();
         cur = nx;
         nx = nx2;
+    }
+    if constexpr (NW == 4) {
+        switch (wave) {
+            case 0: write_part<R, 0, 0, 8>(A.part, tile, acc, lane); break;
+            case 1: write_part<R, 1, 0, 6>(A.part, tile, acc, lane); break;
+            case 2: write_part<R, 2, 0, 6>(A.part, tile, acc, lane); break;
+            default: write_part<R, 3, 0, 10>(A.part, tile, acc, lane); break;
+        }
+        return;
     }
     switch (wave) {
         case 0: write_part<R, 0, 0, 4>(A.part, tile, acc, lane); break;
@@ -486,7 +531,8 @@ static int launch_scatter_t(ereb_ctx* c) {
     A.part = c->d_part;
     hipEvent_t ev;
     prof_begin(c, EREB_K_SCATTER, &ev);
-    hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)c->ntiles), dim3(NTHR), 0, c->stream, A);
+    hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)c->ntiles), dim3(ScatterShape<R>::NW * 64), 0,
+                       c->stream, A);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_SCATTER, ev);
     return EREB_OK;
