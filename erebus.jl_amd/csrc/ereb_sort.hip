@@ -9,10 +9,12 @@
 // ascending key and, inside a key, ascending marker index -- exactly a
 // stable counting sort, which makes every later summation order canonical.
 //
-//   1. k_fine_count : key per marker, per-key counts (wavefront-aggregated
-//                     integer atomics: the counts do not depend on order)
+//   1. k_fine_count : key per marker, per-key counts (one integer atomic
+//                     per run of equal keys in a wavefront: the counts do
+//                     not depend on order); before a scatter also the
+//                     marker properties
 //   2. scan         : key starts
-//   3. k_fine_place : wavefront-aggregated slot reservation per key
+//   3. k_fine_place : slot reservation, one returning atomic per run
 //   4. k_fine_order : each key segment (about 9 markers) is sorted by marker
 //                     index in place (in registers up to 16 markers),
 //                     removing the arrival order of step 3;
@@ -22,34 +24,43 @@
 
 namespace ereb {
 
-// Lanes of a wavefront grouped by key: the lowest lane of each group is its
-// leader; every lane learns its leader, its rank in the group, and a leader
-// the group size.  Ballots and shuffles only, no memory traffic.
+// Lanes of a wavefront grouped into runs: maximal stretches of consecutive
+// active lanes with equal keys (markers in storage order come in runs of a
+// few per fine cell).  The first lane of a run is its leader; every lane
+// learns its leader and its rank in the run, a leader the run length.  A key
+// split over several runs gets one count per run, which the counts and the
+// slot reservations below allow (k_fine_order fixes the order within a key
+// afterwards).  Two ballots and a shuffle, no loop.
 struct KeyGroup {
     int leader, rank, count;  // count is non-zero on leaders only
 };
 __device__ __forceinline__ KeyGroup wave_group(uint32_t key, bool active) {
-    uint64_t todo = __ballot(active);
-    KeyGroup g{(int)__lane_id(), 0, 0};
     const int lane = __lane_id();
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lk = __shfl(key, leader);
-        const bool mine = active && key == lk;
-        const uint64_t m = __ballot(mine);
-        if (mine) {
-            g.leader = leader;
-            g.rank = (int)__popcll(m & lanemask_lt());
-        }
-        if (lane == leader) g.count = (int)__popcll(m);
-        todo &= ~m;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const uint64_t act = __ballot(active);
+    const bool prev_active = lane > 0 && ((act >> (lane - 1)) & 1ull);
+    const bool head = active && (!prev_active || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lanemask_lt() | (1ull << lane);
+    // my run's leader: the highest head at or below this lane
+    const uint64_t below = heads & le;
+    KeyGroup g{lane, 0, 0};
+    if (active) {
+        g.leader = 63 - __clzll((long long)below);
+        g.rank = lane - g.leader;
+    }
+    if (head) {
+        // the run ends at the next head or at the next inactive lane
+        const uint64_t stop = (heads | ~act) & ~le;
+        const int next = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
+        g.count = next - lane;
     }
     return g;
 }
 
 // For every active lane: reserve one slot in counters[key]; returns the slot
 // index.  All leaders issue their atomic together (one round trip per
-// wavefront); within a key the slots follow lane order.
+// wavefront); within a run the slots follow lane order.
 __device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key, bool active) {
     const KeyGroup g = wave_group(key, active);
     int32_t base = 0;
