@@ -284,8 +284,9 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
     CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
-    // grids, 256-byte aligned
-    size_t off = 0;
+    // grids, 256-byte aligned, after 32 guard elements (the RK4 stencils read
+    // one element before a grid at its first node, and discard it)
+    size_t off = 32;
     for (int f = 0; f < EREB_G_NFIELDS; ++f) {
         int64_t n = grid_is_basic(f) ? Nx * Ny : Nx1 * Ny1;
         c->gnumel[f] = n;

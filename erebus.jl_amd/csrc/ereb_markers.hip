@@ -94,7 +94,10 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
 
 // RK4 stage velocity on the Vx grid (E:5057-5091).  `vc` supplies the
 // centre value of the left-half correction: the reference reads vx there
-// also in the fluid loop (E:5173).
+// also in the fluid loop (E:5173).  Every load of the stencil (rows i, i+1
+// of columns j-1 .. j+2) is issued before any is used and the second-order
+// correction is selected, not branched to, so a stage costs one memory round
+// trip.
 __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
                                         const double* __restrict__ vc, double x, double y) {
     const int R = g.Ny1;
@@ -103,23 +106,34 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
     const double dxmj = x - g.x[j - 1], dymi = y - g.yp[i - 1];
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);  // v[i, j]
+    // column j-1 (read at j only where j = 1: it is unused there); column
+    // j+2 <= Nx+1 always exists
+    const double* pl = j > 1 ? p - R : p;
+    const double l0 = pl[0], l1 = pl[1];
+    const double a0 = p[0], a1 = p[1], b0 = p[R], b1 = p[R + 1];
+    const double c0 = p[2 * R], c1 = p[2 * R + 1];
+    const double vcij = *gptr_at(vc, i, j, R);
     const double fx = dxmj * idx, fy = dymi * idy;
-    double v13 = p[0] * (1.0 - fx) + (p[R] * dxmj) * idx;
-    double v24 = p[1] * (1.0 - fx) + (p[R + 1] * dxmj) * idx;
-    // second-order correction on columns j..j+2 (fx >= 0.5) or j-1..j+1; one
-    // code path for both, so lanes on either side do not diverge
+    double v13 = a0 * (1.0 - fx) + (b0 * dxmj) * idx;
+    double v24 = a1 * (1.0 - fx) + (b1 * dxmj) * idx;
+    // second-order correction on columns j..j+2 (fx >= 0.5) or j-1..j+1
     const bool right = fx >= 0.5;
-    if (right ? j < g.Nx - 1 : j > 1) {
-        const double t = fx - 0.5;
-        const double* q = right ? p : p - R;  // first column of the stencil
-        const double c13 = right ? p[R] : *gptr_at(vc, i, j, R);
-        v13 += 0.5 * (t * t) * (q[0] - 2.0 * c13 + q[2 * R]);
-        v24 += 0.5 * (t * t) * (q[1] - 2.0 * q[R + 1] + q[2 * R + 1]);
-    }
+    const bool apply = right ? j < g.Nx - 1 : j > 1;
+    const double t = fx - 0.5, h = 0.5 * (t * t);
+    const double q0 = right ? a0 : l0, q1 = right ? a1 : l1;    // first column
+    const double q20 = right ? c0 : b0, q21 = right ? c1 : b1;  // third column
+    const double c13 = right ? b0 : vcij, c24 = right ? b1 : a1;
+    const double n13 = v13 + h * (q0 - 2.0 * c13 + q20);
+    const double n24 = v24 + h * (q1 - 2.0 * c24 + q21);
+    v13 = apply ? n13 : v13;
+    v24 = apply ? n24 : v24;
     return v13 * (1.0 - fy) + (v24 * dymi) * idy;
 }
 
-// RK4 stage velocity on the Vy grid (E:5093-5127)
+// RK4 stage velocity on the Vy grid (E:5093-5127), rows i-1 .. i+2 of
+// columns j, j+1 loaded together as in rk_vx.  Row i-1 at i = 1 is unused:
+// the address is then the previous column's last row or, for the first
+// column, one of the guard elements the context keeps before its grids.
 __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v, double x,
                                         double y) {
     const int R = g.Ny1;
@@ -128,17 +142,21 @@ __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict
     const double dxmj = x - g.xp[j - 1], dymi = y - g.y[i - 1];
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);
+    const double am = p[-1], a0 = p[0], a1 = p[1], a2 = p[2];
+    const double bm = p[R - 1], b0 = p[R], b1 = p[R + 1], b2 = p[R + 2];
     const double fx = dxmj * idx, fy = dymi * idy;
-    double v12 = p[0] * (1.0 - fy) + (p[1] * dymi) * idy;
-    double v34 = p[R] * (1.0 - fy) + (p[R + 1] * dymi) * idy;
+    double v12 = a0 * (1.0 - fy) + (a1 * dymi) * idy;
+    double v34 = b0 * (1.0 - fy) + (b1 * dymi) * idy;
     // second-order correction on rows i..i+2 (fy >= 0.5) or i-1..i+1
     const bool down = fy >= 0.5;
-    if (down ? i < g.Ny - 1 : i > 1) {
-        const double t = fy - 0.5;
-        const double* q = down ? p : p - 1;  // first row of the stencil
-        v12 += 0.5 * (t * t) * (q[0] - 2.0 * q[1] + q[2]);
-        v34 += 0.5 * (t * t) * (q[R] - 2.0 * q[R + 1] + q[R + 2]);
-    }
+    const bool apply = down ? i < g.Ny - 1 : i > 1;
+    const double t = fy - 0.5, h = 0.5 * (t * t);
+    const double q0 = down ? a0 : am, q1 = down ? a1 : a0, q2 = down ? a2 : a1;
+    const double r0 = down ? b0 : bm, r1 = down ? b1 : b0, r2 = down ? b2 : b1;
+    const double n12 = v12 + h * (q0 - 2.0 * q1 + q2);
+    const double n34 = v34 + h * (r0 - 2.0 * r1 + r2);
+    v12 = apply ? n12 : v12;
+    v34 = apply ? n34 : v34;
     return v12 * (1.0 - fx) + (v34 * dxmj) * idx;
 }
 
