@@ -92,6 +92,16 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
 // results agree to rounding rather than bit for bit)
 #pragma clang fp contract(fast)
 
+// Node k (1-based) of an axis with first node a0 and spacing h, as
+// a0 + (k-1)*h in one fused operation instead of a load from the axis array,
+// which would put a dependent memory round trip before every stencil's
+// arithmetic.  It equals the axis value where the axis is exact (integer or
+// half-integer metres, as in the workloads) and is within an ulp elsewhere,
+// which the RK4 tolerance covers; the gathers above read the axis arrays.
+__device__ __forceinline__ double axis_at(double a0, double h, int k) {
+    return fma((double)(k - 1), h, a0);
+}
+
 // RK4 stage velocity on the Vx grid (E:5057-5091).  `vc` supplies the
 // centre value of the left-half correction: the reference reads vx there
 // also in the fluid loop (E:5173).  Every load of the stencil (rows i, i+1
@@ -101,9 +111,10 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
 __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
                                         const double* __restrict__ vc, double x, double y) {
     const int R = g.Ny1;
-    const int j = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
-    const int i = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
-    const double dxmj = x - g.x[j - 1], dymi = y - g.yp[i - 1];
+    const double x0 = g.x[0], y0 = g.yp[0];
+    const int j = jl_cell((x - x0) * g.idx, 1, g.Nx - 1);
+    const int i = jl_cell((y - y0) * g.idy, 1, g.Ny);
+    const double dxmj = x - axis_at(x0, g.dx, j), dymi = y - axis_at(y0, g.dy, i);
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);  // v[i, j]
     // column j-1 (read at j only where j = 1: it is unused there); column
@@ -137,9 +148,10 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
 __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v, double x,
                                         double y) {
     const int R = g.Ny1;
-    const int j = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
-    const int i = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
-    const double dxmj = x - g.xp[j - 1], dymi = y - g.y[i - 1];
+    const double x0 = g.xp[0], y0 = g.y[0];
+    const int j = jl_cell((x - x0) * g.idx, 1, g.Nx);
+    const int i = jl_cell((y - y0) * g.idy, 1, g.Ny - 1);
+    const double dxmj = x - axis_at(x0, g.dx, j), dymi = y - axis_at(y0, g.dy, i);
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);
     const double am = p[-1], a0 = p[0], a1 = p[1], a2 = p[2];
@@ -217,9 +229,10 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0
 // bilinear P-grid stencil of a position: cell (ip, jp), weights w
 __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, int& ip, int& jp,
                                           double* w) {
-    jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
-    ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
-    weights4((x - g.xp[jp - 1]) * g.idx, (y - g.yp[ip - 1]) * g.idy, w);
+    const double x0 = g.xp[0], y0 = g.yp[0];
+    jp = jl_cell((x - x0) * g.idx, 1, g.Nx);
+    ip = jl_cell((y - y0) * g.idy, 1, g.Ny);
+    weights4((x - axis_at(x0, g.dx, jp)) * g.idx, (y - axis_at(y0, g.dy, ip)) * g.idy, w);
 }
 
 // GATHER = 0: RK4 alone.  GATHER = 1 (Float64) or 2 (FP32 interpolation, the
@@ -247,13 +260,13 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     double w[4];
     // solid temperature at the start position (P grid) and rotation rate
     // (basic grid), E:5013-5040
-    int jp = jl_cell((xr - g.xp[0]) * g.idx, 1, g.Nx);
-    int ip = jl_cell((yr - g.yp[0]) * g.idy, 1, g.Ny);
-    weights4((xr - g.xp[jp - 1]) * g.idx, (yr - g.yp[ip - 1]) * g.idy, w);
+    int jp, ip;
+    p_stencil(g, xr, yr, ip, jp, w);
     const double tks0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
-    const int jb = jl_cell((xr - g.x[0]) * g.idx, 1, g.Nx - 1);
-    const int ib = jl_cell((yr - g.y[0]) * g.idy, 1, g.Ny - 1);
-    weights4((xr - g.x[jb - 1]) * g.idx, (yr - g.y[ib - 1]) * g.idy, w);
+    const double xb0 = g.x[0], yb0 = g.y[0];
+    const int jb = jl_cell((xr - xb0) * g.idx, 1, g.Nx - 1);
+    const int ib = jl_cell((yr - yb0) * g.idy, 1, g.Ny - 1);
+    weights4((xr - axis_at(xb0, g.dx, jb)) * g.idx, (yr - axis_at(yb0, g.dy, ib)) * g.idy, w);
     const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
