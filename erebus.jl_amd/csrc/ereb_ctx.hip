@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -39,6 +40,7 @@ GeomK geomk(const ereb_ctx* c) {
     g.y = c->d_axes + c->Nx;
     g.xp = c->d_axes + c->Nx + c->Ny;
     g.yp = c->d_axes + c->Nx + c->Ny + c->Nx1;
+    g.exact = c->axes_exact ? 1 : 0;
     return g;
 }
 
@@ -280,6 +282,14 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     ax.insert(ax.end(), g->y, g->y + Ny);
     ax.insert(ax.end(), g->xp, g->xp + Nx1);
     ax.insert(ax.end(), g->yp, g->yp + Ny1);
+    // are all nodes node 0 + k*spacing, rounded once?
+    auto exact_axis = [](const double* a, int64_t n, double h) {
+        for (int64_t k = 0; k < n; ++k)
+            if (std::fma((double)k, h, a[0]) != a[k]) return false;
+        return true;
+    };
+    c->axes_exact = exact_axis(g->x, Nx, g->dx) && exact_axis(g->y, Ny, g->dy) &&
+                    exact_axis(g->xp, Nx1, g->dx) && exact_axis(g->yp, Ny1, g->dy);
     CK(hipMalloc(&c->d_mat, sizeof(ereb_materials)));
     CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));

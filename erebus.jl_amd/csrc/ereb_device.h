@@ -21,7 +21,18 @@ struct GeomK {
     const double* y;          // basic y axis (Ny), = yvy
     const double* xp;         // P x axis (Nx1), = xvy
     const double* yp;         // P y axis (Ny1), = yvx
+    int exact;                // every node k of every axis equals fma(k, d, node 0)
 };
+
+// Node k (1-based) of an axis with spacing h: the first node plus (k-1)h,
+// one fused operation, when the context found every axis node equal to that
+// (integer or half-integer metres, as in the workloads), else the loaded
+// value.  Either way bit-exact; computing it spares a dependent load before
+// the stencil arithmetic.
+__device__ __forceinline__ double axis_node(const GeomK& g, const double* __restrict__ ax,
+                                            double h, int k) {
+    return g.exact ? fma((double)(k - 1), h, ax[0]) : ax[k - 1];
+}
 
 // unsafe_trunc(Int, t) + 1 clamped to [lo, hi] (E:1457-1468).  NaN and
 // |t| >= 2^63 give typemin(Int64) on x86-64, hence lo.

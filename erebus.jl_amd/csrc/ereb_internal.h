@@ -92,6 +92,7 @@ struct ereb_ctx {
     int32_t* d_scan = nullptr;     // scan block sums
     int64_t scan_cap = 0;
     bool have_sort = false;
+    bool axes_exact = false;  // node k of every axis == fma(k, spacing, node 0)
     bool fcount_dirty = false;  // a sort stopped between counting and scanning
     // the sort is current while sort_gen == pos_gen; pos_gen advances
     // whenever positions or the marker count may have changed

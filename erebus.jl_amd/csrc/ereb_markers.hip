@@ -40,7 +40,8 @@ __device__ __forceinline__ double gather_T_one(const GeomK& g, double x, double 
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     R w[4];
-    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
+    weights4_r<R>((x - axis_node(g, g.xp, g.dx, jp)) / g.dx,
+                  (y - axis_node(g, g.yp, g.dy, ip)) / g.dy, w);
     const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
     return first ? v : T + v;
 }
@@ -56,8 +57,10 @@ __device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y,
     const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
     const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
     R wp[4], wb[4];
-    weights4_r<R>((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, wp);
-    weights4_r<R>((x - g.x[jb - 1]) / g.dx, (y - g.y[ib - 1]) / g.dy, wb);
+    weights4_r<R>((x - axis_node(g, g.xp, g.dx, jp)) / g.dx,
+                  (y - axis_node(g, g.yp, g.dy, ip)) / g.dy, wp);
+    weights4_r<R>((x - axis_node(g, g.x, g.dx, jb)) / g.dx,
+                  (y - axis_node(g, g.y, g.dy, ib)) / g.dy, wb);
     sxx += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
     sxy += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
 }
