@@ -40,7 +40,6 @@ GeomK geomk(const ereb_ctx* c) {
     g.y = c->d_axes + c->Nx;
     g.xp = c->d_axes + c->Nx + c->Ny;
     g.yp = c->d_axes + c->Nx + c->Ny + c->Nx1;
-    g.exact = c->axes_exact ? 1 : 0;
     return g;
 }
 

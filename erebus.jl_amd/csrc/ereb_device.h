@@ -21,17 +21,17 @@ struct GeomK {
     const double* y;          // basic y axis (Ny), = yvy
     const double* xp;         // P x axis (Nx1), = xvy
     const double* yp;         // P y axis (Ny1), = yvx
-    int exact;                // every node k of every axis equals fma(k, d, node 0)
 };
 
-// Node k (1-based) of an axis with spacing h: the first node plus (k-1)h,
-// one fused operation, when the context found every axis node equal to that
-// (integer or half-integer metres, as in the workloads), else the loaded
-// value.  Either way bit-exact; computing it spares a dependent load before
-// the stencil arithmetic.
-__device__ __forceinline__ double axis_node(const GeomK& g, const double* __restrict__ ax,
-                                            double h, int k) {
-    return g.exact ? fma((double)(k - 1), h, ax[0]) : ax[k - 1];
+// Node k (1-based) of an axis with spacing h.  EXACT (the context found
+// every node of every axis equal to the first node plus k*h rounded once,
+// as for integer or half-integer metres): computed in one fused operation,
+// which spares a dependent load before the stencil arithmetic; otherwise
+// loaded.  Bit-identical either way.
+template <bool EXACT>
+__device__ __forceinline__ double axis_node(const double* __restrict__ ax, double h, int k) {
+    if constexpr (EXACT) return fma((double)(k - 1), h, ax[0]);
+    else return ax[k - 1];
 }
 
 // unsafe_trunc(Int, t) + 1 clamped to [lo, hi] (E:1457-1468).  NaN and

@@ -34,21 +34,21 @@ __device__ __forceinline__ void weights4_r(double a, double b, R* w) {
 }
 
 // update_marker_temperature! (E:4782-4801) for one marker: the new tkm
-template <class R>
+template <class R, bool EXACT>
 __device__ __forceinline__ double gather_T_one(const GeomK& g, double x, double y, double T,
                                                const double* __restrict__ grid, bool first) {
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     R w[4];
-    weights4_r<R>((x - axis_node(g, g.xp, g.dx, jp)) / g.dx,
-                  (y - axis_node(g, g.yp, g.dy, ip)) / g.dy, w);
+    weights4_r<R>((x - axis_node<EXACT>(g.xp, g.dx, jp)) / g.dx,
+                  (y - axis_node<EXACT>(g.yp, g.dy, ip)) / g.dy, w);
     const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
     return first ? v : T + v;
 }
 
 // update_marker_stress! (E:4354-4388) for one marker: sxxm, sxym updated in
 // place; P grid with the clamps 2..Nx-1, 2..Ny-1 of E:4364-4367
-template <class R>
+template <class R, bool EXACT>
 __device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y, double& sxx,
                                              double& sxy, const double* __restrict__ DSXX,
                                              const double* __restrict__ DSXY) {
@@ -57,10 +57,10 @@ __device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y,
     const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
     const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
     R wp[4], wb[4];
-    weights4_r<R>((x - axis_node(g, g.xp, g.dx, jp)) / g.dx,
-                  (y - axis_node(g, g.yp, g.dy, ip)) / g.dy, wp);
-    weights4_r<R>((x - axis_node(g, g.x, g.dx, jb)) / g.dx,
-                  (y - axis_node(g, g.y, g.dy, ib)) / g.dy, wb);
+    weights4_r<R>((x - axis_node<EXACT>(g.xp, g.dx, jp)) / g.dx,
+                  (y - axis_node<EXACT>(g.yp, g.dy, ip)) / g.dy, wp);
+    weights4_r<R>((x - axis_node<EXACT>(g.x, g.dx, jb)) / g.dx,
+                  (y - axis_node<EXACT>(g.y, g.dy, ib)) / g.dy, wb);
     sxx += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
     sxy += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
 }
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_gather_T(GeomK g, const double* __restr
                                                   int first) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    tkm[m] = gather_T_one<R>(g, xm[m], ym[m], first ? 0.0 : tkm[m], grid, first != 0);
+    tkm[m] = gather_T_one<R, false>(g, xm[m], ym[m], first ? 0.0 : tkm[m], grid, first != 0);
 }
 
 template <class R>
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     double sxx = sxxm[m], sxy = sxym[m];
-    gather_S_one<R>(g, xm[m], ym[m], sxx, sxy, DSXX, DSXY);
+    gather_S_one<R, false>(g, xm[m], ym[m], sxx, sxy, DSXX, DSXY);
     sxxm[m] = sxx;
     sxym[m] = sxy;
 }
@@ -95,29 +95,21 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
 // results agree to rounding rather than bit for bit)
 #pragma clang fp contract(fast)
 
-// Node k (1-based) of an axis with first node a0 and spacing h, as
-// a0 + (k-1)*h in one fused operation instead of a load from the axis array,
-// which would put a dependent memory round trip before every stencil's
-// arithmetic.  It equals the axis value where the axis is exact (integer or
-// half-integer metres, as in the workloads) and is within an ulp elsewhere,
-// which the RK4 tolerance covers; the gathers above read the axis arrays.
-__device__ __forceinline__ double axis_at(double a0, double h, int k) {
-    return fma((double)(k - 1), h, a0);
-}
-
 // RK4 stage velocity on the Vx grid (E:5057-5091).  `vc` supplies the
 // centre value of the left-half correction: the reference reads vx there
 // also in the fluid loop (E:5173).  Every load of the stencil (rows i, i+1
 // of columns j-1 .. j+2) is issued before any is used and the second-order
 // correction is selected, not branched to, so a stage costs one memory round
 // trip.
+template <bool EXACT>
 __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
                                         const double* __restrict__ vc, double x, double y) {
     const int R = g.Ny1;
     const double x0 = g.x[0], y0 = g.yp[0];
     const int j = jl_cell((x - x0) * g.idx, 1, g.Nx - 1);
     const int i = jl_cell((y - y0) * g.idy, 1, g.Ny);
-    const double dxmj = x - axis_at(x0, g.dx, j), dymi = y - axis_at(y0, g.dy, i);
+    const double dxmj = x - axis_node<EXACT>(g.x, g.dx, j);
+    const double dymi = y - axis_node<EXACT>(g.yp, g.dy, i);
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);  // v[i, j]
     // column j-1 (read at j only where j = 1: it is unused there); column
@@ -148,13 +140,15 @@ __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict
 // columns j, j+1 loaded together as in rk_vx.  Row i-1 at i = 1 is unused:
 // the address is then the previous column's last row or, for the first
 // column, one of the guard elements the context keeps before its grids.
+template <bool EXACT>
 __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v, double x,
                                         double y) {
     const int R = g.Ny1;
     const double x0 = g.xp[0], y0 = g.y[0];
     const int j = jl_cell((x - x0) * g.idx, 1, g.Nx);
     const int i = jl_cell((y - y0) * g.idy, 1, g.Ny - 1);
-    const double dxmj = x - axis_at(x0, g.dx, j), dymi = y - axis_at(y0, g.dy, i);
+    const double dxmj = x - axis_node<EXACT>(g.xp, g.dx, j);
+    const double dymi = y - axis_node<EXACT>(g.y, g.dy, i);
     const double idx = g.idx, idy = g.idy;
     const double* p = gptr_at(v, i, j, R);
     const double am = p[-1], a0 = p[0], a1 = p[1], a2 = p[2];
@@ -195,6 +189,7 @@ struct RK4Args {
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
 // to (xn, yn) and the fluid back-trace (E:5138-5226) from (xn, yn) to
 // (xb, yb).
+template <bool EXACT>
 __device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0, double& xn,
                                           double& yn, double& xb, double& yb) {
     const GeomK& g = A.g;
@@ -205,8 +200,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0
     double vxr[4], vyr[4];
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx(g, A.vx, A.vx, xr, yr);
-        vyr[rk] = 0.0 + rk_vy(g, A.vy, xr, yr);
+        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vx, A.vx, xr, yr);
+        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vy, xr, yr);
         if (rk < 3) {
             xr = xmm + dt * cr[rk] * vxr[rk];
             yr = ymm + dt * cr[rk] * vyr[rk];
@@ -218,8 +213,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0
     ymm = yr = yn;
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx(g, A.vxf, A.vx, xr, yr);
-        vyr[rk] = 0.0 + rk_vy(g, A.vyf, xr, yr);
+        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vxf, A.vx, xr, yr);
+        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vyf, xr, yr);
         if (rk < 3) {
             xr = xmm - dt * cr[rk] * vxr[rk];
             yr = ymm - dt * cr[rk] * vyr[rk];
@@ -230,12 +225,14 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0
 }
 
 // bilinear P-grid stencil of a position: cell (ip, jp), weights w
+template <bool EXACT>
 __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, int& ip, int& jp,
                                           double* w) {
     const double x0 = g.xp[0], y0 = g.yp[0];
     jp = jl_cell((x - x0) * g.idx, 1, g.Nx);
     ip = jl_cell((y - y0) * g.idy, 1, g.Ny);
-    weights4((x - axis_at(x0, g.dx, jp)) * g.idx, (y - axis_at(y0, g.dy, ip)) * g.idy, w);
+    weights4((x - axis_node<EXACT>(g.xp, g.dx, jp)) * g.idx,
+             (y - axis_node<EXACT>(g.yp, g.dy, ip)) * g.idy, w);
 }
 
 // GATHER = 0: RK4 alone.  GATHER = 1 (Float64) or 2 (FP32 interpolation, the
@@ -243,7 +240,7 @@ __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, in
 // the gathered values, as the three separate kernels would compute them
 // (the gathers' arithmetic is defined outside the fused-FP region), with
 // each marker's state read and written once.
-template <int GATHER>
+template <int GATHER, bool EXACT>
 __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
@@ -253,23 +250,24 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     double s = A.sxxm[m], t = A.sxym[m];
     double T = GATHER && A.first ? 0.0 : A.tkm[m];
     if (GATHER == 1) {
-        gather_S_one<double>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<double>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+        gather_S_one<double, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
+        T = gather_T_one<double, EXACT>(g, xmm, ymm, T, A.tgrid, A.first != 0);
     } else if (GATHER == 2) {
-        gather_S_one<float>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<float>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+        gather_S_one<float, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
+        T = gather_T_one<float, EXACT>(g, xmm, ymm, T, A.tgrid, A.first != 0);
     }
     double xr = xmm, yr = ymm;
     double w[4];
     // solid temperature at the start position (P grid) and rotation rate
     // (basic grid), E:5013-5040
     int jp, ip;
-    p_stencil(g, xr, yr, ip, jp, w);
+    p_stencil<EXACT>(g, xr, yr, ip, jp, w);
     const double tks0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
     const double xb0 = g.x[0], yb0 = g.y[0];
     const int jb = jl_cell((xr - xb0) * g.idx, 1, g.Nx - 1);
     const int ib = jl_cell((yr - yb0) * g.idy, 1, g.Ny - 1);
-    weights4((xr - axis_at(xb0, g.dx, jb)) * g.idx, (yr - axis_at(yb0, g.dy, ib)) * g.idy, w);
+    weights4((xr - axis_node<EXACT>(g.x, g.dx, jb)) * g.idx,
+             (yr - axis_node<EXACT>(g.y, g.dy, ib)) * g.idy, w);
     const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
@@ -283,14 +281,14 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     A.sxym[m] = s * s2 + t * c2;
     // solid RK4 step and fluid back-trace
     double xn, yn;
-    rk4_paths(A, xmm, ymm, xn, yn, xr, yr);
+    rk4_paths<EXACT>(A, xmm, ymm, xn, yn, xr, yr);
     A.xm[m] = xn;
     A.ym[m] = yn;
     if (A.xb_out) {
         A.xb_out[m] = xr;
         A.yb_out[m] = yr;
     }
-    p_stencil(g, xr, yr, ip, jp, w);
+    p_stencil<EXACT>(g, xr, yr, ip, jp, w);
     const double tkf0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
     // fluid-solid temperature correction E:5243-5252
     const double d = tkf0 - tks0;
@@ -394,9 +392,9 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
     const double x0 = A.xm[m], y0 = A.ym[m];
     int ips, jps, ipb, jpb;
     double ws[4], wb[4];
-    p_stencil(g, x0, y0, ips, jps, ws);
+    p_stencil<false>(g, x0, y0, ips, jps, ws);
     const double tks0 = gdot4(A.tk2, g.Ny1, ips, jps, ws);
-    p_stencil(g, xbs[m], ybs[m], ipb, jpb, wb);
+    p_stencil<false>(g, xbs[m], ybs[m], ipb, jpb, wb);
     const double tkf0 = gdot4(A.tk2, g.Ny1, ipb, jpb, wb);
     const double d = tkf0 - tks0;
     const int tt = min(max(A.tm[m], 1), 3);
@@ -537,8 +535,11 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
-    hipLaunchKernelGGL(k_rk4<0>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
-                       A);
+    const dim3 grid((unsigned)((M + 255) / 256));
+    if (c->axes_exact)
+        hipLaunchKernelGGL((k_rk4<0, true>), grid, dim3(256), 0, c->stream, A);
+    else
+        hipLaunchKernelGGL((k_rk4<0, false>), grid, dim3(256), 0, c->stream, A);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;
@@ -558,10 +559,18 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     const dim3 grid((unsigned)((M + 255) / 256));
-    if (c->precision == EREB_MIXED)
-        hipLaunchKernelGGL(k_rk4<2>, grid, dim3(256), 0, c->stream, A);
-    else
-        hipLaunchKernelGGL(k_rk4<1>, grid, dim3(256), 0, c->stream, A);
+    // node coordinates computed (exact axes) or loaded, chosen per kernel
+    if (c->precision == EREB_MIXED) {
+        if (c->axes_exact)
+            hipLaunchKernelGGL((k_rk4<2, true>), grid, dim3(256), 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((k_rk4<2, false>), grid, dim3(256), 0, c->stream, A);
+    } else {
+        if (c->axes_exact)
+            hipLaunchKernelGGL((k_rk4<1, true>), grid, dim3(256), 0, c->stream, A);
+        else
+            hipLaunchKernelGGL((k_rk4<1, false>), grid, dim3(256), 0, c->stream, A);
+    }
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;
