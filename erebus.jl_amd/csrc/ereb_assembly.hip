@@ -379,6 +379,11 @@ __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
 // blockDim.x * (most entries of a column) <= ASM_STAGE, checked on the host.
 constexpr int ASM_STAGE = 4096;
 
+// MAXR: most candidate rows of a column (16, or SLOT_R); every candidate's
+// presence bit and value are loaded before any is used (a value whose bit is
+// clear is read from its allocated slot and dropped), so a column costs one
+// memory round trip instead of one per candidate.
+template <int MAXR>
 __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                                                   const SlotTables* __restrict__ Tg,
                                                   const uint16_t* __restrict__ pres,
@@ -402,17 +407,32 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
         const int u = (int)(col % ne);
         const int64_t k = col / ne;
         const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
-        int32_t p = colptr[col] - base;
         const int nr = T.nrev[u];
-        for (int r = 0; r < nr; ++r) {
+        bool on[MAXR];
+        int32_t rw[MAXR];
+        double vv[MAXR];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            on[r] = false;
+            rw[r] = 0;
+            vv[r] = 0.0;
             const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
-            if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
-            const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
-            const int64_t cell = (int64_t)jr * Ny1 + ir;
-            if (!((pres[(size_t)e * ncell + cell] >> sl) & 1)) continue;
-            srow[p] = (int32_t)(cell * ne + e);
-            sval[p] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
-            ++p;
+            if (r < nr && ir >= 0 && ir < Ny1 && jr >= 0 && jr < Nx1) {
+                const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
+                const int64_t cell = (int64_t)jr * Ny1 + ir;
+                on[r] = (pres[(size_t)e * ncell + cell] >> sl) & 1;
+                vv[r] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
+                rw[r] = (int32_t)(cell * ne + e);
+            }
+        }
+        int32_t p = colptr[col] - base;
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            if (on[r]) {
+                srow[p] = rw[r];
+                sval[p] = vv[r];
+                ++p;
+            }
         }
     }
     __syncthreads();
@@ -574,11 +594,17 @@ static int assemble(ereb_ctx* c, double dt) {
     if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
     // a block's entries must fit the LDS stage: 256 columns of up to 16
     // entries, else 128 of up to 32 (SLOT_R)
-    const int fb = s.maxrev <= ASM_STAGE / 256 ? 256 : 128;
+    const bool narrow = s.maxrev <= ASM_STAGE / 256;
+    static_assert(ASM_STAGE / 256 == 16, "16 candidates per column in the 256-column blocks");
     static_assert(ASM_STAGE / 128 >= SLOT_R, "stage holds 128 columns of SLOT_R entries");
-    hipLaunchKernelGGL(k_asm_fill, dim3((unsigned)((n + fb - 1) / fb)), dim3(fb), 0, c->stream,
-                       c->Nx1, c->Ny1, T, s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval,
-                       s.nzval);
+    const int fb = narrow ? 256 : 128;
+    const dim3 fgrid((unsigned)((n + fb - 1) / fb));
+    if (narrow)
+        hipLaunchKernelGGL(k_asm_fill<16>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
+                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval);
+    else
+        hipLaunchKernelGGL(k_asm_fill<SLOT_R>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
+                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;
