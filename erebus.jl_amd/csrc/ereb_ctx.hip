@@ -194,21 +194,32 @@ static int reserve(ereb_ctx* c, int64_t n) {
         EREB_HIP(c, hipMalloc(&p, es * n));
         if (c->d_m[f]) {
             EREB_HIP(c, hipMemcpy(p, c->d_m[f], es * c->cap, hipMemcpyDeviceToDevice));
+            EREB_HIP(c, hipMemset((char*)p + es * c->cap, 0, es * (n - c->cap)));
             hipFree(c->d_m[f]);
         } else {
             EREB_HIP(c, hipMemset(p, 0, es * n));
         }
         c->d_m[f] = p;
     }
+    // the sort buffers: new ones first, so that a failed allocation leaves
+    // the old (still large enough for the old capacity) in place
+    uint32_t *key = nullptr, *perm = nullptr, *tmp = nullptr;
+    hipError_t e = hipMalloc(&key, sizeof(uint32_t) * n);
+    if (e == hipSuccess) e = hipMalloc(&perm, sizeof(uint32_t) * n);
+    // oversize tile segments are sorted in [2*start, 2*start + pow2(len)),
+    // disjoint between tiles and inside [0, 2n)
+    if (e == hipSuccess) e = hipMalloc(&tmp, sizeof(uint32_t) * 2 * n);
+    if (e != hipSuccess) {
+        hipFree(key);
+        hipFree(perm);
+        EREB_HIP(c, e);
+    }
     hipFree(c->d_key);
     hipFree(c->d_perm);
     hipFree(c->d_tmp);
-    c->d_key = c->d_perm = c->d_tmp = nullptr;
-    EREB_HIP(c, hipMalloc(&c->d_key, sizeof(uint32_t) * n));
-    EREB_HIP(c, hipMalloc(&c->d_perm, sizeof(uint32_t) * n));
-    // oversize tile segments are sorted in [2*start, 2*start + pow2(len)),
-    // disjoint between tiles and inside [0, 2n)
-    EREB_HIP(c, hipMalloc(&c->d_tmp, sizeof(uint32_t) * 2 * n));
+    c->d_key = key;
+    c->d_perm = perm;
+    c->d_tmp = tmp;
     c->tmp_cap = 2 * n;
     c->cap = n;
     c->have_sort = false;

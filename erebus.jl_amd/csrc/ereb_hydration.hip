@@ -104,6 +104,7 @@ template <class R>
 struct MeltOp {
     static constexpr int S = 2;  // XWSSUM, WTPSUM
     static constexpr int GROUP = 4;  // markers of a fine cell loaded together
+    static constexpr bool AHEAD = true;  // next group's loads before this group's sums
     const R* __restrict__ XW0;
     struct In {
         double xw;
@@ -139,6 +140,7 @@ template <class R>
 struct ReactionOp {
     static constexpr int S = 3;  // DMPSUM, DHPSUM, WTPSUM
     static constexpr int GROUP = 1;  // arithmetic-bound: one marker at a time
+    static constexpr bool AHEAD = false;  // a second marker's registers halve occupancy
     const ereb_materials* __restrict__ M;
     ereb_hydration H;
     const int32_t* __restrict__ tm;
@@ -214,12 +216,22 @@ struct ReactionOp {
     }
 };
 
+constexpr int PS_PERM = 4096;  // tile sort-order entries staged in LDS (16 KB)
+
 template <class Op>
 __global__ __launch_bounds__(PS_THREADS) void k_pscatter(PScatterBase B, Op op) {
     constexpr int S = Op::S;
     __shared__ double fl[NFINE * 4 * S];  // fine-cell partials [f][corner][s]
+    __shared__ uint32_t sperm[PS_PERM];   // the tile's part of the sort order
     const int tile = blockIdx.x, t = threadIdx.x;
     const int j0 = (tile / B.nty) * TILE, i0 = (tile % B.nty) * TILE;
+    // the tile's markers are one contiguous run of the sort order: staged in
+    // LDS with coalesced loads when it fits (read from memory otherwise)
+    const int ks = B.fstart[(size_t)tile * NFINE], ke = B.fstart[(size_t)(tile + 1) * NFINE];
+    const bool staged = ke - ks <= PS_PERM;
+    if (staged)
+        for (int k = ks + t; k < ke; k += PS_THREADS) sperm[k - ks] = B.perm[k];
+    __syncthreads();
     if (t < NFINE) {
         // fine cell t = fj * FINE + fi lies in P cell (ip, jp) (1-based): the
         // basic cell refined by its P offset (the fine key's definition)
@@ -235,34 +247,47 @@ __global__ __launch_bounds__(PS_THREADS) void k_pscatter(PScatterBase B, Op op) 
         if (k0 < k1) {
             // markers exist only in fine cells whose P cell is on the grid
             const double xa = B.g.xp[jp - 1], ya = B.g.yp[ip - 1];
-            // groups of Op::GROUP markers: their loads are issued together,
-            // then the markers are summed one after the other in sorted order
-            constexpr int PS_GROUP = Op::GROUP;
-            for (int k = k0; k < k1; k += PS_GROUP) {
-                uint32_t mm[PS_GROUP];
-                double xs[PS_GROUP], ys[PS_GROUP];
-                typename Op::In in[PS_GROUP];
+            // groups of Op::GROUP markers: a group's loads are issued together
+            // (with Op::AHEAD, before the previous group is summed); the
+            // markers are summed one after the other in sorted order
+            constexpr int G = Op::GROUP;
+            struct Group {
+                uint32_t m[G];
+                double x[G], y[G];
+                typename Op::In in[G];
+            };
+            auto fetch = [&](int k, Group& q) {
 #pragma unroll
-                for (int u = 0; u < PS_GROUP; ++u) mm[u] = B.perm[min(k + u, k1 - 1)];
-#pragma unroll
-                for (int u = 0; u < PS_GROUP; ++u) {
-                    xs[u] = B.xm[mm[u]];
-                    ys[u] = B.ym[mm[u]];
-                    in[u] = op.load(mm[u]);
+                for (int u = 0; u < G; ++u) {
+                    const int kk = min(k + u, k1 - 1);
+                    q.m[u] = staged ? sperm[kk - ks] : B.perm[kk];
                 }
 #pragma unroll
-                for (int u = 0; u < PS_GROUP; ++u) {
+                for (int u = 0; u < G; ++u) {
+                    q.x[u] = B.xm[q.m[u]];
+                    q.y[u] = B.ym[q.m[u]];
+                    q.in[u] = op.load(q.m[u]);
+                }
+            };
+            Group cur, nxt;
+            if (Op::AHEAD) fetch(k0, cur);
+            for (int k = k0; k < k1; k += G) {
+                if (!Op::AHEAD) fetch(k, cur);
+                else if (k + G < k1) fetch(k + G, nxt);
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
                     if (k + u >= k1) break;
                     double w[4];
-                    weights4((xs[u] - xa) / B.g.dx, (ys[u] - ya) / B.g.dy, w);
+                    weights4((cur.x[u] - xa) / B.g.dx, (cur.y[u] - ya) / B.g.dy, w);
                     double v[S];
-                    if (op(mm[u], in[u], B.g, w, ip, jp, v)) {
+                    if (op(cur.m[u], cur.in[u], B.g, w, ip, jp, v)) {
 #pragma unroll
                         for (int c = 0; c < 4; ++c)
 #pragma unroll
                             for (int s = 0; s < S; ++s) acc[c][s] += v[s] * w[c];
                     }
                 }
+                if (Op::AHEAD) cur = nxt;
             }
         }
 #pragma unroll
