@@ -51,24 +51,83 @@ __global__ __launch_bounds__(256) void k_anchor_init(uint64_t* __restrict__ dbit
     pick[a] = INT32_MAX;
 }
 
-// pass 1a: smallest distance per anchor
+// pass 1a: smallest distance per anchor.  A workgroup's markers (256
+// consecutive ones: in the lattice order of the reference's markers, a few
+// lattice columns) first take their minima in an LDS window over the
+// anchors they touch, then one global atomic per touched anchor; a
+// workgroup whose anchors span more than the window goes straight to the
+// global minima.
+constexpr int AWIN = 2048;  // anchors in the LDS window (16 KB)
+
 __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __restrict__ xm,
                                                     const double* __restrict__ ym, int64_t M,
                                                     uint64_t* __restrict__ dbits) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
-    const double x = xm[m], y = ym[m];
-    int i, j;
-    mcell(G, x, y, i, j);
+    __shared__ unsigned long long win[AWIN];
+    __shared__ int box[4];  // min cell row, max cell row, min cell column, max cell column
+    const int t = threadIdx.x;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + t;
+    const bool active = m < M;
+    double x = 0.0, y = 0.0;
+    int i = 1, j = 1;
+    if (active) {
+        x = xm[m];
+        y = ym[m];
+        mcell(G, x, y, i, j);
+    }
+    if (t == 0) {
+        box[0] = INT32_MAX;
+        box[1] = INT32_MIN;
+        box[2] = INT32_MAX;
+        box[3] = INT32_MIN;
+    }
+    int imin = active ? i : INT32_MAX, imax = active ? i : INT32_MIN;
+    int jmin = active ? j : INT32_MAX, jmax = active ? j : INT32_MIN;
+    for (int o = 32; o > 0; o >>= 1) {
+        imin = min(imin, __shfl_xor(imin, o));
+        imax = max(imax, __shfl_xor(imax, o));
+        jmin = min(jmin, __shfl_xor(jmin, o));
+        jmax = max(jmax, __shfl_xor(jmax, o));
+    }
+    __syncthreads();
+    if ((t & 63) == 0) {
+        atomicMin(&box[0], imin);
+        atomicMax(&box[1], imax);
+        atomicMin(&box[2], jmin);
+        atomicMax(&box[3], jmax);
+    }
+    __syncthreads();
+    // anchors (1-based) of cells i..i+1 x j..j+1 of every marker
+    const int i0 = box[0], j0 = box[2];
+    const int rows = box[1] - i0 + 2, cols = box[3] - j0 + 2;
+    const bool local = box[1] >= i0 && (int64_t)rows * cols <= AWIN;  // uniform
+    if (local) {
+        for (int k = t; k < rows * cols; k += blockDim.x) win[k] = ~0ull;
+        __syncthreads();
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int ii = i + (q & 1), jj = j + (q >> 1);
-        const double d = mdist(x, y, G.xxm[jj - 1], G.yym[ii - 1]);
+        const double d = active ? mdist(x, y, G.xxm[jj - 1], G.yym[ii - 1]) : G.mdis_init;
         const uint64_t b = (uint64_t)__double_as_longlong(d);
-        uint64_t* t = &dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)];
-        // the stored value only decreases: skip atomics that cannot win
-        if (d < G.mdis_init && b < __atomic_load_n(t, __ATOMIC_RELAXED))
-            atomicMin((unsigned long long*)t, (unsigned long long)b);
+        if (!(d < G.mdis_init)) continue;
+        if (local) {
+            atomicMin(&win[(jj - j0) * rows + (ii - i0)], (unsigned long long)b);
+        } else {
+            uint64_t* a = &dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)];
+            // the stored value only decreases: skip atomics that cannot win
+            if (b < __atomic_load_n(a, __ATOMIC_RELAXED))
+                atomicMin((unsigned long long*)a, (unsigned long long)b);
+        }
+    }
+    if (!local) return;
+    __syncthreads();
+    for (int k = t; k < rows * cols; k += blockDim.x) {
+        const unsigned long long b = win[k];
+        if (b == ~0ull) continue;
+        const int ii = i0 + k % rows, jj = j0 + k / rows;
+        uint64_t* a = &dbits[(size_t)(jj - 1) * G.Nym + (ii - 1)];
+        if (b < __atomic_load_n(a, __ATOMIC_RELAXED))
+            atomicMin((unsigned long long*)a, b);
     }
 }
 
