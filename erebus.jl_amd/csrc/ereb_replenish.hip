@@ -51,29 +51,18 @@ __global__ __launch_bounds__(256) void k_anchor_init(uint64_t* __restrict__ dbit
     pick[a] = INT32_MAX;
 }
 
-// pass 1a: smallest distance per anchor.  A workgroup's markers (256
-// consecutive ones: in the lattice order of the reference's markers, a few
-// lattice columns) first take their minima in an LDS window over the
-// anchors they touch, then one global atomic per touched anchor; a
-// workgroup whose anchors span more than the window goes straight to the
-// global minima.
-constexpr int AWIN = 2048;  // anchors in the LDS window (16 KB)
+constexpr int AWIN = 2048;  // anchors in an LDS window (16 KB of minima)
 
-__global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __restrict__ xm,
-                                                    const double* __restrict__ ym, int64_t M,
-                                                    uint64_t* __restrict__ dbits) {
-    __shared__ unsigned long long win[AWIN];
-    __shared__ int box[4];  // min cell row, max cell row, min cell column, max cell column
+// Anchors touched by a workgroup's markers: rows i0 .. i0+rows-1 and
+// columns j0 .. j0+cols-1 (1-based; cells i..i+1 x j..j+1 of every active
+// marker).  `local` (uniform): the box holds at most AWIN anchors.  Every
+// thread of the workgroup calls it (it synchronises).
+struct AnchorBox {
+    int i0, j0, rows, cols;
+    bool local;
+};
+__device__ AnchorBox anchor_box(bool active, int i, int j, int* box) {
     const int t = threadIdx.x;
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + t;
-    const bool active = m < M;
-    double x = 0.0, y = 0.0;
-    int i = 1, j = 1;
-    if (active) {
-        x = xm[m];
-        y = ym[m];
-        mcell(G, x, y, i, j);
-    }
     if (t == 0) {
         box[0] = INT32_MAX;
         box[1] = INT32_MIN;
@@ -96,10 +85,39 @@ __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __re
         atomicMax(&box[3], jmax);
     }
     __syncthreads();
-    // anchors (1-based) of cells i..i+1 x j..j+1 of every marker
-    const int i0 = box[0], j0 = box[2];
-    const int rows = box[1] - i0 + 2, cols = box[3] - j0 + 2;
-    const bool local = box[1] >= i0 && (int64_t)rows * cols <= AWIN;  // uniform
+    AnchorBox b;
+    b.i0 = box[0];
+    b.j0 = box[2];
+    b.rows = box[1] - b.i0 + 2;
+    b.cols = box[3] - b.j0 + 2;
+    b.local = box[1] >= b.i0 && (int64_t)b.rows * b.cols <= AWIN;
+    return b;
+}
+
+// pass 1a: smallest distance per anchor.  A workgroup's markers (256
+// consecutive ones: in the lattice order of the reference's markers, a few
+// lattice columns) first take their minima in an LDS window over the
+// anchors they touch, then one global atomic per touched anchor; a
+// workgroup whose anchors span more than the window goes straight to the
+// global minima.
+__global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __restrict__ xm,
+                                                    const double* __restrict__ ym, int64_t M,
+                                                    uint64_t* __restrict__ dbits) {
+    __shared__ unsigned long long win[AWIN];
+    __shared__ int box[4];  // min cell row, max cell row, min cell column, max cell column
+    const int t = threadIdx.x;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + t;
+    const bool active = m < M;
+    double x = 0.0, y = 0.0;
+    int i = 1, j = 1;
+    if (active) {
+        x = xm[m];
+        y = ym[m];
+        mcell(G, x, y, i, j);
+    }
+    const AnchorBox B = anchor_box(active, i, j, box);
+    const int i0 = B.i0, j0 = B.j0, rows = B.rows, cols = B.cols;
+    const bool local = B.local;
     if (local) {
         for (int k = t; k < rows * cols; k += blockDim.x) win[k] = ~0ull;
         __syncthreads();
@@ -131,7 +149,8 @@ __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __re
     }
 }
 
-// pass 1b: lowest marker index at that distance
+// pass 1b: lowest marker index at that distance (about one atomic per
+// anchor: an LDS window as in pass 1a measured slower here)
 __global__ __launch_bounds__(256) void k_anchor_pick(MGridK G, const double* __restrict__ xm,
                                                      const double* __restrict__ ym, int64_t M,
                                                      const uint64_t* __restrict__ dbits,
