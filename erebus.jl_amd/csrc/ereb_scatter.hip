@@ -21,24 +21,24 @@
 // mixed form; ScatterShape below).  The tile is walked one basic
 // column at a time: the column's 24 fine cells (fine columns 2k, 2k+1) hold a
 // contiguous run of sorted markers, split into chunks of at most CH markers
-// when it is longer:
-//   1. stage  : two threads per marker (one per half of its inputs; one
-//               thread in the four-wavefront form) stage the 18 property values and the fractions dxmj/dx, dymi/dy of
-//               the basic and P stencils in LDS.  Each thread's inputs for
-//               the next chunk are loaded into registers before the current
-//               chunk's tasks start, so the gather latency overlaps phases 2
-//               and 3;
-//   2. tasks  : wavefronts g and g + 4 own the first and second half of the
+// when it is longer.  Per chunk, between two workgroup barriers:
+//   1. tasks  : wavefronts g and g + 4 own the first and second half of the
 //               sums of grid g (basic, Vx, Vy, P; the second half includes
 //               the weight sum); with four wavefronts, wavefront g owns them
-//               all.  A task is (fine cell, corner column cj):
-//               the sums over the fine
-//               cell's markers, in order, of property * weight for the
-//               corners (0, cj) and (1, cj), and of the weights;
-//   3. combine: lane (r, c) of a wavefront owns window node (r, c) of its
+//               all.  A task is (fine cell, corner column cj): the sums over
+//               the fine cell's markers, in order, of property * weight for
+//               the corners (0, cj) and (1, cj), and of the weights;
+//   2. combine: lane (r, c) of a wavefront owns window node (r, c) of its
 //               grid and adds the partials of the at most 8 fine cells of the
 //               column that touch it, in a fixed (column, fine column, fine
-//               row) order.
+//               row) order; then
+//      stage  : the next chunk's markers (two threads per marker, one per
+//               half of its inputs; one thread in the four-wavefront form)
+//               stage their 18 property values and the fractions dxmj/dx,
+//               dymi/dy of the basic and P stencils in LDS.  Their inputs
+//               were loaded into registers before step 1, so the gather
+//               latency overlaps steps 1 and 2, and the staging overlaps
+//               other wavefronts' combine.
 // No atomics: every sum has exactly one owner and a fixed order, so results
 // are identical run to run.  Tile windows overlap by two node rows/columns;
 // k_combine adds the at most 2 x 2 window partials of each node in a fixed
@@ -381,20 +381,20 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
     Chunk cur = chunk_from(L, 0, __builtin_amdgcn_readfirstlane(L.fs[0]));
     Chunk nx = chunk_next(L, cur);
     double raw[NSET][NRAW];
+    // the first chunk is staged before the loop; each iteration then stages
+    // the next chunk while its combine runs (the combine reads the partials,
+    // the staging writes the staged values), two barriers per chunk
     if (cur.k < TILE && ts < cur.cend - cur.c0) {
         const uint32_t m = chunk_perm(A, cur, ts);
 #pragma unroll
         for (int q = 0; q < NSET; ++q) load_raw(A, part + q, m, raw[q]);
+#pragma unroll
+        for (int q = 0; q < NSET; ++q) stage_raw(A, L, part + q, ts, raw[q], j0, i0);
     }
     uint32_t pnx = chunk_perm(A, nx, ts);
+    __syncthreads();
     while (cur.k < TILE) {
-        // ---- 1. stage the chunk's markers (loaded one chunk ahead) ---------
-        if (ts < cur.cend - cur.c0) {
-#pragma unroll
-            for (int q = 0; q < NSET; ++q) stage_raw(A, L, part + q, ts, raw[q], j0, i0);
-        }
-        __syncthreads();
-        // prefetch: fields of the next chunk, perm of the one after
+        // ---- 1. loads of the next chunk's markers, perm of the one after ----
         const Chunk nx2 = chunk_next(L, nx);
         if (nx.k < TILE && ts < nx.cend - nx.c0) {
 #pragma unroll
@@ -439,6 +439,12 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
             case 5: combine<R, 1, 3, 3>(L, acc, r, c, cur.k); break;
             case 6: combine<R, 2, 3, 3>(L, acc, r, c, cur.k); break;
             default: combine<R, 3, 5, 5>(L, acc, r, c, cur.k); break;
+        }
+        // ---- 4. stage the next chunk (the staged values of this one are no
+        // longer read) ---------------------------------------------------------
+        if (nx.k < TILE && ts < nx.cend - nx.c0) {
+#pragma unroll
+            for (int q = 0; q < NSET; ++q) stage_raw(A, L, part + q, ts, raw[q], j0, i0);
         }
         __syncthreads();
         cur = nx;
