@@ -89,3 +89,36 @@ def test_graph_replay_equals_eager(precision):
     finally:
         A.close()
         B.close()
+
+
+def test_graph_replay_across_replenishment():
+    """A replenishment that appends markers (and grows the resident arrays)
+    between replays: the next step captures again and still equals the
+    eager launches bit for bit."""
+    g, mat = synth.workload_geometry("adj41", mode=1)
+    mk = synth.make_markers(g, mat, seed=6)
+    # an empty patch for the replenishment to fill
+    hole = (np.abs(mk["xm"] - 0.3 * g.xsize) < 0.05 * g.xsize) & \
+           (np.abs(mk["ym"] - 0.6 * g.ysize) < 0.05 * g.ysize)
+    mk = {k: v[~hole] for k, v in mk.items()}
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    dt = ereb.DT_LONGEST
+    A = _ctx(g, mat, mk, graphs=False)
+    B = _ctx(g, mat, mk, graphs=True)
+    try:
+        M0 = A.marknum
+        for ts in range(1, 7):
+            for c in (A, B):
+                c.step_transfer(hrs, hrf, 1, dt, ts)
+            if ts == 3:
+                na, mda, mna = A.step_replenish_markers()
+                nb, mdb, mnb = B.step_replenish_markers()
+                assert na == nb and na > M0, (na, nb, M0)
+                np.testing.assert_array_equal(mna, mnb)
+                np.testing.assert_array_equal(mda, mdb)
+            sa, sb = _state(A), _state(B)
+            for k in sa:
+                assert np.array_equal(sa[k], sb[k], equal_nan=True), (ts, k)
+    finally:
+        A.close()
+        B.close()
