@@ -321,6 +321,46 @@ def handoff_leg(ctx, reps: int = 5):
             "values_pinned": {"ms": round(vq, 3), "GB_per_s": round(vals_b / vq / 1e6, 2)}}
 
 
+def full_timestep_leg(ctx, mat, hrs, hrf, reps: int = 5):
+    """Every GPU row of one reference timestep in the reference's order
+    (E:6419-7065, host solves excluded): scatter, thermochemical reaction,
+    node properties, both assemblies, viscosity, stress and temperature
+    gathers, porosity, melt composition, RK4, pressure back-trace,
+    replenishment; eager launches, host wall time per timestep (the
+    replenishment reads the new marker count back, which synchronises).
+    Run after next_rows_leg, which sets the grids these rows read."""
+    hyd = ereb.Hydration()
+    dt = ereb.DT_LONGEST
+    mode = mat.marker_property_mode
+
+    def timestep(ts):
+        ctx.step_scatter(hrs, hrf, mode)
+        ctx.step_thermochemical_reaction(dt, ts, 3, hyd=hyd)
+        ctx.step_node_properties()
+        ctx.step_assemble_stokes_darcy(dt)
+        ctx.step_assemble_thermal(dt)
+        ctx.step_marker_viscosity()
+        ctx.step_gather_stress()
+        ctx.step_gather_temperature(ts)
+        ctx.step_marker_porosity(dt, hyd=hyd)
+        ctx.step_melt_composition()
+        ctx.step_advect_rk4(dt, mode)
+        ctx.step_backtrace_pressures(dt)
+        ctx.step_replenish_markers(maps=False)  # mdis, mnum stay on the device
+
+    timestep(2)
+    ctx.sync()
+    M = ctx.marknum
+    t0 = time.perf_counter()
+    for k in range(reps):
+        timestep(3 + k)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"ms_per_timestep": round(ms, 4), "marker_steps_per_s": M / (ms * 1e-3),
+            "markers": M, "order": "scatter, reaction, node properties, assemblies, "
+            "viscosity, gathers, porosity, melt composition, RK4, back-trace, replenishment"}
+
+
 BREAKS = (263.0, 268.0, 273.0, 278.0, 410.0)  # kinks and steps of T-dependent properties
 
 
@@ -468,6 +508,9 @@ def main():
     if not a.no_next_rows:
         log("[bench] remaining transfer rows")
         res["next_rows"] = next_rows_leg(ctx64, g, mat, hrs, hrf)
+        log("[bench] full reference timestep on the GPU")
+        res["full_timestep"] = full_timestep_leg(ctx64, mat, hrs, hrf)
+        log(f"[bench] full timestep {res['full_timestep']['ms_per_timestep']:.3f} ms")
     ctx64.close()
     if not a.no_adjoint:
         log("[bench] adjoint leg")

@@ -279,10 +279,10 @@ class Geometry:
         x0 + k*step; for the supported geometries the steps are whole or
         half metres, so the values are exact and equal Julia's range
         elements."""
-        if not self._axes:
+        if "x" not in self._axes:
             dx, dy = self.dx, self.dy
             k = lambda n: np.arange(n, dtype=np.float64)
-            self._axes = {
+            node_axes = {
                 "x": 0.0 + k(self.Nx) * dx,                 # C:40
                 "y": 0.0 + k(self.Ny) * dy,                 # C:43
                 "xvx": 0.0 + k(self.Nx1) * dx,              # C:47
@@ -292,20 +292,25 @@ class Geometry:
                 "xp": -dx / 2 + k(self.Nx1) * dx,           # C:61
                 "yp": -dy / 2 + k(self.Ny1) * dy,           # C:64
             }
-            for a in self._axes.values():
+            for a in node_axes.values():
                 a.setflags(write=False)
-        return self._axes
+            self._axes.update(node_axes)
+        return {n: self._axes[n] for n in ("x", "y", "xvx", "yvx", "xvy", "yvy", "xp", "yp")}
 
     def marker_axes(self) -> tuple:
         """Marker launch lattice xxm = dxm/2:dxm:xsize-dxm/2 and yym (C:116,
         C:119): Julia lifts these float ranges to the rationals
         (k - 1/2) * xsize / Nxm, so each element is that rational rounded
         once."""
-        from fractions import Fraction
-        fx, fy = Fraction(self.xsize) / self.Nxm, Fraction(self.ysize) / self.Nym
-        xxm = np.array([float((k + Fraction(1, 2)) * fx) for k in range(self.Nxm)])
-        yym = np.array([float((k + Fraction(1, 2)) * fy) for k in range(self.Nym)])
-        return xxm, yym
+        if "xxm" not in self._axes:  # exact rationals are slow: computed once
+            from fractions import Fraction
+            fx, fy = Fraction(self.xsize) / self.Nxm, Fraction(self.ysize) / self.Nym
+            xxm = np.array([float((k + Fraction(1, 2)) * fx) for k in range(self.Nxm)])
+            yym = np.array([float((k + Fraction(1, 2)) * fy) for k in range(self.Nym)])
+            for a in (xxm, yym):
+                a.setflags(write=False)
+            self._axes["xxm"], self._axes["yym"] = xxm, yym
+        return self._axes["xxm"], self._axes["yym"]
 
     def marker_grid_c(self) -> CMarkerGrid:
         """ereb_marker_grid of this geometry: dii = 5*Nymc, djj = 5*Nxmc

@@ -357,13 +357,15 @@ class Context:
         self.check(self.lib.ereb_step_backtrace_pressures(self.h, float(dt)),
                    "step_backtrace_pressures")
 
-    def step_replenish_markers(self, randomized: bool = False, seed: int = 0):
+    def step_replenish_markers(self, randomized: bool = False, seed: int = 0,
+                               maps: bool = True):
         """replenish_markers! on the resident markers: (new marknum, mdis,
-        mnum), mdis and mnum (Nym, Nxm) Fortran-ordered as Julia's."""
+        mnum), mdis and mnum (Nym, Nxm) Fortran-ordered as Julia's; with
+        maps=False they stay on the device (None is returned for them)."""
         g = self.geom
         mg = g.marker_grid_c()
-        mdis = np.empty((g.Nym, g.Nxm), order="F")
-        mnum = np.empty((g.Nym, g.Nxm), dtype=np.int64, order="F")
+        mdis = np.empty((g.Nym, g.Nxm), order="F") if maps else None
+        mnum = np.empty((g.Nym, g.Nxm), dtype=np.int64, order="F") if maps else None
         n = ctypes.c_int64()
         self.check(self.lib.ereb_step_replenish_markers(
             self.h, ctypes.addressof(mg), int(bool(randomized)), int(seed), _ptr(mdis),
