@@ -123,3 +123,38 @@ def test_assembly_at_workload_size(name):
                        ("thermal rowval", LT.rowval, rvt), ("thermal nzval", LT.nzval, nzt),
                        ("RT", RTg, RT)):
         _exact(a, b, what)
+
+
+def test_mixed_form_at_workload_size():
+    """The mixed form against the Float64 form over 5 transfer steps of the
+    2.8M-marker workload: every SUM and node field within 1e-6 normwise
+    (max|a - ref| / max|ref|), the bar of the mixed form."""
+    from ereb import precision
+    g, mat = synth.workload_geometry("fp64", mode=1)
+    mk = synth.make_markers(g, mat, seed=0, stress=True)
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    vx, vy = synth.stream_velocity(g)
+    rng = np.random.default_rng(4)
+    S = (g.Ny1, g.Nx1)
+    grids = {"vx": vx, "vy": vy, "vxf": vx, "vyf": vy, "wyx": synth.rotation_rate(g, vx, vy),
+             "ETAP": np.full(S, 1e18), "ETAPHI": np.full(S, 1e19), "gx": np.full(S, -0.1),
+             "gy": np.full(S, -0.2), "DT": 0.01 * rng.random(S), "tk2": 250 + 50 * rng.random(S),
+             "pr0": np.full(S, 1e5), "pf0": np.full(S, 9e4), "DSXX": 1e3 * rng.random(S),
+             "DSXY": 1e3 * rng.random(g.basic_shape)}
+    ctxs = []
+    try:
+        for prec in (ereb.FP64, ereb.MIXED):
+            c = ereb.Context(g, mat, prec)
+            ctxs.append(c)
+            c.set_markers(mk)
+            for n, a in grids.items():
+                c.set_grid(n, a)
+            for ts in range(1, 6):
+                c.step_transfer(hrs, hrf, 1, ereb.DT_LONGEST, ts)
+        errs = precision.field_errors(ctxs[1], ctxs[0])
+        nodal = ereb.SUM_FIELDS + precision.NODE_FIELDS
+        bad = {k: errs[k][0] for k in nodal if errs[k][0] > 1e-6}
+        assert not bad, bad
+    finally:
+        for c in ctxs:
+            c.close()
