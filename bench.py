@@ -207,6 +207,43 @@ def run_form(g, mat, mk, grids, hrs, hrf, precision, steps, warmup):
     return ctx, M * steps / el, el * 1e3 / steps, kern
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Device-to-device copy rate of this GPU in GB/s (bytes read plus
+    bytes written per second), HIP events around hipMemcpy on the null
+    stream: the practical ceiling next to the 8 TB/s vendor peak."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library links
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    ok = hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    gbps = float("nan")
+    if ok:
+        hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+        D2D = 3  # hipMemcpyDeviceToDevice
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), D2D)
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), D2D)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        if hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0 and ms.value > 0:
+            gbps = 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    for e in (e0, e1):
+        if e.value:
+            hip.hipEventDestroy(e)
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    return gbps
+
+
+def _finite(x: float, nd: int = 1):
+    return round(x, nd) if x == x else None
+
+
 def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
     sums = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1  # elements of the 30 SUM grids
     sc_bytes = M * marker_bytes + sums * 8
@@ -471,8 +508,9 @@ def main():
                                f"planetesimal R={g.rplanet / 1e3:.0f} km in 140 km box, "
                                f"stream-function velocity Courant 0.5, dt=1e11 s, mode {mode}",
                    "markers": M, "grid": [g.Ny, g.Nx], "form": "fp64"},
-        "roofline": scatter_roofline(g, M, kern["scatter"], SCATTER_MARKER_BYTES,
-                                     "k_scatter<double>", "fp64", a.workload),
+        "roofline": dict(scatter_roofline(g, M, kern["scatter"], SCATTER_MARKER_BYTES,
+                                          "k_scatter<double>", "fp64", a.workload),
+                         measured_copy_GBps=_finite(copy_bandwidth())),
         "kernel_ms": kern,
     }
     if not a.no_mixed:
