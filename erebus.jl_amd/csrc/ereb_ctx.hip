@@ -364,6 +364,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_flags);
     hipFree(c->d_part);
     hipFree(c->d_scan);
+    hipFree(c->d_react);
     for (auto& s : c->sys) {
         hipFree(s.colcnt);
         hipFree(s.colptr);

@@ -16,7 +16,12 @@
 // summed per node by k_pcombine, which also divides by the weight
 // (compute_molarfraction! E:2239, compute_thermodynamic_xfer! E:2208).
 // No atomics: results are reproducible run to run, and agree with the
-// reference's marker-order sums to rounding.
+// reference's marker-order sums to rounding.  The reaction's per-marker arithmetic
+// (an exp, a log and a dozen divisions per rock marker) runs first, one
+// thread per marker in storage order, and leaves each marker's ΔM, ΔH and
+// whether it contributes; the scatter then sums those in the fine-cell order
+// (one thread per fine cell would otherwise run its markers' arithmetic one
+// after another at a fraction of the occupancy).
 #include "ereb_internal.h"
 
 namespace ereb {
@@ -138,9 +143,6 @@ __device__ __forceinline__ double dtreaction(const ereb_hydration& H, double T, 
 // XWsm0, phim) for the fluid-bearing ones, which contribute ΔM, ΔH and 1
 template <class R>
 struct ReactionOp {
-    static constexpr int S = 3;  // DMPSUM, DHPSUM, WTPSUM
-    static constexpr int GROUP = 1;  // arithmetic-bound: one marker at a time
-    static constexpr bool AHEAD = false;  // a second marker's registers halve occupancy
     const ereb_materials* __restrict__ M;
     ereb_hydration H;
     const int32_t* __restrict__ tm;
@@ -158,8 +160,7 @@ struct ReactionOp {
         int t;
         double pf0, XW0, ph;
     };
-    // every marker's inputs (rock or not), so that the loads of a group of
-    // markers can be issued together
+    // every marker's inputs (rock or not): the loads are issued together
     __device__ __forceinline__ In load(uint32_t m) const {
         return In{tm[m], pfm0[m], (double)XWsm0[m], phim[m]};
     }
@@ -217,6 +218,56 @@ struct ReactionOp {
 };
 
 constexpr int PS_PERM = 4096;  // tile sort-order entries staged in LDS (16 KB)
+
+// the reaction's per-marker pass (E:2587-2686) in storage order: the marker's
+// P cell and weights are those that the fine cell of the scatter gives it
+// (the fine key is derived from the same cells), so the values are the ones
+// the scatter would compute
+template <class R>
+__global__ __launch_bounds__(256) void k_reaction_markers(ReactionOp<R> op, GeomK g,
+                                                          const double* __restrict__ xm,
+                                                          const double* __restrict__ ym,
+                                                          int64_t M, double* __restrict__ dM,
+                                                          double* __restrict__ dH,
+                                                          uint8_t* __restrict__ ok) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const typename ReactionOp<R>::In in = op.load((uint32_t)m);
+    const double x = xm[m], y = ym[m];
+    const Cells cl = marker_cells(g, x, y);
+    double w[4], v[3];
+    weights4((x - g.xp[cl.jp - 1]) / g.dx, (y - g.yp[cl.ip - 1]) / g.dy, w);
+    const bool r = op((uint32_t)m, in, g, w, cl.ip, cl.jp, v);
+    ok[m] = r ? 1 : 0;
+    if (r) {
+        dM[m] = v[0];
+        dH[m] = v[1];
+    }
+}
+
+// the reaction's scatter: ΔM, ΔH and weight 1 of the contributing markers
+struct ReactionSumOp {
+    static constexpr int S = 3;  // DMPSUM, DHPSUM, WTPSUM
+    static constexpr int GROUP = 4;
+    static constexpr bool AHEAD = true;
+    const double* __restrict__ dM;
+    const double* __restrict__ dH;
+    const uint8_t* __restrict__ ok;
+    struct In {
+        double dm, dh;
+        bool ok;
+    };
+    // dM, dH of a marker that does not contribute hold stale values (read, not used)
+    __device__ __forceinline__ In load(uint32_t m) const { return In{dM[m], dH[m], ok[m] != 0}; }
+    __device__ __forceinline__ bool operator()(uint32_t, const In& in, const GeomK&,
+                                               const double*, int, int, double* v) const {
+        if (!in.ok) return false;
+        v[0] = in.dm;
+        v[1] = in.dh;
+        v[2] = 1.0;
+        return true;
+    }
+};
 
 template <class Op>
 __global__ __launch_bounds__(PS_THREADS) void k_pscatter(PScatterBase B, Op op) {
@@ -461,20 +512,52 @@ static ReactionOp<R> reaction_op(ereb_ctx* c, const ereb_hydration& H, R* XWsm0,
     return op;
 }
 
+// dM, dH (double) and the contribute flag (byte) of every marker, for the
+// context's marker capacity (reallocated when it grows)
+static int ensure_react(ereb_ctx* c) {
+    if (c->react_cap >= c->cap) return EREB_OK;
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    hipFree(c->d_react);
+    c->d_react = nullptr;
+    c->react_cap = 0;
+    EREB_HIP(c, hipMalloc(&c->d_react, (size_t)c->cap * (2 * sizeof(double) + 1)));
+    c->react_cap = c->cap;
+    return EREB_OK;
+}
+
+template <class R>
+static void launch_reaction_markers(ereb_ctx* c, const ReactionOp<R>& op, double* dM,
+                                    double* dH, uint8_t* ok) {
+    const int64_t M = c->marknum;
+    if (M <= 0) return;
+    hipLaunchKernelGGL(k_reaction_markers<R>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                       c->stream, op, geomk(c), (const double*)mptr(c, EREB_M_XM),
+                       (const double*)mptr(c, EREB_M_YM), M, dM, dH, ok);
+}
+
 int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64_t timestep,
                           int64_t titer) {
     hipEvent_t ev;
     int rc = ensure_sort(c);
     if (rc) return rc;
+    if ((rc = ensure_react(c))) return rc;
     prof_begin(c, EREB_K_REACTION, &ev);
     double *s0 = gptr(c, EREB_G_DMPSUM), *s1 = gptr(c, EREB_G_DHPSUM),
            *s2 = gptr(c, EREB_G_WTPSUM), *o0 = gptr(c, EREB_G_DMP), *o1 = gptr(c, EREB_G_DHP);
+    double* dM = (double*)c->d_react;
+    double* dH = dM + c->react_cap;
+    uint8_t* ok = (uint8_t*)(dH + c->react_cap);
     if (marker_is_f32(c->precision, EREB_M_XWSOLIDM0))
-        rc = run_pscatter(c, reaction_op(c, H, fptr(c, EREB_M_XWSOLIDM0), dt, timestep, titer),
-                          s0, s1, s2, o0, o1);
+        launch_reaction_markers(c, reaction_op(c, H, fptr(c, EREB_M_XWSOLIDM0), dt, timestep,
+                                               titer), dM, dH, ok);
     else
-        rc = run_pscatter(c, reaction_op(c, H, mptr(c, EREB_M_XWSOLIDM0), dt, timestep, titer),
-                          s0, s1, s2, o0, o1);
+        launch_reaction_markers(c, reaction_op(c, H, mptr(c, EREB_M_XWSOLIDM0), dt, timestep,
+                                               titer), dM, dH, ok);
+    ReactionSumOp sum;
+    sum.dM = dM;
+    sum.dH = dH;
+    sum.ok = ok;
+    rc = run_pscatter(c, sum, s0, s1, s2, o0, o1);
     if (rc) return rc;
     prof_end(c, EREB_K_REACTION, ev);
     return EREB_OK;
