@@ -622,6 +622,7 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     // sort is of the positions before RK4, which then moved them
     c->have_sort = true;
     c->sort_gen = c->pos_gen;
+    c->sort_m = c->marknum;
     ++c->pos_gen;
     c->sys[0].valid = c->sys[1].valid = true;
     return EREB_OK;

@@ -100,6 +100,7 @@ struct ereb_ctx {
     // the sort is current while sort_gen == pos_gen; pos_gen advances
     // whenever positions or the marker count may have changed
     uint64_t pos_gen = 1, sort_gen = 0;
+    int64_t sort_m = 0;  // marker count that the last completed sort ordered
 
     ereb::Sys sys[2];
 

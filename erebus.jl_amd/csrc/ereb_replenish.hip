@@ -94,23 +94,36 @@ __device__ AnchorBox anchor_box(bool active, int i, int j, int* box) {
     return b;
 }
 
+// Both passes visit the markers in `order` when it is given (a permutation
+// of the markers that keeps neighbours together: the fine-key sort of the
+// timestep's scatter, made before the advection moved the markers by less
+// than a cell), else in storage order.  Minima and picks do not depend on
+// the order; the window of pass 1a does: advected markers drift away from
+// their storage-order neighbours, and a workgroup's anchors then span more
+// than the window.
+__device__ __forceinline__ int64_t visit(const uint32_t* order, int64_t k) {
+    return order ? (int64_t)order[k] : k;
+}
+
 // pass 1a: smallest distance per anchor.  A workgroup's markers (256
-// consecutive ones: in the lattice order of the reference's markers, a few
-// lattice columns) first take their minima in an LDS window over the
-// anchors they touch, then one global atomic per touched anchor; a
-// workgroup whose anchors span more than the window goes straight to the
-// global minima.
+// consecutive ones in the visiting order: part of a tile of the sort, or in
+// the lattice order of the reference's markers a few lattice columns) first
+// take their minima in an LDS window over the anchors they touch, then one
+// global atomic per touched anchor; a workgroup whose anchors span more than
+// the window goes straight to the global minima.
 __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __restrict__ xm,
                                                     const double* __restrict__ ym, int64_t M,
+                                                    const uint32_t* __restrict__ order,
                                                     uint64_t* __restrict__ dbits) {
     __shared__ unsigned long long win[AWIN];
     __shared__ int box[4];  // min cell row, max cell row, min cell column, max cell column
     const int t = threadIdx.x;
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + t;
-    const bool active = m < M;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + t;
+    const bool active = k < M;
     double x = 0.0, y = 0.0;
     int i = 1, j = 1;
     if (active) {
+        const int64_t m = visit(order, k);
         x = xm[m];
         y = ym[m];
         mcell(G, x, y, i, j);
@@ -153,10 +166,12 @@ __global__ __launch_bounds__(256) void k_anchor_min(MGridK G, const double* __re
 // anchor: an LDS window as in pass 1a measured slower here)
 __global__ __launch_bounds__(256) void k_anchor_pick(MGridK G, const double* __restrict__ xm,
                                                      const double* __restrict__ ym, int64_t M,
+                                                     const uint32_t* __restrict__ order,
                                                      const uint64_t* __restrict__ dbits,
                                                      int32_t* __restrict__ pick) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    const int64_t m = visit(order, k);
     const double x = xm[m], y = ym[m];
     int i, j;
     mcell(G, x, y, i, j);
@@ -348,10 +363,14 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     std::memcpy(&init_bits, &mg.mdis_init, sizeof init_bits);
     hipLaunchKernelGGL(k_anchor_init, ga, dim3(256), 0, c->stream, dbits, pick, init_bits, A);
     if (M > 0) {
+        // the last completed sort, stale or not, while it orders exactly
+        // these markers (the sort buffers are reallocated, and have_sort
+        // cleared, when the capacity grows)
+        const uint32_t* order = c->have_sort && c->sort_m == M ? c->d_perm : nullptr;
         hipLaunchKernelGGL(k_anchor_min, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
-                           mptr(c, EREB_M_YM), M, dbits);
+                           mptr(c, EREB_M_YM), M, order, dbits);
         hipLaunchKernelGGL(k_anchor_pick, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
-                           mptr(c, EREB_M_YM), M, (const uint64_t*)dbits, pick);
+                           mptr(c, EREB_M_YM), M, order, (const uint64_t*)dbits, pick);
     }
     if ((rc = fill_i32(c, nempty, 0, 1))) return rc;
     hipLaunchKernelGGL(k_anchor_fill, ga, dim3(256), 0, c->stream, G, (const uint64_t*)dbits,

@@ -235,6 +235,8 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
     const int64_t nk = c->nkeys;
     int rc;
     hipEvent_t ev;
+    // perm is rewritten below: it orders no marker set until the sort completes
+    c->have_sort = false;
     prof_begin(c, EREB_K_SORT, &ev);
     // the key counts are zero here (left so by the previous sort's scan)
     // unless a sort stopped half-way; a domain error flag stays set until
@@ -277,6 +279,7 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
     prof_end(c, EREB_K_SORT, ev);
     c->have_sort = true;
     c->sort_gen = c->pos_gen;
+    c->sort_m = M;
     return EREB_OK;
 }
 

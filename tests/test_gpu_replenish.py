@@ -15,6 +15,7 @@ import ereb
 from ereb import api, synth
 from ereb.constants import test_constants as tconsts
 from ereb.lib import MARKER_FIELDS
+from ereb.precision import MARKER_PROPS
 from oracle import Oracle
 from test_oracle_replenish import quarter_markers
 
@@ -96,6 +97,41 @@ def test_replenish_resident_workload_hole():
     assert n > mk["xm"].size
     with ereb.Context(g, mat, ereb.FP64) as ctx:
         ctx.set_markers(mk)
+        nn, mdis, mnum = ctx.step_replenish_markers()
+        assert nn == n == ctx.marknum
+        for f in MARKER_FIELDS:
+            _exact(ctx.get_marker(f), want[f], f)
+    _exact(mnum, mnum_o, "mnum")
+    _exact(mdis, mdis_o, "mdis")
+
+
+def test_replenish_after_advection_in_sort_order():
+    """The timestep's order: a scatter sorts the markers, the advection moves
+    them (here up to 0.6 cells), then the replenishment visits them in the
+    now stale sort order.  The result equals the oracle's on the moved
+    markers, bit for bit."""
+    g, mat = synth.workload_geometry("fp64", mode=1)
+    mk = synth.make_markers(g, mat, seed=2)
+    rng = np.random.default_rng(3)
+    for f in ("XWsolidm", "phinewm", "pfm0"):
+        mk[f] = rng.random(mk["xm"].size)
+    hole = ((mk["xm"] > 40e3) & (mk["xm"] < 55e3) & (mk["ym"] > 60e3) & (mk["ym"] < 70e3))
+    mk = {f: np.ascontiguousarray(v[~hole]) for f, v in mk.items()}
+    M = mk["xm"].size
+    moved = dict(mk)
+    moved["xm"] = mk["xm"] + 0.6 * g.dx * (2 * rng.random(M) - 1)
+    moved["ym"] = mk["ym"] + 0.6 * g.dy * (2 * rng.random(M) - 1)
+    hrs, hrf = mat.radioactive_heating(2.25e6 * 365.25 * 24 * 3600)
+    with ereb.Context(g, mat, ereb.FP64) as ctx:
+        ctx.set_markers(mk)
+        ctx.step_scatter(hrs, hrf, 1)
+        # the scatter's key pass rewrote the marker properties
+        for f in MARKER_PROPS:
+            moved[f] = ctx.get_marker(f)
+        want, n, mdis_o, mnum_o = Oracle(g, mat).replenish_markers(moved)
+        assert n > M
+        ctx.set_marker("xm", moved["xm"])
+        ctx.set_marker("ym", moved["ym"])
         nn, mdis, mnum = ctx.step_replenish_markers()
         assert nn == n == ctx.marknum
         for f in MARKER_FIELDS:
