@@ -227,8 +227,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_reaction_markers(ReactionOp<R> op, GeomK g,
                                                           const double* __restrict__ xm,
                                                           const double* __restrict__ ym,
-                                                          int64_t M, double* __restrict__ dM,
-                                                          double* __restrict__ dH,
+                                                          int64_t M, double2* __restrict__ dMH,
                                                           uint8_t* __restrict__ ok) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
@@ -239,26 +238,26 @@ __global__ __launch_bounds__(256) void k_reaction_markers(ReactionOp<R> op, Geom
     weights4((x - g.xp[cl.jp - 1]) / g.dx, (y - g.yp[cl.ip - 1]) / g.dy, w);
     const bool r = op((uint32_t)m, in, g, w, cl.ip, cl.jp, v);
     ok[m] = r ? 1 : 0;
-    if (r) {
-        dM[m] = v[0];
-        dH[m] = v[1];
-    }
+    if (r) dMH[m] = make_double2(v[0], v[1]);
 }
 
-// the reaction's scatter: ΔM, ΔH and weight 1 of the contributing markers
+// the reaction's scatter: ΔM, ΔH (one 16-byte load) and weight 1 of the
+// contributing markers
 struct ReactionSumOp {
     static constexpr int S = 3;  // DMPSUM, DHPSUM, WTPSUM
     static constexpr int GROUP = 4;
     static constexpr bool AHEAD = true;
-    const double* __restrict__ dM;
-    const double* __restrict__ dH;
+    const double2* __restrict__ dMH;
     const uint8_t* __restrict__ ok;
     struct In {
         double dm, dh;
         bool ok;
     };
-    // dM, dH of a marker that does not contribute hold stale values (read, not used)
-    __device__ __forceinline__ In load(uint32_t m) const { return In{dM[m], dH[m], ok[m] != 0}; }
+    // ΔM, ΔH of a marker that does not contribute hold stale values (read, not used)
+    __device__ __forceinline__ In load(uint32_t m) const {
+        const double2 p = dMH[m];
+        return In{p.x, p.y, ok[m] != 0};
+    }
     __device__ __forceinline__ bool operator()(uint32_t, const In& in, const GeomK&,
                                                const double*, int, int, double* v) const {
         if (!in.ok) return false;
@@ -512,8 +511,8 @@ static ReactionOp<R> reaction_op(ereb_ctx* c, const ereb_hydration& H, R* XWsm0,
     return op;
 }
 
-// dM, dH (double) and the contribute flag (byte) of every marker, for the
-// context's marker capacity (reallocated when it grows)
+// (ΔM, ΔH) pairs (double2) and the contribute flag (byte) of every marker,
+// for the context's marker capacity (reallocated when it grows)
 static int ensure_react(ereb_ctx* c) {
     if (c->react_cap >= c->cap) return EREB_OK;
     EREB_HIP(c, hipStreamSynchronize(c->stream));
@@ -526,13 +525,13 @@ static int ensure_react(ereb_ctx* c) {
 }
 
 template <class R>
-static void launch_reaction_markers(ereb_ctx* c, const ReactionOp<R>& op, double* dM,
-                                    double* dH, uint8_t* ok) {
+static void launch_reaction_markers(ereb_ctx* c, const ReactionOp<R>& op, double2* dMH,
+                                    uint8_t* ok) {
     const int64_t M = c->marknum;
     if (M <= 0) return;
     hipLaunchKernelGGL(k_reaction_markers<R>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
                        c->stream, op, geomk(c), (const double*)mptr(c, EREB_M_XM),
-                       (const double*)mptr(c, EREB_M_YM), M, dM, dH, ok);
+                       (const double*)mptr(c, EREB_M_YM), M, dMH, ok);
 }
 
 int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64_t timestep,
@@ -544,18 +543,16 @@ int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64
     prof_begin(c, EREB_K_REACTION, &ev);
     double *s0 = gptr(c, EREB_G_DMPSUM), *s1 = gptr(c, EREB_G_DHPSUM),
            *s2 = gptr(c, EREB_G_WTPSUM), *o0 = gptr(c, EREB_G_DMP), *o1 = gptr(c, EREB_G_DHP);
-    double* dM = (double*)c->d_react;
-    double* dH = dM + c->react_cap;
-    uint8_t* ok = (uint8_t*)(dH + c->react_cap);
+    double2* dMH = (double2*)c->d_react;
+    uint8_t* ok = (uint8_t*)(dMH + c->react_cap);
     if (marker_is_f32(c->precision, EREB_M_XWSOLIDM0))
         launch_reaction_markers(c, reaction_op(c, H, fptr(c, EREB_M_XWSOLIDM0), dt, timestep,
-                                               titer), dM, dH, ok);
+                                               titer), dMH, ok);
     else
         launch_reaction_markers(c, reaction_op(c, H, mptr(c, EREB_M_XWSOLIDM0), dt, timestep,
-                                               titer), dM, dH, ok);
+                                               titer), dMH, ok);
     ReactionSumOp sum;
-    sum.dM = dM;
-    sum.dH = dH;
+    sum.dMH = dMH;
     sum.ok = ok;
     rc = run_pscatter(c, sum, s0, s1, s2, o0, o1);
     if (rc) return rc;

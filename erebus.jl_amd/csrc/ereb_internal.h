@@ -90,7 +90,7 @@ struct ereb_ctx {
     int32_t* d_flags = nullptr;    // [0] domain error
     double* d_part = nullptr;      // ntiles * NSUM * 64
     int32_t* d_scan = nullptr;     // scan block sums
-    // thermochemical reaction: dM[react_cap], dH[react_cap], contributes[react_cap]
+    // thermochemical reaction: (dM, dH)[react_cap] as double2, contributes[react_cap]
     void* d_react = nullptr;
     int64_t react_cap = 0;
     int64_t scan_cap = 0;

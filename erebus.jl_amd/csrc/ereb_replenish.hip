@@ -369,6 +369,9 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
         const uint32_t* order = c->have_sort && c->sort_m == M ? c->d_perm : nullptr;
         hipLaunchKernelGGL(k_anchor_min, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
                            mptr(c, EREB_M_YM), M, order, dbits);
+        // the index pass in the same order: it re-reads the markers and the
+        // anchors of pass 1a while they are in cache (storage order measured
+        // 198 against 191 us for the whole replenishment)
         hipLaunchKernelGGL(k_anchor_pick, gm, dim3(256), 0, c->stream, G, mptr(c, EREB_M_XM),
                            mptr(c, EREB_M_YM), M, order, (const uint64_t*)dbits, pick);
     }
