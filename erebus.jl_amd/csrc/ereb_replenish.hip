@@ -209,58 +209,86 @@ __global__ __launch_bounds__(256) void k_anchor_fill(MGridK G,
     empty[atomicAdd(nempty, 1)] = (int32_t)a;  // list order does not affect any result
 }
 
-// pass 2b: one wavefront per empty anchor scans its window in the
+// pass 2b: one workgroup per empty anchor scans its window in the
 // reference's order (jj outer, ii inner), reading only pass-1 picks (the
 // reference tests mnum > 0, so anchors filled here never feed another).
-// Lanes take window positions w, w + 64, ...; the wavefront keeps the
-// smallest distance and, among equal ones, the first position -- the pick
-// of the sequential strict-< scan.
-__global__ __launch_bounds__(256) void k_anchor_search(MGridK G, const double* __restrict__ xm,
-                                                       const double* __restrict__ ym,
-                                                       const int32_t* __restrict__ pick,
-                                                       const int32_t* __restrict__ empty,
-                                                       const int32_t* __restrict__ nempty,
-                                                       double* __restrict__ mdis,
-                                                       int32_t* __restrict__ mnum,
-                                                       int32_t* __restrict__ flag) {
-    const int lane = threadIdx.x & 63;
-    const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwv = (int)((gridDim.x * blockDim.x) >> 6);
+// Thread t takes window positions t, t + 256, ..., loading the picks of
+// four positions, then their markers, before comparing; the workgroup keeps
+// the smallest distance and, among equal ones, the first position -- the
+// pick of the sequential strict-< scan.
+constexpr int SEARCH_THREADS = 256;
+
+struct Best {
+    double d;
+    int w, q;
+};
+// the better of two candidates: smaller distance, then earlier position
+__device__ __forceinline__ void take_better(Best& a, const Best& b) {
+    if (b.d < a.d || (b.d == a.d && b.w < a.w)) a = b;
+}
+
+__global__ __launch_bounds__(SEARCH_THREADS) void k_anchor_search(
+    MGridK G, const double* __restrict__ xm, const double* __restrict__ ym,
+    const int32_t* __restrict__ pick, const int32_t* __restrict__ empty,
+    const int32_t* __restrict__ nempty, double* __restrict__ mdis, int32_t* __restrict__ mnum,
+    int32_t* __restrict__ flag) {
+    constexpr int U = 4;
+    __shared__ Best wbest[SEARCH_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ne = *nempty;
-    for (int e = wv; e < ne; e += nwv) {  // wave-uniform bound: every wave exits
+    for (int e = blockIdx.x; e < ne; e += gridDim.x) {  // block-uniform bound: every block exits
         const int64_t a = empty[e];
         const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
         const double xa = G.xxm[j - 1], ya = G.yym[i - 1];
         const int jlo = max(j - G.djj, 1), jhi = min(j + G.djj, G.Nxm);
         const int ilo = max(i - G.dii, 1), ihi = min(i + G.dii, G.Nym);
         const int h = ihi - ilo + 1, nw = h * (jhi - jlo + 1);
-        double best = G.mdis_init;
-        int bw = INT32_MAX, bq = 0;
-        for (int w = lane; w < nw; w += 64) {
-            const int jj = jlo + w / h, ii = ilo + w % h;
-            const int q = pick[(size_t)(jj - 1) * G.Nym + (ii - 1)];
-            if (q == INT32_MAX) continue;
-            const double d = mdist(xm[q - 1], ym[q - 1], xa, ya);
-            if (d < best) {  // positions rise within a lane: strict < keeps the first
-                best = d;
-                bw = w;
-                bq = q;
+        Best best{G.mdis_init, INT32_MAX, 0};
+        for (int w0 = t; w0 < nw; w0 += U * SEARCH_THREADS) {
+            int q[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int w = w0 + u * SEARCH_THREADS;
+                q[u] = INT32_MAX;
+                if (w < nw) {
+                    const int jj = jlo + w / h, ii = ilo + w % h;
+                    q[u] = pick[(size_t)(jj - 1) * G.Nym + (ii - 1)];
+                }
+            }
+            double px[U], py[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                px[u] = py[u] = 0.0;
+                if (q[u] != INT32_MAX) {
+                    px[u] = xm[q[u] - 1];
+                    py[u] = ym[q[u] - 1];
+                }
+            }
+            // positions rise with u and w0: strict < keeps a thread's first
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (q[u] == INT32_MAX) continue;
+                const double d = mdist(px[u], py[u], xa, ya);
+                if (d < best.d) best = Best{d, w0 + u * SEARCH_THREADS, q[u]};
             }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            const double od = __shfl_xor(best, off);
-            const int ow = __shfl_xor(bw, off), oq = __shfl_xor(bq, off);
-            if (od < best || (od == best && ow < bw)) {
-                best = od;
-                bw = ow;
-                bq = oq;
+            const Best o{__shfl_xor(best.d, off), __shfl_xor(best.w, off),
+                         __shfl_xor(best.q, off)};
+            take_better(best, o);
+        }
+        if (lane == 0) wbest[wave] = best;
+        __syncthreads();
+        if (t == 0) {
+            Best b = wbest[0];
+            for (int k = 1; k < SEARCH_THREADS / 64; ++k) take_better(b, wbest[k]);
+            if (b.w != INT32_MAX) {
+                mdis[a] = b.d;
+                mnum[a] = -b.q;
+                flag[a] = 1;
             }
         }
-        if (lane == 0 && bw != INT32_MAX) {
-            mdis[a] = best;
-            mnum[a] = -bq;
-            flag[a] = 1;
-        }
+        __syncthreads();  // wbest is rewritten for the next anchor
     }
 }
 
@@ -378,7 +406,7 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     if ((rc = fill_i32(c, nempty, 0, 1))) return rc;
     hipLaunchKernelGGL(k_anchor_fill, ga, dim3(256), 0, c->stream, G, (const uint64_t*)dbits,
                        (const int32_t*)pick, mdis, mnum, flag, empty, nempty);
-    hipLaunchKernelGGL(k_anchor_search, dim3(1024), dim3(256), 0, c->stream, G,
+    hipLaunchKernelGGL(k_anchor_search, dim3(2048), dim3(SEARCH_THREADS), 0, c->stream, G,
                        mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), (const int32_t*)pick,
                        (const int32_t*)empty, (const int32_t*)nempty, mdis, mnum, flag);
     EREB_HIP(c, hipGetLastError());
