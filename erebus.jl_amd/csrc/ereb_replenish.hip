@@ -306,15 +306,11 @@ __device__ __forceinline__ double u01(uint64_t z) {
     return (double)(z >> 11) * 0x1p-53;
 }
 
-// pass 3: append the new markers (positions from the scan of the flags)
-__global__ __launch_bounds__(256) void k_anchor_append(MGridK G, MarkerFields F,
-                                                       const int32_t* __restrict__ mnum,
-                                                       const int32_t* __restrict__ flag,
-                                                       const int32_t* __restrict__ pos,
-                                                       int64_t M, int randomized, uint64_t seed) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= (int64_t)G.Nxm * G.Nym || !flag[a]) return;
-    const int64_t k = M + pos[a];
+// pass 3: append the new markers in column-major anchor order (E:5638-5683)
+// new marker k (0-based among the new ones) from anchor a
+__device__ __forceinline__ void append_one(const MGridK& G, const MarkerFields& F,
+                                           const int32_t* __restrict__ mnum, int64_t a,
+                                           int64_t k, int randomized, uint64_t seed) {
     const int64_t src = (int64_t)(-mnum[a]) - 1;
 #pragma unroll
     for (int f = 0; f < EREB_M_NFIELDS; ++f) {
@@ -333,6 +329,81 @@ __global__ __launch_bounds__(256) void k_anchor_append(MGridK G, MarkerFields F,
     ((double*)F.f[EREB_M_YM])[k] = y;
 }
 
+// positions from an exclusive scan of the flags of all anchors
+__global__ __launch_bounds__(256) void k_anchor_append(MGridK G, MarkerFields F,
+                                                       const int32_t* __restrict__ mnum,
+                                                       const int32_t* __restrict__ flag,
+                                                       const int32_t* __restrict__ pos,
+                                                       int64_t M, int randomized, uint64_t seed) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= (int64_t)G.Nxm * G.Nym || !flag[a]) return;
+    append_one(G, F, mnum, a, M + pos[a], randomized, seed);
+}
+
+// The new markers come only from the empty anchors.  When there are at most
+// NEW_MAX of them (every timestep but the first of a sparse start), one
+// workgroup sorts the empty-anchor list (filled in atomic order) into anchor
+// order and compacts the flagged ones into list[0..total); res[0] = total,
+// or -1 when the list is longer (the flags of all anchors are then scanned).
+constexpr int NEW_MAX = 4096;
+constexpr int NEW_THREADS = 1024;
+
+__global__ __launch_bounds__(NEW_THREADS) void k_new_order(const int32_t* __restrict__ empty,
+                                                           const int32_t* __restrict__ nempty,
+                                                           const int32_t* __restrict__ flag,
+                                                           int32_t* __restrict__ list,
+                                                           int32_t* __restrict__ res) {
+    constexpr int PER = NEW_MAX / NEW_THREADS;
+    __shared__ uint32_t s[NEW_MAX];
+    __shared__ int32_t part[NEW_THREADS];
+    const int t = threadIdx.x;
+    const int ne = *nempty;
+    if (ne > NEW_MAX) {  // block-uniform: no thread reaches a barrier below
+        if (t == 0) res[0] = -1;
+        return;
+    }
+    int n = 1;
+    while (n < ne) n <<= 1;
+    for (int i = t; i < n; i += NEW_THREADS) s[i] = i < ne ? (uint32_t)empty[i] : 0xFFFFFFFFu;
+    __syncthreads();
+    bitonic(s, n);
+    // thread t owns sorted entries [t*PER, t*PER + PER): flags, then an
+    // exclusive scan of the per-thread counts (Hillis-Steele over the block)
+    int f[PER], cnt = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = t * PER + u;
+        f[u] = i < ne ? (flag[s[i]] != 0) : 0;
+        cnt += f[u];
+    }
+    part[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < NEW_THREADS; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int k = part[t] - cnt;  // exclusive
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (f[u]) list[k++] = (int32_t)s[t * PER + u];
+    if (t == NEW_THREADS - 1) res[0] = part[t];
+}
+
+// positions from the compacted list; the count is read on the device
+// (res[0] = -1 appends nothing), so the launch needs no host round trip
+__global__ __launch_bounds__(256) void k_anchor_append_list(MGridK G, MarkerFields F,
+                                                            const int32_t* __restrict__ mnum,
+                                                            const int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ res,
+                                                            int64_t M, int randomized,
+                                                            uint64_t seed) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= res[0]) return;
+    append_one(G, F, mnum, list[k], M + k, randomized, seed);
+}
+
 int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, uint64_t seed,
                      int64_t* added) {
     const int64_t M = c->marknum;
@@ -343,7 +414,8 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     // scratch: distance bits, picks, mdis, mnum, flags, positions, the
     // empty-anchor list and its count, axes
     const size_t need = A * (sizeof(uint64_t) + sizeof(double) + 5 * sizeof(int32_t)) +
-                        2 * sizeof(int32_t) + sizeof(double) * (mg.Nxm + mg.Nym) + 256 * 10;
+                        2 * sizeof(int32_t) + sizeof(double) * (mg.Nxm + mg.Nym) +
+                        sizeof(int32_t) * (NEW_MAX + 1) + 256 * 12;
     if (need > c->rep_bytes) {
         hipFree(c->rep_pool);
         c->rep_pool = nullptr;
@@ -366,6 +438,8 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     int32_t* empty = (int32_t*)carve(sizeof(int32_t) * A);
     int32_t* nempty = (int32_t*)carve(sizeof(int32_t));
     double* axes = (double*)carve(sizeof(double) * (mg.Nxm + mg.Nym));
+    int32_t* list = (int32_t*)carve(sizeof(int32_t) * NEW_MAX);
+    int32_t* res = (int32_t*)carve(sizeof(int32_t));
     if ((size_t)(p - (char*)c->rep_pool) > c->rep_bytes)
         return fail(c, EREB_ERR_NOMEM, "replenish scratch carve overflow");
     EREB_HIP(c, hipMemcpyAsync(axes, mg.xxm, sizeof(double) * mg.Nxm, hipMemcpyHostToDevice,
@@ -409,24 +483,48 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     hipLaunchKernelGGL(k_anchor_search, dim3(2048), dim3(SEARCH_THREADS), 0, c->stream, G,
                        mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), (const int32_t*)pick,
                        (const int32_t*)empty, (const int32_t*)nempty, mdis, mnum, flag);
-    EREB_HIP(c, hipGetLastError());
-    if ((rc = scan_exclusive(c, flag, pos, A))) return rc;
-    int32_t total = 0;
-    EREB_HIP(c, hipMemcpyAsync(&total, pos + A, sizeof total, hipMemcpyDeviceToHost, c->stream));
-    EREB_HIP(c, hipStreamSynchronize(c->stream));
-    if (M + total > (int64_t)INT32_MAX) return fail(c, EREB_ERR_ARG, "marknum exceeds 2^31-1");
-    if (M + total > c->cap && (rc = reserve_markers(c, std::max<int64_t>(M + total,
-                                                                         c->cap + c->cap / 4))))
-        return rc;
-    if (total > 0) {
+    hipLaunchKernelGGL(k_new_order, dim3(1), dim3(NEW_THREADS), 0, c->stream,
+                       (const int32_t*)empty, (const int32_t*)nempty, (const int32_t*)flag, list,
+                       res);
+    auto fields = [&]() {
         MarkerFields F;
         for (int f = 0; f < EREB_M_NFIELDS; ++f) {
             F.f[f] = c->d_m[f];
             F.es[f] = (int)marker_es(c->precision, f);
         }
-        hipLaunchKernelGGL(k_anchor_append, ga, dim3(256), 0, c->stream, G, F,
-                           (const int32_t*)mnum, (const int32_t*)flag, (const int32_t*)pos, M,
-                           randomized, seed);
+        return F;
+    };
+    // with room for NEW_MAX more markers the listed append goes ahead of the
+    // host's read of the count (a longer list appends nothing there)
+    const bool roomy = M + NEW_MAX <= c->cap;
+    if (roomy)
+        hipLaunchKernelGGL(k_anchor_append_list, dim3(NEW_MAX / 256), dim3(256), 0, c->stream,
+                           G, fields(), (const int32_t*)mnum, (const int32_t*)list,
+                           (const int32_t*)res, M, randomized, seed);
+    EREB_HIP(c, hipGetLastError());
+    int32_t total = 0;
+    EREB_HIP(c, hipMemcpyAsync(&total, res, sizeof total, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    const bool listed = total >= 0;
+    if (!listed) {  // more empty anchors than one workgroup orders: scan all flags
+        if ((rc = scan_exclusive(c, flag, pos, A))) return rc;
+        EREB_HIP(c, hipMemcpyAsync(&total, pos + A, sizeof total, hipMemcpyDeviceToHost,
+                                   c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (M + total > (int64_t)INT32_MAX) return fail(c, EREB_ERR_ARG, "marknum exceeds 2^31-1");
+    if (M + total > c->cap && (rc = reserve_markers(c, std::max<int64_t>(M + total,
+                                                                         c->cap + c->cap / 4))))
+        return rc;
+    if (total > 0 && !(roomy && listed)) {
+        if (listed)
+            hipLaunchKernelGGL(k_anchor_append_list, dim3((unsigned)((total + 255) / 256)),
+                               dim3(256), 0, c->stream, G, fields(), (const int32_t*)mnum,
+                               (const int32_t*)list, (const int32_t*)res, M, randomized, seed);
+        else
+            hipLaunchKernelGGL(k_anchor_append, ga, dim3(256), 0, c->stream, G, fields(),
+                               (const int32_t*)mnum, (const int32_t*)flag, (const int32_t*)pos,
+                               M, randomized, seed);
         EREB_HIP(c, hipGetLastError());
     }
     prof_end(c, EREB_K_REPLENISH, ev);

@@ -188,26 +188,6 @@ __global__ __launch_bounds__(256) void k_fine_order(const int32_t* __restrict__ 
     }
 }
 
-// bitonic sort (ascending) of s[0..n), n a power of two, by the block
-__device__ __forceinline__ void bitonic(uint32_t* s, int n) {
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const uint32_t a = s[i], b = s[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        s[i] = b;
-                        s[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 __global__ __launch_bounds__(1024) void k_fine_order_big(const int32_t* __restrict__ start,
                                                          uint32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ big,
