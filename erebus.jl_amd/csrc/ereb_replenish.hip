@@ -307,26 +307,33 @@ __device__ __forceinline__ double u01(uint64_t z) {
 }
 
 // pass 3: append the new markers in column-major anchor order (E:5638-5683)
-// new marker k (0-based among the new ones) from anchor a
+// field f of new marker k (0-based among all markers) from anchor a: the
+// position of the anchor (jittered when randomized), every other field
+// copied from the anchor's marker
+__device__ __forceinline__ void append_field(const MGridK& G, const MarkerFields& F,
+                                             const int32_t* __restrict__ mnum, int64_t a,
+                                             int64_t k, int f, int randomized, uint64_t seed) {
+    if (f == EREB_M_XM || f == EREB_M_YM) {
+        const bool isx = f == EREB_M_XM;
+        const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
+        double v = isx ? G.xxm[j - 1] : G.yym[i - 1];
+        if (randomized)  // E:5662-5663 with a counter-based generator
+            v = v + (u01(seed ^ (2 * (uint64_t)a + (isx ? 0 : 1))) - 0.5) * (isx ? G.dxm : G.dym);
+        ((double*)F.f[f])[k] = v;
+        return;
+    }
+    const int64_t src = (int64_t)(-mnum[a]) - 1;
+    if (F.es[f] == 8)
+        ((double*)F.f[f])[k] = ((const double*)F.f[f])[src];
+    else
+        ((int32_t*)F.f[f])[k] = ((const int32_t*)F.f[f])[src];
+}
+
 __device__ __forceinline__ void append_one(const MGridK& G, const MarkerFields& F,
                                            const int32_t* __restrict__ mnum, int64_t a,
                                            int64_t k, int randomized, uint64_t seed) {
-    const int64_t src = (int64_t)(-mnum[a]) - 1;
 #pragma unroll
-    for (int f = 0; f < EREB_M_NFIELDS; ++f) {
-        if (F.es[f] == 8)
-            ((double*)F.f[f])[k] = ((const double*)F.f[f])[src];
-        else
-            ((int32_t*)F.f[f])[k] = ((const int32_t*)F.f[f])[src];
-    }
-    const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
-    double x = G.xxm[j - 1], y = G.yym[i - 1];
-    if (randomized) {  // E:5662-5663 with a counter-based generator
-        x = x + (u01(seed ^ (2 * (uint64_t)a)) - 0.5) * G.dxm;
-        y = y + (u01(seed ^ (2 * (uint64_t)a + 1)) - 0.5) * G.dym;
-    }
-    ((double*)F.f[EREB_M_XM])[k] = x;
-    ((double*)F.f[EREB_M_YM])[k] = y;
+    for (int f = 0; f < EREB_M_NFIELDS; ++f) append_field(G, F, mnum, a, k, f, randomized, seed);
 }
 
 // positions from an exclusive scan of the flags of all anchors
@@ -391,7 +398,8 @@ __global__ __launch_bounds__(NEW_THREADS) void k_new_order(const int32_t* __rest
     if (t == NEW_THREADS - 1) res[0] = part[t];
 }
 
-// positions from the compacted list; the count is read on the device
+// positions from the compacted list, one thread per (new marker, field): the
+// copies' loads are independent; the count is read on the device
 // (res[0] = -1 appends nothing), so the launch needs no host round trip
 __global__ __launch_bounds__(256) void k_anchor_append_list(MGridK G, MarkerFields F,
                                                             const int32_t* __restrict__ mnum,
@@ -399,9 +407,11 @@ __global__ __launch_bounds__(256) void k_anchor_append_list(MGridK G, MarkerFiel
                                                             const int32_t* __restrict__ res,
                                                             int64_t M, int randomized,
                                                             uint64_t seed) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = idx / EREB_M_NFIELDS;
+    const int f = (int)(idx % EREB_M_NFIELDS);
     if (k >= res[0]) return;
-    append_one(G, F, mnum, list[k], M + k, randomized, seed);
+    append_field(G, F, mnum, list[k], M + k, f, randomized, seed);
 }
 
 int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, uint64_t seed,
@@ -498,7 +508,9 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
     // host's read of the count (a longer list appends nothing there)
     const bool roomy = M + NEW_MAX <= c->cap;
     if (roomy)
-        hipLaunchKernelGGL(k_anchor_append_list, dim3(NEW_MAX / 256), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_anchor_append_list,
+                           dim3((unsigned)((NEW_MAX * EREB_M_NFIELDS + 255) / 256)), dim3(256),
+                           0, c->stream,
                            G, fields(), (const int32_t*)mnum, (const int32_t*)list,
                            (const int32_t*)res, M, randomized, seed);
     EREB_HIP(c, hipGetLastError());
@@ -518,7 +530,8 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
         return rc;
     if (total > 0 && !(roomy && listed)) {
         if (listed)
-            hipLaunchKernelGGL(k_anchor_append_list, dim3((unsigned)((total + 255) / 256)),
+            hipLaunchKernelGGL(k_anchor_append_list,
+                               dim3((unsigned)(((int64_t)total * EREB_M_NFIELDS + 255) / 256)),
                                dim3(256), 0, c->stream, G, fields(), (const int32_t*)mnum,
                                (const int32_t*)list, (const int32_t*)res, M, randomized, seed);
         else

@@ -105,11 +105,13 @@ def test_replenish_resident_workload_hole():
     _exact(mdis, mdis_o, "mdis")
 
 
-def test_replenish_after_advection_in_sort_order():
+@pytest.mark.parametrize("spare", [0, 10_000])
+def test_replenish_after_advection_in_sort_order(spare):
     """The timestep's order: a scatter sorts the markers, the advection moves
     them (here up to 0.6 cells), then the replenishment visits them in the
     now stale sort order.  The result equals the oracle's on the moved
-    markers, bit for bit."""
+    markers, bit for bit.  With spare capacity for 4,096 more markers the
+    append runs ahead of the host's read of the count."""
     g, mat = synth.workload_geometry("fp64", mode=1)
     mk = synth.make_markers(g, mat, seed=2)
     rng = np.random.default_rng(3)
@@ -123,6 +125,8 @@ def test_replenish_after_advection_in_sort_order():
     moved["ym"] = mk["ym"] + 0.6 * g.dy * (2 * rng.random(M) - 1)
     hrs, hrf = mat.radioactive_heating(2.25e6 * 365.25 * 24 * 3600)
     with ereb.Context(g, mat, ereb.FP64) as ctx:
+        if spare:
+            ctx.check(ctx.lib.ereb_ctx_reserve(ctx.h, M + spare))
         ctx.set_markers(mk)
         ctx.step_scatter(hrs, hrf, 1)
         # the scatter's key pass rewrote the marker properties
