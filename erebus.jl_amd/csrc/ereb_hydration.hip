@@ -376,9 +376,10 @@ __global__ __launch_bounds__(256) void k_pcombine(const double* __restrict__ par
                                                   int ntx, int Ny1, int Nx1, double* sum0,
                                                   double* sum1, double* sum2, double* out0,
                                                   double* out1) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)Ny1 * Nx1) return;
-    const int i0 = (int)(n % Ny1), j0 = (int)(n / Ny1);
+    // node counts stay far below 2^31: 32-bit index arithmetic
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n >= Ny1 * Nx1) return;
+    const int i0 = n % Ny1, j0 = n / Ny1;
     const int tylo = max(0, (i0 - (WIN - 1) + TILE - 1) / TILE), tyhi = min(nty - 1, i0 / TILE);
     const int txlo = max(0, (j0 - (WIN - 1) + TILE - 1) / TILE), txhi = min(ntx - 1, j0 / TILE);
     double a[S];

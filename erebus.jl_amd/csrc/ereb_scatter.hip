@@ -482,9 +482,10 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ part
     const int s = blockIdx.y;
     const bool basic = s < 8;
     const int rows = basic ? Ny : Ny + 1, cols = basic ? Nx : Nx + 1;
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)rows * cols) return;
-    const int i0 = (int)(n % rows), j0 = (int)(n / rows);
+    // node counts stay far below 2^31: 32-bit index arithmetic
+    const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n >= rows * cols) return;
+    const int i0 = n % rows, j0 = n / rows;
     const int tylo = max(0, (i0 - (WIN - 1) + TILE - 1) / TILE), tyhi = min(nty - 1, i0 / TILE);
     const int txlo = max(0, (j0 - (WIN - 1) + TILE - 1) / TILE), txhi = min(ntx - 1, j0 / TILE);
     double acc = 0.0;
