@@ -23,9 +23,14 @@ struct NodeArgs {
     double *RHO, *RHOCP, *ALPHA, *ALPHAF, *HR, *GGGP, *SXX0, *tk1, *PHI, *BETAPHI;
 };
 
+// blockIdx.y picks the grid (0 basic, 1 Vx, 2 Vy, 3 P): four times the
+// wavefronts of one thread per node doing all four, for a pass that is all
+// loads and stores
 __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (A.do_basic && k < A.nb) {
+    const int grid = blockIdx.y;
+    if (grid == 0) {
+        if (!A.do_basic || k >= A.nb) return;
         const double wt = A.WTSUM[k];
         if (wt > 0.0) {
             const double iw = 1.0 / wt;
@@ -39,9 +44,10 @@ __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
             A.TEN[k] = A.TENSUM[k] * iw;
             A.FRI[k] = A.FRISUM[k] * iw;
         }
+        return;
     }
     if (k >= A.ns) return;
-    if (A.do_vx) {
+    if (grid == 1 && A.do_vx) {
         const double wt = A.vxs[5][k];
         if (wt > 0.0) {
             const double iw = 1.0 / wt;
@@ -49,7 +55,7 @@ __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
             for (int q = 0; q < 5; ++q) A.vxo[q][k] = A.vxs[q][k] * iw;
         }
     }
-    if (A.do_vy) {
+    if (grid == 2 && A.do_vy) {
         const double wt = A.vys[5][k];
         if (wt > 0.0) {
             const double iw = 1.0 / wt;
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
             for (int q = 0; q < 5; ++q) A.vyo[q][k] = A.vys[q][k] * iw;
         }
     }
-    if (A.do_p) {
+    if (grid == 3 && A.do_p) {
         const double wt = A.WTPSUM[k];
         if (wt > 0.0) {
             const double iw = 1.0 / wt;
@@ -80,27 +86,24 @@ __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
 // apply_insulating_boundary_conditions! (E:2432-2449): rows 1 and Nyy copy
 // their inner neighbours for columns 2..Nxx-1, then columns 1 and Nxx copy
 // columns 2 and Nxx-1 over all rows (so corners take the updated rows).
-__global__ __launch_bounds__(256) void k_bc_rows(double* t, int Nyy, int Nxx) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;  // 1..Nxx-2 -> col j+1
-    if (j > Nxx - 2) return;
-    double* col = t + (size_t)j * Nyy;
-    col[0] = col[1];
-    col[Nyy - 1] = col[Nyy - 2];
-}
-
-__global__ __launch_bounds__(256) void k_bc_cols(double* t, int Nyy, int Nxx) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nyy) return;
-    t[i] = t[(size_t)Nyy + i];
-    t[(size_t)(Nxx - 1) * Nyy + i] = t[(size_t)(Nxx - 2) * Nyy + i];
+// One workgroup: the rows, a barrier (the workgroup's global stores are
+// visible to it after the barrier), then the columns.
+__global__ __launch_bounds__(1024) void k_bc(double* t, int Nyy, int Nxx) {
+    for (int j = 1 + threadIdx.x; j <= Nxx - 2; j += blockDim.x) {  // columns 2..Nxx-1
+        double* col = t + (size_t)j * Nyy;
+        col[0] = col[1];
+        col[Nyy - 1] = col[Nyy - 2];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < Nyy; i += blockDim.x) {
+        t[i] = t[(size_t)Nyy + i];
+        t[(size_t)(Nxx - 1) * Nyy + i] = t[(size_t)(Nxx - 2) * Nyy + i];
+    }
 }
 
 int launch_insulating_bc(ereb_ctx* c, double* t, int64_t Nyy, int64_t Nxx) {
     if (!(Nyy > 2 && Nxx > 2)) return EREB_OK;
-    hipLaunchKernelGGL(k_bc_rows, dim3((unsigned)((Nxx + 255) / 256)), dim3(256), 0, c->stream, t,
-                       (int)Nyy, (int)Nxx);
-    hipLaunchKernelGGL(k_bc_cols, dim3((unsigned)((Nyy + 255) / 256)), dim3(256), 0, c->stream, t,
-                       (int)Nyy, (int)Nxx);
+    hipLaunchKernelGGL(k_bc, dim3(1), dim3(1024), 0, c->stream, t, (int)Nyy, (int)Nxx);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -146,7 +149,8 @@ int launch_node_properties(ereb_ctx* c, bool basic, bool vx, bool vy, bool p, bo
     const int n = A.ns > A.nb ? A.ns : A.nb;
     hipEvent_t ev;
     prof_begin(c, EREB_K_NODES, &ev);
-    hipLaunchKernelGGL(k_node_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_node_props, dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0,
+                       c->stream, A);
     EREB_HIP(c, hipGetLastError());
     if (bc) {
         int rc = launch_insulating_bc(c, gptr(c, EREB_G_TK1), c->Ny1, c->Nx1);
