@@ -3,34 +3,37 @@ MI355X.
 
 Workload (default "fp64"): 281x281 basic nodes, 140 km box, 6x6 jittered
 markers per cell = 2,822,400 markers, planetesimal R = 50 km in sticky air,
-stream-function velocity at Courant 0.5, dt = 1e11 s, property mode 1.
+stream-function velocity at Courant 0.5, dt near 1e11 s, property mode 1.
 A step is one timestep of the transfer path in the reference loop order
 (src/Erebus.jl:6376-7053) with every input resident in HBM:
   tile sort with the marker properties in its key pass, scatter (E:6419-6517)
   node normalisation + insulating BC (E:6522-6604)
   Stokes-Darcy assembly to CSC (E:6680) and thermal assembly (E:6922)
   stress gather (E:6980), temperature gather (E:7004), RK4 advection (E:7036)
+dt and the heating rates change at every step as in the reference loop (dt
+control E:6619, E:6790, E:6933; calculate_radioactive_heating(timesum)
+E:6413): the step's HIP graph is captured once and replayed with them.
 The host LU solves are outside the step (they stay on the host in the
-reference's design and are not part of this path); the CSC matrices stay in
-HBM.  The grids that the solves and grid-only operations would produce
-(DT, DSXX, DSXY, tk2, pressures, gravity, ETAP/ETAPHI) are fixed synthetic
-fields (erebus.jl_amd/ereb/synth.py).
+reference's design); the CSC matrices stay in HBM.  The grids that the
+solves and grid-only operations would produce (DT, DSXX, DSXY, tk2,
+pressures, gravity, ETAP/ETAPHI) are fixed synthetic fields
+(erebus.jl_amd/ereb/synth.py).
 
 Prints one JSON line.  `value` is the Float64 form.  `roofline` is for the
-scatter kernel (the dominant one), measured live with HIP events on the
-library's stream.  `mixed` runs the same steps in the mixed-precision form
-on identical inputs and reports its throughput and the per-field error
-against the Float64 state after the same steps.  `adjoint` times the
-gradient of sum(tk1^2) after 20 steps (1,000,000 markers, 201x201 nodes,
-checkpoints every 5 steps) against the forward map and checks it against
-central finite differences on 64 markers.  `next_rows` gives the mean
-device time and algorithmic bandwidth of the transfers outside the step
-(viscosity, porosity, melt composition, thermochemical reaction, pressure
-back-trace, marker replenishment) on the same resident state.
-`cpu_baseline` times the
-single-threaded C restatement of the reference CPU path (the oracle) on a
-bounded sample of the Float64 workload.
-"""
+kernel class with the longest device time per step (HIP events on the
+library's stream), with the algorithmic bytes of the reference functions it
+implements; `rooflines` gives every class, `step_roofline` the whole step's
+algorithmic bytes against HBM peak.  `mixed` runs the same steps in the
+mixed-precision form on identical inputs and reports its throughput and the
+per-field error against the Float64 state after the same steps.  `adjoint`
+times the gradient of sum(tk1^2) after 20 steps (1,000,000 markers, 201x201
+nodes, checkpoints every 5 steps) against the forward map and checks it
+against central finite differences on 64 markers.  `next_rows` gives the
+mean device time and algorithmic bandwidth of the transfers outside the
+step.  `host_lu` times the host factorisation and solve of the assembled
+systems (scipy's SuperLU; the reference uses UMFPACK).  `cpu_baseline` times
+the C restatement of the reference CPU path (the oracle, -O3, OpenMP where
+the reference threads) on a bounded sample of the Float64 workload.
 from __future__ import annotations
 
 import argparse
@@ -51,14 +54,42 @@ METRIC = "marker-steps/s, 2.8M markers: FP64 vs mixed-precision; adjoint/forward
 HBM_PEAK_GBS = 8000.0
 START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 
-# algorithmic bytes of the scatter kernel (marker_to_*_nodes!): every marker
-# vector it reads once -- xm, ym and the 18 staged values (13 marker vectors
-# and the 7 properties that the sort's key pass evaluated), i.e. 20 Float64
-# vectors -- and the 30 SUM grids (Float64) written once
-SCATTER_MARKER_BYTES = 20 * 8
-# mixed form: the 5 state vectors (xm, ym, phim, sxxm, sxym) stay Float64,
-# the 8 other property inputs and the 7 properties are float
-SCATTER_MARKER_BYTES_MIXED = 5 * 8 + 15 * 4
+# Algorithmic bytes per kernel class: every Julia-visible input of the
+# reference functions a class implements read once and every output written
+# once (Float64 8 B, mixed-form properties 4 B, tm 4 B, YNY 1 B), per marker
+# and per node of the Ny1 x Nx1 grid, plus 8 B per stored CSC value.
+#   sort    compute_marker_properties! (E:700): r tm tkm phim X_W rhocp, w 7
+#   scatter marker_to_*_nodes! (E:1605-1863): r xm ym + 18 staged values,
+#           w 30 SUM grids
+#   combine the SUM grids' tile overlap (no reference counterpart: 0)
+#   nodes   compute_*_node_properties! + insulating tk1 (E:1970-2189, E:2432):
+#           r 30 SUM, w 27 grids + YNY
+#   rk4     update_marker_stress! (E:4354: r xm ym sxx sxy, w sxx sxy; r DSXX
+#           DSXY) + update_marker_temperature! (E:4782: r xm ym tkm, w tkm;
+#           r DT) + move_markers_rk4! (E:4990: r 7, w 5; r vx vy vxf vyf wyx
+#           tk2): 32 + 48 + 92 B per marker, 8 + 16 + 48 B per node
+#   asm_sd  assemble_hydromechanical_lse! (E:3089): r 20 grids, w R (6 per
+#           cell) and nzval
+#   asm_t   assemble_thermal_lse! (E:4526): r 8 grids, w RT and nzval
+ALG_BYTES = {
+    "fp64": {"sort": (92, 0), "scatter": (160, 240), "combine": (0, 0), "nodes": (0, 457),
+             "rk4": (172, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
+    "mixed": {"sort": (48, 0), "scatter": (88, 240), "combine": (0, 0), "nodes": (0, 457),
+              "rk4": (120, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
+}
+STEP_CLASSES = ["sort", "scatter", "combine", "nodes", "rk4", "asm_sd", "asm_t"]
+# kernel-name prefix of each class's main kernel in the rocprofv3 traffic summary
+TRAFFIC_KERNEL = {"scatter": "void ereb::k_scatter<", "rk4": "void ereb::k_rk4<",
+                  "sort": "void ereb::k_fine_count<"}
+
+
+def step_scalars(mat, k: int):
+    """dt and heating rates of timestep k: dt varies within [0.6, 1] x 1e11 s
+    as the reference's Courant and temperature-change limits make it vary,
+    and timesum advances by dt."""
+    dt = ereb.DT_LONGEST * (1.0 - 0.4 * ((k * 5) % 7) / 7.0)
+    hrs, hrf = mat.radioactive_heating(START_TIME + k * ereb.DT_LONGEST)
+    return dt, hrs, hrf
 
 
 def log(*a):
@@ -103,72 +134,40 @@ def setup(ctx: ereb.Context, mk, grids):
     ctx.sync()
 
 
-def step(ctx, hrs, hrf, mode, dt, timestep):
+def step(ctx, mat, mode, timestep):
     """One transfer step: scatter, node properties, Stokes-Darcy and thermal
     assembly, stress and temperature gathers, RK4 (ereb_step_transfer, a
-    HIP graph replay after the first step; eager while profiling)."""
+    HIP graph replay after the first step; eager while profiling), with
+    this timestep's dt and heating rates."""
+    dt, hrs, hrf = step_scalars(mat, timestep)
     ctx.step_transfer(hrs, hrf, mode, dt, timestep)
 
 
-def cpu_baseline(g, mat, mk, steps: int, budget_s: float = 25.0):
-    """The oracle (single-threaded C restatement of the reference CPU path,
-    -O2, no FMA) on the same workload, in the same step order."""
+def cpu_baseline(g, mat, mk, grids, hrs, hrf, steps: int, budget_s: float = 25.0):
+    """The oracle (C restatement of the reference CPU path) built at -O3 with
+    OpenMP in the gathers, where the reference has @threads, timed inside C
+    over whole timesteps of the same workload in the same step order (no
+    Python in the timed loop; ereb_cpu_step.c)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
+    import oracle as orc_mod
 
-    orc = Oracle(g, mat)
     mode = mat.marker_property_mode
-    dt = ereb.DT_LONGEST
-    hrs, hrf = mat.radioactive_heating(START_TIME)
-    vx, vy = synth.stream_velocity(g)
-    wyx = synth.rotation_rate(g, vx, vy)
-    S = g.stag_shape
     mk = {k: np.array(v, copy=True) for k, v in mk.items()}
-    # node fields for the assemblies, from one untimed pass
-    props, sums = orc.marker_to_nodes(mk, hrs, hrf, mode)
-    B = g.basic_shape
-    outs_b, _ = orc.basic_node_properties(sums[:8], [np.zeros(B, order="F")] * 7,
-                                          np.zeros(B, np.uint8, order="F"))
-    ETA = outs_b[1]
-    outs_p = orc.p_node_properties(sums[20:30], [np.zeros(S, order="F")] * 10)
-    PHI = np.maximum(outs_p[8], 1e-4)
-    ETAP, ETAPHI = synth.bulk_viscosity(ETA, PHI)
-    gx, gy = synth.disk_gravity(g)
-    DT = synth.smooth_field(g, S, 0.05, 2, 1)
-    tk2 = np.asfortranarray(outs_p[7] + DT)
-    DSXX = synth.smooth_field(g, S, 1e3, 3, 2)
-    DSXY = synth.smooth_field(g, B, 1e3, 2, 3)
-    z = np.zeros(S, order="F")
-    t0 = time.perf_counter()
-    done = 0
-    for s in range(steps):
-        props, sums = orc.marker_to_nodes(mk, hrs, hrf, mode)
-        mk.update(props)
-        ob, _ = orc.basic_node_properties(sums[:8], [np.zeros(B, order="F")] * 7,
-                                          np.zeros(B, np.uint8, order="F"))
-        ovx = orc.v_node_properties(sums[8:14], [z] * 5)
-        ovy = orc.v_node_properties(sums[14:20], [z] * 5)
-        op = orc.p_node_properties(sums[20:30], [z] * 10)
-        tk1 = orc.insulating_bc(op[7])
-        orc.assemble_hydromechanical([ob[1], ETAP, ob[2], op[5], ob[3], op[6], ovx[0], ovy[0],
-                                      ovx[1], ovy[1], ovx[4], ovy[4], ETAPHI, op[9], op[8],
-                                      gx, gy, z, z, z], dt)
-        orc.assemble_thermal([tk1, op[1], ovx[2], ovy[2], op[4], z, z, z], dt)
-        mk["sxxm"], mk["sxym"] = orc.update_marker_stress(mk["xm"], mk["ym"], mk["sxxm"],
-                                                          mk["sxym"], DSXX, DSXY)
-        mk["tkm"] = orc.update_marker_temperature(mk["xm"], mk["ym"], mk["tkm"], DT, tk2, 2)
-        r = orc.move_markers_rk4(mk["xm"], mk["ym"], mk["tm"], mk["tkm"], mk["phim"],
-                                 mk["sxym"], mk["sxxm"], vx, vy, vx, vy, wyx, tk2, dt, mode)
-        mk["xm"], mk["ym"], mk["tkm"], mk["sxym"], mk["sxxm"] = r
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
+    # the marker properties of the first step, untimed (the marker vectors
+    # they are written to must exist as Float64 arrays)
+    props, _ = orc_mod.Oracle(g, mat).marker_to_nodes(mk, hrs, hrf, mode)
+    mk.update(props)
+    for f in orc_mod.BENCH_MARKERS:
+        mk[f] = np.ascontiguousarray(mk[f], dtype=np.float64)
+    done, el, threads = orc_mod.bench_transfer(g, mat, mk, grids, hrs, hrf, mode,
+                                               ereb.DT_LONGEST, steps, budget_s)
     M = mk["xm"].size
-    return {"value": M * done / el, "unit": "marker-steps/s", "cores": 1, "kind": "port",
+    return {"value": M * done / el, "unit": "marker-steps/s", "cores": threads, "kind": "port",
             "sample": f"{done} full timestep(s) of the {M:,}-marker workload, same step "
-                      f"order, single-threaded C restatement of the reference CPU path "
-                      f"(-O2, no FMA), {el:.1f} s"}
+                      f"order, C restatement of the reference CPU path (-O3, no FMA, OpenMP "
+                      f"with {threads} threads in the two gathers where the reference has "
+                      f"@threads; scatter, assemblies and RK4 single-threaded as in the "
+                      f"reference), timed inside C, {el:.1f} s"}
 
 
 def run_form(g, mat, mk, grids, hrs, hrf, precision, steps, warmup):
@@ -178,24 +177,25 @@ def run_form(g, mat, mk, grids, hrs, hrf, precision, steps, warmup):
     (state after warmup + 2 x steps), marker-steps/s and ms per step of the
     timed steps, and the mean duration of each kernel class."""
     mode = mat.marker_property_mode
-    dt = ereb.DT_LONGEST
     ctx = ereb.Context(g, mat, precision)
     setup(ctx, mk, grids)
     ts = 2
     for _ in range(warmup):
-        step(ctx, hrs, hrf, mode, dt, ts)
+        step(ctx, mat, mode, ts)
         ts += 1
     ctx.sync()
+    cap0, _ = ctx.graph_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
-        step(ctx, hrs, hrf, mode, dt, ts)
+        step(ctx, mat, mode, ts)
         ts += 1
     ctx.sync()
     el = time.perf_counter() - t0
+    recaptures = ctx.graph_stats()[0] - cap0
     ctx.profile(True)
     ctx.profile_reset()
     for _ in range(steps):
-        step(ctx, hrs, hrf, mode, dt, ts)
+        step(ctx, mat, mode, ts)
         ts += 1
     ctx.sync()
     kern = {}
@@ -204,7 +204,7 @@ def run_form(g, mat, mk, grids, hrs, hrf, precision, steps, warmup):
         kern[k] = round(ms / max(n, 1), 5)
     ctx.profile(False)
     M = mk["xm"].size
-    return ctx, M * steps / el, el * 1e3 / steps, kern
+    return ctx, M * steps / el, el * 1e3 / steps, kern, recaptures
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
@@ -244,22 +244,51 @@ def _finite(x: float, nd: int = 1):
     return round(x, nd) if x == x else None
 
 
-def scatter_roofline(g, M, kern_ms, marker_bytes, kernel_name, form, workload):
-    sums = 8 * g.Nx * g.Ny + 22 * g.Nx1 * g.Ny1  # elements of the 30 SUM grids
-    sc_bytes = M * marker_bytes + sums * 8
-    achieved = sc_bytes / (kern_ms * 1e-3) / 1e9
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # summary (scaled by a plain copy of known size), when one exists
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
-    if os.path.exists(pmc):
-        for k, v in json.load(open(pmc)).get("kernels", {}).items():
-            if k.startswith(f"void ereb::k_scatter<{kernel_name.split('<')[1]}"):
-                traffic = v["bytes_per_launch"]
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": f"{kernel_name} (marker->node scatter, one launch per step)",
-            "algorithmic_bytes": sc_bytes}
+def _traffic(workload: str, form: str, kclass: str):
+    """HBM bytes per launch of a class's main kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by what the counter
+    reads for a plain copy of known size), when one exists."""
+    pre = TRAFFIC_KERNEL.get(kclass)
+    pmc = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{form}.json")
+    if not pre or not os.path.exists(pmc):
+        return None
+    for k, v in json.load(open(pmc)).get("kernels", {}).items():
+        if k.startswith(pre):
+            return v["bytes_per_launch"]
+    return None
+
+
+def class_bytes(form, kclass, M, N, nnz):
+    bm, bn = ALG_BYTES[form][kclass]
+    extra = {"asm_sd": 8 * nnz[0], "asm_t": 8 * nnz[1]}.get(kclass, 0)
+    return M * bm + N * bn + extra
+
+
+def rooflines(g, M, kern, form, workload, nnz, ms_per_step):
+    """Per-class rooflines (algorithmic bytes over the class's mean device
+    time per step), the dominant class first, and the whole step's."""
+    N = g.Nx1 * g.Ny1
+    out = {}
+    for k in STEP_CLASSES:
+        ms = kern.get(k, 0.0)
+        if ms <= 0:
+            continue
+        nb = class_bytes(form, k, M, N, nnz)
+        if nb == 0:
+            continue
+        ach = nb / (ms * 1e-3) / 1e9
+        out[k] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                  "traffic": _traffic(workload, form, k), "ms": ms,
+                  "algorithmic_bytes": nb}
+    dom = max(STEP_CLASSES, key=lambda k: kern.get(k, 0.0))
+    tot = sum(class_bytes(form, k, M, N, nnz) for k in STEP_CLASSES)
+    step_gbs = tot / (ms_per_step * 1e-3) / 1e9
+    step = {"algorithmic_bytes": tot, "ms_per_step": ms_per_step,
+            "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(step_gbs / HBM_PEAK_GBS, 4), "target_frac": 0.60,
+            "kernel_ms_sum": round(sum(kern.get(k, 0.0) for k in STEP_CLASSES), 4)}
+    return dom, out, step
 
 
 # algorithmic bytes per marker of the transfers outside the timed step
@@ -464,6 +493,31 @@ def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str =
     }
 
 
+def host_lu_leg(systems):
+    """Host factorisation and solve of the last assembled systems, as the
+    reference solves them on the host (UMFPACK through LinearSolve, E:6719-6749;
+    LT \\ RT, E:6925).  UMFPACK is not installable here: scipy's SuperLU
+    (splu, COLAMD ordering) stands in, labelled as such."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    out = {"solver": "SuperLU (scipy.sparse.linalg.splu, COLAMD), not UMFPACK",
+           "threads": 1}
+    for name, (cp, rv, nz, R) in systems.items():
+        n = R.size
+        A = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(n, n))
+        t0 = time.perf_counter()
+        lu = sla.splu(A, permc_spec="COLAMD")
+        t1 = time.perf_counter()
+        x = lu.solve(R)
+        t2 = time.perf_counter()
+        res = float(np.linalg.norm(A @ x - R) / max(np.linalg.norm(R), 1e-300))
+        out[name] = {"n": n, "nnz": int(A.nnz), "factor_s": round(t1 - t0, 3),
+                     "solve_s": round(t2 - t1, 4), "rel_residual": float(f"{res:.3e}")}
+        log(f"[bench] host LU {name}: n={n:,} factor {t1 - t0:.2f} s, solve {t2 - t1:.3f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -475,6 +529,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-next-rows", action="store_true")
+    ap.add_argument("--no-host-lu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -488,8 +543,11 @@ def main():
     M = mk["xm"].size
     grids, hrs, hrf = make_inputs(g, mat, mk)
     log(f"[bench] {M:,} markers; FP64 form: warmup {a.warmup}, {a.steps} timed steps")
-    ctx64, value, ms, kern = run_form(g, mat, mk, grids, hrs, hrf, ereb.FP64, a.steps, a.warmup)
+    ctx64, value, ms, kern, recap = run_form(g, mat, mk, grids, hrs, hrf, ereb.FP64, a.steps,
+                                             a.warmup)
     log(f"[bench] FP64 {ms:.3f} ms/step")
+    nnz = (ctx64.csc_size(ereb.SYS_STOKES_DARCY)[1], ctx64.csc_size(ereb.SYS_THERMAL)[1])
+    dom, roofs, step_roof = rooflines(g, M, kern, "fp64", a.workload, nnz, ms)
     res = {
         "metric": METRIC,
         "value": value,
@@ -506,26 +564,31 @@ def main():
         "config": {"workload": f"{a.workload}: {g.Nx}x{g.Ny} basic nodes, "
                                f"{g.Nxmc}x{g.Nymc} jittered markers/cell ({M:,} markers), "
                                f"planetesimal R={g.rplanet / 1e3:.0f} km in 140 km box, "
-                               f"stream-function velocity Courant 0.5, dt=1e11 s, mode {mode}",
-                   "markers": M, "grid": [g.Ny, g.Nx], "form": "fp64"},
-        "roofline": dict(scatter_roofline(g, M, kern["scatter"], SCATTER_MARKER_BYTES,
-                                          "k_scatter<double>", "fp64", a.workload),
+                               f"stream-function velocity Courant 0.5, dt 0.6-1e11 s varying "
+                               f"per step, mode {mode}",
+                   "markers": M, "grid": [g.Ny, g.Nx], "form": "fp64",
+                   "graph_recaptures_in_timed_steps": recap},
+        "roofline": dict(roofs[dom], kernel_class=dom,
                          measured_copy_GBps=_finite(copy_bandwidth())),
+        "rooflines": roofs,
+        "step_roofline": step_roof,
         "kernel_ms": kern,
     }
     if not a.no_mixed:
         log("[bench] mixed-precision form")
-        ctxm, vm, msm, kernm = run_form(g, mat, mk, grids, hrs, hrf, ereb.MIXED, a.steps,
-                                        a.warmup)
+        ctxm, vm, msm, kernm, recapm = run_form(g, mat, mk, grids, hrs, hrf, ereb.MIXED,
+                                                a.steps, a.warmup)
         errs = precision.field_errors(ctxm, ctx64)
         wk, wv = precision.worst(errs)
         nodal = set(ereb.SUM_FIELDS + precision.NODE_FIELDS)
         wn, wnv = precision.worst(errs, nodal)
+        nnzm = (ctxm.csc_size(ereb.SYS_STOKES_DARCY)[1], ctxm.csc_size(ereb.SYS_THERMAL)[1])
+        domm, roofsm, steproofm = rooflines(g, M, kernm, "mixed", a.workload, nnzm, msm)
         res["mixed"] = {
             "value": vm, "ms_per_step": msm, "speedup_vs_fp64": vm / value,
             "dtype": "f32 properties/weights/interpolation, f64 state/normalisation/coefficients",
-            "roofline": scatter_roofline(g, M, kernm["scatter"], SCATTER_MARKER_BYTES_MIXED,
-                                         "k_scatter<float>", "mixed", a.workload),
+            "roofline": dict(roofsm[domm], kernel_class=domm), "rooflines": roofsm,
+            "step_roofline": steproofm, "graph_recaptures_in_timed_steps": recapm,
             "kernel_ms": kernm,
             "pattern_difference": {
                 "stokes_darcy": precision.pattern_difference(ctxm.csc(0), ctx64.csc(0)),
@@ -543,6 +606,8 @@ def main():
         ctxm.close()
     log("[bench] solver hand-off")
     res["handoff"] = handoff_leg(ctx64)
+    systems = {"thermal": ctx64.csc(ereb.SYS_THERMAL),
+               "stokes_darcy": ctx64.csc(ereb.SYS_STOKES_DARCY)}
     if not a.no_next_rows:
         log("[bench] remaining transfer rows")
         res["next_rows"] = next_rows_leg(ctx64, g, mat, hrs, hrf)
@@ -555,8 +620,11 @@ def main():
         res["adjoint"] = adjoint_leg()
         log(f"[bench] adjoint/forward {res['adjoint']['adjoint_forward_ratio']:.2f}")
     if not a.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle, 1 thread)")
-        res["cpu_baseline"] = cpu_baseline(g, mat, mk, a.cpu_steps)
+        log("[bench] CPU baseline (oracle at -O3, OpenMP in the gathers)")
+        res["cpu_baseline"] = cpu_baseline(g, mat, mk, grids, hrs, hrf, a.cpu_steps)
+    if not a.no_host_lu:
+        log("[bench] host LU of the assembled systems (SuperLU)")
+        res["host_lu"] = host_lu_leg(systems)
     print(json.dumps(res), flush=True)
 
 

@@ -339,6 +339,10 @@ static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int6
     return EREB_OK;
 }
 
+static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t every, int64_t nseg,
+                        double dt, int md, const double* hrsolidm, const double* hrfluidm,
+                        double* loss, double* tk1_out, double* grad_tkm, double* grad_phim);
+
 }  // namespace ereb
 
 using namespace ereb;
@@ -361,15 +365,39 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
     if (M <= 0) return fail(c, EREB_ERR_STATE, "no markers");
     const int md = (int)mode;
     const int64_t nseg = (nsteps + every - 1) / every;
-    const size_t MB = sizeof(double) * (size_t)M;
     const size_t gsz = grid_doubles(c);
-    const size_t GB = sizeof(double) * gsz;
     AdjBuffers B;
     int rc;
     if ((rc = adj_buffers(c, B, grad, nseg, std::min(every, std::max<int64_t>(nsteps, 1)), gsz)))
         return rc;
-    // the markers are restored at the end
+    // the markers are restored at the end, on every exit path once saved
     if ((rc = state_copy(c, B.save, 5, true))) return rc;
+    rc = adjoint_body(c, B, nsteps, every, nseg, dt, md, hrsolidm, hrfluidm, loss, tk1_out,
+                      grad_tkm, grad_phim);
+    const std::string err = c->err;
+    const int rc_restore = state_copy(c, B.save, 5, false);
+    ++c->pos_gen;  // the last sort was of moved positions
+    if (rc) {
+        c->err = err;  // the first failure is the one reported
+        hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    if (rc_restore) return rc_restore;
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    return check_flags(c);
+}
+
+namespace ereb {
+
+static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t every, int64_t nseg,
+                        double dt, int md, const double* hrsolidm, const double* hrfluidm,
+                        double* loss, double* tk1_out, double* grad_tkm, double* grad_phim) {
+    const bool grad = grad_tkm || grad_phim;
+    const int64_t M = c->marknum;
+    const size_t MB = sizeof(double) * (size_t)M;
+    const size_t gsz = grid_doubles(c);
+    const size_t GB = sizeof(double) * gsz;
+    int rc;
     // record step k of a segment starting at k0: start state before it,
     // grids and back-trace end points from it
     auto run_step = [&](int64_t k, int64_t k0, bool record) -> int {
@@ -465,8 +493,7 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
         if (grad_phim)
             EREB_HIP(c, hipMemcpyAsync(grad_phim, B.gP, MB, hipMemcpyDeviceToHost, c->stream));
     }
-    if ((rc = state_copy(c, B.save, 5, false))) return rc;
-    ++c->pos_gen;  // the last sort was of moved positions
-    EREB_HIP(c, hipStreamSynchronize(c->stream));
-    return check_flags(c);
+    return EREB_OK;
 }
+
+}  // namespace ereb

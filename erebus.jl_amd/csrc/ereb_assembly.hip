@@ -321,12 +321,15 @@ __host__ __device__ __forceinline__ void stokes_eq(const AsmArgs& A, int e, int 
 
 // 1. rows: thread per P cell, blockIdx.y = equation (wave-uniform)
 template <int SYS>
-__global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A, const SlotTables* __restrict__ T,
+__global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A0, const StepScalars* __restrict__ S,
+                                                  const SlotTables* __restrict__ T,
                                                   uint16_t* __restrict__ pres_out,
                                                   double* __restrict__ slotv,
                                                   int32_t* __restrict__ flags) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.Nx1 * A.Ny1) return;
+    if (k >= A0.Nx1 * A0.Ny1) return;
+    AsmArgs A = A0;
+    A.dt = S->dt;  // the step's dt from the device block
     const int e = blockIdx.y;
     const int i = k % A.Ny1 + 1, j = k / A.Ny1 + 1;
     const int64_t ncell = (int64_t)A.Nx1 * A.Ny1;
@@ -581,13 +584,14 @@ static int assemble(ereb_ctx* c, double dt) {
     const int64_t cap = ncell * s.nstot;  // every slot present
     int rc = ensure_sys(c, SYS, n, cap, ncell);
     if (rc) return rc;
+    if ((rc = set_dt(c, dt))) return rc;
     AsmArgs A = asm_args(c, dt, s);
     const SlotTables* T = (const SlotTables*)s.tables;
     hipEvent_t ev;
     prof_begin(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, &ev);
     const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
-    hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, T, s.pres, s.slotv,
-                       c->d_flags);
+    hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, c->d_scal, T, s.pres,
+                       s.slotv, c->d_flags);
     const dim3 cgrid((unsigned)((n + 255) / 256));
     hipLaunchKernelGGL(k_asm_count, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
                        s.colcnt);

@@ -117,6 +117,40 @@ int ensure_scan(ereb_ctx* c, int64_t n) {
     return EREB_OK;
 }
 
+__global__ void k_set_scalars(StepScalars* __restrict__ d, StepScalars v) {
+    if (threadIdx.x == 0) *d = v;
+}
+
+int scal_sync(ereb_ctx* c) {
+    if (c->scal_written && std::memcmp(&c->scal, &c->scal_dev, sizeof c->scal) == 0)
+        return EREB_OK;
+    if (c->capturing)
+        return fail(c, EREB_ERR_STATE, "step scalars changed inside a captured step");
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c->stream, c->d_scal, c->scal);
+    EREB_HIP(c, hipGetLastError());
+    c->scal_dev = c->scal;
+    c->scal_written = true;
+    return EREB_OK;
+}
+
+int set_heating(ereb_ctx* c, const double* hrs, const double* hrf) {
+    for (int k = 0; k < 3; ++k) {
+        c->scal.hrs[k] = hrs[k];
+        c->scal.hrf[k] = hrf[k];
+    }
+    return scal_sync(c);
+}
+
+int set_dt(ereb_ctx* c, double dt) {
+    c->scal.dt = dt;
+    return scal_sync(c);
+}
+
+int set_first(ereb_ctx* c, bool first) {
+    c->scal.first = first ? 1 : 0;
+    return scal_sync(c);
+}
+
 int check_flags(ereb_ctx* c) {
     int32_t f[4];
     EREB_HIP(c, hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost, c->stream));
@@ -302,6 +336,7 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
                     exact_axis(g->xp, Nx1, g->dx) && exact_axis(g->yp, Ny1, g->dy);
     CK(hipMalloc(&c->d_mat, sizeof(ereb_materials)));
     CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
+    CK(hipMalloc(&c->d_scal, sizeof(StepScalars)));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
     CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
     // grids, 256-byte aligned, after 32 guard elements (the RK4 stencils read
@@ -349,6 +384,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     for (int f = 0; f < EREB_M_NFIELDS; ++f) hipFree(c->d_m[f]);
     hipFree(c->d_axes);
     hipFree(c->d_mat);
+    hipFree(c->d_scal);
     if (c->step_exec) hipGraphExecDestroy(c->step_exec);
     hipFree(c->adj_pool);
     hipFree(c->rep_pool);
@@ -548,7 +584,8 @@ int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrflui
 static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
                              double dt, int64_t timestep) {
     int rc;
-    if ((rc = ereb_step_scatter(c, hrs, hrf, mode))) return rc;
+    if ((rc = launch_scatter(c, hrs, hrf, (int)mode))) return rc;
+    if ((rc = launch_combine(c))) return rc;
     if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
     if ((rc = launch_assemble_stokes(c, dt))) return rc;
     if ((rc = launch_assemble_thermal(c, dt))) return rc;
@@ -556,23 +593,20 @@ static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, 
     return launch_gathers_rk4(c, dt, (int)mode, timestep);
 }
 
-static StepKey step_key(const ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
-                        double dt, int64_t timestep) {
+// dt, the heating rates and the first-timestep flag are not part of the key:
+// the kernels read them from the context's device block (StepScalars),
+// written before each replay
+static StepKey step_key(const ereb_ctx* c, int64_t mode) {
     StepKey k;
     std::memset(&k, 0, sizeof k);  // padding included: keys are compared bytewise
     k.marknum = c->marknum;
     k.cap = c->cap;
     k.mode = mode;
     k.precision = c->precision;
-    k.first = timestep == 1;
-    k.dt = dt;
-    for (int q = 0; q < 3; ++q) {
-        k.hrs[q] = hrs[q];
-        k.hrf[q] = hrf[q];
-    }
     int n = 0;
     for (int f = 0; f < EREB_M_NFIELDS; ++f) k.bufs[n++] = c->d_m[f];
-    const void* more[] = {c->d_key, c->d_perm, c->d_tmp, c->d_scan, c->d_grids, c->d_part};
+    const void* more[] = {c->d_key, c->d_perm, c->d_tmp, c->d_scan, c->d_grids, c->d_part,
+                          c->d_scal};
     for (const void* p : more) k.bufs[n++] = p;
     for (const Sys& s : c->sys) {
         const void* sb[] = {s.colcnt, s.colptr, s.pres, s.slotv, s.rowval, s.nzval};
@@ -586,6 +620,16 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     if (!c || !hrsolidm || !hrfluidm) return EREB_ERR_ARG;
     int rc = valid_mode(c, mode);
     if (rc) return rc;
+    if ((rc = reserve(c, std::max<int64_t>(c->marknum, 1)))) return rc;
+    // the step's scalars go to the device block before the launches (eager)
+    // or the replay, outside any capture
+    for (int q = 0; q < 3; ++q) {
+        c->scal.hrs[q] = hrsolidm[q];
+        c->scal.hrf[q] = hrfluidm[q];
+    }
+    c->scal.dt = dt;
+    c->scal.first = timestep == 1 ? 1 : 0;
+    if ((rc = scal_sync(c))) return rc;
     // eager: when asked, while profiling (events around every launch), and
     // for the first step (it makes the allocations a capture must not)
     if (!c->use_graphs || c->prof || !c->step_warm) {
@@ -593,7 +637,7 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
         if (!rc) c->step_warm = true;
         return rc;
     }
-    const StepKey k = step_key(c, hrsolidm, hrfluidm, mode, dt, timestep);
+    const StepKey k = step_key(c, mode);
     if (!c->step_exec || std::memcmp(&k, &c->step_key, sizeof k) != 0) {
         if (c->step_exec) {
             hipGraphExecDestroy(c->step_exec);
@@ -602,7 +646,9 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
         const uint64_t pos0 = c->pos_gen;
         hipGraph_t g = nullptr;
         EREB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         rc = transfer_launches(c, hrsolidm, hrfluidm, mode, dt, timestep);
+        c->capturing = false;
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc || e != hipSuccess) {
             if (g) hipGraphDestroy(g);
@@ -616,8 +662,10 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
             return hipfail(c, ei, "hipGraphInstantiate");
         }
         c->step_key = k;
+        ++c->step_captures;
     }
     EREB_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    ++c->step_replays;
     // host state of the replayed step, as the eager launches leave it: the
     // sort is of the positions before RK4, which then moved them
     c->have_sort = true;
@@ -631,6 +679,13 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
 int ereb_ctx_set_graphs(ereb_ctx* c, int on) {
     if (!c) return EREB_ERR_ARG;
     c->use_graphs = on != 0;
+    return EREB_OK;
+}
+
+int ereb_step_graph_stats(const ereb_ctx* c, int64_t* captures, int64_t* replays) {
+    if (!c) return EREB_ERR_ARG;
+    if (captures) *captures = c->step_captures;
+    if (replays) *replays = c->step_replays;
     return EREB_OK;
 }
 

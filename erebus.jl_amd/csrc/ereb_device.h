@@ -325,6 +325,18 @@ __device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t,
     return p;
 }
 
+// Per-step scalars read by the kernels from device memory rather than taken
+// as launch arguments, so that a captured step graph stays valid while they
+// change from step to step: dt (E:6619, E:6790, E:6933), the radiogenic
+// heating rates of calculate_radioactive_heating(timesum) (E:6413-6414,
+// E:1336-1357) and whether the temperature gather is the first timestep's.
+struct StepScalars {
+    double dt;
+    double hrs[3], hrf[3];
+    int32_t first;
+    int32_t pad;
+};
+
 // compute_marker_properties! (E:700-750) for marker m in storage order:
 // reads tm, tkm, phim, XWsolidm0 (rocks) or the stored rhototal, rhocptotal,
 // etatotal, hrtotal, ktotal (air, which keeps them) and writes the property
@@ -333,7 +345,7 @@ __device__ __forceinline__ PropsF marker_props_f(const ereb_materials& M, int t,
 template <class R>
 struct PropArgs {
     const ereb_materials* __restrict__ M;
-    double hrs[3], hrf[3];
+    const StepScalars* __restrict__ S;  // hrs, hrf
     int mode;
     const int32_t* __restrict__ tm;
     const double *__restrict__ tkm, *__restrict__ phim;
@@ -350,6 +362,8 @@ __device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
         st = Props{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0, 0.0};
     else
         xw = XW[m];
+    const double hrs[3] = {S->hrs[0], S->hrs[1], S->hrs[2]};
+    const double hrf[3] = {S->hrf[0], S->hrf[1], S->hrf[2]};
     const Props p = marker_props(*M, t, tkm[m], phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;
@@ -370,6 +384,8 @@ __device__ __forceinline__ void PropArgs<float>::apply(int64_t m) const {
         st = PropsF{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0f, 0.0f};
     else
         xw = XW[m];
+    const double hrs[3] = {S->hrs[0], S->hrs[1], S->hrs[2]};
+    const double hrf[3] = {S->hrf[0], S->hrf[1], S->hrf[2]};
     const PropsF p = marker_props_f(*M, t, tkm[m], (float)phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;

@@ -51,9 +51,8 @@ struct ProfPair {
 // all of them are unchanged.
 struct StepKey {
     int64_t marknum, cap, mode;
-    int precision, first;
-    double dt, hrs[3], hrf[3];
-    const void* bufs[EREB_M_NFIELDS + 18];
+    int precision, pad;
+    const void* bufs[EREB_M_NFIELDS + 19];
 };
 
 }  // namespace ereb
@@ -67,6 +66,13 @@ struct ereb_ctx {
     double* d_axes = nullptr;  // x (Nx) | y (Ny) | xp (Nx1) | yp (Ny1)
     ereb_materials mat{};
     ereb_materials* d_mat = nullptr;  // device copy, read by kernels through a pointer
+    // per-step scalars (dt, heating rates, first-timestep flag): the kernels
+    // read the device block; `scal` is what the host wants there, `scal_dev`
+    // what was last written (a write is a one-thread kernel launch)
+    ereb::StepScalars* d_scal = nullptr;
+    ereb::StepScalars scal{}, scal_dev{};
+    bool scal_written = false;
+    bool capturing = false;  // a step is being captured into a HIP graph
 
     int64_t marknum = 0, cap = 0;
     void* d_m[EREB_M_NFIELDS] = {};
@@ -120,6 +126,7 @@ struct ereb_ctx {
     bool step_warm = false;  // one eager step has run (allocations made)
     hipGraphExec_t step_exec = nullptr;
     ereb::StepKey step_key{};
+    int64_t step_captures = 0, step_replays = 0;
 
     bool prof = false;
     std::vector<ereb::ProfPair> prof_pending;
@@ -152,6 +159,13 @@ int32_t* tm_ptr(ereb_ctx* c);
 int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell);
 int ensure_scan(ereb_ctx* c, int64_t n);
 int check_flags(ereb_ctx* c);
+// write c->scal to the device block when it differs from the last write
+// (stream-ordered; never called while a step is being captured: the values
+// a capture needs are written before it)
+int scal_sync(ereb_ctx* c);
+int set_heating(ereb_ctx* c, const double* hrs, const double* hrf);
+int set_dt(ereb_ctx* c, double dt);
+int set_first(ereb_ctx* c, bool first);
 // grow the resident marker arrays to hold n markers (contents kept)
 int reserve_markers(ereb_ctx* c, int64_t n);
 void prof_begin(ereb_ctx* c, int kclass, hipEvent_t* a);

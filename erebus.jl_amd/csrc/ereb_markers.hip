@@ -172,7 +172,7 @@ __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict
 struct RK4Args {
     GeomK g;
     ereb_materials M;
-    double dt;
+    const StepScalars* __restrict__ S;  // dt and, for the fused gathers, first
     int mode;
     int64_t n;
     double *xm, *ym, *tkm, *sxxm, *sxym;
@@ -180,20 +180,18 @@ struct RK4Args {
     const double* phim;
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
     double *xb_out, *yb_out;  // optional: fluid back-trace end points (adjoint recording)
-    // fused gathers (k_rk4<1 or 2>): DSXX, DSXY and the temperature grid
-    // (tk2 at timestep 1, else DT)
-    const double *DSXX, *DSXY, *tgrid;
-    int first;
+    // fused gathers (k_rk4<1 or 2>): DSXX, DSXY and the DT grid (the
+    // temperature gather reads tk2 at timestep 1, else DT)
+    const double *DSXX, *DSXY, *DTg;
 };
 
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
 // to (xn, yn) and the fluid back-trace (E:5138-5226) from (xn, yn) to
 // (xb, yb).
 template <bool EXACT>
-__device__ __forceinline__ void rk4_paths(const RK4Args& A, double x0, double y0, double& xn,
-                                          double& yn, double& xb, double& yb) {
+__device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0, double y0,
+                                          double& xn, double& yn, double& xb, double& yb) {
     const GeomK& g = A.g;
-    const double dt = A.dt;
     const double* b = A.M.brk4;
     const double* cr = A.M.crk4;
     double xmm = x0, ymm = y0, xr = x0, yr = y0;
@@ -245,16 +243,18 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
     const GeomK& g = A.g;
-    const double dt = A.dt;
+    const double dt = A.S->dt;
+    const bool first = GATHER && A.S->first != 0;
+    const double* tgrid = first ? A.tk2 : A.DTg;
     const double xmm = A.xm[m], ymm = A.ym[m];
     double s = A.sxxm[m], t = A.sxym[m];
-    double T = GATHER && A.first ? 0.0 : A.tkm[m];
+    double T = first ? 0.0 : A.tkm[m];
     if (GATHER == 1) {
         gather_S_one<double, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<double, EXACT>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+        T = gather_T_one<double, EXACT>(g, xmm, ymm, T, tgrid, first);
     } else if (GATHER == 2) {
         gather_S_one<float, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<float, EXACT>(g, xmm, ymm, T, A.tgrid, A.first != 0);
+        T = gather_T_one<float, EXACT>(g, xmm, ymm, T, tgrid, first);
     }
     double xr = xmm, yr = ymm;
     double w[4];
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     A.sxym[m] = s * s2 + t * c2;
     // solid RK4 step and fluid back-trace
     double xn, yn;
-    rk4_paths<EXACT>(A, xmm, ymm, xn, yn, xr, yr);
+    rk4_paths<EXACT>(A, dt, xmm, ymm, xn, yn, xr, yr);
     A.xm[m] = xn;
     A.ym[m] = yn;
     if (A.xb_out) {
@@ -451,13 +451,15 @@ int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid
     return EREB_OK;
 }
 
-static RK4Args rk4_args(ereb_ctx* c, double dt, int mode);
+static RK4Args rk4_args(ereb_ctx* c, int mode);
 
 int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,
                    const double* gout, double* gin, double* gphi, double* gtk2) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
-    RK4Args A = rk4_args(c, dt, mode);
+    int rc;
+    if ((rc = set_dt(c, dt))) return rc;
+    RK4Args A = rk4_args(c, mode);
     hipLaunchKernelGGL(k_rk4_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
                        A, xb, yb, gout, gin, gphi, gtk2);
     EREB_HIP(c, hipGetLastError());
@@ -499,12 +501,12 @@ int launch_gather_stress(ereb_ctx* c) {
     return EREB_OK;
 }
 
-static RK4Args rk4_args(ereb_ctx* c, double dt, int mode) {
+static RK4Args rk4_args(ereb_ctx* c, int mode) {
     const int64_t M = c->marknum;
     RK4Args A;
     A.g = geomk(c);
     A.M = c->mat;
-    A.dt = dt;
+    A.S = c->d_scal;
     A.mode = mode;
     A.n = M;
     A.xm = mptr(c, EREB_M_XM);
@@ -521,15 +523,16 @@ static RK4Args rk4_args(ereb_ctx* c, double dt, int mode) {
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
     A.xb_out = A.yb_out = nullptr;
-    A.DSXX = A.DSXY = A.tgrid = nullptr;
-    A.first = 0;
+    A.DSXX = A.DSXY = A.DTg = nullptr;
     return A;
 }
 
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
-    RK4Args A = rk4_args(c, dt, mode);
+    int rc;
+    if ((rc = set_dt(c, dt))) return rc;
+    RK4Args A = rk4_args(c, mode);
     A.xb_out = xb_out;
     A.yb_out = yb_out;
     ++c->pos_gen;
@@ -550,11 +553,12 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
     if (M <= 0) return EREB_OK;
     if (c->Nx < 4 || c->Ny < 4)
         return fail(c, EREB_ERR_ARG, "update_marker_stress! needs Nx, Ny >= 4 (P clamps 2..N-1)");
-    RK4Args A = rk4_args(c, dt, mode);
+    int rc;
+    if ((rc = set_dt(c, dt)) || (rc = set_first(c, timestep == 1))) return rc;
+    RK4Args A = rk4_args(c, mode);
     A.DSXX = gptr(c, EREB_G_DSXX);
     A.DSXY = gptr(c, EREB_G_DSXY);
-    A.first = timestep == 1 ? 1 : 0;
-    A.tgrid = gptr(c, A.first ? EREB_G_TK2 : EREB_G_DT);
+    A.DTg = gptr(c, EREB_G_DT);
     ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);

@@ -297,13 +297,17 @@ struct MarkerFields {
     int es[EREB_M_NFIELDS];  // element size in bytes: 4 or 8
 };
 
-// splitmix64 finaliser: a counter-based uniform in [0, 1)
-__device__ __forceinline__ double u01(uint64_t z) {
+// splitmix64 step: golden-ratio increment, then the finaliser
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    return (double)(z >> 11) * 0x1p-53;
+    return z ^ (z >> 31);
+}
+
+// a counter-based uniform in [0, 1)
+__device__ __forceinline__ double u01(uint64_t z) {
+    return (double)(splitmix64(z) >> 11) * 0x1p-53;
 }
 
 // pass 3: append the new markers in column-major anchor order (E:5638-5683)
@@ -318,7 +322,7 @@ __device__ __forceinline__ void append_field(const MGridK& G, const MarkerFields
         const int i = (int)(a % G.Nym) + 1, j = (int)(a / G.Nym) + 1;
         double v = isx ? G.xxm[j - 1] : G.yym[i - 1];
         if (randomized)  // E:5662-5663 with a counter-based generator
-            v = v + (u01(seed ^ (2 * (uint64_t)a + (isx ? 0 : 1))) - 0.5) * (isx ? G.dxm : G.dym);
+            v = v + (u01(seed + (2 * (uint64_t)a + (isx ? 0 : 1))) - 0.5) * (isx ? G.dxm : G.dym);
         ((double*)F.f[f])[k] = v;
         return;
     }
@@ -418,6 +422,10 @@ int launch_replenish(ereb_ctx* c, const ereb_marker_grid& mg, int randomized, ui
                      int64_t* added) {
     const int64_t M = c->marknum;
     const int64_t A = mg.Nxm * mg.Nym;
+    // the jitter stream of anchor a is u01(h(seed) + 2a + {0, 1}): the seed
+    // is mixed once, so nearby seeds (a timestep counter, say) give unrelated
+    // streams rather than the same draws moved between anchors
+    seed = splitmix64(seed);
     if (mg.Nxm < 2 || mg.Nym < 2 || !mg.xxm || !mg.yym || A > INT32_MAX)
         return fail(c, EREB_ERR_ARG, "marker lattice needs Nxm, Nym >= 2 and both axes");
     int rc;

@@ -115,10 +115,7 @@ template <class R>
 static PropArgs<R> prop_args(ereb_ctx* c, const PropsReq& q) {
     PropArgs<R> P;
     P.M = c->d_mat;
-    for (int k = 0; k < 3; ++k) {
-        P.hrs[k] = q.hrs[k];
-        P.hrf[k] = q.hrf[k];
-    }
+    P.S = c->d_scal;  // hrs, hrf written there by set_heating
     P.mode = q.mode;
     P.tm = tm_ptr(c);
     P.tkm = mptr(c, EREB_M_TKM);
@@ -215,6 +212,7 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
     const int64_t nk = c->nkeys;
     int rc;
     hipEvent_t ev;
+    if (props && (rc = set_heating(c, props->hrs, props->hrf))) return rc;
     // perm is rewritten below: it orders no marker set until the sort completes
     c->have_sort = false;
     prof_begin(c, EREB_K_SORT, &ev);

@@ -117,6 +117,7 @@ PROTOTYPES = {
     "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
     "ereb_step_transfer": [_P, _P, _P, _I64, _D, _I64],
     "ereb_ctx_set_graphs": [_P, _I],
+    "ereb_step_graph_stats": [_P, _P, _P],
     "ereb_step_marker_viscosity": [_P],
     "ereb_step_marker_porosity": [_P, _P, _D],
     "ereb_step_melt_composition": [_P],
@@ -305,6 +306,13 @@ class Context:
 
     def set_graphs(self, on: bool):
         self.check(self.lib.ereb_ctx_set_graphs(self.h, int(bool(on))), "set_graphs")
+
+    def graph_stats(self):
+        """(captures, replays) of the transfer-step graph so far."""
+        cap, rep = ctypes.c_int64(), ctypes.c_int64()
+        self.check(self.lib.ereb_step_graph_stats(self.h, ctypes.byref(cap), ctypes.byref(rep)),
+                   "graph_stats")
+        return cap.value, rep.value
 
     def step_node_properties(self):
         self.check(self.lib.ereb_step_node_properties(self.h), "step_node_properties")

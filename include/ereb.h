@@ -206,14 +206,22 @@ int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
  * ereb_step_advect_rk4 (E:6419-7053 without the host solves); the two
  * gathers and RK4 run as one kernel (kernel class EREB_K_RK4).  After the
  * first call the step's launches are captured once into a HIP graph and
- * replayed; the graph is captured again when an argument, the marker count,
- * the precision form or a device buffer changes.  Results are those of the
- * separate calls, bit for bit.  Eager while profiling is enabled. */
+ * replayed; the graph is captured again when the mode, the marker count,
+ * the precision form or a device buffer changes (not when dt, the heating
+ * rates or the timestep change).  Results are those of the separate calls,
+ * bit for bit.  Eager while profiling is enabled. */
 int ereb_step_transfer(ereb_ctx* ctx, const double hrsolidm[3],
                        const double hrfluidm[3], int64_t mode, double dt,
                        int64_t timestep);
 /* graph replay of ereb_step_transfer on (default) or off */
 int ereb_ctx_set_graphs(ereb_ctx* ctx, int on);
+/* number of captures of the step graph and of replays since the context was
+ * created.  dt, hrsolidm/hrfluidm and timestep == 1 are read by the kernels
+ * from a device block written before each replay, so they change from step to
+ * step without a new capture, as in the reference loop where dt follows the
+ * Courant and temperature-change limits (E:6619, E:6790, E:6933) and the
+ * heating rates follow calculate_radioactive_heating(timesum) (E:6413). */
+int ereb_step_graph_stats(const ereb_ctx* ctx, int64_t* captures, int64_t* replays);
 
 #define EREB_SYS_STOKES_DARCY 0
 #define EREB_SYS_THERMAL      1

@@ -397,6 +397,11 @@ void orc_update_marker_temperature(const ereb_geometry* g, const double* xm,
     const double* ym, double* tkm, const double* DT, const double* tk2,
     int64_t timestep, int64_t marknum) {
     const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    /* the reference threads this loop (@threads, E:4786/E:4793); markers
+     * are independent, so the result does not depend on the thread count */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int64_t m = 0; m < marknum; ++m) {
         int64_t i, j;
         double w[4];
@@ -414,6 +419,10 @@ void orc_update_marker_stress(const ereb_geometry* g, const double* xm,
     const double* ym, double* sxxm, double* sxym, const double* DSXX,
     const double* DSXY, int64_t marknum) {
     const int64_t Nx = g->Nx, Ny = g->Ny, Ny1 = Ny + 1;
+    /* @threads in the reference (E:4356) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int64_t m = 0; m < marknum; ++m) {
         int64_t ip, jp, ib, jb;
         double wp[4], wb[4];
@@ -856,6 +865,10 @@ void orc_update_marker_viscosity(const ereb_geometry* g, const ereb_materials* M
     double* etatotalm, double* etavpm, const uint8_t* YNY,
     const double* YNY_inv_ETA, int64_t marknum) {
     const int64_t Nx = g->Nx, Ny = g->Ny;
+    /* @threads in the reference loop (E:6950) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int64_t m = 0; m < marknum; ++m) {
         int64_t i, j;
         double w[4];

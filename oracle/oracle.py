@@ -391,3 +391,43 @@ class Oracle:
         vx, vy = _F(vx), _F(vy)
         self.L.orc_compute_rotation_rate(self.pg, _p(vx), _p(vy), _p(w))
         return w
+
+
+BENCH_LIB = os.path.join(HERE, "_build", "libereb_oracle_bench.so")
+# marker vectors of orc_bench_transfer, in its order (ereb_cpu_step.c)
+BENCH_MARKERS = ["xm", "ym", "tkm", "phim", "sxxm", "sxym", "XWsolidm0", "etavpm",
+                 "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
+                 "alphasolidcur", "alphafluidcur", "rhototalm", "rhocptotalm", "etatotalm",
+                 "hrtotalm", "ktotalm", "tkm_rhocptotalm", "etafluidcur_inv_kphim"]
+BENCH_GRIDS = ["vx", "vy", "wyx", "tk2", "DT", "DSXX", "DSXY", "ETAP", "ETAPHI", "gx", "gy"]
+
+
+def bench_transfer(geom: Geometry, mat: Materials, mk: dict, grids: dict, hrs, hrf, mode: int,
+                   dt: float, max_steps: int, budget_s: float):
+    """CPU baseline: whole transfer timesteps of the C restatement at -O3,
+    with OpenMP in the loops that the reference threads, timed inside C
+    (ereb_cpu_step.c).  `mk` is updated in place.  Returns (steps done,
+    seconds, threads)."""
+    if not os.path.exists(BENCH_LIB):
+        build()
+    L = ctypes.CDLL(BENCH_LIB)
+    L.orc_bench_transfer.restype = _I64
+    L.orc_bench_transfer.argtypes = [_P, _P, _P, _P, _I64, _P, _P, _P, _I64, _D, _I64, _D, _P]
+    L.orc_bench_threads.restype = ctypes.c_int
+    cg = geom.to_c()
+    cm = mat.to_c(geom.xsize, geom.ysize)
+    M = mk["xm"].size
+    for f in BENCH_MARKERS:
+        assert mk[f].dtype == np.float64 and mk[f].flags.c_contiguous and mk[f].size == M, f
+    tm = _c(mk["tm"], np.int64)
+    mptrs = (ctypes.c_void_p * len(BENCH_MARKERS))(*[mk[f].ctypes.data for f in BENCH_MARKERS])
+    gs = [_F(grids[n]) for n in BENCH_GRIDS] + [np.zeros(geom.stag_shape, order="F")]
+    gptrs = (ctypes.c_void_p * len(gs))(*[a.ctypes.data for a in gs])
+    hs, hf = _c(hrs), _c(hrf)
+    sec = ctypes.c_double()
+    done = L.orc_bench_transfer(ctypes.addressof(cg), ctypes.addressof(cm), mptrs, _p(tm), M,
+                                gptrs, _p(hs), _p(hf), int(mode), float(dt), int(max_steps),
+                                float(budget_s), ctypes.byref(sec))
+    if done < 0:
+        raise RuntimeError(f"orc_bench_transfer failed with status {-done}")
+    return int(done), sec.value, int(L.orc_bench_threads())

@@ -2,8 +2,11 @@
 launches run eagerly, bit for bit, step after step: marker state and
 properties, SUM and node grids, both assembled systems.  Between replays the
 host does its own work (reads, a melt-composition scatter that relies on the
-sort bookkeeping of the replayed step) and an argument change forces a new
-capture."""
+sort bookkeeping of the replayed step).  dt and the heating rates change at
+every step, as in the reference loop (dt control E:6619, E:6790, E:6933;
+calculate_radioactive_heating(timesum) E:6413), and the graph is captured
+once: the kernels read them from a device block written before each replay.
+A mode change forces a new capture."""
 from __future__ import annotations
 
 import numpy as np
@@ -35,6 +38,15 @@ def _ctx(g, mat, mk, graphs, precision=ereb.FP64):
     return ctx
 
 
+def _scalars(mat, ts):
+    """dt and heating rates of timestep ts: dt shrinks and grows as the
+    reference's step control would make it, timesum advances by dt."""
+    dt = ereb.DT_LONGEST * (1.0 - 0.37 * ((ts * 5) % 7) / 7.0)
+    timesum = START_TIME + ts * ereb.DT_LONGEST
+    hrs, hrf = mat.radioactive_heating(timesum)
+    return dt, hrs, hrf
+
+
 def _state(ctx):
     out = {n: ctx.get_marker(n) for n in ereb.MARKER_FIELDS}
     for n in ereb.SUM_FIELDS + ["ETA", "RHO", "tk1", "RX", "KY", "XWS"]:
@@ -51,35 +63,34 @@ def test_graph_replay_equals_eager(precision):
     separate gather and RK4 kernels (the third context), in both forms."""
     g, mat = synth.workload_geometry("adj41", mode=1)
     mk = synth.make_markers(g, mat, seed=5)
-    hrs, hrf = mat.radioactive_heating(START_TIME)
-    dt = ereb.DT_LONGEST
     A = _ctx(g, mat, mk, graphs=False, precision=precision)
     B = _ctx(g, mat, mk, graphs=True, precision=precision)
     try:
-        for ts in range(1, 8):
-            if ts == 5:
-                dt = 0.5 * dt  # new scalar: B captures again
+        for ts in range(1, 9):
+            dt, hrs, hrf = _scalars(mat, ts)
+            mode = 9 if ts == 7 else 1  # a mode change captures again
             for c in (A, B):
-                c.step_transfer(hrs, hrf, 1, dt, ts)
+                c.step_transfer(hrs, hrf, mode, dt, ts)
             if ts == 3:  # reuses or redoes the sort: must match
                 for c in (A, B):
                     c.step_melt_composition()
             sa, sb = _state(A), _state(B)
             for k in sa:
                 assert np.array_equal(sa[k], sb[k], equal_nan=True), (ts, k)
+        # step 1 eager; captures at steps 2, 7 (mode 9) and 8 (mode 1 again)
+        assert B.graph_stats() == (3, 7), B.graph_stats()
         # the replayed steps also equal the separate resident calls
         C = _ctx(g, mat, mk, graphs=False, precision=precision)
-        dt = ereb.DT_LONGEST
-        for ts in range(1, 8):
-            if ts == 5:
-                dt = 0.5 * dt
-            C.step_scatter(hrs, hrf, 1)
+        for ts in range(1, 9):
+            dt, hrs, hrf = _scalars(mat, ts)
+            mode = 9 if ts == 7 else 1
+            C.step_scatter(hrs, hrf, mode)
             C.step_node_properties()
             C.step_assemble_stokes_darcy(dt)
             C.step_assemble_thermal(dt)
             C.step_gather_stress()
             C.step_gather_temperature(ts)
-            C.step_advect_rk4(dt, 1)
+            C.step_advect_rk4(dt, mode)
             if ts == 3:
                 C.step_melt_composition()
         sc = _state(C)

@@ -105,19 +105,28 @@ def test_replenish_resident_workload_hole():
     _exact(mdis, mdis_o, "mdis")
 
 
-@pytest.mark.parametrize("spare", [0, 10_000])
-def test_replenish_after_advection_in_sort_order(spare):
+NEW_MAX = 4096  # most empty anchors that the listed append handles
+
+
+@pytest.mark.parametrize("spare,hole_km", [(0, (15, 10)), (10_000, (15, 10)),
+                                           (10_000, (1.5, 1.0))])
+def test_replenish_after_advection_in_sort_order(spare, hole_km):
     """The timestep's order: a scatter sorts the markers, the advection moves
     them (here up to 0.6 cells), then the replenishment visits them in the
     now stale sort order.  The result equals the oracle's on the moved
-    markers, bit for bit.  With spare capacity for 4,096 more markers the
-    append runs ahead of the host's read of the count."""
+    markers, bit for bit.  A 15 x 10 km hole leaves ~19k empty anchors, more
+    than the listed append takes (the flag scan over all anchors runs); a
+    1.5 x 1 km hole with spare capacity for 10,000 markers is the case of
+    every later timestep: at most 4,096 empty anchors, appended from the
+    compacted list ahead of the host's read of the count."""
     g, mat = synth.workload_geometry("fp64", mode=1)
     mk = synth.make_markers(g, mat, seed=2)
     rng = np.random.default_rng(3)
     for f in ("XWsolidm", "phinewm", "pfm0"):
         mk[f] = rng.random(mk["xm"].size)
-    hole = ((mk["xm"] > 40e3) & (mk["xm"] < 55e3) & (mk["ym"] > 60e3) & (mk["ym"] < 70e3))
+    hw, hh = hole_km[0] * 1e3, hole_km[1] * 1e3
+    hole = ((mk["xm"] > 40e3) & (mk["xm"] < 40e3 + hw) & (mk["ym"] > 60e3)
+            & (mk["ym"] < 60e3 + hh))
     mk = {f: np.ascontiguousarray(v[~hole]) for f, v in mk.items()}
     M = mk["xm"].size
     moved = dict(mk)
@@ -134,6 +143,8 @@ def test_replenish_after_advection_in_sort_order(spare):
             moved[f] = ctx.get_marker(f)
         want, n, mdis_o, mnum_o = Oracle(g, mat).replenish_markers(moved)
         assert n > M
+        listed = spare >= NEW_MAX and n - M <= NEW_MAX
+        assert listed == (hole_km[0] < 10), "case does not reach the intended append path"
         ctx.set_marker("xm", moved["xm"])
         ctx.set_marker("ym", moved["ym"])
         nn, mdis, mnum = ctx.step_replenish_markers()
