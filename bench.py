@@ -34,6 +34,7 @@ step.  `host_lu` times the host factorisation and solve of the assembled
 systems (scipy's SuperLU; the reference uses UMFPACK).  `cpu_baseline` times
 the C restatement of the reference CPU path (the oracle, -O3, OpenMP where
 the reference threads) on a bounded sample of the Float64 workload.
+"""
 from __future__ import annotations
 
 import argparse
