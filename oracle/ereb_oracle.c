@@ -184,6 +184,15 @@ static inline void add4(double* g, int64_t rows, int64_t i, int64_t j,
     G(g, i + 1, j + 1, rows) += p * w[3];
 }
 
+/* the same with |p * w|: the scale of a sum taken in another order */
+static inline void add4abs(double* g, int64_t rows, int64_t i, int64_t j,
+                           const double* w, double p) {
+    G(g, i, j, rows) += fabs(p * w[0]);
+    G(g, i + 1, j, rows) += fabs(p * w[1]);
+    G(g, i, j + 1, rows) += fabs(p * w[2]);
+    G(g, i + 1, j + 1, rows) += fabs(p * w[3]);
+}
+
 /* ------------------------------------------------------- step 1 + 2 */
 
 /* compute_marker_properties! E:700-750 for one marker (0-based m) */
@@ -241,7 +250,7 @@ int orc_compute_marker_properties(const ereb_materials* M, int64_t marknum,
  * TENSUM FRISUM WTSUM | RHOXSUM RHOFXSUM KXSUM PHIXSUM RXSUM WTXSUM |
  * RHOYSUM RHOFYSUM KYSUM PHIYSUM RYSUM WTYSUM | RHOSUM RHOCPSUM ALPHASUM
  * ALPHAFSUM HRSUM GGGPSUM SXXSUM TKSUM PHISUM WTPSUM */
-int orc_marker_to_nodes(const ereb_geometry* g, const ereb_materials* M,
+static int marker_to_nodes_impl(const ereb_geometry* g, const ereb_materials* M,
     const double* xm, const double* ym, const int64_t* tm, const double* tkm,
     const double* phim, const double* XW, const double* etavpm,
     const double* inv_gggtotalm, const double* fricttotalm,
@@ -252,8 +261,10 @@ int orc_marker_to_nodes(const ereb_geometry* g, const ereb_materials* M,
     double* hrtotalm, double* ktotalm, double* tkm_rhocptotalm,
     double* etafluidcur_inv_kphim, const double* hrsolidm,
     const double* hrfluidm, int64_t mode, double* const* sums,
-    int64_t marknum) {
+    int64_t marknum, int absval) {
     if (mode != 1 && mode != 9) return EREB_ERR_MODE;
+    void (*add)(double*, int64_t, int64_t, int64_t, const double*, double) =
+        absval ? add4abs : add4;
     const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
     for (int k = 0; k < 8; ++k) memset(sums[k], 0, sizeof(double) * Nx * Ny);
     for (int k = 8; k < 30; ++k) memset(sums[k], 0, sizeof(double) * Nx1 * Ny1);
@@ -267,47 +278,89 @@ int orc_marker_to_nodes(const ereb_geometry* g, const ereb_materials* M,
         /* marker_to_basic_nodes! E:1605-1648 */
         orc_fix_weights(xm[m], ym[m], g->x, g->y, g->dx, g->dy, 1, Nx - 1, 1,
                         Ny - 1, &i, &j, w);
-        add4(sums[0], Ny, i, j, w, etatotalm[m]);
-        add4(sums[1], Ny, i, j, w, etavpm[m]);
-        add4(sums[2], Ny, i, j, w, inv_gggtotalm[m]);
-        add4(sums[3], Ny, i, j, w, sxym[m]);
-        add4(sums[4], Ny, i, j, w, cohestotalm[m]);
-        add4(sums[5], Ny, i, j, w, tenstotalm[m]);
-        add4(sums[6], Ny, i, j, w, fricttotalm[m]);
-        add4(sums[7], Ny, i, j, w, 1.0);
+        add(sums[0], Ny, i, j, w, etatotalm[m]);
+        add(sums[1], Ny, i, j, w, etavpm[m]);
+        add(sums[2], Ny, i, j, w, inv_gggtotalm[m]);
+        add(sums[3], Ny, i, j, w, sxym[m]);
+        add(sums[4], Ny, i, j, w, cohestotalm[m]);
+        add(sums[5], Ny, i, j, w, tenstotalm[m]);
+        add(sums[6], Ny, i, j, w, fricttotalm[m]);
+        add(sums[7], Ny, i, j, w, 1.0);
         /* marker_to_vx_nodes! E:1676-1713 */
         orc_fix_weights(xm[m], ym[m], g->xvx, g->yvx, g->dx, g->dy, 1, Nx - 1, 1,
                         Ny, &i, &j, w);
-        add4(sums[8], Ny1, i, j, w, rhototalm[m]);
-        add4(sums[9], Ny1, i, j, w, rhofluidcur[m]);
-        add4(sums[10], Ny1, i, j, w, ktotalm[m]);
-        add4(sums[11], Ny1, i, j, w, phim[m]);
-        add4(sums[12], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
-        add4(sums[13], Ny1, i, j, w, 1.0);
+        add(sums[8], Ny1, i, j, w, rhototalm[m]);
+        add(sums[9], Ny1, i, j, w, rhofluidcur[m]);
+        add(sums[10], Ny1, i, j, w, ktotalm[m]);
+        add(sums[11], Ny1, i, j, w, phim[m]);
+        add(sums[12], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
+        add(sums[13], Ny1, i, j, w, 1.0);
         /* marker_to_vy_nodes! E:1741-1778 */
         orc_fix_weights(xm[m], ym[m], g->xvy, g->yvy, g->dx, g->dy, 1, Nx, 1,
                         Ny - 1, &i, &j, w);
-        add4(sums[14], Ny1, i, j, w, rhototalm[m]);
-        add4(sums[15], Ny1, i, j, w, rhofluidcur[m]);
-        add4(sums[16], Ny1, i, j, w, ktotalm[m]);
-        add4(sums[17], Ny1, i, j, w, phim[m]);
-        add4(sums[18], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
-        add4(sums[19], Ny1, i, j, w, 1.0);
+        add(sums[14], Ny1, i, j, w, rhototalm[m]);
+        add(sums[15], Ny1, i, j, w, rhofluidcur[m]);
+        add(sums[16], Ny1, i, j, w, ktotalm[m]);
+        add(sums[17], Ny1, i, j, w, phim[m]);
+        add(sums[18], Ny1, i, j, w, etafluidcur_inv_kphim[m]);
+        add(sums[19], Ny1, i, j, w, 1.0);
         /* marker_to_p_nodes! E:1814-1863 */
         orc_fix_weights(xm[m], ym[m], g->xp, g->yp, g->dx, g->dy, 1, Nx, 1, Ny,
                         &i, &j, w);
-        add4(sums[25], Ny1, i, j, w, inv_gggtotalm[m]); /* GGGPSUM */
-        add4(sums[26], Ny1, i, j, w, sxxm[m]);          /* SXXSUM */
-        add4(sums[20], Ny1, i, j, w, rhototalm[m]);     /* RHOSUM */
-        add4(sums[21], Ny1, i, j, w, rhocptotalm[m]);   /* RHOCPSUM */
-        add4(sums[22], Ny1, i, j, w, alphasolidcur[m]); /* ALPHASUM */
-        add4(sums[23], Ny1, i, j, w, alphafluidcur[m]); /* ALPHAFSUM */
-        add4(sums[24], Ny1, i, j, w, hrtotalm[m]);      /* HRSUM */
-        add4(sums[28], Ny1, i, j, w, phim[m]);          /* PHISUM */
-        add4(sums[27], Ny1, i, j, w, tkm_rhocptotalm[m]); /* TKSUM */
-        add4(sums[29], Ny1, i, j, w, 1.0);              /* WTPSUM */
+        add(sums[25], Ny1, i, j, w, inv_gggtotalm[m]); /* GGGPSUM */
+        add(sums[26], Ny1, i, j, w, sxxm[m]);          /* SXXSUM */
+        add(sums[20], Ny1, i, j, w, rhototalm[m]);     /* RHOSUM */
+        add(sums[21], Ny1, i, j, w, rhocptotalm[m]);   /* RHOCPSUM */
+        add(sums[22], Ny1, i, j, w, alphasolidcur[m]); /* ALPHASUM */
+        add(sums[23], Ny1, i, j, w, alphafluidcur[m]); /* ALPHAFSUM */
+        add(sums[24], Ny1, i, j, w, hrtotalm[m]);      /* HRSUM */
+        add(sums[28], Ny1, i, j, w, phim[m]);          /* PHISUM */
+        add(sums[27], Ny1, i, j, w, tkm_rhocptotalm[m]); /* TKSUM */
+        add(sums[29], Ny1, i, j, w, 1.0);              /* WTPSUM */
     }
     return EREB_OK;
+}
+
+int orc_marker_to_nodes(const ereb_geometry* g, const ereb_materials* M,
+    const double* xm, const double* ym, const int64_t* tm, const double* tkm,
+    const double* phim, const double* XW, const double* etavpm,
+    const double* inv_gggtotalm, const double* fricttotalm,
+    const double* cohestotalm, const double* tenstotalm,
+    const double* rhofluidcur, const double* alphasolidcur,
+    const double* alphafluidcur, const double* sxxm, const double* sxym,
+    double* rhototalm, double* rhocptotalm, double* etatotalm,
+    double* hrtotalm, double* ktotalm, double* tkm_rhocptotalm,
+    double* etafluidcur_inv_kphim, const double* hrsolidm,
+    const double* hrfluidm, int64_t mode, double* const* sums,
+    int64_t marknum) {
+    return marker_to_nodes_impl(g, M, xm, ym, tm, tkm, phim, XW, etavpm, inv_gggtotalm,
+                                fricttotalm, cohestotalm, tenstotalm, rhofluidcur,
+                                alphasolidcur, alphafluidcur, sxxm, sxym, rhototalm,
+                                rhocptotalm, etatotalm, hrtotalm, ktotalm, tkm_rhocptotalm,
+                                etafluidcur_inv_kphim, hrsolidm, hrfluidm, mode, sums,
+                                marknum, 0);
+}
+
+/* Per node sum of |value * weight| over the same loop: the scale against
+ * which the GPU's reordered sums are compared (tests only) */
+int orc_marker_to_nodes_abs(const ereb_geometry* g, const ereb_materials* M,
+    const double* xm, const double* ym, const int64_t* tm, const double* tkm,
+    const double* phim, const double* XW, const double* etavpm,
+    const double* inv_gggtotalm, const double* fricttotalm,
+    const double* cohestotalm, const double* tenstotalm,
+    const double* rhofluidcur, const double* alphasolidcur,
+    const double* alphafluidcur, const double* sxxm, const double* sxym,
+    double* rhototalm, double* rhocptotalm, double* etatotalm,
+    double* hrtotalm, double* ktotalm, double* tkm_rhocptotalm,
+    double* etafluidcur_inv_kphim, const double* hrsolidm,
+    const double* hrfluidm, int64_t mode, double* const* sums,
+    int64_t marknum) {
+    return marker_to_nodes_impl(g, M, xm, ym, tm, tkm, phim, XW, etavpm, inv_gggtotalm,
+                                fricttotalm, cohestotalm, tenstotalm, rhofluidcur,
+                                alphasolidcur, alphafluidcur, sxxm, sxym, rhototalm,
+                                rhocptotalm, etatotalm, hrtotalm, ktotalm, tkm_rhocptotalm,
+                                etafluidcur_inv_kphim, hrsolidm, hrfluidm, mode, sums,
+                                marknum, 1);
 }
 
 /* --------------------------------------------------- node properties */

@@ -56,6 +56,8 @@ def lib():
             "orc_compute_marker_properties": (ctypes.c_int, [_P, _I64] + [_P] * 13 + [_I64]),
             "orc_marker_to_nodes": (ctypes.c_int, [_P, _P] + [_P] * 23 + [_P, _P, _I64, _P,
                                                                          _I64]),
+            "orc_marker_to_nodes_abs": (ctypes.c_int, [_P, _P] + [_P] * 23 + [_P, _P, _I64, _P,
+                                                                             _I64]),
             "orc_compute_basic_node_properties": (None, [_P] * 17),
             "orc_compute_v_node_properties": (None, [_P] * 12),
             "orc_compute_p_node_properties": (None, [_P] * 21),
@@ -158,8 +160,10 @@ class Oracle:
         return self.L.orc_compute_kfluidm(self.pm, T, mode, None)
 
     # ---- step 1 + 2 ---------------------------------------------------------
-    def marker_to_nodes(self, mk: dict, hrs, hrf, mode):
-        """Loop E:6419-6517; returns (props dict, list of 30 SUM grids)."""
+    def marker_to_nodes(self, mk: dict, hrs, hrf, mode, absval: bool = False):
+        """Loop E:6419-6517; returns (props dict, list of 30 SUM grids).  With
+        absval, the grids hold the per-node sums of |value * weight| instead
+        (the scale of a sum taken in another order)."""
         g = self.geom
         M = mk["xm"].size
         ins = [_c(mk[k], np.int64 if k == "tm" else np.float64) for k in
@@ -173,9 +177,9 @@ class Oracle:
                [np.zeros((g.Ny1, g.Nx1), order="F") for _ in range(22)]
         sp = (ctypes.c_void_p * 30)(*[s.ctypes.data for s in sums])
         hs, hf = _c(hrs), _c(hrf)
-        rc = self.L.orc_marker_to_nodes(self.pg, self.pm, *[_p(a) for a in ins],
-                                        *[_p(a) for a in outs], _p(hs), _p(hf), int(mode),
-                                        ctypes.addressof(sp), M)
+        f = self.L.orc_marker_to_nodes_abs if absval else self.L.orc_marker_to_nodes
+        rc = f(self.pg, self.pm, *[_p(a) for a in ins], *[_p(a) for a in outs], _p(hs), _p(hf),
+               int(mode), ctypes.addressof(sp), M)
         if rc != 0:
             raise ValueError(f"unknown mode {mode}")
         return dict(zip(names, outs)), sums
@@ -431,3 +435,163 @@ def bench_transfer(geom: Geometry, mat: Materials, mk: dict, grids: dict, hrs, h
     if done < 0:
         raise RuntimeError(f"orc_bench_transfer failed with status {-done}")
     return int(done), sec.value, int(L.orc_bench_threads())
+
+
+DUAL_LIB = os.path.join(HERE, "_build", "libereb_oracle_dual.so")
+_dual = None
+
+# marker seeds of the dual functions
+RK4_SEEDS = {"xm": 0, "ym": 1, "tkm": 2, "phim": 3, "sxym": 4, "sxxm": 5,
+             "vx": 6, "vy": 7, "vxf": 8, "vyf": 9, "wyx": 10, "tk2": 11}
+GATHER_S_SEEDS = {"xm": 0, "ym": 1, "sxxm": 2, "sxym": 3, "DSXX": 4, "DSXY": 5}
+GATHER_T_SEEDS = {"xm": 0, "ym": 1, "tkm": 2, "grid": 3}
+
+
+def dual_lib():
+    global _dual
+    if _dual is None:
+        if not os.path.exists(DUAL_LIB):
+            build()
+        L = ctypes.CDLL(DUAL_LIB)
+        ci = ctypes.c_int
+        sig = {
+            "dual_marker_to_nodes_vjp": (ci, [_P, _P, _P, _I64, _P, _P, _I64, _P, _P, ci, _P]),
+            "dual_marker_to_nodes_dot": (ci, [_P, _P, _P, _I64, _P, _P, _I64, _P, _P, _P]),
+            "dual_gather_T_vjp": (ci, [_P, _P, _P, _P, _P, _P, _I64, _I64, _P, ci, _P, _P]),
+            "dual_gather_S_vjp": (ci, [_P] * 7 + [_I64, _P, _P, ci, _P, _P]),
+            "dual_rk4_vjp": (ci, [_P, _P] + [_P] * 13 + [_I64, _D, _I64] + [_P] * 5 +
+                             [ci, _P, _P]),
+            "dual_rk4_double": (ci, [_P, _P] + [_P] * 13 + [_I64, _D, _I64]),
+            "dual_tk1_loss_grad": (ci, [_P, _P, _P, _I64, _P, _P, _I64, _D, _I64, _P, _P,
+                                        _I64, _P, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _dual = L
+    return _dual
+
+
+class DualOracle:
+    """Exact (forward-mode) derivatives of the oracle's transfer functions
+    (oracle/ereb_dual.cpp), bound to one geometry and material set."""
+
+    def __init__(self, geom: Geometry, mat: Materials):
+        self.L = dual_lib()
+        self.geom = geom
+        self.cg = geom.to_c()
+        self.cm = mat.to_c(geom.xsize, geom.ysize)
+        self.pg = ctypes.addressof(self.cg)
+        self.pm = ctypes.addressof(self.cm)
+
+    def _markers(self, mk):
+        keep = {}
+        ptrs = (ctypes.c_void_p * len(MARKER_FIELDS))()
+        for k, f in enumerate(MARKER_FIELDS):
+            if f in mk:
+                a = _c(mk[f], np.int64 if f == "tm" else np.float64)
+                keep[f] = a
+                ptrs[k] = a.ctypes.data
+        return ptrs, keep
+
+    @staticmethod
+    def _grid_ptrs(arrs, n):
+        keep = [None if a is None else _F(a) for a in arrs]
+        ptrs = (ctypes.c_void_p * n)(*[None if a is None else a.ctypes.data for a in keep])
+        return ptrs, keep
+
+    def marker_to_nodes_vjp(self, mk, hrs, hrf, mode, sum_bar, prop_bar, field):
+        """d/d(field of marker m) of sum_bar . (marker m's SUM contributions)
+        + prop_bar . (marker m's properties), for every marker."""
+        ptrs, keep = self._markers(mk)
+        sb, ks = self._grid_ptrs(sum_bar, 30)
+        pb_keep = None if prop_bar is None else [None if a is None else _c(a) for a in prop_bar]
+        pb = None if prop_bar is None else (ctypes.c_void_p * 7)(
+            *[None if a is None else a.ctypes.data for a in pb_keep])
+        M = mk["xm"].size
+        out = np.empty(M)
+        hs, hf = _c(hrs), _c(hrf)
+        rc = self.L.dual_marker_to_nodes_vjp(self.pg, self.pm, ptrs, M, _p(hs), _p(hf),
+                                             int(mode), sb, pb, MARKER_FIELDS.index(field),
+                                             _p(out))
+        assert rc == 0, rc
+        return out
+
+    def marker_to_nodes_dot(self, mk, hrs, hrf, mode, sum_bar):
+        """(per-marker sum_bar . contributions, the 7 property vectors) with
+        the template at S = double."""
+        ptrs, keep = self._markers(mk)
+        sb, ks = self._grid_ptrs(sum_bar, 30)
+        M = mk["xm"].size
+        out = np.empty(M)
+        props = np.empty((7, M))
+        hs, hf = _c(hrs), _c(hrf)
+        rc = self.L.dual_marker_to_nodes_dot(self.pg, self.pm, ptrs, M, _p(hs), _p(hf), int(mode),
+                                             sb, _p(out), _p(props))
+        assert rc == 0, rc
+        return out, props
+
+    def gather_T_vjp(self, xm, ym, tkm, DT, tk2, timestep, tkm_bar, which, gdot=None):
+        M = xm.size
+        out = np.empty(M)
+        a = [_c(xm), _c(ym), _c(tkm), _F(DT), _F(tk2), _c(tkm_bar)]
+        gd = None if gdot is None else _F(gdot)
+        rc = self.L.dual_gather_T_vjp(self.pg, *[_p(v) for v in a[:5]], int(timestep), M,
+                                      _p(a[5]), GATHER_T_SEEDS[which], _p(gd), _p(out))
+        assert rc == 0
+        return out
+
+    def gather_S_vjp(self, xm, ym, sxxm, sxym, DSXX, DSXY, sxx_bar, sxy_bar, which, gdot=None):
+        M = xm.size
+        out = np.empty(M)
+        a = [_c(xm), _c(ym), _c(sxxm), _c(sxym), _F(DSXX), _F(DSXY)]
+        b = [_c(sxx_bar), _c(sxy_bar)]
+        gd = None if gdot is None else _F(gdot)
+        rc = self.L.dual_gather_S_vjp(self.pg, *[_p(v) for v in a], M, _p(b[0]), _p(b[1]),
+                                      GATHER_S_SEEDS[which], _p(gd), _p(out))
+        assert rc == 0
+        return out
+
+    def rk4_vjp(self, mk, grids, dt, mode, bars, which, gdot=None):
+        """bars: dict of output cotangents (xm ym tkm sxym sxxm; missing = 0);
+        grids: vx vy vxf vyf wyx tk2."""
+        M = mk["xm"].size
+        m = [_c(mk[f]) for f in ("xm", "ym")] + [_c(mk["tm"], np.int64)] + \
+            [_c(mk[f]) for f in ("tkm", "phim", "sxym", "sxxm")]
+        gr = [_F(grids[n]) for n in ("vx", "vy", "vxf", "vyf", "wyx", "tk2")]
+        bb = [None if bars.get(f) is None else _c(bars[f])
+              for f in ("xm", "ym", "tkm", "sxym", "sxxm")]
+        gd = None if gdot is None else _F(gdot)
+        out = np.empty(M)
+        rc = self.L.dual_rk4_vjp(self.pg, self.pm, *[_p(v) for v in m], *[_p(v) for v in gr], M,
+                                 float(dt), int(mode), *[_p(v) for v in bb], RK4_SEEDS[which],
+                                 _p(gd), _p(out))
+        assert rc == 0
+        return out
+
+    def rk4_double(self, mk, grids, dt, mode):
+        m = [_c(mk[f]).copy() for f in ("xm", "ym")] + [_c(mk["tm"], np.int64)] + \
+            [_c(mk[f]).copy() for f in ("tkm", "phim", "sxym", "sxxm")]
+        gr = [_F(grids[n]) for n in ("vx", "vy", "vxf", "vyf", "wyx", "tk2")]
+        rc = self.L.dual_rk4_double(self.pg, self.pm, *[_p(v) for v in m], *[_p(v) for v in gr],
+                                    m[0].size, float(dt), int(mode))
+        assert rc == 0
+        return m[0], m[1], m[3], m[5], m[6]
+
+    def tk1_loss_grad(self, mk, vx, vy, nsteps, dt, mode, hrs, hrf, seeds):
+        """Exact dL/d(tkm or phim) of the listed (marker, kind) seeds, kind 0
+        tkm, 1 phim, for L = sum(tk1^2) after nsteps steps of the map of
+        ereb_adjoint_tk1_loss; returns (L, grad)."""
+        ptrs, keep = self._markers(mk)
+        sm = np.ascontiguousarray([s[0] for s in seeds], dtype=np.int64)
+        sk = np.ascontiguousarray([s[1] for s in seeds], dtype=np.int32)
+        grad = np.empty(len(seeds))
+        L = ctypes.c_double()
+        a, b = _F(vx), _F(vy)
+        hs, hf = _c(hrs), _c(hrf)
+        rc = self.L.dual_tk1_loss_grad(self.pg, self.pm, ptrs, mk["xm"].size, _p(a), _p(b),
+                                       int(nsteps), float(dt), int(mode), _p(hs), _p(hf),
+                                       len(seeds), _p(sm), _p(sk), _p(grad), ctypes.byref(L))
+        assert rc == 0
+        return L.value, grad
