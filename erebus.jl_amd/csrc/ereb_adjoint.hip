@@ -11,6 +11,8 @@
 // forward kernels.  Marker positions and the grid weights do not depend on
 // tkm or phim (the velocity is prescribed), so they carry no cotangent.
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "ereb_internal.h"
@@ -308,18 +310,52 @@ static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const d
 struct AdjBuffers {
     double *save = nullptr, *cps = nullptr, *st = nullptr, *snap = nullptr, *xb = nullptr;
     double *gT = nullptr, *gT2 = nullptr, *gP = nullptr, *gg = nullptr;
+    // fixed-point sums of the tk2 cotangent of RK4^T (A) and of the gather^T
+    // (B): max bits [0..1], flag [2], then 2 words per P node for A and for B
+    unsigned long long* fix = nullptr;
 };
+
+// fixed-point grids of the tk2 cotangent (ereb_fixsum.h) over the pool piece
+static FixGrid fix_grid(const AdjBuffers& B, int which, int64_t n) {
+    return FixGrid{B.fix + 8 + (size_t)which * 2 * n, B.fix + which, 1.0};
+}
+
+// the scale from the first pass's largest contribution
+static int fix_scale(ereb_ctx* c, FixGrid& F) {
+    unsigned long long mb = 0;
+    EREB_HIP(c, hipMemcpyAsync(&mb, F.maxbits, sizeof mb, hipMemcpyDeviceToHost, c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    double v;
+    std::memcpy(&v, &mb, sizeof v);
+    if (!std::isfinite(v))
+        return fail(c, EREB_ERR_DOMAIN, "adjoint: a tk2 cotangent contribution is not finite");
+    F.up = fix_scale_of(v);
+    return EREB_OK;
+}
+
+// out = value(A) + value(B) per node (B optional)
+__global__ void k_fix_to_grid(const unsigned long long* __restrict__ wa, double upa,
+                              const unsigned long long* __restrict__ wb, double upb, int64_t n,
+                              double* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double v = fix_value(wa, k, upa);
+    if (wb) v += fix_value(wb, k, upb);
+    out[k] = v;
+}
 
 // the buffers of one call, carved from the context's grow-only pool
 static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int64_t every,
                        size_t gsz) {
     const size_t M = (size_t)c->marknum;
-    const size_t sizes[9] = {5 * M,
-                             grad ? 3 * M * (size_t)nseg : 0,
-                             grad ? 3 * M * (size_t)every : 0,
-                             grad ? gsz * (size_t)every : 0,
-                             grad ? 2 * M * (size_t)every : 0,
-                             grad ? M : 0, grad ? M : 0, grad ? M : 0, grad ? gsz : 0};
+    const size_t np = (size_t)c->Nx1 * c->Ny1;
+    const size_t sizes[10] = {5 * M,
+                              grad ? 3 * M * (size_t)nseg : 0,
+                              grad ? 3 * M * (size_t)every : 0,
+                              grad ? gsz * (size_t)every : 0,
+                              grad ? 2 * M * (size_t)every : 0,
+                              grad ? M : 0, grad ? M : 0, grad ? M : 0, grad ? gsz : 0,
+                              grad ? 8 + 4 * np : 0};
     size_t tot = 0;
     for (size_t z : sizes) tot += (z + 31) / 32 * 32;  // 256-byte aligned pieces
     if (tot > c->adj_pool_n) {
@@ -331,11 +367,14 @@ static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int6
         c->adj_pool_n = tot;
     }
     double* p = c->adj_pool;
-    double** dst[9] = {&B.save, &B.cps, &B.st, &B.snap, &B.xb, &B.gT, &B.gT2, &B.gP, &B.gg};
-    for (int k = 0; k < 9; ++k) {
+    double* fix = nullptr;
+    double** dst[10] = {&B.save, &B.cps, &B.st, &B.snap, &B.xb, &B.gT, &B.gT2, &B.gP, &B.gg,
+                        &fix};
+    for (int k = 0; k < 10; ++k) {
         *dst[k] = sizes[k] ? p : nullptr;
         p += (sizes[k] + 31) / 32 * 32;
     }
+    B.fix = (unsigned long long*)fix;
     return EREB_OK;
 }
 
@@ -469,11 +508,33 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 if ((rc = state_copy(c, stk, 3, false))) return rc;
                 if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
                 if ((rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz))) return rc;
-                // RK4^T (tkm as RK4 saw it), then gather^T at timestep 1
-                if ((rc = launch_rk4_adj(c, dt, md, xb, xb + M, B.gT, B.gT2, B.gP,
-                                         G(EREB_G_TK2))))
+                // RK4^T (tkm as RK4 saw it), then gather^T at timestep 1; their
+                // tk2 cotangents are summed in fixed point (first pass: the
+                // scale; second: the sums), so the gradient is the same on
+                // every run
+                if ((rc = fill_f64(c, (double*)B.fix, 0.0, 8 + 4 * n))) return rc;
+                FixGrid FA = fix_grid(B, 0, n), FB = fix_grid(B, 1, n);
+                int* flag = (int*)(B.fix + 2);
+                if ((rc = launch_rk4_adj(c, 0, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag)) ||
+                    (rc = fix_scale(c, FA)) ||
+                    (rc = launch_rk4_adj(c, 1, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag)))
                     return rc;
-                if (k == 0 && (rc = launch_gather_T_adj(c, 1, B.gT2, G(EREB_G_TK2)))) return rc;
+                if (k == 0 && ((rc = launch_gather_T_adj(c, 0, 1, B.gT2, FB, flag)) ||
+                               (rc = fix_scale(c, FB)) ||
+                               (rc = launch_gather_T_adj(c, 1, 1, B.gT2, FB, flag))))
+                    return rc;
+                hipLaunchKernelGGL(k_fix_to_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                                   c->stream, (const unsigned long long*)FA.w, FA.up,
+                                   k == 0 ? (const unsigned long long*)FB.w : nullptr, FB.up, n,
+                                   G(EREB_G_TK2));
+                EREB_HIP(c, hipGetLastError());
+                int fl = 0;
+                EREB_HIP(c, hipMemcpyAsync(&fl, flag, sizeof fl, hipMemcpyDeviceToHost, c->stream));
+                EREB_HIP(c, hipStreamSynchronize(c->stream));
+                if (fl)
+                    return fail(c, EREB_ERR_DOMAIN,
+                                "adjoint: a tk2 cotangent contribution is outside the fixed-point "
+                                "range of its pass");
                 // tk2 := tk1, then node properties^T
                 EREB_HIP(c, hipMemcpyAsync(G(EREB_G_TK1), G(EREB_G_TK2), sizeof(double) * n,
                                            hipMemcpyDeviceToDevice, c->stream));

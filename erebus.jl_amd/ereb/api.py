@@ -292,3 +292,105 @@ def replenish_markers_(ctx: Context, xm, ym, tm, tkm, phim, sxxm, sxym, etavpm, 
         _io(mnum, np.int64, grid=True), n0, cap, ctypes.addressof(mg), int(bool(randomized)),
         int(seed), ctypes.addressof(n)), "replenish_markers!")
     return [b[:n.value].copy() for b in buf], n.value
+
+
+# ---- pullbacks (the reverse-mode kernels behind ChainRulesCore rrules) -------
+
+def _opt_in(a, n, keep, grid=False):
+    return None if a is None else _in(a, grid=grid, keep=keep)
+
+
+def _opt_out(want, shape, grid=False):
+    if not want:
+        return None
+    return np.zeros(shape, order="F") if grid else np.zeros(shape)
+
+
+def marker_to_nodes_pullback(ctx: Context, markers: dict, hrsolidm, hrfluidm, mode,
+                             sum_bar, prop_bar=None, want=None):
+    """Pullback of marker_to_nodes_ (compute_marker_properties! E:700 +
+    marker_to_*_nodes! E:1605-1863): sum_bar is a list of 30 SUM-grid
+    cotangents (None = 0), prop_bar a list of 7 property-output cotangents or
+    None.  Returns {field: cotangent} for the marker fields in `want`
+    (default: every differentiable input)."""
+    from .lib import MARKER_FIELDS
+    keep = []
+    M = markers["xm"].size
+    mptr = (ctypes.c_void_p * len(MARKER_FIELDS))()
+    for k, f in enumerate(MARKER_FIELDS):
+        if f in markers:
+            mptr[k] = _in(markers[f], np.int64 if f == "tm" else np.float64, keep=keep)
+    sb = (ctypes.c_void_p * 30)(*[None if a is None else _in(a, grid=True, keep=keep)
+                                  for a in sum_bar])
+    pb = None if prop_bar is None else (ctypes.c_void_p * 7)(
+        *[None if a is None else _in(a, keep=keep) for a in prop_bar])
+    want = want or ["xm", "ym", "tkm", "phim", "XWsolidm0", "sxxm", "sxym", "etavpm",
+                    "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
+                    "alphasolidcur", "alphafluidcur", "rhototalm", "rhocptotalm", "etatotalm",
+                    "hrtotalm", "ktotalm"]
+    out = {f: np.zeros(M) for f in want}
+    optr = (ctypes.c_void_p * len(MARKER_FIELDS))(
+        *[out[f].ctypes.data if f in out else None for f in MARKER_FIELDS])
+    hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
+    hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
+    ctx.check(ctx.lib.ereb_marker_to_nodes_pullback(ctx.h, mptr, M, hs.ctypes.data,
+                                                    hf.ctypes.data, int(mode), sb, pb, optr),
+              "marker_to_nodes pullback")
+    return out
+
+
+def update_marker_temperature_pullback(ctx: Context, xm, ym, DT, tk2, timestep, marknum,
+                                       tkm_bar):
+    """Pullback of update_marker_temperature! (E:4782): returns (tkm_bar_in,
+    DT_bar, tk2_bar, xm_bar, ym_bar)."""
+    keep = []
+    M = int(marknum)
+    g = ctx.grid_shape("tk2")
+    outs = [np.zeros(M), np.zeros(g, order="F"), np.zeros(g, order="F"), np.zeros(M),
+            np.zeros(M)]
+    ctx.check(ctx.lib.ereb_update_marker_temperature_pullback(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _opt_in(DT, M, keep, grid=True),
+        _opt_in(tk2, M, keep, grid=True), int(timestep), M, _in(tkm_bar, keep=keep),
+        *[o.ctypes.data for o in outs]), "update_marker_temperature pullback")
+    return tuple(outs)
+
+
+def update_marker_stress_pullback(ctx: Context, xm, ym, DSXX, DSXY, marknum, sxxm_bar,
+                                  sxym_bar):
+    """Pullback of update_marker_stress! (E:4354): returns (DSXX_bar,
+    DSXY_bar, xm_bar, ym_bar); the stress cotangents pass through."""
+    keep = []
+    M = int(marknum)
+    outs = [np.zeros(ctx.grid_shape("DSXX"), order="F"),
+            np.zeros(ctx.grid_shape("DSXY"), order="F"), np.zeros(M), np.zeros(M)]
+    ctx.check(ctx.lib.ereb_update_marker_stress_pullback(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _in(DSXX, grid=True, keep=keep),
+        _in(DSXY, grid=True, keep=keep), M, _in(sxxm_bar, keep=keep), _in(sxym_bar, keep=keep),
+        *[o.ctypes.data for o in outs]), "update_marker_stress pullback")
+    return tuple(outs)
+
+
+RK4_IN = ["xm", "ym", "tkm", "phim", "sxym", "sxxm"]
+RK4_GRIDS = ["vx", "vy", "vxf", "vyf", "wyx", "tk2"]
+
+
+def move_markers_rk4_pullback(ctx: Context, xm, ym, tm, tkm, phim, sxym, sxxm, vx, vy, vxf,
+                              vyf, wyx, tk2, marknum, dt, mode, bars: dict, grids=True):
+    """Pullback of move_markers_rk4! (E:4990): bars holds the cotangents of
+    the outputs (xm, ym, tkm, sxym, sxxm; missing = 0).  Returns
+    ({input marker field: cotangent}, {grid: cotangent} or {})."""
+    keep = []
+    M = int(marknum)
+    b = [_opt_in(bars.get(f), M, keep) for f in ("xm", "ym", "tkm", "sxym", "sxxm")]
+    mo = {f: np.zeros(M) for f in RK4_IN}
+    go = {}
+    if grids:
+        for n in RK4_GRIDS:
+            go[n] = np.zeros(ctx.grid_shape(n), order="F")
+    ctx.check(ctx.lib.ereb_move_markers_rk4_pullback(
+        ctx.h, _in(xm, keep=keep), _in(ym, keep=keep), _in(tm, np.int64, keep=keep),
+        _in(tkm, keep=keep), _in(phim, keep=keep), _in(sxym, keep=keep), _in(sxxm, keep=keep),
+        *[_in(a, grid=True, keep=keep) for a in (vx, vy, vxf, vyf, wyx, tk2)], M, float(dt),
+        int(mode), *b, *[mo[f].ctypes.data for f in RK4_IN],
+        *[go[n].ctypes.data if n in go else None for n in RK4_GRIDS]), "move_markers_rk4 pullback")
+    return mo, go

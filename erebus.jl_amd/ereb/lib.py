@@ -118,6 +118,10 @@ PROTOTYPES = {
     "ereb_step_transfer": [_P, _P, _P, _I64, _D, _I64],
     "ereb_ctx_set_graphs": [_P, _I],
     "ereb_step_graph_stats": [_P, _P, _P],
+    "ereb_marker_to_nodes_pullback": [_P, _P, _I64, _P, _P, _I64, _P, _P, _P],
+    "ereb_update_marker_temperature_pullback": [_P, _P, _P, _P, _P, _I64, _I64] + [_P] * 6,
+    "ereb_update_marker_stress_pullback": [_P, _P, _P, _P, _P, _I64] + [_P] * 6,
+    "ereb_move_markers_rk4_pullback": [_P] + [_P] * 13 + [_I64, _D, _I64] + [_P] * 17,
     "ereb_step_marker_viscosity": [_P],
     "ereb_step_marker_porosity": [_P, _P, _D],
     "ereb_step_melt_composition": [_P],

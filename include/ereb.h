@@ -358,6 +358,74 @@ int ereb_adjoint_tk1_loss(ereb_ctx* ctx, int64_t nsteps, int64_t checkpoint_ever
                           const double hrfluidm[3], double* loss, double* tk1_out,
                           double* grad_tkm, double* grad_phim);
 
+/* ---- pullbacks: the reverse-mode kernels behind ChainRulesCore rrules --- */
+/* Each takes the forward call's inputs (host arrays, as the drop-in entry
+ * points) and the cotangents of its outputs, recomputes what it needs (no
+ * tape is kept between calls) and writes the cotangents of its inputs.
+ * Output cotangent arguments that are NULL are zero; input cotangent
+ * arguments that are NULL are not computed.  Cotangents of grids (a marker's
+ * cotangent spread to the nodes of its stencils) are summed in fixed point,
+ * so every pullback gives the same bits on every run.  Float64 form only.
+ * The derivatives are those of the piece of each piecewise function (cell
+ * index, property branch) the inputs lie in, as forward-mode automatic
+ * differentiation of the reference gives them. */
+
+/* Pullback of ereb_marker_to_nodes (compute_marker_properties! E:700-750 and
+ * marker_to_*_nodes! E:1605-1863 over the loop E:6419-6517).  markers[f] are
+ * the marker vectors by ereb_marker_field (tm as int64; xm, ym, tm, tkm,
+ * phim required, the others as the forward call used them, NULL = 0);
+ * sum_bar[30] the cotangents of the 30 SUM grids in the order of
+ * reset_interpolated_properties! (E:968-999); prop_bar[7] (may be NULL, or
+ * hold NULLs) those of the property outputs rhototalm, rhocptotalm,
+ * etatotalm, hrtotalm, ktotalm, tkm_rhocptotalm, etafluidcur_inv_kphim.
+ * marker_bar[f] receives the cotangent of input field f: xm, ym, tkm, phim,
+ * XWsolidm0, sxxm, sxym, etavpm, inv_gggtotalm, fricttotalm, cohestotalm,
+ * tenstotalm, rhofluidcur, alphasolidcur, alphafluidcur, and, for air
+ * markers (which keep them, E:738-741), the stored rhototalm, rhocptotalm,
+ * etatotalm, hrtotalm, ktotalm (zero for rocks, whose values are replaced).
+ * The scatter's transpose is a gather: no reduction across markers. */
+int ereb_marker_to_nodes_pullback(ereb_ctx* ctx, const void* const* markers,
+    int64_t marknum, const double hrsolidm[3], const double hrfluidm[3],
+    int64_t mode, const double* const* sum_bar, const double* const* prop_bar,
+    double* const* marker_bar);
+
+/* Pullback of update_marker_temperature! E:4782-4801: tkm_bar is the
+ * cotangent of the updated tkm; tkm_bar_in receives that of the tkm passed
+ * in (zero at timestep 1), DT_bar or, at timestep 1, tk2_bar that of the
+ * grid read ((Ny1, Nx1); the other one is zeroed), xm_bar and ym_bar those
+ * of the positions. */
+int ereb_update_marker_temperature_pullback(ereb_ctx* ctx, const double* xm,
+    const double* ym, const double* DT, const double* tk2, int64_t timestep,
+    int64_t marknum, const double* tkm_bar, double* tkm_bar_in,
+    double* DT_bar, double* tk2_bar, double* xm_bar, double* ym_bar);
+
+/* Pullback of update_marker_stress! E:4354-4388 (the stress cotangents pass
+ * through unchanged): DSXX_bar ((Ny1, Nx1), P clamps 2..Nx-1, 2..Ny-1 as
+ * E:4364-4367), DSXY_bar ((Ny, Nx)), xm_bar, ym_bar. */
+int ereb_update_marker_stress_pullback(ereb_ctx* ctx, const double* xm,
+    const double* ym, const double* DSXX, const double* DSXY, int64_t marknum,
+    const double* sxxm_bar, const double* sxym_bar, double* DSXX_bar,
+    double* DSXY_bar, double* xm_bar, double* ym_bar);
+
+/* Pullback of move_markers_rk4! E:4990-5257 (argument order as the forward:
+ * sxym before sxxm; the marker arrays are the values before the step).
+ * Output cotangents xm_bar .. sxxm_bar; input cotangents of the markers
+ * (xm, ym, tkm, phim, sxym, sxxm) and of the grids vx, vy, vxf, vyf
+ * ((Ny1, Nx1)), wyx ((Ny, Nx)) and tk2 ((Ny1, Nx1)).  The reverse sweep
+ * recomputes the stage positions of the solid step and of the fluid
+ * back-trace; the centre node of the left-half correction in the fluid
+ * stages is read from vx (E:5173), so its cotangent goes to vx_bar. */
+int ereb_move_markers_rk4_pullback(ereb_ctx* ctx, const double* xm,
+    const double* ym, const int64_t* tm, const double* tkm, const double* phim,
+    const double* sxym, const double* sxxm, const double* vx, const double* vy,
+    const double* vxf, const double* vyf, const double* wyx, const double* tk2,
+    int64_t marknum, double dt, int64_t mode, const double* xm_bar,
+    const double* ym_bar, const double* tkm_bar, const double* sxym_bar,
+    const double* sxxm_bar, double* xm_bar_in, double* ym_bar_in,
+    double* tkm_bar_in, double* phim_bar, double* sxym_bar_in,
+    double* sxxm_bar_in, double* vx_bar, double* vy_bar, double* vxf_bar,
+    double* vyf_bar, double* wyx_bar, double* tk2_bar);
+
 /* ---- hydration, compaction and viscoplastic viscosity ------------------- */
 /* Constants of the compaction and silicate (de)hydration model, C:227-279
  * (plain data so that test and production constant sets can be passed). */

@@ -3,14 +3,18 @@ map (scatter, node properties, tk2 := tk1, temperature gather, RK4) with
 respect to the initial marker temperature and porosity.
 
 Pins:
+  - the exact gradient of the dual-number oracle (oracle/ereb_dual.cpp, a
+    forward-mode restatement of the oracle's map, in place of ForwardDiff of
+    the reference) at 41x41 nodes, 40,000 markers, 5 steps: every checked
+    entry within 1e-10 of it;
   - central finite differences of the GPU forward map (h = 1e-6 relative),
     agreement within 1e-6 relative to the largest gradient entry checked;
   - central finite differences of the CPU oracle's forward map (the
     restatement of the reference functions, composed the same way), the
     same bar, so the adjoint is checked against the reference's arithmetic
     and not only against the GPU forward;
-  - checkpoint spacing does not change the gradient (beyond the rounding of
-    the atomic accumulation of the tk2 cotangent).
+  - the tk2 cotangents are summed in fixed point: the gradient is the same
+    bits on every run and for every checkpoint spacing.
 The loss difference of a finite-difference pair is summed node by node as
 sum((a - b) * (a + b)) so that it keeps its relative accuracy.
 """
@@ -21,7 +25,7 @@ import pytest
 
 import ereb
 from ereb import synth
-from oracle import Oracle
+from oracle import DualOracle, Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -97,9 +101,30 @@ def test_adjoint_checkpoint_spacing_and_state(adj41):
             assert np.array_equal(ctx.get_marker(n), mk[n]), n
     assert L1 == L5 == L2
     for a in (gT5, gT2):
-        np.testing.assert_allclose(a, gT1, rtol=1e-12, atol=1e-12 * np.abs(gT1).max())
+        assert np.array_equal(a, gT1)
     for a in (gP5, gP2):
-        np.testing.assert_allclose(a, gP1, rtol=1e-12, atol=1e-12 * np.abs(gP1).max())
+        assert np.array_equal(a, gP1)
+
+
+def test_adjoint_vs_dual_exact_gradient(adj41):
+    """Per entry against the exact (dual-number) gradient of the oracle's map,
+    and bit-identical on a second run."""
+    g, mat, mk, hrs, hrf = adj41
+    nsteps, every, mode = 5, 2, 1
+    rng = np.random.default_rng(9)
+    idx = _pick(mk, 48, rng)
+    with ereb.Context(g, mat, ereb.FP64) as ctx:
+        vx, vy = _setup(ctx, g, mk)
+        L, gT, gP, _ = ctx.adjoint_tk1_loss(nsteps, every, DT, mode, hrs, hrf)
+        L2, gT2, gP2, _ = ctx.adjoint_tk1_loss(nsteps, every, DT, mode, hrs, hrf)
+    assert L == L2 and np.array_equal(gT, gT2) and np.array_equal(gP, gP2)
+    seeds = [(int(m), 0) for m in idx] + [(int(m), 1) for m in idx]
+    Ld, gd = DualOracle(g, mat).tk1_loss_grad(mk, vx, vy, nsteps, DT, mode, hrs, hrf, seeds)
+    assert abs(L - Ld) <= 1e-12 * Ld
+    n = len(idx)
+    for name, got, ref in (("tkm", gT[idx], gd[:n]), ("phim", gP[idx], gd[n:])):
+        err = np.abs(got - ref) / (np.abs(ref) + 1e-6 * np.abs(ref).max())
+        assert err.max() <= 1e-10, (name, float(err.max()))
 
 
 def _oracle_tk1(orc, g, mk, vx, vy, nsteps, mode, hrs, hrf):
