@@ -27,6 +27,7 @@ SOURCES = [
     "ereb_ctx.hip", "ereb_scan.hip", "ereb_sort.hip", "ereb_scatter.hip",
     "ereb_nodes.hip", "ereb_markers.hip", "ereb_assembly.hip", "ereb_adjoint.hip",
     "ereb_hydration.hip", "ereb_backtrace.hip", "ereb_replenish.hip",
+    "ereb_pullback.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -20,9 +20,18 @@
 //              (the reverse slot table) in ascending row order and counts
 //              the present entries; an exclusive scan gives the column
 //              pointers;
-//   3. fill  : the same walk writes rowval/nzval of each column contiguously.
+//   3. fill  : the same walk writes rowval/nzval of each column contiguously,
+//              and for every stored entry the index of its slot value.
 // No atomics and no sort: rows come out ascending in every column.
+// Between 1 and 2 the presence bits are compared with the previous
+// assembly's (k_pattern_check, a device flag): when the pattern is unchanged
+// -- the common case from one timestep or plastic iteration to the next --
+// steps 2 and 3 return at once and k_asm_values refills nzval with one
+// gather through the stored slot indices, colptr and rowval being those of
+// the previous assembly.  The condition is evaluated on the device, so a
+// captured step graph takes either path.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "ereb_internal.h"
@@ -346,10 +355,28 @@ __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A0, const StepScalars*
 // 2. column counts: one thread per column walks the rows that can hold an
 // entry in it (the reverse slot table of its unknown), in ascending row
 // order, and counts the present ones; the slot tables are staged in LDS
+// chg[0] = 1 when the presence bits differ from the previous assembly's (the
+// host sets it to 1 when there is none); prev is then updated.  Every writer
+// stores the same value.
+__global__ __launch_bounds__(256) void k_pattern_check(const uint16_t* __restrict__ pres,
+                                                       uint16_t* __restrict__ prev, int64_t n,
+                                                       int32_t* __restrict__ chg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t a = pres[i];
+    if (a != prev[i]) {
+        chg[0] = 1;
+        prev[i] = a;
+    }
+}
+
+// run (optional): the kernel does its work only when run[0] != 0
 __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
                                                    const SlotTables* __restrict__ Tg,
                                                    const uint16_t* __restrict__ pres,
-                                                   int32_t* __restrict__ colcnt) {
+                                                   int32_t* __restrict__ colcnt,
+                                                   const int32_t* __restrict__ run) {
+    if (run && run[0] == 0) return;
     __shared__ SlotTables T;
     static_assert(sizeof(SlotTables) % 4 == 0, "table size");
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
@@ -386,16 +413,24 @@ constexpr int ASM_STAGE = 4096;
 // presence bit and value are loaded before any is used (a value whose bit is
 // clear is read from its allocated slot and dropped), so a column costs one
 // memory round trip instead of one per candidate.
-template <int MAXR>
+// IDX, BASE: the row index type and base of the output (int32 0-based for
+// the context's device CSC, Int64 1-based for SparseMatrixCSC{Float64,Int64})
+// run (optional): only when run[0] != 0; map (optional): the slot value's
+// index of every stored entry, for the values-only refill
+template <int MAXR, class IDX = int32_t, int BASE = 0>
 __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                                                   const SlotTables* __restrict__ Tg,
                                                   const uint16_t* __restrict__ pres,
                                                   const double* __restrict__ slotv,
                                                   const int32_t* __restrict__ colptr,
-                                                  int32_t* __restrict__ rowval,
-                                                  double* __restrict__ nzval) {
+                                                  IDX* __restrict__ rowval,
+                                                  double* __restrict__ nzval,
+                                                  const int32_t* __restrict__ run,
+                                                  int32_t* __restrict__ map) {
+    if (run && run[0] == 0) return;
     __shared__ SlotTables T;
     __shared__ int32_t srow[ASM_STAGE];
+    __shared__ int32_t smap[ASM_STAGE];
     __shared__ double sval[ASM_STAGE];
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
         ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
@@ -412,19 +447,20 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
         const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
         const int nr = T.nrev[u];
         bool on[MAXR];
-        int32_t rw[MAXR];
+        int32_t rw[MAXR], mp[MAXR];
         double vv[MAXR];
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             on[r] = false;
-            rw[r] = 0;
+            rw[r] = mp[r] = 0;
             vv[r] = 0.0;
             const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
             if (r < nr && ir >= 0 && ir < Ny1 && jr >= 0 && jr < Nx1) {
                 const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
                 const int64_t cell = (int64_t)jr * Ny1 + ir;
                 on[r] = (pres[(size_t)e * ncell + cell] >> sl) & 1;
-                vv[r] = slotv[((size_t)T.slot0[e] + sl) * ncell + cell];
+                mp[r] = (int32_t)(((int64_t)T.slot0[e] + sl) * ncell + cell);
+                vv[r] = slotv[mp[r]];
                 rw[r] = (int32_t)(cell * ne + e);
             }
         }
@@ -433,6 +469,7 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
         for (int r = 0; r < MAXR; ++r) {
             if (on[r]) {
                 srow[p] = rw[r];
+                smap[p] = mp[r];
                 sval[p] = vv[r];
                 ++p;
             }
@@ -440,9 +477,23 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
     }
     __syncthreads();
     for (int q = threadIdx.x; q < total; q += blockDim.x) {
-        rowval[base + q] = srow[q];
+        rowval[base + q] = (IDX)srow[q] + BASE;
         nzval[base + q] = sval[q];
+        if (map) map[base + q] = smap[q];
     }
+}
+
+// nzval refilled through the stored slot indices when the pattern is
+// unchanged (chg[0] == 0): one coalesced write, one gather per entry
+__global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ chg,
+                                                    const int32_t* __restrict__ colptr_end,
+                                                    const int32_t* __restrict__ map,
+                                                    const double* __restrict__ slotv,
+                                                    double* __restrict__ nzval) {
+    if (chg[0] != 0) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= colptr_end[0]) return;
+    nzval[q] = slotv[map[q]];
 }
 
 // ---- slot tables, discovered on the host from the equation code -----------
@@ -592,10 +643,16 @@ static int assemble(ereb_ctx* c, double dt) {
     const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, c->d_scal, T, s.pres,
                        s.slotv, c->d_flags);
+    // pattern unchanged since the previous assembly?  (with none, changed)
+    if ((rc = fill_i32(c, s.chg, s.have_prev ? 0 : 1, 1))) return rc;
     const dim3 cgrid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_pattern_check, cgrid, dim3(256), 0, c->stream,
+                       (const uint16_t*)s.pres, s.pres_prev, n, s.chg);
+    s.have_prev = true;
+    const int32_t* chg = s.chg;
     hipLaunchKernelGGL(k_asm_count, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
-                       s.colcnt);
-    if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n))) return rc;
+                       s.colcnt, chg);
+    if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n, nullptr, false, chg))) return rc;
     // a block's entries must fit the LDS stage: 256 columns of up to 16
     // entries, else 128 of up to 32 (SLOT_R)
     const bool narrow = s.maxrev <= ASM_STAGE / 256;
@@ -605,10 +662,15 @@ static int assemble(ereb_ctx* c, double dt) {
     const dim3 fgrid((unsigned)((n + fb - 1) / fb));
     if (narrow)
         hipLaunchKernelGGL(k_asm_fill<16>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
-                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval);
+                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval, chg,
+                           s.map);
     else
         hipLaunchKernelGGL(k_asm_fill<SLOT_R>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
-                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval);
+                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval, chg,
+                           s.map);
+    hipLaunchKernelGGL(k_asm_values, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0,
+                       c->stream, chg, (const int32_t*)s.colptr + n, (const int32_t*)s.map,
+                       (const double*)s.slotv, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;

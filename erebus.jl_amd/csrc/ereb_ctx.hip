@@ -85,12 +85,16 @@ int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell) {
     hipFree(s.pres_fetched);
     hipFree(s.diff);
     hipFree(s.stage);
-    s.colcnt = s.colptr = s.rowval = s.diff = nullptr;
-    s.pres = s.pres_fetched = nullptr;
+    hipFree(s.pres_prev);
+    hipFree(s.chg);
+    hipFree(s.map);
+    s.colcnt = s.colptr = s.rowval = s.diff = s.chg = s.map = nullptr;
+    s.pres = s.pres_fetched = s.pres_prev = nullptr;
     s.slotv = s.nzval = s.rhs = nullptr;
     s.stage = nullptr;
     s.valid = false;
     s.have_fetched = false;
+    s.have_prev = false;
     EREB_HIP(c, hipMalloc(&s.colcnt, sizeof(int32_t) * n));
     EREB_HIP(c, hipMalloc(&s.colptr, sizeof(int32_t) * (n + 1)));
     EREB_HIP(c, hipMalloc(&s.pres, sizeof(uint16_t) * n));
@@ -101,6 +105,9 @@ int ensure_sys(ereb_ctx* c, int which, int64_t n, int64_t cap, int64_t ncell) {
     EREB_HIP(c, hipMalloc(&s.pres_fetched, sizeof(uint16_t) * n));
     EREB_HIP(c, hipMalloc(&s.diff, sizeof(int32_t)));
     EREB_HIP(c, hipMalloc(&s.stage, sizeof(int64_t) * (std::max(n, cap) + 1)));
+    EREB_HIP(c, hipMalloc(&s.pres_prev, sizeof(uint16_t) * n));
+    EREB_HIP(c, hipMalloc(&s.chg, sizeof(int32_t)));
+    EREB_HIP(c, hipMalloc(&s.map, sizeof(int32_t) * cap));
     s.n = n;
     s.cap = cap;
     s.ncell = ncell;
@@ -337,6 +344,7 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     CK(hipMalloc(&c->d_mat, sizeof(ereb_materials)));
     CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_scal, sizeof(StepScalars)));
+    CK(hipMalloc(&c->d_rkd, sizeof(double) * 5 * (size_t)(Nx1 * Ny1)));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
     CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
     // grids, 256-byte aligned, after 32 guard elements (the RK4 stencils read
@@ -385,6 +393,8 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_axes);
     hipFree(c->d_mat);
     hipFree(c->d_scal);
+    hipFree(c->d_rkd);
+    hipFree(c->pb_pool);
     if (c->step_exec) hipGraphExecDestroy(c->step_exec);
     hipFree(c->adj_pool);
     hipFree(c->rep_pool);
@@ -413,6 +423,9 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.pres_fetched);
         hipFree(s.diff);
         hipFree(s.stage);
+        hipFree(s.pres_prev);
+        hipFree(s.chg);
+        hipFree(s.map);
     }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

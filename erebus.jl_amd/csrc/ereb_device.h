@@ -201,6 +201,7 @@ __device__ __forceinline__ double d_kfluidm(const ereb_materials& M, double T, i
 // partial derivatives of the properties that depend on T or phi
 struct PropsD {
     double rho_p;                  // d rhototal / d phi (d/dT is zero: step in T)
+    double rho_xw;                 // d rhototal / d XWsolidm0
     double rhocp_t, rhocp_p;       // rhocptotal
     double hr_p;                   // hrtotal
     double k_t, k_p;               // ktotal
@@ -218,6 +219,10 @@ __device__ __forceinline__ PropsD marker_props_d(const ereb_materials& M, int t,
         const double rhosolidm0 = (M.MD + M.MH2O * XW) / (M.VDs * XD + M.VWs * XW);
         const double rhofluidm0 = T > M.tmfluidphase ? M.rhoH2Of : M.rhoH2Ofi;
         d.rho_p = rhofluidm0 - rhosolidm0;
+        // rhosolidm0 = (MD + MH2O XW) / (VDs (1 - XW) + VWs XW)
+        const double den = M.VDs * XD + M.VWs * XW;
+        d.rho_xw = (1.0 - phi) * (M.MH2O * den - (M.MD + M.MH2O * XW) * (M.VWs - M.VDs)) /
+                   (den * den);
         const double rcs = sel3(M.rhocpsolidm, t), rcf = rhocpfluidm(M, T, mode);
         d.rhocp_t = phi * d_rhocpfluidm(M, T, mode);
         d.rhocp_p = rcf - rcs;

@@ -9,6 +9,7 @@
 
 #include "../../include/ereb.h"
 #include "ereb_device.h"
+#include "ereb_fixsum.h"
 
 namespace ereb {
 
@@ -39,6 +40,13 @@ struct Sys {
     int32_t* diff = nullptr;           // 1
     int64_t* stage = nullptr;          // cap + 1
     bool have_fetched = false;
+    // values-only assembly: the pattern of the previous assembly, a device
+    // flag (pattern changed), and for every stored entry its slot value's
+    // index, so that an unchanged pattern refills nzval with one gather
+    uint16_t* pres_prev = nullptr;     // n
+    int32_t* chg = nullptr;            // 1
+    int32_t* map = nullptr;            // cap
+    bool have_prev = false;
 };
 
 struct ProfPair {
@@ -78,6 +86,7 @@ struct ereb_ctx {
     void* d_m[EREB_M_NFIELDS] = {};
 
     double* d_grids = nullptr;
+    double* d_rkd = nullptr;  // 5 x Ny1*Nx1: second differences of the RK4 velocities
     size_t goff[EREB_G_NFIELDS] = {};
     int64_t gnumel[EREB_G_NFIELDS] = {};
     uint8_t* d_yny = nullptr;
@@ -112,6 +121,8 @@ struct ereb_ctx {
 
     double* adj_pool = nullptr;  // grow-only device pool of the adjoint driver
     size_t adj_pool_n = 0;       // doubles
+    void* pb_pool = nullptr;     // grow-only device pool of the pullbacks
+    size_t pb_bytes = 0;
 
     // marker replenishment: grow-only scratch, and the last call's lattice
     // results (views into it)
@@ -175,8 +186,15 @@ int prof_collect(ereb_ctx* c);
 // ---- kernel launchers ---------------------------------------------------
 // ereb_scan.hip: exclusive scan of n int32 into out[0..n], out[n] = total;
 // optionally a copy of out[0..n) in out2 and in[] zeroed once read
+// optionally only when run[0] != 0 (a device-side condition)
 int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n,
-                   int32_t* out2 = nullptr, bool zero_in = false);
+                   int32_t* out2 = nullptr, bool zero_in = false,
+                   const int32_t* run = nullptr);
+// the same with a caller-provided block-sum buffer of scan_blocks(n) ints
+int64_t scan_blocks(int64_t n);
+int scan_exclusive_buf(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n,
+                       int32_t* blocksums, int32_t* out2 = nullptr, bool zero_in = false,
+                       const int32_t* run = nullptr);
 // fill kernel (graph-safe replacement for hipMemsetAsync)
 int fill_i32(ereb_ctx* c, int32_t* p, int32_t v, int64_t n);
 int fill_f64(ereb_ctx* c, double* p, double v, int64_t n);
@@ -238,8 +256,12 @@ int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mo
                        const double* gsum_base, const int64_t* gsum_off, double* gT,
                        double* gphi, bool only_tk);
 int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off);
-int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid);
-int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,
-                   const double* gout, double* gin, double* gphi, double* gtk2);
+// the grid cotangent in fixed point (ereb_fixsum.h): pass 0 records the
+// largest contribution, pass 1 adds (and writes the marker cotangents)
+int launch_gather_T_adj(ereb_ctx* c, int pass, int64_t timestep, double* gT,
+                        const FixGrid& ggrid, int* flag);
+int launch_rk4_adj(ereb_ctx* c, int pass, double dt, int mode, const double* xb,
+                   const double* yb, const double* gout, double* gin, double* gphi,
+                   const FixGrid& gtk2, int* flag);
 
 }  // namespace ereb

@@ -10,6 +10,7 @@
 // sums may fuse into fma, so positions and temperatures agree with the
 // reference to ~1e-15 relative rather than bit for bit; cell indices
 // computed from given positions stay exact.
+#include "ereb_fixsum.h"
 #include "ereb_internal.h"
 
 namespace ereb {
@@ -91,82 +92,109 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
     sxym[m] = sxy;
 }
 
+// Second differences of the stage velocities for the RK4 corrections
+// (E:5068-5088, E:5104-5124), one thread per node of the (Ny1, Nx1) grids,
+// in the reference's association (q0 - 2 c) + q2:
+//   DX[i, j]  = vx[i, j-1] - 2 vx[i, j] + vx[i, j+1]     for 2 <= j <= Nx-1
+//   DXf       = the same of vxf
+//   DXq[i, j] = vxf[i, j-1] - 2 vx[i, j] + vxf[i, j+1]   (E:5173)
+//   DY[i, j]  = vy[i-1, j] - 2 vy[i, j] + vy[i+1, j]     for 2 <= i <= Ny-1
+//   DYf       = the same of vyf
+// and zero elsewhere: a correction centred there is the one the reference
+// skips (j == 1 left, j == Nx-1 right; i == 1 up, i == Ny-1 down).
+__global__ __launch_bounds__(256) void k_rk4_prep(int Nx, int Ny, const double* __restrict__ vx,
+                                                  const double* __restrict__ vy,
+                                                  const double* __restrict__ vxf,
+                                                  const double* __restrict__ vyf,
+                                                  double* __restrict__ DX, double* __restrict__ DXf,
+                                                  double* __restrict__ DXq, double* __restrict__ DY,
+                                                  double* __restrict__ DYf) {
+    const int R = Ny + 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= R * (Nx + 1)) return;
+    const int i = k % R + 1, j = k / R + 1;
+    const bool cx = j >= 2 && j <= Nx - 1, cy = i >= 2 && i <= Ny - 1;
+    DX[k] = cx ? (vx[k - R] - 2.0 * vx[k]) + vx[k + R] : 0.0;
+    DXf[k] = cx ? (vxf[k - R] - 2.0 * vxf[k]) + vxf[k + R] : 0.0;
+    DXq[k] = cx ? (vxf[k - R] - 2.0 * vx[k]) + vxf[k + R] : 0.0;
+    DY[k] = cy ? (vy[k - 1] - 2.0 * vy[k]) + vy[k + 1] : 0.0;
+    DYf[k] = cy ? (vyf[k - 1] - 2.0 * vyf[k]) + vyf[k + 1] : 0.0;
+}
+
 // RK4 region: a*b + c may be fused (the quotients above already make the
 // results agree to rounding rather than bit for bit)
 #pragma clang fp contract(fast)
 
-// RK4 stage velocity on the Vx grid (E:5057-5091).  `vc` supplies the
-// centre value of the left-half correction: the reference reads vx there
-// also in the fluid loop (E:5173).  Every load of the stencil (rows i, i+1
-// of columns j-1 .. j+2) is issued before any is used and the second-order
-// correction is selected, not branched to, so a stage costs one memory round
-// trip.
-template <bool EXACT>
-__device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
-                                        const double* __restrict__ vc, double x, double y) {
-    const int R = g.Ny1;
-    const double x0 = g.x[0], y0 = g.yp[0];
-    const int j = jl_cell((x - x0) * g.idx, 1, g.Nx - 1);
-    const int i = jl_cell((y - y0) * g.idy, 1, g.Ny);
-    const double dxmj = x - axis_node<EXACT>(g.x, g.dx, j);
-    const double dymi = y - axis_node<EXACT>(g.yp, g.dy, i);
-    const double idx = g.idx, idy = g.idy;
-    const double* p = gptr_at(v, i, j, R);  // v[i, j]
-    // column j-1 (read at j only where j = 1: it is unused there); column
-    // j+2 <= Nx+1 always exists
-    const double* pl = j > 1 ? p - R : p;
-    const double l0 = pl[0], l1 = pl[1];
-    const double a0 = p[0], a1 = p[1], b0 = p[R], b1 = p[R + 1];
-    const double c0 = p[2 * R], c1 = p[2 * R + 1];
-    const double vcij = *gptr_at(vc, i, j, R);
-    const double fx = dxmj * idx, fy = dymi * idy;
-    double v13 = a0 * (1.0 - fx) + (b0 * dxmj) * idx;
-    double v24 = a1 * (1.0 - fx) + (b1 * dxmj) * idx;
-    // second-order correction on columns j..j+2 (fx >= 0.5) or j-1..j+1
-    const bool right = fx >= 0.5;
-    const bool apply = right ? j < g.Nx - 1 : j > 1;
-    const double t = fx - 0.5, h = 0.5 * (t * t);
-    const double q0 = right ? a0 : l0, q1 = right ? a1 : l1;    // first column
-    const double q20 = right ? c0 : b0, q21 = right ? c1 : b1;  // third column
-    const double c13 = right ? b0 : vcij, c24 = right ? b1 : a1;
-    const double n13 = v13 + h * (q0 - 2.0 * c13 + q20);
-    const double n24 = v24 + h * (q1 - 2.0 * c24 + q21);
-    v13 = apply ? n13 : v13;
-    v24 = apply ? n24 : v24;
-    return v13 * (1.0 - fy) + (v24 * dymi) * idy;
+// Cell of an RK4 stage position: the clamp to [lo - 1, hi - 1] before the
+// conversion equals fix's truncate-then-clamp (E:1457-1468) for every t
+// below 2^63 in magnitude, NaN included (it maps to lo in both); stage
+// positions are those of the marker moved by dt * velocity.
+__device__ __forceinline__ int stage_cell(double t, int lo, int hi) {
+    return (int)fmin(fmax(t, (double)(lo - 1)), (double)(hi - 1)) + 1;
 }
 
-// RK4 stage velocity on the Vy grid (E:5093-5127), rows i-1 .. i+2 of
-// columns j, j+1 loaded together as in rk_vx.  Row i-1 at i = 1 is unused:
-// the address is then the previous column's last row or, for the first
-// column, one of the guard elements the context keeps before its grids.
+// Fraction of a position within its cell along an axis: t = (x - x0) / h
+// less the cell's index when the axis nodes are x0 + k h (EXACT), else from
+// the loaded node.
 template <bool EXACT>
-__device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v, double x,
-                                        double y) {
+__device__ __forceinline__ double stage_frac(const double* __restrict__ ax, double x, double t,
+                                             double ih, int k) {
+    if constexpr (EXACT) return t - (double)(k - 1);
+    else return (x - ax[k - 1]) * ih;
+}
+
+// RK4 stage velocity on the Vx grid (E:5057-5091):
+//   v13 = v[i,j] (1 - fx) + v[i,j+1] fx + h (v[i,j-1] - 2 v[i,j] + v[i,j+1])
+// (the correction on columns j..j+2 instead when fx >= 0.5; only where
+// those columns exist), the same for row i+1, then v13 (1 - fy) + v24 fy.
+// The second differences are read from grids prepared once per step
+// (k_rk4_prep): D at the correction's centre column, zero where the
+// reference skips the correction, and Dq for the first row of the left-half
+// correction, whose centre the reference reads from vx also in the fluid
+// loop (E:5173); Dq == D in the solid loop.  Six loads, no selects of
+// values.
+template <bool EXACT>
+__device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
+                                        const double* __restrict__ D,
+                                        const double* __restrict__ Dq, double x, double y) {
     const int R = g.Ny1;
-    const double x0 = g.xp[0], y0 = g.y[0];
-    const int j = jl_cell((x - x0) * g.idx, 1, g.Nx);
-    const int i = jl_cell((y - y0) * g.idy, 1, g.Ny - 1);
-    const double dxmj = x - axis_node<EXACT>(g.xp, g.dx, j);
-    const double dymi = y - axis_node<EXACT>(g.y, g.dy, i);
-    const double idx = g.idx, idy = g.idy;
-    const double* p = gptr_at(v, i, j, R);
-    const double am = p[-1], a0 = p[0], a1 = p[1], a2 = p[2];
-    const double bm = p[R - 1], b0 = p[R], b1 = p[R + 1], b2 = p[R + 2];
-    const double fx = dxmj * idx, fy = dymi * idy;
-    double v12 = a0 * (1.0 - fy) + (a1 * dymi) * idy;
-    double v34 = b0 * (1.0 - fy) + (b1 * dymi) * idy;
-    // second-order correction on rows i..i+2 (fy >= 0.5) or i-1..i+1
+    const double tx = (x - g.x[0]) * g.idx, ty = (y - g.yp[0]) * g.idy;
+    const int j = stage_cell(tx, 1, g.Nx - 1);
+    const int i = stage_cell(ty, 1, g.Ny);
+    const double fx = stage_frac<EXACT>(g.x, x, tx, g.idx, j);
+    const double fy = stage_frac<EXACT>(g.yp, y, ty, g.idy, i);
+    const uint32_t k = (uint32_t)((j - 1) * R + (i - 1));
+    const bool right = fx >= 0.5;
+    const uint32_t kc = right ? k + R : k;
+    const double a0 = v[k], a1 = v[k + 1], b0 = v[k + R], b1 = v[k + R + 1];
+    const double d0 = (right ? D : Dq)[kc], d1 = D[kc + 1];
+    const double t = fx - 0.5, h = 0.5 * (t * t);
+    const double v13 = a0 + fx * (b0 - a0) + h * d0;
+    const double v24 = a1 + fx * (b1 - a1) + h * d1;
+    return v13 + fy * (v24 - v13);
+}
+
+// RK4 stage velocity on the Vy grid (E:5093-5127), the transpose: second
+// differences along y at the correction's centre row (D, zero where the
+// reference skips it).
+template <bool EXACT>
+__device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v,
+                                        const double* __restrict__ D, double x, double y) {
+    const int R = g.Ny1;
+    const double tx = (x - g.xp[0]) * g.idx, ty = (y - g.y[0]) * g.idy;
+    const int j = stage_cell(tx, 1, g.Nx);
+    const int i = stage_cell(ty, 1, g.Ny - 1);
+    const double fx = stage_frac<EXACT>(g.xp, x, tx, g.idx, j);
+    const double fy = stage_frac<EXACT>(g.y, y, ty, g.idy, i);
+    const uint32_t k = (uint32_t)((j - 1) * R + (i - 1));
     const bool down = fy >= 0.5;
-    const bool apply = down ? i < g.Ny - 1 : i > 1;
+    const uint32_t kc = down ? k + 1 : k;
+    const double a0 = v[k], a1 = v[k + 1], b0 = v[k + R], b1 = v[k + R + 1];
+    const double d0 = D[kc], d1 = D[kc + R];
     const double t = fy - 0.5, h = 0.5 * (t * t);
-    const double q0 = down ? a0 : am, q1 = down ? a1 : a0, q2 = down ? a2 : a1;
-    const double r0 = down ? b0 : bm, r1 = down ? b1 : b0, r2 = down ? b2 : b1;
-    const double n12 = v12 + h * (q0 - 2.0 * q1 + q2);
-    const double n34 = v34 + h * (r0 - 2.0 * r1 + r2);
-    v12 = apply ? n12 : v12;
-    v34 = apply ? n34 : v34;
-    return v12 * (1.0 - fx) + (v34 * dxmj) * idx;
+    const double v12 = a0 + fy * (a1 - a0) + h * d0;
+    const double v34 = b0 + fy * (b1 - b0) + h * d1;
+    return v12 + fx * (v34 - v12);
 }
 
 struct RK4Args {
@@ -179,6 +207,9 @@ struct RK4Args {
     const int32_t* tm;
     const double* phim;
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
+    // second differences of the stage velocities (k_rk4_prep): vx, vxf, the
+    // fluid loop's left-half first row (centre from vx), vy, vyf
+    const double *DX, *DXf, *DXq, *DY, *DYf;
     double *xb_out, *yb_out;  // optional: fluid back-trace end points (adjoint recording)
     // fused gathers (k_rk4<1 or 2>): DSXX, DSXY and the DT grid (the
     // temperature gather reads tk2 at timestep 1, else DT)
@@ -198,8 +229,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     double vxr[4], vyr[4];
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vx, A.vx, xr, yr);
-        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vy, xr, yr);
+        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vx, A.DX, A.DX, xr, yr);
+        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vy, A.DY, xr, yr);
         if (rk < 3) {
             xr = xmm + dt * cr[rk] * vxr[rk];
             yr = ymm + dt * cr[rk] * vyr[rk];
@@ -211,8 +242,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     ymm = yr = yn;
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vxf, A.vx, xr, yr);
-        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vyf, xr, yr);
+        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vxf, A.DXf, A.DXq, xr, yr);
+        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vyf, A.DYf, xr, yr);
         if (rk < 3) {
             xr = xmm - dt * cr[rk] * vxr[rk];
             yr = ymm - dt * cr[rk] * vyr[rk];
@@ -301,31 +332,34 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
 
 // ---- adjoints (with respect to tkm, phim and the tk2 grid) ---------------
 
-__device__ __forceinline__ void atomic_add4(double* gr, int rows, int i, int j, const double* w,
-                                            double v) {
-    double* p = gr + (size_t)(j - 1) * rows + (i - 1);
-    atomicAdd(p, w[0] * v);
-    atomicAdd(p + 1, w[1] * v);
-    atomicAdd(p + rows, w[2] * v);
-    atomicAdd(p + rows + 1, w[3] * v);
-}
+constexpr int ADJ_WIN = 2048;  // LDS words of the workgroup pre-aggregation (2 per node)
 
-constexpr int ADJ_WIN = 2048;  // LDS window of the workgroup pre-aggregation (doubles)
-
-struct AddWin {
-    double v[ADJ_WIN];
+struct FixWin {
+    unsigned long long v[ADJ_WIN];
     int box[4];  // imin, imax, jmin, jmax (1-based cells)
 };
 
-// Workgroup-wide gr[stencil of (i, j)] += w * v (bilinear, `rows` rows): the
-// contributions are first summed in an LDS window over the workgroup's cells
-// (LDS atomics), then added to gr with one atomic per touched node; when the
-// cells span more than the window, every lane adds straight to gr.  Every
-// thread of the workgroup must call it (it synchronises); `on` marks the
-// lanes that contribute.
-__device__ void block_add4(double* gr, int rows, bool on, int i, int j, const double* w,
-                           double v, AddWin& W) {
+// Workgroup-wide F[stencil of (i, j)] += w * v (bilinear, `rows` rows) in
+// fixed point (ereb_fixsum.h), so the sums do not depend on the order of
+// the atomics.  PASS 0 records the largest |w v| (one atomic per wavefront).
+// PASS 1 sums the contributions' words in an LDS window over the
+// workgroup's cells (integer LDS atomics), then adds each touched node's
+// words to F with one atomic pair; when the cells span more than the
+// window, every lane adds straight to F.  Every thread of the workgroup
+// must call it (PASS 1 synchronises); `on` marks the lanes that contribute.
+template <int PASS>
+__device__ void block_fix_add4(const FixGrid& F, int rows, bool on, int i, int j,
+                               const double* w, double v, FixWin& W, int* flag) {
     on = on && v != 0.0;
+    if (PASS == 0) {
+        double mx = 0.0;
+        if (on)
+            for (int c = 0; c < 4; ++c) mx = fmax(mx, fabs(w[c] * v));
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if (__lane_id() == 0 && mx > 0.0)
+            atomicMax(F.maxbits, (unsigned long long)__double_as_longlong(mx));
+        return;
+    }
     int imin = on ? i : INT32_MAX, imax = on ? i : INT32_MIN;
     int jmin = on ? j : INT32_MAX, jmax = on ? j : INT32_MIN;
     for (int o = 32; o > 0; o >>= 1) {
@@ -352,39 +386,56 @@ __device__ void block_add4(double* gr, int rows, bool on, int i, int j, const do
     const int i0 = W.box[0], i1 = W.box[1], j0 = W.box[2], j1 = W.box[3];
     if (i0 > i1) return;  // uniform: no contribution in the workgroup
     const int h = i1 - i0 + 2, wdt = j1 - j0 + 2;  // nodes i0..i1+1, j0..j1+1
-    if ((int64_t)h * wdt > ADJ_WIN) {
-        if (on) atomic_add4(gr, rows, i, j, w, v);
+    if (2 * (int64_t)h * wdt > ADJ_WIN) {
+        if (on) {
+            const int64_t k = (int64_t)(j - 1) * rows + (i - 1);
+            fix_add<1>(F, k, w[0] * v, flag);
+            fix_add<1>(F, k + 1, w[1] * v, flag);
+            fix_add<1>(F, k + rows, w[2] * v, flag);
+            fix_add<1>(F, k + rows + 1, w[3] * v, flag);
+        }
         return;
     }
-    for (int q = threadIdx.x; q < h * wdt; q += blockDim.x) W.v[q] = 0.0;
+    for (int q = threadIdx.x; q < 2 * h * wdt; q += blockDim.x) W.v[q] = 0ull;
     __syncthreads();
     if (on) {
-        double* p = W.v + (j - j0) * h + (i - i0);
-        atomicAdd(p, w[0] * v);
-        atomicAdd(p + 1, w[1] * v);
-        atomicAdd(p + h, w[2] * v);
-        atomicAdd(p + h + 1, w[3] * v);
+        const int q0 = (j - j0) * h + (i - i0);
+        const int qs[4] = {q0, q0 + 1, q0 + h, q0 + h + 1};
+        for (int c = 0; c < 4; ++c) {
+            unsigned long long H, L;
+            if (!fix_split(w[c] * v, F.up, H, L)) {
+                atomicOr(flag, 1);
+                continue;
+            }
+            atomicAdd(&W.v[2 * qs[c]], H);
+            atomicAdd(&W.v[2 * qs[c] + 1], L);
+        }
     }
     __syncthreads();
     for (int q = threadIdx.x; q < h * wdt; q += blockDim.x) {
-        const double a = W.v[q];
-        if (a != 0.0) atomicAdd(&gr[(size_t)(j0 - 1 + q / h) * rows + (i0 - 1 + q % h)], a);
+        const unsigned long long H = W.v[2 * q], L = W.v[2 * q + 1];
+        if (H | L) {
+            const int64_t k = (int64_t)(j0 - 1 + q / h) * rows + (i0 - 1 + q % h);
+            atomicAdd(&F.w[2 * k], H);
+            atomicAdd(&F.w[2 * k + 1], L);
+        }
     }
     __syncthreads();  // the window is reused by the next call
 }
 
 // RK4^T of the temperature correction E:5243-5252: given the cotangent gout of
 // tkm after the step, the cotangent of tkm before it (gin), the increment of
-// the phim cotangent and the tk2 cotangent (both stencils, by atomics).
-// Positions and stresses do not depend on tkm, phim or tk2; the fluid
-// back-trace end points (xb, yb) were recorded by the forward step.
+// the phim cotangent (PASS 1) and the tk2 cotangent (both stencils, in fixed
+// point).  Positions and stresses do not depend on tkm, phim or tk2; the
+// fluid back-trace end points (xb, yb) were recorded by the forward step.
+template <int PASS>
 __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __restrict__ xbs,
                                                  const double* __restrict__ ybs,
                                                  const double* __restrict__ gout,
                                                  double* __restrict__ gin,
-                                                 double* __restrict__ gphi,
-                                                 double* __restrict__ gtk2) {
-    __shared__ AddWin W;
+                                                 double* __restrict__ gphi, FixGrid gtk2,
+                                                 int* flag) {
+    __shared__ FixWin W;
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < A.n;
     const int64_t m = on ? m0 : A.n - 1;  // idle lanes read a valid marker, add nothing
@@ -408,13 +459,13 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
                        num * (ph * drcf) / (den * den);
     const double F_p = (-T * rcs + (T + d) * rcf) / den - num * (rcf - rcs) / (den * den);
     const double go = gout[m];
-    if (on) {
+    if (on && PASS == 1) {
         gin[m] = go * F_T;
         gphi[m] += go * F_p;
     }
     const double gd = go * F_d;
-    block_add4(gtk2, g.Ny1, on, ipb, jpb, wb, gd, W);
-    block_add4(gtk2, g.Ny1, on, ips, jps, ws, -gd, W);
+    block_fix_add4<PASS>(gtk2, g.Ny1, on, ipb, jpb, wb, gd, W, flag);
+    block_fix_add4<PASS>(gtk2, g.Ny1, on, ips, jps, ws, -gd, W, flag);
 }
 
 #pragma clang fp contract(off)
@@ -422,12 +473,12 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
 // gather^T of update_marker_temperature! (E:4782-4801): the marker cotangent
 // goes to the interpolated grid (tk2 at timestep 1, DT later); at timestep 1
 // the marker value was overwritten, so its cotangent becomes zero.
+template <int PASS>
 __global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __restrict__ xm,
                                                       const double* __restrict__ ym,
-                                                      double* __restrict__ gT,
-                                                      double* __restrict__ ggrid, int64_t M,
-                                                      int first) {
-    __shared__ AddWin W;
+                                                      double* __restrict__ gT, FixGrid ggrid,
+                                                      int* flag, int64_t M, int first) {
+    __shared__ FixWin W;
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < M;
     const int64_t m = on ? m0 : M - 1;
@@ -437,15 +488,18 @@ __global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __r
     double w[4];
     weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
     const double v = gT[m];
-    block_add4(ggrid, g.Ny1, on, ip, jp, w, v, W);
-    if (on && first) gT[m] = 0.0;
+    block_fix_add4<PASS>(ggrid, g.Ny1, on, ip, jp, w, v, W, flag);
+    if (PASS == 1 && on && first) gT[m] = 0.0;
 }
 
-int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid) {
+int launch_gather_T_adj(ereb_ctx* c, int pass, int64_t timestep, double* gT, const FixGrid& F,
+                        int* flag) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
-    hipLaunchKernelGGL(k_gather_T_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                       c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM), gT, ggrid, M,
+    auto kern = pass == 0 ? k_gather_T_adj<0> : k_gather_T_adj<1>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
+                       geomk(c), (const double*)mptr(c, EREB_M_XM),
+                       (const double*)mptr(c, EREB_M_YM), gT, F, flag, M,
                        timestep == 1 ? 1 : 0);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
@@ -453,15 +507,17 @@ int launch_gather_T_adj(ereb_ctx* c, int64_t timestep, double* gT, double* ggrid
 
 static RK4Args rk4_args(ereb_ctx* c, int mode);
 
-int launch_rk4_adj(ereb_ctx* c, double dt, int mode, const double* xb, const double* yb,
-                   const double* gout, double* gin, double* gphi, double* gtk2) {
+int launch_rk4_adj(ereb_ctx* c, int pass, double dt, int mode, const double* xb,
+                   const double* yb, const double* gout, double* gin, double* gphi,
+                   const FixGrid& gtk2, int* flag) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     int rc;
     if ((rc = set_dt(c, dt))) return rc;
     RK4Args A = rk4_args(c, mode);
-    hipLaunchKernelGGL(k_rk4_adj, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
-                       A, xb, yb, gout, gin, gphi, gtk2);
+    auto kern = pass == 0 ? k_rk4_adj<0> : k_rk4_adj<1>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A, xb,
+                       yb, gout, gin, gphi, gtk2, flag);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -522,9 +578,23 @@ static RK4Args rk4_args(ereb_ctx* c, int mode) {
     A.vyf = gptr(c, EREB_G_VYF);
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
+    const size_t ns = (size_t)c->Nx1 * c->Ny1;
+    A.DX = c->d_rkd;
+    A.DXf = c->d_rkd + ns;
+    A.DXq = c->d_rkd + 2 * ns;
+    A.DY = c->d_rkd + 3 * ns;
+    A.DYf = c->d_rkd + 4 * ns;
     A.xb_out = A.yb_out = nullptr;
     A.DSXX = A.DSXY = A.DTg = nullptr;
     return A;
+}
+
+// the second-difference grids of this step's velocities (k_rk4_prep)
+static void launch_rk4_prep(ereb_ctx* c, const RK4Args& A) {
+    const int n = c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_rk4_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->Nx, c->Ny, A.vx, A.vy, A.vxf, A.vyf, (double*)A.DX, (double*)A.DXf,
+                       (double*)A.DXq, (double*)A.DY, (double*)A.DYf);
 }
 
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {
@@ -538,6 +608,7 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
+    launch_rk4_prep(c, A);
     const dim3 grid((unsigned)((M + 255) / 256));
     if (c->axes_exact)
         hipLaunchKernelGGL((k_rk4<0, true>), grid, dim3(256), 0, c->stream, A);
@@ -562,6 +633,7 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
     ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
+    launch_rk4_prep(c, A);
     const dim3 grid((unsigned)((M + 255) / 256));
     // node coordinates computed (exact axes) or loaded, chosen per kernel
     if (c->precision == EREB_MIXED) {

@@ -40,9 +40,13 @@ __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, 
 }
 
 // sum of each block's SCAN_B inputs
+// run (optional): the scan runs only when run[0] != 0 (a device-side
+// condition, so that a captured graph can skip it)
 __global__ __launch_bounds__(SCAN_T) void k_scan_blocksum(const int32_t* __restrict__ in,
-                                                          int64_t n, int32_t* __restrict__ bs) {
+                                                          int64_t n, int32_t* __restrict__ bs,
+                                                          const int32_t* __restrict__ run) {
     __shared__ int32_t sh[SCAN_T / 64 + 1];
+    if (run && run[0] == 0) return;
     int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_PER;
     int32_t s = 0;
 #pragma unroll
@@ -60,8 +64,10 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_blocksum(const int32_t* __restr
 __global__ __launch_bounds__(SCAN_T) void k_scan_apply(int32_t* __restrict__ in, int64_t n,
                                                        const int32_t* __restrict__ bs,
                                                        int32_t* __restrict__ out,
-                                                       int32_t* __restrict__ out2, int zero_in) {
+                                                       int32_t* __restrict__ out2, int zero_in,
+                                                       const int32_t* __restrict__ run) {
     __shared__ int32_t sh[SCAN_T / 64 + 1];
+    if (run && run[0] == 0) return;
     int32_t off = 0, tot;
     for (int64_t b0 = 0; b0 < blockIdx.x; b0 += SCAN_T) {
         const int64_t b = b0 + threadIdx.x;
@@ -89,19 +95,25 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(int32_t* __restrict__ in,
     }
 }
 
-int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* out2,
-                   bool zero_in) {
-    int rc = ensure_scan(c, n);
-    if (rc) return rc;
-    int64_t nb = (n + SCAN_B - 1) / SCAN_B;
-    if (nb == 0) nb = 1;
+int64_t scan_blocks(int64_t n) { return std::max<int64_t>((n + SCAN_B - 1) / SCAN_B, 1); }
+
+int scan_exclusive_buf(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n,
+                       int32_t* blocksums, int32_t* out2, bool zero_in, const int32_t* run) {
+    const int64_t nb = scan_blocks(n);
     hipLaunchKernelGGL(k_scan_blocksum, dim3((unsigned)nb), dim3(SCAN_T), 0, c->stream, in, n,
-                       c->d_scan);
+                       blocksums, run);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_T), 0, c->stream,
-                       const_cast<int32_t*>(in), n, (const int32_t*)c->d_scan, out, out2,
-                       zero_in ? 1 : 0);
+                       const_cast<int32_t*>(in), n, (const int32_t*)blocksums, out, out2,
+                       zero_in ? 1 : 0, run);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
+}
+
+int scan_exclusive(ereb_ctx* c, const int32_t* in, int32_t* out, int64_t n, int32_t* out2,
+                   bool zero_in, const int32_t* run) {
+    int rc = ensure_scan(c, n);
+    if (rc) return rc;
+    return scan_exclusive_buf(c, in, out, n, c->d_scan, out2, zero_in, run);
 }
 
 template <typename T>

@@ -110,6 +110,18 @@ def test_stokes_darcy_values_only_and_pinned_fetch():
         np.testing.assert_array_equal(got[3], R)
         with pytest.raises(ValueError):
             ctx.csc(ereb.SYS_STOKES_DARCY, out=(None, None, pinned[3], pinned[3]))
+        # new Darcy coefficients (diagonal entries only): the pattern is
+        # unchanged and the assembly refills nzval alone (the values-only
+        # path); then a pattern change takes the full path again
+        g2 = [a.copy() for a in grids]
+        g2[10] = g2[10] * 1.75  # RX
+        ref2 = Oracle(g, mat).assemble_hydromechanical(g2, dt)
+        assert _same_pattern(ref2[:2], (cp, rv))
+        for inputs, ref in ((g2, ref2), (grids, (cp, rv, nz, R)), (g2, ref2)):
+            _assemble(ctx, ereb.SYS_STOKES_DARCY, inputs, Rg, dt)
+            got = ctx.csc(ereb.SYS_STOKES_DARCY)
+            for a, b, what in zip(got, ref, ("colptr", "rowval", "nzval", "R")):
+                np.testing.assert_array_equal(a, b, err_msg=what)
 
 
 def test_pattern_query_before_assembly_is_an_error():

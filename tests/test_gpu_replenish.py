@@ -108,16 +108,17 @@ def test_replenish_resident_workload_hole():
 NEW_MAX = 4096  # most empty anchors that the listed append handles
 
 
-@pytest.mark.parametrize("spare,hole_km", [(0, (15, 10)), (10_000, (15, 10)),
-                                           (10_000, (1.5, 1.0))])
-def test_replenish_after_advection_in_sort_order(spare, hole_km):
+@pytest.mark.parametrize("spare,hole_km,move", [(0, (15, 10), 0.6), (10_000, (15, 10), 0.6),
+                                                (10_000, (1.5, 1.0), 0.1)])
+def test_replenish_after_advection_in_sort_order(spare, hole_km, move):
     """The timestep's order: a scatter sorts the markers, the advection moves
-    them (here up to 0.6 cells), then the replenishment visits them in the
+    them (by up to `move` cells), then the replenishment visits them in the
     now stale sort order.  The result equals the oracle's on the moved
-    markers, bit for bit.  A 15 x 10 km hole leaves ~19k empty anchors, more
-    than the listed append takes (the flag scan over all anchors runs); a
-    1.5 x 1 km hole with spare capacity for 10,000 markers is the case of
-    every later timestep: at most 4,096 empty anchors, appended from the
+    markers, bit for bit.  A 15 x 10 km hole and moves of 0.6 cells leave
+    tens of thousands of empty anchors, more than the listed append takes
+    (the flag scan over all anchors runs); a 1.5 x 1 km hole and moves of
+    0.1 cells with spare capacity for 10,000 markers is the case of every
+    later timestep: about a thousand empty anchors, appended from the
     compacted list ahead of the host's read of the count."""
     g, mat = synth.workload_geometry("fp64", mode=1)
     mk = synth.make_markers(g, mat, seed=2)
@@ -130,8 +131,8 @@ def test_replenish_after_advection_in_sort_order(spare, hole_km):
     mk = {f: np.ascontiguousarray(v[~hole]) for f, v in mk.items()}
     M = mk["xm"].size
     moved = dict(mk)
-    moved["xm"] = mk["xm"] + 0.6 * g.dx * (2 * rng.random(M) - 1)
-    moved["ym"] = mk["ym"] + 0.6 * g.dy * (2 * rng.random(M) - 1)
+    moved["xm"] = mk["xm"] + move * g.dx * (2 * rng.random(M) - 1)
+    moved["ym"] = mk["ym"] + move * g.dy * (2 * rng.random(M) - 1)
     hrs, hrf = mat.radioactive_heating(2.25e6 * 365.25 * 24 * 3600)
     with ereb.Context(g, mat, ereb.FP64) as ctx:
         if spare:
