@@ -26,7 +26,7 @@ LIB = os.path.join(LIBDIR, "libereb_hip.so")
 SOURCES = [
     "ereb_ctx.hip", "ereb_scan.hip", "ereb_sort.hip", "ereb_scatter.hip",
     "ereb_nodes.hip", "ereb_markers.hip", "ereb_assembly.hip", "ereb_adjoint.hip",
-    "ereb_hydration.hip", "ereb_backtrace.hip", "ereb_replenish.hip",
+    "ereb_hydration.hip", "ereb_backtrace.hip", "ereb_replenish.hip", "ereb_noalloc.hip",
     "ereb_pullback.hip",
 ]
 

@@ -496,6 +496,13 @@ __global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ 
     nzval[q] = slotv[map[q]];
 }
 
+// 0-based int32 column pointers to 1-based Int64
+__global__ void k_colptr_1based(const int32_t* __restrict__ a, int64_t* __restrict__ b,
+                                int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (int64_t)a[i] + 1;
+}
+
 // ---- slot tables, discovered on the host from the equation code -----------
 
 struct RecordSink {
@@ -679,5 +686,103 @@ static int assemble(ereb_ctx* c, double dt) {
 
 int launch_assemble_stokes(ereb_ctx* c, double dt) { return assemble<0>(c, dt); }
 int launch_assemble_thermal(ereb_ctx* c, double dt) { return assemble<1>(c, dt); }
+
+// ---- thermal assembly into caller buffers, scratch supplied ------------------
+// Scratch layout (each piece 256-byte aligned): the step scalars (dt), the
+// flags, the slot tables, the presence bits, the slot values, the column
+// counts, the 0-based column pointers and the scan's block sums.
+namespace {
+struct ThermalScratch {
+    StepScalars* scal;
+    int32_t* flags;
+    SlotTables* tables;
+    uint16_t* pres;
+    double* slotv;
+    int32_t* colcnt;
+    int32_t* colptr;
+    int32_t* blocks;
+    size_t bytes;
+};
+constexpr int THERMAL_SLOTS = 5;  // slots per cell of the 5-point equation
+
+ThermalScratch thermal_carve(char* base, int64_t ncell) {
+    ThermalScratch t{};
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        char* q = base ? base + off : nullptr;
+        off += (b + 255) / 256 * 256;
+        return q;
+    };
+    t.scal = (StepScalars*)take(sizeof(StepScalars));
+    t.flags = (int32_t*)take(4 * sizeof(int32_t));
+    t.tables = (SlotTables*)take(sizeof(SlotTables));
+    t.pres = (uint16_t*)take(sizeof(uint16_t) * ncell);
+    t.slotv = (double*)take(sizeof(double) * THERMAL_SLOTS * ncell);
+    t.colcnt = (int32_t*)take(sizeof(int32_t) * ncell);
+    t.colptr = (int32_t*)take(sizeof(int32_t) * (ncell + 1));
+    t.blocks = (int32_t*)take(sizeof(int32_t) * scan_blocks(ncell));
+    t.bytes = off;
+    return t;
+}
+}  // namespace
+
+size_t thermal_scratch_bytes(int64_t ncell) { return thermal_carve(nullptr, ncell).bytes; }
+
+ThermalScratchHead thermal_scratch_head(char* scratch) {
+    const ThermalScratch t = thermal_carve(scratch, 0);  // these pieces do not depend on ncell
+    return ThermalScratchHead{t.scal, t.flags, (void*)t.tables};
+}
+
+int thermal_tables_bytes() { return (int)sizeof(SlotTables); }
+
+int thermal_tables_host(void* out) {
+    SlotTables T;
+    if (!discover_slots<1>(T) || T.nstot != THERMAL_SLOTS || T.ne != 1) return EREB_ERR_STATE;
+    for (int u = 0; u < T.ne; ++u)
+        if (T.nrev[u] > ASM_STAGE / 256) return EREB_ERR_STATE;
+    std::memcpy(out, &T, sizeof T);
+    return EREB_OK;
+}
+
+// assemble_thermal_lse! (E:4526-4611) with every buffer supplied: the slot
+// tables already in the scratch, dt in its StepScalars
+int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8], double* RT,
+                           int64_t* colptr, int64_t* rowval, double* nzval) {
+    const int64_t ncell = (int64_t)c->Nx1 * c->Ny1;
+    const ThermalScratch t = thermal_carve(scratch, ncell);
+    AsmArgs A{};
+    A.Nx = c->Nx;
+    A.Ny = c->Ny;
+    A.Nx1 = c->Nx1;
+    A.Ny1 = c->Ny1;
+    A.dx = c->dx;
+    A.dy = c->dy;
+    A.tk1 = in[0];
+    A.RHOCP = in[1];
+    A.KX = in[2];
+    A.KY = in[3];
+    A.HR = in[4];
+    A.HA = in[5];
+    A.HS = in[6];
+    A.DHP = in[7];
+    A.R = RT;
+    const dim3 grid((unsigned)((ncell + 255) / 256), 1);
+    hipLaunchKernelGGL(k_asm_rows<1>, grid, dim3(256), 0, c->stream, A,
+                       (const StepScalars*)t.scal, (const SlotTables*)t.tables, t.pres, t.slotv,
+                       t.flags);
+    hipLaunchKernelGGL(k_asm_count, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1,
+                       (const SlotTables*)t.tables, (const uint16_t*)t.pres, t.colcnt,
+                       (const int32_t*)nullptr);
+    int rc = scan_exclusive_buf(c, t.colcnt, t.colptr, ncell, t.blocks);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_asm_fill<16, int64_t, 1>), grid, dim3(256), 0, c->stream, c->Nx1,
+                       c->Ny1, (const SlotTables*)t.tables, (const uint16_t*)t.pres,
+                       (const double*)t.slotv, (const int32_t*)t.colptr, rowval, nzval,
+                       (const int32_t*)nullptr, (int32_t*)nullptr);
+    hipLaunchKernelGGL(k_colptr_1based, dim3((unsigned)((ncell + 1 + 255) / 256)), dim3(256), 0,
+                       c->stream, (const int32_t*)t.colptr, colptr, ncell + 1);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
 
 }  // namespace ereb

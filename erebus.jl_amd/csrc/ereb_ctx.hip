@@ -327,6 +327,9 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     } while (0)
     CK(hipSetDevice(device));
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // the no-alloc entry point's scratch on this device (once per device); a
+    // failure here is reported by that entry point, which tries again
+    if (noalloc_init(c)) c->err.clear();
     // axes
     std::vector<double> ax;
     ax.insert(ax.end(), g->x, g->x + Nx);

@@ -232,6 +232,21 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep);
 // ereb_assembly.hip
 int launch_assemble_stokes(ereb_ctx* c, double dt);
 int launch_assemble_thermal(ereb_ctx* c, double dt);
+// Thermal assembly into caller buffers with no allocation: grids in (device
+// pointers, (Ny1, Nx1)), RT and a 1-based Int64 CSC out; `scratch` holds at
+// least thermal_scratch_bytes(ncell) bytes, 256-byte aligned; dt is read
+// from the StepScalars at its start (written by the caller)
+size_t thermal_scratch_bytes(int64_t ncell);
+struct ThermalScratchHead {  // the pieces at the start of that scratch
+    StepScalars* scal;
+    int32_t* flags;  // 4 ints; [1] = a slot-table violation
+    void* tables;
+};
+ThermalScratchHead thermal_scratch_head(char* scratch);
+int thermal_tables_bytes();
+int thermal_tables_host(void* out);  // the slot tables, discovered on the host
+int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8], double* RT,
+                           int64_t* colptr, int64_t* rowval, double* nzval);
 
 // grid field of SUM slot s (order of reset_interpolated_properties!)
 int sum_grid_field(int s);
@@ -242,6 +257,10 @@ int launch_marker_porosity(ereb_ctx* c, const ereb_hydration& H, double dt);
 int launch_melt_composition(ereb_ctx* c);
 int launch_thermochemical(ereb_ctx* c, const ereb_hydration& H, double dt, int64_t timestep,
                           int64_t titer);
+
+// ereb_noalloc.hip: scratch address and thermal slot tables of the no-alloc
+// entry point on the context's device (once per device; ereb_ctx_create)
+int noalloc_init(ereb_ctx* c);
 
 // ereb_backtrace.hip
 int launch_backtrace_pressures(ereb_ctx* c, double dt);

@@ -114,6 +114,7 @@ PROTOTYPES = {
     "ereb_move_markers_rk4": [_P] + [_P] * 13 + [_I64, _D, _I64],
     "ereb_assemble_hydromechanical_lse": [_P] + [_P] * 20 + [_D, _P, _P],
     "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
+    "ereb_assemble_thermal_lse_noalloc": [_P] + [_P] * 8 + [_P, _D, _P, _P, _P, _I64, _P],
     "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
     "ereb_step_transfer": [_P, _P, _P, _I64, _D, _I64],
     "ereb_ctx_set_graphs": [_P, _I],

@@ -336,6 +336,24 @@ int ereb_assemble_thermal_lse(ereb_ctx* ctx, const double* tk1,
     const double* HA, const double* HS, const double* DHP, double* RT,
     double dt, int64_t* nnz);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight loops, harnesses that forbid
+ * hipMalloc): ereb_assemble_thermal_lse with every buffer on the device and
+ * supplied by the caller, allocating nothing.  The 8 input grids and RT are
+ * device arrays of Ny1*Nx1 Float64 (column-major); the matrix is written as
+ * a 1-based Int64 CSC into colptr (Ny1*Nx1 + 1), rowval and nzval (capacity
+ * entries, at least 5*Ny1*Nx1); *nnz (optional, synchronises the stream)
+ * receives the entry count.  The kernels' scratch is a static device array
+ * per GPU, held until the process ends: grids of more than 524,288 P cells
+ * (Ny1*Nx1) are refused with EREB_ERR_ARG.  Calls that overlap on one GPU
+ * (other contexts or host threads) share that array and race: serialise
+ * them.  Results equal ereb_assemble_thermal_lse's bit for bit. */
+int ereb_assemble_thermal_lse_noalloc(ereb_ctx* ctx, const double* tk1,
+    const double* RHOCP, const double* KX, const double* KY, const double* HR,
+    const double* HA, const double* HS, const double* DHP, double* RT,
+    double dt, int64_t* colptr, int64_t* rowval, double* nzval,
+    int64_t capacity, int64_t* nnz);
+
 /* ---- adjoint (no counterpart in the reference) -------------------------- */
 /* Gradient of L = sum over the P nodes of tk1^2 after `nsteps` timesteps of
  * the transfer map on the resident markers, with respect to the initial
