@@ -30,7 +30,8 @@ times the gradient of sum(tk1^2) after 20 steps (1,000,000 markers, 201x201
 nodes, checkpoints every 5 steps) against the forward map and checks it
 against central finite differences on 64 markers.  `next_rows` gives the
 mean device time and algorithmic bandwidth of the transfers outside the
-step.  `host_lu` times the host factorisation and solve of the assembled
+step.  `large` runs the 20,070,400-marker workloads (R = 40 and 65 km) in
+both forms and times one checkpointed adjoint step there.  `host_lu` times the host factorisation and solve of the assembled
 systems (scipy's SuperLU; the reference uses UMFPACK).  `cpu_baseline` times
 the C restatement of the reference CPU path (the oracle, -O3, OpenMP where
 the reference threads) on a bounded sample of the Float64 workload.
@@ -79,9 +80,19 @@ ALG_BYTES = {
               "rk4": (120, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
 }
 STEP_CLASSES = ["sort", "scatter", "combine", "nodes", "rk4", "asm_sd", "asm_t"]
-# kernel-name prefix of each class's main kernel in the rocprofv3 traffic summary
-TRAFFIC_KERNEL = {"scatter": "void ereb::k_scatter<", "rk4": "void ereb::k_rk4<",
-                  "sort": "void ereb::k_fine_count<"}
+# kernel-name prefix of each class's main kernel, per form, in the rocprofv3
+# traffic summary (profiles/hbm_traffic_<workload>.json, made by
+# erebus.jl_amd/ereb/rocprof_summary.py from FETCH_SIZE / WRITE_SIZE runs)
+TRAFFIC_KERNEL = {
+    "fp64": {"scatter": "void ereb::k_scatter<double>", "rk4": "void ereb::k_rk4<1, true>",
+             "sort": "void ereb::k_fine_count<ereb::PropArgs<double>",
+             "nodes": "ereb::k_node_props", "asm_sd": "void ereb::k_asm_rows<0>",
+             "asm_t": "void ereb::k_asm_rows<1>"},
+    "mixed": {"scatter": "void ereb::k_scatter<float>", "rk4": "void ereb::k_rk4<2, true>",
+              "sort": "void ereb::k_fine_count<ereb::PropArgs<float>",
+              "nodes": "ereb::k_node_props", "asm_sd": "void ereb::k_asm_rows<0>",
+              "asm_t": "void ereb::k_asm_rows<1>"},
+}
 
 
 def step_scalars(mat, k: int):
@@ -249,8 +260,8 @@ def _traffic(workload: str, form: str, kclass: str):
     """HBM bytes per launch of a class's main kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by what the counter
     reads for a plain copy of known size), when one exists."""
-    pre = TRAFFIC_KERNEL.get(kclass)
-    pmc = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}_{form}.json")
+    pre = TRAFFIC_KERNEL.get(form, {}).get(kclass)
+    pmc = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not pre or not os.path.exists(pmc):
         return None
     for k, v in json.load(open(pmc)).get("kernels", {}).items():
@@ -431,12 +442,67 @@ def full_timestep_leg(ctx, mat, hrs, hrf, reps: int = 5):
 BREAKS = (263.0, 268.0, 273.0, 278.0, 410.0)  # kinks and steps of T-dependent properties
 
 
+def large_leg(steps: int = 5, warmup: int = 2):
+    """The 20,070,400-marker workloads (561x561 basic nodes, 8x8 markers per
+    cell, planetesimal R = 40 and 65 km): the transfer step of both forms
+    (marker-steps/s, the dominant class's roofline and the step's HBM
+    fraction) and one checkpointed adjoint step of the Float64 form against
+    its forward step."""
+    out = {}
+    for name in ("large40", "large65"):
+        g, mat = synth.workload_geometry(name, mode=1)
+        mode = mat.marker_property_mode
+        mk = synth.make_markers(g, mat, seed=0)
+        M = mk["xm"].size
+        grids, hrs, hrf = make_inputs(g, mat, mk)
+        res = {"markers": M, "grid": [g.Ny, g.Nx], "planet_radius_km": g.rplanet / 1e3}
+        for prec, form in ((ereb.FP64, "fp64"), (ereb.MIXED, "mixed")):
+            ctx, v, ms, kern, recap = run_form(g, mat, mk, grids, hrs, hrf, prec, steps, warmup)
+            nnz = (ctx.csc_size(ereb.SYS_STOKES_DARCY)[1], ctx.csc_size(ereb.SYS_THERMAL)[1])
+            dom, roofs, step_roof = rooflines(g, M, kern, form, name, nnz, ms)
+            res[form] = {"value": v, "ms_per_step": ms,
+                         "roofline": dict(roofs[dom], kernel_class=dom),
+                         "step_roofline": step_roof, "kernel_ms": kern,
+                         "graph_recaptures_in_timed_steps": recap}
+            ctx.close()
+            log(f"[bench] {name} {form} {ms:.3f} ms/step")
+        with ereb.Context(g, mat, ereb.FP64) as ctx:
+            vx, vy = synth.stream_velocity(g)
+            ctx.set_markers(mk)
+            for n, a in (("vx", vx), ("vy", vy), ("vxf", vx), ("vyf", vy),
+                         ("wyx", np.zeros(g.basic_shape, order="F"))):
+                ctx.set_grid(n, a)
+            dt = ereb.DT_LONGEST
+            ctx.adjoint_tk1_loss(1, 1, dt, mode, hrs, hrf)  # warm
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.adjoint_tk1_loss(1, 1, dt, mode, hrs, hrf, grad=False)
+            t_fwd = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.adjoint_tk1_loss(1, 1, dt, mode, hrs, hrf)
+            t_adj = (time.perf_counter() - t0) / reps
+        res["adjoint_step"] = {"forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
+                               "adjoint_forward_ratio": t_adj / t_fwd}
+        out[name] = res
+    return out
+
+
+# relative central-difference steps of the adjoint checks (adjoint_leg)
+FD_STEP = {"tkm": 1e-6, "phim": 1e-4}
+
+
 def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str = "adj"):
     """Gradient of L = sum(tk1^2) after nsteps steps with respect to the
     initial marker temperature and porosity: time of the forward map, time
     of the checkpointed adjoint (forward with checkpoints + reverse sweep),
-    and central finite differences (h = 1e-6 relative) on nfd random planet
-    markers away from the property kinks."""
+    and central finite differences on nfd random planet markers 2 K or more
+    from the temperature breakpoints of the property laws.  Steps: 1e-6
+    relative for tkm; 1e-4 for phim, whose effect on tk1 at 1e-6 (~1e-8 K)
+    is a few hundred thousand ulps of tk1, a quotient noise of ~1e-5; at 1e-4
+    the noise is ~1e-7 and the truncation error (tk1 is a ratio of sums
+    linear in phim) ~1e-8."""
     g, mat = synth.workload_geometry(workload, mode=1)
     mode = mat.marker_property_mode
     mk = synth.make_markers(g, mat, seed=0)
@@ -461,13 +527,13 @@ def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str =
         t_adj = (time.perf_counter() - t0) / reps
         rng = np.random.default_rng(12345)
         ok = np.flatnonzero((mk["tm"] < 3) & np.all(
-            np.abs(mk["tkm"][:, None] - np.array(BREAKS)[None, :]) > 0.5, axis=1))
+            np.abs(mk["tkm"][:, None] - np.array(BREAKS)[None, :]) >= 2.0, axis=1))
         idx = rng.choice(ok, nfd, replace=False)
         fd_err = {}
         for name, grad in (("tkm", gT), ("phim", gP)):
             fd = np.empty(nfd)
             for q, m in enumerate(idx):
-                h = 1e-6 * abs(mk[name][m])
+                h = FD_STEP[name] * abs(mk[name][m])
                 tk = []
                 for sgn in (1.0, -1.0):
                     a = mk[name].copy()
@@ -490,7 +556,7 @@ def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str =
         "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3,
         "adjoint_forward_ratio": t_adj / t_fwd,
         "forward_marker_steps_per_s": M * nsteps / t_fwd,
-        "fd_check": {"markers": nfd, "h_rel": 1e-6, **fd_err},
+        "fd_check": {"markers": nfd, "h_rel": FD_STEP, **fd_err},
     }
 
 
@@ -531,6 +597,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-next-rows", action="store_true")
     ap.add_argument("--no-host-lu", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -620,6 +687,9 @@ def main():
         log("[bench] adjoint leg")
         res["adjoint"] = adjoint_leg()
         log(f"[bench] adjoint/forward {res['adjoint']['adjoint_forward_ratio']:.2f}")
+    if not a.no_large:
+        log("[bench] 20M-marker workloads (R = 40 and 65 km)")
+        res["large"] = large_leg()
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (oracle at -O3, OpenMP in the gathers)")
         res["cpu_baseline"] = cpu_baseline(g, mat, mk, grids, hrs, hrf, a.cpu_steps)

@@ -311,7 +311,8 @@ struct AdjBuffers {
     double *save = nullptr, *cps = nullptr, *st = nullptr, *snap = nullptr, *xb = nullptr;
     double *gT = nullptr, *gT2 = nullptr, *gP = nullptr, *gg = nullptr;
     // fixed-point sums of the tk2 cotangent of RK4^T (A) and of the gather^T
-    // (B): max bits [0..1], flag [2], then 2 words per P node for A and for B
+    // (B): bounds [0..1] (bits of the largest marker cotangent), flag [2],
+    // then 2 words per P node for A and for B
     unsigned long long* fix = nullptr;
 };
 
@@ -320,28 +321,43 @@ static FixGrid fix_grid(const AdjBuffers& B, int which, int64_t n) {
     return FixGrid{B.fix + 8 + (size_t)which * 2 * n, B.fix + which, 1.0};
 }
 
-// the scale from the first pass's largest contribution
-static int fix_scale(ereb_ctx* c, FixGrid& F) {
-    unsigned long long mb = 0;
-    EREB_HIP(c, hipMemcpyAsync(&mb, F.maxbits, sizeof mb, hipMemcpyDeviceToHost, c->stream));
-    EREB_HIP(c, hipStreamSynchronize(c->stream));
-    double v;
-    std::memcpy(&v, &mb, sizeof v);
-    if (!std::isfinite(v))
-        return fail(c, EREB_ERR_DOMAIN, "adjoint: a tk2 cotangent contribution is not finite");
-    F.up = fix_scale_of(v);
-    return EREB_OK;
-}
-
-// out = value(A) + value(B) per node (B optional)
-__global__ void k_fix_to_grid(const unsigned long long* __restrict__ wa, double upa,
-                              const unsigned long long* __restrict__ wb, double upb, int64_t n,
+// out = value(A) + value(B) per node (B optional), the scales from the bounds
+__global__ void k_fix_to_grid(const unsigned long long* __restrict__ wa,
+                              const unsigned long long* __restrict__ ba,
+                              const unsigned long long* __restrict__ wb,
+                              const unsigned long long* __restrict__ bb, int64_t n,
                               double* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    double v = fix_value(wa, k, upa);
-    if (wb) v += fix_value(wb, k, upb);
+    double v = fix_value(wa, k, fix_up_of_bound(ba));
+    if (wb) v += fix_value(wb, k, fix_up_of_bound(bb));
     out[k] = v;
+}
+
+// bits[0] = max(bits[0], max |a[i]|) over i < n, one atomic per wavefront:
+// the bound of a fixed-point pass (non-negative doubles order as their bits)
+__global__ __launch_bounds__(256) void k_absmax_bits(const double* __restrict__ a, int64_t n,
+                                                     unsigned long long* __restrict__ bits) {
+    double mx = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = fabs(a[i]);
+        mx = v > mx || v != v ? v : mx;  // a NaN wins, and is flagged by the pass
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(mx, o);
+        mx = y > mx || y != y ? y : mx;
+    }
+    if (__lane_id() == 0 && mx != 0.0)
+        atomicMax(bits, (unsigned long long)__double_as_longlong(mx));
+}
+
+static int absmax_bits(ereb_ctx* c, const double* a, int64_t n, unsigned long long* bits) {
+    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_absmax_bits, dim3(std::max(nb, 1u)), dim3(256), 0, c->stream, a, n,
+                       bits);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
 }
 
 // the buffers of one call, carved from the context's grow-only pool
@@ -492,6 +508,10 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
         if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
         if ((rc = launch_scatter_adj(c, hrsolidm, hrfluidm, md, B.gg, soff, B.gT, B.gP, true)))
             return rc;
+        // set when a tk2 cotangent contribution falls outside the range of
+        // its fixed-point scale (or is not finite); checked after the sweep
+        int* flag = (int*)(B.fix + 2);
+        if ((rc = fill_i32(c, flag, 0, 2))) return rc;
         for (int64_t sg = nseg - 1; sg >= 0; --sg) {
             const int64_t k0 = sg * every, k1 = std::min(nsteps, k0 + every);
             if (sg != nseg - 1) {  // the last segment was recorded by the forward pass
@@ -509,32 +529,26 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
                 if ((rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz))) return rc;
                 // RK4^T (tkm as RK4 saw it), then gather^T at timestep 1; their
-                // tk2 cotangents are summed in fixed point (first pass: the
-                // scale; second: the sums), so the gradient is the same on
-                // every run
-                if ((rc = fill_f64(c, (double*)B.fix, 0.0, 8 + 4 * n))) return rc;
+                // tk2 cotangents are summed in fixed point, each with a scale
+                // from a bound on its contributions (the largest marker
+                // cotangent, ereb_fixsum.h), so the gradient is the same on
+                // every run and the sweep needs no host round trip
+                if ((rc = fill_f64(c, (double*)B.fix + 8, 0.0, 4 * n))) return rc;
+                if ((rc = fill_f64(c, (double*)B.fix, 0.0, 2))) return rc;  // the bounds
                 FixGrid FA = fix_grid(B, 0, n), FB = fix_grid(B, 1, n);
-                int* flag = (int*)(B.fix + 2);
-                if ((rc = launch_rk4_adj(c, 0, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag)) ||
-                    (rc = fix_scale(c, FA)) ||
+                FA.bound = FA.maxbits;
+                FB.bound = FB.maxbits;
+                if ((rc = absmax_bits(c, B.gT, M, FA.maxbits)) ||
                     (rc = launch_rk4_adj(c, 1, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag)))
                     return rc;
-                if (k == 0 && ((rc = launch_gather_T_adj(c, 0, 1, B.gT2, FB, flag)) ||
-                               (rc = fix_scale(c, FB)) ||
+                if (k == 0 && ((rc = absmax_bits(c, B.gT2, M, FB.maxbits)) ||
                                (rc = launch_gather_T_adj(c, 1, 1, B.gT2, FB, flag))))
                     return rc;
                 hipLaunchKernelGGL(k_fix_to_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                                   c->stream, (const unsigned long long*)FA.w, FA.up,
-                                   k == 0 ? (const unsigned long long*)FB.w : nullptr, FB.up, n,
-                                   G(EREB_G_TK2));
+                                   c->stream, (const unsigned long long*)FA.w, FA.bound,
+                                   k == 0 ? (const unsigned long long*)FB.w : nullptr, FB.bound,
+                                   n, G(EREB_G_TK2));
                 EREB_HIP(c, hipGetLastError());
-                int fl = 0;
-                EREB_HIP(c, hipMemcpyAsync(&fl, flag, sizeof fl, hipMemcpyDeviceToHost, c->stream));
-                EREB_HIP(c, hipStreamSynchronize(c->stream));
-                if (fl)
-                    return fail(c, EREB_ERR_DOMAIN,
-                                "adjoint: a tk2 cotangent contribution is outside the fixed-point "
-                                "range of its pass");
                 // tk2 := tk1, then node properties^T
                 EREB_HIP(c, hipMemcpyAsync(G(EREB_G_TK1), G(EREB_G_TK2), sizeof(double) * n,
                                            hipMemcpyDeviceToDevice, c->stream));
@@ -549,6 +563,13 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 std::swap(B.gT, B.gT2);
             }
         }
+        int fl = 0;
+        EREB_HIP(c, hipMemcpyAsync(&fl, flag, sizeof fl, hipMemcpyDeviceToHost, c->stream));
+        EREB_HIP(c, hipStreamSynchronize(c->stream));
+        if (fl)
+            return fail(c, EREB_ERR_DOMAIN,
+                        "adjoint: a tk2 cotangent contribution is not finite or outside the "
+                        "fixed-point range of its scale");
         if (grad_tkm)
             EREB_HIP(c, hipMemcpyAsync(grad_tkm, B.gT, MB, hipMemcpyDeviceToHost, c->stream));
         if (grad_phim)

@@ -23,8 +23,8 @@
 //   3. fill  : the same walk writes rowval/nzval of each column contiguously,
 //              and for every stored entry the index of its slot value.
 // No atomics and no sort: rows come out ascending in every column.
-// Between 1 and 2 the presence bits are compared with the previous
-// assembly's (k_pattern_check, a device flag): when the pattern is unchanged
+// Step 1 also compares the presence bits with the previous assembly's (a
+// device flag): when the pattern is unchanged
 // -- the common case from one timestep or plastic iteration to the next --
 // steps 2 and 3 return at once and k_asm_values refills nzval with one
 // gather through the stored slot indices, colptr and rowval being those of
@@ -330,11 +330,16 @@ __host__ __device__ __forceinline__ void stokes_eq(const AsmArgs& A, int e, int 
 
 // 1. rows: thread per P cell, blockIdx.y = equation (wave-uniform)
 template <int SYS>
+// prev, chg (optional): the presence bits of the previous assembly; where
+// they differ, prev is updated and chg[0] set to 1 (every writer stores the
+// same value), so chg[0] tells whether the pattern changed
 __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A0, const StepScalars* __restrict__ S,
                                                   const SlotTables* __restrict__ T,
                                                   uint16_t* __restrict__ pres_out,
                                                   double* __restrict__ slotv,
-                                                  int32_t* __restrict__ flags) {
+                                                  int32_t* __restrict__ flags,
+                                                  uint16_t* __restrict__ prev,
+                                                  int32_t* __restrict__ chg) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= A0.Nx1 * A0.Ny1) return;
     AsmArgs A = A0;
@@ -349,27 +354,17 @@ __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A0, const StepScalars*
         stokes_eq(A, e, i, j, sink, true);
     else
         thermal_cell(A, i, j, sink, true);
-    pres_out[(size_t)e * ncell + k] = (uint16_t)pres;
+    const size_t q = (size_t)e * ncell + k;
+    pres_out[q] = (uint16_t)pres;
+    if (prev && prev[q] != (uint16_t)pres) {
+        prev[q] = (uint16_t)pres;
+        chg[0] = 1;
+    }
 }
 
 // 2. column counts: one thread per column walks the rows that can hold an
 // entry in it (the reverse slot table of its unknown), in ascending row
 // order, and counts the present ones; the slot tables are staged in LDS
-// chg[0] = 1 when the presence bits differ from the previous assembly's (the
-// host sets it to 1 when there is none); prev is then updated.  Every writer
-// stores the same value.
-__global__ __launch_bounds__(256) void k_pattern_check(const uint16_t* __restrict__ pres,
-                                                       uint16_t* __restrict__ prev, int64_t n,
-                                                       int32_t* __restrict__ chg) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint16_t a = pres[i];
-    if (a != prev[i]) {
-        chg[0] = 1;
-        prev[i] = a;
-    }
-}
-
 // run (optional): the kernel does its work only when run[0] != 0
 __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
                                                    const SlotTables* __restrict__ Tg,
@@ -484,16 +479,20 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
 }
 
 // nzval refilled through the stored slot indices when the pattern is
-// unchanged (chg[0] == 0): one coalesced write, one gather per entry
+// unchanged (chg[0] == 0): one coalesced write, one gather per entry; a
+// grid-stride loop over a grid of at most ASM_VALUES_BLOCKS workgroups, so
+// that the launch costs little when the pattern changed and it returns
+constexpr int ASM_VALUES_BLOCKS = 1024;
 __global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ chg,
                                                     const int32_t* __restrict__ colptr_end,
                                                     const int32_t* __restrict__ map,
                                                     const double* __restrict__ slotv,
                                                     double* __restrict__ nzval) {
     if (chg[0] != 0) return;
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= colptr_end[0]) return;
-    nzval[q] = slotv[map[q]];
+    const int64_t nnz = colptr_end[0];
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (int64_t)gridDim.x * blockDim.x)
+        nzval[q] = slotv[map[q]];
 }
 
 // 0-based int32 column pointers to 1-based Int64
@@ -593,6 +592,16 @@ static bool discover_slots(SlotTables& T) {
     return true;
 }
 
+int host_slot_check(int64_t* slots_sd, int64_t* slots_t) {
+    SlotTables A, B;
+    if (!discover_slots<0>(A) || !discover_slots<1>(B)) return EREB_ERR_STATE;
+    // six unknowns per Stokes-Darcy cell; the thermal equation's 5-point stencil
+    if (A.ne != 6 || B.ne != 1 || B.nstot != 5) return EREB_ERR_STATE;
+    if (slots_sd) *slots_sd = A.nstot;
+    if (slots_t) *slots_t = B.nstot;
+    return EREB_OK;
+}
+
 static AsmArgs asm_args(ereb_ctx* c, double dt, Sys& s) {
     AsmArgs A;
     A.Nx = c->Nx;
@@ -647,15 +656,14 @@ static int assemble(ereb_ctx* c, double dt) {
     const SlotTables* T = (const SlotTables*)s.tables;
     hipEvent_t ev;
     prof_begin(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, &ev);
+    // pattern unchanged since the previous assembly?  (with none, changed):
+    // the rows kernel compares the presence bits as it writes them
+    if ((rc = fill_i32(c, s.chg, s.have_prev ? 0 : 1, 1))) return rc;
     const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, c->d_scal, T, s.pres,
-                       s.slotv, c->d_flags);
-    // pattern unchanged since the previous assembly?  (with none, changed)
-    if ((rc = fill_i32(c, s.chg, s.have_prev ? 0 : 1, 1))) return rc;
-    const dim3 cgrid((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL(k_pattern_check, cgrid, dim3(256), 0, c->stream,
-                       (const uint16_t*)s.pres, s.pres_prev, n, s.chg);
+                       s.slotv, c->d_flags, s.pres_prev, s.chg);
     s.have_prev = true;
+    const dim3 cgrid((unsigned)((n + 255) / 256));
     const int32_t* chg = s.chg;
     hipLaunchKernelGGL(k_asm_count, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
                        s.colcnt, chg);
@@ -675,9 +683,10 @@ static int assemble(ereb_ctx* c, double dt) {
         hipLaunchKernelGGL(k_asm_fill<SLOT_R>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
                            s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval, chg,
                            s.map);
-    hipLaunchKernelGGL(k_asm_values, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0,
-                       c->stream, chg, (const int32_t*)s.colptr + n, (const int32_t*)s.map,
-                       (const double*)s.slotv, s.nzval);
+    hipLaunchKernelGGL(k_asm_values,
+                       dim3((unsigned)std::min<int64_t>((cap + 255) / 256, ASM_VALUES_BLOCKS)),
+                       dim3(256), 0, c->stream, chg, (const int32_t*)s.colptr + n,
+                       (const int32_t*)s.map, (const double*)s.slotv, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;
@@ -769,7 +778,7 @@ int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8]
     const dim3 grid((unsigned)((ncell + 255) / 256), 1);
     hipLaunchKernelGGL(k_asm_rows<1>, grid, dim3(256), 0, c->stream, A,
                        (const StepScalars*)t.scal, (const SlotTables*)t.tables, t.pres, t.slotv,
-                       t.flags);
+                       t.flags, (uint16_t*)nullptr, (int32_t*)nullptr);
     hipLaunchKernelGGL(k_asm_count, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1,
                        (const SlotTables*)t.tables, (const uint16_t*)t.pres, t.colcnt,
                        (const int32_t*)nullptr);

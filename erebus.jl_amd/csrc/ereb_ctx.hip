@@ -276,6 +276,10 @@ extern "C" {
 
 int ereb_version(void) { return 1; }
 
+int ereb_host_selfcheck(int64_t* slots_stokes_darcy, int64_t* slots_thermal) {
+    return host_slot_check(slots_stokes_darcy, slots_thermal);
+}
+
 const char* ereb_last_error(const ereb_ctx* c) {
     return c ? c->err.c_str() : "null context";
 }
@@ -291,9 +295,10 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         return EREB_ERR_ARG;
     const int64_t Nx = g->Nx, Ny = g->Ny, Nx1 = Nx + 1, Ny1 = Ny + 1;
     if (Nx > 1000000 || Ny > 1000000) return EREB_ERR_ARG;
-    // grid fields stay below 2^28 elements: kernels index them with 32-bit
-    // byte offsets
-    if (Nx1 * Ny1 > ((int64_t)1 << 28)) return EREB_ERR_ARG;
+    // grids stay within 2^24 nodes (4096 x 4096): kernels index grid fields,
+    // and the RK4 coefficient tables of 96 bytes per node, with 32-bit byte
+    // offsets
+    if (Nx1 * Ny1 > ((int64_t)1 << 24)) return EREB_ERR_ARG;
     // the staggered axes coincide element for element (C:40-64)
     for (int64_t k = 0; k < Nx; ++k)
         if (g->xvx[k] != g->x[k]) return EREB_ERR_ARG;
@@ -347,7 +352,7 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     CK(hipMalloc(&c->d_mat, sizeof(ereb_materials)));
     CK(hipMemcpy(c->d_mat, &c->mat, sizeof(ereb_materials), hipMemcpyHostToDevice));
     CK(hipMalloc(&c->d_scal, sizeof(StepScalars)));
-    CK(hipMalloc(&c->d_rkd, sizeof(double) * 5 * (size_t)(Nx1 * Ny1)));
+    CK(hipMalloc(&c->d_rkd, sizeof(double) * 4 * 12 * (size_t)(Nx1 * Ny1)));
     CK(hipMalloc(&c->d_axes, sizeof(double) * ax.size()));
     CK(hipMemcpy(c->d_axes, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
     // grids, 256-byte aligned, after 32 guard elements (the RK4 stencils read

@@ -71,18 +71,32 @@ __device__ __forceinline__ void weights4(double a, double b, double* w) {
 }
 
 // Address of grid element (i, j), 1-based, column-major.  Grid fields hold
-// fewer than 2^28 elements (checked when the context is created), so the
+// at most 2^24 elements (checked when the context is created), so the
 // element index is computed in 32-bit arithmetic.
 __device__ __forceinline__ const double* gptr_at(const double* __restrict__ base, int i, int j,
                                                  int rows) {
     return base + (uint32_t)((j - 1) * rows + (i - 1));
 }
 
+// Grid element at a byte offset below 2^31 (grid fields hold fewer than
+// 2^24 elements): the 32-bit offset lets a load take the grid's base address
+// from scalar registers and the offset from one vector register, with no
+// 64-bit address arithmetic per element.
+__device__ __forceinline__ double ldg_at(const double* __restrict__ base, uint32_t byte_off) {
+    return *(const double*)((const char*)base + byte_off);
+}
+
+// byte offset of grid element (i, j), 1-based, column-major
+__device__ __forceinline__ uint32_t goff(int i, int j, int rows) {
+    return (uint32_t)((j - 1) * rows + (i - 1)) << 3;
+}
+
 // dot4(grid_vector(i, j, grid), w) (E:1137-1159), 1-based (i, j)
 __device__ __forceinline__ double gdot4(const double* __restrict__ gr, int rows,
                                         int i, int j, const double* w) {
-    const double* p = gptr_at(gr, i, j, rows);
-    return p[0] * w[0] + p[1] * w[1] + p[rows] * w[2] + p[rows + 1] * w[3];
+    const uint32_t o = goff(i, j, rows), oR = o + ((uint32_t)rows << 3);
+    return ldg_at(gr, o) * w[0] + ldg_at(gr, o + 8) * w[1] + ldg_at(gr, oR) * w[2] +
+           ldg_at(gr, oR + 8) * w[3];
 }
 
 // ---- marker properties: compute_marker_properties! (E:700-750) ----------

@@ -26,11 +26,15 @@ constexpr double FIX_HI_MAX = 0x1p36;
 
 // One grid's accumulator: words w[2k] (hi, signed) and w[2k+1] (lo, >= 0)
 // of node k, and the scale up = 2^(S-40).  With w == nullptr the grid is not
-// wanted and contributions are dropped.
+// wanted and contributions are dropped.  The scale comes either from the
+// host (up, after a first pass recorded the largest |v| in maxbits), or, when
+// bound != nullptr, from a bound on |v| that a kernel wrote there before
+// (fix_up_of_bound, read by every kernel that adds or reads the grid).
 struct FixGrid {
     unsigned long long* w;
     unsigned long long* maxbits;  // first pass: largest |v| as bits
     double up;
+    const unsigned long long* bound = nullptr;  // |v| <= this (as bits), or null
 };
 
 // the two words of v at scale up; false when v is outside the scale's range
@@ -68,6 +72,25 @@ __device__ __forceinline__ void fix_add(const FixGrid& F, int64_t k, double v, i
 inline double fix_scale_of(double vmax) {
     return vmax > 0.0 ? std::ldexp(1.0, std::min(35 - std::ilogb(vmax), 1000)) : 1.0;
 }
+
+// Bounds for the single-pass form: a contribution is a marker cotangent
+// times factors of magnitude at most 1 times a bilinear weight; FIX_WEIGHT_MAX
+// allows weights up to 256 (a position up to 15 cells outside its clamped
+// cell).  The scale then rounds contributions to 2^-68 of the largest
+// marker cotangent instead of 2^-76 of the largest contribution.
+constexpr double FIX_WEIGHT_MAX = 256.0;
+
+__device__ __forceinline__ double fix_up_of_bound(const unsigned long long* bound) {
+    const double v = __longlong_as_double((long long)*bound) * FIX_WEIGHT_MAX;
+    if (!(v > 0.0) || !(v < 1e300)) return 1.0;  // zero, or not finite: flagged on use
+    return ldexp(1.0, min(35 - ilogb(v), 1000));
+}
+
+// the scale of F inside a kernel
+__device__ __forceinline__ double fix_up(const FixGrid& F) {
+    return F.bound ? fix_up_of_bound(F.bound) : F.up;
+}
+
 
 // value of node k: carry the lo words' whole units into hi, then one rounding
 __device__ __forceinline__ double fix_value(const unsigned long long* w, int64_t k, double up) {

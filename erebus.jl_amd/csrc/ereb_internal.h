@@ -86,7 +86,7 @@ struct ereb_ctx {
     void* d_m[EREB_M_NFIELDS] = {};
 
     double* d_grids = nullptr;
-    double* d_rkd = nullptr;  // 5 x Ny1*Nx1: second differences of the RK4 velocities
+    double* d_rkd = nullptr;  // 4 x 12 x Ny1*Nx1: RK4 stage-velocity coefficient tables
     size_t goff[EREB_G_NFIELDS] = {};
     int64_t gnumel[EREB_G_NFIELDS] = {};
     uint8_t* d_yny = nullptr;
@@ -121,7 +121,8 @@ struct ereb_ctx {
 
     double* adj_pool = nullptr;  // grow-only device pool of the adjoint driver
     size_t adj_pool_n = 0;       // doubles
-    void* pb_pool = nullptr;     // grow-only device pool of the pullbacks
+    void* pb_pool = nullptr;     // grow-only device pool of the pullbacks and the
+                                 // save_state copy (widened marker vectors)
     size_t pb_bytes = 0;
 
     // marker replenishment: grow-only scratch, and the last call's lattice
@@ -247,6 +248,10 @@ int thermal_tables_bytes();
 int thermal_tables_host(void* out);  // the slot tables, discovered on the host
 int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8], double* RT,
                            int64_t* colptr, int64_t* rowval, double* nzval);
+
+// stencil slot discovery of both systems on the host (no device work):
+// slots per cell of the Stokes-Darcy and thermal systems
+int host_slot_check(int64_t* slots_sd, int64_t* slots_t);
 
 // grid field of SUM slot s (order of reset_interpolated_properties!)
 int sum_grid_field(int s);

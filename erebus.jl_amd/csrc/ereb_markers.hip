@@ -21,8 +21,9 @@ namespace ereb {
 template <class R>
 __device__ __forceinline__ R dot4_r(const double* __restrict__ gr, int rows, int i, int j,
                                     const R* w) {
-    const double* p = gr + (size_t)(j - 1) * rows + (i - 1);
-    return (R)p[0] * w[0] + (R)p[1] * w[1] + (R)p[rows] * w[2] + (R)p[rows + 1] * w[3];
+    const uint32_t o = goff(i, j, rows), oR = o + ((uint32_t)rows << 3);
+    return (R)ldg_at(gr, o) * w[0] + (R)ldg_at(gr, o + 8) * w[1] + (R)ldg_at(gr, oR) * w[2] +
+           (R)ldg_at(gr, oR + 8) * w[3];
 }
 
 template <class R>
@@ -34,25 +35,43 @@ __device__ __forceinline__ void weights4_r(double a, double b, R* w) {
     w[3] = br * ar;
 }
 
-// update_marker_temperature! (E:4782-4801) for one marker: the new tkm
+// Bilinear stencil of a marker: byte offset of its cell's first node and
+// the four weights.  The RK4 kernel reuses the gathers' stencils for the
+// start-point interpolations of tk2 (P cell, clamps 1..Nx, 1..Ny) and wyx
+// (basic cell, clamps 1..Nx-1, 1..Ny-1), which use the same cells.
+template <class R>
+struct Stencil4 {
+    uint32_t off;
+    R w[4];
+};
+
+// update_marker_temperature! (E:4782-4801) for one marker: the new tkm; P
+// stencil (clamps 1..Nx, 1..Ny) in *st
 template <class R, bool EXACT>
 __device__ __forceinline__ double gather_T_one(const GeomK& g, double x, double y, double T,
-                                               const double* __restrict__ grid, bool first) {
+                                               const double* __restrict__ grid, bool first,
+                                               Stencil4<R>* st = nullptr) {
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     R w[4];
     weights4_r<R>((x - axis_node<EXACT>(g.xp, g.dx, jp)) / g.dx,
                   (y - axis_node<EXACT>(g.yp, g.dy, ip)) / g.dy, w);
     const double v = (double)dot4_r<R>(grid, g.Ny1, ip, jp, w);
+    if (st) {
+        st->off = goff(ip, jp, g.Ny1);
+        for (int c = 0; c < 4; ++c) st->w[c] = w[c];
+    }
     return first ? v : T + v;
 }
 
 // update_marker_stress! (E:4354-4388) for one marker: sxxm, sxym updated in
-// place; P grid with the clamps 2..Nx-1, 2..Ny-1 of E:4364-4367
+// place; P grid with the clamps 2..Nx-1, 2..Ny-1 of E:4364-4367; basic
+// stencil (clamps 1..Nx-1, 1..Ny-1) in *st
 template <class R, bool EXACT>
 __device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y, double& sxx,
                                              double& sxy, const double* __restrict__ DSXX,
-                                             const double* __restrict__ DSXY) {
+                                             const double* __restrict__ DSXY,
+                                             Stencil4<R>* st = nullptr) {
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 2, g.Nx - 1);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 2, g.Ny - 1);
     const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
@@ -64,6 +83,44 @@ __device__ __forceinline__ void gather_S_one(const GeomK& g, double x, double y,
                   (y - axis_node<EXACT>(g.y, g.dy, ib)) / g.dy, wb);
     sxx += (double)dot4_r<R>(DSXX, g.Ny1, ip, jp, wp);
     sxy += (double)dot4_r<R>(DSXY, g.Ny, ib, jb, wb);
+    if (st) {
+        st->off = goff(ib, jb, g.Ny);
+        for (int c = 0; c < 4; ++c) st->w[c] = wb[c];
+    }
+}
+
+// The RK4 kernel's start-point stencils, computed as the Float64 gathers
+// compute theirs (so every path of the kernel gets the same values): P cell
+// with the clamps 1..Nx, 1..Ny (the temperature gather's) and basic cell
+// with the clamps 1..Nx-1, 1..Ny-1 (the stress gather's DSXY stencil).
+template <bool EXACT>
+__device__ __forceinline__ void start_stencil_p(const GeomK& g, double x, double y,
+                                                Stencil4<double>& st) {
+    const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
+    const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
+    weights4_r<double>((x - axis_node<EXACT>(g.xp, g.dx, jp)) / g.dx,
+                       (y - axis_node<EXACT>(g.yp, g.dy, ip)) / g.dy, st.w);
+    st.off = goff(ip, jp, g.Ny1);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void start_stencil_b(const GeomK& g, double x, double y,
+                                                Stencil4<double>& st) {
+    const int jb = jl_cell((x - g.x[0]) * g.idx, 1, g.Nx - 1);
+    const int ib = jl_cell((y - g.y[0]) * g.idy, 1, g.Ny - 1);
+    weights4_r<double>((x - axis_node<EXACT>(g.x, g.dx, jb)) / g.dx,
+                       (y - axis_node<EXACT>(g.y, g.dy, ib)) / g.dy, st.w);
+    st.off = goff(ib, jb, g.Ny);
+}
+
+// dot4 of a grid with a stencil (E:1137-1159), products and sums rounded one
+// by one (defined outside the fused-FP region: every kernel that calls it
+// gets the same bits)
+__device__ __forceinline__ double stencil_dot(const double* __restrict__ gr, int rows,
+                                              const Stencil4<double>& st) {
+    const uint32_t oR = st.off + ((uint32_t)rows << 3);
+    return ldg_at(gr, st.off) * st.w[0] + ldg_at(gr, st.off + 8) * st.w[1] +
+           ldg_at(gr, oR) * st.w[2] + ldg_at(gr, oR + 8) * st.w[3];
 }
 
 template <class R>
@@ -92,33 +149,92 @@ __global__ __launch_bounds__(256) void k_gather_S(GeomK g, const double* __restr
     sxym[m] = sxy;
 }
 
-// Second differences of the stage velocities for the RK4 corrections
-// (E:5068-5088, E:5104-5124), one thread per node of the (Ny1, Nx1) grids,
-// in the reference's association (q0 - 2 c) + q2:
-//   DX[i, j]  = vx[i, j-1] - 2 vx[i, j] + vx[i, j+1]     for 2 <= j <= Nx-1
-//   DXf       = the same of vxf
-//   DXq[i, j] = vxf[i, j-1] - 2 vx[i, j] + vxf[i, j+1]   (E:5173)
-//   DY[i, j]  = vy[i-1, j] - 2 vy[i, j] + vy[i+1, j]     for 2 <= i <= Ny-1
-//   DYf       = the same of vyf
-// and zero elsewhere: a correction centred there is the one the reference
-// skips (j == 1 left, j == Nx-1 right; i == 1 up, i == Ny-1 down).
-__global__ __launch_bounds__(256) void k_rk4_prep(int Nx, int Ny, const double* __restrict__ vx,
-                                                  const double* __restrict__ vy,
-                                                  const double* __restrict__ vxf,
-                                                  const double* __restrict__ vyf,
-                                                  double* __restrict__ DX, double* __restrict__ DXf,
-                                                  double* __restrict__ DXq, double* __restrict__ DY,
-                                                  double* __restrict__ DYf) {
-    const int R = Ny + 1;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= R * (Nx + 1)) return;
+// RK4 stage velocities as quadratics, prepared once per step (k_rk4_coef).
+// On the Vx grid (E:5057-5091) the stage velocity in cell (i, j) is
+//   v = v_i(fx) + fy (v_{i+1}(fx) - v_i(fx)),
+//   v_r(fx) = a_r + fx (b_r - a_r) + 0.5 (fx - 0.5)^2 d_r,
+// a_r = v[r, j], b_r = v[r, j+1], and d_r the second difference
+// v[r, c-1] - 2 v[r, c] + v[r, c+1] (the reference's association) at the
+// correction's centre column c = j for fx < 0.5 ("left" half), c = j + 1
+// otherwise, zero where the reference skips the correction (c == 1 left,
+// c == Nx right, i.e. outside 2 <= c <= Nx-1).  In the fluid loop the
+// centre of the first row's left-half correction is read from vx (E:5173).
+// Expanding 0.5 (fx - 0.5)^2 = 0.125 - 0.5 fx + 0.5 fx^2, each half of each
+// cell is v = P(fx) + fy Q(fx) with
+//   P = (a_i + d_i/8) + fx ((b_i - a_i) - d_i/2) + fx^2 (d_i/2),
+//   Q = the same of row i+1 minus P,
+// six coefficients [P0 P1 P2 Q0 Q1 Q2]; the Vy grid (E:5093-5127) is the
+// transpose (quadratic in fy, linear in fx).  Table layout: per cell of the
+// (Ny1, Nx1) node numbering, 12 doubles (left half, then right half), so a
+// stage reads its half with three 16-byte loads; four tables (Vx and Vy of
+// the solid and of the fluid loop).
+constexpr int RKC = 12;  // doubles per cell and table
+
+struct RKCoefArgs {
+    int Nx, Ny;
+    const double *vx, *vy, *vxf, *vyf;
+    double *cxs, *cys, *cxf, *cyf;  // tables: Vx solid, Vy solid, Vx fluid, Vy fluid
+};
+
+// second difference of g along a line at index k with stride s, if the
+// centre's position c lies in [2, n-1] (else the correction is skipped)
+__device__ __forceinline__ double d2(const double* __restrict__ q, const double* __restrict__ ctr,
+                                     int k, int s, int c, int n) {
+    return (c >= 2 && c <= n - 1) ? (q[k - s] - 2.0 * ctr[k]) + q[k + s] : 0.0;
+}
+
+// coefficients of one half: rows (a0, b0, d0) and (a1, b1, d1) along the
+// quadratic direction
+__device__ __forceinline__ void rk_half(double* __restrict__ out, double a0, double b0,
+                                        double d0, double a1, double b1, double d1) {
+    const double p0 = a0 + 0.125 * d0, p1 = (b0 - a0) - 0.5 * d0, p2 = 0.5 * d0;
+    const double r0 = a1 + 0.125 * d1, r1 = (b1 - a1) - 0.5 * d1, r2 = 0.5 * d1;
+    out[0] = p0;
+    out[1] = p1;
+    out[2] = p2;
+    out[3] = r0 - p0;
+    out[4] = r1 - p1;
+    out[5] = r2 - p2;
+}
+
+// One thread per 16-byte piece of the tables: piece q (0..5) of cell k of
+// table t (gridDim.y = 4) holds coefficients 2(q mod 3), 2(q mod 3) + 1 of
+// half q / 3, so consecutive threads write consecutive 16 bytes.  Each
+// thread evaluates its half's six coefficients (the inputs are in L1/L2).
+__global__ __launch_bounds__(256) void k_rk4_coef(RKCoefArgs A) {
+    const int R = A.Ny + 1;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)6 * R * (A.Nx + 1)) return;
+    const int k = (int)(p / 6), q = (int)(p % 6);
     const int i = k % R + 1, j = k / R + 1;
-    const bool cx = j >= 2 && j <= Nx - 1, cy = i >= 2 && i <= Ny - 1;
-    DX[k] = cx ? (vx[k - R] - 2.0 * vx[k]) + vx[k + R] : 0.0;
-    DXf[k] = cx ? (vxf[k - R] - 2.0 * vxf[k]) + vxf[k + R] : 0.0;
-    DXq[k] = cx ? (vxf[k - R] - 2.0 * vx[k]) + vxf[k + R] : 0.0;
-    DY[k] = cy ? (vy[k - 1] - 2.0 * vy[k]) + vy[k + 1] : 0.0;
-    DYf[k] = cy ? (vyf[k - 1] - 2.0 * vyf[k]) + vyf[k + 1] : 0.0;
+    const int t = blockIdx.y;  // 0 Vx solid, 1 Vy solid, 2 Vx fluid, 3 Vy fluid
+    const bool xg = (t & 1) == 0, fluid = t >= 2;
+    const bool second = q >= 3;
+    double c[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (xg && i <= A.Ny && j <= A.Nx - 1) {
+        // Vx cell (i, j): rows i, i+1 of columns j, j+1; centre column j
+        // (left half) or j+1 (right half)
+        const double* v = fluid ? A.vxf : A.vx;
+        const int k1 = k + 1, kR = k + R, kR1 = k + R + 1;
+        const int c0 = second ? kR : k, c1 = second ? kR1 : k1, cc = second ? j + 1 : j;
+        // the fluid loop's first-row left-half centre is read from vx (E:5173)
+        const double* ctr0 = (fluid && !second) ? A.vx : v;
+        rk_half(c, v[k], v[kR], d2(v, ctr0, c0, R, cc, A.Nx), v[k1], v[kR1],
+                d2(v, v, c1, R, cc, A.Nx));
+    } else if (!xg && i <= A.Ny - 1 && j <= A.Nx) {
+        // Vy cell (i, j): rows i, i+1 of columns j, j+1; centre row i (upper
+        // half) or i+1 (lower half)
+        const double* v = fluid ? A.vyf : A.vy;
+        const int k1 = k + 1, kR = k + R, kR1 = k + R + 1;
+        const int c0 = second ? k1 : k, c1 = second ? kR1 : kR, cc = second ? i + 1 : i;
+        rk_half(c, v[k], v[k1], d2(v, v, c0, 1, cc, A.Ny), v[kR], v[kR1],
+                d2(v, v, c1, 1, cc, A.Ny));
+    }
+    const int r = q - (second ? 3 : 0);
+    const double lo = r == 0 ? c[0] : (r == 1 ? c[2] : c[4]);
+    const double hi = r == 0 ? c[1] : (r == 1 ? c[3] : c[5]);
+    double* out = t == 0 ? A.cxs : (t == 1 ? A.cys : (t == 2 ? A.cxf : A.cyf));
+    *(double2*)(out + (size_t)k * RKC + 2 * q) = make_double2(lo, hi);
 }
 
 // RK4 region: a*b + c may be fused (the quotients above already make the
@@ -143,58 +259,41 @@ __device__ __forceinline__ double stage_frac(const double* __restrict__ ax, doub
     else return (x - ax[k - 1]) * ih;
 }
 
-// RK4 stage velocity on the Vx grid (E:5057-5091):
-//   v13 = v[i,j] (1 - fx) + v[i,j+1] fx + h (v[i,j-1] - 2 v[i,j] + v[i,j+1])
-// (the correction on columns j..j+2 instead when fx >= 0.5; only where
-// those columns exist), the same for row i+1, then v13 (1 - fy) + v24 fy.
-// The second differences are read from grids prepared once per step
-// (k_rk4_prep): D at the correction's centre column, zero where the
-// reference skips the correction, and Dq for the first row of the left-half
-// correction, whose centre the reference reads from vx also in the fluid
-// loop (E:5173); Dq == D in the solid loop.  Six loads, no selects of
-// values.
+// RK4 stage velocity on the Vx grid (E:5057-5091) from the cell's quadratic
+// coefficients (k_rk4_coef): cell (i, j) with the clamps 1..Ny, 1..Nx-1,
+// fractions fx, fy, the half chosen by fx >= 0.5, then
+// P(fx) + fy Q(fx).  The Vy grid (E:5093-5127) is the transpose.
+__device__ __forceinline__ double rk_poly(const double* __restrict__ T, uint32_t node,
+                                          bool second, double u, double w) {
+    const uint32_t off = node * (uint32_t)(RKC * 8) + (second ? 48u : 0u);  // < 2^31 bytes
+    const char* p = (const char*)T + off;
+    const double2 c01 = *(const double2*)p, c23 = *(const double2*)(p + 16),
+                  c45 = *(const double2*)(p + 32);
+    const double P = c01.x + u * (c01.y + u * c23.x);
+    const double Q = c23.y + u * (c45.x + u * c45.y);
+    return P + w * Q;
+}
+
 template <bool EXACT>
-__device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ v,
-                                        const double* __restrict__ D,
-                                        const double* __restrict__ Dq, double x, double y) {
-    const int R = g.Ny1;
+__device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ T, double x,
+                                        double y) {
     const double tx = (x - g.x[0]) * g.idx, ty = (y - g.yp[0]) * g.idy;
     const int j = stage_cell(tx, 1, g.Nx - 1);
     const int i = stage_cell(ty, 1, g.Ny);
     const double fx = stage_frac<EXACT>(g.x, x, tx, g.idx, j);
     const double fy = stage_frac<EXACT>(g.yp, y, ty, g.idy, i);
-    const uint32_t k = (uint32_t)((j - 1) * R + (i - 1));
-    const bool right = fx >= 0.5;
-    const uint32_t kc = right ? k + R : k;
-    const double a0 = v[k], a1 = v[k + 1], b0 = v[k + R], b1 = v[k + R + 1];
-    const double d0 = (right ? D : Dq)[kc], d1 = D[kc + 1];
-    const double t = fx - 0.5, h = 0.5 * (t * t);
-    const double v13 = a0 + fx * (b0 - a0) + h * d0;
-    const double v24 = a1 + fx * (b1 - a1) + h * d1;
-    return v13 + fy * (v24 - v13);
+    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fx >= 0.5, fx, fy);
 }
 
-// RK4 stage velocity on the Vy grid (E:5093-5127), the transpose: second
-// differences along y at the correction's centre row (D, zero where the
-// reference skips it).
 template <bool EXACT>
-__device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ v,
-                                        const double* __restrict__ D, double x, double y) {
-    const int R = g.Ny1;
+__device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ T, double x,
+                                        double y) {
     const double tx = (x - g.xp[0]) * g.idx, ty = (y - g.y[0]) * g.idy;
     const int j = stage_cell(tx, 1, g.Nx);
     const int i = stage_cell(ty, 1, g.Ny - 1);
     const double fx = stage_frac<EXACT>(g.xp, x, tx, g.idx, j);
     const double fy = stage_frac<EXACT>(g.y, y, ty, g.idy, i);
-    const uint32_t k = (uint32_t)((j - 1) * R + (i - 1));
-    const bool down = fy >= 0.5;
-    const uint32_t kc = down ? k + 1 : k;
-    const double a0 = v[k], a1 = v[k + 1], b0 = v[k + R], b1 = v[k + R + 1];
-    const double d0 = D[kc], d1 = D[kc + R];
-    const double t = fy - 0.5, h = 0.5 * (t * t);
-    const double v12 = a0 + fy * (a1 - a0) + h * d0;
-    const double v34 = b0 + fy * (b1 - b0) + h * d1;
-    return v12 + fx * (v34 - v12);
+    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fy >= 0.5, fy, fx);
 }
 
 struct RK4Args {
@@ -207,9 +306,9 @@ struct RK4Args {
     const int32_t* tm;
     const double* phim;
     const double *vx, *vy, *vxf, *vyf, *wyx, *tk2;
-    // second differences of the stage velocities (k_rk4_prep): vx, vxf, the
-    // fluid loop's left-half first row (centre from vx), vy, vyf
-    const double *DX, *DXf, *DXq, *DY, *DYf;
+    // quadratic coefficients of the stage velocities (k_rk4_coef): Vx and Vy
+    // of the solid step, Vx and Vy of the fluid back-trace
+    const double *CXs, *CYs, *CXf, *CYf;
     double *xb_out, *yb_out;  // optional: fluid back-trace end points (adjoint recording)
     // fused gathers (k_rk4<1 or 2>): DSXX, DSXY and the DT grid (the
     // temperature gather reads tk2 at timestep 1, else DT)
@@ -229,8 +328,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     double vxr[4], vyr[4];
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vx, A.DX, A.DX, xr, yr);
-        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vy, A.DY, xr, yr);
+        vxr[rk] = rk_vx<EXACT>(g, A.CXs, xr, yr);
+        vyr[rk] = rk_vy<EXACT>(g, A.CYs, xr, yr);
         if (rk < 3) {
             xr = xmm + dt * cr[rk] * vxr[rk];
             yr = ymm + dt * cr[rk] * vyr[rk];
@@ -242,8 +341,8 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     ymm = yr = yn;
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = 0.0 + rk_vx<EXACT>(g, A.vxf, A.DXf, A.DXq, xr, yr);
-        vyr[rk] = 0.0 + rk_vy<EXACT>(g, A.vyf, A.DYf, xr, yr);
+        vxr[rk] = rk_vx<EXACT>(g, A.CXf, xr, yr);
+        vyr[rk] = rk_vy<EXACT>(g, A.CYf, xr, yr);
         if (rk < 3) {
             xr = xmm - dt * cr[rk] * vxr[rk];
             yr = ymm - dt * cr[rk] * vyr[rk];
@@ -264,8 +363,9 @@ __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, in
              (y - axis_node<EXACT>(g.yp, g.dy, ip)) * g.idy, w);
 }
 
-// GATHER = 0: RK4 alone.  GATHER = 1 (Float64) or 2 (FP32 interpolation, the
-// mixed form): the stress gather, then the temperature gather, then RK4 on
+// GATHER = 0: RK4 alone.  GATHER = 1 (Float64) or 2 (the mixed form: the
+// stress gather's interpolation in FP32; the temperature gather is Float64 in
+// both forms): the stress gather, then the temperature gather, then RK4 on
 // the gathered values, as the three separate kernels would compute them
 // (the gathers' arithmetic is defined outside the fused-FP region), with
 // each marker's state read and written once.
@@ -280,26 +380,27 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const double xmm = A.xm[m], ymm = A.ym[m];
     double s = A.sxxm[m], t = A.sxym[m];
     double T = first ? 0.0 : A.tkm[m];
-    if (GATHER == 1) {
-        gather_S_one<double, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<double, EXACT>(g, xmm, ymm, T, tgrid, first);
-    } else if (GATHER == 2) {
-        gather_S_one<float, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
-        T = gather_T_one<float, EXACT>(g, xmm, ymm, T, tgrid, first);
-    }
-    double xr = xmm, yr = ymm;
+    double xr, yr;
     double w[4];
-    // solid temperature at the start position (P grid) and rotation rate
-    // (basic grid), E:5013-5040
     int jp, ip;
-    p_stencil<EXACT>(g, xr, yr, ip, jp, w);
-    const double tks0 = gdot4(A.tk2, g.Ny1, ip, jp, w);
-    const double xb0 = g.x[0], yb0 = g.y[0];
-    const int jb = jl_cell((xr - xb0) * g.idx, 1, g.Nx - 1);
-    const int ib = jl_cell((yr - yb0) * g.idy, 1, g.Ny - 1);
-    weights4((xr - axis_node<EXACT>(g.x, g.dx, jb)) * g.idx,
-             (yr - axis_node<EXACT>(g.y, g.dy, ib)) * g.idy, w);
-    const double om = gdot4(A.wyx, g.Ny, ib, jb, w);
+    // solid temperature at the start position (P grid) and rotation rate
+    // (basic grid), E:5013-5040, on the stencils of the Float64 gathers (the
+    // temperature gather's P stencil, the stress gather's basic one),
+    // computed the same way when there is no gather, so that the fused and
+    // the separate kernels give the same values
+    Stencil4<double> sp, sb;
+    if (GATHER)
+        T = gather_T_one<double, EXACT>(g, xmm, ymm, T, tgrid, first, &sp);
+    else
+        start_stencil_p<EXACT>(g, xmm, ymm, sp);
+    if (GATHER == 1) {
+        gather_S_one<double, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY, &sb);
+    } else {
+        if (GATHER == 2) gather_S_one<float, EXACT>(g, xmm, ymm, s, t, A.DSXX, A.DSXY);
+        start_stencil_b<EXACT>(g, xmm, ymm, sb);
+    }
+    const double tks0 = stencil_dot(A.tk2, g.Ny1, sp);
+    const double om = stencil_dot(A.wyx, g.Ny, sb);
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
     // sin(2θ) and cos(2θ) from the double-angle identities (within a few ulp
@@ -436,6 +537,7 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
                                                  double* __restrict__ gphi, FixGrid gtk2,
                                                  int* flag) {
     __shared__ FixWin W;
+    gtk2.up = fix_up(gtk2);
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < A.n;
     const int64_t m = on ? m0 : A.n - 1;  // idle lanes read a valid marker, add nothing
@@ -479,6 +581,7 @@ __global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __r
                                                       double* __restrict__ gT, FixGrid ggrid,
                                                       int* flag, int64_t M, int first) {
     __shared__ FixWin W;
+    ggrid.up = fix_up(ggrid);
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < M;
     const int64_t m = on ? m0 : M - 1;
@@ -528,14 +631,13 @@ int launch_gather_temperature(ereb_ctx* c, int64_t timestep) {
     hipEvent_t ev;
     prof_begin(c, EREB_K_GATHER_T, &ev);
     const double* grid = gptr(c, timestep == 1 ? EREB_G_TK2 : EREB_G_DT);
-    if (c->precision == EREB_MIXED)
-        hipLaunchKernelGGL(k_gather_T<float>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                           c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM),
-                           mptr(c, EREB_M_TKM), grid, M, timestep == 1 ? 1 : 0);
-    else
-        hipLaunchKernelGGL(k_gather_T<double>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
-                           c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM),
-                           mptr(c, EREB_M_TKM), grid, M, timestep == 1 ? 1 : 0);
+    // Float64 in both forms: the property laws switch branch at fixed
+    // temperatures (E:2272-2358), and an FP32 interpolation error (~3e-5 K at
+    // 300 K) would move markers near a breakpoint to the other branch, a
+    // jump of up to 16x in the fluid heat capacity
+    hipLaunchKernelGGL(k_gather_T<double>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                       c->stream, geomk(c), mptr(c, EREB_M_XM), mptr(c, EREB_M_YM),
+                       mptr(c, EREB_M_TKM), grid, M, timestep == 1 ? 1 : 0);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_GATHER_T, ev);
     return EREB_OK;
@@ -578,23 +680,23 @@ static RK4Args rk4_args(ereb_ctx* c, int mode) {
     A.vyf = gptr(c, EREB_G_VYF);
     A.wyx = gptr(c, EREB_G_WYX);
     A.tk2 = gptr(c, EREB_G_TK2);
-    const size_t ns = (size_t)c->Nx1 * c->Ny1;
-    A.DX = c->d_rkd;
-    A.DXf = c->d_rkd + ns;
-    A.DXq = c->d_rkd + 2 * ns;
-    A.DY = c->d_rkd + 3 * ns;
-    A.DYf = c->d_rkd + 4 * ns;
+    const size_t ns = (size_t)c->Nx1 * c->Ny1 * RKC;
+    A.CXs = c->d_rkd;
+    A.CYs = c->d_rkd + ns;
+    A.CXf = c->d_rkd + 2 * ns;
+    A.CYf = c->d_rkd + 3 * ns;
     A.xb_out = A.yb_out = nullptr;
     A.DSXX = A.DSXY = A.DTg = nullptr;
     return A;
 }
 
-// the second-difference grids of this step's velocities (k_rk4_prep)
+// the stage-velocity coefficient tables of this step's velocities (k_rk4_coef)
 static void launch_rk4_prep(ereb_ctx* c, const RK4Args& A) {
-    const int n = c->Nx1 * c->Ny1;
-    hipLaunchKernelGGL(k_rk4_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       c->Nx, c->Ny, A.vx, A.vy, A.vxf, A.vyf, (double*)A.DX, (double*)A.DXf,
-                       (double*)A.DXq, (double*)A.DY, (double*)A.DYf);
+    const int64_t n = (int64_t)6 * c->Nx1 * c->Ny1;
+    RKCoefArgs P{c->Nx, c->Ny, A.vx, A.vy, A.vxf, A.vyf, (double*)A.CXs, (double*)A.CYs,
+                 (double*)A.CXf, (double*)A.CYf};
+    hipLaunchKernelGGL(k_rk4_coef, dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, c->stream,
+                       P);
 }
 
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {

@@ -75,6 +75,7 @@ _I = ctypes.c_int
 # exported C entry points: name -> argtypes (restype int unless noted)
 PROTOTYPES = {
     "ereb_version": [],
+    "ereb_host_selfcheck": [_P, _P],
     "ereb_last_error": [_P],
     "ereb_ctx_create": [ctypes.POINTER(CGeometry), ctypes.POINTER(CMaterials), _I, _I,
                         ctypes.POINTER(_P)],
@@ -86,6 +87,9 @@ PROTOTYPES = {
     "ereb_markers_set": [_P, _I, _P, _I64],
     "ereb_markers_get": [_P, _I, _P, _I64],
     "ereb_grid_numel": [_P, _I],
+    "ereb_save_field": [_I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_I),
+                        ctypes.POINTER(_I)],
+    "ereb_save_state_fields": [_P, _P, _I64],
     "ereb_grid_set": [_P, _I, _P],
     "ereb_grid_get": [_P, _I, _P],
     "ereb_yny_set": [_P, _P],
@@ -173,6 +177,22 @@ def load(path: str | None = None) -> ctypes.CDLL:
     if path is None:
         _lib = lib
     return lib
+
+
+SAVE_GRID, SAVE_YNY, SAVE_MARKER = 0, 1, 2
+
+
+def save_fields(lib: ctypes.CDLL | None = None):
+    """The library's save_state field set: [(key, kind, field enum)] in the
+    order of the record's keyword list (E:5864-5957)."""
+    lib = lib or load()
+    name, kind, field = ctypes.c_char_p(), _I(), _I()
+    out, k = [], 0
+    while lib.ereb_save_field(k, ctypes.byref(name), ctypes.byref(kind),
+                              ctypes.byref(field)) == OK:
+        out.append((name.value.decode(), kind.value, field.value))
+        k += 1
+    return out
 
 
 def _ptr(a: np.ndarray | None):
@@ -293,6 +313,32 @@ class Context:
     def get_yny(self) -> np.ndarray:
         out = np.empty((self.geom.Ny, self.geom.Nx), dtype=np.uint8, order="F")
         self.check(self.lib.ereb_yny_get(self.h, _ptr(out)))
+        return out
+
+    def save_state_fields(self, names=None, marknum: int | None = None) -> dict:
+        """The device-resident arrays of save_state's record (E:5718-5957) in
+        one library call: {key: array} for `names` (default: all 76), grids
+        and YNY shaped as the reference's matrices, marker vectors of
+        `marknum` entries (default: the resident count)."""
+        n = self.marknum if marknum is None else int(marknum)
+        table = save_fields(self.lib)
+        want = [k for k, _, _ in table] if names is None else list(names)
+        out, ptrs = {}, (ctypes.c_void_p * len(table))()
+        for i, (key, kind, field) in enumerate(table):
+            if key not in want:
+                continue
+            if kind == SAVE_GRID:
+                a = np.empty(self.grid_shape(GRID_FIELDS[field]), dtype=np.float64, order="F")
+            elif kind == SAVE_YNY:
+                a = np.empty((self.geom.Ny, self.geom.Nx), dtype=np.uint8, order="F")
+            else:
+                a = np.empty(n, dtype=np.int64 if field == MARKER_ID["tm"] else np.float64)
+            out[key] = a
+            ptrs[i] = a.ctypes.data
+        missing = set(want) - set(out)
+        if missing:
+            raise KeyError(f"not in the save_state field set: {sorted(missing)}")
+        self.check(self.lib.ereb_save_state_fields(self.h, ptrs, n), "save_state_fields")
         return out
 
     # ---- resident steps ---------------------------------------------------

@@ -112,6 +112,36 @@ function csc_get(ctx::Ctx, which, n, nnz)
     return SparseMatrixCSC(n, n, colptr, rowval, nzval)   # 1-based, rows sorted
 end
 
+# save_state (E:5718-5957): the 76 arrays of the JLD2 record that the device
+# holds, fetched in one call, as a Dict keyed by the record's names; the
+# caller merges its host-side entries (scalars, axes, the solver outputs)
+# and writes the record with jldsave(fid; pairs...).
+function save_state_fields(ctx::Ctx, Nx, Ny, marknum)
+    n = 76
+    name = Ref{Cstring}(); kind = Ref{Cint}(0); field = Ref{Cint}(0)
+    out = Dict{Symbol,Any}()
+    ptrs = Vector{Ptr{Cvoid}}(undef, n)
+    for k in 0:n-1
+        check(ctx, ccall((:ereb_save_field, LIB), Cint, (Cint, Ref{Cstring}, Ref{Cint}, Ref{Cint}),
+                         k, name, kind, field))
+        a = if kind[] == 0         # EREB_SAVE_GRID
+            m = ccall((:ereb_grid_numel, LIB), Int64, (Ptr{Cvoid}, Cint), ctx.h, field[])
+            m == Int64(Ny) * Nx ? Matrix{Float64}(undef, Ny, Nx) :
+                                  Matrix{Float64}(undef, Ny + 1, Nx + 1)
+        elseif kind[] == 1         # EREB_SAVE_YNY: one byte per node, 0/1
+            Matrix{UInt8}(undef, Ny, Nx)
+        else                       # EREB_SAVE_MARKER
+            field[] == 2 ? Vector{Int64}(undef, marknum) : Vector{Float64}(undef, marknum)
+        end
+        out[Symbol(unsafe_string(name[]))] = a
+        ptrs[k+1] = pointer(a)
+    end
+    GC.@preserve out check(ctx, ccall((:ereb_save_state_fields, LIB), Cint,
+                                      (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Int64), ctx.h, ptrs, marknum))
+    out[:YNY] = out[:YNY] .!= 0                   # Matrix{Bool}, as the reference stores it
+    return out
+end
+
 # Solver hand-off (E:6719-6749): the reference sets hydromech_cache.A = L
 # at every plastic iteration, and UMFPACK keeps its symbolic analysis while
 # colptr/rowval are unchanged.  While the library reports an unchanged

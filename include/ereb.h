@@ -108,6 +108,13 @@ int ereb_ctx_set_precision(ereb_ctx* ctx, int precision);
 int ereb_ctx_reserve(ereb_ctx* ctx, int64_t marknum);
 int ereb_sync(ereb_ctx* ctx);
 int ereb_version(void);
+/* Host-side set-up check, no device needed: discovers the stencil slot
+ * tables of the Stokes-Darcy and thermal systems by running the equation
+ * code (E:3128-3566, E:4544-4605) over a small grid on the host, as the
+ * first assembly does, and returns the slots per P cell (44 and 5).
+ * EREB_ERR_STATE when an equation references a column outside the 3x3-cell
+ * neighbourhood or a table overflows. */
+int ereb_host_selfcheck(int64_t* slots_stokes_darcy, int64_t* slots_thermal);
 
 /* ---- resident marker state --------------------------------------------- */
 /* Names follow the reference's marker vectors (E:55-71, E:593-668). */
@@ -175,6 +182,25 @@ int ereb_yny_set(ereb_ctx* ctx, const uint8_t* host);
 int ereb_yny_get(ereb_ctx* ctx, uint8_t* host);
 /* number of elements of a grid field (Ny*Nx or Ny1*Nx1) */
 int64_t ereb_grid_numel(const ereb_ctx* ctx, int field);
+
+/* ---- save_state field set (E:5718-5957) --------------------------------- */
+/* save_state writes one JLD2 record per output step (calls E:6193, E:7106).
+ * Of its arrays, the library holds 76: 52 node grids, YNY and 23 marker
+ * vectors.  Entry k of the set, in the order of the record's keyword list
+ * (E:5864-5957), is described by ereb_save_field: the record's key, its kind
+ * and its ereb_grid_field / ereb_marker_field value.  The record's scalars,
+ * axes and the 17 arrays produced by the host solves and plastic iterations
+ * (EXY, SXY, qxD, qyD, EXX, SXX, vxp, vyp, vxpf, vypf, FI, ETA5, ETA00, YNY5,
+ * YNY00, EII, SII) stay with the caller, as does the JLD2 write. */
+#define EREB_SAVE_GRID    0 /* double, ereb_grid_numel(field) entries */
+#define EREB_SAVE_YNY     1 /* uint8, Ny*Nx entries */
+#define EREB_SAVE_MARKER  2 /* marknum entries: double, tm as int64 */
+#define EREB_SAVE_NFIELDS 76
+int ereb_save_field(int k, const char** name, int* kind, int* field);
+/* One call: dst[k] (EREB_SAVE_NFIELDS host pointers; NULL skips entry k)
+ * receives entry k of the set, as ereb_grid_get / ereb_yny_get /
+ * ereb_markers_get would write it; marker vectors get `marknum` entries. */
+int ereb_save_state_fields(ereb_ctx* ctx, void* const* dst, int64_t marknum);
 
 /* ---- resident steps ----------------------------------------------------- */
 /* Step 1 + 2: the marker loop E:6419-6517 (compute_marker_properties! E:700

@@ -16,7 +16,10 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "libereb_oracle.so")
+# EREB_ORACLE_BUILD selects another build of the same sources (the tests'
+# sanitizer run sets it to "_build_san")
+_BUILD = os.path.join(HERE, os.environ.get("EREB_ORACLE_BUILD", "_build"))
+LIB = os.path.join(_BUILD, "libereb_oracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "erebus.jl_amd"))
 
 from ereb.constants import CGeometry, CMaterials, Geometry, Hydration, Materials  # noqa: E402
@@ -397,7 +400,7 @@ class Oracle:
         return w
 
 
-BENCH_LIB = os.path.join(HERE, "_build", "libereb_oracle_bench.so")
+BENCH_LIB = os.path.join(_BUILD, "libereb_oracle_bench.so")
 # marker vectors of orc_bench_transfer, in its order (ereb_cpu_step.c)
 BENCH_MARKERS = ["xm", "ym", "tkm", "phim", "sxxm", "sxym", "XWsolidm0", "etavpm",
                  "inv_gggtotalm", "fricttotalm", "cohestotalm", "tenstotalm", "rhofluidcur",
@@ -437,7 +440,7 @@ def bench_transfer(geom: Geometry, mat: Materials, mk: dict, grids: dict, hrs, h
     return int(done), sec.value, int(L.orc_bench_threads())
 
 
-DUAL_LIB = os.path.join(HERE, "_build", "libereb_oracle_dual.so")
+DUAL_LIB = os.path.join(_BUILD, "libereb_oracle_dual.so")
 _dual = None
 
 # marker seeds of the dual functions

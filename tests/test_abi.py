@@ -69,3 +69,35 @@ def test_version_and_null_handling():
 def test_library_path_is_in_tree():
     assert os.path.commonpath([ereb.LIB_PATH, ROOT]) == ROOT
     assert os.path.exists(ereb.LIB_PATH)
+
+
+# The arrays among save_state's jldsave keywords (E:5864-5957), in order.
+SAVE_STATE_ARRAYS = """
+ETA ETA0 GGG EXY SXY SXY0 wyx COH TEN FRI YNY RHOX RHOFX KX PHIX vx vxf RX qxD gx
+RHOY RHOFY KY PHIY vy vyf RY qyD gy RHO RHOCP ALPHA ALPHAF HR HA HS ETAP GGGP EXX
+SXX SXX0 tk1 tk2 vxp vyp vxpf vypf pr pf ps pr0 pf0 ps0 ETAPHI BETAPHI PHI APHI FI
+ETA5 ETA00 YNY5 YNY00 YNY_inv_ETA DSXY EII SII DSXX DMP DHP XWS XWsolidm0 xm ym tm
+tkm sxxm sxym etavpm phim rhototalm rhocptotalm etatotalm hrtotalm ktotalm
+tkm_rhocptotalm etafluidcur_inv_kphim inv_gggtotalm fricttotalm cohestotalm
+tenstotalm rhofluidcur alphasolidcur alphafluidcur""".split()
+# produced by the host solves and the plastic iterations, not held on the device
+NOT_RESIDENT = set("EXY SXY qxD qyD EXX SXX vxp vyp vxpf vypf FI ETA5 ETA00 YNY5 YNY00 "
+                   "EII SII".split())
+
+
+def test_save_state_field_set():
+    """The library's save_state field set is the record's arrays minus the 17
+    the device does not hold, in the record's order, each key naming the
+    resident field of the same name."""
+    table = L.save_fields()
+    assert len(table) == 76
+    assert L.load().ereb_save_field(-1, None, None, None) == L.ERR_ARG
+    assert [k for k, _, _ in table] == [a for a in SAVE_STATE_ARRAYS if a not in NOT_RESIDENT]
+    for key, kind, field in table:
+        if kind == L.SAVE_GRID:
+            assert L.GRID_FIELDS[field] == key
+        elif kind == L.SAVE_MARKER:
+            assert L.MARKER_FIELDS[field] == key
+        else:
+            assert (kind, key) == (L.SAVE_YNY, "YNY")
+    assert L.load().ereb_save_field(76, None, None, None) == L.ERR_ARG

@@ -55,10 +55,18 @@ def _pick(ctx, g, mk, hrs, hrf, n, rng, margin):
     return rng.choice(ok, n, replace=False)
 
 
+# Relative central-difference steps.  phim moves tk1 through sums that are
+# linear in it: at a step of 1e-6 the change of tk1 (~1e-8 K) is a few
+# hundred thousand ulps of tk1, so the quotient carries a rounding noise of
+# ~1e-5 (measured 3e-6 to 5e-5 on single markers); at 1e-4 the noise is
+# ~1e-7 and the truncation error ~1e-8.
+FD_STEP = {"tkm": 1e-6, "phim": 1e-4}
+
+
 def _fd_check(ctx, mk, idx, nsteps, every, hrs, hrf, gT, gP, bar):
     for name, grad in (("tkm", gT), ("phim", gP)):
         for m in idx:
-            h = 1e-6 * abs(mk[name][m])
+            h = FD_STEP[name] * abs(mk[name][m])
             tk = []
             for sgn in (1.0, -1.0):
                 a = mk[name].copy()
