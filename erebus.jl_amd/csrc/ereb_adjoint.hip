@@ -336,8 +336,10 @@ __global__ void k_fix_to_grid(const unsigned long long* __restrict__ wa,
 
 // bits[0] = max(bits[0], max |a[i]|) over i < n, one atomic per wavefront:
 // the bound of a fixed-point pass (non-negative doubles order as their bits)
+// (one atomic per workgroup: many atomics on one address serialise)
 __global__ __launch_bounds__(256) void k_absmax_bits(const double* __restrict__ a, int64_t n,
                                                      unsigned long long* __restrict__ bits) {
+    __shared__ double wmax[4];
     double mx = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -348,12 +350,17 @@ __global__ __launch_bounds__(256) void k_absmax_bits(const double* __restrict__ 
         const double y = __shfl_xor(mx, o);
         mx = y > mx || y != y ? y : mx;
     }
-    if (__lane_id() == 0 && mx != 0.0)
-        atomicMax(bits, (unsigned long long)__double_as_longlong(mx));
+    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+            mx = wmax[w] > mx || wmax[w] != wmax[w] ? wmax[w] : mx;
+        if (mx != 0.0) atomicMax(bits, (unsigned long long)__double_as_longlong(mx));
+    }
 }
 
 static int absmax_bits(ereb_ctx* c, const double* a, int64_t n, unsigned long long* bits) {
-    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 256);
     hipLaunchKernelGGL(k_absmax_bits, dim3(std::max(nb, 1u)), dim3(256), 0, c->stream, a, n,
                        bits);
     EREB_HIP(c, hipGetLastError());

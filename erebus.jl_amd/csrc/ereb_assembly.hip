@@ -362,17 +362,50 @@ __global__ __launch_bounds__(256) void k_asm_rows(AsmArgs A0, const StepScalars*
     }
 }
 
-// 2. column counts: one thread per column walks the rows that can hold an
-// entry in it (the reverse slot table of its unknown), in ascending row
-// order, and counts the present ones; the slot tables are staged in LDS
-// run (optional): the kernel does its work only when run[0] != 0
+// 2. column counts per block of columns: one thread per column walks the
+// rows that can hold an entry in it (the reverse slot table of its unknown),
+// and the block writes the number of entries of its columns, bsum[block];
+// the fill then computes each column's start from these and its own count,
+// so the column pointers need no separate scan.  The slot tables are staged
+// in LDS.  run (optional): the kernel does its work only when run[0] != 0.
+
+// exclusive scan over the workgroup (blockDim.x <= 1024); sh holds 17 ints
+__device__ __forceinline__ int32_t asm_block_scan(int32_t v, int32_t* sh, int32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        const int32_t w = lane < nw ? sh[lane] : 0;
+        int32_t t = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < nw) sh[lane] = t - w;
+        if (lane == nw - 1) sh[16] = t;
+    }
+    __syncthreads();
+    const int32_t r = sh[wid] + x - v;
+    *total = sh[16];
+    __syncthreads();  // sh is reused by the caller
+    return r;
+}
+
 __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
                                                    const SlotTables* __restrict__ Tg,
                                                    const uint16_t* __restrict__ pres,
-                                                   int32_t* __restrict__ colcnt,
+                                                   int32_t* __restrict__ bsum,
                                                    const int32_t* __restrict__ run) {
     if (run && run[0] == 0) return;
     __shared__ SlotTables T;
+    __shared__ int32_t sh[17];
     static_assert(sizeof(SlotTables) % 4 == 0, "table size");
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
         ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
@@ -380,27 +413,33 @@ __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
     const int ne = T.ne;
     const int64_t ncell = (int64_t)Nx1 * Ny1;
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncell * ne) return;
-    const int u = (int)(col % ne);
-    const int64_t k = col / ne;
-    const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
     int32_t p = 0;
-    const int nr = T.nrev[u];
-    for (int r = 0; r < nr; ++r) {
-        const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
-        if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
-        const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
-        const int64_t cell = (int64_t)jr * Ny1 + ir;
-        p += (pres[(size_t)e * ncell + cell] >> sl) & 1;
+    if (col < ncell * ne) {
+        const int u = (int)(col % ne);
+        const int64_t k = col / ne;
+        const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
+        const int nr = T.nrev[u];
+        for (int r = 0; r < nr; ++r) {
+            const int ir = ic - T.rev_di[u][r], jr = jc - T.rev_dj[u][r];
+            if (ir < 0 || ir >= Ny1 || jr < 0 || jr >= Nx1) continue;
+            const int e = T.rev_e[u][r], sl = T.rev_s[u][r];
+            const int64_t cell = (int64_t)jr * Ny1 + ir;
+            p += (pres[(size_t)e * ncell + cell] >> sl) & 1;
+        }
     }
-    colcnt[col] = p;
+    int32_t tot;
+    asm_block_scan(p, sh, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// 3. rowval/nzval: the same column walk, with the entries of the block's
-// columns staged in LDS at their CSC positions and then written out by
-// consecutive threads (a thread's own entries are ~7 apart from its
-// neighbours', so direct stores would fill a fraction of each cache line).
-// Rows come out ascending in every column with no sort.
+// 3. colptr/rowval/nzval: the same column walk; a column's start is the sum
+// of the block sums of the blocks before its own plus the exclusive scan of
+// the counts within the block (colptr written here, the last block adding
+// colptr[n]).  The entries of the block's columns are staged in LDS at their
+// CSC positions and then written out by consecutive threads (a thread's own
+// entries are ~7 apart from its neighbours', so direct stores would fill a
+// fraction of each cache line).  Rows come out ascending in every column
+// with no sort.
 // blockDim.x * (most entries of a column) <= ASM_STAGE, checked on the host.
 constexpr int ASM_STAGE = 4096;
 
@@ -417,7 +456,8 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                                                   const SlotTables* __restrict__ Tg,
                                                   const uint16_t* __restrict__ pres,
                                                   const double* __restrict__ slotv,
-                                                  const int32_t* __restrict__ colptr,
+                                                  const int32_t* __restrict__ bsum,
+                                                  int32_t* __restrict__ colptr,
                                                   IDX* __restrict__ rowval,
                                                   double* __restrict__ nzval,
                                                   const int32_t* __restrict__ run,
@@ -427,23 +467,27 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
     __shared__ int32_t srow[ASM_STAGE];
     __shared__ int32_t smap[ASM_STAGE];
     __shared__ double sval[ASM_STAGE];
+    __shared__ int32_t sh[17];
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
         ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
-    __syncthreads();
+    // entries of the blocks before this one
+    int32_t before = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += blockDim.x) before += bsum[b];
+    int32_t base;
+    asm_block_scan(before, sh, &base);  // synchronises: T is staged
     const int ne = T.ne;
     const int64_t ncell = (int64_t)Nx1 * Ny1, n = ncell * ne;
     const int64_t col0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t colN = min(col0 + (int64_t)blockDim.x, n);
-    const int32_t base = colptr[col0], total = colptr[colN] - base;
     const int64_t col = col0 + threadIdx.x;
+    bool on[MAXR];
+    int32_t rw[MAXR], mp[MAXR];
+    double vv[MAXR];
+    int32_t cnt = 0;
     if (col < n) {
         const int u = (int)(col % ne);
         const int64_t k = col / ne;
         const int ic = (int)(k % Ny1), jc = (int)(k / Ny1);
         const int nr = T.nrev[u];
-        bool on[MAXR];
-        int32_t rw[MAXR], mp[MAXR];
-        double vv[MAXR];
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             on[r] = false;
@@ -458,8 +502,13 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                 vv[r] = slotv[mp[r]];
                 rw[r] = (int32_t)(cell * ne + e);
             }
+            cnt += on[r] ? 1 : 0;
         }
-        int32_t p = colptr[col] - base;
+    }
+    int32_t total;
+    int32_t p = asm_block_scan(cnt, sh, &total);
+    if (col < n) {
+        colptr[col] = base + p;
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             if (on[r]) {
@@ -470,6 +519,7 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
             }
         }
     }
+    if (col == n - 1) colptr[n] = base + total;
     __syncthreads();
     for (int q = threadIdx.x; q < total; q += blockDim.x) {
         rowval[base + q] = (IDX)srow[q] + BASE;
@@ -663,26 +713,25 @@ static int assemble(ereb_ctx* c, double dt) {
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, c->d_scal, T, s.pres,
                        s.slotv, c->d_flags, s.pres_prev, s.chg);
     s.have_prev = true;
-    const dim3 cgrid((unsigned)((n + 255) / 256));
     const int32_t* chg = s.chg;
-    hipLaunchKernelGGL(k_asm_count, cgrid, dim3(256), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
-                       s.colcnt, chg);
-    if ((rc = scan_exclusive(c, s.colcnt, s.colptr, n, nullptr, false, chg))) return rc;
     // a block's entries must fit the LDS stage: 256 columns of up to 16
-    // entries, else 128 of up to 32 (SLOT_R)
+    // entries, else 128 of up to 32 (SLOT_R); the count's blocks are the
+    // fill's
     const bool narrow = s.maxrev <= ASM_STAGE / 256;
     static_assert(ASM_STAGE / 256 == 16, "16 candidates per column in the 256-column blocks");
     static_assert(ASM_STAGE / 128 >= SLOT_R, "stage holds 128 columns of SLOT_R entries");
     const int fb = narrow ? 256 : 128;
     const dim3 fgrid((unsigned)((n + fb - 1) / fb));
+    hipLaunchKernelGGL(k_asm_count, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T, s.pres,
+                       s.colcnt, chg);
     if (narrow)
         hipLaunchKernelGGL(k_asm_fill<16>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
-                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval, chg,
-                           s.map);
+                           s.pres, s.slotv, (const int32_t*)s.colcnt, s.colptr, s.rowval,
+                           s.nzval, chg, s.map);
     else
         hipLaunchKernelGGL(k_asm_fill<SLOT_R>, fgrid, dim3(fb), 0, c->stream, c->Nx1, c->Ny1, T,
-                           s.pres, s.slotv, (const int32_t*)s.colptr, s.rowval, s.nzval, chg,
-                           s.map);
+                           s.pres, s.slotv, (const int32_t*)s.colcnt, s.colptr, s.rowval,
+                           s.nzval, chg, s.map);
     hipLaunchKernelGGL(k_asm_values,
                        dim3((unsigned)std::min<int64_t>((cap + 255) / 256, ASM_VALUES_BLOCKS)),
                        dim3(256), 0, c->stream, chg, (const int32_t*)s.colptr + n,
@@ -698,8 +747,8 @@ int launch_assemble_thermal(ereb_ctx* c, double dt) { return assemble<1>(c, dt);
 
 // ---- thermal assembly into caller buffers, scratch supplied ------------------
 // Scratch layout (each piece 256-byte aligned): the step scalars (dt), the
-// flags, the slot tables, the presence bits, the slot values, the column
-// counts, the 0-based column pointers and the scan's block sums.
+// flags, the slot tables, the presence bits, the slot values, the block sums
+// of the column counts and the 0-based column pointers.
 namespace {
 struct ThermalScratch {
     StepScalars* scal;
@@ -707,9 +756,8 @@ struct ThermalScratch {
     SlotTables* tables;
     uint16_t* pres;
     double* slotv;
-    int32_t* colcnt;
+    int32_t* colcnt;  // block sums of the column counts
     int32_t* colptr;
-    int32_t* blocks;
     size_t bytes;
 };
 constexpr int THERMAL_SLOTS = 5;  // slots per cell of the 5-point equation
@@ -729,7 +777,6 @@ ThermalScratch thermal_carve(char* base, int64_t ncell) {
     t.slotv = (double*)take(sizeof(double) * THERMAL_SLOTS * ncell);
     t.colcnt = (int32_t*)take(sizeof(int32_t) * ncell);
     t.colptr = (int32_t*)take(sizeof(int32_t) * (ncell + 1));
-    t.blocks = (int32_t*)take(sizeof(int32_t) * scan_blocks(ncell));
     t.bytes = off;
     return t;
 }
@@ -782,12 +829,10 @@ int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8]
     hipLaunchKernelGGL(k_asm_count, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1,
                        (const SlotTables*)t.tables, (const uint16_t*)t.pres, t.colcnt,
                        (const int32_t*)nullptr);
-    int rc = scan_exclusive_buf(c, t.colcnt, t.colptr, ncell, t.blocks);
-    if (rc) return rc;
     hipLaunchKernelGGL((k_asm_fill<16, int64_t, 1>), grid, dim3(256), 0, c->stream, c->Nx1,
                        c->Ny1, (const SlotTables*)t.tables, (const uint16_t*)t.pres,
-                       (const double*)t.slotv, (const int32_t*)t.colptr, rowval, nzval,
-                       (const int32_t*)nullptr, (int32_t*)nullptr);
+                       (const double*)t.slotv, (const int32_t*)t.colcnt, t.colptr, rowval,
+                       nzval, (const int32_t*)nullptr, (int32_t*)nullptr);
     hipLaunchKernelGGL(k_colptr_1based, dim3((unsigned)((ncell + 1 + 255) / 256)), dim3(256), 0,
                        c->stream, (const int32_t*)t.colptr, colptr, ncell + 1);
     EREB_HIP(c, hipGetLastError());

@@ -197,17 +197,18 @@ __device__ __forceinline__ void rk_half(double* __restrict__ out, double a0, dou
     out[5] = r2 - p2;
 }
 
-// One thread per 16-byte piece of the tables: piece q (0..5) of cell k of
-// table t (gridDim.y = 4) holds coefficients 2(q mod 3), 2(q mod 3) + 1 of
-// half q / 3, so consecutive threads write consecutive 16 bytes.  Each
-// thread evaluates its half's six coefficients (the inputs are in L1/L2).
+// One thread per 16-byte piece of the tables: piece q (0..5) of cell (i, j)
+// of table t holds coefficients 2(q mod 3), 2(q mod 3) + 1 of half q / 3, so
+// consecutive threads write consecutive 16 bytes.  Grid: x over the 6 Ny1
+// pieces of a node column, y = column j - 1, z = table t.  Each thread
+// evaluates its half's six coefficients (the inputs are in L1/L2).
 __global__ __launch_bounds__(256) void k_rk4_coef(RKCoefArgs A) {
     const int R = A.Ny + 1;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)6 * R * (A.Nx + 1)) return;
-    const int k = (int)(p / 6), q = (int)(p % 6);
-    const int i = k % R + 1, j = k / R + 1;
-    const int t = blockIdx.y;  // 0 Vx solid, 1 Vy solid, 2 Vx fluid, 3 Vy fluid
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 6 * R) return;
+    const int i = p / 6 + 1, q = p % 6, j = blockIdx.y + 1;
+    const int k = (j - 1) * R + (i - 1);
+    const int t = blockIdx.z;  // 0 Vx solid, 1 Vy solid, 2 Vx fluid, 3 Vy fluid
     const bool xg = (t & 1) == 0, fluid = t >= 2;
     const bool second = q >= 3;
     double c[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -371,7 +372,14 @@ __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, in
 // each marker's state read and written once.
 template <int GATHER, bool EXACT>
 __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
+    // round robin, so workgroup b runs on XCD b mod 8; numbering them
+    // XCD-major gives each XCD one contiguous run of markers (one band of
+    // the domain), whose coefficient-table lines then stay in that XCD's L2
+    // instead of being fetched by all eight
+    const uint32_t nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
+    const uint32_t bb = x * q + min(x, r) + (b >> 3);
+    const int64_t m = (int64_t)bb * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
     const GeomK& g = A.g;
     const double dt = A.S->dt;
@@ -404,10 +412,34 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     // analytic stress rotation E:5042-5048
     const double th = dt * om;
     // sin(2θ) and cos(2θ) from the double-angle identities (within a few ulp
-    // of the library values; one argument reduction and polynomial instead
-    // of two)
+    // of the library values)
     double ct, st;
-    sincos(th, &st, &ct);
+    if (fabs(th) <= 0.78539816339744828) {
+        // Taylor series to the 17th (sine) and 16th (cosine) power, Horner in
+        // th^2: the first omitted terms are below 2^-60 of the result for
+        // |th| <= pi/4; beyond that the library sincos
+        const double t2 = th * th;
+        double ps = 1.0 / 355687428096000.0, pc = 1.0 / 6402373705728000.0;
+        ps = fma(ps, -t2, 1.0 / 1307674368000.0);
+        pc = fma(pc, -t2, 1.0 / 20922789888000.0);
+        ps = fma(ps, -t2, 1.0 / 6227020800.0);
+        pc = fma(pc, -t2, 1.0 / 87178291200.0);
+        ps = fma(ps, -t2, 1.0 / 39916800.0);
+        pc = fma(pc, -t2, 1.0 / 479001600.0);
+        ps = fma(ps, -t2, 1.0 / 362880.0);
+        pc = fma(pc, -t2, 1.0 / 3628800.0);
+        ps = fma(ps, -t2, 1.0 / 5040.0);
+        pc = fma(pc, -t2, 1.0 / 40320.0);
+        ps = fma(ps, -t2, 1.0 / 120.0);
+        pc = fma(pc, -t2, 1.0 / 720.0);
+        ps = fma(ps, -t2, 1.0 / 6.0);
+        pc = fma(pc, -t2, 1.0 / 24.0);
+        pc = fma(pc, -t2, 0.5);
+        st = fma(-ps * t2, th, th);
+        ct = fma(-pc, t2, 1.0);
+    } else {
+        sincos(th, &st, &ct);
+    }
     const double s2 = 2.0 * (st * ct), c2 = (ct - st) * (ct + st);
     A.sxxm[m] = s * (ct * ct) - s * (st * st) - t * s2;
     A.sxym[m] = s * s2 + t * c2;
@@ -692,11 +724,10 @@ static RK4Args rk4_args(ereb_ctx* c, int mode) {
 
 // the stage-velocity coefficient tables of this step's velocities (k_rk4_coef)
 static void launch_rk4_prep(ereb_ctx* c, const RK4Args& A) {
-    const int64_t n = (int64_t)6 * c->Nx1 * c->Ny1;
     RKCoefArgs P{c->Nx, c->Ny, A.vx, A.vy, A.vxf, A.vyf, (double*)A.CXs, (double*)A.CYs,
                  (double*)A.CXf, (double*)A.CYf};
-    hipLaunchKernelGGL(k_rk4_coef, dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0, c->stream,
-                       P);
+    hipLaunchKernelGGL(k_rk4_coef, dim3((unsigned)((6 * c->Ny1 + 255) / 256), c->Nx1, 4),
+                       dim3(256), 0, c->stream, P);
 }
 
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {

@@ -7,6 +7,8 @@ counters read for a plain device copy of known size.
     python -m ereb.rocprof_summary traffic --fetch D --write D --copy-fetch D \\
         --copy-write D --copy-bytes N --workload W --form F --command C \\
         --out-json OUT.json --out-md OUT.md
+    python -m ereb.rocprof_summary counters D [D ...] --kernel PREFIX [...] \\
+        --title T --command C [--command C ...] --out OUT.md
 
 TRACE_DIR and the counter directories are rocprofv3 output directories
 (searched recursively for ``*kernel_trace.csv`` / ``*counter_collection.csv``).
@@ -126,6 +128,54 @@ def write_traffic(a):
     open(a.out_md, "w").write("\n".join(lines) + "\n")
 
 
+DERIVED = [
+    ("waves", lambda c: c.get("SQ_WAVES")),
+    ("VALU instr per wave", lambda c: _div(c.get("SQ_INSTS_VALU"), c.get("SQ_WAVES"))),
+    ("VALU active", lambda c: _div(c.get("SQ_ACTIVE_INST_VALU"), c.get("SQ_WAVE_CYCLES"))),
+    ("waiting on any", lambda c: _div(c.get("SQ_WAIT_INST_ANY"), c.get("SQ_WAVE_CYCLES"))),
+    ("LDS instr per wave", lambda c: _div(c.get("SQ_INSTS_LDS"), c.get("SQ_WAVES"))),
+    ("LDS bank-conflict share",
+     lambda c: _div(c.get("SQ_LDS_BANK_CONFLICT"), c.get("SQ_ACTIVE_INST_LDS"))),
+    ("L2 hit rate", lambda c: _div(c.get("TCC_HIT"), _add(c.get("TCC_HIT"), c.get("TCC_MISS")))),
+]
+
+
+def _div(a, b):
+    return None if a is None or not b else a / b
+
+
+def _add(a, b):
+    return None if a is None or b is None else a + b
+
+
+def write_counters(dirs, kernels, title, commands, out):
+    """Per-dispatch means of every counter found in the runs `dirs`, for the
+    kernels whose names start with one of `kernels`, and derived rates."""
+    vals = defaultdict(dict)
+    for d in dirs:
+        rows = list(_rows(d, "counter_collection.csv"))
+        for cname in sorted({r["counter_name"].strip() for r in rows}):
+            base = cname.replace("_sum", "")
+            for k, v in counter_means(d, cname).items():
+                vals[k][base] = v
+    lines = [f"# {title}", "", "Commands (each its own run, from /tmp with TMPDIR=/tmp; "
+             "`$R` = repository root):", ""] + [f"    {c}" for c in commands] + [
+             "", "Per-dispatch means.  Cycle counters are summed over waves, so the fractions "
+             "are per wave (a wave shares its SIMD with the others resident there); the LDS "
+             "bank-conflict share is `SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS`.", "",
+             "| kernel | " + " | ".join(n for n, _ in DERIVED) + " |",
+             "|---|" + "---:|" * len(DERIVED)]
+    for pre in kernels:
+        for k in sorted(vals):
+            if k.startswith(pre):
+                cells = []
+                for _, f in DERIVED:
+                    v = f(vals[k])
+                    cells.append("—" if v is None else (f"{v:.0f}" if v >= 100 else f"{v:.3g}"))
+                lines.append(f"| `{_short(k, 60)}` | " + " | ".join(cells) + " |")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -139,9 +189,17 @@ def main():
               "--command", "--out-json", "--out-md"):
         t.add_argument(n, required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--kernel", action="append", required=True, help="kernel name prefix")
+    c.add_argument("--title", required=True)
+    c.add_argument("--command", action="append", required=True)
+    c.add_argument("--out", required=True)
     a = ap.parse_args()
     if a.cmd == "kernels":
         write_kernels(a.trace_dir, a.title, a.command, a.out)
+    elif a.cmd == "counters":
+        write_counters(a.dirs, a.kernel, a.title, a.command, a.out)
     else:
         write_traffic(a)
 
