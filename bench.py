@@ -76,8 +76,12 @@ START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 ALG_BYTES = {
     "fp64": {"sort": (92, 0), "scatter": (160, 240), "combine": (0, 0), "nodes": (0, 457),
              "rk4": (172, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
-    "mixed": {"sort": (48, 0), "scatter": (88, 240), "combine": (0, 0), "nodes": (0, 457),
-              "rk4": (120, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
+    # mixed form: the 16 material and property vectors 4 B, the marker state
+    # (xm ym tkm phim sxxm sxym) and every grid 8 B
+    #   sort    r tm 4 + tkm phim 16 + X_W rhocp 8, w 7 x 4
+    #   scatter r xm ym 16 + phim sxx sxy 24 + 15 properties x 4
+    "mixed": {"sort": (56, 0), "scatter": (100, 240), "combine": (0, 0), "nodes": (0, 457),
+              "rk4": (172, 72), "asm_sd": (0, 208), "asm_t": (0, 72)},
 }
 STEP_CLASSES = ["sort", "scatter", "combine", "nodes", "rk4", "asm_sd", "asm_t"]
 # kernel-name prefix of each class's main kernel, per form, in the rocprofv3

@@ -310,6 +310,9 @@ static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const d
 struct AdjBuffers {
     double *save = nullptr, *cps = nullptr, *st = nullptr, *snap = nullptr, *xb = nullptr;
     double *gT = nullptr, *gT2 = nullptr, *gP = nullptr, *gg = nullptr;
+    // per recorded step: the scatter's sort order of the step's start
+    // positions, the order the transposed kernels walk the markers in
+    uint32_t* perm = nullptr;
     // fixed-point sums of the tk2 cotangent of RK4^T (A) and of the gather^T
     // (B): bounds [0..1] (bits of the largest marker cotangent), flag [2],
     // then 2 words per P node for A and for B
@@ -372,13 +375,14 @@ static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int6
                        size_t gsz) {
     const size_t M = (size_t)c->marknum;
     const size_t np = (size_t)c->Nx1 * c->Ny1;
-    const size_t sizes[10] = {5 * M,
+    const size_t sizes[11] = {5 * M,
                               grad ? 3 * M * (size_t)nseg : 0,
                               grad ? 3 * M * (size_t)every : 0,
                               grad ? gsz * (size_t)every : 0,
                               grad ? 2 * M * (size_t)every : 0,
                               grad ? M : 0, grad ? M : 0, grad ? M : 0, grad ? gsz : 0,
-                              grad ? 8 + 4 * np : 0};
+                              grad ? 8 + 4 * np : 0,
+                              grad ? (M * (size_t)every + 1) / 2 : 0};
     size_t tot = 0;
     for (size_t z : sizes) tot += (z + 31) / 32 * 32;  // 256-byte aligned pieces
     if (tot > c->adj_pool_n) {
@@ -390,14 +394,15 @@ static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int6
         c->adj_pool_n = tot;
     }
     double* p = c->adj_pool;
-    double* fix = nullptr;
-    double** dst[10] = {&B.save, &B.cps, &B.st, &B.snap, &B.xb, &B.gT, &B.gT2, &B.gP, &B.gg,
-                        &fix};
-    for (int k = 0; k < 10; ++k) {
+    double *fix = nullptr, *perm = nullptr;
+    double** dst[11] = {&B.save, &B.cps, &B.st, &B.snap, &B.xb, &B.gT, &B.gT2, &B.gP, &B.gg,
+                        &fix, &perm};
+    for (int k = 0; k < 11; ++k) {
         *dst[k] = sizes[k] ? p : nullptr;
         p += (sizes[k] + 31) / 32 * 32;
     }
     B.fix = (unsigned long long*)fix;
+    B.perm = (uint32_t*)perm;
     return EREB_OK;
 }
 
@@ -472,9 +477,15 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
             yb = xb + M;
         }
         if ((r = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, xb, yb))) return r;
-        if (record)
+        if (record) {
             EREB_HIP(c, hipMemcpyAsync(B.snap + gsz * (size_t)(k - k0), c->d_grids, GB,
                                        hipMemcpyDeviceToDevice, c->stream));
+            // the step's scatter sorted its start positions (RK4 moved them
+            // afterwards without a new sort)
+            EREB_HIP(c, hipMemcpyAsync(B.perm + (size_t)M * (k - k0), c->d_perm,
+                                       sizeof(uint32_t) * (size_t)M, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
         return EREB_OK;
     };
     for (int64_t k = 0; k < nsteps; ++k) {
@@ -545,11 +556,16 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 FixGrid FA = fix_grid(B, 0, n), FB = fix_grid(B, 1, n);
                 FA.bound = FA.maxbits;
                 FB.bound = FB.maxbits;
+                // the transposed kernels walk the markers in the step's sort
+                // order: a workgroup's stencils then cover a compact patch of
+                // nodes, and its window needs few device-wide atomics
+                const uint32_t* order = B.perm + (size_t)M * (k - k0);
                 if ((rc = absmax_bits(c, B.gT, M, FA.maxbits)) ||
-                    (rc = launch_rk4_adj(c, 1, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag)))
+                    (rc = launch_rk4_adj(c, 1, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag,
+                                         order)))
                     return rc;
                 if (k == 0 && ((rc = absmax_bits(c, B.gT2, M, FB.maxbits)) ||
-                               (rc = launch_gather_T_adj(c, 1, 1, B.gT2, FB, flag))))
+                               (rc = launch_gather_T_adj(c, 1, 1, B.gT2, FB, flag, order))))
                     return rc;
                 hipLaunchKernelGGL(k_fix_to_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                                    c->stream, (const unsigned long long*)FA.w, FA.bound,

@@ -18,18 +18,19 @@
 //              R is written here;
 //   2. count : one thread per column walks the rows that can reference it
 //              (the reverse slot table) in ascending row order and counts
-//              the present entries; an exclusive scan gives the column
-//              pointers;
-//   3. fill  : the same walk writes rowval/nzval of each column contiguously,
-//              and for every stored entry the index of its slot value.
+//              the present entries, summed per block of columns;
+//   3. fill  : the same walk gives each column's start (the block sums
+//              before it plus a scan within the block: colptr), writes
+//              rowval of each column contiguously and, for every stored
+//              entry, the index of its slot value;
+//   4. values: nzval, one gather per entry through those indices.
 // No atomics and no sort: rows come out ascending in every column.
 // Step 1 also compares the presence bits with the previous assembly's (a
 // device flag): when the pattern is unchanged
-// -- the common case from one timestep or plastic iteration to the next --
-// steps 2 and 3 return at once and k_asm_values refills nzval with one
-// gather through the stored slot indices, colptr and rowval being those of
-// the previous assembly.  The condition is evaluated on the device, so a
-// captured step graph takes either path.
+// -- the common case from one plastic iteration to the next --
+// steps 2 and 3 return at once and step 4 alone refills nzval, colptr and
+// rowval being those of the previous assembly.  The condition is evaluated
+// on the device, so a captured step graph takes either path.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -444,14 +445,17 @@ __global__ __launch_bounds__(256) void k_asm_count(int Nx1, int Ny1,
 constexpr int ASM_STAGE = 4096;
 
 // MAXR: most candidate rows of a column (16, or SLOT_R); every candidate's
-// presence bit and value are loaded before any is used (a value whose bit is
-// clear is read from its allocated slot and dropped), so a column costs one
-// memory round trip instead of one per candidate.
+// presence bit is loaded before any is used, so a column costs one memory
+// round trip instead of one per candidate.  VALUES: nzval written here too
+// (each candidate's value loaded with its bit, a value whose bit is clear
+// dropped); otherwise only rowval and the slot index map, and k_asm_values
+// writes nzval by one gather per stored entry (a quarter of the candidate
+// loads, and half the LDS: twice the workgroups per CU).
 // IDX, BASE: the row index type and base of the output (int32 0-based for
 // the context's device CSC, Int64 1-based for SparseMatrixCSC{Float64,Int64})
 // run (optional): only when run[0] != 0; map (optional): the slot value's
 // index of every stored entry, for the values-only refill
-template <int MAXR, class IDX = int32_t, int BASE = 0>
+template <int MAXR, class IDX = int32_t, int BASE = 0, bool VALUES = false>
 __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                                                   const SlotTables* __restrict__ Tg,
                                                   const uint16_t* __restrict__ pres,
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
     __shared__ SlotTables T;
     __shared__ int32_t srow[ASM_STAGE];
     __shared__ int32_t smap[ASM_STAGE];
-    __shared__ double sval[ASM_STAGE];
+    __shared__ double sval[VALUES ? ASM_STAGE : 1];
     __shared__ int32_t sh[17];
     for (int q = threadIdx.x; q < (int)(sizeof(SlotTables) / 4); q += blockDim.x)
         ((int32_t*)&T)[q] = ((const int32_t*)Tg)[q];
@@ -499,7 +503,7 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
                 const int64_t cell = (int64_t)jr * Ny1 + ir;
                 on[r] = (pres[(size_t)e * ncell + cell] >> sl) & 1;
                 mp[r] = (int32_t)(((int64_t)T.slot0[e] + sl) * ncell + cell);
-                vv[r] = slotv[mp[r]];
+                if (VALUES) vv[r] = slotv[mp[r]];
                 rw[r] = (int32_t)(cell * ne + e);
             }
             cnt += on[r] ? 1 : 0;
@@ -514,7 +518,7 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
             if (on[r]) {
                 srow[p] = rw[r];
                 smap[p] = mp[r];
-                sval[p] = vv[r];
+                if (VALUES) sval[p] = vv[r];
                 ++p;
             }
         }
@@ -523,22 +527,19 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
     __syncthreads();
     for (int q = threadIdx.x; q < total; q += blockDim.x) {
         rowval[base + q] = (IDX)srow[q] + BASE;
-        nzval[base + q] = sval[q];
+        if (VALUES) nzval[base + q] = sval[q];
         if (map) map[base + q] = smap[q];
     }
 }
 
-// nzval refilled through the stored slot indices when the pattern is
-// unchanged (chg[0] == 0): one coalesced write, one gather per entry; a
-// grid-stride loop over a grid of at most ASM_VALUES_BLOCKS workgroups, so
-// that the launch costs little when the pattern changed and it returns
-constexpr int ASM_VALUES_BLOCKS = 1024;
-__global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ chg,
-                                                    const int32_t* __restrict__ colptr_end,
+// nzval through the stored slot indices (after every fill, and alone while
+// the pattern is unchanged): one coalesced write, one gather per entry, in a
+// grid-stride loop over at most ASM_VALUES_BLOCKS workgroups
+constexpr int ASM_VALUES_BLOCKS = 2048;
+__global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ colptr_end,
                                                     const int32_t* __restrict__ map,
                                                     const double* __restrict__ slotv,
                                                     double* __restrict__ nzval) {
-    if (chg[0] != 0) return;
     const int64_t nnz = colptr_end[0];
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
          q += (int64_t)gridDim.x * blockDim.x)
@@ -734,7 +735,7 @@ static int assemble(ereb_ctx* c, double dt) {
                            s.nzval, chg, s.map);
     hipLaunchKernelGGL(k_asm_values,
                        dim3((unsigned)std::min<int64_t>((cap + 255) / 256, ASM_VALUES_BLOCKS)),
-                       dim3(256), 0, c->stream, chg, (const int32_t*)s.colptr + n,
+                       dim3(256), 0, c->stream, (const int32_t*)s.colptr + n,
                        (const int32_t*)s.map, (const double*)s.slotv, s.nzval);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
@@ -829,7 +830,7 @@ int launch_thermal_noalloc(ereb_ctx* c, char* scratch, const double* const in[8]
     hipLaunchKernelGGL(k_asm_count, grid, dim3(256), 0, c->stream, c->Nx1, c->Ny1,
                        (const SlotTables*)t.tables, (const uint16_t*)t.pres, t.colcnt,
                        (const int32_t*)nullptr);
-    hipLaunchKernelGGL((k_asm_fill<16, int64_t, 1>), grid, dim3(256), 0, c->stream, c->Nx1,
+    hipLaunchKernelGGL((k_asm_fill<16, int64_t, 1, true>), grid, dim3(256), 0, c->stream, c->Nx1,
                        c->Ny1, (const SlotTables*)t.tables, (const uint16_t*)t.pres,
                        (const double*)t.slotv, (const int32_t*)t.colcnt, t.colptr, rowval,
                        nzval, (const int32_t*)nullptr, (int32_t*)nullptr);

@@ -282,10 +282,12 @@ int launch_scatter_adj(ereb_ctx* c, const double* hrs, const double* hrf, int mo
 int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off);
 // the grid cotangent in fixed point (ereb_fixsum.h): pass 0 records the
 // largest contribution, pass 1 adds (and writes the marker cotangents)
+// order (optional): the markers are walked as order[0], order[1], ... (a
+// sort by cell, so a workgroup's stencils cover few nodes)
 int launch_gather_T_adj(ereb_ctx* c, int pass, int64_t timestep, double* gT,
-                        const FixGrid& ggrid, int* flag);
+                        const FixGrid& ggrid, int* flag, const uint32_t* order = nullptr);
 int launch_rk4_adj(ereb_ctx* c, int pass, double dt, int mode, const double* xb,
                    const double* yb, const double* gout, double* gin, double* gphi,
-                   const FixGrid& gtk2, int* flag);
+                   const FixGrid& gtk2, int* flag, const uint32_t* order = nullptr);
 
 }  // namespace ereb

@@ -472,29 +472,42 @@ struct FixWin {
     int box[4];  // imin, imax, jmin, jmax (1-based cells)
 };
 
-// Workgroup-wide F[stencil of (i, j)] += w * v (bilinear, `rows` rows) in
-// fixed point (ereb_fixsum.h), so the sums do not depend on the order of
-// the atomics.  PASS 0 records the largest |w v| (one atomic per wavefront).
-// PASS 1 sums the contributions' words in an LDS window over the
-// workgroup's cells (integer LDS atomics), then adds each touched node's
-// words to F with one atomic pair; when the cells span more than the
-// window, every lane adds straight to F.  Every thread of the workgroup
-// must call it (PASS 1 synchronises); `on` marks the lanes that contribute.
-template <int PASS>
-__device__ void block_fix_add4(const FixGrid& F, int rows, bool on, int i, int j,
-                               const double* w, double v, FixWin& W, int* flag) {
-    on = on && v != 0.0;
+// Workgroup-wide F[stencil s of (i_s, j_s)] += w_s * v_s for NS stencils
+// per lane (bilinear, `rows` rows) in fixed point (ereb_fixsum.h), so the
+// sums do not depend on the order of the atomics.  PASS 0 records the
+// largest |w v| (one atomic per wavefront).  PASS 1 sums the contributions'
+// words in an LDS window over the workgroup's cells (integer LDS atomics),
+// then adds each touched node's words to F with one atomic pair (device-wide
+// atomics are carried out at the memory side on this chip, so their number
+// sets the cost: one window for all of a lane's stencils); when the cells
+// span more than the window, every lane adds straight to F.  Every thread of
+// the workgroup must call it (PASS 1 synchronises); `on` marks the lanes
+// that contribute.
+template <int PASS, int NS>
+__device__ void block_fix_add(const FixGrid& F, int rows, bool on, const int (&is)[NS],
+                              const int (&js)[NS], const double (&ws)[NS][4],
+                              const double (&vs)[NS], FixWin& W, int* flag) {
     if (PASS == 0) {
         double mx = 0.0;
         if (on)
-            for (int c = 0; c < 4; ++c) mx = fmax(mx, fabs(w[c] * v));
+            for (int q = 0; q < NS; ++q)
+                for (int c = 0; c < 4; ++c) mx = fmax(mx, fabs(ws[q][c] * vs[q]));
         for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
         if (__lane_id() == 0 && mx > 0.0)
             atomicMax(F.maxbits, (unsigned long long)__double_as_longlong(mx));
         return;
     }
-    int imin = on ? i : INT32_MAX, imax = on ? i : INT32_MIN;
-    int jmin = on ? j : INT32_MAX, jmax = on ? j : INT32_MIN;
+    bool any = false;
+    int imin = INT32_MAX, imax = INT32_MIN, jmin = INT32_MAX, jmax = INT32_MIN;
+    for (int q = 0; q < NS; ++q) {
+        if (on && vs[q] != 0.0) {
+            any = true;
+            imin = min(imin, is[q]);
+            imax = max(imax, is[q]);
+            jmin = min(jmin, js[q]);
+            jmax = max(jmax, js[q]);
+        }
+    }
     for (int o = 32; o > 0; o >>= 1) {
         imin = min(imin, __shfl_xor(imin, o));
         imax = max(imax, __shfl_xor(imax, o));
@@ -520,28 +533,34 @@ __device__ void block_fix_add4(const FixGrid& F, int rows, bool on, int i, int j
     if (i0 > i1) return;  // uniform: no contribution in the workgroup
     const int h = i1 - i0 + 2, wdt = j1 - j0 + 2;  // nodes i0..i1+1, j0..j1+1
     if (2 * (int64_t)h * wdt > ADJ_WIN) {
-        if (on) {
-            const int64_t k = (int64_t)(j - 1) * rows + (i - 1);
-            fix_add<1>(F, k, w[0] * v, flag);
-            fix_add<1>(F, k + 1, w[1] * v, flag);
-            fix_add<1>(F, k + rows, w[2] * v, flag);
-            fix_add<1>(F, k + rows + 1, w[3] * v, flag);
+        if (any) {
+            for (int q = 0; q < NS; ++q) {
+                if (vs[q] == 0.0) continue;
+                const int64_t k = (int64_t)(js[q] - 1) * rows + (is[q] - 1);
+                fix_add<1>(F, k, ws[q][0] * vs[q], flag);
+                fix_add<1>(F, k + 1, ws[q][1] * vs[q], flag);
+                fix_add<1>(F, k + rows, ws[q][2] * vs[q], flag);
+                fix_add<1>(F, k + rows + 1, ws[q][3] * vs[q], flag);
+            }
         }
         return;
     }
     for (int q = threadIdx.x; q < 2 * h * wdt; q += blockDim.x) W.v[q] = 0ull;
     __syncthreads();
-    if (on) {
-        const int q0 = (j - j0) * h + (i - i0);
-        const int qs[4] = {q0, q0 + 1, q0 + h, q0 + h + 1};
-        for (int c = 0; c < 4; ++c) {
-            unsigned long long H, L;
-            if (!fix_split(w[c] * v, F.up, H, L)) {
-                atomicOr(flag, 1);
-                continue;
+    if (any) {
+        for (int q = 0; q < NS; ++q) {
+            if (vs[q] == 0.0) continue;
+            const int q0 = (js[q] - j0) * h + (is[q] - i0);
+            const int qs[4] = {q0, q0 + 1, q0 + h, q0 + h + 1};
+            for (int c = 0; c < 4; ++c) {
+                unsigned long long H, L;
+                if (!fix_split(ws[q][c] * vs[q], F.up, H, L)) {
+                    atomicOr(flag, 1);
+                    continue;
+                }
+                atomicAdd(&W.v[2 * qs[c]], H);
+                atomicAdd(&W.v[2 * qs[c] + 1], L);
             }
-            atomicAdd(&W.v[2 * qs[c]], H);
-            atomicAdd(&W.v[2 * qs[c] + 1], L);
         }
     }
     __syncthreads();
@@ -567,12 +586,13 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
                                                  const double* __restrict__ gout,
                                                  double* __restrict__ gin,
                                                  double* __restrict__ gphi, FixGrid gtk2,
-                                                 int* flag) {
+                                                 int* flag, const uint32_t* __restrict__ order) {
     __shared__ FixWin W;
     gtk2.up = fix_up(gtk2);
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < A.n;
-    const int64_t m = on ? m0 : A.n - 1;  // idle lanes read a valid marker, add nothing
+    // idle lanes read a valid marker and add nothing
+    const int64_t m = on ? (order ? (int64_t)order[m0] : m0) : A.n - 1;
     const GeomK& g = A.g;
     const double x0 = A.xm[m], y0 = A.ym[m];
     int ips, jps, ipb, jpb;
@@ -598,8 +618,11 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
         gphi[m] += go * F_p;
     }
     const double gd = go * F_d;
-    block_fix_add4<PASS>(gtk2, g.Ny1, on, ipb, jpb, wb, gd, W, flag);
-    block_fix_add4<PASS>(gtk2, g.Ny1, on, ips, jps, ws, -gd, W, flag);
+    // both stencils (back-trace end: +gd, start: -gd) in one window
+    const int is2[2] = {ipb, ips}, js2[2] = {jpb, jps};
+    const double ws2[2][4] = {{wb[0], wb[1], wb[2], wb[3]}, {ws[0], ws[1], ws[2], ws[3]}};
+    const double vs2[2] = {gd, -gd};
+    block_fix_add<PASS, 2>(gtk2, g.Ny1, on, is2, js2, ws2, vs2, W, flag);
 }
 
 #pragma clang fp contract(off)
@@ -611,31 +634,35 @@ template <int PASS>
 __global__ __launch_bounds__(256) void k_gather_T_adj(GeomK g, const double* __restrict__ xm,
                                                       const double* __restrict__ ym,
                                                       double* __restrict__ gT, FixGrid ggrid,
-                                                      int* flag, int64_t M, int first) {
+                                                      int* flag, int64_t M, int first,
+                                                      const uint32_t* __restrict__ order) {
     __shared__ FixWin W;
     ggrid.up = fix_up(ggrid);
     const int64_t m0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool on = m0 < M;
-    const int64_t m = on ? m0 : M - 1;
+    const int64_t m = on ? (order ? (int64_t)order[m0] : m0) : M - 1;
     const double x = xm[m], y = ym[m];
     const int jp = jl_cell((x - g.xp[0]) * g.idx, 1, g.Nx);
     const int ip = jl_cell((y - g.yp[0]) * g.idy, 1, g.Ny);
     double w[4];
     weights4((x - g.xp[jp - 1]) / g.dx, (y - g.yp[ip - 1]) / g.dy, w);
     const double v = gT[m];
-    block_fix_add4<PASS>(ggrid, g.Ny1, on, ip, jp, w, v, W, flag);
+    const int is1[1] = {ip}, js1[1] = {jp};
+    const double ws1[1][4] = {{w[0], w[1], w[2], w[3]}};
+    const double vs1[1] = {v};
+    block_fix_add<PASS, 1>(ggrid, g.Ny1, on, is1, js1, ws1, vs1, W, flag);
     if (PASS == 1 && on && first) gT[m] = 0.0;
 }
 
 int launch_gather_T_adj(ereb_ctx* c, int pass, int64_t timestep, double* gT, const FixGrid& F,
-                        int* flag) {
+                        int* flag, const uint32_t* order) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     auto kern = pass == 0 ? k_gather_T_adj<0> : k_gather_T_adj<1>;
     hipLaunchKernelGGL(kern, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream,
                        geomk(c), (const double*)mptr(c, EREB_M_XM),
                        (const double*)mptr(c, EREB_M_YM), gT, F, flag, M,
-                       timestep == 1 ? 1 : 0);
+                       timestep == 1 ? 1 : 0, order);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -644,7 +671,7 @@ static RK4Args rk4_args(ereb_ctx* c, int mode);
 
 int launch_rk4_adj(ereb_ctx* c, int pass, double dt, int mode, const double* xb,
                    const double* yb, const double* gout, double* gin, double* gphi,
-                   const FixGrid& gtk2, int* flag) {
+                   const FixGrid& gtk2, int* flag, const uint32_t* order) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     int rc;
@@ -652,7 +679,7 @@ int launch_rk4_adj(ereb_ctx* c, int pass, double dt, int mode, const double* xb,
     RK4Args A = rk4_args(c, mode);
     auto kern = pass == 0 ? k_rk4_adj<0> : k_rk4_adj<1>;
     hipLaunchKernelGGL(kern, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A, xb,
-                       yb, gout, gin, gphi, gtk2, flag);
+                       yb, gout, gin, gphi, gtk2, flag, order);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
