@@ -114,8 +114,7 @@ __device__ __forceinline__ void start_stencil_b(const GeomK& g, double x, double
 }
 
 // dot4 of a grid with a stencil (E:1137-1159), products and sums rounded one
-// by one (defined outside the fused-FP region: every kernel that calls it
-// gets the same bits)
+// by one, so every kernel that calls it gets the same bits
 __device__ __forceinline__ double stencil_dot(const double* __restrict__ gr, int rows,
                                               const Stencil4<double>& st) {
     const uint32_t oR = st.off + ((uint32_t)rows << 3);
@@ -238,10 +237,6 @@ __global__ __launch_bounds__(256) void k_rk4_coef(RKCoefArgs A) {
     *(double2*)(out + (size_t)k * RKC + 2 * q) = make_double2(lo, hi);
 }
 
-// RK4 region: a*b + c may be fused (the quotients above already make the
-// results agree to rounding rather than bit for bit)
-#pragma clang fp contract(fast)
-
 // Cell of an RK4 stage position: the clamp to [lo - 1, hi - 1] before the
 // conversion equals fix's truncate-then-clamp (E:1457-1468) for every t
 // below 2^63 in magnitude, NaN included (it maps to lo in both); stage
@@ -260,21 +255,31 @@ __device__ __forceinline__ double stage_frac(const double* __restrict__ ax, doub
     else return (x - ax[k - 1]) * ih;
 }
 
-// RK4 stage velocity on the Vx grid (E:5057-5091) from the cell's quadratic
-// coefficients (k_rk4_coef): cell (i, j) with the clamps 1..Ny, 1..Nx-1,
-// fractions fx, fy, the half chosen by fx >= 0.5, then
-// P(fx) + fy Q(fx).  The Vy grid (E:5093-5127) is the transpose.
+// P(u) + w Q(u) of one half's coefficients [P0 P1 P2 Q0 Q1 Q2] in Horner form.
+// The RK4 code fuses multiply-adds only where it says so (__builtin_fma; the
+// library is built without contraction), so its rounding does not depend on
+// the kernel it is inlined into.
+__device__ __forceinline__ double rk_eval(double2 c01, double2 c23, double2 c45, double u,
+                                          double w) {
+    const double P = __builtin_fma(u, __builtin_fma(u, c23.x, c01.y), c01.x);
+    const double Q = __builtin_fma(u, __builtin_fma(u, c45.y, c45.x), c23.y);
+    return __builtin_fma(w, Q, P);
+}
+
+// The stage velocity of the half `second` of the cell whose coefficients start
+// at node index `node` of table T
 __device__ __forceinline__ double rk_poly(const double* __restrict__ T, uint32_t node,
                                           bool second, double u, double w) {
     const uint32_t off = node * (uint32_t)(RKC * 8) + (second ? 48u : 0u);  // < 2^31 bytes
     const char* p = (const char*)T + off;
-    const double2 c01 = *(const double2*)p, c23 = *(const double2*)(p + 16),
-                  c45 = *(const double2*)(p + 32);
-    const double P = c01.x + u * (c01.y + u * c23.x);
-    const double Q = c23.y + u * (c45.x + u * c45.y);
-    return P + w * Q;
+    return rk_eval(*(const double2*)p, *(const double2*)(p + 16), *(const double2*)(p + 32), u,
+                   w);
 }
 
+// RK4 stage velocity on the Vx grid (E:5057-5091) from the cell's quadratic
+// coefficients (k_rk4_coef): cell (i, j) with the clamps 1..Ny, 1..Nx-1,
+// fractions fx, fy, the half chosen by fx >= 0.5, then
+// P(fx) + fy Q(fx).  The Vy grid (E:5093-5127) is the transpose.
 template <bool EXACT>
 __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ T, double x,
                                         double y) {
@@ -316,6 +321,12 @@ struct RK4Args {
     const double *DSXX, *DSXY, *DTg;
 };
 
+// b0 v0 + b1 v1 + b2 v2 + b3 v3 with the RK4 weights b (dot4, E:5135-5136)
+__device__ __forceinline__ double rk_comb(const double* b, const double* v) {
+    const double s = __builtin_fma(b[1], v[1], b[0] * v[0]);
+    return __builtin_fma(b[3], v[3], __builtin_fma(b[2], v[2], s));
+}
+
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
 // to (xn, yn) and the fluid back-trace (E:5138-5226) from (xn, yn) to
 // (xb, yb).
@@ -332,12 +343,12 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
         vxr[rk] = rk_vx<EXACT>(g, A.CXs, xr, yr);
         vyr[rk] = rk_vy<EXACT>(g, A.CYs, xr, yr);
         if (rk < 3) {
-            xr = xmm + dt * cr[rk] * vxr[rk];
-            yr = ymm + dt * cr[rk] * vyr[rk];
+            xr = __builtin_fma(dt * cr[rk], vxr[rk], xmm);
+            yr = __builtin_fma(dt * cr[rk], vyr[rk], ymm);
         }
     }
-    xn = xmm + dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
-    yn = ymm + dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
+    xn = __builtin_fma(dt, rk_comb(b, vxr), xmm);
+    yn = __builtin_fma(dt, rk_comb(b, vyr), ymm);
     xmm = xr = xn;
     ymm = yr = yn;
 #pragma unroll
@@ -345,12 +356,12 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
         vxr[rk] = rk_vx<EXACT>(g, A.CXf, xr, yr);
         vyr[rk] = rk_vy<EXACT>(g, A.CYf, xr, yr);
         if (rk < 3) {
-            xr = xmm - dt * cr[rk] * vxr[rk];
-            yr = ymm - dt * cr[rk] * vyr[rk];
+            xr = __builtin_fma(-dt * cr[rk], vxr[rk], xmm);
+            yr = __builtin_fma(-dt * cr[rk], vyr[rk], ymm);
         }
     }
-    xb = xmm - dt * (b[0] * vxr[0] + b[1] * vxr[1] + b[2] * vxr[2] + b[3] * vxr[3]);
-    yb = ymm - dt * (b[0] * vyr[0] + b[1] * vyr[1] + b[2] * vyr[2] + b[3] * vyr[3]);
+    xb = __builtin_fma(-dt, rk_comb(b, vxr), xmm);
+    yb = __builtin_fma(-dt, rk_comb(b, vyr), ymm);
 }
 
 // bilinear P-grid stencil of a position: cell (ip, jp), weights w
@@ -368,19 +379,10 @@ __device__ __forceinline__ void p_stencil(const GeomK& g, double x, double y, in
 // stress gather's interpolation in FP32; the temperature gather is Float64 in
 // both forms): the stress gather, then the temperature gather, then RK4 on
 // the gathered values, as the three separate kernels would compute them
-// (the gathers' arithmetic is defined outside the fused-FP region), with
-// each marker's state read and written once.
+// (the same helpers, no contraction), with each marker's state read and
+// written once.
 template <int GATHER, bool EXACT>
-__global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
-    // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
-    // round robin, so workgroup b runs on XCD b mod 8; numbering them
-    // XCD-major gives each XCD one contiguous run of markers (one band of
-    // the domain), whose coefficient-table lines then stay in that XCD's L2
-    // instead of being fetched by all eight
-    const uint32_t nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
-    const uint32_t bb = x * q + min(x, r) + (b >> 3);
-    const int64_t m = (int64_t)bb * blockDim.x + threadIdx.x;
-    if (m >= A.n) return;
+__device__ __forceinline__ void rk4_marker(const RK4Args& A, int64_t m) {
     const GeomK& g = A.g;
     const double dt = A.S->dt;
     const bool first = GATHER && A.S->first != 0;
@@ -461,6 +463,22 @@ __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
     const double rcs = sel3(A.M.rhocpsolidm, tt);
     const double rcf = rhocpfluidm(A.M, T, A.mode);
     A.tkm[m] = ((1.0 - ph) * T * rcs + ph * (T + d) * rcf) / ((1.0 - ph) * rcs + ph * rcf);
+}
+
+// One thread per marker in storage order (coalesced marker reads and writes),
+// the stage tables read from global memory.
+template <int GATHER, bool EXACT>
+__global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
+    // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
+    // round robin, so workgroup b runs on XCD b mod 8; numbering them
+    // XCD-major gives each XCD one contiguous run of markers (one band of
+    // the domain), whose coefficient-table lines then stay in that XCD's L2
+    // instead of being fetched by all eight
+    const uint32_t nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
+    const uint32_t bb = x * q + min(x, r) + (b >> 3);
+    const int64_t m = (int64_t)bb * blockDim.x + threadIdx.x;
+    if (m >= A.n) return;
+    rk4_marker<GATHER, EXACT>(A, m);
 }
 
 // ---- adjoints (with respect to tkm, phim and the tk2 grid) ---------------
@@ -625,8 +643,6 @@ __global__ __launch_bounds__(256) void k_rk4_adj(RK4Args A, const double* __rest
     block_fix_add<PASS, 2>(gtk2, g.Ny1, on, is2, js2, ws2, vs2, W, flag);
 }
 
-#pragma clang fp contract(off)
-
 // gather^T of update_marker_temperature! (E:4782-4801): the marker cotangent
 // goes to the interpolated grid (tk2 at timestep 1, DT later); at timestep 1
 // the marker value was overwritten, so its cotangent becomes zero.
@@ -757,6 +773,15 @@ static void launch_rk4_prep(ereb_ctx* c, const RK4Args& A) {
                        dim3(256), 0, c->stream, P);
 }
 
+template <int GATHER>
+static void launch_rk4_kernel(ereb_ctx* c, const RK4Args& A) {
+    const dim3 grid((unsigned)((c->marknum + 255) / 256));
+    if (c->axes_exact)
+        hipLaunchKernelGGL((k_rk4<GATHER, true>), grid, dim3(256), 0, c->stream, A);
+    else
+        hipLaunchKernelGGL((k_rk4<GATHER, false>), grid, dim3(256), 0, c->stream, A);
+}
+
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
@@ -765,15 +790,11 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     RK4Args A = rk4_args(c, mode);
     A.xb_out = xb_out;
     A.yb_out = yb_out;
-    ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     launch_rk4_prep(c, A);
-    const dim3 grid((unsigned)((M + 255) / 256));
-    if (c->axes_exact)
-        hipLaunchKernelGGL((k_rk4<0, true>), grid, dim3(256), 0, c->stream, A);
-    else
-        hipLaunchKernelGGL((k_rk4<0, false>), grid, dim3(256), 0, c->stream, A);
+    launch_rk4_kernel<0>(c, A);
+    ++c->pos_gen;
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;
@@ -790,23 +811,13 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
     A.DSXX = gptr(c, EREB_G_DSXX);
     A.DSXY = gptr(c, EREB_G_DSXY);
     A.DTg = gptr(c, EREB_G_DT);
-    ++c->pos_gen;
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
     launch_rk4_prep(c, A);
-    const dim3 grid((unsigned)((M + 255) / 256));
     // node coordinates computed (exact axes) or loaded, chosen per kernel
-    if (c->precision == EREB_MIXED) {
-        if (c->axes_exact)
-            hipLaunchKernelGGL((k_rk4<2, true>), grid, dim3(256), 0, c->stream, A);
-        else
-            hipLaunchKernelGGL((k_rk4<2, false>), grid, dim3(256), 0, c->stream, A);
-    } else {
-        if (c->axes_exact)
-            hipLaunchKernelGGL((k_rk4<1, true>), grid, dim3(256), 0, c->stream, A);
-        else
-            hipLaunchKernelGGL((k_rk4<1, false>), grid, dim3(256), 0, c->stream, A);
-    }
+    if (c->precision == EREB_MIXED) launch_rk4_kernel<2>(c, A);
+    else launch_rk4_kernel<1>(c, A);
+    ++c->pos_gen;
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;
