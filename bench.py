@@ -493,20 +493,22 @@ def large_leg(steps: int = 5, warmup: int = 2):
     return out
 
 
-# relative central-difference steps of the adjoint checks (adjoint_leg)
-FD_STEP = {"tkm": 1e-6, "phim": 1e-4}
+# relative steps of the fourth-order central differences of the adjoint
+# checks (adjoint_leg)
+FD_STEP = {"tkm": 1e-5, "phim": 1e-3}
 
 
 def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str = "adj"):
     """Gradient of L = sum(tk1^2) after nsteps steps with respect to the
     initial marker temperature and porosity: time of the forward map, time
     of the checkpointed adjoint (forward with checkpoints + reverse sweep),
-    and central finite differences on nfd random planet markers 2 K or more
-    from the temperature breakpoints of the property laws.  Steps: 1e-6
-    relative for tkm; 1e-4 for phim, whose effect on tk1 at 1e-6 (~1e-8 K)
-    is a few hundred thousand ulps of tk1, a quotient noise of ~1e-5; at 1e-4
-    the noise is ~1e-7 and the truncation error (tk1 is a ratio of sums
-    linear in phim) ~1e-8."""
+    and fourth-order central differences, (8 (L(x+h) - L(x-h)) - (L(x+2h) -
+    L(x-2h))) / 12h, on nfd random planet markers 2 K or more from the
+    temperature breakpoints of the property laws.  Relative steps 1e-5 for
+    tkm (2h = 6e-3 K) and 1e-3 for phim: one marker moves tk1 through node
+    sums of ~100 markers, so the quotient's rounding noise (~1.5 u / h, u
+    the relative rounding of a loss difference) is ~1e-7 at these steps,
+    and the truncation error (O(h^4)) below 1e-12."""
     g, mat = synth.workload_geometry(workload, mode=1)
     mode = mat.marker_property_mode
     mk = synth.make_markers(g, mat, seed=0)
@@ -538,14 +540,15 @@ def adjoint_leg(nsteps: int = 20, every: int = 5, nfd: int = 64, workload: str =
             fd = np.empty(nfd)
             for q, m in enumerate(idx):
                 h = FD_STEP[name] * abs(mk[name][m])
-                tk = []
-                for sgn in (1.0, -1.0):
+                tk = {}
+                for k in (1, -1, 2, -2):
                     a = mk[name].copy()
-                    a[m] += sgn * h
+                    a[m] += k * h
                     ctx.set_marker(name, a)
-                    tk.append(ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf,
-                                                   grad=False, want_tk1=True)[3])
-                fd[q] = float(np.sum((tk[0] - tk[1]) * (tk[0] + tk[1]))) / (2 * h)
+                    tk[k] = ctx.adjoint_tk1_loss(nsteps, every, dt, mode, hrs, hrf,
+                                                 grad=False, want_tk1=True)[3]
+                dL = lambda u, v: float(np.sum((tk[u] - tk[v]) * (tk[u] + tk[v])))  # noqa: E731
+                fd[q] = (8.0 * dL(1, -1) - dL(2, -2)) / (12.0 * h)
             ctx.set_marker(name, mk[name])
             ad = grad[idx]
             fd_err[name] = {

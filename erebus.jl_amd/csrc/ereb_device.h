@@ -23,6 +23,17 @@ struct GeomK {
     const double* yp;         // P y axis (Ny1), = yvx
 };
 
+// XCD-contiguous workgroup number: the hardware deals the workgroups of a
+// launch to the 8 XCDs round robin (workgroup b runs on XCD b mod 8), each
+// XCD with its own L2; renumbered XCD-major, each XCD takes one contiguous
+// run of the grid, so neighbouring workgroups (neighbouring tiles, runs of
+// markers) share one L2 and the lines straddling their data are fetched
+// once.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // Node k (1-based) of an axis with spacing h.  EXACT (the context found
 // every node of every axis equal to the first node plus k*h rounded once,
 // as for integer or half-integer metres): computed in one fused operation,

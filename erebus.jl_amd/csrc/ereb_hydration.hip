@@ -273,7 +273,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_pscatter(PScatterBase B, Op op) 
     constexpr int S = Op::S;
     __shared__ double fl[NFINE * 4 * S];  // fine-cell partials [f][corner][s]
     __shared__ uint32_t sperm[PS_PERM];   // the tile's part of the sort order
-    const int tile = blockIdx.x, t = threadIdx.x;
+    const int tile = (int)xcd_block(), t = threadIdx.x;  // XCD-contiguous, as k_scatter
     const int j0 = (tile / B.nty) * TILE, i0 = (tile % B.nty) * TILE;
     // the tile's markers are one contiguous run of the sort order: staged in
     // LDS with coalesced loads when it fits (read from memory otherwise)

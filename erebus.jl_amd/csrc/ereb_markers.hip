@@ -196,45 +196,55 @@ __device__ __forceinline__ void rk_half(double* __restrict__ out, double a0, dou
     out[5] = r2 - p2;
 }
 
-// One thread per 16-byte piece of the tables: piece q (0..5) of cell (i, j)
-// of table t holds coefficients 2(q mod 3), 2(q mod 3) + 1 of half q / 3, so
-// consecutive threads write consecutive 16 bytes.  Grid: x over the 6 Ny1
-// pieces of a node column, y = column j - 1, z = table t.  Each thread
-// evaluates its half's six coefficients (the inputs are in L1/L2).
-__global__ __launch_bounds__(256) void k_rk4_coef(RKCoefArgs A) {
+// One thread per cell of one table (grid y = table t), cells in table order
+// (k = (j - 1) Ny1 + (i - 1)): the thread loads the cell's velocity
+// neighbourhood once and evaluates both halves; the workgroup's 256 cells
+// (24 KB of coefficients) are assembled in LDS and written with consecutive
+// 16-byte stores.
+constexpr int RKCOEF_CELLS = 256;
+
+__global__ __launch_bounds__(RKCOEF_CELLS) void k_rk4_coef(RKCoefArgs A) {
+    __shared__ double sh[RKCOEF_CELLS * RKC];
     const int R = A.Ny + 1;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 6 * R) return;
-    const int i = p / 6 + 1, q = p % 6, j = blockIdx.y + 1;
-    const int k = (j - 1) * R + (i - 1);
-    const int t = blockIdx.z;  // 0 Vx solid, 1 Vy solid, 2 Vx fluid, 3 Vy fluid
+    const int N = R * (A.Nx + 1);
+    const int b0 = blockIdx.x * RKCOEF_CELLS;
+    const int k = b0 + (int)threadIdx.x;
+    const int t = blockIdx.y;  // 0 Vx solid, 1 Vy solid, 2 Vx fluid, 3 Vy fluid
     const bool xg = (t & 1) == 0, fluid = t >= 2;
-    const bool second = q >= 3;
-    double c[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (xg && i <= A.Ny && j <= A.Nx - 1) {
-        // Vx cell (i, j): rows i, i+1 of columns j, j+1; centre column j
-        // (left half) or j+1 (right half)
-        const double* v = fluid ? A.vxf : A.vx;
+    double c[RKC];
+#pragma unroll
+    for (int q = 0; q < RKC; ++q) c[q] = 0.0;
+    if (k < N) {
+        const int i = k % R + 1, j = k / R + 1;
         const int k1 = k + 1, kR = k + R, kR1 = k + R + 1;
-        const int c0 = second ? kR : k, c1 = second ? kR1 : k1, cc = second ? j + 1 : j;
-        // the fluid loop's first-row left-half centre is read from vx (E:5173)
-        const double* ctr0 = (fluid && !second) ? A.vx : v;
-        rk_half(c, v[k], v[kR], d2(v, ctr0, c0, R, cc, A.Nx), v[k1], v[kR1],
-                d2(v, v, c1, R, cc, A.Nx));
-    } else if (!xg && i <= A.Ny - 1 && j <= A.Nx) {
-        // Vy cell (i, j): rows i, i+1 of columns j, j+1; centre row i (upper
-        // half) or i+1 (lower half)
-        const double* v = fluid ? A.vyf : A.vy;
-        const int k1 = k + 1, kR = k + R, kR1 = k + R + 1;
-        const int c0 = second ? k1 : k, c1 = second ? kR1 : kR, cc = second ? i + 1 : i;
-        rk_half(c, v[k], v[k1], d2(v, v, c0, 1, cc, A.Ny), v[kR], v[kR1],
-                d2(v, v, c1, 1, cc, A.Ny));
+        if (xg && i <= A.Ny && j <= A.Nx - 1) {
+            // Vx cell (i, j): rows i, i+1 of columns j, j+1; centre column j
+            // (left half) or j+1 (right half); the fluid loop's first-row
+            // left-half centre is read from vx (E:5173)
+            const double* v = fluid ? A.vxf : A.vx;
+            const double a0 = v[k], b0v = v[kR], a1 = v[k1], b1 = v[kR1];
+            rk_half(c, a0, b0v, d2(v, fluid ? A.vx : v, k, R, j, A.Nx), a1, b1,
+                    d2(v, v, k1, R, j, A.Nx));
+            rk_half(c + 6, a0, b0v, d2(v, v, kR, R, j + 1, A.Nx), a1, b1,
+                    d2(v, v, kR1, R, j + 1, A.Nx));
+        } else if (!xg && i <= A.Ny - 1 && j <= A.Nx) {
+            // Vy cell (i, j): rows i, i+1 of columns j, j+1; centre row i
+            // (upper half) or i+1 (lower half)
+            const double* v = fluid ? A.vyf : A.vy;
+            const double a0 = v[k], b0v = v[k1], a1 = v[kR], b1 = v[kR1];
+            rk_half(c, a0, b0v, d2(v, v, k, 1, i, A.Ny), a1, b1, d2(v, v, kR, 1, i, A.Ny));
+            rk_half(c + 6, a0, b0v, d2(v, v, k1, 1, i + 1, A.Ny), a1, b1,
+                    d2(v, v, kR1, 1, i + 1, A.Ny));
+        }
     }
-    const int r = q - (second ? 3 : 0);
-    const double lo = r == 0 ? c[0] : (r == 1 ? c[2] : c[4]);
-    const double hi = r == 0 ? c[1] : (r == 1 ? c[3] : c[5]);
+#pragma unroll
+    for (int q = 0; q < RKC; q += 2)
+        *(double2*)(sh + threadIdx.x * RKC + q) = make_double2(c[q], c[q + 1]);
+    __syncthreads();
     double* out = t == 0 ? A.cxs : (t == 1 ? A.cys : (t == 2 ? A.cxf : A.cyf));
-    *(double2*)(out + (size_t)k * RKC + 2 * q) = make_double2(lo, hi);
+    const int npiece = min(RKCOEF_CELLS, N - b0) * (RKC / 2);
+    for (int p = threadIdx.x; p < npiece; p += RKCOEF_CELLS)
+        *(double2*)(out + (size_t)b0 * RKC + 2 * p) = *(const double2*)(sh + 2 * p);
 }
 
 // Cell of an RK4 stage position: the clamp to [lo - 1, hi - 1] before the
@@ -469,14 +479,10 @@ __device__ __forceinline__ void rk4_marker(const RK4Args& A, int64_t m) {
 // the stage tables read from global memory.
 template <int GATHER, bool EXACT>
 __global__ __launch_bounds__(256) void k_rk4(RK4Args A) {
-    // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
-    // round robin, so workgroup b runs on XCD b mod 8; numbering them
-    // XCD-major gives each XCD one contiguous run of markers (one band of
-    // the domain), whose coefficient-table lines then stay in that XCD's L2
-    // instead of being fetched by all eight
-    const uint32_t nb = gridDim.x, b = blockIdx.x, q = nb >> 3, r = nb & 7, x = b & 7;
-    const uint32_t bb = x * q + min(x, r) + (b >> 3);
-    const int64_t m = (int64_t)bb * blockDim.x + threadIdx.x;
+    // XCD-contiguous block order: each XCD moves one band of the domain, whose
+    // coefficient-table lines then stay in that XCD's L2 instead of being
+    // fetched by all eight
+    const int64_t m = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (m >= A.n) return;
     rk4_marker<GATHER, EXACT>(A, m);
 }
@@ -769,8 +775,9 @@ static RK4Args rk4_args(ereb_ctx* c, int mode) {
 static void launch_rk4_prep(ereb_ctx* c, const RK4Args& A) {
     RKCoefArgs P{c->Nx, c->Ny, A.vx, A.vy, A.vxf, A.vyf, (double*)A.CXs, (double*)A.CYs,
                  (double*)A.CXf, (double*)A.CYf};
-    hipLaunchKernelGGL(k_rk4_coef, dim3((unsigned)((6 * c->Ny1 + 255) / 256), c->Nx1, 4),
-                       dim3(256), 0, c->stream, P);
+    const int64_t N = (int64_t)c->Nx1 * c->Ny1;
+    hipLaunchKernelGGL(k_rk4_coef, dim3((unsigned)((N + RKCOEF_CELLS - 1) / RKCOEF_CELLS), 4),
+                       dim3(RKCOEF_CELLS), 0, c->stream, P);
 }
 
 template <int GATHER>

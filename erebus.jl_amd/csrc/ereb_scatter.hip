@@ -361,7 +361,9 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
     constexpr int NW = ScatterShape<R>::NW, NTHR = NW * 64;
     constexpr int NSET = NW == 8 ? 1 : 2;  // input halves per thread
     __shared__ ScatterLDS<R> L;
-    const int tile = blockIdx.x;
+    // XCD-contiguous: vertically neighbouring tiles (consecutive numbers)
+    // read adjacent runs of the same marker columns
+    const int tile = (int)xcd_block();
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int part = NW == 8 ? wave >> 2 : 0, ts = t % CH;  // first input half, marker slot
     const int r = lane & 7, c = lane >> 3;

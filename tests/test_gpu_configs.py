@@ -4,10 +4,11 @@
     radius 40 and 65 km: five transfer steps of the mixed-precision form
     against the Float64 form, every SUM and node field within 1e-6 normwise
     (max|mixed - fp64| / max|fp64|); one checkpointed adjoint step with its
-    gradient checked by central differences of the GPU forward map on four
-    planet markers per input, each within 1e-6 of the difference quotient;
+    gradient checked by fourth-order central differences of the GPU forward
+    map on four planet markers per input, each within 1e-6 of the difference
+    quotient;
   - 1,000,000 markers on 201x201 basic nodes (5x5 per cell), 20 steps with
-    checkpoints every 5: central differences on 64 planet markers per input,
+    checkpoints every 5: the same differences on 64 planet markers per input,
     each within 1e-6 relative of its own difference quotient (a per-marker
     bar).  The markers are picked with their temperature after the first
     step's gather at least 2 K away from the kinks of the property laws, so
@@ -55,27 +56,32 @@ def _pick(ctx, g, mk, hrs, hrf, n, rng, margin):
     return rng.choice(ok, n, replace=False)
 
 
-# Relative central-difference steps.  phim moves tk1 through sums that are
-# linear in it: at a step of 1e-6 the change of tk1 (~1e-8 K) is a few
-# hundred thousand ulps of tk1, so the quotient carries a rounding noise of
-# ~1e-5 (measured 3e-6 to 5e-5 on single markers); at 1e-4 the noise is
-# ~1e-7 and the truncation error ~1e-8.
-FD_STEP = {"tkm": 1e-6, "phim": 1e-4}
+# Fourth-order central differences,
+#   (8 (L(x+h) - L(x-h)) - (L(x+2h) - L(x-2h))) / 12h,
+# truncation error O(h^4), with relative steps h.  The quotient's rounding
+# noise is ~1.5 u / h, u the relative rounding error of a loss difference:
+# one marker moves tk1 through node sums of ~100 markers, so at a relative
+# step of 1e-4 (second order) the change of tk1 is ~1e-8 K, ~1e5 ulps of tk1,
+# and single markers measured 1e-7 to 1.4e-6 of noise, depending on the
+# summation order of the node sums.  Steps of 1e-5 (tkm: 2h = 6e-3 K, far
+# inside the 2 K margin from the kinks) and 1e-3 (phim, which enters
+# smoothly) keep the noise near 1e-7 and the truncation below 1e-12.
+FD_STEP = {"tkm": 1e-5, "phim": 1e-3}
 
 
 def _fd_check(ctx, mk, idx, nsteps, every, hrs, hrf, gT, gP, bar):
     for name, grad in (("tkm", gT), ("phim", gP)):
         for m in idx:
             h = FD_STEP[name] * abs(mk[name][m])
-            tk = []
-            for sgn in (1.0, -1.0):
+            tk = {}
+            for k in (1, -1, 2, -2):
                 a = mk[name].copy()
-                a[m] += sgn * h
+                a[m] += k * h
                 ctx.set_marker(name, a)
-                tk.append(ctx.adjoint_tk1_loss(nsteps, every, DT, 1, hrs, hrf, grad=False,
-                                               want_tk1=True)[3])
+                tk[k] = ctx.adjoint_tk1_loss(nsteps, every, DT, 1, hrs, hrf, grad=False,
+                                             want_tk1=True)[3]
             ctx.set_marker(name, mk[name])
-            fd = _dL(tk[0], tk[1]) / (2 * h)
+            fd = (8.0 * _dL(tk[1], tk[-1]) - _dL(tk[2], tk[-2])) / (12.0 * h)
             assert abs(fd - grad[m]) <= bar * abs(fd), (name, int(m), fd, grad[m])
 
 
