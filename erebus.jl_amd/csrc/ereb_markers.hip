@@ -331,47 +331,46 @@ struct RK4Args {
     const double *DSXX, *DSXY, *DTg;
 };
 
-// b0 v0 + b1 v1 + b2 v2 + b3 v3 with the RK4 weights b (dot4, E:5135-5136)
-__device__ __forceinline__ double rk_comb(const double* b, const double* v) {
-    const double s = __builtin_fma(b[1], v[1], b[0] * v[0]);
-    return __builtin_fma(b[3], v[3], __builtin_fma(b[2], v[2], s));
-}
-
 // Marker paths of one RK4 step: the solid step (E:5055-5136) from (x0, y0)
 // to (xn, yn) and the fluid back-trace (E:5138-5226) from (xn, yn) to
-// (xb, yb).
+// (xb, yb).  The weighted stage sum b0 v0 + b1 v1 + b2 v2 + b3 v3 (dot4,
+// E:5135-5136) is accumulated as the stages come, left to right with fmas,
+// so no stage velocity stays live.
 template <bool EXACT>
 __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0, double y0,
                                           double& xn, double& yn, double& xb, double& yb) {
     const GeomK& g = A.g;
     const double* b = A.M.brk4;
     const double* cr = A.M.crk4;
-    double xmm = x0, ymm = y0, xr = x0, yr = y0;
-    double vxr[4], vyr[4];
+    double xr = x0, yr = y0, sx = 0.0, sy = 0.0;
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = rk_vx<EXACT>(g, A.CXs, xr, yr);
-        vyr[rk] = rk_vy<EXACT>(g, A.CYs, xr, yr);
+        const double vx = rk_vx<EXACT>(g, A.CXs, xr, yr);
+        const double vy = rk_vy<EXACT>(g, A.CYs, xr, yr);
+        sx = rk == 0 ? b[0] * vx : __builtin_fma(b[rk], vx, sx);
+        sy = rk == 0 ? b[0] * vy : __builtin_fma(b[rk], vy, sy);
         if (rk < 3) {
-            xr = __builtin_fma(dt * cr[rk], vxr[rk], xmm);
-            yr = __builtin_fma(dt * cr[rk], vyr[rk], ymm);
+            xr = __builtin_fma(dt * cr[rk], vx, x0);
+            yr = __builtin_fma(dt * cr[rk], vy, y0);
         }
     }
-    xn = __builtin_fma(dt, rk_comb(b, vxr), xmm);
-    yn = __builtin_fma(dt, rk_comb(b, vyr), ymm);
-    xmm = xr = xn;
-    ymm = yr = yn;
+    xn = __builtin_fma(dt, sx, x0);
+    yn = __builtin_fma(dt, sy, y0);
+    xr = xn;
+    yr = yn;
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        vxr[rk] = rk_vx<EXACT>(g, A.CXf, xr, yr);
-        vyr[rk] = rk_vy<EXACT>(g, A.CYf, xr, yr);
+        const double vx = rk_vx<EXACT>(g, A.CXf, xr, yr);
+        const double vy = rk_vy<EXACT>(g, A.CYf, xr, yr);
+        sx = rk == 0 ? b[0] * vx : __builtin_fma(b[rk], vx, sx);
+        sy = rk == 0 ? b[0] * vy : __builtin_fma(b[rk], vy, sy);
         if (rk < 3) {
-            xr = __builtin_fma(-dt * cr[rk], vxr[rk], xmm);
-            yr = __builtin_fma(-dt * cr[rk], vyr[rk], ymm);
+            xr = __builtin_fma(-dt * cr[rk], vx, xn);
+            yr = __builtin_fma(-dt * cr[rk], vy, yn);
         }
     }
-    xb = __builtin_fma(-dt, rk_comb(b, vxr), xmm);
-    yb = __builtin_fma(-dt, rk_comb(b, vyr), ymm);
+    xb = __builtin_fma(-dt, sx, xn);
+    yb = __builtin_fma(-dt, sy, yn);
 }
 
 // bilinear P-grid stencil of a position: cell (ip, jp), weights w
