@@ -23,7 +23,8 @@
 //              before it plus a scan within the block: colptr), writes
 //              rowval of each column contiguously and, for every stored
 //              entry, the index of its slot value;
-//   4. values: nzval, one gather per entry through those indices.
+//   4. values: nzval, one gather per entry through those indices (and the
+//              pattern flag of step 1 cleared for the next assembly).
 // No atomics and no sort: rows come out ascending in every column.
 // Step 1 also compares the presence bits with the previous assembly's (a
 // device flag): when the pattern is unchanged
@@ -534,12 +535,16 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
 
 // nzval through the stored slot indices (after every fill, and alone while
 // the pattern is unchanged): one coalesced write, one gather per entry, in a
-// grid-stride loop over at most ASM_VALUES_BLOCKS workgroups
+// grid-stride loop over at most ASM_VALUES_BLOCKS workgroups.  chg
+// (optional): the pattern-change flag, read by the kernels before this one
+// and cleared here for the next assembly's rows kernel.
 constexpr int ASM_VALUES_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ colptr_end,
                                                     const int32_t* __restrict__ map,
                                                     const double* __restrict__ slotv,
-                                                    double* __restrict__ nzval) {
+                                                    double* __restrict__ nzval,
+                                                    int32_t* __restrict__ chg) {
+    if (chg && blockIdx.x == 0 && threadIdx.x == 0) chg[0] = 0;
     const int64_t nnz = colptr_end[0];
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
          q += (int64_t)gridDim.x * blockDim.x)
@@ -707,9 +712,11 @@ static int assemble(ereb_ctx* c, double dt) {
     const SlotTables* T = (const SlotTables*)s.tables;
     hipEvent_t ev;
     prof_begin(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, &ev);
-    // pattern unchanged since the previous assembly?  (with none, changed):
-    // the rows kernel compares the presence bits as it writes them
-    if ((rc = fill_i32(c, s.chg, s.have_prev ? 0 : 1, 1))) return rc;
+    // pattern unchanged since the previous assembly?  The rows kernel compares
+    // the presence bits as it writes them and raises the flag, which the
+    // values kernel clears at the end of every assembly; with no previous
+    // assembly (a fresh system) it is raised here.
+    if (!s.have_prev && (rc = fill_i32(c, s.chg, 1, 1))) return rc;
     const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)s.ne);
     hipLaunchKernelGGL(k_asm_rows<SYS>, grid, dim3(256), 0, c->stream, A, c->d_scal, T, s.pres,
                        s.slotv, c->d_flags, s.pres_prev, s.chg);
@@ -736,7 +743,7 @@ static int assemble(ereb_ctx* c, double dt) {
     hipLaunchKernelGGL(k_asm_values,
                        dim3((unsigned)std::min<int64_t>((cap + 255) / 256, ASM_VALUES_BLOCKS)),
                        dim3(256), 0, c->stream, (const int32_t*)s.colptr + n,
-                       (const int32_t*)s.map, (const double*)s.slotv, s.nzval);
+                       (const int32_t*)s.map, (const double*)s.slotv, s.nzval, s.chg);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, SYS == 0 ? EREB_K_ASM_SD : EREB_K_ASM_T, ev);
     s.valid = true;

@@ -377,7 +377,6 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     // the sort leaves the key counts at zero for the next sort
     CK(hipMemset(c->d_fcount, 0, sizeof(int32_t) * c->nkeys));
     CK(hipMalloc(&c->d_fstart, sizeof(int32_t) * (c->nkeys + 1)));
-    CK(hipMalloc(&c->d_fcursor, sizeof(int32_t) * c->nkeys));
     CK(hipMalloc(&c->d_big, sizeof(int32_t) * (c->nkeys + 1)));
     CK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
     CK(hipMemset(c->d_flags, 0, sizeof(int32_t) * 4));
@@ -413,7 +412,6 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_tmp);
     hipFree(c->d_fcount);
     hipFree(c->d_fstart);
-    hipFree(c->d_fcursor);
     hipFree(c->d_big);
     hipFree(c->d_flags);
     hipFree(c->d_part);

@@ -96,11 +96,10 @@ struct ereb_ctx {
     int64_t nkeys = 0;             // ntiles * NFINE
     uint32_t* d_key = nullptr;     // cap
     uint32_t* d_perm = nullptr;    // cap
-    uint32_t* d_tmp = nullptr;     // 2*cap scratch for oversize key segments
+    uint32_t* d_tmp = nullptr;     // 2*cap scratch: the sort's ranks, then oversize key segments
     int64_t tmp_cap = 0;
     int32_t* d_fcount = nullptr;   // nkeys
     int32_t* d_fstart = nullptr;   // nkeys + 1
-    int32_t* d_fcursor = nullptr;  // nkeys
     int32_t* d_big = nullptr;      // nkeys + 1: [0] = count, then key ids
     int32_t* d_flags = nullptr;    // [0] domain error
     double* d_part = nullptr;      // ntiles * NSUM * 64

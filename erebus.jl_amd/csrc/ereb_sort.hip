@@ -9,12 +9,13 @@
 // ascending key and, inside a key, ascending marker index -- exactly a
 // stable counting sort, which makes every later summation order canonical.
 //
-//   1. k_fine_count : key per marker, per-key counts (one integer atomic
-//                     per run of equal keys in a wavefront: the counts do
-//                     not depend on order); before a scatter also the
-//                     marker properties
+//   1. k_fine_count : key per marker and its rank among the markers of its
+//                     key in arrival order (one returning integer atomic per
+//                     run of equal keys in a wavefront: the counts do not
+//                     depend on order); before a scatter also the marker
+//                     properties
 //   2. scan         : key starts
-//   3. k_fine_place : slot reservation, one returning atomic per run
+//   3. k_fine_place : perm[start of the key + rank] = marker, no atomics
 //   4. k_fine_order : each key segment (about 9 markers) is sorted by marker
 //                     index in place (in registers up to 16 markers),
 //                     removing the arrival order of step 3;
@@ -69,12 +70,6 @@ __device__ __forceinline__ int32_t wave_reserve(int32_t* counters, uint32_t key,
     return base + g.rank;
 }
 
-// counters[key] += number of active lanes with that key (no return value)
-__device__ __forceinline__ void wave_count(int32_t* counters, uint32_t key, bool active) {
-    const KeyGroup g = wave_group(key, active);
-    if (g.count > 0) atomicAdd(&counters[key], g.count);
-}
-
 // fine key of a marker (see the file header); sets *bad when the P cell is
 // not within one cell of the basic cell (coordinates beyond ~1e15 m)
 __device__ __forceinline__ uint32_t fine_key(const Cells& c, int nty, bool* bad) {
@@ -88,12 +83,14 @@ __device__ __forceinline__ uint32_t fine_key(const Cells& c, int nty, bool* bad)
     return (uint32_t)((tx * nty + ty) * NFINE + fj * FINE + fi);
 }
 
-// fine key and count of every marker; with PropArgs also the marker
-// properties, in the same pass over the markers in storage order
+// fine key, rank within the key (arrival order) and count of every marker;
+// with PropArgs also the marker properties, in the same pass over the markers
+// in storage order
 template <class P>
 __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const double* __restrict__ xm,
                                                     const double* __restrict__ ym, int64_t M,
                                                     uint32_t* __restrict__ key,
+                                                    int32_t* __restrict__ rank,
                                                     int32_t* __restrict__ count,
                                                     int32_t* __restrict__ flags,
                                                     int32_t* __restrict__ big, P props) {
@@ -107,7 +104,8 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
         if (bad) flags[0] = 1;
         key[m] = k;
     }
-    wave_count(count, k, active);
+    const int32_t r = wave_reserve(count, k, active);
+    if (active) rank[m] = r;
     if (active) props.apply(m);
 }
 
@@ -132,14 +130,13 @@ static PropArgs<R> prop_args(ereb_ctx* c, const PropsReq& q) {
     return P;
 }
 
-__global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key, int64_t M,
-                                                    int32_t* __restrict__ cursor,
+// each marker to its sorted slot: the key's start plus its rank
+__global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key,
+                                                    const int32_t* __restrict__ rank, int64_t M,
+                                                    const int32_t* __restrict__ start,
                                                     uint32_t* __restrict__ perm) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = m < M;
-    const uint32_t k = active ? key[m] : 0u;
-    const int32_t pos = wave_reserve(cursor, k, active);
-    if (active) perm[pos] = (uint32_t)m;
+    if (m < M) perm[start[key[m]] + rank[m]] = (uint32_t)m;
 }
 
 constexpr int SEG_REG = 16;     // segments ordered in registers by rank
@@ -220,6 +217,9 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
     // unless a sort stopped half-way; a domain error flag stays set until
     // the host reads it (check_flags)
     if (c->fcount_dirty && (rc = fill_i32(c, c->d_fcount, 0, nk))) return rc;
+    // ranks live in the first M words of the oversize-segment scratch, which
+    // k_fine_order_big uses only after the placement
+    int32_t* rank = (int32_t*)c->d_tmp;
     if (M > 0) {
         c->fcount_dirty = true;
         const unsigned nb = (unsigned)((M + 255) / 256);
@@ -228,26 +228,25 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
         // the property vectors are float exactly when the context is in the mixed form
         if (!props)
             hipLaunchKernelGGL(k_fine_count<NoProps>, dim3(nb), dim3(256), 0, c->stream, g,
-                               c->nty, xm, ym, M, c->d_key, c->d_fcount, c->d_flags, c->d_big,
-                               NoProps{});
+                               c->nty, xm, ym, M, c->d_key, rank, c->d_fcount, c->d_flags,
+                               c->d_big, NoProps{});
         else if (marker_is_f32(c->precision, EREB_M_ETATOTALM))
             hipLaunchKernelGGL(k_fine_count<PropArgs<float>>, dim3(nb), dim3(256), 0, c->stream,
-                               g, c->nty, xm, ym, M, c->d_key, c->d_fcount, c->d_flags,
-                               c->d_big, prop_args<float>(c, *props));
+                               g, c->nty, xm, ym, M, c->d_key, rank, c->d_fcount,
+                               c->d_flags, c->d_big, prop_args<float>(c, *props));
         else
             hipLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256), 0,
-                               c->stream, g, c->nty, xm, ym, M, c->d_key, c->d_fcount,
-                               c->d_flags, c->d_big, prop_args<double>(c, *props));
+                               c->stream, g, c->nty, xm, ym, M, c->d_key, rank,
+                               c->d_fcount, c->d_flags, c->d_big, prop_args<double>(c, *props));
     }
-    // key starts; the cursors of the placement start as a copy of them; the
-    // counts are cleared for the next sort
-    if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk, c->d_fcursor, true))) return rc;
+    // key starts; the counts are cleared for the next sort
+    if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk, nullptr, true))) return rc;
     c->fcount_dirty = false;
     if (M > 0) {
         const unsigned nkb = (unsigned)((nk + 255) / 256);
         const unsigned nb = (unsigned)((M + 255) / 256);
-        hipLaunchKernelGGL(k_fine_place, dim3(nb), dim3(256), 0, c->stream, c->d_key, M,
-                           c->d_fcursor, c->d_perm);
+        hipLaunchKernelGGL(k_fine_place, dim3(nb), dim3(256), 0, c->stream, c->d_key,
+                           (const int32_t*)rank, M, c->d_fstart, c->d_perm);
         hipLaunchKernelGGL(k_fine_order, dim3(nkb), dim3(256), 0, c->stream, c->d_fstart, nk,
                            c->d_perm, c->d_big);
         hipLaunchKernelGGL(k_fine_order_big, dim3(64), dim3(1024), SEG_BIG * sizeof(uint32_t),
