@@ -535,10 +535,12 @@ __global__ __launch_bounds__(256) void k_asm_fill(int Nx1, int Ny1,
 
 // nzval through the stored slot indices (after every fill, and alone while
 // the pattern is unchanged): one coalesced write, one gather per entry, in a
-// grid-stride loop over at most ASM_VALUES_BLOCKS workgroups.  chg
-// (optional): the pattern-change flag, read by the kernels before this one
-// and cleared here for the next assembly's rows kernel.
-constexpr int ASM_VALUES_BLOCKS = 2048;
+// grid-stride loop over the entries (nnz is known on the device only) with
+// ASM_VALUES_PER entries in flight per thread.  chg (optional): the
+// pattern-change flag, read by the kernels before this one and cleared here
+// for the next assembly's rows kernel.
+constexpr int ASM_VALUES_PER = 4;
+constexpr int ASM_VALUES_BLOCKS = 8192;
 __global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ colptr_end,
                                                     const int32_t* __restrict__ map,
                                                     const double* __restrict__ slotv,
@@ -546,9 +548,24 @@ __global__ __launch_bounds__(256) void k_asm_values(const int32_t* __restrict__ 
                                                     int32_t* __restrict__ chg) {
     if (chg && blockIdx.x == 0 && threadIdx.x == 0) chg[0] = 0;
     const int64_t nnz = colptr_end[0];
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
-         q += (int64_t)gridDim.x * blockDim.x)
-        nzval[q] = slotv[map[q]];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q0 < nnz;
+         q0 += ASM_VALUES_PER * stride) {
+        int32_t mi[ASM_VALUES_PER];
+        double v[ASM_VALUES_PER];
+#pragma unroll
+        for (int u = 0; u < ASM_VALUES_PER; ++u) {
+            const int64_t q = q0 + u * stride;
+            mi[u] = q < nnz ? map[q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < ASM_VALUES_PER; ++u) v[u] = slotv[mi[u]];
+#pragma unroll
+        for (int u = 0; u < ASM_VALUES_PER; ++u) {
+            const int64_t q = q0 + u * stride;
+            if (q < nnz) nzval[q] = v[u];
+        }
+    }
 }
 
 // 0-based int32 column pointers to 1-based Int64
@@ -741,7 +758,9 @@ static int assemble(ereb_ctx* c, double dt) {
                            s.pres, s.slotv, (const int32_t*)s.colcnt, s.colptr, s.rowval,
                            s.nzval, chg, s.map);
     hipLaunchKernelGGL(k_asm_values,
-                       dim3((unsigned)std::min<int64_t>((cap + 255) / 256, ASM_VALUES_BLOCKS)),
+                       dim3((unsigned)std::min<int64_t>(
+                           (cap + 256 * ASM_VALUES_PER - 1) / (256 * ASM_VALUES_PER),
+                           ASM_VALUES_BLOCKS)),
                        dim3(256), 0, c->stream, (const int32_t*)s.colptr + n,
                        (const int32_t*)s.map, (const double*)s.slotv, s.nzval, s.chg);
     EREB_HIP(c, hipGetLastError());

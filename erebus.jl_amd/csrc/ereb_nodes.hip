@@ -10,6 +10,7 @@ namespace ereb {
 struct NodeArgs {
     int nb, ns;  // basic and staggered node counts
     int do_basic, do_vx, do_vy, do_p;
+    int bc, R, C;  // insulating conditions on tk1 (P grid of R rows, C columns) fused
     // basic
     const double *ETA0SUM, *ETASUM, *GGGSUM, *SXYSUM, *COHSUM, *TENSUM, *FRISUM, *WTSUM;
     double *ETA0, *ETA, *GGG, *SXY0, *COH, *TEN, *FRI;
@@ -80,6 +81,19 @@ __global__ __launch_bounds__(256) void k_node_props(NodeArgs A) {
             A.PHI[k] = phi;
             A.BETAPHI[k] = (1.0 / gggp) * phi;
         }
+        if (A.bc) {
+            // apply_insulating_boundary_conditions! (E:2432-2449) fused: a
+            // boundary node takes the value of its nearest inner node
+            // (rows first, then columns, so a corner takes the inner
+            // diagonal), which that node's thread writes from the same sums
+            // when its weight is positive and leaves unchanged otherwise
+            const int i = k % A.R, j = k / A.R;
+            if (i == 0 || i == A.R - 1 || j == 0 || j == A.C - 1) {
+                const int si = min(max(i, 1), A.R - 2), sj = min(max(j, 1), A.C - 2);
+                const int q = sj * A.R + si;
+                A.tk1[k] = A.WTPSUM[q] > 0.0 ? A.TKSUM[q] * (1.0 / A.RHOCPSUM[q]) : A.tk1[q];
+            }
+        }
     }
 }
 
@@ -116,6 +130,10 @@ int launch_node_properties(ereb_ctx* c, bool basic, bool vx, bool vy, bool p, bo
     A.do_vx = vx;
     A.do_vy = vy;
     A.do_p = p;
+    // the boundary conditions ride on the P-grid pass when it runs
+    A.R = (int)c->Ny1;
+    A.C = (int)c->Nx1;
+    A.bc = bc && p && A.R > 2 && A.C > 2;
     A.ETA0SUM = gptr(c, EREB_G_ETA0SUM); A.ETASUM = gptr(c, EREB_G_ETASUM);
     A.GGGSUM = gptr(c, EREB_G_GGGSUM); A.SXYSUM = gptr(c, EREB_G_SXYSUM);
     A.COHSUM = gptr(c, EREB_G_COHSUM); A.TENSUM = gptr(c, EREB_G_TENSUM);
@@ -152,7 +170,7 @@ int launch_node_properties(ereb_ctx* c, bool basic, bool vx, bool vy, bool p, bo
     hipLaunchKernelGGL(k_node_props, dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0,
                        c->stream, A);
     EREB_HIP(c, hipGetLastError());
-    if (bc) {
+    if (bc && !p) {
         int rc = launch_insulating_bc(c, gptr(c, EREB_G_TK1), c->Ny1, c->Nx1);
         if (rc) return rc;
     }

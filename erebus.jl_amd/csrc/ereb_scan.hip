@@ -68,20 +68,21 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(int32_t* __restrict__ in,
                                                        const int32_t* __restrict__ run) {
     __shared__ int32_t sh[SCAN_T / 64 + 1];
     if (run && run[0] == 0) return;
+    // the block's inputs are loaded first, so that their latency overlaps the
+    // reduction of the block sums before this block
+    int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_PER;
+    int32_t v[SCAN_PER];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) v[k] = base + k < n ? in[base + k] : 0;
     int32_t off = 0, tot;
     for (int64_t b0 = 0; b0 < blockIdx.x; b0 += SCAN_T) {
         const int64_t b = b0 + threadIdx.x;
         block_exclusive_scan(b < blockIdx.x ? bs[b] : 0, sh, &tot);
         off += tot;
     }
-    int64_t base = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_PER;
-    int32_t v[SCAN_PER];
     int32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0;
-        s += v[k];
-    }
+    for (int k = 0; k < SCAN_PER; ++k) s += v[k];
     int32_t e = block_exclusive_scan(s, sh, &tot) + off;
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
 #pragma unroll

@@ -130,12 +130,14 @@ static PropArgs<R> prop_args(ereb_ctx* c, const PropsReq& q) {
     return P;
 }
 
-// each marker to its sorted slot: the key's start plus its rank
+// each marker to its sorted slot: the key's start plus its rank.  The
+// workgroups are numbered XCD-contiguously, so that neighbouring marker
+// columns, whose markers fill the same key segments, write through one L2.
 __global__ __launch_bounds__(256) void k_fine_place(const uint32_t* __restrict__ key,
                                                     const int32_t* __restrict__ rank, int64_t M,
                                                     const int32_t* __restrict__ start,
                                                     uint32_t* __restrict__ perm) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t m = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (m < M) perm[start[key[m]] + rank[m]] = (uint32_t)m;
 }
 
