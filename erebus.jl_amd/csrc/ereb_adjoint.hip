@@ -298,8 +298,7 @@ static int adj_forward_step(ereb_ctx* c, int64_t k, double dt, int mode, const d
                             const double* hrf, double* xb, double* yb) {
     int rc;
     if ((rc = launch_scatter(c, hrs, hrf, mode))) return rc;
-    if ((rc = launch_combine(c))) return rc;
-    if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
+    if ((rc = launch_combine_nodes(c))) return rc;
     EREB_HIP(c, hipMemcpyAsync(gptr(c, EREB_G_TK2), gptr(c, EREB_G_TK1),
                                sizeof(double) * c->gnumel[EREB_G_TK1], hipMemcpyDeviceToDevice,
                                c->stream));
@@ -496,9 +495,7 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
         if ((rc = run_step(k, sg * every, grad && sg == nseg - 1))) return rc;
     }
     // loss on the final state
-    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, md)) ||
-        (rc = launch_combine(c)) ||
-        (rc = launch_node_properties(c, true, true, true, true, true)))
+    if ((rc = launch_scatter(c, hrsolidm, hrfluidm, md)) || (rc = launch_combine_nodes(c)))
         return rc;
     const int64_t n = c->gnumel[EREB_G_TK1];
     std::vector<double> tk1((size_t)n);

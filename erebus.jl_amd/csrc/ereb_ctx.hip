@@ -604,8 +604,7 @@ static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, 
                              double dt, int64_t timestep) {
     int rc;
     if ((rc = launch_scatter(c, hrs, hrf, (int)mode))) return rc;
-    if ((rc = launch_combine(c))) return rc;
-    if ((rc = launch_node_properties(c, true, true, true, true, true))) return rc;
+    if ((rc = launch_combine_nodes(c))) return rc;
     if ((rc = launch_assemble_stokes(c, dt))) return rc;
     if ((rc = launch_assemble_thermal(c, dt))) return rc;
     // the stress and temperature gathers and RK4 in one pass over the markers

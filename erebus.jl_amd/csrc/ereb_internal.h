@@ -18,6 +18,29 @@ namespace ereb {
 constexpr int TILE = 6;
 constexpr int WIN = TILE + 2;
 constexpr int NSUM = 30;  // SUM grids, order of reset_interpolated_properties!
+
+// Scatter window partials: tile t's partial of sum s at node (r, c) of its
+// (TILE+2)^2 window, r and c relative to the tile's first node, is
+// part[(t * NSUM + s) * 64 + 8 c + r].  A node's sum adds the partials of
+// the at most 2 x 2 tiles whose windows hold it, tiles in ascending (tx, ty)
+// order; (i0, j0): 0-based node row and column.
+struct WinTiles {
+    int tylo, tyhi, txlo, txhi;
+};
+__device__ __forceinline__ WinTiles win_tiles(int i0, int j0, int nty, int ntx) {
+    return WinTiles{max(0, (i0 - (WIN - 1) + TILE - 1) / TILE), min(nty - 1, i0 / TILE),
+                    max(0, (j0 - (WIN - 1) + TILE - 1) / TILE), min(ntx - 1, j0 / TILE)};
+}
+__device__ __forceinline__ double win_sum(const double* __restrict__ part, int nty,
+                                          const WinTiles& w, int s, int i0, int j0) {
+    double acc = 0.0;
+    for (int tx = w.txlo; tx <= w.txhi; ++tx)
+        for (int ty = w.tylo; ty <= w.tyhi; ++ty) {
+            const int lane = (j0 - tx * TILE) * 8 + (i0 - ty * TILE);
+            acc += part[((size_t)(tx * nty + ty) * NSUM + s) * 64 + lane];
+        }
+    return acc;
+}
 constexpr int FINE = 2 * TILE;  // fine cells per tile side
 constexpr int NFINE = FINE * FINE;  // fine cells (sort keys) per tile
 
@@ -215,6 +238,8 @@ int ensure_sort(ereb_ctx* c);
 // then the scatter of the properties
 int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode);
 int launch_combine(ereb_ctx* c);
+// launch_combine + launch_node_properties(all grids, boundary) in one kernel
+int launch_combine_nodes(ereb_ctx* c);
 
 // ereb_nodes.hip
 int launch_node_properties(ereb_ctx* c, bool basic, bool vx, bool vy, bool p, bool bc);

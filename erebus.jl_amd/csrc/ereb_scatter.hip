@@ -488,15 +488,7 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ part
     const int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (n >= rows * cols) return;
     const int i0 = n % rows, j0 = n / rows;
-    const int tylo = max(0, (i0 - (WIN - 1) + TILE - 1) / TILE), tyhi = min(nty - 1, i0 / TILE);
-    const int txlo = max(0, (j0 - (WIN - 1) + TILE - 1) / TILE), txhi = min(ntx - 1, j0 / TILE);
-    double acc = 0.0;
-    for (int tx = txlo; tx <= txhi; ++tx)
-        for (int ty = tylo; ty <= tyhi; ++ty) {
-            const int lane = (j0 - tx * TILE) * 8 + (i0 - ty * TILE);
-            acc += part[((size_t)(tx * nty + ty) * NSUM + s) * 64 + lane];
-        }
-    out.p[s][n] = acc;
+    out.p[s][n] = win_sum(part, nty, win_tiles(i0, j0, nty, ntx), s, i0, j0);
 }
 
 static const int kSumGrid[NSUM] = {

@@ -151,6 +151,42 @@ def test_node_properties_bit_exact():
     _exact(tt, orc.insulating_bc(t), "insulating BC")
 
 
+P_SUMS = ["RHOSUM", "RHOCPSUM", "ALPHASUM", "ALPHAFSUM", "HRSUM", "GGGPSUM", "SXXSUM", "TKSUM",
+          "PHISUM", "WTPSUM"]
+P_OUTS = ["RHO", "RHOCP", "ALPHA", "ALPHAF", "HR", "GGGP", "SXX0", "tk1", "PHI", "BETAPHI"]
+
+
+def test_step_node_pass_applies_insulating_bc():
+    """The resident node pass (ereb_step_node_properties) applies the
+    insulating conditions to tk1 inside its P-grid pass: the P outputs equal
+    compute_p_node_properties! followed by
+    apply_insulating_boundary_conditions! on tk1 (E:2432-2449, oracle) bit
+    for bit, with zero weights on part of the inner ring, whose nodes keep
+    their previous tk1, which the boundary then copies (corners included)."""
+    g, mat = tconsts()
+    rng = np.random.default_rng(21)
+    S = (g.Ny1, g.Nx1)
+    sums_p = [cases.rand_grid(rng, S) for _ in range(10)]
+    w = sums_p[9]
+    w[rng.random(S) < 0.3] = 0.0
+    w[1, 1::3] = 0.0
+    w[-2, 2::4] = 0.0
+    w[1::5, 1] = 0.0
+    w[2::2, -2] = 0.0
+    w[1, 1] = w[-2, -2] = 0.0
+    outs_p = [cases.rand_grid(rng, S) for _ in range(10)]
+    orc = Oracle(g, mat)
+    op = orc.p_node_properties(sums_p, outs_p)
+    op[7] = orc.insulating_bc(op[7])
+    with _ctx(g, mat) as ctx:
+        for n, a in zip(P_SUMS + P_OUTS, sums_p + outs_p):
+            ctx.set_grid(n, a)
+        ctx.step_node_properties()
+        got = [ctx.get_grid(n) for n in P_OUTS]
+    for n, a, b in zip(P_OUTS, got, op):
+        _exact(a, b, n)
+
+
 @pytest.mark.parametrize("timestep", [1, 2])
 def test_gather_temperature_bit_exact(timestep):
     g, mat = tconsts()
