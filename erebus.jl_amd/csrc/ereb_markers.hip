@@ -276,14 +276,28 @@ __device__ __forceinline__ double rk_eval(double2 c01, double2 c23, double2 c45,
     return __builtin_fma(w, Q, P);
 }
 
+// The last half-cell of one table a thread looked up: consecutive RK4
+// stages of a marker mostly fall in the same half-cell, which then costs no
+// load (key = 2 node + half; ~0u = none yet).
+struct RkCache {
+    uint32_t key;
+    double2 c01, c23, c45;
+};
+
 // The stage velocity of the half `second` of the cell whose coefficients start
 // at node index `node` of table T
 __device__ __forceinline__ double rk_poly(const double* __restrict__ T, uint32_t node,
-                                          bool second, double u, double w) {
-    const uint32_t off = node * (uint32_t)(RKC * 8) + (second ? 48u : 0u);  // < 2^31 bytes
-    const char* p = (const char*)T + off;
-    return rk_eval(*(const double2*)p, *(const double2*)(p + 16), *(const double2*)(p + 32), u,
-                   w);
+                                          bool second, double u, double w, RkCache& C) {
+    const uint32_t key = (node << 1) | (second ? 1u : 0u);
+    if (key != C.key) {
+        const uint32_t off = node * (uint32_t)(RKC * 8) + (second ? 48u : 0u);  // < 2^31 bytes
+        const char* p = (const char*)T + off;
+        C.key = key;
+        C.c01 = *(const double2*)p;
+        C.c23 = *(const double2*)(p + 16);
+        C.c45 = *(const double2*)(p + 32);
+    }
+    return rk_eval(C.c01, C.c23, C.c45, u, w);
 }
 
 // RK4 stage velocity on the Vx grid (E:5057-5091) from the cell's quadratic
@@ -292,24 +306,24 @@ __device__ __forceinline__ double rk_poly(const double* __restrict__ T, uint32_t
 // P(fx) + fy Q(fx).  The Vy grid (E:5093-5127) is the transpose.
 template <bool EXACT>
 __device__ __forceinline__ double rk_vx(const GeomK& g, const double* __restrict__ T, double x,
-                                        double y) {
+                                        double y, RkCache& C) {
     const double tx = (x - g.x[0]) * g.idx, ty = (y - g.yp[0]) * g.idy;
     const int j = stage_cell(tx, 1, g.Nx - 1);
     const int i = stage_cell(ty, 1, g.Ny);
     const double fx = stage_frac<EXACT>(g.x, x, tx, g.idx, j);
     const double fy = stage_frac<EXACT>(g.yp, y, ty, g.idy, i);
-    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fx >= 0.5, fx, fy);
+    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fx >= 0.5, fx, fy, C);
 }
 
 template <bool EXACT>
 __device__ __forceinline__ double rk_vy(const GeomK& g, const double* __restrict__ T, double x,
-                                        double y) {
+                                        double y, RkCache& C) {
     const double tx = (x - g.xp[0]) * g.idx, ty = (y - g.y[0]) * g.idy;
     const int j = stage_cell(tx, 1, g.Nx);
     const int i = stage_cell(ty, 1, g.Ny - 1);
     const double fx = stage_frac<EXACT>(g.xp, x, tx, g.idx, j);
     const double fy = stage_frac<EXACT>(g.y, y, ty, g.idy, i);
-    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fy >= 0.5, fy, fx);
+    return rk_poly(T, (uint32_t)((j - 1) * g.Ny1 + (i - 1)), fy >= 0.5, fy, fx, C);
 }
 
 struct RK4Args {
@@ -343,10 +357,11 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     const double* b = A.M.brk4;
     const double* cr = A.M.crk4;
     double xr = x0, yr = y0, sx = 0.0, sy = 0.0;
+    RkCache cx{~0u, {}, {}, {}}, cy{~0u, {}, {}, {}};
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        const double vx = rk_vx<EXACT>(g, A.CXs, xr, yr);
-        const double vy = rk_vy<EXACT>(g, A.CYs, xr, yr);
+        const double vx = rk_vx<EXACT>(g, A.CXs, xr, yr, cx);
+        const double vy = rk_vy<EXACT>(g, A.CYs, xr, yr, cy);
         sx = rk == 0 ? b[0] * vx : __builtin_fma(b[rk], vx, sx);
         sy = rk == 0 ? b[0] * vy : __builtin_fma(b[rk], vy, sy);
         if (rk < 3) {
@@ -358,10 +373,11 @@ __device__ __forceinline__ void rk4_paths(const RK4Args& A, double dt, double x0
     yn = __builtin_fma(dt, sy, y0);
     xr = xn;
     yr = yn;
+    cx.key = cy.key = ~0u;  // the fluid tables
 #pragma unroll
     for (int rk = 0; rk < 4; ++rk) {
-        const double vx = rk_vx<EXACT>(g, A.CXf, xr, yr);
-        const double vy = rk_vy<EXACT>(g, A.CYf, xr, yr);
+        const double vx = rk_vx<EXACT>(g, A.CXf, xr, yr, cx);
+        const double vy = rk_vy<EXACT>(g, A.CYf, xr, yr, cy);
         sx = rk == 0 ? b[0] * vx : __builtin_fma(b[rk], vx, sx);
         sy = rk == 0 ? b[0] * vy : __builtin_fma(b[rk], vy, sy);
         if (rk < 3) {
