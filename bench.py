@@ -63,7 +63,8 @@ START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 #   sort    compute_marker_properties! (E:700): r tm tkm phim X_W rhocp, w 7
 #   scatter marker_to_*_nodes! (E:1605-1863): r xm ym + 18 staged values,
 #           w 30 SUM grids
-#   combine the SUM grids' tile overlap (no reference counterpart: 0)
+#   combine the SUM grids' tile overlap (no reference counterpart: 0; in the
+#           transfer step part of the nodes class, k_combine_nodes)
 #   nodes   compute_*_node_properties! + insulating tk1 (E:1970-2189, E:2432):
 #           r 30 SUM, w 27 grids + YNY
 #   rk4     update_marker_stress! (E:4354: r xm ym sxx sxy, w sxx sxy; r DSXX
@@ -90,11 +91,11 @@ STEP_CLASSES = ["sort", "scatter", "combine", "nodes", "rk4", "asm_sd", "asm_t"]
 TRAFFIC_KERNEL = {
     "fp64": {"scatter": "void ereb::k_scatter<double>", "rk4": "void ereb::k_rk4<1, true>",
              "sort": "void ereb::k_fine_count<ereb::PropArgs<double>",
-             "nodes": "ereb::k_node_props", "asm_sd": "void ereb::k_asm_rows<0>",
+             "nodes": "ereb::k_combine_nodes", "asm_sd": "void ereb::k_asm_rows<0>",
              "asm_t": "void ereb::k_asm_rows<1>"},
     "mixed": {"scatter": "void ereb::k_scatter<float>", "rk4": "void ereb::k_rk4<2, true>",
               "sort": "void ereb::k_fine_count<ereb::PropArgs<float>",
-              "nodes": "ereb::k_node_props", "asm_sd": "void ereb::k_asm_rows<0>",
+              "nodes": "ereb::k_combine_nodes", "asm_sd": "void ereb::k_asm_rows<0>",
               "asm_t": "void ereb::k_asm_rows<1>"},
 }
 
