@@ -22,7 +22,7 @@ struct NodeArgs {
     uint8_t* YNY;
 };
 
-// compute_basic_node_properties! (E:1970-2009) at node k; sum(q) gives the
+// compute_basic_node_properties! (E:1970-2007) at node k; sum(q) gives the
 // node's sum q
 template <class S>
 __device__ __forceinline__ void basic_props(const NodeArgs& A, int k, const S& sum) {
@@ -41,7 +41,8 @@ __device__ __forceinline__ void basic_props(const NodeArgs& A, int k, const S& s
     }
 }
 
-// compute_vx_node_properties! / compute_vy_node_properties! (E:2011-2089)
+// compute_vx_node_properties! / compute_vy_node_properties! (E:2033-2060,
+// E:2086-2113)
 template <class S>
 __device__ __forceinline__ void v_props(double* const* out, int k, const S& sum) {
     const double wt = sum(5);
@@ -58,7 +59,7 @@ __device__ __forceinline__ double p_tk1(const S& sum) {
     return sum(7) * (1.0 / sum(1));
 }
 
-// compute_p_node_properties! (E:2091-2189) at node k, then, with A.bc, the
+// compute_p_node_properties! (E:2148-2189) at node k, then, with A.bc, the
 // insulating conditions of tk1 (E:2432-2449) fused: a boundary node takes the
 // value of its nearest inner node (rows first, then columns, so a corner
 // takes the inner diagonal): that node's temperature from its own sums
