@@ -381,6 +381,7 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     CK(hipMalloc(&c->d_flags, sizeof(int32_t) * 4));
     CK(hipMemset(c->d_flags, 0, sizeof(int32_t) * 4));
     CK(hipMalloc(&c->d_part, sizeof(double) * (size_t)c->ntiles * NSUM * 64));
+    CK(hipMalloc(&c->d_typed, sizeof(int32_t)));
 #undef CK
     rc = ensure_scan(c, std::max<int64_t>(6 * Nx1 * Ny1, c->nkeys));
     if (rc) return bail(rc);
@@ -415,6 +416,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_big);
     hipFree(c->d_flags);
     hipFree(c->d_part);
+    hipFree(c->d_typed);
     hipFree(c->d_scan);
     hipFree(c->d_react);
     for (auto& s : c->sys) {
@@ -461,6 +463,7 @@ int ereb_ctx_set_precision(ereb_ctx* c, int precision) {
         c->d_m[f] = p;
     }
     c->precision = precision;
+    c->typed = -1;
     return EREB_OK;
 }
 
@@ -507,6 +510,11 @@ int ereb_markers_set(ereb_ctx* c, int field, const void* host, int64_t marknum) 
         EREB_HIP(c, hipStreamSynchronize(c->stream));
     }
     if (field == EREB_M_XM || field == EREB_M_YM || marknum != c->marknum) ++c->pos_gen;
+    if (field == EREB_M_TM || field == EREB_M_INV_GGGTOTALM || field == EREB_M_FRICTTOTALM ||
+        field == EREB_M_COHESTOTALM || field == EREB_M_TENSTOTALM ||
+        field == EREB_M_RHOFLUIDCUR || field == EREB_M_ALPHASOLIDCUR ||
+        field == EREB_M_ALPHAFLUIDCUR || marknum != c->marknum)
+        c->typed = -1;
     c->marknum = marknum;
     return EREB_OK;
 }
@@ -621,6 +629,7 @@ static StepKey step_key(const ereb_ctx* c, int64_t mode) {
     k.cap = c->cap;
     k.mode = mode;
     k.precision = c->precision;
+    k.typed = c->typed;
     int n = 0;
     for (int f = 0; f < EREB_M_NFIELDS; ++f) k.bufs[n++] = c->d_m[f];
     const void* more[] = {c->d_key, c->d_perm, c->d_tmp, c->d_scan, c->d_grids, c->d_part,
@@ -648,6 +657,8 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     c->scal.dt = dt;
     c->scal.first = timestep == 1 ? 1 : 0;
     if ((rc = scal_sync(c))) return rc;
+    // which scatter loads the step takes is part of its key (typed_refresh)
+    if ((rc = typed_refresh(c))) return rc;
     // eager: when asked, while profiling (events around every launch), and
     // for the first step (it makes the allocations a capture must not)
     if (!c->use_graphs || c->prof || !c->step_warm) {

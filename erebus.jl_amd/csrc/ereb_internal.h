@@ -82,7 +82,7 @@ struct ProfPair {
 // all of them are unchanged.
 struct StepKey {
     int64_t marknum, cap, mode;
-    int precision, pad;
+    int precision, typed;
     const void* bufs[EREB_M_NFIELDS + 19];
 };
 
@@ -132,6 +132,11 @@ struct ereb_ctx {
     int64_t react_cap = 0;
     int64_t scan_cap = 0;
     bool have_sort = false;
+    // the seven material vectors of the scatter that the reference derives
+    // from the marker type (E:659-665) hold exactly those values for every
+    // resident marker: 1 yes, 0 no, -1 not checked since they last changed
+    int typed = -1;
+    int32_t* d_typed = nullptr;  // flag of the check kernel
     bool axes_exact = false;  // node k of every axis == fma(k, spacing, node 0)
     bool fcount_dirty = false;  // a sort stopped between counting and scanning
     // the sort is current while sort_gen == pos_gen; pos_gen advances
@@ -238,6 +243,9 @@ int ensure_sort(ereb_ctx* c);
 // then the scatter of the properties
 int launch_scatter(ereb_ctx* c, const double* hrs, const double* hrf, int mode);
 int launch_combine(ereb_ctx* c);
+// refresh ctx->typed when unknown (a check kernel and a host read; not under
+// graph capture)
+int typed_refresh(ereb_ctx* c);
 // launch_combine + launch_node_properties(all grids, boundary) in one kernel
 int launch_combine_nodes(ereb_ctx* c);
 

@@ -71,6 +71,32 @@ enum {
 // R is the type of the property vectors, the LDS staging and the per-fine-cell
 // sums: double in the Float64 form, float in the mixed form.  The marker
 // state (coordinates, temperature, porosity, stresses) is double in both.
+// Per-type values of the seven material vectors (E:659-665): 1/G (inv),
+// cohesion, tension, friction, fluid density, solid and fluid expansivity
+struct TypedTab {
+    double v[7][3];
+};
+
+__host__ static TypedTab typed_tab(const ereb_materials& M) {
+    TypedTab T;
+    for (int k = 0; k < 3; ++k) {
+        T.v[0][k] = 1.0 / M.gggsolidm[k];  // inv(gggsolidm[tm]), E:659
+        T.v[1][k] = M.cohessolidm[k];
+        T.v[2][k] = M.tenssolidm[k];
+        T.v[3][k] = M.frictsolidm[k];
+        T.v[4][k] = M.rhofluidm[k];
+        T.v[5][k] = M.alphasolidm[k];
+        T.v[6][k] = M.alphafluidm[k];
+    }
+    return T;
+}
+
+// row q of the table at type index t (0..2), by selects (no dynamic
+// indexing of the kernel argument)
+__device__ __forceinline__ double typed_at(const TypedTab& T, int q, int t) {
+    return t == 0 ? T.v[q][0] : (t == 1 ? T.v[q][1] : T.v[q][2]);
+}
+
 template <class R>
 struct ScatterArgs {
     GeomK g;
@@ -81,6 +107,13 @@ struct ScatterArgs {
     const R *etavp, *invggg, *frict, *cohes, *tens, *rhof, *alphas, *alphaf;
     const R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
     double* part;
+    // typed: 1/G, cohesion, tension, friction, fluid density and the two
+    // expansivities are the material table values of the marker's type, as
+    // the reference sets them (E:659-665, checked on the device by
+    // typed_refresh): read through tm from `tab` instead of seven vectors
+    int typed;
+    const int32_t* tm;
+    TypedTab tab;
 };
 
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return fma(a, b, c); }
@@ -145,6 +178,7 @@ struct ScatterLDS {
     R sfr[4][CHP];  // dxmj/dx basic, dymi/dy basic, dxmj/dx P, dymi/dy P
     R part[NPART];
     double ax[4][WIN];   // tile window of the x, y, xp, yp axes
+    double typed[7][3];  // the typed material table (TypedTab)
     int32_t fs[NFINE + 1];
 };
 
@@ -256,8 +290,8 @@ __device__ __forceinline__ void write_part(double* part, int tile, const NodeAcc
 constexpr int NRAW = 10;
 
 template <class R>
-__device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, int part, uint32_t m,
-                                         double (&v)[NRAW]) {
+__device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, const ScatterLDS<R>& L,
+                                         int part, uint32_t m, double (&v)[NRAW]) {
     if (part == 0) {
         v[0] = A.xm[m];
         v[1] = A.ym[m];
@@ -265,14 +299,29 @@ __device__ __forceinline__ void load_raw(const ScatterArgs<R>& A, int part, uint
         v[3] = A.sxx[m];
         v[4] = A.sxy[m];
         v[5] = A.etavp[m];
-        v[6] = A.invggg[m];
-        v[7] = A.cohes[m];
-        v[8] = A.tens[m];
-        v[9] = A.frict[m];
+        if (A.typed) {
+            const int t = min(max(A.tm[m], 1), 3) - 1;
+            v[6] = L.typed[0][t];
+            v[7] = L.typed[1][t];
+            v[8] = L.typed[2][t];
+            v[9] = L.typed[3][t];
+        } else {
+            v[6] = A.invggg[m];
+            v[7] = A.cohes[m];
+            v[8] = A.tens[m];
+            v[9] = A.frict[m];
+        }
     } else {
-        v[0] = A.rhof[m];
-        v[1] = A.alphas[m];
-        v[2] = A.alphaf[m];
+        if (A.typed) {
+            const int t = min(max(A.tm[m], 1), 3) - 1;
+            v[0] = L.typed[4][t];
+            v[1] = L.typed[5][t];
+            v[2] = L.typed[6][t];
+        } else {
+            v[0] = A.rhof[m];
+            v[1] = A.alphas[m];
+            v[2] = A.alphaf[m];
+        }
         v[3] = A.rhototal[m];
         v[4] = A.rhocptotal[m];
         v[5] = A.etatotal[m];
@@ -369,6 +418,7 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
     const int r = lane & 7, c = lane >> 3;
     const int j0 = (tile / A.nty) * TILE, i0 = (tile % A.nty) * TILE;
     for (int i = t; i <= NFINE; i += NTHR) L.fs[i] = A.fstart[(size_t)tile * NFINE + i];
+    if (t < 21) L.typed[t / 3][t % 3] = A.tab.v[t / 3][t % 3];
     if (t < 4 * WIN) {
         const int a = t / WIN, k = t % WIN;
         const double* axis = a == 0 ? A.g.x : a == 1 ? A.g.y : a == 2 ? A.g.xp : A.g.yp;
@@ -389,7 +439,7 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
     if (cur.k < TILE && ts < cur.cend - cur.c0) {
         const uint32_t m = chunk_perm(A, cur, ts);
 #pragma unroll
-        for (int q = 0; q < NSET; ++q) load_raw(A, part + q, m, raw[q]);
+        for (int q = 0; q < NSET; ++q) load_raw(A, L, part + q, m, raw[q]);
 #pragma unroll
         for (int q = 0; q < NSET; ++q) stage_raw(A, L, part + q, ts, raw[q], j0, i0);
     }
@@ -400,7 +450,7 @@ __global__ __launch_bounds__(ScatterShape<R>::NW * 64, 2) void k_scatter(Scatter
         const Chunk nx2 = chunk_next(L, nx);
         if (nx.k < TILE && ts < nx.cend - nx.c0) {
 #pragma unroll
-            for (int q = 0; q < NSET; ++q) load_raw(A, part + q, pnx, raw[q]);
+            for (int q = 0; q < NSET; ++q) load_raw(A, L, part + q, pnx, raw[q]);
         }
         pnx = chunk_perm(A, nx2, ts);
         // ---- 2. (fine cell, corner column) partials: wavefronts g and g + 4
@@ -501,6 +551,64 @@ static const int kSumGrid[NSUM] = {
 
 int sum_grid_field(int s) { return kSumGrid[s]; }
 
+// flag[0] := 1 if any marker's seven material values differ (bitwise) from
+// its type's table values (in the vectors' precision)
+__device__ __forceinline__ bool same_bits(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ __forceinline__ bool same_bits(float a, double b) {
+    return __float_as_uint(a) == __float_as_uint((float)b);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_typed_check(ScatterArgs<R> A, int64_t M,
+                                                     int32_t* __restrict__ flag) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int t = min(max(A.tm[m], 1), 3) - 1;
+    const R* v[7] = {A.invggg, A.cohes, A.tens, A.frict, A.rhof, A.alphas, A.alphaf};
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ok = ok && same_bits(v[q][m], typed_at(A.tab, q, t));
+    if (!ok) flag[0] = 1;
+}
+
+template <class R>
+static int typed_check_t(ereb_ctx* c) {
+    auto P = [&](int f) { return (const R*)c->d_m[f]; };
+    ScatterArgs<R> A{};
+    A.invggg = P(EREB_M_INV_GGGTOTALM);
+    A.cohes = P(EREB_M_COHESTOTALM);
+    A.tens = P(EREB_M_TENSTOTALM);
+    A.frict = P(EREB_M_FRICTTOTALM);
+    A.rhof = P(EREB_M_RHOFLUIDCUR);
+    A.alphas = P(EREB_M_ALPHASOLIDCUR);
+    A.alphaf = P(EREB_M_ALPHAFLUIDCUR);
+    A.tm = tm_ptr(c);
+    A.tab = typed_tab(c->mat);
+    int rc = fill_i32(c, c->d_typed, 0, 1);
+    if (rc) return rc;
+    const int64_t M = c->marknum;
+    if (M > 0)
+        hipLaunchKernelGGL(k_typed_check<R>, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
+                           c->stream, A, M, c->d_typed);
+    EREB_HIP(c, hipGetLastError());
+    int32_t flag = 1;
+    EREB_HIP(c, hipMemcpyAsync(&flag, c->d_typed, sizeof flag, hipMemcpyDeviceToHost,
+                               c->stream));
+    EREB_HIP(c, hipStreamSynchronize(c->stream));
+    c->typed = flag ? 0 : 1;
+    return EREB_OK;
+}
+
+int typed_refresh(ereb_ctx* c) {
+    if (c->typed >= 0) return EREB_OK;
+    if (c->capturing) return fail(c, EREB_ERR_STATE, "material check inside a captured step");
+    // the vectors are float exactly when the context is in the mixed form
+    if (marker_is_f32(c->precision, EREB_M_INV_GGGTOTALM)) return typed_check_t<float>(c);
+    return typed_check_t<double>(c);
+}
+
 template <class R>
 static int launch_scatter_t(ereb_ctx* c) {
     auto P = [&](int f) { return (const R*)c->d_m[f]; };
@@ -530,6 +638,13 @@ static int launch_scatter_t(ereb_ctx* c) {
     A.tkrhocp = P(EREB_M_TKM_RHOCPTOTALM);
     A.etafkphi = P(EREB_M_ETAFLUIDCUR_INV_KPHIM);
     A.part = c->d_part;
+    if (c->typed < 0 && !c->capturing) {
+        const int rc = typed_refresh(c);
+        if (rc) return rc;
+    }
+    A.typed = c->typed == 1 ? 1 : 0;
+    A.tm = tm_ptr(c);
+    A.tab = typed_tab(c->mat);
     hipEvent_t ev;
     prof_begin(c, EREB_K_SCATTER, &ev);
     hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)c->ntiles), dim3(ScatterShape<R>::NW * 64), 0,

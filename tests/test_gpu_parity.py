@@ -78,6 +78,39 @@ def test_scatter_dropin_vs_oracle(mode):
     _exact(perm, operm, "sort order")
 
 
+def test_scatter_material_vectors_by_type_and_fallback():
+    """The seven material vectors that the reference derives from the marker
+    type (E:659-665: 1/G, cohesion, tension, friction, fluid density and the
+    two expansivities) are read through tm while every marker holds its
+    type's values (checked on the device after any of them changes); one
+    marker that does not makes the scatter read the vectors again.  Each
+    state against the oracle on its own data."""
+    g, mat = ereb.synth.workload_geometry("cpu41", mode=1)
+    mk = ereb.synth.make_markers(g, mat, seed=1, stress=True)
+    hrs, hrf = mat.radioactive_heating(START_TIME)
+    orc = Oracle(g, mat)
+    coh = mk["cohestotalm"].copy()
+    coh[123] += 7.5e7
+    rho = mk["rhofluidcur"].copy()
+    rho[-5] *= 1.5
+    states = [("typed", {}), ("one marker differs", {"cohestotalm": coh}),
+              ("another differs", {"cohestotalm": mk["cohestotalm"], "rhofluidcur": rho}),
+              ("typed again", {"rhofluidcur": mk["rhofluidcur"]})]
+    data = dict(mk)
+    with _ctx(g, mat) as ctx:
+        ctx.set_markers(mk)
+        for name, change in states:
+            for k, v in change.items():
+                ctx.set_marker(k, v)
+                data[k] = v
+            ctx.step_scatter(hrs, hrf, 1)
+            sums = [ctx.get_grid(n) for n in ereb.SUM_FIELDS]
+            props_o, sums_o = orc.marker_to_nodes(data, hrs, hrf, 1)
+            absb = cases.abs_sums(g, data, props_o)
+            for n, a, b, c in zip(ereb.SUM_FIELDS, sums, sums_o, absb):
+                _sum_close(a, b, c, f"{name}: {n}")
+
+
 def test_scatter_unknown_mode_raises():
     g, mat = tconsts()
     mk = cases.ref_markers(g, mat, extra=10)
