@@ -332,6 +332,19 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
     } while (0)
     CK(hipSetDevice(device));
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        // the second stream with every CU enabled: a stream with a CU mask gets
+        // a hardware queue of its own, while plain streams share the runtime's
+        // few queues by usage, so that with other streams open in the process
+        // both streams of a context could land on one queue and run in turn
+        hipDeviceProp_t prop;
+        CK(hipGetDeviceProperties(&prop, device));
+        std::vector<uint32_t> all((size_t)(prop.multiProcessorCount + 31) / 32, ~0u);
+        CK(hipExtStreamCreateWithCUMask(&c->stream2, (uint32_t)all.size(), all.data()));
+    }
+    CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_scatter, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     // the no-alloc entry point's scratch on this device (once per device); a
     // failure here is reported by that entry point, which tries again
     if (noalloc_init(c)) c->err.clear();
@@ -403,7 +416,8 @@ int ereb_ctx_destroy(ereb_ctx* c) {
     hipFree(c->d_scal);
     hipFree(c->d_rkd);
     hipFree(c->pb_pool);
-    if (c->step_exec) hipGraphExecDestroy(c->step_exec);
+    for (hipGraphExec_t e : c->step_exec)
+        if (e) hipGraphExecDestroy(e);
     hipFree(c->adj_pool);
     hipFree(c->rep_pool);
     hipFree(c->d_grids);
@@ -435,6 +449,9 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.chg);
         hipFree(s.map);
     }
+    for (hipEvent_t e : {c->ev_fork, c->ev_scatter, c->ev_join})
+        if (e) hipEventDestroy(e);
+    if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return EREB_OK;
@@ -608,15 +625,124 @@ int ereb_step_scatter(ereb_ctx* c, const double hrsolidm[3], const double hrflui
 // (E:6419-6517), node properties + insulating tk1 boundary (E:6522-6604),
 // Stokes-Darcy and thermal assembly (E:6680, E:6922), stress and
 // temperature gathers (E:6980, E:7004), RK4 (E:7036).
-static int transfer_launches(ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
-                             double dt, int64_t timestep) {
+// Launches of c->stream go to `s` while it lives (the launch helpers all use
+// c->stream)
+struct OnStream {
+    ereb_ctx* c;
+    hipStream_t s0;
+    OnStream(ereb_ctx* c_, hipStream_t s) : c(c_), s0(c_->stream) { c->stream = s; }
+    ~OnStream() { c->stream = s0; }
+};
+
+// The step in four segments on two streams.  The stress and temperature
+// gathers and RK4 (one pass over the markers) read the velocity, rotation,
+// stress-change and temperature grids given to the step, which no kernel of
+// the step writes, and write marker state that the sort and the scatter read;
+// the node pass and the two assemblies read the scatter's window sums and
+// node grids only.  So the marker pass (SEG_MARKERS) runs on the second
+// stream as soon as the scatter (SEG_SCATTER) is done, beside the node pass
+// and the assemblies (SEG_NODES), and its coefficient tables (SEG_TABLES, from
+// the velocities alone) from the start of the step, beside the sort; the main
+// stream waits for the second at the end.  Results are those of the launches
+// in sequence: no kernel of one stream touches what the other's write.
+enum { SEG_TABLES = 0, SEG_SCATTER, SEG_MARKERS, SEG_NODES };
+
+static int seg_launches(ereb_ctx* c, int seg, const double* hrs, const double* hrf,
+                        int64_t mode, double dt, int64_t timestep) {
     int rc;
-    if ((rc = launch_scatter(c, hrs, hrf, (int)mode))) return rc;
-    if ((rc = launch_combine_nodes(c))) return rc;
-    if ((rc = launch_assemble_stokes(c, dt))) return rc;
-    if ((rc = launch_assemble_thermal(c, dt))) return rc;
-    // the stress and temperature gathers and RK4 in one pass over the markers
-    return launch_gathers_rk4(c, dt, (int)mode, timestep);
+    switch (seg) {
+        case SEG_TABLES: return launch_gathers_rk4(c, dt, (int)mode, timestep, RK4_TABLES);
+        case SEG_SCATTER: return launch_scatter(c, hrs, hrf, (int)mode);
+        case SEG_MARKERS: return launch_gathers_rk4(c, dt, (int)mode, timestep, RK4_MARKERS);
+        default:
+            if ((rc = launch_combine_nodes(c))) return rc;
+            if ((rc = launch_assemble_stokes(c, dt))) return rc;
+            return launch_assemble_thermal(c, dt);
+    }
+}
+
+// one segment on stream s: its launches, or the replay of its captured graph
+static int seg_run(ereb_ctx* c, int seg, hipStream_t s, bool replay, const double* hrs,
+                   const double* hrf, int64_t mode, double dt, int64_t timestep) {
+    if (replay) {
+        // a segment that launches nothing (no markers) has no graph
+        if (c->step_exec[seg]) EREB_HIP(c, hipGraphLaunch(c->step_exec[seg], s));
+        return EREB_OK;
+    }
+    OnStream on(c, s);
+    return seg_launches(c, seg, hrs, hrf, mode, dt, timestep);
+}
+
+static int transfer_run(ereb_ctx* c, bool replay, const double* hrs, const double* hrf,
+                        int64_t mode, double dt, int64_t timestep) {
+    int rc;
+    // with per-launch timing on: every launch in sequence on the main stream,
+    // the coefficient tables and the marker pass timed together as one class
+    if (c->prof) {
+        for (int seg : {SEG_SCATTER, SEG_NODES})
+            if ((rc = seg_launches(c, seg, hrs, hrf, mode, dt, timestep))) return rc;
+        return launch_gathers_rk4(c, dt, (int)mode, timestep);
+    }
+    hipStream_t s1 = c->stream, s2 = c->stream2;
+    EREB_HIP(c, hipEventRecord(c->ev_fork, s1));
+    EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
+    rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
+    if (!rc) rc = seg_run(c, SEG_SCATTER, s1, replay, hrs, hrf, mode, dt, timestep);
+    if (!rc) {
+        EREB_HIP(c, hipEventRecord(c->ev_scatter, s1));
+        EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_scatter, 0));
+        rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);
+    }
+    if (!rc) rc = seg_run(c, SEG_NODES, s1, replay, hrs, hrf, mode, dt, timestep);
+    // the join also after a failure: later work on the main stream follows
+    // everything the second stream was given
+    EREB_HIP(c, hipEventRecord(c->ev_join, s2));
+    EREB_HIP(c, hipStreamWaitEvent(s1, c->ev_join, 0));
+    return rc;
+}
+
+static void step_graphs_free(ereb_ctx* c) {
+    for (hipGraphExec_t& e : c->step_exec) {
+        if (e) hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+    c->step_graphs = false;
+}
+
+// each segment captured on the main stream as a graph of its own (a graph of
+// one stream's launches replays them back to back)
+static int step_capture(ereb_ctx* c, const double* hrs, const double* hrf, int64_t mode,
+                        double dt, int64_t timestep) {
+    step_graphs_free(c);
+    const uint64_t pos0 = c->pos_gen;
+    for (int seg = 0; seg < NSTEP_SEG; ++seg) {
+        hipGraph_t g = nullptr;
+        EREB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        const int rc = seg_launches(c, seg, hrs, hrf, mode, dt, timestep);
+        c->capturing = false;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            step_graphs_free(c);
+            c->pos_gen = pos0;
+            return rc ? rc : hipfail(c, e, "hipStreamEndCapture");
+        }
+        size_t nodes = 0;
+        hipError_t ei = hipGraphGetNodes(g, nullptr, &nodes);
+        if (ei == hipSuccess && nodes > 0)
+            ei = hipGraphInstantiate(&c->step_exec[seg], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->step_exec[seg] = nullptr;
+            step_graphs_free(c);
+            c->pos_gen = pos0;
+            return hipfail(c, ei, "hipGraphInstantiate");
+        }
+    }
+    c->pos_gen = pos0;  // the captured launches have not run yet
+    c->step_graphs = true;
+    return EREB_OK;
 }
 
 // dt, the heating rates and the first-timestep flag are not part of the key:
@@ -662,38 +788,17 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     // eager: when asked, while profiling (events around every launch), and
     // for the first step (it makes the allocations a capture must not)
     if (!c->use_graphs || c->prof || !c->step_warm) {
-        rc = transfer_launches(c, hrsolidm, hrfluidm, mode, dt, timestep);
+        rc = transfer_run(c, false, hrsolidm, hrfluidm, mode, dt, timestep);
         if (!rc) c->step_warm = true;
         return rc;
     }
     const StepKey k = step_key(c, mode);
-    if (!c->step_exec || std::memcmp(&k, &c->step_key, sizeof k) != 0) {
-        if (c->step_exec) {
-            hipGraphExecDestroy(c->step_exec);
-            c->step_exec = nullptr;
-        }
-        const uint64_t pos0 = c->pos_gen;
-        hipGraph_t g = nullptr;
-        EREB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = true;
-        rc = transfer_launches(c, hrsolidm, hrfluidm, mode, dt, timestep);
-        c->capturing = false;
-        const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc || e != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return rc ? rc : hipfail(c, e, "hipStreamEndCapture");
-        }
-        c->pos_gen = pos0;  // the captured launches have not run yet
-        const hipError_t ei = hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (ei != hipSuccess) {
-            c->step_exec = nullptr;
-            return hipfail(c, ei, "hipGraphInstantiate");
-        }
+    if (!c->step_graphs || std::memcmp(&k, &c->step_key, sizeof k) != 0) {
+        if ((rc = step_capture(c, hrsolidm, hrfluidm, mode, dt, timestep))) return rc;
         c->step_key = k;
         ++c->step_captures;
     }
-    EREB_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    if ((rc = transfer_run(c, true, hrsolidm, hrfluidm, mode, dt, timestep))) return rc;
     ++c->step_replays;
     // host state of the replayed step, as the eager launches leave it: the
     // sort is of the positions before RK4, which then moved them

@@ -86,11 +86,20 @@ struct StepKey {
     const void* bufs[EREB_M_NFIELDS + 19];
 };
 
+// segments of the transfer step, each captured as a graph of its own
+// (ereb_step_transfer)
+constexpr int NSTEP_SEG = 4;
+
 }  // namespace ereb
 
 struct ereb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the transfer step (default priority, like `stream`):
+    // the gathers and RK4 run there beside the node pass and the assemblies;
+    // fork, scatter-done and join events
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_scatter = nullptr, ev_join = nullptr;
     int precision = EREB_FP64;
     int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
     double dx = 0, dy = 0;
@@ -160,10 +169,14 @@ struct ereb_ctx {
     int32_t* rep_mnum = nullptr;
     int64_t rep_n = 0;
 
-    // HIP graph of the resident transfer step (ereb_step_transfer)
-    bool use_graphs = true;
+    // graph replay of the resident transfer step (ereb_step_transfer), off
+    // by default: its launches on the two streams take less time at every
+    // size measured
+    bool use_graphs = false;
     bool step_warm = false;  // one eager step has run (allocations made)
-    hipGraphExec_t step_exec = nullptr;
+    // one graph per segment of the step, replayed on the two streams
+    hipGraphExec_t step_exec[ereb::NSTEP_SEG] = {};
+    bool step_graphs = false;  // step_exec holds the graphs of step_key
     ereb::StepKey step_key{};
     int64_t step_captures = 0, step_replays = 0;
 
@@ -259,8 +272,12 @@ int launch_gather_stress(ereb_ctx* c);
 // xb_out/yb_out (optional): fluid back-trace end points, for the adjoint
 int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out = nullptr,
                double* yb_out = nullptr);
-// stress gather, temperature gather (timestep) and RK4 in one kernel
-int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep);
+// stress gather, temperature gather (timestep) and RK4 in one kernel.  parts:
+// RK4_TABLES launches the stage-velocity coefficient tables (they depend on
+// the velocity grids only), RK4_MARKERS the marker kernel, which reads them
+constexpr int RK4_TABLES = 1, RK4_MARKERS = 2;
+int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep,
+                       int parts = RK4_TABLES | RK4_MARKERS);
 
 // ereb_assembly.hip
 int launch_assemble_stokes(ereb_ctx* c, double dt);

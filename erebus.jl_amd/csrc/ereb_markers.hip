@@ -822,7 +822,7 @@ int launch_rk4(ereb_ctx* c, double dt, int mode, double* xb_out, double* yb_out)
     return EREB_OK;
 }
 
-int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
+int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep, int parts) {
     const int64_t M = c->marknum;
     if (M <= 0) return EREB_OK;
     if (c->Nx < 4 || c->Ny < 4)
@@ -835,11 +835,13 @@ int launch_gathers_rk4(ereb_ctx* c, double dt, int mode, int64_t timestep) {
     A.DTg = gptr(c, EREB_G_DT);
     hipEvent_t ev;
     prof_begin(c, EREB_K_RK4, &ev);
-    launch_rk4_prep(c, A);
+    if (parts & RK4_TABLES) launch_rk4_prep(c, A);
     // node coordinates computed (exact axes) or loaded, chosen per kernel
-    if (c->precision == EREB_MIXED) launch_rk4_kernel<2>(c, A);
-    else launch_rk4_kernel<1>(c, A);
-    ++c->pos_gen;
+    if (parts & RK4_MARKERS) {
+        if (c->precision == EREB_MIXED) launch_rk4_kernel<2>(c, A);
+        else launch_rk4_kernel<1>(c, A);
+        ++c->pos_gen;
+    }
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_RK4, ev);
     return EREB_OK;

@@ -349,7 +349,8 @@ class Context:
                    "step_scatter")
 
     def step_transfer(self, hrsolidm, hrfluidm, mode: int, dt: float, timestep: int):
-        """The whole resident transfer step (graph-replayed after the first)."""
+        """The whole resident transfer step (on the context's two streams; graph
+        replay when set_graphs(True))."""
         hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
         hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
         self.check(self.lib.ereb_step_transfer(self.h, _ptr(hs), _ptr(hf), int(mode), float(dt),

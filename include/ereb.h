@@ -230,16 +230,20 @@ int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
  * ereb_step_assemble_stokes_darcy, ereb_step_assemble_thermal,
  * ereb_step_gather_stress, ereb_step_gather_temperature(timestep),
  * ereb_step_advect_rk4 (E:6419-7053 without the host solves); the two
- * gathers and RK4 run as one kernel (kernel class EREB_K_RK4).  After the
- * first call the step's launches are captured once into a HIP graph and
- * replayed; the graph is captured again when the mode, the marker count,
- * the precision form or a device buffer changes (not when dt, the heating
- * rates or the timestep change).  Results are those of the separate calls,
- * bit for bit.  Eager while profiling is enabled. */
+ * gathers and RK4 run as one kernel (kernel class EREB_K_RK4), on a second
+ * stream of the context once the scatter is done, beside the node pass and
+ * the assemblies, which read none of the marker state it writes; the call
+ * returns with the context's stream ordered after both.  With graph replay
+ * on (ereb_ctx_set_graphs), after the first call the step's launches are
+ * captured once into four graphs (the two streams' parts) and replayed; they
+ * are captured again when the mode, the marker count, the precision form or
+ * a device buffer changes (not when dt, the heating rates or the timestep
+ * change).  Results are those of the separate calls, bit for bit.  Every
+ * launch in sequence on the context's stream while profiling is enabled. */
 int ereb_step_transfer(ereb_ctx* ctx, const double hrsolidm[3],
                        const double hrfluidm[3], int64_t mode, double dt,
                        int64_t timestep);
-/* graph replay of ereb_step_transfer on (default) or off */
+/* graph replay of ereb_step_transfer on or off (default) */
 int ereb_ctx_set_graphs(ereb_ctx* ctx, int on);
 /* number of captures of the step graph and of replays since the context was
  * created.  dt, hrsolidm/hrfluidm and timestep == 1 are read by the kernels
