@@ -369,15 +369,16 @@ static int absmax_bits(ereb_ctx* c, const double* a, int64_t n, unsigned long lo
     return EREB_OK;
 }
 
-// the buffers of one call, carved from the context's grow-only pool
+// the buffers of one call, carved from the context's grow-only pool; nsnap:
+// grid snapshots kept (every step's, or one segment's)
 static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int64_t every,
-                       size_t gsz) {
+                       int64_t nsnap, size_t gsz) {
     const size_t M = (size_t)c->marknum;
     const size_t np = (size_t)c->Nx1 * c->Ny1;
     const size_t sizes[11] = {5 * M,
                               grad ? 3 * M * (size_t)nseg : 0,
                               grad ? 3 * M * (size_t)every : 0,
-                              grad ? gsz * (size_t)every : 0,
+                              grad ? gsz * (size_t)nsnap : 0,
                               grad ? 2 * M * (size_t)every : 0,
                               grad ? M : 0, grad ? M : 0, grad ? M : 0, grad ? gsz : 0,
                               grad ? 8 + 4 * np : 0,
@@ -406,8 +407,9 @@ static int adj_buffers(ereb_ctx* c, AdjBuffers& B, bool grad, int64_t nseg, int6
 }
 
 static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t every, int64_t nseg,
-                        double dt, int md, const double* hrsolidm, const double* hrfluidm,
-                        double* loss, double* tk1_out, double* grad_tkm, double* grad_phim);
+                        bool keep_grids, double dt, int md, const double* hrsolidm,
+                        const double* hrfluidm, double* loss, double* tk1_out, double* grad_tkm,
+                        double* grad_phim);
 
 }  // namespace ereb
 
@@ -415,9 +417,19 @@ using namespace ereb;
 
 // Forward with checkpoints of (xm, ym, tkm) every `every` steps; the reverse
 // sweep recomputes each segment from its checkpoint, recording per step the
-// start state, the grid fields and the fluid back-trace end points (the last
-// segment is recorded during the forward pass itself), then runs the adjoint
-// kernels backwards through the segment.
+// start state, the sort order, the grid fields and the fluid back-trace end
+// points (the last segment is recorded during the forward pass itself), then
+// runs the adjoint kernels backwards through the segment.  The grid fields
+// are grid-sized: while every step's fit the context's budget, the forward
+// pass keeps them all, and the recomputation runs only the marker side of
+// each step (sort of the start positions, temperature gather at timestep 1,
+// RK4) instead of the scatter and the node pass again.
+extern "C" int ereb_adjoint_set_grid_budget(ereb_ctx* c, int64_t bytes) {
+    if (!c || bytes < 0) return EREB_ERR_ARG;
+    c->adj_grid_budget = bytes;
+    return EREB_OK;
+}
+
 extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every, double dt,
                                      int64_t mode, const double hrsolidm[3],
                                      const double hrfluidm[3], double* loss, double* tk1_out,
@@ -431,15 +443,18 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
     if (M <= 0) return fail(c, EREB_ERR_STATE, "no markers");
     const int md = (int)mode;
     const int64_t nseg = (nsteps + every - 1) / every;
+    const int64_t seg = std::min(every, std::max<int64_t>(nsteps, 1));
     const size_t gsz = grid_doubles(c);
+    const bool keep_grids =
+        grad && nsteps > seg &&
+        (double)gsz * sizeof(double) * (double)nsteps <= (double)c->adj_grid_budget;
     AdjBuffers B;
     int rc;
-    if ((rc = adj_buffers(c, B, grad, nseg, std::min(every, std::max<int64_t>(nsteps, 1)), gsz)))
-        return rc;
+    if ((rc = adj_buffers(c, B, grad, nseg, seg, keep_grids ? nsteps : seg, gsz))) return rc;
     // the markers are restored at the end, on every exit path once saved
     if ((rc = state_copy(c, B.save, 5, true))) return rc;
-    rc = adjoint_body(c, B, nsteps, every, nseg, dt, md, hrsolidm, hrfluidm, loss, tk1_out,
-                      grad_tkm, grad_phim);
+    rc = adjoint_body(c, B, nsteps, every, nseg, keep_grids, dt, md, hrsolidm, hrfluidm, loss,
+                      tk1_out, grad_tkm, grad_phim);
     const std::string err = c->err;
     const int rc_restore = state_copy(c, B.save, 5, false);
     ++c->pos_gen;  // the last sort was of moved positions
@@ -456,36 +471,63 @@ extern "C" int ereb_adjoint_tk1_loss(ereb_ctx* c, int64_t nsteps, int64_t every,
 namespace ereb {
 
 static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t every, int64_t nseg,
-                        double dt, int md, const double* hrsolidm, const double* hrfluidm,
-                        double* loss, double* tk1_out, double* grad_tkm, double* grad_phim) {
+                        bool keep_grids, double dt, int md, const double* hrsolidm,
+                        const double* hrfluidm, double* loss, double* tk1_out, double* grad_tkm,
+                        double* grad_phim) {
     const bool grad = grad_tkm || grad_phim;
     const int64_t M = c->marknum;
     const size_t MB = sizeof(double) * (size_t)M;
     const size_t gsz = grid_doubles(c);
     const size_t GB = sizeof(double) * gsz;
     int rc;
-    // record step k of a segment starting at k0: start state before it,
-    // grids and back-trace end points from it
+    // the records of step k in the segment starting at k0: start state, fluid
+    // back-trace end points, sort order (one segment's), grid fields (every
+    // step's with keep_grids, else one segment's)
+    auto st_of = [&](int64_t k, int64_t k0) { return B.st + 3 * (size_t)M * (k - k0); };
+    auto xb_of = [&](int64_t k, int64_t k0) { return B.xb + 2 * (size_t)M * (k - k0); };
+    auto perm_of = [&](int64_t k, int64_t k0) { return B.perm + (size_t)M * (k - k0); };
+    auto snap_of = [&](int64_t k, int64_t k0) {
+        return B.snap + gsz * (size_t)(keep_grids ? k : k - k0);
+    };
+    // forward step k of the segment starting at k0; record: the step's records
+    // (keep_grids: its grid fields in any case)
     auto run_step = [&](int64_t k, int64_t k0, bool record) -> int {
         int r;
         double* xb = nullptr;
         double* yb = nullptr;
         if (record) {
-            if ((r = state_copy(c, B.st + 3 * (size_t)M * (k - k0), 3, true))) return r;
-            xb = B.xb + 2 * (size_t)M * (k - k0);
+            if ((r = state_copy(c, st_of(k, k0), 3, true))) return r;
+            xb = xb_of(k, k0);
             yb = xb + M;
         }
         if ((r = adj_forward_step(c, k, dt, md, hrsolidm, hrfluidm, xb, yb))) return r;
-        if (record) {
-            EREB_HIP(c, hipMemcpyAsync(B.snap + gsz * (size_t)(k - k0), c->d_grids, GB,
-                                       hipMemcpyDeviceToDevice, c->stream));
+        if (record || keep_grids)
+            EREB_HIP(c, hipMemcpyAsync(snap_of(k, k0), c->d_grids, GB, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        if (record)
             // the step's scatter sorted its start positions (RK4 moved them
             // afterwards without a new sort)
-            EREB_HIP(c, hipMemcpyAsync(B.perm + (size_t)M * (k - k0), c->d_perm,
-                                       sizeof(uint32_t) * (size_t)M, hipMemcpyDeviceToDevice,
-                                       c->stream));
-        }
+            EREB_HIP(c, hipMemcpyAsync(perm_of(k, k0), c->d_perm, sizeof(uint32_t) * (size_t)M,
+                                       hipMemcpyDeviceToDevice, c->stream));
         return EREB_OK;
+    };
+    // step k recomputed from its start state with its grid fields kept: only
+    // the marker side runs -- the sort of the start positions (the order the
+    // transposed kernels walk), the temperature gather at timestep 1 and RK4
+    // on the step's grids, recording the back-trace end points -- with the
+    // results of run_step bit for bit (the scatter and the node pass only
+    // produce the grid fields, and the sort depends on the positions alone)
+    auto replay_step = [&](int64_t k, int64_t k0) -> int {
+        int r;
+        if ((r = state_copy(c, st_of(k, k0), 3, true))) return r;
+        EREB_HIP(c, hipMemcpyAsync(c->d_grids, snap_of(k, k0), GB, hipMemcpyDeviceToDevice,
+                                   c->stream));
+        if ((r = launch_fine_sort(c, nullptr))) return r;
+        EREB_HIP(c, hipMemcpyAsync(perm_of(k, k0), c->d_perm, sizeof(uint32_t) * (size_t)M,
+                                   hipMemcpyDeviceToDevice, c->stream));
+        if (k == 0 && (r = launch_gather_temperature(c, 1))) return r;
+        double* xb = xb_of(k, k0);
+        return launch_rk4(c, dt, md, xb, xb + M);
     };
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t sg = k / every;
@@ -532,13 +574,13 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
             if (sg != nseg - 1) {  // the last segment was recorded by the forward pass
                 if ((rc = state_copy(c, B.cps + 3 * (size_t)M * sg, 3, false))) return rc;
                 for (int64_t k = k0; k < k1; ++k)
-                    if ((rc = run_step(k, k0, true))) return rc;
+                    if ((rc = keep_grids ? replay_step(k, k0) : run_step(k, k0, true))) return rc;
             }
             for (int64_t k = k1 - 1; k >= k0; --k) {
-                double* stk = B.st + 3 * (size_t)M * (k - k0);
-                const double* xb = B.xb + 2 * (size_t)M * (k - k0);
+                double* stk = st_of(k, k0);
+                const double* xb = xb_of(k, k0);
                 // the grids and the start state of step k
-                EREB_HIP(c, hipMemcpyAsync(c->d_grids, B.snap + gsz * (size_t)(k - k0), GB,
+                EREB_HIP(c, hipMemcpyAsync(c->d_grids, snap_of(k, k0), GB,
                                            hipMemcpyDeviceToDevice, c->stream));
                 if ((rc = state_copy(c, stk, 3, false))) return rc;
                 if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
@@ -556,7 +598,7 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 // the transposed kernels walk the markers in the step's sort
                 // order: a workgroup's stencils then cover a compact patch of
                 // nodes, and its window needs few device-wide atomics
-                const uint32_t* order = B.perm + (size_t)M * (k - k0);
+                const uint32_t* order = perm_of(k, k0);
                 if ((rc = absmax_bits(c, B.gT, M, FA.maxbits)) ||
                     (rc = launch_rk4_adj(c, 1, dt, md, xb, xb + M, B.gT, B.gT2, B.gP, FA, flag,
                                          order)))

@@ -157,6 +157,9 @@ struct ereb_ctx {
 
     double* adj_pool = nullptr;  // grow-only device pool of the adjoint driver
     size_t adj_pool_n = 0;       // doubles
+    // most bytes of per-step grid fields the adjoint driver keeps
+    // (ereb_adjoint_set_grid_budget)
+    int64_t adj_grid_budget = int64_t(4) << 30;
     void* pb_pool = nullptr;     // grow-only device pool of the pullbacks and the
                                  // save_state copy (widened marker vectors)
     size_t pb_bytes = 0;

@@ -120,6 +120,7 @@ PROTOTYPES = {
     "ereb_assemble_thermal_lse": [_P] + [_P] * 8 + [_P, _D, _P],
     "ereb_assemble_thermal_lse_noalloc": [_P] + [_P] * 8 + [_P, _D, _P, _P, _P, _I64, _P],
     "ereb_adjoint_tk1_loss": [_P, _I64, _I64, _D, _I64, _P, _P, _P, _P, _P, _P],
+    "ereb_adjoint_set_grid_budget": [_P, _I64],
     "ereb_step_transfer": [_P, _P, _P, _I64, _D, _I64],
     "ereb_ctx_set_graphs": [_P, _I],
     "ereb_step_graph_stats": [_P, _P, _P],
@@ -506,6 +507,11 @@ class Context:
             self.h, int(nsteps), int(checkpoint_every), float(dt), int(mode), _ptr(hs),
             _ptr(hf), ctypes.addressof(L), _ptr(tk1), _ptr(gT), _ptr(gP)), "adjoint")
         return L.value, gT, gP, tk1
+
+    def adjoint_set_grid_budget(self, nbytes: int):
+        """Most bytes of per-step grid fields the adjoint keeps (0: recompute
+        every step in full)."""
+        self.check(self.lib.ereb_adjoint_set_grid_budget(self.h, int(nbytes)), "grid budget")
 
     # ---- profiling --------------------------------------------------------
     def profile(self, on: bool = True):

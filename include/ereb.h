@@ -396,7 +396,11 @@ int ereb_assemble_thermal_lse_noalloc(ereb_ctx* ctx, const double* tk1,
  * final state.  The reverse pass runs the hand-written adjoint kernels
  * (scatter^T as a gather, node properties^T, gather^T and RK4^T), recomputing
  * the forward steps from checkpoints of (xm, ym, tkm) taken every
- * `checkpoint_every` steps.  Positions do not depend on tkm or phim.
+ * `checkpoint_every` steps.  While the grid fields of every step fit the
+ * context's grid budget (ereb_adjoint_set_grid_budget), the forward pass keeps
+ * them, and a recomputed step runs only its sort, temperature gather and RK4;
+ * otherwise it runs the scatter and the node pass again.  The gradient is the
+ * same bit for bit either way.  Positions do not depend on tkm or phim.
  * loss: required; tk1_out (Ny1*Nx1, column-major), grad_tkm and grad_phim
  * (marknum) may be NULL; with both gradients NULL only the forward map runs.
  * The resident markers are left as they were.  Float64 form only for the
@@ -405,6 +409,11 @@ int ereb_adjoint_tk1_loss(ereb_ctx* ctx, int64_t nsteps, int64_t checkpoint_ever
                           double dt, int64_t mode, const double hrsolidm[3],
                           const double hrfluidm[3], double* loss, double* tk1_out,
                           double* grad_tkm, double* grad_phim);
+/* the most device memory, in bytes, that ereb_adjoint_tk1_loss keeps for the
+ * grid fields of every step (default 4 GiB; nsteps times the size of the
+ * context's grid fields is compared with it); 0 recomputes every step in
+ * full.  Marker-sized records stay one checkpoint segment's either way. */
+int ereb_adjoint_set_grid_budget(ereb_ctx* ctx, int64_t bytes);
 
 /* ---- pullbacks: the reverse-mode kernels behind ChainRulesCore rrules --- */
 /* Each takes the forward call's inputs (host arrays, as the drop-in entry

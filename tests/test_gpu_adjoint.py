@@ -106,6 +106,27 @@ def test_adjoint_checkpoint_spacing_and_state(adj41):
         assert np.array_equal(a, gP1)
 
 
+def test_adjoint_grid_budget_paths_agree(adj41):
+    """While every step's grid fields fit the budget, a recomputed step runs
+    only its sort, temperature gather and RK4; with a budget of 0 it runs the
+    scatter and the node pass again.  Same loss, tk1 and gradient bits, in
+    both property modes; the markers are left as they were."""
+    g, mat, mk, hrs, hrf = adj41
+    with ereb.Context(g, mat, ereb.FP64) as ctx:
+        _setup(ctx, g, mk)
+        for mode in (1, 9):
+            ctx.adjoint_set_grid_budget(4 << 30)
+            La, gTa, gPa, tka = ctx.adjoint_tk1_loss(6, 2, DT, mode, hrs, hrf, want_tk1=True)
+            ctx.adjoint_set_grid_budget(0)
+            Lb, gTb, gPb, tkb = ctx.adjoint_tk1_loss(6, 2, DT, mode, hrs, hrf, want_tk1=True)
+            assert La == Lb, mode
+            assert np.array_equal(tka, tkb), mode
+            assert np.array_equal(gTa, gTb) and np.array_equal(gPa, gPb), mode
+            assert np.all(np.isfinite(gTa)) and np.abs(gTa).max() > 0
+        for n in ("xm", "ym", "tkm", "sxxm", "sxym"):
+            assert np.array_equal(ctx.get_marker(n), mk[n]), n
+
+
 def test_adjoint_vs_dual_exact_gradient(adj41):
     """Per entry against the exact (dual-number) gradient of the oracle's map,
     and bit-identical on a second run."""
