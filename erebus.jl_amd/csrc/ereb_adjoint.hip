@@ -106,9 +106,7 @@ struct NodesAdjArgs {
     double* gsum[10];         // RHOSUM ... PHISUM, WTPSUM (set to 0)
 };
 
-__global__ __launch_bounds__(256) void k_nodes_adj_p(NodesAdjArgs A) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.n) return;
+__device__ __forceinline__ void nodes_adj_p(const NodesAdjArgs& A, int k) {
     const double wt = A.WTPSUM[k];
     double out[10] = {};
     if (wt > 0.0) {
@@ -135,22 +133,29 @@ __global__ __launch_bounds__(256) void k_nodes_adj_p(NodesAdjArgs A) {
 }
 
 // nodes^T on a Vx or Vy grid: RHO RHOF K PHI R = SUM * iw
-__global__ __launch_bounds__(256) void k_nodes_adj_v(int n, const double* __restrict__ wts,
-                                                     const double* g0, const double* g1,
-                                                     const double* g2, const double* g3,
-                                                     const double* g4, double* s0, double* s1,
-                                                     double* s2, double* s3, double* s4,
-                                                     double* s5) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+struct NodesAdjV {
+    const double* wts;   // the grid's weight sum
+    const double* g[5];  // cotangents of RHO RHOF K PHI R
+    double* s[6];        // of their SUM grids and of the weight sum (set to 0)
+};
+
+__device__ __forceinline__ void nodes_adj_v(const double* __restrict__ wts,
+                                            const double* const (&g)[5], double* const (&s)[6],
+                                            int k) {
     const double wt = wts[k];
     const double iw = wt > 0.0 ? 1.0 / wt : 0.0;
-    s0[k] = g0[k] * iw;
-    s1[k] = g1[k] * iw;
-    s2[k] = g2[k] * iw;
-    s3[k] = g3[k] * iw;
-    s4[k] = g4[k] * iw;
-    s5[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) s[q][k] = g[q][k] * iw;
+    s[5][k] = 0.0;
+}
+
+// grid row 0: the P grid, 1: Vx, 2: Vy
+__global__ __launch_bounds__(256) void k_nodes_adj(NodesAdjArgs A, NodesAdjV X, NodesAdjV Y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.n) return;
+    if (blockIdx.y == 0) nodes_adj_p(A, k);
+    else if (blockIdx.y == 1) nodes_adj_v(X.wts, X.g, X.s, k);
+    else nodes_adj_v(Y.wts, Y.g, Y.s, k);
 }
 
 // insulating boundary copy on a (Nyy, Nxx) array, transposed: the forward
@@ -242,17 +247,16 @@ int launch_nodes_adj(ereb_ctx* c, double* gbase, const int64_t* off) {
         A.gnode[q] = G(pin[q]);
         A.gsum[q] = G(pout[q]);
     }
-    hipLaunchKernelGGL(k_nodes_adj_p, dim3(nb), dim3(256), 0, c->stream, A);
-    hipLaunchKernelGGL(k_nodes_adj_v, dim3(nb), dim3(256), 0, c->stream, n,
-                       (const double*)gptr(c, EREB_G_WTXSUM), G(EREB_G_RHOX), G(EREB_G_RHOFX),
-                       G(EREB_G_KX), G(EREB_G_PHIX), G(EREB_G_RX), G(EREB_G_RHOXSUM),
-                       G(EREB_G_RHOFXSUM), G(EREB_G_KXSUM), G(EREB_G_PHIXSUM),
-                       G(EREB_G_RXSUM), G(EREB_G_WTXSUM));
-    hipLaunchKernelGGL(k_nodes_adj_v, dim3(nb), dim3(256), 0, c->stream, n,
-                       (const double*)gptr(c, EREB_G_WTYSUM), G(EREB_G_RHOY), G(EREB_G_RHOFY),
-                       G(EREB_G_KY), G(EREB_G_PHIY), G(EREB_G_RY), G(EREB_G_RHOYSUM),
-                       G(EREB_G_RHOFYSUM), G(EREB_G_KYSUM), G(EREB_G_PHIYSUM),
-                       G(EREB_G_RYSUM), G(EREB_G_WTYSUM));
+    const NodesAdjV X{gptr(c, EREB_G_WTXSUM),
+                      {G(EREB_G_RHOX), G(EREB_G_RHOFX), G(EREB_G_KX), G(EREB_G_PHIX), G(EREB_G_RX)},
+                      {G(EREB_G_RHOXSUM), G(EREB_G_RHOFXSUM), G(EREB_G_KXSUM), G(EREB_G_PHIXSUM),
+                       G(EREB_G_RXSUM), G(EREB_G_WTXSUM)}};
+    const NodesAdjV Y{gptr(c, EREB_G_WTYSUM),
+                      {G(EREB_G_RHOY), G(EREB_G_RHOFY), G(EREB_G_KY), G(EREB_G_PHIY), G(EREB_G_RY)},
+                      {G(EREB_G_RHOYSUM), G(EREB_G_RHOFYSUM), G(EREB_G_KYSUM), G(EREB_G_PHIYSUM),
+                       G(EREB_G_RYSUM), G(EREB_G_WTYSUM)}};
+    // the P, Vx and Vy grids are all Ny1 x Nx1
+    hipLaunchKernelGGL(k_nodes_adj, dim3(nb, 3), dim3(256), 0, c->stream, A, X, Y);
     EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
@@ -271,16 +275,64 @@ __global__ void k_twice(const double* __restrict__ a, double* __restrict__ b, in
 static const int kStateFields[5] = {EREB_M_XM, EREB_M_YM, EREB_M_TKM, EREB_M_SXXM,
                                     EREB_M_SXYM};
 
+// up to five vectors of n doubles copied in one launch (grid row y = vector y)
+struct CopyList {
+    const double* src[5];
+    double* dst[5];
+};
+
+__global__ __launch_bounds__(256) void k_copy_list(CopyList L, int64_t n) {
+    // the row's pair by selects (no dynamic indexing of the argument)
+    const int y = blockIdx.y;
+    const double* __restrict__ s = y == 0 ? L.src[0] : y == 1 ? L.src[1] : y == 2 ? L.src[2]
+                                 : y == 3 ? L.src[3] : L.src[4];
+    double* __restrict__ d = y == 0 ? L.dst[0] : y == 1 ? L.dst[1] : y == 2 ? L.dst[2]
+                           : y == 3 ? L.dst[3] : L.dst[4];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
 // copy the first nf of (xm, ym, tkm, sxxm, sxym) of the resident markers to
 // (save) or from (restore) buf, M doubles each
 static int state_copy(ereb_ctx* c, double* buf, int nf, bool save) {
-    const size_t bytes = sizeof(double) * (size_t)c->marknum;
+    const int64_t M = c->marknum;
+    if (M <= 0 || nf <= 0) return EREB_OK;
+    CopyList L{};
     for (int q = 0; q < nf; ++q) {
         double* dev = mptr(c, kStateFields[q]);
-        double* b = buf + (size_t)q * c->marknum;
-        EREB_HIP(c, hipMemcpyAsync(save ? b : dev, save ? dev : b, bytes,
-                                   hipMemcpyDeviceToDevice, c->stream));
+        double* b = buf + (size_t)q * M;
+        L.src[q] = save ? dev : b;
+        L.dst[q] = save ? b : dev;
     }
+    const unsigned nb = (unsigned)std::min<int64_t>((M + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_copy_list, dim3(nb, (unsigned)nf), dim3(256), 0, c->stream, L, M);
+    EREB_HIP(c, hipGetLastError());
+    return EREB_OK;
+}
+
+// up to three ranges set to zero in one launch (grid row y = range y)
+struct ZeroList {
+    double* p[3];
+    int64_t n[3];
+};
+
+__global__ __launch_bounds__(256) void k_zero_list(ZeroList Z) {
+    const int y = blockIdx.y;
+    double* __restrict__ p = y == 0 ? Z.p[0] : y == 1 ? Z.p[1] : Z.p[2];
+    const int64_t n = y == 0 ? Z.n[0] : y == 1 ? Z.n[1] : Z.n[2];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+static int zero_list(ereb_ctx* c, const ZeroList& Z, int nr) {
+    int64_t mx = 0;
+    for (int r = 0; r < nr; ++r) mx = std::max(mx, Z.n[r]);
+    if (mx <= 0) return EREB_OK;
+    const unsigned nb = (unsigned)std::min<int64_t>((mx + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_zero_list, dim3(nb, (unsigned)nr), dim3(256), 0, c->stream, Z);
+    EREB_HIP(c, hipGetLastError());
     return EREB_OK;
 }
 
@@ -328,12 +380,13 @@ __global__ void k_fix_to_grid(const unsigned long long* __restrict__ wa,
                               const unsigned long long* __restrict__ ba,
                               const unsigned long long* __restrict__ wb,
                               const unsigned long long* __restrict__ bb, int64_t n,
-                              double* __restrict__ out) {
+                              double* __restrict__ out, double* __restrict__ out2) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double v = fix_value(wa, k, fix_up_of_bound(ba));
     if (wb) v += fix_value(wb, k, fix_up_of_bound(bb));
     out[k] = v;
+    out2[k] = v;
 }
 
 // bits[0] = max(bits[0], max |a[i]|) over i < n, one atomic per wavefront:
@@ -556,8 +609,7 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
         int64_t soff[NSUM];
         for (int s = 0; s < NSUM; ++s) soff[s] = goff[sum_grid_field(s)];
         const unsigned nbg = (unsigned)((n + 255) / 256);
-        if ((rc = fill_f64(c, B.gT, 0.0, M)) || (rc = fill_f64(c, B.gP, 0.0, M)) ||
-            (rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz)))
+        if ((rc = zero_list(c, ZeroList{{B.gT, B.gP, B.gg}, {M, M, (int64_t)gsz}}, 3)))
             return rc;
         // d L / d tk1 = 2 tk1 on the final state, then nodes^T and scatter^T there
         hipLaunchKernelGGL(k_twice, dim3(nbg), dim3(256), 0, c->stream,
@@ -584,14 +636,16 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                                            hipMemcpyDeviceToDevice, c->stream));
                 if ((rc = state_copy(c, stk, 3, false))) return rc;
                 if (k == 0 && (rc = launch_gather_temperature(c, 1))) return rc;
-                if ((rc = fill_f64(c, B.gg, 0.0, (int64_t)gsz))) return rc;
                 // RK4^T (tkm as RK4 saw it), then gather^T at timestep 1; their
                 // tk2 cotangents are summed in fixed point, each with a scale
                 // from a bound on its contributions (the largest marker
                 // cotangent, ereb_fixsum.h), so the gradient is the same on
-                // every run and the sweep needs no host round trip
-                if ((rc = fill_f64(c, (double*)B.fix + 8, 0.0, 4 * n))) return rc;
-                if ((rc = fill_f64(c, (double*)B.fix, 0.0, 2))) return rc;  // the bounds
+                // every run and the sweep needs no host round trip.  Cleared
+                // first: the grid cotangents, the fixed-point sums, the bounds
+                if ((rc = zero_list(c, ZeroList{{B.gg, (double*)B.fix + 8, (double*)B.fix},
+                                                {(int64_t)gsz, 4 * n, 2}},
+                                    3)))
+                    return rc;
                 FixGrid FA = fix_grid(B, 0, n), FB = fix_grid(B, 1, n);
                 FA.bound = FA.maxbits;
                 FB.bound = FB.maxbits;
@@ -606,14 +660,13 @@ static int adjoint_body(ereb_ctx* c, AdjBuffers& B, int64_t nsteps, int64_t ever
                 if (k == 0 && ((rc = absmax_bits(c, B.gT2, M, FB.maxbits)) ||
                                (rc = launch_gather_T_adj(c, 1, 1, B.gT2, FB, flag, order))))
                     return rc;
+                // the tk2 cotangent, also the tk1 one (tk2 := tk1), then node
+                // properties^T
                 hipLaunchKernelGGL(k_fix_to_grid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                                    c->stream, (const unsigned long long*)FA.w, FA.bound,
                                    k == 0 ? (const unsigned long long*)FB.w : nullptr, FB.bound,
-                                   n, G(EREB_G_TK2));
+                                   n, G(EREB_G_TK2), G(EREB_G_TK1));
                 EREB_HIP(c, hipGetLastError());
-                // tk2 := tk1, then node properties^T
-                EREB_HIP(c, hipMemcpyAsync(G(EREB_G_TK1), G(EREB_G_TK2), sizeof(double) * n,
-                                           hipMemcpyDeviceToDevice, c->stream));
                 if ((rc = launch_nodes_adj(c, B.gg, goff))) return rc;
                 // scatter^T with the temperature the scatter saw
                 if (k == 0)

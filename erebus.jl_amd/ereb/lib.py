@@ -491,17 +491,27 @@ class Context:
 
     # ---- adjoint ----------------------------------------------------------
     def adjoint_tk1_loss(self, nsteps: int, checkpoint_every: int, dt: float, mode: int,
-                         hrsolidm, hrfluidm, grad: bool = True, want_tk1: bool = False):
+                         hrsolidm, hrfluidm, grad: bool = True, want_tk1: bool = False,
+                         out=None):
         """L = sum(tk1^2) after nsteps steps of the transfer map and, with
         grad, dL/dtkm and dL/dphim of the initial markers (adjoint kernels,
         checkpoints every checkpoint_every steps).  Returns (L, gT, gphi,
-        tk1) with None for what was not asked; the markers are unchanged."""
+        tk1) with None for what was not asked; the markers are unchanged.
+        out: optional (gT, gphi) arrays of marknum float64 to write the
+        gradient into (page-locked ones from host_array take the device copy
+        without staging)."""
         hs = np.ascontiguousarray(hrsolidm, dtype=np.float64)
         hf = np.ascontiguousarray(hrfluidm, dtype=np.float64)
         n = self.marknum
         L = ctypes.c_double()
-        gT = np.empty(n) if grad else None
-        gP = np.empty(n) if grad else None
+        if grad and out is not None:
+            gT, gP = out
+            for a in (gT, gP):
+                if a.dtype != np.float64 or a.shape != (n,) or not a.flags.c_contiguous:
+                    raise ValueError("adjoint out arrays must be contiguous float64 of marknum")
+        else:
+            gT = np.empty(n) if grad else None
+            gP = np.empty(n) if grad else None
         tk1 = np.empty(self.grid_shape("tk1"), order="F") if want_tk1 else None
         self.check(self.lib.ereb_adjoint_tk1_loss(
             self.h, int(nsteps), int(checkpoint_every), float(dt), int(mode), _ptr(hs),
