@@ -129,6 +129,7 @@ __global__ void k_set_scalars(StepScalars* __restrict__ d, StepScalars v) {
 }
 
 int scal_sync(ereb_ctx* c) {
+    if (c->scal_ride) return EREB_OK;  // written by the key pass of the step
     if (c->scal_written && std::memcmp(&c->scal, &c->scal_dev, sizeof c->scal) == 0)
         return EREB_OK;
     if (c->capturing)
@@ -342,7 +343,6 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         std::vector<uint32_t> all((size_t)(prop.multiProcessorCount + 31) / 32, ~0u);
         CK(hipExtStreamCreateWithCUMask(&c->stream2, (uint32_t)all.size(), all.data()));
     }
-    CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&c->ev_scatter, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     // the no-alloc entry point's scratch on this device (once per device); a
@@ -449,7 +449,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.chg);
         hipFree(s.map);
     }
-    for (hipEvent_t e : {c->ev_fork, c->ev_scatter, c->ev_join})
+    for (hipEvent_t e : {c->ev_scatter, c->ev_join})
         if (e) hipEventDestroy(e);
     if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -639,12 +639,14 @@ struct OnStream {
 // stress-change and temperature grids given to the step, which no kernel of
 // the step writes, and write marker state that the sort and the scatter read;
 // the node pass and the two assemblies read the scatter's window sums and
-// node grids only.  So the marker pass (SEG_MARKERS) runs on the second
-// stream as soon as the scatter (SEG_SCATTER) is done, beside the node pass
-// and the assemblies (SEG_NODES), and its coefficient tables (SEG_TABLES, from
-// the velocities alone) from the start of the step, beside the sort; the main
-// stream waits for the second at the end.  Results are those of the launches
-// in sequence: no kernel of one stream touches what the other's write.
+// node grids only.  So the marker pass (SEG_MARKERS), after its coefficient
+// tables (SEG_TABLES, from the velocities alone), runs on the second stream
+// as soon as the scatter (SEG_SCATTER) is done, beside the node pass and the
+// assemblies (SEG_NODES); the main stream waits for the second at the end.
+// (The tables beside the sort's key pass, which is bound by HBM traffic,
+// stretched it by more than their own duration.)  Results are those of the
+// launches in sequence: no kernel of one stream touches what the other's
+// write.
 enum { SEG_TABLES = 0, SEG_SCATTER, SEG_MARKERS, SEG_NODES };
 
 static int seg_launches(ereb_ctx* c, int seg, const double* hrs, const double* hrf,
@@ -684,14 +686,12 @@ static int transfer_run(ereb_ctx* c, bool replay, const double* hrs, const doubl
         return launch_gathers_rk4(c, dt, (int)mode, timestep);
     }
     hipStream_t s1 = c->stream, s2 = c->stream2;
-    EREB_HIP(c, hipEventRecord(c->ev_fork, s1));
-    EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
-    rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
-    if (!rc) rc = seg_run(c, SEG_SCATTER, s1, replay, hrs, hrf, mode, dt, timestep);
+    rc = seg_run(c, SEG_SCATTER, s1, replay, hrs, hrf, mode, dt, timestep);
     if (!rc) {
         EREB_HIP(c, hipEventRecord(c->ev_scatter, s1));
         EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_scatter, 0));
-        rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);
+        rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
+        if (!rc) rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);
     }
     if (!rc) rc = seg_run(c, SEG_NODES, s1, replay, hrs, hrf, mode, dt, timestep);
     // the join also after a failure: later work on the main stream follows
@@ -782,13 +782,22 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     }
     c->scal.dt = dt;
     c->scal.first = timestep == 1 ? 1 : 0;
+    // eager: when asked, while profiling (events around every launch), and
+    // for the first step (it makes the allocations a capture must not)
+    const bool eager = !c->use_graphs || c->prof || !c->step_warm;
+    // eager with markers: the scalars reach the device with the sort's key
+    // pass, the step's first kernel that reads them, instead of by a launch
+    // of their own; graph replays read them from the block written before
+    c->scal_ride = eager && c->marknum > 0;
     if ((rc = scal_sync(c))) return rc;
     // which scatter loads the step takes is part of its key (typed_refresh)
     if ((rc = typed_refresh(c))) return rc;
-    // eager: when asked, while profiling (events around every launch), and
-    // for the first step (it makes the allocations a capture must not)
-    if (!c->use_graphs || c->prof || !c->step_warm) {
+    if (eager) {
         rc = transfer_run(c, false, hrsolidm, hrfluidm, mode, dt, timestep);
+        if (c->scal_ride) {  // not taken (a failure before the key pass)
+            c->scal_ride = false;
+            if (!rc) rc = fail(c, EREB_ERR_STATE, "step scalars not written");
+        }
         if (!rc) c->step_warm = true;
         return rc;
     }

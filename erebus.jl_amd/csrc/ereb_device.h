@@ -376,12 +376,19 @@ template <class R>
 struct PropArgs {
     const ereb_materials* __restrict__ M;
     const StepScalars* __restrict__ S;  // hrs, hrf
+    // Sw set: the step's scalars come with the launch (Sv) instead of from S,
+    // and the pass writes them to Sw for the kernels after it
+    StepScalars Sv;
+    StepScalars* Sw;
     int mode;
     const int32_t* __restrict__ tm;
     const double *__restrict__ tkm, *__restrict__ phim;
     const R* __restrict__ XW;
     R *rhototal, *rhocptotal, *etatotal, *hrtotal, *ktotal, *tkrhocp, *etafkphi;
     __device__ void apply(int64_t m) const;
+    __device__ void publish() const {
+        if (Sw) *Sw = Sv;
+    }
 };
 template <>
 __device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
@@ -392,8 +399,10 @@ __device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
         st = Props{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0, 0.0};
     else
         xw = XW[m];
-    const double hrs[3] = {S->hrs[0], S->hrs[1], S->hrs[2]};
-    const double hrf[3] = {S->hrf[0], S->hrf[1], S->hrf[2]};
+    const double* hs = Sw ? Sv.hrs : S->hrs;
+    const double* hf = Sw ? Sv.hrf : S->hrf;
+    const double hrs[3] = {hs[0], hs[1], hs[2]};
+    const double hrf[3] = {hf[0], hf[1], hf[2]};
     const Props p = marker_props(*M, t, tkm[m], phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;
@@ -414,8 +423,10 @@ __device__ __forceinline__ void PropArgs<float>::apply(int64_t m) const {
         st = PropsF{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0f, 0.0f};
     else
         xw = XW[m];
-    const double hrs[3] = {S->hrs[0], S->hrs[1], S->hrs[2]};
-    const double hrf[3] = {S->hrf[0], S->hrf[1], S->hrf[2]};
+    const double* hs = Sw ? Sv.hrs : S->hrs;
+    const double* hf = Sw ? Sv.hrf : S->hrf;
+    const double hrs[3] = {hs[0], hs[1], hs[2]};
+    const double hrf[3] = {hf[0], hf[1], hf[2]};
     const PropsF p = marker_props_f(*M, t, tkm[m], (float)phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;
@@ -430,6 +441,7 @@ __device__ __forceinline__ void PropArgs<float>::apply(int64_t m) const {
 // no property evaluation
 struct NoProps {
     __device__ void apply(int64_t) const {}
+    __device__ void publish() const {}
 };
 
 // Compensated FP32 accumulation (Knuth's TwoSum): s + c carries the sum of

@@ -97,9 +97,9 @@ struct ereb_ctx {
     hipStream_t stream = nullptr;
     // second stream of the transfer step (default priority, like `stream`):
     // the gathers and RK4 run there beside the node pass and the assemblies;
-    // fork, scatter-done and join events
+    // scatter-done and join events
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_scatter = nullptr, ev_join = nullptr;
+    hipEvent_t ev_scatter = nullptr, ev_join = nullptr;
     int precision = EREB_FP64;
     int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
     double dx = 0, dy = 0;
@@ -112,6 +112,10 @@ struct ereb_ctx {
     ereb::StepScalars* d_scal = nullptr;
     ereb::StepScalars scal{}, scal_dev{};
     bool scal_written = false;
+    // set by an eager transfer step with markers: the scalars go to the
+    // device block with the sort's key pass (PropArgs::publish) instead of a
+    // launch of their own; scal_sync leaves them until then
+    bool scal_ride = false;
     bool capturing = false;  // a step is being captured into a HIP graph
 
     int64_t marknum = 0, cap = 0;

@@ -96,7 +96,10 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
                                                     int32_t* __restrict__ big, P props) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = m < M;
-    if (m == 0) big[0] = 0;  // k_fine_order, which fills the list, runs after
+    if (m == 0) {
+        big[0] = 0;  // k_fine_order, which fills the list, runs after
+        props.publish();
+    }
     uint32_t k = 0;
     if (active) {
         bool bad;
@@ -114,6 +117,9 @@ static PropArgs<R> prop_args(ereb_ctx* c, const PropsReq& q) {
     PropArgs<R> P;
     P.M = c->d_mat;
     P.S = c->d_scal;  // hrs, hrf written there by set_heating
+    // the step's scalars with this launch when they ride with the key pass
+    P.Sv = c->scal;
+    P.Sw = c->scal_ride && c->marknum > 0 ? c->d_scal : nullptr;
     P.mode = q.mode;
     P.tm = tm_ptr(c);
     P.tkm = mptr(c, EREB_M_TKM);
@@ -240,6 +246,11 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
             hipLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256), 0,
                                c->stream, g, c->nty, xm, ym, M, c->d_key, rank,
                                c->d_fcount, c->d_flags, c->d_big, prop_args<double>(c, *props));
+        if (props && c->scal_ride) {  // the device block now follows this launch
+            c->scal_ride = false;
+            c->scal_dev = c->scal;
+            c->scal_written = true;
+        }
     }
     // key starts; the counts are cleared for the next sort
     if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk, nullptr, true))) return rc;
