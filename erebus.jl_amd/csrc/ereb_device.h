@@ -389,6 +389,16 @@ struct PropArgs {
     __device__ void publish() const {
         if (Sw) *Sw = Sv;
     }
+    // the heating rates: values selected, not pointers (a pointer to either
+    // the launch argument or the device block would make every load a
+    // per-lane generic one)
+    __device__ __forceinline__ void heating(double (&hrs)[3], double (&hrf)[3]) const {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            hrs[q] = Sw ? Sv.hrs[q] : S->hrs[q];
+            hrf[q] = Sw ? Sv.hrf[q] : S->hrf[q];
+        }
+    }
 };
 template <>
 __device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
@@ -399,10 +409,8 @@ __device__ __forceinline__ void PropArgs<double>::apply(int64_t m) const {
         st = Props{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0, 0.0};
     else
         xw = XW[m];
-    const double* hs = Sw ? Sv.hrs : S->hrs;
-    const double* hf = Sw ? Sv.hrf : S->hrf;
-    const double hrs[3] = {hs[0], hs[1], hs[2]};
-    const double hrf[3] = {hf[0], hf[1], hf[2]};
+    double hrs[3], hrf[3];
+    heating(hrs, hrf);
     const Props p = marker_props(*M, t, tkm[m], phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;
@@ -423,10 +431,8 @@ __device__ __forceinline__ void PropArgs<float>::apply(int64_t m) const {
         st = PropsF{rhototal[m], rhocptotal[m], etatotal[m], hrtotal[m], ktotal[m], 0.0f, 0.0f};
     else
         xw = XW[m];
-    const double* hs = Sw ? Sv.hrs : S->hrs;
-    const double* hf = Sw ? Sv.hrf : S->hrf;
-    const double hrs[3] = {hs[0], hs[1], hs[2]};
-    const double hrf[3] = {hf[0], hf[1], hf[2]};
+    double hrs[3], hrf[3];
+    heating(hrs, hrf);
     const PropsF p = marker_props_f(*M, t, tkm[m], (float)phim[m], xw, hrs, hrf, mode, st);
     if (t < 3) {
         rhototal[m] = p.rhototal;

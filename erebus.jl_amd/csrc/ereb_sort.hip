@@ -96,10 +96,7 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
                                                     int32_t* __restrict__ big, P props) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = m < M;
-    if (m == 0) {
-        big[0] = 0;  // k_fine_order, which fills the list, runs after
-        props.publish();
-    }
+    if (m == 0) big[0] = 0;  // k_fine_order, which fills the list, runs after
     uint32_t k = 0;
     if (active) {
         bool bad;
@@ -110,6 +107,9 @@ __global__ __launch_bounds__(256) void k_fine_count(GeomK g, int nty, const doub
     const int32_t r = wave_reserve(count, k, active);
     if (active) rank[m] = r;
     if (active) props.apply(m);
+    // last, where nothing else is live: the step scalars, when they come with
+    // this launch, to the device block for the kernels after it
+    if (m == 0) props.publish();
 }
 
 template <class R>
