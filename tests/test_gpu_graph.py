@@ -133,3 +133,38 @@ def test_graph_replay_across_replenishment():
     finally:
         A.close()
         B.close()
+
+
+@pytest.mark.parametrize("M", [0, 1])
+def test_step_with_no_marker_and_one_marker(M):
+    """The two-stream step and its graph replays with 0 and 1 markers (the
+    marker branch then launches nothing, or one workgroup's worth): equal to
+    the separate resident calls bit for bit, step after step."""
+    g, mat = synth.workload_geometry("adj41", mode=1)
+    mk = synth.make_markers(g, mat, seed=9)
+    pick = np.arange(M) * 977 % mk["xm"].size
+    mk = {k: np.ascontiguousarray(v[pick]) for k, v in mk.items()}
+    A = _ctx(g, mat, mk, graphs=False)
+    B = _ctx(g, mat, mk, graphs=True)
+    C = _ctx(g, mat, mk, graphs=False)
+    try:
+        for ts in range(1, 5):
+            dt, hrs, hrf = _scalars(mat, ts)
+            for c in (A, B):
+                c.step_transfer(hrs, hrf, 1, dt, ts)
+            C.step_scatter(hrs, hrf, 1)
+            C.step_node_properties()
+            C.step_assemble_stokes_darcy(dt)
+            C.step_assemble_thermal(dt)
+            C.step_gather_stress()
+            C.step_gather_temperature(ts)
+            C.step_advect_rk4(dt, 1)
+            sa, sb, sc = _state(A), _state(B), _state(C)
+            for k in sa:
+                assert np.array_equal(sa[k], sb[k], equal_nan=True), (ts, k)
+                assert np.array_equal(sa[k], sc[k], equal_nan=True), (ts, k)
+        assert A.marknum == M and B.marknum == M
+    finally:
+        A.close()
+        B.close()
+        C.close()
