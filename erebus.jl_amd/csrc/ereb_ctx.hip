@@ -774,8 +774,8 @@ int ereb_step_transfer(ereb_ctx* c, const double hrsolidm[3], const double hrflu
     int rc = valid_mode(c, mode);
     if (rc) return rc;
     if ((rc = reserve(c, std::max<int64_t>(c->marknum, 1)))) return rc;
-    // the step's scalars go to the device block before the launches (eager)
-    // or the replay, outside any capture
+    // the step's scalars: written to the device block before a replay
+    // (outside any capture), or taken by the eager launches (below)
     for (int q = 0; q < 3; ++q) {
         c->scal.hrs[q] = hrsolidm[q];
         c->scal.hrf[q] = hrfluidm[q];
