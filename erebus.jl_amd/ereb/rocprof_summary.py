@@ -9,6 +9,8 @@ counters read for a plain device copy of known size.
         --out-json OUT.json --out-md OUT.md
     python -m ereb.rocprof_summary counters D [D ...] --kernel PREFIX [...] \\
         --title T --command C [--command C ...] --out OUT.md
+    python -m ereb.rocprof_summary timeline TRACE_DIR --first PREFIX --index N \\
+        --title T --command C --out OUT.md
 
 TRACE_DIR and the counter directories are rocprofv3 output directories
 (searched recursively for ``*kernel_trace.csv`` / ``*counter_collection.csv``).
@@ -176,6 +178,36 @@ def write_counters(dirs, kernels, title, commands, out):
     open(out, "w").write("\n".join(lines) + "\n")
 
 
+def step_timeline(trace_dir: str, first: str, index: int):
+    """Dispatches from the index-th dispatch whose name starts with `first`
+    up to the next one, in start order: [(start_us, end_us, queue, name)]
+    relative to the first dispatch's start."""
+    rows = sorted(((int(r["start_timestamp"]), int(r["end_timestamp"]),
+                    r.get("queue_id", "?"), r["kernel_name"])
+                   for r in _rows(trace_dir, "kernel_trace.csv")), key=lambda x: x[0])
+    marks = [i for i, r in enumerate(rows) if r[3].startswith(first)]
+    if index + 1 >= len(marks):
+        raise ValueError(f"fewer than {index + 2} dispatches of {first!r}")
+    seg = rows[marks[index]:marks[index + 1]]
+    t0 = seg[0][0]
+    return [((a - t0) / 1e3, (b - t0) / 1e3, q, n) for a, b, q, n in seg]
+
+
+def write_timeline(trace_dir, first, index, title, command, out):
+    tl = step_timeline(trace_dir, first, index)
+    span = max(b for _, b, _, _ in tl)
+    busy = sum(b - a for a, b, _, _ in tl)
+    lines = [f"# {title}", "", "Command (run from /tmp with TMPDIR=/tmp; `$R` = repository root):",
+             "", f"    {command}", "",
+             f"One step: dispatch {index} of `{_short(first, 60)}` up to the next one, "
+             "times in µs from its start, by hardware queue.  Span "
+             f"{span:.1f} µs, sum of the durations {busy:.1f} µs.", "",
+             "| start | end | duration | queue | kernel |", "|---:|---:|---:|---:|---|"]
+    for a, b, q, n in tl:
+        lines.append(f"| {a:.1f} | {b:.1f} | {b - a:.1f} | {q} | `{_short(n, 70)}` |")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -195,8 +227,17 @@ def main():
     c.add_argument("--title", required=True)
     c.add_argument("--command", action="append", required=True)
     c.add_argument("--out", required=True)
+    tl = sub.add_parser("timeline")
+    tl.add_argument("trace_dir")
+    tl.add_argument("--first", required=True, help="name prefix of a step's first kernel")
+    tl.add_argument("--index", type=int, required=True)
+    tl.add_argument("--title", required=True)
+    tl.add_argument("--command", required=True)
+    tl.add_argument("--out", required=True)
     a = ap.parse_args()
-    if a.cmd == "kernels":
+    if a.cmd == "timeline":
+        write_timeline(a.trace_dir, a.first, a.index, a.title, a.command, a.out)
+    elif a.cmd == "kernels":
         write_kernels(a.trace_dir, a.title, a.command, a.out)
     elif a.cmd == "counters":
         write_counters(a.dirs, a.kernel, a.title, a.command, a.out)
