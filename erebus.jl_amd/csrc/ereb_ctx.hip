@@ -344,6 +344,7 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         CK(hipExtStreamCreateWithCUMask(&c->stream2, (uint32_t)all.size(), all.data()));
     }
     CK(hipEventCreateWithFlags(&c->ev_scatter, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->ev_keys, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     // the no-alloc entry point's scratch on this device (once per device); a
     // failure here is reported by that entry point, which tries again
@@ -449,7 +450,7 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.chg);
         hipFree(s.map);
     }
-    for (hipEvent_t e : {c->ev_scatter, c->ev_join})
+    for (hipEvent_t e : {c->ev_scatter, c->ev_join, c->ev_keys})
         if (e) hipEventDestroy(e);
     if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -686,11 +687,22 @@ static int transfer_run(ereb_ctx* c, bool replay, const double* hrs, const doubl
         return launch_gathers_rk4(c, dt, (int)mode, timestep);
     }
     hipStream_t s1 = c->stream, s2 = c->stream2;
+    // eager: the coefficient tables start once the sort's key pass is done,
+    // beside the rest of the sort (short launches that leave most of the
+    // chip idle); a replay runs them after the scatter (a segment graph
+    // holds no event)
+    c->rec_keys = !replay && c->marknum > 0;
     rc = seg_run(c, SEG_SCATTER, s1, replay, hrs, hrf, mode, dt, timestep);
+    const bool keys = c->rec_keys == false && !replay && c->marknum > 0;
+    c->rec_keys = false;
+    if (!rc && keys) {
+        EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_keys, 0));
+        rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
+    }
     if (!rc) {
         EREB_HIP(c, hipEventRecord(c->ev_scatter, s1));
         EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_scatter, 0));
-        rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
+        if (!keys) rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
         if (!rc) rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);
     }
     if (!rc) rc = seg_run(c, SEG_NODES, s1, replay, hrs, hrf, mode, dt, timestep);

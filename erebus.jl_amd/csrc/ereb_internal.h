@@ -97,9 +97,11 @@ struct ereb_ctx {
     hipStream_t stream = nullptr;
     // second stream of the transfer step (default priority, like `stream`):
     // the gathers and RK4 run there beside the node pass and the assemblies;
-    // scatter-done and join events
+    // scatter-done and join events; ev_keys: recorded after the sort's key
+    // pass while rec_keys is set
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev_scatter = nullptr, ev_join = nullptr;
+    hipEvent_t ev_scatter = nullptr, ev_join = nullptr, ev_keys = nullptr;
+    bool rec_keys = false;
     int precision = EREB_FP64;
     int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
     double dx = 0, dy = 0;

@@ -251,6 +251,10 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
             c->scal_dev = c->scal;
             c->scal_written = true;
         }
+        if (c->rec_keys) {  // the two-stream step: its key pass is done here
+            EREB_HIP(c, hipEventRecord(c->ev_keys, c->stream));
+            c->rec_keys = false;
+        }
     }
     // key starts; the counts are cleared for the next sort
     if ((rc = scan_exclusive(c, c->d_fcount, c->d_fstart, nk, nullptr, true))) return rc;
