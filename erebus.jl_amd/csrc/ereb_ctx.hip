@@ -640,14 +640,14 @@ struct OnStream {
 // stress-change and temperature grids given to the step, which no kernel of
 // the step writes, and write marker state that the sort and the scatter read;
 // the node pass and the two assemblies read the scatter's window sums and
-// node grids only.  So the marker pass (SEG_MARKERS), after its coefficient
-// tables (SEG_TABLES, from the velocities alone), runs on the second stream
-// as soon as the scatter (SEG_SCATTER) is done, beside the node pass and the
-// assemblies (SEG_NODES); the main stream waits for the second at the end.
-// (The tables beside the sort's key pass, which is bound by HBM traffic,
-// stretched it by more than their own duration.)  Results are those of the
-// launches in sequence: no kernel of one stream touches what the other's
-// write.
+// node grids only.  So the marker pass (SEG_MARKERS) runs on the second
+// stream as soon as the scatter (SEG_SCATTER) is done, beside the node pass
+// and the assemblies (SEG_NODES), and its coefficient tables (SEG_TABLES,
+// from the velocities alone) before it, from the end of the sort's key pass
+// (beside the key pass itself, which is bound by HBM traffic, they stretched
+// it by more than their own duration); the main stream waits for the second
+// at the end.  Results are those of the launches in sequence: no kernel of
+// one stream touches what the other's write.
 enum { SEG_TABLES = 0, SEG_SCATTER, SEG_MARKERS, SEG_NODES };
 
 static int seg_launches(ereb_ctx* c, int seg, const double* hrs, const double* hrf,
