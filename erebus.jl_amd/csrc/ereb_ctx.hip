@@ -343,8 +343,9 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         std::vector<uint32_t> all((size_t)(prop.multiProcessorCount + 31) / 32, ~0u);
         CK(hipExtStreamCreateWithCUMask(&c->stream2, (uint32_t)all.size(), all.data()));
     }
-    CK(hipEventCreateWithFlags(&c->ev_scatter, hipEventDisableTiming));
-    CK(hipEventCreateWithFlags(&c->ev_keys, hipEventDisableTiming));
+    // recorded as kernels' stop events (hipExtLaunchKernelGGL): default flags
+    CK(hipEventCreate(&c->ev_scatter));
+    CK(hipEventCreate(&c->ev_keys));
     CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     // the no-alloc entry point's scratch on this device (once per device); a
     // failure here is reported by that entry point, which tries again
@@ -691,16 +692,19 @@ static int transfer_run(ereb_ctx* c, bool replay, const double* hrs, const doubl
     // beside the rest of the sort (short launches that leave most of the
     // chip idle); a replay runs them after the scatter (a segment graph
     // holds no event)
-    c->rec_keys = !replay && c->marknum > 0;
+    // (the key pass and the scatter record their events themselves, as
+    // their completion; a replay records the scatter's after its graph)
+    const bool evs = !replay && c->marknum > 0;
+    c->rec_keys = c->rec_scatter = evs;
     rc = seg_run(c, SEG_SCATTER, s1, replay, hrs, hrf, mode, dt, timestep);
-    const bool keys = c->rec_keys == false && !replay && c->marknum > 0;
-    c->rec_keys = false;
+    const bool keys = evs && !c->rec_keys, scattered = evs && !c->rec_scatter;
+    c->rec_keys = c->rec_scatter = false;
     if (!rc && keys) {
         EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_keys, 0));
         rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
     }
     if (!rc) {
-        EREB_HIP(c, hipEventRecord(c->ev_scatter, s1));
+        if (!scattered) EREB_HIP(c, hipEventRecord(c->ev_scatter, s1));
         EREB_HIP(c, hipStreamWaitEvent(s2, c->ev_scatter, 0));
         if (!keys) rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
         if (!rc) rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);

@@ -1,6 +1,7 @@
 // ereb_internal.h -- host-side context of libereb_hip.so and the launch
 // interface between the C-ABI layer and the kernel files.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -97,11 +98,13 @@ struct ereb_ctx {
     hipStream_t stream = nullptr;
     // second stream of the transfer step (default priority, like `stream`):
     // the gathers and RK4 run there beside the node pass and the assemblies;
-    // scatter-done and join events; ev_keys: recorded after the sort's key
-    // pass while rec_keys is set
+    // scatter-done, join and key-pass-done events; while rec_keys /
+    // rec_scatter is set, the next key pass / scatter launch records ev_keys /
+    // ev_scatter as its own completion (hipExtLaunchKernelGGL's stop event,
+    // no marker packet between two kernels of the stream) and clears the flag
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_scatter = nullptr, ev_join = nullptr, ev_keys = nullptr;
-    bool rec_keys = false;
+    bool rec_keys = false, rec_scatter = false;
     int precision = EREB_FP64;
     int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
     double dx = 0, dy = 0;

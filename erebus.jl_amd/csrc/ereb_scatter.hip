@@ -651,8 +651,14 @@ static int launch_scatter_t(ereb_ctx* c) {
     A.tab = typed_tab(c->mat);
     hipEvent_t ev;
     prof_begin(c, EREB_K_SCATTER, &ev);
-    hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)c->ntiles), dim3(ScatterShape<R>::NW * 64), 0,
-                       c->stream, A);
+    // the scatter's completion as ev_scatter when asked (rec_scatter)
+    const dim3 grid((unsigned)c->ntiles), block(ScatterShape<R>::NW * 64);
+    if (c->rec_scatter)
+        hipExtLaunchKernelGGL(k_scatter<R>, grid, block, 0, c->stream, nullptr, c->ev_scatter, 0,
+                              A);
+    else
+        hipLaunchKernelGGL(k_scatter<R>, grid, block, 0, c->stream, A);
+    c->rec_scatter = false;
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_SCATTER, ev);
     return EREB_OK;

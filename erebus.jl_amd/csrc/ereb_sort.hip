@@ -238,22 +238,36 @@ int launch_fine_sort(ereb_ctx* c, const PropsReq* props) {
             hipLaunchKernelGGL(k_fine_count<NoProps>, dim3(nb), dim3(256), 0, c->stream, g,
                                c->nty, xm, ym, M, c->d_key, rank, c->d_fcount, c->d_flags,
                                c->d_big, NoProps{});
-        else if (marker_is_f32(c->precision, EREB_M_ETATOTALM))
-            hipLaunchKernelGGL(k_fine_count<PropArgs<float>>, dim3(nb), dim3(256), 0, c->stream,
-                               g, c->nty, xm, ym, M, c->d_key, rank, c->d_fcount,
-                               c->d_flags, c->d_big, prop_args<float>(c, *props));
-        else
-            hipLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256), 0,
-                               c->stream, g, c->nty, xm, ym, M, c->d_key, rank,
-                               c->d_fcount, c->d_flags, c->d_big, prop_args<double>(c, *props));
+        else {
+            // the key pass's completion as ev_keys when asked (rec_keys)
+            hipEvent_t done = c->rec_keys ? c->ev_keys : nullptr;
+            c->rec_keys = false;
+            if (marker_is_f32(c->precision, EREB_M_ETATOTALM)) {
+                const auto P = prop_args<float>(c, *props);
+                if (done)
+                    hipExtLaunchKernelGGL(k_fine_count<PropArgs<float>>, dim3(nb), dim3(256), 0,
+                                          c->stream, nullptr, done, 0, g, c->nty, xm, ym, M,
+                                          c->d_key, rank, c->d_fcount, c->d_flags, c->d_big, P);
+                else
+                    hipLaunchKernelGGL(k_fine_count<PropArgs<float>>, dim3(nb), dim3(256), 0,
+                                       c->stream, g, c->nty, xm, ym, M, c->d_key, rank,
+                                       c->d_fcount, c->d_flags, c->d_big, P);
+            } else {
+                const auto P = prop_args<double>(c, *props);
+                if (done)
+                    hipExtLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256),
+                                          0, c->stream, nullptr, done, 0, g, c->nty, xm, ym, M,
+                                          c->d_key, rank, c->d_fcount, c->d_flags, c->d_big, P);
+                else
+                    hipLaunchKernelGGL(k_fine_count<PropArgs<double>>, dim3(nb), dim3(256), 0,
+                                       c->stream, g, c->nty, xm, ym, M, c->d_key, rank,
+                                       c->d_fcount, c->d_flags, c->d_big, P);
+            }
+        }
         if (props && c->scal_ride) {  // the device block now follows this launch
             c->scal_ride = false;
             c->scal_dev = c->scal;
             c->scal_written = true;
-        }
-        if (c->rec_keys) {  // the two-stream step: its key pass is done here
-            EREB_HIP(c, hipEventRecord(c->ev_keys, c->stream));
-            c->rec_keys = false;
         }
     }
     // key starts; the counts are cleared for the next sort
