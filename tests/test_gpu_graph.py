@@ -168,3 +168,40 @@ def test_step_with_no_marker_and_one_marker(M):
         A.close()
         B.close()
         C.close()
+
+
+def test_two_stream_step_at_workload_size():
+    """At 2,822,400 markers, where the marker branch and the node pass and
+    assemblies really run side by side: the eager two-stream steps and the
+    graph replays equal the separate resident calls bit for bit after each
+    of four steps with varying dt (marker state, SUM and node grids, both
+    assembled systems)."""
+    g, mat = synth.workload_geometry("fp64", mode=1)
+    mk = synth.make_markers(g, mat, seed=0, stress=True)
+    A = _ctx(g, mat, mk, graphs=False)
+    B = _ctx(g, mat, mk, graphs=True)
+    C = _ctx(g, mat, mk, graphs=False)
+    try:
+        for ts in range(1, 5):
+            dt, hrs, hrf = _scalars(mat, ts)
+            for c in (A, B):
+                c.step_transfer(hrs, hrf, 1, dt, ts)
+            C.step_scatter(hrs, hrf, 1)
+            C.step_node_properties()
+            C.step_assemble_stokes_darcy(dt)
+            C.step_assemble_thermal(dt)
+            C.step_gather_stress()
+            C.step_gather_temperature(ts)
+            C.step_advect_rk4(dt, 1)
+            for n in ("xm", "ym", "tkm", "sxxm", "sxym"):
+                a, b, c_ = A.get_marker(n), B.get_marker(n), C.get_marker(n)
+                assert np.array_equal(a, c_) and np.array_equal(b, c_), (ts, n)
+        sa, sb, sc = _state(A), _state(B), _state(C)
+        for k in sc:
+            assert np.array_equal(sa[k], sc[k], equal_nan=True), k
+            assert np.array_equal(sb[k], sc[k], equal_nan=True), k
+        assert B.graph_stats()[1] == 3  # steps 2-4 replayed
+    finally:
+        A.close()
+        B.close()
+        C.close()
