@@ -170,17 +170,18 @@ def test_step_with_no_marker_and_one_marker(M):
         C.close()
 
 
-def test_two_stream_step_at_workload_size():
+@pytest.mark.parametrize("precision", [ereb.FP64, ereb.MIXED])
+def test_two_stream_step_at_workload_size(precision):
     """At 2,822,400 markers, where the marker branch and the node pass and
     assemblies really run side by side: the eager two-stream steps and the
     graph replays equal the separate resident calls bit for bit after each
     of four steps with varying dt (marker state, SUM and node grids, both
-    assembled systems)."""
+    assembled systems), in both forms."""
     g, mat = synth.workload_geometry("fp64", mode=1)
     mk = synth.make_markers(g, mat, seed=0, stress=True)
-    A = _ctx(g, mat, mk, graphs=False)
-    B = _ctx(g, mat, mk, graphs=True)
-    C = _ctx(g, mat, mk, graphs=False)
+    A = _ctx(g, mat, mk, graphs=False, precision=precision)
+    B = _ctx(g, mat, mk, graphs=True, precision=precision)
+    C = _ctx(g, mat, mk, graphs=False, precision=precision)
     try:
         for ts in range(1, 5):
             dt, hrs, hrf = _scalars(mat, ts)
