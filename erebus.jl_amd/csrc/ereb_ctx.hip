@@ -651,8 +651,10 @@ struct OnStream {
 // from the velocities alone) before it, from the end of the sort's key pass
 // (beside the key pass itself, which is bound by HBM traffic, they stretched
 // it by more than their own duration); the main stream waits for the second
-// at the end.  Results are those of the launches in sequence: no kernel of
-// one stream touches what the other's write.
+// at the end.  In an eager step the thermal assembly runs on a third stream
+// beside the Stokes-Darcy one, after the node pass.  Results are those of the
+// launches in sequence: no kernel of one stream touches what another's
+// write.
 enum { SEG_TABLES = 0, SEG_SCATTER, SEG_MARKERS, SEG_NODES };
 
 static int seg_launches(ereb_ctx* c, int seg, const double* hrs, const double* hrf,
