@@ -12,9 +12,8 @@ A step is one timestep of the transfer path in the reference loop order
   stress gather (E:6980), temperature gather (E:7004), RK4 advection (E:7036)
 dt and the heating rates change at every step as in the reference loop (dt
 control E:6619, E:6790, E:6933; calculate_radioactive_heating(timesum)
-E:6413).  The step's launches go to three streams: the gathers + RK4 kernel
-runs beside the node pass and the assemblies once the scatter is done, and
-the two assemblies side by side.
+E:6413).  The step's launches go to two streams: the gathers + RK4 kernel
+runs beside the node pass and the assemblies once the scatter is done.
 The host LU solves are outside the step (they stay on the host in the
 reference's design); the CSC matrices stay in HBM.  The grids that the
 solves and grid-only operations would produce (DT, DSXX, DSXY, tk2,
@@ -56,8 +55,8 @@ from ereb import precision, synth  # noqa: E402
 
 METRIC = "marker-steps/s, 2.8M markers: FP64 vs mixed-precision; adjoint/forward ratio"
 # how ereb_step_transfer issues a step (the library's default)
-STEP_LAUNCHES = ("eager on three streams: gathers + RK4 beside the node pass and assemblies "
-                 "once the scatter is done, the two assemblies side by side (graph replay off)")
+STEP_LAUNCHES = ("eager on two streams: gathers + RK4 beside the node pass and assemblies "
+                 "once the scatter is done (graph replay off)")
 HBM_PEAK_GBS = 8000.0
 START_TIME = 2.25e6 * 365.25 * 24 * 3600  # C:339
 
@@ -159,7 +158,7 @@ def setup(ctx: ereb.Context, mk, grids):
 def step(ctx, mat, mode, timestep):
     """One transfer step: scatter, node properties, Stokes-Darcy and thermal
     assembly, stress and temperature gathers, RK4 (ereb_step_transfer, on
-    the context's streams; everything on one stream while profiling),
+    the context's two streams; everything on one stream while profiling),
     with this timestep's dt and heating rates."""
     dt, hrs, hrf = step_scalars(mat, timestep)
     ctx.step_transfer(hrs, hrf, mode, dt, timestep)

@@ -342,13 +342,10 @@ int ereb_ctx_create(const ereb_geometry* g, const ereb_materials* mat, int preci
         CK(hipGetDeviceProperties(&prop, device));
         std::vector<uint32_t> all((size_t)(prop.multiProcessorCount + 31) / 32, ~0u);
         CK(hipExtStreamCreateWithCUMask(&c->stream2, (uint32_t)all.size(), all.data()));
-        CK(hipExtStreamCreateWithCUMask(&c->stream3, (uint32_t)all.size(), all.data()));
     }
     // recorded as kernels' stop events (hipExtLaunchKernelGGL): default flags
     CK(hipEventCreate(&c->ev_scatter));
     CK(hipEventCreate(&c->ev_keys));
-    CK(hipEventCreate(&c->ev_nodes));
-    CK(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     // the no-alloc entry point's scratch on this device (once per device); a
     // failure here is reported by that entry point, which tries again
@@ -454,9 +451,8 @@ int ereb_ctx_destroy(ereb_ctx* c) {
         hipFree(s.chg);
         hipFree(s.map);
     }
-    for (hipEvent_t e : {c->ev_scatter, c->ev_join, c->ev_keys, c->ev_nodes, c->ev_join3})
+    for (hipEvent_t e : {c->ev_scatter, c->ev_join, c->ev_keys})
         if (e) hipEventDestroy(e);
-    if (c->stream3) hipStreamDestroy(c->stream3);
     if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -651,10 +647,8 @@ struct OnStream {
 // from the velocities alone) before it, from the end of the sort's key pass
 // (beside the key pass itself, which is bound by HBM traffic, they stretched
 // it by more than their own duration); the main stream waits for the second
-// at the end.  In an eager step the thermal assembly runs on a third stream
-// beside the Stokes-Darcy one, after the node pass.  Results are those of the
-// launches in sequence: no kernel of one stream touches what another's
-// write.
+// at the end.  Results are those of the launches in sequence: no kernel of
+// one stream touches what the other's write.
 enum { SEG_TABLES = 0, SEG_SCATTER, SEG_MARKERS, SEG_NODES };
 
 static int seg_launches(ereb_ctx* c, int seg, const double* hrs, const double* hrf,
@@ -715,32 +709,9 @@ static int transfer_run(ereb_ctx* c, bool replay, const double* hrs, const doubl
         if (!keys) rc = seg_run(c, SEG_TABLES, s2, replay, hrs, hrf, mode, dt, timestep);
         if (!rc) rc = seg_run(c, SEG_MARKERS, s2, replay, hrs, hrf, mode, dt, timestep);
     }
-    bool third = false;  // the thermal assembly was given to the third stream
-    if (!rc && !evs) {
-        rc = seg_run(c, SEG_NODES, s1, replay, hrs, hrf, mode, dt, timestep);
-    } else if (!rc) {
-        // eager: the node pass, then the Stokes-Darcy assembly here and the
-        // thermal one on the third stream (each reads the node grids and
-        // writes its own system)
-        c->rec_nodes = true;
-        rc = launch_combine_nodes(c);
-        const bool nodes = !c->rec_nodes;
-        c->rec_nodes = false;
-        if (!rc && !nodes) rc = fail(c, EREB_ERR_STATE, "node pass event not recorded");
-        if (!rc) {
-            EREB_HIP(c, hipStreamWaitEvent(c->stream3, c->ev_nodes, 0));
-            third = true;
-            OnStream on(c, c->stream3);
-            rc = launch_assemble_thermal(c, dt);
-        }
-        if (!rc) rc = launch_assemble_stokes(c, dt);
-    }
-    // the joins also after a failure: later work on the main stream follows
-    // everything the other streams were given
-    if (third) {
-        EREB_HIP(c, hipEventRecord(c->ev_join3, c->stream3));
-        EREB_HIP(c, hipStreamWaitEvent(s1, c->ev_join3, 0));
-    }
+    if (!rc) rc = seg_run(c, SEG_NODES, s1, replay, hrs, hrf, mode, dt, timestep);
+    // the join also after a failure: later work on the main stream follows
+    // everything the second stream was given
     EREB_HIP(c, hipEventRecord(c->ev_join, s2));
     EREB_HIP(c, hipStreamWaitEvent(s1, c->ev_join, 0));
     return rc;

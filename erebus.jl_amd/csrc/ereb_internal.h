@@ -105,12 +105,6 @@ struct ereb_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_scatter = nullptr, ev_join = nullptr, ev_keys = nullptr;
     bool rec_keys = false, rec_scatter = false;
-    // third stream (a queue of its own too): the thermal assembly of an eager
-    // step, beside the Stokes-Darcy one, once the node pass (ev_nodes, its
-    // stop event while rec_nodes is set) is done; ev_join3 its end
-    hipStream_t stream3 = nullptr;
-    hipEvent_t ev_nodes = nullptr, ev_join3 = nullptr;
-    bool rec_nodes = false;
     int precision = EREB_FP64;
     int Nx = 0, Ny = 0, Nx1 = 0, Ny1 = 0;
     double dx = 0, dy = 0;

@@ -248,15 +248,8 @@ int launch_combine_nodes(ereb_ctx* c) {
     const int n = A.ns > A.nb ? A.ns : A.nb;
     hipEvent_t ev;
     prof_begin(c, EREB_K_NODES, &ev);
-    const dim3 grid((unsigned)((n + 255) / 256), 4);
-    // the pass's completion as ev_nodes when asked (rec_nodes)
-    if (c->rec_nodes)
-        hipExtLaunchKernelGGL(k_combine_nodes, grid, dim3(256), 0, c->stream, nullptr,
-                              c->ev_nodes, 0, A, (const double*)c->d_part, c->nty, c->ntx);
-    else
-        hipLaunchKernelGGL(k_combine_nodes, grid, dim3(256), 0, c->stream, A,
-                           (const double*)c->d_part, c->nty, c->ntx);
-    c->rec_nodes = false;
+    hipLaunchKernelGGL(k_combine_nodes, dim3((unsigned)((n + 255) / 256), 4), dim3(256), 0,
+                       c->stream, A, (const double*)c->d_part, c->nty, c->ntx);
     EREB_HIP(c, hipGetLastError());
     prof_end(c, EREB_K_NODES, ev);
     return EREB_OK;

@@ -232,9 +232,8 @@ int ereb_step_assemble_thermal(ereb_ctx* ctx, double dt, int64_t* nnz);
  * ereb_step_advect_rk4 (E:6419-7053 without the host solves); the two
  * gathers and RK4 run as one kernel (kernel class EREB_K_RK4), on a second
  * stream of the context once the scatter is done, beside the node pass and
- * the assemblies, which read none of the marker state it writes, and the
- * thermal assembly on a third beside the Stokes-Darcy one; the call returns
- * with the context's stream ordered after all of them.  With graph replay
+ * the assemblies, which read none of the marker state it writes; the call
+ * returns with the context's stream ordered after both.  With graph replay
  * on (ereb_ctx_set_graphs), after the first call the step's launches are
  * captured once into four graphs (the two streams' parts) and replayed; they
  * are captured again when the mode, the marker count, the precision form or
